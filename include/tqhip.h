@@ -1,0 +1,308 @@
+/*
+ * tqhip.h — C ABI of the MI355X (gfx950) kernels behind the relational hot path:
+ * predicate filter + stream compaction, LSD radix sort / top-k, sort-based equi-join and
+ * group-by aggregation.
+ *
+ * The reference engine has no native layer: each entry point below replaces a chain of ATen
+ * calls inside its Python operators.  The replaced call sites are cited on each declaration
+ * (paths are relative to the reference tree).
+ *
+ * Conventions
+ *  - Every entry point returns TQ_OK (0) or a negative TQ_E* code; tq_last_error() returns the
+ *    message of the calling thread's last failure.
+ *  - All pointers are device pointers unless the parameter says "host".  Caller allocates every
+ *    output and workspace buffer; the library never allocates or frees device memory.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the legacy default stream.  Calls are
+ *    asynchronous on that stream; results are valid once the stream has been synchronised.
+ *  - Row counts are int64_t; a single launch handles up to 2^31-1 rows.
+ */
+#ifndef TQHIP_H
+#define TQHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define TQ_OK 0
+#define TQ_EINVAL (-1)       /* bad argument: Python raises ValueError                  */
+#define TQ_EUNSUPPORTED (-2) /* unsupported operation/type: Python raises NotImplemented */
+#define TQ_EHIP (-3)         /* HIP runtime / launch failure: RuntimeError              */
+#define TQ_ECAPACITY (-4)    /* workspace or table too small: MemoryError               */
+
+/* ---------------------------------------------------------------- element types */
+#define TQ_BOOL 0
+#define TQ_I8 1
+#define TQ_I16 2
+#define TQ_I32 3
+#define TQ_I64 4
+#define TQ_F32 5
+#define TQ_F64 6
+
+/* ---------------------------------------------------------------- expression programs
+ * A predicate or aggregate argument is compiled on the host into a register program that one
+ * kernel evaluates per row.  Registers hold 64 raw bits; each opcode fixes how they are read:
+ * integers as int64, booleans as 0/1 int64, f64 as a double, f32 as a float in the low half.
+ * The host compiler resolves torch's type promotion, so the program computes in exactly the
+ * dtype the reference's ATen ops would (e.g. int32 column vs float literal compares in f32).
+ */
+#define TQ_MAX_INSTR 64
+#define TQ_MAX_COLS 16
+#define TQ_NREG 8
+#define TQ_STRPOOL 128
+#define TQ_MAX_SEGS 12
+#define TQ_MAX_LOADS 8
+
+enum tq_opcode {
+  TQ_OP_NOP = 0,
+  /* dst <- value of load slot a for this row (see tq_program_t.load_*) */
+  TQ_OP_LOAD,
+  TQ_OP_CONST, /* dst <- imm (raw bits) */
+  /* conversions */
+  TQ_OP_CVT_I64_F64, TQ_OP_CVT_I64_F32, TQ_OP_CVT_F32_F64, TQ_OP_CVT_F64_F32,
+  /* narrow an int64 result back to torch's result dtype (two's-complement wrap) */
+  TQ_OP_WRAP_I8, TQ_OP_WRAP_I16, TQ_OP_WRAP_I32,
+  /* arithmetic */
+  TQ_OP_ADD_I64, TQ_OP_SUB_I64, TQ_OP_MUL_I64,
+  TQ_OP_ADD_F64, TQ_OP_SUB_F64, TQ_OP_MUL_F64, TQ_OP_DIV_F64,
+  TQ_OP_ADD_F32, TQ_OP_SUB_F32, TQ_OP_MUL_F32, TQ_OP_DIV_F32,
+  /* comparisons -> bool */
+  TQ_OP_LT_I64, TQ_OP_LE_I64, TQ_OP_GT_I64, TQ_OP_GE_I64, TQ_OP_EQ_I64, TQ_OP_NE_I64,
+  TQ_OP_LT_F64, TQ_OP_LE_F64, TQ_OP_GT_F64, TQ_OP_GE_F64, TQ_OP_EQ_F64, TQ_OP_NE_F64,
+  TQ_OP_LT_F32, TQ_OP_LE_F32, TQ_OP_GT_F32, TQ_OP_GE_F32, TQ_OP_EQ_F32, TQ_OP_NE_F32,
+  /* torch.isclose(a, b, rtol=1e-5, atol=imm as f64) */
+  TQ_OP_ISCLOSE_F64,
+  /* boolean */
+  TQ_OP_AND, TQ_OP_OR, TQ_OP_NOT,
+  /* dst <- reg[c] ? reg[a] : reg[b] (raw bits) */
+  TQ_OP_WHERE,
+  /* dst <- all(string column a [row, 0:len] == pool[off:off+len]) (imm = off | len << 32);
+     EQ also requires the column's remaining bytes to be 0 (torch pads literals with 0) */
+  TQ_OP_STR_EQ, TQ_OP_STR_PREFIX,
+  /* dst <- calendar year of int64 day count since 1990-01-01 (bucketize on year edges) */
+  TQ_OP_YEAR,
+  TQ_OP_COUNT_
+};
+
+typedef struct {
+  uint8_t op, dst, a, b, c, pad0, pad1, pad2;
+  int64_t imm;
+} tq_instr_t;
+
+/* A program is a bank of `nseg` segments run in order; segment s ends before instruction
+ * seg_end[s] and leaves its value in register seg_result[s].  Registers persist across
+ * segments, so later segments may reuse values of earlier ones.  A predicate has one segment;
+ * a group-by passes one segment per aggregate argument.
+ * Column reads are declared up front in the load table: slot i reads column load_col[i] of
+ * type load_dtype[i] (TQ_BOOL..TQ_F64) for every row of a tile before any instruction runs, so
+ * all of a tile's loads are in flight together; TQ_OP_LOAD then reads the slot. */
+typedef struct {
+  int32_t n_instr;
+  int32_t nseg;
+  int16_t seg_end[TQ_MAX_SEGS];
+  uint8_t seg_result[TQ_MAX_SEGS];
+  int32_t nload;
+  uint8_t load_col[TQ_MAX_LOADS];
+  uint8_t load_dtype[TQ_MAX_LOADS];
+  tq_instr_t code[TQ_MAX_INSTR];
+  uint8_t strpool[TQ_STRPOOL];
+} tq_program_t;
+
+typedef struct {
+  const void* ptr[TQ_MAX_COLS]; /* device column base pointers */
+  int32_t width[TQ_MAX_COLS];   /* bytes per row for 2-D int8 string columns, else 1 */
+} tq_colset_t;
+
+/* ---------------------------------------------------------------- housekeeping */
+int tq_version(void);
+const char* tq_last_error(void);
+/* Rows covered by one filter/compaction tile (the unit of tile_counts / tile_offsets). */
+int tq_tile_rows(void);
+
+/* ---------------------------------------------------------------- filter + compaction */
+
+/* Evaluate a predicate program over n rows into a bool mask (1 byte/row) and, if tile_counts is
+ * non-NULL, the number of true rows of each tile of tq_tile_rows() rows.
+ * Replaces operators/filter.py:14-162 (`evaluate`: one ATen kernel per tree node,
+ * torch.le/lt/ge/gt/eq/isclose, logical_and/or, ones for isnotnull). */
+int tq_filter_mask(const tq_program_t* prog /* host */, const tq_colset_t* cols /* host */,
+                   int64_t n, uint8_t* mask_out, uint32_t* tile_counts, void* stream);
+
+/* Evaluate segment 0 of a program per row into a column of out_bytes-wide values: the low
+ * bytes of the result register (bool/int8: 1, int16: 2, int32/f32: 4, int64/f64: 8).
+ * Replaces the elementwise ATen chains of operators/aggregate.py:197-233 when they combine
+ * aggregate results (e.g. 0.2 * avg(x)) and of operators/project.py:26-40. */
+int tq_eval_column(const tq_program_t* prog /* host */, const tq_colset_t* cols /* host */,
+                   int64_t n, void* out, int32_t out_bytes, void* stream);
+
+/* Per-tile popcount of an existing bool mask (for masks not made by tq_filter_mask). */
+int tq_mask_tile_counts(const uint8_t* mask, int64_t n, uint32_t* tile_counts, void* stream);
+
+/* Exclusive scan of per-tile counts: tile_offsets[t] = sum of counts before t; total[0] = sum.
+ * ws must hold tq_scan_ws_bytes(ntiles) bytes. */
+size_t tq_scan_ws_bytes(int64_t ntiles);
+int tq_scan_tiles(const uint32_t* tile_counts, int64_t ntiles, int64_t* tile_offsets,
+                  int64_t* total, void* ws, void* stream);
+
+/* Order-preserving stream compaction of up to TQ_MAX_COLS columns by one mask:
+ * dst[c][tile_offsets[t] + rank] = src[c][row] for every selected row.
+ * elem_bytes[c] is the row size of column c (1/2/4/8, or the width of a 2-D int8 column).
+ * A NULL src[c] makes dst[c] an int64 column of the selected row numbers (torch.nonzero).
+ * Replaces IO/vortex_pipeline.py:58 (`tensor[mask]`: nonzero + index_select per column). */
+int tq_compact(const uint8_t* mask, int64_t n, const int64_t* tile_offsets, int32_t ncols,
+               const void* const* src /* host array */, const int32_t* elem_bytes /* host */,
+               void* const* dst /* host array */, void* stream);
+
+/* Multi-column row gather: dst[c][i] = src[c][idx[i]] for i < m.  idx is int64.  When
+ * null_fill is non-zero an index equal to -64 yields the value -64 in the column's type
+ * instead; float_kind[c] (host, may be NULL without null_fill) is 4 for f32, 8 for f64 and 0
+ * for integer/bool columns.
+ * Replaces operators/hashjoin.py:161-175 and operators/sort.py:44-54 (per-column gathers) and
+ * conversion.py:52-57 (index_with_null). */
+int tq_gather(const int64_t* idx, int64_t m, int32_t ncols, const void* const* src /* host */,
+              const int32_t* elem_bytes /* host */, const int32_t* float_kind /* host */,
+              void* const* dst /* host */, int32_t null_fill, void* stream);
+
+/* Row flags from an index list: flags[0:n] = 0, then flags[idx[i]] = 1 for i < m.
+ * Replaces operators/hashjoin.py:118-141 (torch.unique of matched probe rows, ones + index_put
+ * of the semi/anti/outer finishing). */
+int tq_mark_rows(const int64_t* idx, int64_t m, uint8_t* flags, int64_t n, void* stream);
+
+/* ---------------------------------------------------------------- sort / top-k */
+typedef struct {
+  const void* ptr;    /* device column */
+  int32_t dtype;      /* TQ_I8..TQ_F64 or TQ_BOOL */
+  int32_t width;      /* columns of a 2-D key (compared lexicographically), else 1 */
+  int32_t descending; /* 0 ascending, 1 descending */
+  int32_t pad;
+} tq_sortkey_t;
+
+/* Workspace for sorting n rows. */
+size_t tq_sort_ws_bytes(int64_t n);
+
+/* Stable multi-key sort permutation: perm_out[0:min(n,limit)] lists row ids in the order a
+ * stable sort by key 0, then key 1, ... gives (ties keep input order).  Float keys follow
+ * torch.sort: -0.0 ties +0.0 and NaN sorts above +inf.  limit < 0 means all rows; a limit
+ * below n returns the first `limit` rows of that order (top-k).
+ * Replaces operators/sort.py:20-42 (per-key torch.sort(stable=True) passes, unique(dim=0) for
+ * 2-D keys, perm[:limit]) and the build-side torch.sort of operators/hashjoin.py:43. */
+int tq_sort_perm(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n, int64_t limit,
+                 int64_t* perm_out, void* ws, size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------- equi-join */
+/* Host-side handle of a built join index (filled by tq_join_build, read by the probe passes).
+ * Build keys are stored order-transformed (sign bit flipped) and sorted, with their build row
+ * ids in the same order; ties keep build row order, so every bucket lists rows ascending. */
+typedef struct {
+  int64_t nsel;         /* selected build rows = length of keys/rows */
+  int32_t key_bits;     /* 32 or 64: width of the stored keys */
+  int32_t nsample;      /* entries in the LDS search sample */
+  int64_t step;         /* sample stride: sample[s] = keys[min((s+1)*step, nsel) - 1] */
+  const void* keys;     /* device: sorted transformed keys */
+  const uint32_t* rows; /* device: build row ids */
+  const void* sample;   /* device: search sample */
+} tq_join_state_t;
+
+/* Workspace for building over nb build rows. */
+size_t tq_join_build_ws_bytes(int64_t nb);
+
+/* Build: sort the keys of the selected build rows (bmask may be NULL) with their row ids.
+ * Keys are integer columns (TQ_BOOL..TQ_I64).  Synchronises the stream (the row count and key
+ * range decide the sort passes) and fills *state; ws must outlive the probe passes.
+ * Replaces operators/hashjoin.py:41-52 (arange, torch.sort, unique_consecutive, cumsum) and the
+ * build-side mask compaction of IO/vortex_pipeline.py:58. */
+int tq_join_build(const void* bkeys, int32_t bdtype, const uint8_t* bmask, int64_t nb, void* ws,
+                  size_t ws_bytes, tq_join_state_t* state /* host out */, void* stream);
+
+/* Workspace for probing np rows. */
+size_t tq_join_probe_ws_bytes(int64_t np);
+
+/* Probe, count pass: number of (build, probe) pairs with equal keys among selected probe rows
+ * (pmask may be NULL), per probe tile and in total.  The total is left in device memory inside
+ * pws and, when total_host is non-NULL, also returned there (which synchronises the stream).
+ * Replaces operators/hashjoin.py:57-98 (searchsorted, eq, nonzero, lenR.max(), masked_select,
+ * repeat_interleave) up to the pair count. */
+int tq_join_count(const tq_join_state_t* state /* host */, const void* pkeys, int32_t pdtype,
+                  const uint8_t* pmask, int64_t np, void* pws, int64_t* total_host,
+                  void* stream);
+
+/* Probe, write pass: emits every pair in canonical order (probe row ascending, then build row
+ * ascending) and materialises output columns directly: left column c gets lsrc[c][build row],
+ * right column c gets rsrc[c][probe row].  build_idx / probe_idx (int64, may be NULL) receive
+ * the pair row ids.  Replaces operators/hashjoin.py:100-113 (key re-check, pair selection) and
+ * operators/hashjoin.py:161-172 (output gathers). */
+int tq_join_write(const tq_join_state_t* state /* host */, const void* pkeys, int32_t pdtype,
+                  const uint8_t* pmask, int64_t np, const void* pws, int32_t nleft,
+                  const void* const* lsrc /* host */, const int32_t* lbytes /* host */,
+                  void* const* ldst /* host */, int32_t nright, const void* const* rsrc /* host */,
+                  const int32_t* rbytes /* host */, void* const* rdst /* host */,
+                  int64_t* build_idx, int64_t* probe_idx, void* stream);
+
+/* ---------------------------------------------------------------- group-by aggregate */
+#define TQ_AGG_SUM 0   /* f64 sum for f64 args, f32 for f32, exact int64 for integer/bool args */
+#define TQ_AGG_COUNT 1 /* int64 count of rows whose argument != -64                            */
+#define TQ_AGG_AVG 2   /* f64 sum / row count                                                  */
+#define TQ_AGG_MIN 3   /* argument dtype; init = dtype max                                     */
+#define TQ_AGG_MAX 4   /* argument dtype; init = 0, as the reference does                      */
+
+#define TQ_GB_MAX_KEYS 8
+#define TQ_GB_MAX_AGGS TQ_MAX_SEGS
+
+/* Table modes (chosen by tq_groupby_plan).  Slots are addressed by the packed key. */
+#define TQ_GB_DENSE_LDS 0    /* whole key space in one LDS array per workgroup, then merged  */
+#define TQ_GB_DENSE_GLOBAL 1 /* key space addressed directly in one global array            */
+#define TQ_GB_HASH 2         /* global open-addressing hash table on the packed key         */
+#define TQ_GB_LDS_FRONT 8    /* flag: an LDS hash table absorbs repeated keys first         */
+
+typedef struct {
+  int32_t nkeys;
+  int32_t naggs;
+  int32_t key_dtype[TQ_GB_MAX_KEYS]; /* TQ_BOOL..TQ_I64                                    */
+  const void* key_ptr[TQ_GB_MAX_KEYS];
+  int64_t key_min[TQ_GB_MAX_KEYS];    /* from tq_groupby_bounds                            */
+  int64_t key_range[TQ_GB_MAX_KEYS];  /* max - min + 1                                    */
+  int64_t key_stride[TQ_GB_MAX_KEYS]; /* mixed-radix weights, filled by tq_groupby_plan    */
+  int32_t agg_op[TQ_GB_MAX_AGGS];     /* TQ_AGG_*                                          */
+  int32_t agg_dtype[TQ_GB_MAX_AGGS];  /* dtype of the argument segment's value            */
+} tq_groupby_desc_t;
+
+/* Pass 1: per-key-column min and max over selected rows (mask may be NULL) and the selected row
+ * count.  out (device): int64[2*nkeys + 1] = {min0, max0, min1, max1, ..., nsel}. */
+int tq_groupby_bounds(const tq_groupby_desc_t* desc /* host */, const uint8_t* mask, int64_t n,
+                      int64_t* out, void* stream);
+
+/* Host planning from the bounds: fills key_stride, picks the table mode and slot count and
+ * returns the workspace size.  The packed key is sum((k_i - min_i) * stride_i), so packed order
+ * is lexicographic key order and packed values are < product of ranges.
+ * Fails with TQ_EUNSUPPORTED if the key space does not fit 62 bits. */
+int tq_groupby_plan(tq_groupby_desc_t* desc /* host, in/out */, int64_t nsel, int32_t nload,
+                    int32_t* mode, int64_t* slots, size_t* ws_bytes);
+
+/* Pass 2: clear the table and accumulate every selected row.  Segment j of `prog` computes
+ * aggregate j's argument per row; the key columns are read directly.
+ * Replaces operators/aggregate.py:302-327 (key stacking, linear hash, torch.unique) and
+ * operators/aggregate.py:127-193 (bincount / 2-step scatter_reduce sums, counts, averages). */
+int tq_groupby_accumulate(const tq_groupby_desc_t* desc /* host */,
+                          const tq_program_t* prog /* host */, const tq_colset_t* cols /* host */,
+                          const uint8_t* mask, int64_t n, int32_t mode, int64_t slots, void* ws,
+                          size_t ws_bytes, void* stream);
+
+/* Pass 3: number the occupied slots.  ngroups (device int64) receives the group count; when
+ * ngroups_host is non-NULL it is also returned there (synchronising the stream). */
+int tq_groupby_count(const tq_groupby_desc_t* desc /* host */, int32_t mode, int64_t slots,
+                     void* ws, int64_t* ngroups_host, void* stream);
+
+/* Pass 4: write the ngroups groups in ascending key order: key_out[i] gets key i (in its dtype)
+ * and agg_out[j] aggregate j (types as listed at TQ_AGG_*).  Hash tables are ordered by a
+ * radix sort of the packed keys (synchronises the stream). */
+int tq_groupby_write(const tq_groupby_desc_t* desc /* host */, int32_t mode, int64_t slots,
+                     void* ws, int64_t ngroups, void* const* key_out /* host */,
+                     void* const* agg_out /* host */, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TQHIP_H */

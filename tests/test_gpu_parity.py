@@ -1,0 +1,291 @@
+"""Parity of the HIP operators with the CPU oracle on the same seeded inputs (GPU).
+
+Bar: bit-exact masks, compacted columns, join outputs (canonical pair order), group keys,
+counts, integer sums and sort orders; float64 sums and averages within 1e-12 relative of the
+oracle (whose own sums are pinned to sequential sums in tests/test_oracle.py).
+"""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import closed_form as CF
+from oracle import ref_ops as R
+
+pytestmark = pytest.mark.gpu
+
+FP_RTOL = 1e-12
+
+
+def _gpu_group(tables, dev):
+    from tqp_amd import workloads
+    return workloads.to_tensor_group(tables, dev)
+
+
+def _assert_col(got, want, name, fp=False):
+    g = got.detach().cpu()
+    w = want.detach().cpu()
+    assert tuple(g.shape) == tuple(w.shape), f"{name}: shape {tuple(g.shape)} vs {tuple(w.shape)}"
+    if fp and w.dtype.is_floating_point:
+        assert g.dtype == w.dtype, f"{name}: dtype {g.dtype} vs {w.dtype}"
+        np.testing.assert_allclose(g.numpy(), w.numpy(), rtol=FP_RTOL, atol=0, err_msg=name)
+    else:
+        assert g.dtype == w.dtype, f"{name}: dtype {g.dtype} vs {w.dtype}"
+        assert torch.equal(g, w), f"{name}: values differ"
+
+
+# ------------------------------------------------------------------ filter + compaction
+PREDICATES = [
+    [[['isnotnull', ['ship_day#1']], 'AND', ['ship_day#1', '<=', '2400']]],
+    ['ship_day#1', '>', '1200'],
+    ['ship_day#1', '<', '1200.5'],
+    [['price#3', '>=', '50000.0'], 'AND', ['discount#4', '=', '0.05']],
+    [['flag_a#6', '=', '2'], 'OR', ['NOT', ['tax#5', '<=', '0.04']]],
+    ['flag_a#6', 'IN', ['0', '2']],
+    [['quantity#2', '<', '25'], 'AND', ['ship_day#1', '>=', '1995-01-01']],
+]
+
+
+@pytest.mark.parametrize("pi", range(len(PREDICATES)))
+def test_filter_mask_bit_exact(device, pi):
+    from tqp_amd import kernels, workloads
+    tables = workloads.config0_tables(n=300_001)
+    tree = PREDICATES[pi]
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    mask, counts = kernels.filter_mask(tree, gpu, None)
+    want = R.filter_evaluate(cpu, tree, None, None)
+    _assert_col(mask, want, f"mask {pi}")
+    assert int(counts.sum().item()) == int(want.sum().item())
+
+
+@pytest.mark.parametrize("n", [1, 4095, 4096, 4097, 1_000_003])
+def test_compaction_bit_exact_and_ordered(device, n):
+    from tqp_amd import kernels
+    g = np.random.default_rng(n)
+    m = g.random(n) < 0.37
+    a = g.integers(-100, 100, n, dtype=np.int8)
+    b = g.random(n)
+    c = g.integers(0, 1 << 40, n, dtype=np.int64)
+    s = g.integers(0, 127, (n, 5), dtype=np.int8)
+    tm = torch.from_numpy(m).to(device)
+    outs = kernels.compact(tm, [torch.from_numpy(x).to(device) for x in (a, b, c, s)])
+    for x, o in zip((a, b, c, s), outs):
+        assert np.array_equal(o.cpu().numpy(), CF.compact(m, x))
+    rows = kernels.compact_rows(tm)
+    assert np.array_equal(rows.cpu().numpy(), np.flatnonzero(m))
+
+
+def test_compaction_empty_and_full(device):
+    from tqp_amd import kernels
+    for m in (np.zeros(10000, bool), np.ones(10000, bool)):
+        v = np.arange(10000, dtype=np.int32)
+        o = kernels.compact(torch.from_numpy(m).to(device), [torch.from_numpy(v).to(device)])[0]
+        assert np.array_equal(o.cpu().numpy(), v[m])
+
+
+# ------------------------------------------------------------------ sort
+@pytest.mark.parametrize("n", [5, 4097, 300_000])
+def test_sort_multikey_bit_exact(device, n):
+    from tqp_amd import kernels
+    g = np.random.default_rng(n)
+    a = g.integers(0, 4, n, dtype=np.int8)
+    f = np.round(g.normal(size=n), 2)
+    f[g.random(n) < 0.05] = 0.0
+    f[g.random(n) < 0.05] = -0.0
+    f[g.random(n) < 0.02] = np.nan
+    f[g.random(n) < 0.01] = np.inf
+    i = g.integers(-(1 << 40), 1 << 40, n, dtype=np.int64)
+    for descs in ([False, True, False], [True, False, True], [False, False, False]):
+        keys = [(torch.from_numpy(x).to(device), d) for x, d in zip((a, f, i), descs)]
+        perm = kernels.sort_perm(keys, n)
+        assert np.array_equal(perm.cpu().numpy(), CF.sort_perm([a, f, i], descs)), descs
+    perm = kernels.sort_perm([(torch.from_numpy(f).to(device), True)], n, limit=10)
+    assert np.array_equal(perm.cpu().numpy(), CF.sort_perm([f], [True])[:10])
+
+
+def test_sort_2d_string_and_int32_keys(device):
+    from tqp_amd import kernels
+    g = np.random.default_rng(7)
+    n = 20000
+    s = g.integers(65, 70, (n, 3), dtype=np.int8)
+    k = g.integers(-5, 5, n, dtype=np.int32)
+    perm = kernels.sort_perm([(torch.from_numpy(s).to(device), False),
+                              (torch.from_numpy(k).to(device), True)], n)
+    assert np.array_equal(perm.cpu().numpy(), CF.sort_perm([s, k], [False, True]))
+
+
+# ------------------------------------------------------------------ join
+def _run_join(device, bk, bm, bv, pk, pm, pv, kind='inner', conds=()):
+    from tqp_amd.operators import hashjoin
+    tables = {1: ('int', bk), 2: ('int', bv), 3: ('int', pk), 4: ('float', pv),
+              901: ('bool', bm), 902: ('bool', pm)}
+    args = {'left group': [1, 2], 'right group': [3, 4], 'materialized in': [1, 2, 3, 4],
+            'mask map': {1: 901, 2: 901, 3: 902, 4: 902}}
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    hashjoin.join_vortex(True, gpu, copy.deepcopy(args), 1, 3, list(conds), kind, None, None, "j")
+    R.join_vortex(cpu, copy.deepcopy(args), 1, 3, list(conds), kind)
+    return gpu, cpu
+
+
+@pytest.mark.parametrize("case", ["unique", "dups", "skew", "empty_build", "empty_probe", "int64"])
+def test_join_inner_bit_exact(device, case):
+    g = np.random.default_rng(sum(map(ord, case)))
+    nb, np_ = 50_000, 200_000
+    if case == "unique":
+        bk = g.permutation(np.arange(1, nb + 1, dtype=np.int32))
+        pk = g.integers(1, nb + 1, np_, dtype=np.int32)
+    elif case == "dups":
+        bk = g.integers(0, 5000, nb, dtype=np.int32)
+        pk = g.integers(0, 6000, np_, dtype=np.int32)
+    elif case == "skew":
+        bk = np.where(g.random(nb) < 0.3, 7, g.integers(0, 100000, nb)).astype(np.int32)
+        pk = np.where(g.random(np_) < 0.05, 7, g.integers(0, 100000, np_)).astype(np.int32)
+    elif case == "int64":
+        bk = g.integers(-(1 << 50), 1 << 50, nb, dtype=np.int64)
+        pk = np.where(g.random(np_) < 0.5, bk[g.integers(0, nb, np_)], g.integers(0, 1 << 50, np_))
+    else:
+        bk = g.integers(0, 1000, nb, dtype=np.int32)
+        pk = g.integers(0, 1000, np_, dtype=np.int32)
+    bm = g.random(nb) < (0.0 if case == "empty_build" else 0.2)
+    pm = g.random(np_) < (0.0 if case == "empty_probe" else 0.53)
+    bv = g.integers(0, 1 << 40, nb, dtype=np.int64)
+    pv = g.random(np_)
+    gpu, cpu = _run_join(device, bk, bm, bv, pk, pm, pv)
+    for k in (1, 2, 3, 4):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"{case} col {k}")
+
+
+@pytest.mark.parametrize("kind", ["right-semi", "right-anti", "right-outer"])
+def test_join_other_kinds_bit_exact(device, kind):
+    g = np.random.default_rng(3)
+    nb, np_ = 3000, 9000
+    bk = g.integers(0, 2000, nb, dtype=np.int32)
+    pk = g.integers(0, 3000, np_, dtype=np.int32)
+    gpu, cpu = _run_join(device, bk, g.random(nb) < 0.5, g.integers(0, 99, nb, dtype=np.int64),
+                         pk, g.random(np_) < 0.5, g.random(np_), kind)
+    for k in (1, 2, 3, 4):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"{kind} col {k}")
+
+
+def test_join_with_condition(device):
+    g = np.random.default_rng(4)
+    nb, np_ = 3000, 9000
+    bk = g.integers(0, 500, nb, dtype=np.int32)
+    pk = g.integers(0, 500, np_, dtype=np.int32)
+    bv = g.integers(0, 4, nb, dtype=np.int64)
+    pv = g.integers(0, 4, np_).astype(np.float64)
+    gpu, cpu = _run_join(device, bk, g.random(nb) < 0.8, bv, pk, g.random(np_) < 0.8, pv,
+                         conds=[['#2', '<=', '#4']])
+    for k in (1, 2, 3, 4):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"cond col {k}")
+
+
+# ------------------------------------------------------------------ aggregate
+def _agg_case(device, keys, vals, mask, results, key_ids, agg_ids):
+    from tqp_amd.operators import aggregate
+    tables = dict(keys)
+    tables.update(vals)
+    input_ids = list(tables)
+    if mask is not None:
+        tables[900] = ('bool', mask)
+    args = {'keys': [f"k#{k}" for k in key_ids], 'results': results, 'out agg ids': agg_ids,
+            'is partial': False, 'input ids': list(input_ids)}
+    if mask is not None:
+        args['mask map'] = {i: 900 for i in input_ids}
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    aggregate.tqp_hash_aggregate(True, gpu, copy.deepcopy(args), None, None)
+    R.tqp_hash_aggregate(cpu, copy.deepcopy(args))
+    return gpu, cpu
+
+
+AGG_RESULTS = lambda: [  # noqa: E731
+    [[['sum', ['v#20']], '#', '1'], 'AS', 's#50'],
+    [[['sum', [['v#20', '*', ['1.0', '-', 'd#21']]]], '#', '2'], 'AS', 'sd#51'],
+    [[['avg', ['d#21']], '#', '3'], 'AS', 'a#52'],
+    [[['count', ['1']], '#', '4'], 'AS', 'c#53'],
+    [[['sum', ['i#22']], '#', '5'], 'AS', 'si#54'],
+    [[['min', ['i#22']], '#', '6'], 'AS', 'mn#55'],
+    [[['max', ['v#20']], '#', '7'], 'AS', 'mx#56'],
+]
+
+
+@pytest.mark.parametrize("case", ["few_groups", "dense_global", "hash_sparse", "skew_front",
+                                  "no_keys", "all_masked"])
+def test_aggregate_parity(device, case):
+    g = np.random.default_rng(sum(map(ord, case)) + 1)
+    n = 400_000
+    if case == "few_groups":
+        k = {10: ('string', g.integers(0, 3, n, dtype=np.int8)), 11: ('string', g.integers(0, 2, n, dtype=np.int8))}
+    elif case == "dense_global":
+        k = {10: ('int', g.integers(1, 300_000, n, dtype=np.int32))}
+    elif case == "hash_sparse":
+        k = {10: ('int', g.integers(0, 1 << 45, 50_000)[g.integers(0, 50_000, n)].astype(np.int64)),
+             11: ('int', g.integers(-3, 3, n, dtype=np.int32))}
+    elif case == "skew_front":
+        hot = np.where(g.random(n) < 0.6, 5, g.integers(0, 2000, n))
+        k = {10: ('int', hot.astype(np.int32)), 11: ('int', g.integers(0, 100, n, dtype=np.int32))}
+    else:
+        k = {}
+    vals = {20: ('float', g.uniform(900, 105000, n)), 21: ('float', g.integers(0, 11, n) / 100.0),
+            22: ('int', g.integers(-1000, 1000, n, dtype=np.int64))}
+    mask = None if case in ("dense_global",) else (g.random(n) < (0.0 if case == "all_masked" else 0.9))
+    key_ids = sorted(k)
+    results = [f"k#{i}" for i in key_ids] + AGG_RESULTS()
+    agg_ids = [50, 51, 52, 53, 54, 55, 56]
+    gpu, cpu = _agg_case(device, k, vals, mask, results, key_ids, agg_ids)
+    for i in key_ids:
+        _assert_col(gpu[i].tensor, cpu[i].tensor, f"{case} key {i}")
+    for a in agg_ids:
+        _assert_col(gpu[a].tensor, cpu[a].tensor, f"{case} agg {a}", fp=True)
+
+
+# ------------------------------------------------------------------ full pipelines
+def _run_pipeline(device, tables, plan):
+    from tqp_amd import executor
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    executor.execute(copy.deepcopy(plan), gpu)
+    R.execute(copy.deepcopy(plan), cpu)
+    return gpu, cpu
+
+
+def test_config0_pipeline(device):
+    from tqp_amd import workloads
+    print("config0 full size", flush=True)
+    gpu, cpu = _run_pipeline(device, workloads.config0_tables(), workloads.config0_plan())
+    for k in (6, 7, 28):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"c0 {k}")
+    for k in (22, 23, 24, 25, 26, 27):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"c0 {k}", fp=True)
+    # sum(quantity) is integer-valued and below 2^53: exact in any order
+    _assert_col(gpu[22].tensor, cpu[22].tensor, "c0 sum_qty exact")
+
+
+def test_config1_pipeline(device):
+    from tqp_amd import workloads
+    print("config1 full size", flush=True)
+    gpu, cpu = _run_pipeline(device, workloads.config1_tables(), workloads.config1_plan())
+    _assert_col(gpu[101].tensor, cpu[101].tensor, "c1 top keys")
+    _assert_col(gpu[301].tensor, cpu[301].tensor, "c1 top revenue", fp=True)
+
+
+def test_config1_join_aggregate_all_groups(device):
+    """Every group of the 264k-group aggregate (not only the top 10)."""
+    from tqp_amd import workloads
+    plan = workloads.config1_plan()[:4]
+    gpu, cpu = _run_pipeline(device, workloads.config1_tables(), plan)
+    _assert_col(gpu[101].tensor, cpu[101].tensor, "c1 group keys")
+    _assert_col(gpu[301].tensor, cpu[301].tensor, "c1 group revenue", fp=True)
+
+
+def test_config2_pipeline_reduced(device):
+    from tqp_amd import workloads
+    tables = workloads.config2_tables(n_fact=2_000_000, n_a=500_000, n_b=100_000)
+    gpu, cpu = _run_pipeline(device, tables, workloads.config2_plan())
+    for k in (322, 313, 402):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"c2 {k}")
+    _assert_col(gpu[401].tensor, cpu[401].tensor, "c2 sum", fp=True)

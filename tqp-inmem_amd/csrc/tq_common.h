@@ -1,0 +1,179 @@
+// Shared device/host helpers for the gfx950 kernels: error reporting, launch geometry and
+// wave64 primitives.  Every kernel in this library is memory-bound; the helpers keep loads
+// coalesced (64 consecutive rows per wave instruction) and reductions inside LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/tqhip.h"
+
+namespace tq {
+
+// ------------------------------------------------------------------ geometry
+constexpr int kBlock = 256;                // threads per workgroup (4 wave64s)
+constexpr int kWaves = kBlock / 64;
+constexpr int kItems = 16;                 // rows per thread in a filter/compaction tile
+constexpr int kTileRows = kBlock * kItems; // 4096 rows per tile
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// ------------------------------------------------------------------ errors
+int set_error(int code, const char* fmt, ...);
+int check_launch(const char* what);
+
+#define TQ_TRY_HIP(expr)                                                            \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return ::tq::set_error(TQ_EHIP, "%s: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+#define TQ_TRY(expr)          \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != TQ_OK) return r_; \
+  } while (0)
+
+// ------------------------------------------------------------------ wave64 primitives
+__device__ __forceinline__ int lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// number of set bits of m at lanes strictly below this lane
+__device__ __forceinline__ int rank_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T o = __shfl_up(v, d, 64);
+    if (l >= d) v += o;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Exclusive prefix sum across the 256 threads of a block; returns the block total in *total.
+// `sh` must hold kWaves elements of T.  Contains __syncthreads.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_sum(T v, T* sh, T* total) {
+  const int l = lane_id();
+  const int w = threadIdx.x >> 6;
+  T inc = wave_inclusive_sum(v);
+  if (l == 63) sh[w] = inc;
+  __syncthreads();
+  T base = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) {
+    T s = sh[i];
+    if (i < w) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+// Order-preserving maps of typed values to unsigned integers (used by sort and group-by).
+__device__ __forceinline__ uint64_t f64_order_bits(double x) {
+  // torch.sort compares values: -0.0 ties +0.0 and every NaN ranks above +inf.
+  if (x == 0.0) x = 0.0;
+  uint64_t b = __double_as_longlong(x);
+  if (x != x) b = 0x7ff8000000000000ull;
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ uint32_t f32_order_bits(float x) {
+  if (x == 0.0f) x = 0.0f;
+  uint32_t b = __float_as_uint(x);
+  if (x != x) b = 0x7fc00000u;
+  return (b >> 31) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ int64_t load_int(const void* p, int dtype, int64_t i) {
+  switch (dtype) {
+    case TQ_BOOL: return ((const uint8_t*)p)[i] != 0;
+    case TQ_I8: return ((const int8_t*)p)[i];
+    case TQ_I16: return ((const int16_t*)p)[i];
+    case TQ_I32: return ((const int32_t*)p)[i];
+    default: return ((const int64_t*)p)[i];
+  }
+}
+
+inline int dtype_bytes(int dtype) {
+  switch (dtype) {
+    case TQ_BOOL: case TQ_I8: return 1;
+    case TQ_I16: return 2;
+    case TQ_I32: case TQ_F32: return 4;
+    default: return 8;
+  }
+}
+
+// Copy one row of `bytes` bytes (power-of-two widths take a single load/store).
+__device__ __forceinline__ void copy_row(void* dst, int64_t di, const void* src, int64_t si,
+                                         int bytes) {
+  switch (bytes) {
+    case 1: ((uint8_t*)dst)[di] = ((const uint8_t*)src)[si]; break;
+    case 2: ((uint16_t*)dst)[di] = ((const uint16_t*)src)[si]; break;
+    case 4: ((uint32_t*)dst)[di] = ((const uint32_t*)src)[si]; break;
+    case 8: ((uint64_t*)dst)[di] = ((const uint64_t*)src)[si]; break;
+    default: {
+      const uint8_t* s = (const uint8_t*)src + si * bytes;
+      uint8_t* d = (uint8_t*)dst + di * bytes;
+      for (int b = 0; b < bytes; ++b) d[b] = s[b];
+    }
+  }
+}
+
+// Column pointer bundles passed by value as kernel arguments.
+struct ColPack {
+  const void* src[TQ_MAX_COLS];
+  void* dst[TQ_MAX_COLS];
+  int32_t bytes[TQ_MAX_COLS];
+  int32_t fkind[TQ_MAX_COLS];  // 4 = f32, 8 = f64, 0 = integer/bool/bytes (null-fill type)
+  int32_t n;
+};
+
+// Write the NULL sentinel (-64) into row di of a column of the given width/kind.
+__device__ __forceinline__ void write_null(void* dst, int64_t di, int bytes, int fkind) {
+  if (fkind == 8) {
+    ((double*)dst)[di] = -64.0;
+  } else if (fkind == 4) {
+    ((float*)dst)[di] = -64.0f;
+  } else {
+    switch (bytes) {
+      case 1: ((int8_t*)dst)[di] = -64; break;
+      case 2: ((int16_t*)dst)[di] = -64; break;
+      case 4: ((int32_t*)dst)[di] = -64; break;
+      case 8: ((int64_t*)dst)[di] = -64; break;
+      default: {
+        int8_t* d = (int8_t*)dst + di * bytes;
+        for (int b = 0; b < bytes; ++b) d[b] = -64;
+      }
+    }
+  }
+}
+
+// Graph-capture-safe fills (a kernel, never hipMemsetAsync): count elements of value v.
+int fill_u32(void* p, uint32_t v, int64_t count, hipStream_t s);
+int fill_u64(void* p, uint64_t v, int64_t count, hipStream_t s);
+unsigned stream_grid(int64_t m);
+template <typename TIn, typename TOut>
+int scan_exclusive(const TIn* in, int64_t n, TOut* out, TOut* total, TOut* ws, hipStream_t stream);
+
+// Fill a host-side ColPack from caller arrays; returns TQ_EINVAL text on bad widths.
+int make_colpack(ColPack* cp, int32_t ncols, const void* const* src, const int32_t* bytes,
+                 const int32_t* fkind, void* const* dst);
+
+}  // namespace tq

@@ -1,0 +1,202 @@
+// Per-row evaluation of host-compiled expression programs (predicates and aggregate arguments).
+//
+// Two phases per sub-tile of kSubRows rows (thread t owns rows sub0 + j*256 + t, j < kSub):
+//  1. stage: for each load slot of the program, every thread issues its kSub independent,
+//     coalesced loads and parks the values (widened to 64 bits) in LDS.  All loads of a slot
+//     are in flight together, which is what keeps HBM busy.
+//  2. interpret: for each of its rows, the thread runs the program on an 8-register file.  The
+//     program sits in the kernel-argument segment, so instruction fetches are scalar loads and
+//     the opcode switch is a uniform branch; a single row's register file is small enough to
+//     stay in VGPRs (indexed moves), so nothing spills to scratch.
+// Each thread only reads back LDS words it wrote itself, so no barrier separates the phases.
+//
+// Arithmetic matches the ATen kernels the reference runs (one rounding per operation, no FMA
+// contraction: the library is built with -ffp-contract=off).
+#pragma once
+
+#include "tq_common.h"
+
+namespace tq {
+
+constexpr int kSub = 4;                  // rows per thread per staged sub-tile
+constexpr int kSubRows = kBlock * kSub;  // 1024 rows
+
+__device__ __forceinline__ double as_f64(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ uint64_t f64_bits(double d) { return (uint64_t)__double_as_longlong(d); }
+__device__ __forceinline__ float as_f32(uint64_t b) { return __uint_as_float((uint32_t)b); }
+__device__ __forceinline__ uint64_t f32_bits(float f) { return (uint64_t)__float_as_uint(f); }
+
+__device__ __forceinline__ const void* col_ptr(const tq_colset_t& C, int i) {
+  const void* p = C.ptr[0];
+#pragma unroll
+  for (int j = 1; j < TQ_MAX_COLS; ++j) p = (i == j) ? C.ptr[j] : p;
+  return p;
+}
+__device__ __forceinline__ int col_width(const tq_colset_t& C, int i) {
+  int w = C.width[0];
+#pragma unroll
+  for (int j = 1; j < TQ_MAX_COLS; ++j) w = (i == j) ? C.width[j] : w;
+  return w;
+}
+
+// Day count since 1990-01-01 -> calendar year, as torch.bucketize(right=True) over the
+// cumulative year lengths 1990..1999 indexes year_range (conversion.py:10-11).
+__device__ __forceinline__ int64_t year_of_day(int64_t d) {
+  return 1990 + (d >= 365) + (d >= 730) + (d >= 1096) + (d >= 1461) + (d >= 1826) +
+         (d >= 2191) + (d >= 2557) + (d >= 2922) + (d >= 3287) + (d >= 3652);
+}
+
+// Bytes of LDS the stage needs for a program.
+inline size_t stage_bytes(const tq_program_t& P) {
+  return (size_t)(P.nload > 0 ? P.nload : 1) * kSubRows * sizeof(uint64_t);
+}
+
+// Phase 1: stage every load slot for the sub-tile starting at row sub0.  Rows with
+// live[j] == false (past the end, or unselected by a mask) are not read.
+__device__ __forceinline__ void stage_loads(const tq_program_t& P, const tq_colset_t& C,
+                                            int64_t sub0, const bool (&live)[kSub],
+                                            uint64_t* __restrict__ stage) {
+  const int t = threadIdx.x;
+  for (int s = 0; s < P.nload; ++s) {
+    const void* col = col_ptr(C, P.load_col[s]);
+    const int dt = P.load_dtype[s];
+    uint64_t v[kSub];
+#define TQ_STAGE(T, EXPR)                                          \
+  {                                                                \
+    const T* p = (const T*)col;                                    \
+    _Pragma("unroll") for (int j = 0; j < kSub; ++j) {             \
+      const int64_t r = sub0 + (int64_t)j * kBlock + t;            \
+      v[j] = live[j] ? (EXPR) : 0;                                 \
+    }                                                              \
+  }
+    switch (dt) {
+      case TQ_BOOL: TQ_STAGE(uint8_t, (uint64_t)(p[r] != 0)) break;
+      case TQ_I8: TQ_STAGE(int8_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_I16: TQ_STAGE(int16_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_I32: TQ_STAGE(int32_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_F32: TQ_STAGE(uint32_t, (uint64_t)p[r]) break;
+      default: TQ_STAGE(uint64_t, p[r]) break;
+    }
+#undef TQ_STAGE
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) stage[(s * kSub + j) * kBlock + t] = v[j];
+  }
+}
+
+// One row's eight registers, kept as named scalars and selected by switch: an indexable array
+// would be placed in private scratch memory by the compiler.
+struct Regs {
+  uint64_t r0, r1, r2, r3, r4, r5, r6, r7;
+  __device__ __forceinline__ uint64_t get(int i) const {
+    switch (i & 7) {
+      case 0: return r0;
+      case 1: return r1;
+      case 2: return r2;
+      case 3: return r3;
+      case 4: return r4;
+      case 5: return r5;
+      case 6: return r6;
+      default: return r7;
+    }
+  }
+  __device__ __forceinline__ void put(int i, uint64_t v) {
+    switch (i & 7) {
+      case 0: r0 = v; break;
+      case 1: r1 = v; break;
+      case 2: r2 = v; break;
+      case 3: r3 = v; break;
+      case 4: r4 = v; break;
+      case 5: r5 = v; break;
+      case 6: r6 = v; break;
+      default: r7 = v; break;
+    }
+  }
+};
+
+// Phase 2: run instructions [begin, end) for row j of the sub-tile (global row `row`).
+__device__ __forceinline__ void run_program(const tq_program_t& P, const tq_colset_t& C, int begin,
+                                            int end, int j, int64_t row, bool valid,
+                                            const uint64_t* __restrict__ stage, Regs& R) {
+  const int t = threadIdx.x;
+  for (int pc = begin; pc < end; ++pc) {
+    const tq_instr_t I = P.code[pc];
+    const uint64_t a = R.get(I.a);
+    const uint64_t b = R.get(I.b);
+    uint64_t o;
+    switch (I.op) {
+      case TQ_OP_LOAD: o = stage[(I.a * kSub + j) * kBlock + t]; break;
+      case TQ_OP_CONST: o = (uint64_t)I.imm; break;
+      case TQ_OP_STR_EQ:
+      case TQ_OP_STR_PREFIX: {
+        const int8_t* p = (const int8_t*)col_ptr(C, I.a);
+        const int w = col_width(C, I.a);
+        const int off = (int)(I.imm & 0xffffffff);
+        const int len = (int)(I.imm >> 32);
+        const int upto = (I.op == TQ_OP_STR_EQ) ? w : len;
+        bool eq = valid;
+        if (eq) {
+          const int8_t* s = p + row * (int64_t)w;
+          for (int c = 0; c < upto; ++c) {
+            const int8_t want = c < len ? (int8_t)P.strpool[off + c] : (int8_t)0;
+            eq = eq && (s[c] == want);
+          }
+        }
+        o = eq;
+        break;
+      }
+      case TQ_OP_CVT_I64_F64: o = f64_bits((double)(int64_t)a); break;
+      case TQ_OP_CVT_I64_F32: o = f32_bits((float)(int64_t)a); break;
+      case TQ_OP_CVT_F32_F64: o = f64_bits((double)as_f32(a)); break;
+      case TQ_OP_CVT_F64_F32: o = f32_bits((float)as_f64(a)); break;
+      case TQ_OP_WRAP_I8: o = (uint64_t)(int64_t)(int8_t)a; break;
+      case TQ_OP_WRAP_I16: o = (uint64_t)(int64_t)(int16_t)a; break;
+      case TQ_OP_WRAP_I32: o = (uint64_t)(int64_t)(int32_t)a; break;
+      case TQ_OP_ADD_I64: o = a + b; break;
+      case TQ_OP_SUB_I64: o = a - b; break;
+      case TQ_OP_MUL_I64: o = a * b; break;
+      case TQ_OP_ADD_F64: o = f64_bits(as_f64(a) + as_f64(b)); break;
+      case TQ_OP_SUB_F64: o = f64_bits(as_f64(a) - as_f64(b)); break;
+      case TQ_OP_MUL_F64: o = f64_bits(as_f64(a) * as_f64(b)); break;
+      case TQ_OP_DIV_F64: o = f64_bits(as_f64(a) / as_f64(b)); break;
+      case TQ_OP_ADD_F32: o = f32_bits(as_f32(a) + as_f32(b)); break;
+      case TQ_OP_SUB_F32: o = f32_bits(as_f32(a) - as_f32(b)); break;
+      case TQ_OP_MUL_F32: o = f32_bits(as_f32(a) * as_f32(b)); break;
+      case TQ_OP_DIV_F32: o = f32_bits(as_f32(a) / as_f32(b)); break;
+      case TQ_OP_LT_I64: o = (int64_t)a < (int64_t)b; break;
+      case TQ_OP_LE_I64: o = (int64_t)a <= (int64_t)b; break;
+      case TQ_OP_GT_I64: o = (int64_t)a > (int64_t)b; break;
+      case TQ_OP_GE_I64: o = (int64_t)a >= (int64_t)b; break;
+      case TQ_OP_EQ_I64: o = a == b; break;
+      case TQ_OP_NE_I64: o = a != b; break;
+      case TQ_OP_LT_F64: o = as_f64(a) < as_f64(b); break;
+      case TQ_OP_LE_F64: o = as_f64(a) <= as_f64(b); break;
+      case TQ_OP_GT_F64: o = as_f64(a) > as_f64(b); break;
+      case TQ_OP_GE_F64: o = as_f64(a) >= as_f64(b); break;
+      case TQ_OP_EQ_F64: o = as_f64(a) == as_f64(b); break;
+      case TQ_OP_NE_F64: o = as_f64(a) != as_f64(b); break;
+      case TQ_OP_LT_F32: o = as_f32(a) < as_f32(b); break;
+      case TQ_OP_LE_F32: o = as_f32(a) <= as_f32(b); break;
+      case TQ_OP_GT_F32: o = as_f32(a) > as_f32(b); break;
+      case TQ_OP_GE_F32: o = as_f32(a) >= as_f32(b); break;
+      case TQ_OP_EQ_F32: o = as_f32(a) == as_f32(b); break;
+      case TQ_OP_NE_F32: o = as_f32(a) != as_f32(b); break;
+      case TQ_OP_ISCLOSE_F64: {
+        // torch.isclose: a == b, or |a-b| finite and <= atol + |rtol * b| with rtol = 1e-5
+        const double x = as_f64(a), y = as_f64(b);
+        const double err = fabs(x - y);
+        const double allowed = __longlong_as_double(I.imm) + fabs(1e-5 * y);
+        o = (x == y) || (isfinite(err) && err <= allowed);
+        break;
+      }
+      case TQ_OP_AND: o = (a != 0) && (b != 0); break;
+      case TQ_OP_OR: o = (a != 0) || (b != 0); break;
+      case TQ_OP_NOT: o = a == 0; break;
+      case TQ_OP_WHERE: o = R.get(I.c) ? a : b; break;
+      case TQ_OP_YEAR: o = (uint64_t)year_of_day((int64_t)a); break;
+      default: o = 0; break;
+    }
+    R.put(I.dst, o);
+  }
+}
+
+}  // namespace tq

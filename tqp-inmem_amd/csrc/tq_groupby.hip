@@ -1,0 +1,654 @@
+// Group-by aggregation on packed keys.
+//
+// Keys are packed in mixed radix from their [min, max] ranges, so the packed value is the key
+// tuple's rank in lexicographic order and every table below is addressed by it:
+//   DENSE_LDS    - the whole key space fits a per-workgroup LDS array; each workgroup
+//                  accumulates its rows there and merges occupied slots into the global array
+//                  with one atomic per slot and aggregate;
+//   DENSE_GLOBAL - slots of one global array addressed directly by the packed key;
+//   HASH         - a global open-addressing table (linear probing, CAS on the key word).
+// With the LDS_FRONT flag, a per-workgroup LDS open-addressing table (half-full limit) absorbs
+// repeated keys first and only keys it cannot hold go to the global table, which keeps hot
+// groups (skewed keys) off global atomics.
+// Dense tables come out in key order by compacting occupied slots; hash tables are ordered by a
+// radix sort of their packed keys.  Sums use float64 atomics (hardware global/LDS f64 add), so
+// their summation order varies between runs; counts and integer sums are exact.
+#include "tq_common.h"
+#include "tq_expr.h"
+#include "tq_sort.h"
+
+namespace tq {
+
+enum AccKind { ACC_FSUM = 0, ACC_ISUM, ACC_COUNT, ACC_FMIN, ACC_FMAX, ACC_IMIN, ACC_IMAX };
+
+constexpr unsigned long long kEmpty = ~0ull;
+constexpr int kLdsBudget = 64 * 1024;
+
+struct GbSpec {
+  int32_t nkeys, naggs;
+  int32_t key_slot[TQ_GB_MAX_KEYS];  // load slot holding key i
+  int64_t key_min[TQ_GB_MAX_KEYS];
+  int64_t key_stride[TQ_GB_MAX_KEYS];
+  int64_t key_range[TQ_GB_MAX_KEYS];
+  int32_t key_dtype[TQ_GB_MAX_KEYS];
+  int32_t kind[TQ_GB_MAX_AGGS];
+  int32_t arg_dtype[TQ_GB_MAX_AGGS];
+  uint64_t init[TQ_GB_MAX_AGGS];
+};
+
+struct GbTable {
+  uint32_t* cnt;
+  unsigned long long* hkey;
+  uint64_t* acc;  // acc[j * slots + slot]
+  int64_t slots;
+  int32_t log2;
+};
+
+__device__ __forceinline__ uint64_t mix(uint64_t p) { return p * 0x9E3779B97F4A7C15ull; }
+
+// Convert an argument value (program register bits) into its accumulator domain.
+__device__ __forceinline__ uint64_t to_acc(int kind, int dt, uint64_t v) {
+  const bool isf = dt == TQ_F64 || dt == TQ_F32;
+  const double d = dt == TQ_F64 ? as_f64(v) : (double)as_f32(v);
+  switch (kind) {
+    case ACC_FSUM: case ACC_FMIN: case ACC_FMAX: return f64_bits(isf ? d : (double)(int64_t)v);
+    case ACC_COUNT: return isf ? (uint64_t)(d != -64.0) : (uint64_t)((int64_t)v != -64);
+    default: return v;  // integer domain
+  }
+}
+
+__device__ __forceinline__ void acc_apply(uint64_t* a, int kind, uint64_t v) {
+  switch (kind) {
+    case ACC_FSUM: atomicAdd((double*)a, as_f64(v)); break;
+    case ACC_ISUM:
+    case ACC_COUNT: atomicAdd((unsigned long long*)a, (unsigned long long)v); break;
+    case ACC_FMIN: atomicMin((double*)a, as_f64(v)); break;
+    case ACC_FMAX: atomicMax((double*)a, as_f64(v)); break;
+    case ACC_IMIN: atomicMin((long long*)a, (long long)v); break;
+    default: atomicMax((long long*)a, (long long)v); break;
+  }
+}
+
+__device__ __forceinline__ int64_t gb_insert(const GbTable& T, uint64_t p) {
+  const uint64_t mask = (uint64_t)T.slots - 1;
+  uint64_t h = mix(p) >> (64 - T.log2);
+  while (true) {
+    const unsigned long long k = T.hkey[h];
+    if (k == p) return (int64_t)h;
+    if (k == kEmpty) {
+      const unsigned long long old = atomicCAS(&T.hkey[h], kEmpty, (unsigned long long)p);
+      if (old == kEmpty || old == p) return (int64_t)h;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ int64_t global_slot(int base_mode, const GbTable& T, uint64_t p) {
+  return base_mode == TQ_GB_HASH ? gb_insert(T, p) : (int64_t)p;
+}
+
+// ------------------------------------------------------------------ bounds
+__global__ __launch_bounds__(kBlock) void k_gb_init_bounds(int64_t* out, int nkeys) {
+  const int i = threadIdx.x;
+  if (i < nkeys) {
+    out[2 * i] = INT64_MAX;
+    out[2 * i + 1] = INT64_MIN;
+  }
+  if (i == 0) out[2 * nkeys] = 0;
+}
+
+struct KeyCols {
+  const void* ptr[TQ_GB_MAX_KEYS];
+  int32_t dtype[TQ_GB_MAX_KEYS];
+  int32_t nkeys;
+};
+
+__global__ __launch_bounds__(kBlock) void k_gb_bounds(KeyCols K, const uint8_t* __restrict__ mask,
+                                                      int64_t n, int64_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int64_t cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    cnt += (!mask || mask[i] != 0);
+  cnt = wave_sum(cnt);
+  if (lane_id() == 0 && cnt) atomicAdd((unsigned long long*)&out[2 * K.nkeys], (unsigned long long)cnt);
+  for (int k = 0; k < K.nkeys; ++k) {
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+      if (mask && mask[i] == 0) continue;
+      const int64_t v = load_int(K.ptr[k], K.dtype[k], i);
+      lo = v < lo ? v : lo;
+      hi = v > hi ? v : hi;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int64_t a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
+      lo = a < lo ? a : lo;
+      hi = b > hi ? b : hi;
+    }
+    if (lane_id() == 0) {
+      if (lo != INT64_MAX) atomicMin((long long*)&out[2 * k], (long long)lo);
+      if (hi != INT64_MIN) atomicMax((long long*)&out[2 * k + 1], (long long)hi);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ accumulate
+__global__ __launch_bounds__(kBlock) void k_gb_clear(GbTable T, GbSpec G, int hashed) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < T.slots; s += stride) {
+    T.cnt[s] = 0;
+    if (hashed) T.hkey[s] = kEmpty;
+    for (int j = 0; j < G.naggs; ++j) T.acc[(int64_t)j * T.slots + s] = G.init[j];
+  }
+}
+
+__device__ __forceinline__ uint64_t pack_key(const GbSpec& G, const uint64_t* stage, int j) {
+  uint64_t p = 0;
+  for (int k = 0; k < G.nkeys; ++k) {
+    const int64_t v = (int64_t)stage[(G.key_slot[k] * kSub + j) * kBlock + threadIdx.x];
+    p += (uint64_t)(v - G.key_min[k]) * (uint64_t)G.key_stride[k];
+  }
+  return p;
+}
+
+// Accumulate one row (sub-tile row j) into slot `slot` of a table whose accumulators live at
+// acc[j * stride + slot].
+__device__ __forceinline__ void accumulate_row(const GbSpec& G, const tq_program_t& P,
+                                               const tq_colset_t& C, int j, int64_t row,
+                                               const uint64_t* stage, uint32_t* cnt,
+                                               uint64_t* acc, int64_t stride, int64_t slot) {
+  atomicAdd(&cnt[slot], 1u);
+  Regs R{};
+  int begin = 0;
+  for (int a = 0; a < G.naggs; ++a) {
+    const int end = P.seg_end[a];
+    run_program(P, C, begin, end, j, row, true, stage, R);
+    begin = end;
+    const uint64_t v = to_acc(G.kind[a], G.arg_dtype[a], R.get(P.seg_result[a]));
+    acc_apply(&acc[(int64_t)a * stride + slot], G.kind[a], v);
+  }
+}
+
+template <int BASE, bool FRONT>
+__global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, tq_colset_t C,
+                                                     const uint8_t* __restrict__ mask, int64_t n,
+                                                     GbTable T, int32_t cap, int32_t stage_words) {
+  extern __shared__ uint64_t smem[];
+  uint64_t* stage = smem;
+  uint64_t* lacc = smem + stage_words;           // naggs * cap accumulators
+  unsigned long long* lkey = (unsigned long long*)(lacc + (int64_t)G.naggs * cap);
+  uint32_t* lcnt = (uint32_t*)(lkey + (FRONT ? cap : 0));
+  __shared__ uint32_t fill;
+  const bool use_lds = BASE == TQ_GB_DENSE_LDS || FRONT;
+  const int t = threadIdx.x;
+  if (use_lds) {
+    for (int s = t; s < cap; s += kBlock) {
+      lcnt[s] = 0;
+      if (FRONT) lkey[s] = kEmpty;
+      for (int a = 0; a < G.naggs; ++a) lacc[(int64_t)a * cap + s] = G.init[a];
+    }
+    if (t == 0) fill = 0;
+    __syncthreads();
+  }
+  const int64_t nsub = ceil_div(n, kSubRows);
+  const uint32_t limit = (uint32_t)cap / 2;
+  for (int64_t sub = blockIdx.x; sub < nsub; sub += gridDim.x) {
+    const int64_t sub0 = sub * kSubRows;
+    bool live[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      const int64_t r = sub0 + (int64_t)j * kBlock + t;
+      live[j] = r < n && (!mask || mask[r] != 0);
+    }
+    stage_loads(P, C, sub0, live, stage);
+#pragma unroll 1
+    for (int j = 0; j < kSub; ++j) {
+      if (!live[j]) continue;
+      const int64_t row = sub0 + (int64_t)j * kBlock + t;
+      const uint64_t p = pack_key(G, stage, j);
+      if (BASE == TQ_GB_DENSE_LDS) {
+        accumulate_row(G, P, C, j, row, stage, lcnt, lacc, cap, (int64_t)p);
+        continue;
+      }
+      if (FRONT) {
+        const uint32_t m = (uint32_t)cap - 1;
+        uint32_t h = (uint32_t)(mix(p) >> 40) & m;
+        int64_t ls = -1;
+        for (int probe = 0; probe < cap; ++probe) {
+          const unsigned long long k = lkey[h];
+          if (k == p) { ls = h; break; }
+          if (k == kEmpty) {
+            if (fill >= limit) break;
+            const unsigned long long old = atomicCAS(&lkey[h], kEmpty, (unsigned long long)p);
+            if (old == kEmpty) { atomicAdd(&fill, 1u); ls = h; break; }
+            if (old == p) { ls = h; break; }
+          }
+          h = (h + 1) & m;
+        }
+        if (ls >= 0) {
+          accumulate_row(G, P, C, j, row, stage, lcnt, lacc, cap, ls);
+          continue;
+        }
+      }
+      const int64_t gs = global_slot(BASE, T, p);
+      accumulate_row(G, P, C, j, row, stage, T.cnt, T.acc, T.slots, gs);
+    }
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int s = t; s < cap; s += kBlock) {
+      const uint32_t c = lcnt[s];
+      if (c == 0) continue;
+      const uint64_t p = FRONT ? (uint64_t)lkey[s] : (uint64_t)s;
+      const int64_t gs = global_slot(BASE, T, p);
+      atomicAdd(&T.cnt[gs], c);
+      for (int a = 0; a < G.naggs; ++a)
+        acc_apply(&T.acc[(int64_t)a * T.slots + gs], G.kind[a], lacc[(int64_t)a * cap + s]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ count / write
+__global__ __launch_bounds__(kBlock) void k_gb_occupied(const uint32_t* __restrict__ cnt,
+                                                        int64_t slots, int force0,
+                                                        uint32_t* __restrict__ tile_counts) {
+  __shared__ uint32_t sh[kWaves];
+  const int64_t base = (int64_t)blockIdx.x * kTileRows + threadIdx.x;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t s = base + (int64_t)j * kBlock;
+    if (s < slots) c += (cnt[s] > 0) || (force0 && s == 0);
+  }
+  c = wave_sum(c);
+  if (lane_id() == 0) sh[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s += sh[w];
+    tile_counts[blockIdx.x] = s;
+  }
+}
+
+struct OutCols {
+  void* key[TQ_GB_MAX_KEYS];
+  void* agg[TQ_GB_MAX_AGGS];
+  int32_t op[TQ_GB_MAX_AGGS];
+};
+
+__device__ __forceinline__ void store_int(void* p, int dt, int64_t i, int64_t v) {
+  switch (dt) {
+    case TQ_BOOL: ((uint8_t*)p)[i] = v != 0; break;
+    case TQ_I8: ((int8_t*)p)[i] = (int8_t)v; break;
+    case TQ_I16: ((int16_t*)p)[i] = (int16_t)v; break;
+    case TQ_I32: ((int32_t*)p)[i] = (int32_t)v; break;
+    default: ((int64_t*)p)[i] = v; break;
+  }
+}
+
+__device__ __forceinline__ void write_group(const GbSpec& G, const OutCols& O, const GbTable& T,
+                                            int64_t o, int64_t slot, uint64_t p) {
+  for (int k = 0; k < G.nkeys; ++k) {
+    const int64_t v = (int64_t)((p / (uint64_t)G.key_stride[k]) % (uint64_t)G.key_range[k]) + G.key_min[k];
+    store_int(O.key[k], G.key_dtype[k], o, v);
+  }
+  const uint32_t rows = T.cnt[slot];
+  for (int a = 0; a < G.naggs; ++a) {
+    const uint64_t acc = T.acc[(int64_t)a * T.slots + slot];
+    const int dt = G.arg_dtype[a];
+    switch (O.op[a]) {
+      case TQ_AGG_AVG: {
+        const double sum = G.kind[a] == ACC_FSUM ? as_f64(acc) : (double)(int64_t)acc;
+        const double r = sum / (double)rows;
+        if (dt == TQ_F32) ((float*)O.agg[a])[o] = (float)r; else ((double*)O.agg[a])[o] = r;
+        break;
+      }
+      case TQ_AGG_COUNT: ((int64_t*)O.agg[a])[o] = (int64_t)acc; break;
+      case TQ_AGG_SUM:
+        if (G.kind[a] == ACC_ISUM) ((int64_t*)O.agg[a])[o] = (int64_t)acc;
+        else if (dt == TQ_F32) ((float*)O.agg[a])[o] = (float)as_f64(acc);
+        else ((double*)O.agg[a])[o] = as_f64(acc);
+        break;
+      default:  // MIN / MAX in the argument dtype
+        if (dt == TQ_F64) ((double*)O.agg[a])[o] = as_f64(acc);
+        else if (dt == TQ_F32) ((float*)O.agg[a])[o] = (float)as_f64(acc);
+        else store_int(O.agg[a], dt, o, (int64_t)acc);
+        break;
+    }
+  }
+}
+
+// Dense tables: groups are the occupied slots in slot (= key) order.
+__global__ __launch_bounds__(kBlock) void k_gb_write_dense(GbSpec G, OutCols O, GbTable T,
+                                                           const int64_t* __restrict__ tile_off,
+                                                           int force0) {
+  __shared__ uint32_t wcnt[kItems][kWaves];
+  const int t = threadIdx.x, w = t >> 6, l = lane_id();
+  const int64_t tile0 = (int64_t)blockIdx.x * kTileRows;
+  bool f[kItems];
+  uint64_t bal[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t s = tile0 + (int64_t)j * kBlock + t;
+    f[j] = s < T.slots && ((T.cnt[s] > 0) || (force0 && s == 0));
+    bal[j] = ballot(f[j]);
+    if (l == 0) wcnt[j][w] = (uint32_t)__popcll(bal[j]);
+  }
+  __syncthreads();
+  int64_t run = tile_off[blockIdx.x];
+#pragma unroll 1
+  for (int j = 0; j < kItems; ++j) {
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      const uint32_t c = wcnt[j][v];
+      tot += c;
+      before += (v < w) ? c : 0;
+    }
+    if (f[j]) {
+      const int64_t s = tile0 + (int64_t)j * kBlock + t;
+      write_group(G, O, T, run + before + rank_below(bal[j]), s, (uint64_t)s);
+    }
+    run += tot;
+  }
+}
+
+// Hash tables, step 1: compact (packed key, slot) of occupied slots.
+__global__ __launch_bounds__(kBlock) void k_gb_compact_hash(GbTable T,
+                                                            const int64_t* __restrict__ tile_off,
+                                                            uint64_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ vals) {
+  __shared__ uint32_t wcnt[kItems][kWaves];
+  const int t = threadIdx.x, w = t >> 6, l = lane_id();
+  const int64_t tile0 = (int64_t)blockIdx.x * kTileRows;
+  bool f[kItems];
+  uint64_t bal[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t s = tile0 + (int64_t)j * kBlock + t;
+    f[j] = s < T.slots && T.cnt[s] > 0;
+    bal[j] = ballot(f[j]);
+    if (l == 0) wcnt[j][w] = (uint32_t)__popcll(bal[j]);
+  }
+  __syncthreads();
+  int64_t run = tile_off[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      const uint32_t c = wcnt[j][v];
+      tot += c;
+      before += (v < w) ? c : 0;
+    }
+    if (f[j]) {
+      const int64_t s = tile0 + (int64_t)j * kBlock + t;
+      const int64_t o = run + before + rank_below(bal[j]);
+      keys[o] = T.hkey[s];
+      vals[o] = (uint32_t)s;
+    }
+    run += tot;
+  }
+}
+
+// Hash tables, step 3: write groups in sorted packed-key order.
+__global__ __launch_bounds__(kBlock) void k_gb_write_sorted(GbSpec G, OutCols O, GbTable T,
+                                                            const uint64_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ vals,
+                                                            int64_t ngroups) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < ngroups; i += stride)
+    write_group(G, O, T, i, vals[i], keys[i]);
+}
+
+// ------------------------------------------------------------------ host side
+struct GbWs {
+  GbTable T;
+  uint32_t* tile_counts;
+  int64_t* tile_off;
+  int64_t* ngroups;
+  int64_t* scan_ws;
+  uint64_t* k0;
+  uint64_t* k1;
+  uint32_t* v0;
+  uint32_t* v1;
+  void* stats;
+  void* radix;
+  size_t bytes;
+};
+
+static int ilog2_ceil(int64_t v) {
+  int l = 0;
+  while (((int64_t)1 << l) < v) ++l;
+  return l;
+}
+
+static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs) {
+  const int bm = mode & 7;
+  char* p = (char*)base;
+  const int64_t nt = ceil_div(slots, kTileRows);
+  GbWs w{};
+  w.T.slots = slots;
+  w.T.log2 = ilog2_ceil(slots);
+  w.T.cnt = (uint32_t*)p; p += align_up((size_t)slots * 4, 256);
+  w.T.hkey = (unsigned long long*)p; p += bm == TQ_GB_HASH ? align_up((size_t)slots * 8, 256) : 0;
+  w.T.acc = (uint64_t*)p; p += align_up((size_t)slots * 8 * (naggs > 0 ? naggs : 1), 256);
+  w.tile_counts = (uint32_t*)p; p += align_up((size_t)nt * 4, 256);
+  w.tile_off = (int64_t*)p; p += align_up((size_t)nt * 8, 256);
+  w.ngroups = (int64_t*)p; p += 256;
+  w.scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(nt);
+  if (bm == TQ_GB_HASH) {
+    w.k0 = (uint64_t*)p; p += align_up((size_t)slots * 8, 256);
+    w.k1 = (uint64_t*)p; p += align_up((size_t)slots * 8, 256);
+    w.v0 = (uint32_t*)p; p += align_up((size_t)slots * 4, 256);
+    w.v1 = (uint32_t*)p; p += align_up((size_t)slots * 4, 256);
+    w.stats = p; p += 256;
+    w.radix = p; p += radix_ws_bytes(slots);
+  }
+  w.bytes = (size_t)(p - (char*)base);
+  return w;
+}
+
+static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpec* G) {
+  *G = GbSpec{};
+  if (d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS) return set_error(TQ_EINVAL, "group-by: %d keys", d->nkeys);
+  if (d->naggs < 0 || d->naggs > TQ_GB_MAX_AGGS) return set_error(TQ_EINVAL, "group-by: %d aggregates", d->naggs);
+  G->nkeys = d->nkeys;
+  G->naggs = d->naggs;
+  for (int k = 0; k < d->nkeys; ++k) {
+    G->key_slot[k] = k;  // key i is load slot i of the program
+    G->key_min[k] = d->key_min[k];
+    G->key_stride[k] = d->key_stride[k] > 0 ? d->key_stride[k] : 1;
+    G->key_range[k] = d->key_range[k] > 0 ? d->key_range[k] : 1;
+    G->key_dtype[k] = d->key_dtype[k];
+    if (d->key_dtype[k] < TQ_BOOL || d->key_dtype[k] > TQ_I64)
+      return set_error(TQ_EUNSUPPORTED, "group-by key %d must be an integer column", k);
+    if (prog && (prog->nload <= k || prog->load_dtype[k] != d->key_dtype[k]))
+      return set_error(TQ_EINVAL, "group-by: load slot %d must hold key %d", k, k);
+  }
+  for (int a = 0; a < d->naggs; ++a) {
+    const int dt = d->agg_dtype[a];
+    const bool isf = dt == TQ_F64 || dt == TQ_F32;
+    G->arg_dtype[a] = dt;
+    switch (d->agg_op[a]) {
+      case TQ_AGG_SUM:
+      case TQ_AGG_AVG: G->kind[a] = isf ? ACC_FSUM : ACC_ISUM; G->init[a] = 0; break;
+      case TQ_AGG_COUNT: G->kind[a] = ACC_COUNT; G->init[a] = 0; break;
+      case TQ_AGG_MIN:
+        G->kind[a] = isf ? ACC_FMIN : ACC_IMIN;
+        if (dt == TQ_F64) { double v = 1.7976931348623157e308; memcpy(&G->init[a], &v, 8); }
+        else if (dt == TQ_F32) { double v = 3.4028234663852886e38; memcpy(&G->init[a], &v, 8); }
+        else {
+          const int64_t mx = dt == TQ_I8 ? 127 : dt == TQ_I16 ? 32767 : dt == TQ_I32 ? 2147483647
+                           : dt == TQ_BOOL ? 1 : INT64_MAX;
+          G->init[a] = (uint64_t)mx;
+        }
+        break;
+      case TQ_AGG_MAX: G->kind[a] = isf ? ACC_FMAX : ACC_IMAX; G->init[a] = 0; break;
+      default: return set_error(TQ_EUNSUPPORTED, "aggregate op %d", d->agg_op[a]);
+    }
+  }
+  if (prog && prog->nseg < d->naggs) return set_error(TQ_EINVAL, "group-by: program has %d segments for %d aggregates", prog->nseg, d->naggs);
+  return TQ_OK;
+}
+
+static size_t lds_stage_bytes(int32_t nload) { return (size_t)(nload > 0 ? nload : 1) * kSubRows * 8; }
+
+static int32_t lds_cap(int mode, int32_t nload, int naggs, int64_t slots) {
+  const size_t avail = kLdsBudget - lds_stage_bytes(nload);
+  if ((mode & 7) == TQ_GB_DENSE_LDS) return (int32_t)slots;
+  if (!(mode & TQ_GB_LDS_FRONT)) return 0;
+  const size_t per = 12 + 8 * (size_t)naggs;
+  int32_t cap = 1;
+  while ((size_t)cap * 2 * per <= avail) cap *= 2;
+  return cap;
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" int tq_groupby_bounds(const tq_groupby_desc_t* d, const uint8_t* mask, int64_t n,
+                                 int64_t* out, void* stream) {
+  if (!d || d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS) return set_error(TQ_EINVAL, "group-by bounds: bad descriptor");
+  hipStream_t s = (hipStream_t)stream;
+  KeyCols K{};
+  K.nkeys = d->nkeys;
+  for (int k = 0; k < d->nkeys; ++k) {
+    K.ptr[k] = d->key_ptr[k];
+    K.dtype[k] = d->key_dtype[k];
+    if (K.dtype[k] < TQ_BOOL || K.dtype[k] > TQ_I64)
+      return set_error(TQ_EUNSUPPORTED, "group-by key %d must be an integer column", k);
+  }
+  k_gb_init_bounds<<<1, kBlock, 0, s>>>(out, d->nkeys);
+  TQ_TRY(check_launch("bounds init"));
+  if (n <= 0) return TQ_OK;
+  int64_t g = ceil_div(n, kBlock * 8);
+  if (g > 2048) g = 2048;
+  k_gb_bounds<<<(unsigned)g, kBlock, 0, s>>>(K, mask, n, out);
+  return check_launch("bounds");
+}
+
+extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload, int32_t* mode,
+                               int64_t* slots, size_t* ws_bytes) {
+  if (!d || d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS || d->naggs < 0 || d->naggs > TQ_GB_MAX_AGGS)
+    return set_error(TQ_EINVAL, "group-by plan: bad descriptor");
+  // key space = product of ranges (a range <= 0 means no rows: one empty slot)
+  __int128 space = 1;
+  for (int k = d->nkeys - 1; k >= 0; --k) {
+    if (d->key_range[k] <= 0) d->key_range[k] = 1;
+    d->key_stride[k] = (int64_t)space;
+    space *= d->key_range[k];
+    if (space > ((__int128)1 << 62))
+      return set_error(TQ_EUNSUPPORTED, "group-by key space exceeds 62 bits");
+  }
+  const int64_t S = (int64_t)space;
+  const int64_t ns = nsel > 0 ? nsel : 0;
+  const size_t stage = lds_stage_bytes(nload);
+  const size_t dense_per = 4 + 8 * (size_t)d->naggs;
+  int32_t m;
+  int64_t sl;
+  if ((size_t)S * dense_per + stage <= (size_t)kLdsBudget) {
+    m = TQ_GB_DENSE_LDS;
+    sl = S;
+  } else {
+    const int64_t distinct = ns < S ? ns : S;
+    if (S <= ((int64_t)1 << 27) && S <= (4 * ns > 65536 ? 4 * ns : 65536)) {
+      m = TQ_GB_DENSE_GLOBAL;
+      sl = S;
+    } else {
+      m = TQ_GB_HASH;
+      sl = (int64_t)1 << ilog2_ceil(2 * (distinct > 2048 ? distinct : 2048));
+    }
+    const size_t per = 12 + 8 * (size_t)d->naggs;
+    const bool room = stage + 1024 * per <= (size_t)kLdsBudget;
+    if (room && distinct > 0 && ns >= 4 * distinct) m |= TQ_GB_LDS_FRONT;
+  }
+  *mode = m;
+  *slots = sl;
+  *ws_bytes = carve_gb(nullptr, m, sl, d->naggs).bytes;
+  return TQ_OK;
+}
+
+extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_program_t* prog,
+                                     const tq_colset_t* cols, const uint8_t* mask, int64_t n,
+                                     int32_t mode, int64_t slots, void* ws, size_t ws_bytes,
+                                     void* stream) {
+  if (!d || !prog || !cols) return set_error(TQ_EINVAL, "group-by accumulate: null argument");
+  GbSpec G;
+  TQ_TRY(make_spec(d, prog, &G));
+  if (prog->nload > TQ_MAX_LOADS || prog->n_instr > TQ_MAX_INSTR) return set_error(TQ_EINVAL, "group-by: malformed program");
+  GbWs w = carve_gb(ws, mode, slots, d->naggs);
+  if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "group-by workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int base = mode & 7;
+  const bool front = (mode & TQ_GB_LDS_FRONT) != 0;
+  k_gb_clear<<<stream_grid(slots), kBlock, 0, s>>>(w.T, G, base == TQ_GB_HASH);
+  TQ_TRY(check_launch("group-by clear"));
+  if (n <= 0) return TQ_OK;
+  const int32_t cap = lds_cap(mode, prog->nload, d->naggs, slots);
+  const int32_t stage_words = (int32_t)(lds_stage_bytes(prog->nload) / 8);
+  size_t lds = lds_stage_bytes(prog->nload);
+  if (base == TQ_GB_DENSE_LDS) lds += (size_t)cap * (8 * d->naggs + 4);
+  else if (front) lds += (size_t)cap * (8 * d->naggs + 12);
+  if (lds > (size_t)kLdsBudget) return set_error(TQ_ECAPACITY, "group-by: LDS table too large");
+  int64_t g = ceil_div(n, kSubRows);
+  if (g > 1024) g = 1024;
+  const unsigned grid = (unsigned)g;
+  if (base == TQ_GB_DENSE_LDS)
+    k_gb_accum<TQ_GB_DENSE_LDS, false><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, cap, stage_words);
+  else if (base == TQ_GB_DENSE_GLOBAL && front)
+    k_gb_accum<TQ_GB_DENSE_GLOBAL, true><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, cap, stage_words);
+  else if (base == TQ_GB_DENSE_GLOBAL)
+    k_gb_accum<TQ_GB_DENSE_GLOBAL, false><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, 1, stage_words);
+  else if (front)
+    k_gb_accum<TQ_GB_HASH, true><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, cap, stage_words);
+  else
+    k_gb_accum<TQ_GB_HASH, false><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, 1, stage_words);
+  return check_launch("group-by accumulate");
+}
+
+extern "C" int tq_groupby_count(const tq_groupby_desc_t* d, int32_t mode, int64_t slots, void* ws,
+                                int64_t* ngroups_host, void* stream) {
+  if (!d) return set_error(TQ_EINVAL, "group-by count: null descriptor");
+  GbWs w = carve_gb(ws, mode, slots, d->naggs);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nt = ceil_div(slots, kTileRows);
+  k_gb_occupied<<<(unsigned)nt, kBlock, 0, s>>>(w.T.cnt, slots, d->nkeys == 0, w.tile_counts);
+  TQ_TRY(check_launch("group-by occupied"));
+  TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.ngroups, w.scan_ws, s));
+  if (ngroups_host) TQ_TRY(read_back(w.ngroups, 8, ngroups_host, s));
+  return TQ_OK;
+}
+
+extern "C" int tq_groupby_write(const tq_groupby_desc_t* d, int32_t mode, int64_t slots, void* ws,
+                                int64_t ngroups, void* const* key_out, void* const* agg_out,
+                                void* stream) {
+  if (!d) return set_error(TQ_EINVAL, "group-by write: null descriptor");
+  GbSpec G;
+  TQ_TRY(make_spec(d, nullptr, &G));
+  GbWs w = carve_gb(ws, mode, slots, d->naggs);
+  hipStream_t s = (hipStream_t)stream;
+  if (ngroups <= 0) return TQ_OK;
+  OutCols O{};
+  for (int k = 0; k < d->nkeys; ++k) O.key[k] = key_out[k];
+  for (int a = 0; a < d->naggs; ++a) {
+    O.agg[a] = agg_out[a];
+    O.op[a] = d->agg_op[a];
+  }
+  const int64_t nt = ceil_div(slots, kTileRows);
+  if ((mode & 7) != TQ_GB_HASH) {
+    k_gb_write_dense<<<(unsigned)nt, kBlock, 0, s>>>(G, O, w.T, w.tile_off, d->nkeys == 0);
+    return check_launch("group-by write");
+  }
+  k_gb_compact_hash<<<(unsigned)nt, kBlock, 0, s>>>(w.T, w.tile_off, w.k0, w.v0);
+  TQ_TRY(check_launch("group-by compact"));
+  int lo, hi;
+  TQ_TRY(key_bit_range<uint64_t>(w.k0, ngroups, w.stats, s, &lo, &hi));
+  bool alt;
+  TQ_TRY(radix_sort_pairs<uint64_t>(w.k0, w.v0, w.k1, w.v1, ngroups, lo, hi, w.radix, s, &alt));
+  k_gb_write_sorted<<<stream_grid(ngroups), kBlock, 0, s>>>(G, O, w.T, alt ? w.k1 : w.k0,
+                                                            alt ? w.v1 : w.v0, ngroups);
+  return check_launch("group-by write");
+}

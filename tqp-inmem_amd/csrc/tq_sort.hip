@@ -1,0 +1,408 @@
+// LSD radix sort of (key, row id) pairs and the multi-key sort / top-k built on it.
+//
+// One pass sorts by an 8-bit digit in three launches:
+//   hist    - per 4096-key tile, a digit histogram (per-wave LDS counters), stored digit-major
+//             so that one exclusive scan yields every (digit, tile) output offset;
+//   scan    - device-wide exclusive scan of the 256 x tiles histogram;
+//   scatter - each tile ranks its keys stably: wave w owns the contiguous quarter
+//             [w*1024, (w+1)*1024) of the tile and walks it 64 keys at a time; lanes holding
+//             the same digit find each other with 8 ballots, the lowest of them bumps that
+//             digit's per-wave counter, and each lane's rank is the counter plus its mbcnt
+//             within the peer mask.  The tile is then reordered in LDS and written out in runs
+//             of equal digits, so global writes are coalesced.
+// Passes cover only the bit range in which keys differ (found by one OR/AND reduction), and
+// keys are order-preserving transforms of the typed columns, so unsigned order = value order.
+#include "tq_common.h"
+#include "tq_sort.h"
+
+namespace tq {
+
+constexpr int kBins = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;  // 4096
+constexpr int kWaveSeg = kSortTile / kWaves;    // 1024 keys per wave
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, int64_t n,
+                                                       int shift, uint32_t* __restrict__ hist,
+                                                       int64_t ntiles) {
+  __shared__ uint32_t cnt[kWaves][kBins];
+  const int t = threadIdx.x, w = t >> 6;
+  for (int i = t; i < kWaves * kBins; i += kBlock) (&cnt[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = tile0 + (int64_t)j * kBlock + t;
+    if (e < n) atomicAdd(&cnt[w][(uint32_t)(keys[e] >> shift) & (kBins - 1)], 1u);
+  }
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int v = 0; v < kWaves; ++v) s += cnt[v][t];
+  hist[(int64_t)t * ntiles + blockIdx.x] = s;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(
+    const K* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
+    const uint32_t* __restrict__ offs, int64_t ntiles, K* __restrict__ kout,
+    uint32_t* __restrict__ vout) {
+  __shared__ uint32_t wcnt[kWaves][kBins];
+  __shared__ uint32_t lstart[kBins];
+  __shared__ uint32_t gbase[kBins];
+  __shared__ uint32_t shs[kWaves];
+  __shared__ K skey[kSortTile];
+  __shared__ uint32_t sval[kSortTile];
+  const int t = threadIdx.x, w = t >> 6;
+  for (int i = t; i < kWaves * kBins; i += kBlock) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+  const int64_t wave0 = tile0 + (int64_t)w * kWaveSeg;
+  const int l = lane_id();
+  K key[kSortItems];
+  uint32_t val[kSortItems], dig[kSortItems], rnk[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = wave0 + (int64_t)j * 64 + l;
+    const bool valid = e < n;
+    key[j] = valid ? kin[e] : (K)0;
+    val[j] = valid ? vin[e] : 0u;
+    const uint32_t d = (uint32_t)(key[j] >> shift) & (kBins - 1);
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t m = ballot(valid && bit);
+      peers &= bit ? m : ~m;
+    }
+    const int below = rank_below(peers);
+    uint32_t base = 0;
+    if (valid) base = wcnt[w][d];
+    if (valid && below == 0) wcnt[w][d] = base + (uint32_t)__popcll(peers);
+    dig[j] = d;
+    rnk[j] = base + below;
+  }
+  __syncthreads();
+  {  // thread t owns digit t: exclusive prefix over waves, then over digits
+    uint32_t run = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      const uint32_t c = wcnt[v][t];
+      wcnt[v][t] = run;
+      run += c;
+    }
+    uint32_t tot;
+    lstart[t] = block_exclusive_sum(run, shs, &tot);
+    gbase[t] = offs[(int64_t)t * ntiles + blockIdx.x];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = wave0 + (int64_t)j * 64 + l;
+    if (e < n) {
+      const uint32_t lp = lstart[dig[j]] + wcnt[w][dig[j]] + rnk[j];
+      skey[lp] = key[j];
+      sval[lp] = val[j];
+    }
+  }
+  __syncthreads();
+  const int64_t rem = n - tile0;
+  const int nvalid = rem < kSortTile ? (int)rem : kSortTile;
+  for (int i = t; i < nvalid; i += kBlock) {
+    const K k = skey[i];
+    const uint32_t d = (uint32_t)(k >> shift) & (kBins - 1);
+    const uint32_t gp = gbase[d] + (uint32_t)i - lstart[d];
+    kout[gp] = k;
+    vout[gp] = sval[i];
+  }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_key_bits(const K* __restrict__ keys, int64_t n,
+                                                     unsigned long long* __restrict__ stats) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  uint64_t o = 0, a = ~0ull;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const uint64_t k = (uint64_t)keys[i];
+    o |= k;
+    a &= k;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    o |= __shfl_xor(o, d, 64);
+    a &= __shfl_xor(a, d, 64);
+  }
+  if (lane_id() == 0) {
+    atomicOr(&stats[0], (unsigned long long)o);
+    atomicAnd(&stats[1], (unsigned long long)a);
+  }
+}
+
+// ------------------------------------------------------------------ host driver
+size_t radix_ws_bytes(int64_t n) {
+  const int64_t ntiles = ceil_div(n > 0 ? n : 1, kSortTile);
+  const int64_t h = kBins * ntiles;
+  return align_up((size_t)h * 4, 256) * 2 + align_up((size_t)(ceil_div(h, kSortTile) + 1) * 4, 256) + 256;
+}
+
+template <typename K>
+int radix_sort_pairs(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
+                     int lo_bit, int hi_bit, void* ws, hipStream_t s, bool* in_alt) {
+  *in_alt = false;
+  if (n <= 1 || hi_bit <= lo_bit) return TQ_OK;
+  const int64_t ntiles = ceil_div(n, kSortTile);
+  const int64_t h = kBins * ntiles;
+  uint32_t* hist = (uint32_t*)ws;
+  uint32_t* offs = (uint32_t*)((char*)ws + align_up((size_t)h * 4, 256));
+  uint32_t* sws = (uint32_t*)((char*)offs + align_up((size_t)h * 4, 256));
+  K *ki = keys, *ko = keys_alt;
+  uint32_t *vi = vals, *vo = vals_alt;
+  for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+    k_radix_hist<K><<<(unsigned)ntiles, kBlock, 0, s>>>(ki, n, shift, hist, ntiles);
+    TQ_TRY((scan_exclusive<uint32_t, uint32_t>(hist, h, offs, nullptr, sws, s)));
+    k_radix_scatter<K><<<(unsigned)ntiles, kBlock, 0, s>>>(ki, vi, n, shift, offs, ntiles, ko, vo);
+    TQ_TRY(check_launch("radix pass"));
+    K* tk = ki; ki = ko; ko = tk;
+    uint32_t* tv = vi; vi = vo; vo = tv;
+    *in_alt = !*in_alt;
+  }
+  return TQ_OK;
+}
+
+template int radix_sort_pairs<uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, int64_t, int,
+                                        int, void*, hipStream_t, bool*);
+template int radix_sort_pairs<uint64_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, int64_t, int,
+                                        int, void*, hipStream_t, bool*);
+
+// Pinned host staging for the few values a host decision needs.
+int64_t* host_scratch() {
+  static thread_local int64_t* p = nullptr;
+  if (!p) {
+    if (hipHostMalloc((void**)&p, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) p = nullptr;
+  }
+  return p;
+}
+
+int read_back(const void* dev, size_t bytes, int64_t* host_out, hipStream_t s) {
+  int64_t* h = host_scratch();
+  if (!h) return set_error(TQ_EHIP, "pinned staging allocation failed");
+  TQ_TRY_HIP(hipMemcpyAsync(h, dev, bytes, hipMemcpyDeviceToHost, s));
+  TQ_TRY_HIP(hipStreamSynchronize(s));
+  memcpy(host_out, h, bytes);
+  return TQ_OK;
+}
+
+template <typename K>
+int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int* lo, int* hi) {
+  *lo = *hi = 0;
+  if (n <= 1) return TQ_OK;
+  unsigned long long* st = (unsigned long long*)stats_dev;
+  TQ_TRY(fill_u64(st, 0ull, 1, s));
+  TQ_TRY(fill_u64(st + 1, ~0ull, 1, s));
+  int64_t g = ceil_div(n, kBlock);
+  if (g > 2048) g = 2048;
+  k_key_bits<K><<<(unsigned)g, kBlock, 0, s>>>(keys, n, st);
+  TQ_TRY(check_launch("key bits"));
+  int64_t hv[2];
+  TQ_TRY(read_back(st, 16, hv, s));
+  const uint64_t diff = (uint64_t)hv[0] ^ (uint64_t)hv[1];
+  if (diff == 0) return TQ_OK;
+  const int low = __builtin_ctzll(diff), high = 64 - __builtin_clzll(diff);
+  *lo = (low / 8) * 8;
+  *hi = high;
+  return TQ_OK;
+}
+
+template int key_bit_range<uint32_t>(const uint32_t*, int64_t, void*, hipStream_t, int*, int*);
+template int key_bit_range<uint64_t>(const uint64_t*, int64_t, void*, hipStream_t, int*, int*);
+
+// ------------------------------------------------------------------ multi-key sort
+// A field is one typed scalar of a key (a 2-D key contributes one int8 field per column).
+struct Field {
+  const void* ptr;
+  int32_t dtype;
+  int32_t width;  // row stride in elements (2-D keys), 1 otherwise
+  int32_t col;    // column within a 2-D key
+  int32_t desc;
+  int32_t bits;
+  int32_t shift;  // position inside the packed key
+};
+
+struct FieldPack {
+  Field f[16];
+  int32_t n;
+};
+
+__device__ __forceinline__ uint64_t field_bits(const Field& f, int64_t r) {
+  uint64_t v;
+  switch (f.dtype) {
+    case TQ_BOOL: v = ((const uint8_t*)f.ptr)[r] != 0; break;
+    case TQ_I8: v = (uint8_t)(((const int8_t*)f.ptr)[r * f.width + f.col]) ^ 0x80u; break;
+    case TQ_I16: v = (uint16_t)((const int16_t*)f.ptr)[r] ^ 0x8000u; break;
+    case TQ_I32: v = (uint32_t)((const int32_t*)f.ptr)[r] ^ 0x80000000u; break;
+    case TQ_I64: v = (uint64_t)((const int64_t*)f.ptr)[r] ^ 0x8000000000000000ull; break;
+    case TQ_F32: v = f32_order_bits(((const float*)f.ptr)[r]); break;
+    default: v = f64_order_bits(((const double*)f.ptr)[r]); break;
+  }
+  if (f.desc) v = ~v;
+  if (f.bits < 64) v &= (1ull << f.bits) - 1;
+  return v << f.shift;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_pack_keys(FieldPack fp, const uint32_t* __restrict__ perm,
+                                                      int64_t n, K* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const uint32_t r = perm ? perm[i] : (uint32_t)i;
+    uint64_t k = 0;
+    for (int f = 0; f < fp.n; ++f) k |= field_bits(fp.f[f], r);
+    keys[i] = (K)k;
+    vals[i] = r;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_perm_out(const uint32_t* __restrict__ v, int64_t m,
+                                                     int64_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += stride) out[i] = v[i];
+}
+
+static int bits_of(int dtype) {
+  switch (dtype) {
+    case TQ_BOOL: case TQ_I8: return 8;
+    case TQ_I16: return 16;
+    case TQ_I32: case TQ_F32: return 32;
+    default: return 64;
+  }
+}
+
+struct SortWs {
+  uint64_t* k0;
+  uint64_t* k1;
+  uint32_t* v0;
+  uint32_t* v1;
+  void* stats;
+  void* radix;
+};
+
+static SortWs carve_sort_ws(void* ws, int64_t n) {
+  char* p = (char*)ws;
+  SortWs w;
+  const size_t kb = align_up((size_t)n * 8, 256), vb = align_up((size_t)n * 4, 256);
+  w.k0 = (uint64_t*)p; p += kb;
+  w.k1 = (uint64_t*)p; p += kb;
+  w.v0 = (uint32_t*)p; p += vb;
+  w.v1 = (uint32_t*)p; p += vb;
+  w.stats = p; p += 256;
+  w.radix = p;
+  return w;
+}
+
+size_t sort_ws_bytes(int64_t n) {
+  if (n < 1) n = 1;
+  return 2 * align_up((size_t)n * 8, 256) + 2 * align_up((size_t)n * 4, 256) + 256 +
+         radix_ws_bytes(n);
+}
+
+// Sort one packed group of fields stably, starting from permutation `perm` (null = identity);
+// leaves the new permutation in *perm_out (one of the workspace value buffers).
+static int sort_group(const FieldPack& fp, int total_bits, const uint32_t* perm, int64_t n,
+                      SortWs& w, hipStream_t s, uint32_t** perm_out) {
+  unsigned g = stream_grid(n);
+  if (total_bits <= 32) {
+    uint32_t* k0 = (uint32_t*)w.k0;
+    uint32_t* k1 = (uint32_t*)w.k1;
+    uint32_t* v0 = perm == w.v0 ? w.v1 : w.v0;
+    uint32_t* v1 = perm == w.v0 ? w.v0 : w.v1;
+    k_pack_keys<uint32_t><<<g, kBlock, 0, s>>>(fp, perm, n, k0, v0);
+    TQ_TRY(check_launch("pack keys"));
+    int lo, hi;
+    TQ_TRY(key_bit_range<uint32_t>(k0, n, w.stats, s, &lo, &hi));
+    bool alt;
+    TQ_TRY(radix_sort_pairs<uint32_t>(k0, v0, k1, v1, n, lo, hi, w.radix, s, &alt));
+    *perm_out = alt ? v1 : v0;
+  } else {
+    uint32_t* v0 = perm == w.v0 ? w.v1 : w.v0;
+    uint32_t* v1 = perm == w.v0 ? w.v0 : w.v1;
+    k_pack_keys<uint64_t><<<g, kBlock, 0, s>>>(fp, perm, n, w.k0, v0);
+    TQ_TRY(check_launch("pack keys"));
+    int lo, hi;
+    TQ_TRY(key_bit_range<uint64_t>(w.k0, n, w.stats, s, &lo, &hi));
+    bool alt;
+    TQ_TRY(radix_sort_pairs<uint64_t>(w.k0, v0, w.k1, v1, n, lo, hi, w.radix, s, &alt));
+    *perm_out = alt ? v1 : v0;
+  }
+  return TQ_OK;
+}
+
+int sort_perm_u32(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, void* ws, size_t ws_bytes,
+                  hipStream_t s, uint32_t** perm_out) {
+  if (ws_bytes < sort_ws_bytes(n)) return set_error(TQ_ECAPACITY, "sort workspace too small");
+  if (n >= (int64_t)1 << 32) return set_error(TQ_EINVAL, "sort of %lld rows exceeds 2^32", (long long)n);
+  // Expand keys into fields, most significant first.
+  Field fields[64];
+  int nf = 0;
+  for (int k = 0; k < nkeys; ++k) {
+    const tq_sortkey_t& sk = keys[k];
+    const int w = sk.width > 0 ? sk.width : 1;
+    if (w > 1 && sk.dtype != TQ_I8 && sk.dtype != TQ_BOOL)
+      return set_error(TQ_EUNSUPPORTED, "2-D sort keys must be int8");
+    for (int c = 0; c < w; ++c) {
+      if (nf >= 64) return set_error(TQ_EUNSUPPORTED, "more than 64 sort key fields");
+      fields[nf++] = Field{sk.ptr, sk.dtype, w, c, sk.descending ? 1 : 0, bits_of(sk.dtype), 0};
+    }
+  }
+  SortWs w = carve_sort_ws(ws, n);
+  uint32_t* perm = nullptr;
+  // Group fields from the least significant end into packed keys of <= 64 bits; sort the
+  // groups least significant first (LSD across groups, each group a stable sort).
+  int end = nf;
+  while (end > 0) {
+    int begin = end, bits = 0;
+    while (begin > 0 && bits + fields[begin - 1].bits <= 64 && (end - begin) < 16) {
+      bits += fields[begin - 1].bits;
+      --begin;
+    }
+    FieldPack fp{};
+    fp.n = end - begin;
+    int shift = 0;
+    for (int f = end - 1; f >= begin; --f) {
+      fp.f[f - begin] = fields[f];
+      fp.f[f - begin].shift = shift;
+      shift += fields[f].bits;
+    }
+    uint32_t* np;
+    TQ_TRY(sort_group(fp, bits, perm, n, w, s, &np));
+    perm = np;
+    end = begin;
+  }
+  if (!perm) {  // no keys: identity
+    FieldPack fp{};
+    k_pack_keys<uint32_t><<<stream_grid(n), kBlock, 0, s>>>(fp, nullptr, n, (uint32_t*)w.k0, w.v0);
+    TQ_TRY(check_launch("identity"));
+    perm = w.v0;
+  }
+  *perm_out = perm;
+  return TQ_OK;
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" size_t tq_sort_ws_bytes(int64_t n) { return sort_ws_bytes(n); }
+
+extern "C" int tq_sort_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t limit,
+                            int64_t* perm_out, void* ws, size_t ws_bytes, void* stream) {
+  if (n <= 0) return TQ_OK;
+  if (nkeys < 0 || (nkeys > 0 && !keys)) return set_error(TQ_EINVAL, "sort: bad key list");
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t* perm;
+  TQ_TRY(sort_perm_u32(keys, nkeys, n, ws, ws_bytes, s, &perm));
+  const int64_t m = (limit >= 0 && limit < n) ? limit : n;
+  k_perm_out<<<stream_grid(m), kBlock, 0, s>>>(perm, m, perm_out);
+  return check_launch("sort perm");
+}

@@ -1,0 +1,219 @@
+"""ctypes binding of libtqhip.so (the gfx950 kernels of the relational hot path).
+
+The structures below mirror include/tqhip.h field for field.  `call()` turns a negative status
+into the Python exception the reference raises for the same kind of failure (ValueError,
+NotImplementedError, RuntimeError, MemoryError).  There is no fallback: importing this module
+without the built library, or calling it without a GPU, raises.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtqhip.so")
+
+# ---------------------------------------------------------------- constants (tqhip.h)
+TQ_OK, TQ_EINVAL, TQ_EUNSUPPORTED, TQ_EHIP, TQ_ECAPACITY = 0, -1, -2, -3, -4
+TQ_BOOL, TQ_I8, TQ_I16, TQ_I32, TQ_I64, TQ_F32, TQ_F64 = range(7)
+
+TQ_MAX_INSTR = 64
+TQ_MAX_COLS = 16
+TQ_NREG = 8
+TQ_STRPOOL = 128
+TQ_MAX_SEGS = 12
+TQ_MAX_LOADS = 8
+
+OPCODES = [
+    "NOP", "LOAD", "CONST",
+    "CVT_I64_F64", "CVT_I64_F32", "CVT_F32_F64", "CVT_F64_F32",
+    "WRAP_I8", "WRAP_I16", "WRAP_I32",
+    "ADD_I64", "SUB_I64", "MUL_I64",
+    "ADD_F64", "SUB_F64", "MUL_F64", "DIV_F64",
+    "ADD_F32", "SUB_F32", "MUL_F32", "DIV_F32",
+    "LT_I64", "LE_I64", "GT_I64", "GE_I64", "EQ_I64", "NE_I64",
+    "LT_F64", "LE_F64", "GT_F64", "GE_F64", "EQ_F64", "NE_F64",
+    "LT_F32", "LE_F32", "GT_F32", "GE_F32", "EQ_F32", "NE_F32",
+    "ISCLOSE_F64",
+    "AND", "OR", "NOT",
+    "WHERE",
+    "STR_EQ", "STR_PREFIX",
+    "YEAR",
+]
+OP = {name: i for i, name in enumerate(OPCODES)}
+
+TQ_AGG_SUM, TQ_AGG_COUNT, TQ_AGG_AVG, TQ_AGG_MIN, TQ_AGG_MAX = range(5)
+TQ_GB_MAX_KEYS = 8
+TQ_GB_DENSE_LDS, TQ_GB_DENSE_GLOBAL, TQ_GB_HASH, TQ_GB_LDS_FRONT = 0, 1, 2, 8
+
+DTYPE_CODE = {
+    torch.bool: TQ_BOOL, torch.int8: TQ_I8, torch.int16: TQ_I16, torch.int32: TQ_I32,
+    torch.int64: TQ_I64, torch.float32: TQ_F32, torch.float64: TQ_F64,
+}
+CODE_DTYPE = {v: k for k, v in DTYPE_CODE.items()}
+
+
+# ---------------------------------------------------------------- structures
+class Instr(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint8), ("dst", ctypes.c_uint8), ("a", ctypes.c_uint8),
+                ("b", ctypes.c_uint8), ("c", ctypes.c_uint8), ("pad", ctypes.c_uint8 * 3),
+                ("imm", ctypes.c_int64)]
+
+
+class Program(ctypes.Structure):
+    _fields_ = [("n_instr", ctypes.c_int32), ("nseg", ctypes.c_int32),
+                ("seg_end", ctypes.c_int16 * TQ_MAX_SEGS),
+                ("seg_result", ctypes.c_uint8 * TQ_MAX_SEGS),
+                ("nload", ctypes.c_int32),
+                ("load_col", ctypes.c_uint8 * TQ_MAX_LOADS),
+                ("load_dtype", ctypes.c_uint8 * TQ_MAX_LOADS),
+                ("code", Instr * TQ_MAX_INSTR),
+                ("strpool", ctypes.c_uint8 * TQ_STRPOOL)]
+
+
+class ColSet(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p * TQ_MAX_COLS), ("width", ctypes.c_int32 * TQ_MAX_COLS)]
+
+
+class SortKey(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("dtype", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("descending", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class JoinState(ctypes.Structure):
+    _fields_ = [("nsel", ctypes.c_int64), ("key_bits", ctypes.c_int32),
+                ("nsample", ctypes.c_int32), ("step", ctypes.c_int64),
+                ("keys", ctypes.c_void_p), ("rows", ctypes.c_void_p), ("sample", ctypes.c_void_p)]
+
+
+class GroupByDesc(ctypes.Structure):
+    _fields_ = [("nkeys", ctypes.c_int32), ("naggs", ctypes.c_int32),
+                ("key_dtype", ctypes.c_int32 * TQ_GB_MAX_KEYS),
+                ("key_ptr", ctypes.c_void_p * TQ_GB_MAX_KEYS),
+                ("key_min", ctypes.c_int64 * TQ_GB_MAX_KEYS),
+                ("key_range", ctypes.c_int64 * TQ_GB_MAX_KEYS),
+                ("key_stride", ctypes.c_int64 * TQ_GB_MAX_KEYS),
+                ("agg_op", ctypes.c_int32 * TQ_MAX_SEGS),
+                ("agg_dtype", ctypes.c_int32 * TQ_MAX_SEGS)]
+
+
+# ---------------------------------------------------------------- loading
+_lib = None
+
+P, I32, I64, SZ = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
+_SIGNATURES = {
+    "tq_version": ([], I32),
+    "tq_last_error": ([], ctypes.c_char_p),
+    "tq_tile_rows": ([], I32),
+    "tq_filter_mask": ([P, P, I64, P, P, P], I32),
+    "tq_mask_tile_counts": ([P, I64, P, P], I32),
+    "tq_scan_ws_bytes": ([I64], SZ),
+    "tq_scan_tiles": ([P, I64, P, P, P, P], I32),
+    "tq_compact": ([P, I64, P, I32, P, P, P, P], I32),
+    "tq_gather": ([P, I64, I32, P, P, P, P, I32, P], I32),
+    "tq_mark_rows": ([P, I64, P, I64, P], I32),
+    "tq_eval_column": ([P, P, I64, P, I32, P], I32),
+    "tq_sort_ws_bytes": ([I64], SZ),
+    "tq_sort_perm": ([P, I32, I64, I64, P, P, SZ, P], I32),
+    "tq_join_build_ws_bytes": ([I64], SZ),
+    "tq_join_build": ([P, I32, P, I64, P, SZ, P, P], I32),
+    "tq_join_probe_ws_bytes": ([I64], SZ),
+    "tq_join_count": ([P, P, I32, P, I64, P, P, P], I32),
+    "tq_join_write": ([P, P, I32, P, I64, P, I32, P, P, P, I32, P, P, P, P, P, P], I32),
+    "tq_groupby_bounds": ([P, P, I64, P, P], I32),
+    "tq_groupby_plan": ([P, I64, I32, P, P, P], I32),
+    "tq_groupby_accumulate": ([P, P, P, P, I64, I32, I64, P, SZ, P], I32),
+    "tq_groupby_count": ([P, I32, I64, P, P, P], I32),
+    "tq_groupby_write": ([P, I32, I64, P, I64, P, P, P], I32),
+}
+
+
+def load():
+    """Load the library (once).  Raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+_ERRORS = {TQ_EINVAL: ValueError, TQ_EUNSUPPORTED: NotImplementedError, TQ_EHIP: RuntimeError,
+           TQ_ECAPACITY: MemoryError}
+
+# When a list, every call appends (name, algorithmic bytes, start event, end event); the events
+# are recorded on torch's current stream, the stream the kernels are launched on.
+TRACE = None
+
+
+def call(name, *args, alg_bytes=None):
+    lib = load()
+    fn = getattr(lib, name)
+    if TRACE is None:
+        rc = fn(*args)
+    else:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = fn(*args)
+        e1.record()
+        TRACE.append((name, alg_bytes, e0, e1))
+    if rc != TQ_OK:
+        msg = lib.tq_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, RuntimeError)(f"{name}: {msg}")
+    return rc
+
+
+def size_call(name, *args) -> int:
+    return int(getattr(load(), name)(*args))
+
+
+# ---------------------------------------------------------------- helpers
+def stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def require_gpu(t: torch.Tensor, what="column"):
+    if not t.is_cuda:
+        raise RuntimeError(f"{what} must be a GPU tensor (the HIP path has no CPU fallback)")
+
+
+def dtype_code(dt) -> int:
+    try:
+        return DTYPE_CODE[dt]
+    except KeyError:
+        raise NotImplementedError(f"unsupported column dtype {dt}")
+
+
+def row_bytes(t: torch.Tensor) -> int:
+    """Bytes per row: element size, times the width of a 2-D column."""
+    w = t.shape[1] if t.dim() == 2 else 1
+    return t.element_size() * w
+
+
+def ptr_array(tensors):
+    arr = (ctypes.c_void_p * max(len(tensors), 1))()
+    for i, t in enumerate(tensors):
+        arr[i] = t.data_ptr()
+    return arr
+
+
+def int_array(vals):
+    arr = (ctypes.c_int32 * max(len(vals), 1))()
+    for i, v in enumerate(vals):
+        arr[i] = int(v)
+    return arr
+
+
+def workspace(nbytes: int, device) -> torch.Tensor:
+    """Raw device workspace from torch's caching allocator."""
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)

@@ -1,0 +1,358 @@
+"""Tensor-level wrappers over the C ABI: each takes and returns torch GPU tensors.
+
+These are the seams where the reference calls ATen (filter.evaluate, the `tensor[mask]` of the
+pipeline, join_kernel_new, agg_wrapper, the sort passes); the operator modules call them.
+Outputs are allocated here with torch's caching allocator, workspaces likewise; the library only
+launches kernels on the current stream.
+"""
+import ctypes
+
+import torch
+
+from . import _lib as L
+from . import expr as E
+
+
+def _dev(t):
+    return t.device
+
+
+# ---------------------------------------------------------------- filter / compaction
+def tile_rows() -> int:
+    return int(L.load().tq_tile_rows())
+
+
+def ntiles(n: int) -> int:
+    tr = tile_rows()
+    return (n + tr - 1) // tr
+
+
+def filter_mask(tree, tensors, subquery_result=None, n=None):
+    """Bool mask of a predicate tree (operators/filter.py:14-162) plus per-tile counts."""
+    b = E.ProgramBuilder()
+    E.compile_predicate(b, tree, tensors, subquery_result)
+    rows = b.n if b.n is not None else n
+    if rows is None:
+        rows = next(iter(tensors.values())).tensor.shape[0]
+    dev = b.col_tensors[0].device if b.col_tensors else next(iter(tensors.values())).tensor.device
+    mask = torch.empty(rows, dtype=torch.bool, device=dev)
+    counts = torch.empty(max(ntiles(rows), 1), dtype=torch.int32, device=dev)
+    if not b.col_tensors:
+        # no column referenced (e.g. only isnotnull): give the kernel a dummy column
+        b.column(mask)
+    # read each loaded column once, write one mask byte per row
+    alg = rows * (1 + b.load_bytes())
+    L.call("tq_filter_mask", E.program_ptr(b), E.colset_ptr(b), rows, L.ptr(mask), L.ptr(counts),
+           L.stream_handle(), alg_bytes=alg)
+    return mask, counts
+
+
+def mask_tile_counts(mask: torch.Tensor):
+    n = mask.shape[0]
+    counts = torch.empty(max(ntiles(n), 1), dtype=torch.int32, device=mask.device)
+    L.call("tq_mask_tile_counts", L.ptr(mask), n, L.ptr(counts), L.stream_handle())
+    return counts
+
+
+def scan_tiles(counts: torch.Tensor, nt: int):
+    """Exclusive scan of tile counts -> (offsets int64[nt], total int64[1]) on the device."""
+    dev = counts.device
+    offs = torch.empty(max(nt, 1), dtype=torch.int64, device=dev)
+    total = torch.empty(1, dtype=torch.int64, device=dev)
+    ws = L.workspace(L.size_call("tq_scan_ws_bytes", nt), dev)
+    L.call("tq_scan_tiles", L.ptr(counts), nt, L.ptr(offs), L.ptr(total), L.ptr(ws),
+           L.stream_handle())
+    return offs, total
+
+
+def compact(mask: torch.Tensor, columns, tile_counts=None):
+    """`[c[mask] for c in columns]` in one launch per 16 columns (vortex_pipeline.py:58)."""
+    L.require_gpu(mask, "mask")
+    if mask.dtype != torch.bool:
+        raise ValueError("compaction mask must be bool")
+    n = mask.shape[0]
+    for c in columns:
+        L.require_gpu(c)
+        if c.shape[0] != n:
+            raise IndexError(f"mask of {n} rows applied to a column of {c.shape[0]} rows")
+    nt = ntiles(n)
+    if tile_counts is None:
+        tile_counts = mask_tile_counts(mask)
+    offs, total = scan_tiles(tile_counts, nt)
+    m = int(total.item()) if n > 0 else 0
+    outs = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=c.device) for c in columns]
+    for i in range(0, len(columns), L.TQ_MAX_COLS):
+        src = [c.contiguous() for c in columns[i:i + L.TQ_MAX_COLS]]
+        dst = outs[i:i + L.TQ_MAX_COLS]
+        if m == 0:
+            continue
+        rb = sum(L.row_bytes(c) for c in src)
+        L.call("tq_compact", L.ptr(mask), n, L.ptr(offs), len(src), L.ptr_array(src),
+               L.int_array([L.row_bytes(c) for c in src]), L.ptr_array(dst), L.stream_handle(),
+               alg_bytes=n + 2 * m * rb)
+    return outs
+
+
+def gather(idx: torch.Tensor, columns, null_fill=False):
+    """`[c[idx] for c in columns]`; with null_fill, idx == -64 yields -64 (index_with_null)."""
+    m = idx.shape[0]
+    idx = idx.to(torch.int64).contiguous()
+    outs = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=c.device) for c in columns]
+    if m == 0:
+        return outs
+    for i in range(0, len(columns), L.TQ_MAX_COLS):
+        src = [c.contiguous() for c in columns[i:i + L.TQ_MAX_COLS]]
+        fk = [8 if c.dtype == torch.float64 else 4 if c.dtype == torch.float32 else 0 for c in src]
+        L.call("tq_gather", L.ptr(idx), m, len(src), L.ptr_array(src),
+               L.int_array([L.row_bytes(c) for c in src]), L.int_array(fk),
+               L.ptr_array(outs[i:i + L.TQ_MAX_COLS]), 1 if null_fill else 0, L.stream_handle())
+    return outs
+
+
+def eval_program(b: E.ProgramBuilder, n: int, out_dtype, device):
+    out = torch.empty(n, dtype=out_dtype, device=device)
+    if n > 0:
+        if not b.col_tensors:
+            b.column(out)
+        L.call("tq_eval_column", E.program_ptr(b), E.colset_ptr(b), n, L.ptr(out),
+               out.element_size(), L.stream_handle())
+    return out
+
+
+# ---------------------------------------------------------------- sort
+def sort_perm(keys, n: int, limit=None, device=None):
+    """Stable multi-key sort permutation (int64[min(n, limit)]).  keys: list of
+    (tensor, descending) from most to least significant; 2-D int8 keys compare row-wise."""
+    m = n if limit is None or limit < 0 or limit > n else int(limit)
+    dev = device if device is not None else keys[0][0].device
+    perm = torch.empty(m, dtype=torch.int64, device=dev)
+    if n == 0 or m == 0:
+        return perm
+    arr = (L.SortKey * max(len(keys), 1))()
+    hold = []
+    for i, (t, desc) in enumerate(keys):
+        L.require_gpu(t, "sort key")
+        if t.dim() == 2 and t.shape[1] == 1:
+            t = t.reshape(-1)
+        t = t.contiguous()
+        hold.append(t)
+        arr[i].ptr = t.data_ptr()
+        arr[i].dtype = L.dtype_code(t.dtype)
+        arr[i].width = t.shape[1] if t.dim() == 2 else 1
+        arr[i].descending = 1 if desc else 0
+    ws_bytes = L.size_call("tq_sort_ws_bytes", n)
+    ws = L.workspace(ws_bytes, dev)
+    L.call("tq_sort_perm", arr, len(keys), n, -1 if limit is None else int(limit), L.ptr(perm),
+           L.ptr(ws), ws_bytes, L.stream_handle())
+    return perm
+
+
+# ---------------------------------------------------------------- join
+class JoinIndex:
+    """A built join index over the selected rows of a build key column."""
+
+    def __init__(self, bkey: torch.Tensor, bmask=None):
+        L.require_gpu(bkey, "join key")
+        self.bkey = bkey.contiguous()
+        self.bmask = bmask
+        nb = self.bkey.shape[0]
+        ws_bytes = L.size_call("tq_join_build_ws_bytes", nb)
+        self.ws = L.workspace(ws_bytes, bkey.device)
+        self.state = L.JoinState()
+        L.call("tq_join_build", L.ptr(self.bkey), L.dtype_code(self.bkey.dtype),
+               L.ptr(bmask) if bmask is not None else None, nb, L.ptr(self.ws), ws_bytes,
+               ctypes.byref(self.state), L.stream_handle())
+
+    @property
+    def nsel(self):
+        return int(self.state.nsel)
+
+    def probe(self, pkey: torch.Tensor, pmask, left_cols, right_cols, want_index=False):
+        """All (build, probe) pairs with equal keys, probe row ascending then build row
+        ascending.  Returns (left outputs, right outputs, build_idx, probe_idx)."""
+        L.require_gpu(pkey, "join key")
+        pkey = pkey.contiguous()
+        np_ = pkey.shape[0]
+        dev = pkey.device
+        pws = L.workspace(L.size_call("tq_join_probe_ws_bytes", np_), dev)
+        total = ctypes.c_int64(0)
+        pm = L.ptr(pmask) if pmask is not None else None
+        L.call("tq_join_count", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype), pm,
+               np_, L.ptr(pws), ctypes.byref(total), L.stream_handle(),
+               alg_bytes=np_ * (pkey.element_size() + (1 if pmask is not None else 0)))
+        m = int(total.value)
+        lout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in left_cols]
+        rout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in right_cols]
+        bidx = torch.empty(m, dtype=torch.int64, device=dev) if want_index else None
+        pidx = torch.empty(m, dtype=torch.int64, device=dev) if want_index else None
+        if m > 0:
+            if len(left_cols) > L.TQ_MAX_COLS or len(right_cols) > L.TQ_MAX_COLS:
+                raise NotImplementedError("join with more than 16 columns per side")
+            lsrc = [c.contiguous() for c in left_cols]
+            rsrc = [c.contiguous() for c in right_cols]
+            # probe keys + mask once; per pair: its build row id, the gathered inputs and the
+            # written outputs (and the index pairs when requested)
+            per_pair = 4 + 2 * sum(L.row_bytes(c) for c in lsrc + rsrc) + (16 if want_index else 0)
+            alg = np_ * (pkey.element_size() + (1 if pmask is not None else 0)) + m * per_pair
+            L.call("tq_join_write", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype),
+                   pm, np_, L.ptr(pws), len(lsrc), L.ptr_array(lsrc),
+                   L.int_array([L.row_bytes(c) for c in lsrc]), L.ptr_array(lout), len(rsrc),
+                   L.ptr_array(rsrc), L.int_array([L.row_bytes(c) for c in rsrc]),
+                   L.ptr_array(rout), L.ptr(bidx), L.ptr(pidx), L.stream_handle(), alg_bytes=alg)
+        return lout, rout, bidx, pidx
+
+
+# ---------------------------------------------------------------- group-by
+class GroupBy:
+    """Group rows by integer key columns and reduce aggregate arguments.
+
+    `aggs` is a list of (op, tree) where op is one of 'sum', 'avg', 'count', 'min', 'max' and
+    tree the argument's row expression (aggregate.py dialect) over `tensors`."""
+
+    OPS = {'sum': L.TQ_AGG_SUM, 'count': L.TQ_AGG_COUNT, 'avg': L.TQ_AGG_AVG,
+           'min': L.TQ_AGG_MIN, 'max': L.TQ_AGG_MAX}
+
+    def __init__(self, keys, aggs, tensors, mask=None, n=None):
+        self.keys = []
+        for k in keys:
+            L.require_gpu(k, "group key")
+            if k.dim() == 2:
+                if k.shape[1] != 1:
+                    raise NotImplementedError("multi-column (string) group keys")
+                k = k.reshape(-1)
+            if k.dtype.is_floating_point:
+                raise NotImplementedError("floating-point group keys")
+            self.keys.append(k.contiguous())
+        if len(self.keys) > L.TQ_GB_MAX_KEYS:
+            raise NotImplementedError("more than 8 group keys")
+        self.mask = mask
+        b = E.ProgramBuilder()
+        for k in self.keys:          # key columns take load slots 0..nkeys-1
+            b.load_slot(k)
+        self.arg_dtypes = []
+        for op, tree in aggs:
+            v = E.compile_row_expr(b, tree, tensors)
+            if v.kind == "lit":
+                v = b.const(v.value, E.literal_dtype(v.value))
+            if v.kind != "reg":
+                raise NotImplementedError("string-valued aggregate argument")
+            self.arg_dtypes.append(v.dtype)
+            b.end_segment(v)
+        b.finish()
+        self.builder = b
+        rows = n
+        if rows is None:
+            rows = self.keys[0].shape[0] if self.keys else (b.n if b.n is not None else 0)
+        self.n = rows
+        if not b.col_tensors:
+            dummy = torch.zeros(max(rows, 1), dtype=torch.int8,
+                                device=mask.device if mask is not None else 'cuda')
+            b.column(dummy)
+        self.ops = [self.OPS[op] for op, _ in aggs]
+
+    def run(self):
+        """Returns (key columns, aggregate columns), groups in ascending key order."""
+        b = self.builder
+        dev = b.col_tensors[0].device
+        d = L.GroupByDesc()
+        d.nkeys = len(self.keys)
+        d.naggs = len(self.ops)
+        for i, k in enumerate(self.keys):
+            d.key_dtype[i] = L.dtype_code(k.dtype)
+            d.key_ptr[i] = k.data_ptr()
+        for j, (op, dt) in enumerate(zip(self.ops, self.arg_dtypes)):
+            d.agg_op[j] = op
+            d.agg_dtype[j] = L.dtype_code(dt)
+        s = L.stream_handle()
+        mptr = L.ptr(self.mask) if self.mask is not None else None
+        bounds = torch.empty(2 * len(self.keys) + 1, dtype=torch.int64, device=dev)
+        L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), s)
+        hb = bounds.cpu().tolist()
+        nsel = hb[-1]
+        for i in range(len(self.keys)):
+            lo, hi = hb[2 * i], hb[2 * i + 1]
+            d.key_min[i] = lo if nsel > 0 else 0
+            d.key_range[i] = (hi - lo + 1) if nsel > 0 else 1
+        mode = ctypes.c_int32()
+        slots = ctypes.c_int64()
+        wsb = ctypes.c_size_t()
+        L.call("tq_groupby_plan", ctypes.byref(d), nsel, b.nload, ctypes.byref(mode),
+               ctypes.byref(slots), ctypes.byref(wsb))
+        self.mode, self.slots = mode.value, slots.value
+        ws = L.workspace(wsb.value, dev)
+        # every selected row's loaded columns (keys included) and the mask byte of every row
+        alg = nsel * b.load_bytes() + (self.n if self.mask is not None else 0)
+        L.call("tq_groupby_accumulate", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr,
+               self.n, mode, slots, L.ptr(ws), wsb.value, s, alg_bytes=alg)
+        ng = ctypes.c_int64()
+        L.call("tq_groupby_count", ctypes.byref(d), mode, slots, L.ptr(ws), ctypes.byref(ng), s)
+        G = int(ng.value)
+        key_out = [torch.empty(G, dtype=k.dtype, device=dev) for k in self.keys]
+        agg_out = []
+        for op, dt in zip(self.ops, self.arg_dtypes):
+            if op == L.TQ_AGG_COUNT:
+                odt = torch.int64
+            elif op == L.TQ_AGG_AVG:
+                odt = torch.float32 if dt == torch.float32 else torch.float64
+            elif op == L.TQ_AGG_SUM:
+                odt = dt if dt.is_floating_point else torch.int64
+            else:
+                odt = dt
+            agg_out.append(torch.empty(G, dtype=odt, device=dev))
+        if G > 0:
+            L.call("tq_groupby_write", ctypes.byref(d), mode, slots, L.ptr(ws), G,
+                   L.ptr_array(key_out), L.ptr_array(agg_out), s)
+        return key_out, agg_out
+
+
+def mark_rows(idx: torch.Tensor, n: int, device):
+    """bool[n] with True at every row listed in idx."""
+    flags = torch.empty(n, dtype=torch.bool, device=device)
+    if n > 0:
+        L.call("tq_mark_rows", L.ptr(idx) if idx.numel() else None, idx.numel(), L.ptr(flags), n,
+               L.stream_handle())
+    return flags
+
+
+def mask_and_not(keep, drop, n: int, device):
+    """keep & ~drop over bool columns (keep may be None = all rows)."""
+    b = E.ProgramBuilder()
+    d = b.load_column(drop)
+    b.emit("NOT", dst=d.reg, a=d.reg)
+    if keep is not None:
+        k = b.load_column(keep)
+        b.emit("AND", dst=d.reg, a=d.reg, b=k.reg)
+    b.end_segment(d)
+    b.finish()
+    return eval_program(b, n, torch.bool, device)
+
+
+def mask_and(a: torch.Tensor, b: torch.Tensor):
+    """a & b over bool columns."""
+    pb = E.ProgramBuilder()
+    x = pb.load_column(a)
+    y = pb.load_column(b)
+    pb.emit("AND", dst=x.reg, a=x.reg, b=y.reg)
+    pb.end_segment(x)
+    pb.finish()
+    return eval_program(pb, a.shape[0], torch.bool, a.device)
+
+
+def compact_rows(mask: torch.Tensor):
+    """int64 row numbers of the set rows of a mask, ascending (torch.nonzero of a 1-D mask)."""
+    n = mask.shape[0]
+    offs, total = scan_tiles(mask_tile_counts(mask), ntiles(n))
+    m = int(total.item()) if n > 0 else 0
+    out = torch.empty(m, dtype=torch.int64, device=mask.device)
+    if m > 0:
+        src = (ctypes.c_void_p * 1)()
+        src[0] = None
+        L.call("tq_compact", L.ptr(mask), n, L.ptr(offs), 1, src, L.int_array([8]),
+               L.ptr_array([out]), L.stream_handle())
+    return out
+
+
+def count_true(mask: torch.Tensor) -> int:
+    counts = mask_tile_counts(mask)
+    _, total = scan_tiles(counts, ntiles(mask.shape[0]))
+    return int(total.item()) if mask.shape[0] > 0 else 0

@@ -1,0 +1,190 @@
+"""Group-by aggregate operator (operators/aggregate.py:236-393 of the reference).
+
+`tqp_hash_aggregate` keeps the reference's signature, argument schema and side effects
+(`args['input ids']` gains the mask ids in place; key outputs are 2-D [G, 1] columns of the key's
+dtype and type name; aggregate outputs are 1-D with the type name of their dtype; the return
+value is the row count of the group's first entry).
+
+Grouping is exact: groups are the distinct key tuples, emitted in ascending lexicographic key
+order.  The reference numbers groups by the rank of a linear hash with random coefficients
+(aggregate.py:320-327), so its group order is arbitrary and distinct keys can collide into one
+group; ascending key order is the canonical order of its output, and collisions are not
+reproduced.
+
+All aggregates of one operator run in one accumulate launch that reads the uncompacted input
+columns and the filter mask (no compaction), evaluates every aggregate argument per row with the
+expression kernel, and reduces into an LDS or global table (see the group-by kernels).  Results
+that combine aggregates (e.g. `0.2 * avg(x)`) are evaluated over the group rows with the same
+expression kernel.
+"""
+import torch
+
+from .. import constants, kernels
+from .. import expr as E
+from ..conversion import get_id, right_most
+from ..IO.vortex_pipeline import InMemoryPipeline
+from ..utility.logger import message_logger, perf_logger
+from ..variable import Variable, torch_to_type
+
+
+def get_result_type(tree, transfer_set, transfer_types):
+    """Declared dtype of a result expression (aggregate.py:23-44)."""
+    if isinstance(tree, str):
+        if '#' in tree and not tree.startswith('Brand#') and tree != '#':
+            if get_id(tree) in transfer_set and \
+                    transfer_types[transfer_set.index(get_id(tree))] == constants.float_dtype:
+                return constants.float_dtype
+            return constants.int_dtype
+        if tree.isdigit():
+            return constants.int_dtype
+        if E.is_float(tree):
+            return constants.float_dtype
+    if isinstance(tree, list):
+        if tree and tree[0] == 'CASE':
+            return get_result_type(tree[4], transfer_set, transfer_types)
+        for e in tree:
+            s = get_result_type(e, transfer_set, transfer_types)
+            if s == constants.float_dtype:
+                return constants.float_dtype
+            if s == torch.int64:
+                return torch.int64
+        return constants.int_dtype
+
+
+def _is_key(tree):
+    return isinstance(tree, str) or (len(tree) == 3 and tree[1] == 'AS' and isinstance(tree[0], str))
+
+
+def _unwrap(tree):
+    while isinstance(tree, list) and len(tree) == 1:
+        tree = tree[0]
+    return tree
+
+
+def _strip_alias(tree):
+    """[expr, '#', id] and single-element lists evaluate to expr (aggregate.py:69-74)."""
+    while isinstance(tree, list) and (len(tree) == 1 or (len(tree) == 3 and tree[1] == '#')):
+        tree = tree[0]
+    return tree
+
+
+def agg_group(columns, key_ids, results, mask=None):
+    """Group `columns` by `key_ids` and evaluate each result tree (aggregate.py:282-364).
+    Returns {output id: tensor}."""
+    keys = [columns[k].tensor for k in key_ids]
+    n = next(iter(columns.values())).tensor.shape[0]
+    calls = []               # aggregate call nodes in order
+    plans = []               # per result: ('key', key_id) or ('agg', tree)
+    for tree in results:
+        tree = _unwrap(tree)
+        out_id = get_id(right_most(tree))
+        if _is_key(tree):
+            plans.append((out_id, 'key', get_id(tree if isinstance(tree, str) else tree[0])))
+        else:
+            body = tree[0]
+            found = E.find_aggregates(body, [])
+            if not found:
+                raise ValueError(f"aggregate result without an aggregate function: {tree}")
+            calls.extend(found)
+            plans.append((out_id, 'agg', body))
+    gb = kernels.GroupBy(keys, [(c[0], c[1]) for c in calls], columns, mask=mask, n=n)
+    key_out, agg_out = gb.run()
+    by_call = {id(c): t for c, t in zip(calls, agg_out)}
+    out = {}
+    for out_id, kind, what in plans:
+        if kind == 'key':
+            pos = key_ids.index(what)
+            out[out_id] = key_out[pos].reshape(-1, 1).to(columns[what].tensor.dtype)
+            continue
+        body = _strip_alias(what)
+        if id(body) in by_call:
+            out[out_id] = by_call[id(body)]
+            continue
+        b = E.ProgramBuilder()
+        v = E.compile_row_expr(b, what, columns, agg_ref=by_call)
+        if v.kind == "lit":
+            v = b.const(v.value, E.literal_dtype(v.value))
+        b.end_segment(v)
+        b.finish()
+        G = key_out[0].shape[0] if key_out else 1
+        out[out_id] = kernels.eval_program(b, G, v.dtype, agg_out[0].device)
+    return out
+
+
+def evaluate(device_name, tensors, tree, cnt_rows, cnt_groups, inverted_index):
+    """The reference's per-aggregate evaluation over a precomputed group index has no
+    counterpart here: all aggregates of an operator are reduced together by agg_group."""
+    raise NotImplementedError("use agg_group: aggregates are reduced in one fused launch")
+
+
+def tqp_hash_aggregate(gpu_enable, tensor_group, args, gpu_mem_pool, cpu_mem_pool,
+                       name="aggregation"):
+    perf_logger().start(f"{name}")
+    if not gpu_enable:
+        raise NotImplementedError("this build executes operators on the GPU only")
+    assert not args['is partial']
+
+    transfer_set = args['input ids']
+    transfer_types = []
+    for i in transfer_set:
+        transfer_types.extend(tensor_group[i].get_type_torch())
+
+    agg_mask_map = {}
+    if 'mask map' in args:
+        for k, v in args['mask map'].items():
+            if k in transfer_set:
+                agg_mask_map[k] = v
+                if v not in transfer_set:
+                    transfer_set.append(v)
+
+    key_ids, key_ids_new, key_types, key_types_name = [], [], [], []
+    agg_types = []
+    for tree in args['results']:
+        tree = _unwrap(tree)
+        if _is_key(tree):
+            key_ids.append(get_id(tree if isinstance(tree, str) else tree[0]))
+            key_ids_new.append(get_id(tree if isinstance(tree, str) else tree[-1]))
+            key_types.extend(tensor_group[key_ids[-1]].get_type_torch())
+            key_types_name.append(tensor_group[key_ids[-1]].tensor_type)
+        else:
+            agg_types.append(get_result_type(tree, transfer_set, transfer_types))
+    agg_ids = args['out agg ids']
+
+    def agg_wrapper(columns):
+        masks = {id(v.mask): v.mask for v in columns.values() if getattr(v, 'mask', None) is not None}
+        if len(masks) > 1 or (masks and any(getattr(v, 'mask', None) is None for v in columns.values())):
+            for v in columns.values():   # mixed masks: compact each column by its own mask
+                if getattr(v, 'mask', None) is not None:
+                    v.tensor = kernels.compact(v.mask, [v.tensor])[0]
+                    v.mask = None
+            mask = None
+        else:
+            mask = next(iter(masks.values())) if masks else None
+        key_ids_d = [get_id(key) for key in args['keys']] if args['keys'] else []
+        out = agg_group(columns, key_ids_d, args['results'], mask=mask)
+        res = [out[i] for i in key_ids_new] + [out[i] for i in agg_ids]
+        return [r.contiguous() for r in res]
+
+    agg_pipeline = InMemoryPipeline(
+        {i: tensor_group[i] for i in transfer_set},
+        key_types + agg_types,
+        agg_wrapper,
+        mask_map=agg_mask_map,
+        gpu_mem_pool=gpu_mem_pool,
+        cpu_mem_pool=cpu_mem_pool,
+        chunk_size=2_000_000_000,
+        name=f"{name} aggregation pipe",
+        fuse_masks=True,
+    )
+    agg_pipeline.do_exchange(20_000_000)
+    result = agg_pipeline.get_result()
+    message_logger().debug(result)
+
+    rows_keys = [result[i].tensor for i in range(len(key_types))]
+    rows_vals = [result[i].tensor for i in range(len(key_types), len(key_types) + len(agg_types))]
+    for i in range(len(rows_vals)):
+        tensor_group[agg_ids[i]] = Variable(rows_vals[i], torch_to_type(rows_vals[i].dtype))
+    for i in range(len(rows_keys)):
+        tensor_group[key_ids_new[i]] = Variable(rows_keys[i], key_types_name[i])
+    perf_logger().stop(f"{name}")
+    return list(tensor_group.values())[0].tensor.size(0)

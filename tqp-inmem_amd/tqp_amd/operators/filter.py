@@ -1,0 +1,67 @@
+"""Filter operator: predicate tree -> bool mask (operators/filter.py:164-205 of the reference).
+
+`tqp_filter` keeps the reference's signature, argument schema and side effects: it evaluates
+`args['filter tree']` over the `materialized in` columns, stores the mask as
+`tensor_group[args['out mask id']] = Variable(mask, 'bool', ...)` and returns -1; consumers
+compact by the mask later (late materialisation).  The whole tree runs as one HIP kernel (the
+reference launches one ATen kernel per tree node) that also writes per-tile match counts, which
+ride along on the mask Variable for the consumer's compaction.
+"""
+import torch
+
+from .. import kernels
+from ..IO.vortex_pipeline import InMemoryPipeline
+from ..utility.logger import message_logger, perf_logger
+from ..variable import Variable, VariableState
+
+
+def evaluate(device_name, tensors, tree, rows, subquery_result, leaf_function=None):
+    """Bool mask of `tree` over `tensors` (operators/filter.py:14-162), computed on the GPU."""
+    if leaf_function is not None:
+        raise NotImplementedError("leaf_function evaluation runs through the join's pair path")
+    mask, _ = kernels.filter_mask(tree, tensors, subquery_result, n=rows)
+    return mask
+
+
+def tqp_filter(gpu_enable, tensor_group, args, subquery_result, cpu_mem_pool, gpu_mem_pool, name):
+    perf_logger().start(name)
+    assert 'mask map' not in args, "Filter not accepting masks"
+    if not gpu_enable:
+        raise NotImplementedError("this build executes operators on the GPU only")
+    tree = args['filter tree']
+    input_columns = {k: tensor_group[k] for k in args['materialized in']}
+
+    if len(input_columns) == 0:
+        message_logger().debug("Filter not dispatched")
+        perf_logger().stop(name)
+        return -1
+
+    counts_box = []
+
+    def filter_wrapper(cols):
+        mask, counts = kernels.filter_mask(tree, cols, subquery_result)
+        counts_box.append(counts)
+        return (mask,)
+
+    pipe = InMemoryPipeline(
+        input_columns=input_columns,
+        output_columns_type=[torch.bool],
+        operator=filter_wrapper,
+        chunk_size=2_000_000_000,
+        cpu_mem_pool=cpu_mem_pool,
+        gpu_mem_pool=gpu_mem_pool,
+        name=f"{name} filter pipe",
+    )
+    pipe.do_exchange(20_000_000)
+    mask = pipe.get_result()[0]
+
+    t = mask.tensor
+    if t.dim() == 2:
+        assert t.shape[0] == 1, "squeeze failed."
+        t = t.squeeze(0)
+    out = Variable(t, 'bool', mask.memory_ptr, VariableState.GPU)
+    if counts_box:
+        out.tile_counts = counts_box[0]
+    tensor_group[args['out mask id']] = out
+    perf_logger().stop(name)
+    return -1

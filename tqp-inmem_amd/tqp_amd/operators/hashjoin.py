@@ -1,0 +1,216 @@
+"""Sort-based equi-join operator (operators/hashjoin.py:15-248 of the reference).
+
+`join_vortex` and `join_kernel_new` keep the reference's names, signatures, argument schema and
+side effects (the `mask map` entries of the joined columns are popped from `args`, every
+`materialized in` column is overwritten by its joined gather, the return value is the row count
+of the group's first entry).  LEFT is the build side and RIGHT the probe side.
+
+Pairs come out in probe-row order, and within one probe row in ascending build-row order.  The
+reference sorts the build keys with an unstable sort (hashjoin.py:43), so its order inside a
+duplicate-key bucket is unspecified; ascending build row is the canonical order of that bucket.
+
+Data path: the filter masks are not applied by compaction.  The build kernel compacts only the
+selected build keys (with their row ids) into the sort, and the probe kernels read the probe
+mask directly; output columns are gathered straight from the uncompacted inputs by row id, in
+the same launch that emits the pairs (inner joins without extra conditions).  Joins with extra
+conditions or of the semi/anti/outer kinds first materialise the pair row ids, evaluate the
+conditions over gathered columns with the predicate kernel, then finish per join type.
+"""
+from typing import Literal
+
+import torch
+
+from .. import constants, kernels
+from .. import expr as E
+from ..IO.vortex_pipeline import InMemoryPipeline
+from ..utility.logger import message_logger, perf_logger
+from ..variable import Variable, VariableState
+
+LEFT = 0
+RIGHT = 1
+
+
+def _side_mask(tensors):
+    """The one mask shared by a side's columns (None if unmasked).  Columns masked by
+    different masks are compacted here so the side is consistent."""
+    masks = {id(v.mask): v.mask for v in tensors.values() if getattr(v, 'mask', None) is not None}
+    if not masks:
+        return None
+    if len(masks) == 1 and all(getattr(v, 'mask', None) is not None for v in tensors.values()):
+        return next(iter(masks.values()))
+    for k, v in tensors.items():
+        if getattr(v, 'mask', None) is not None:
+            v.tensor = kernels.compact(v.mask, [v.tensor])[0]
+            v.mask = None
+    return None
+
+
+def _empty_like_reference():
+    # the reference returns torch.tensor([]) (float32) for a side with no rows
+    return torch.tensor([], device='cuda')
+
+
+def evaluate_join(device_name, tensorsa, tensorsb, tree, leftidx, rightidx):
+    """Bool mask of a join condition over pairs (operators/hashjoin.py:15-22): leaves gather the
+    left/right column at the pair's row ids; the predicate runs as one kernel."""
+    cache = {}
+
+    def leaf(cid):
+        if cid not in cache:
+            src, idx = (tensorsa, leftidx) if cid in tensorsa else (tensorsb, rightidx)
+            t = src[cid].tensor
+            if t.dim() == 2 and t.shape[1] == 1:
+                t = t.reshape(-1)
+            cache[cid] = kernels.gather(idx, [t])[0]
+        return cache[cid]
+
+    b = E.ProgramBuilder()
+    E.compile_predicate(b, tree, None, None, leaf_function=leaf)
+    n = leftidx.shape[0]
+    return kernels.eval_program(b, n, torch.bool, leftidx.device)
+
+
+def join_kernel_new(gpu_enable, tensors_left, tensors_right, left_key, right_key, condition_list,
+                    join_output_keys, return_global_index,
+                    type: Literal['inner', 'right-semi', 'right-anti', 'right-outer'],
+                    left_global_idx, right_global_idx):
+    assert type in {'inner', 'right-semi', 'right-anti', 'right-outer'}, "Invalid Join type."
+    if not gpu_enable:
+        raise NotImplementedError("this build executes operators on the GPU only")
+
+    tensors_left[left_key].normalize()
+    tensors_right[right_key].normalize()
+    bmask = _side_mask(tensors_left)
+    pmask = _side_mask(tensors_right)
+    left = tensors_left[left_key].tensor
+    right = tensors_right[right_key].tensor
+    nR = right.shape[0]
+    dev = right.device
+
+    index = kernels.JoinIndex(left, bmask)
+    left_keys_out = [k for k in join_output_keys if k in tensors_left]
+    right_keys_out = [k for k in join_output_keys if k not in tensors_left]
+
+    def left_nonempty():
+        return index.nsel > 0
+
+    def right_nonempty():
+        return nR > 0 and (pmask is None or kernels.count_true(pmask) > 0)
+
+    fast = type == 'inner' and not condition_list and not return_global_index
+    if fast:
+        lsrc = [tensors_left[k].tensor for k in left_keys_out]
+        rsrc = [tensors_right[k].tensor for k in right_keys_out]
+        lout, rout, _, _ = index.probe(right, pmask, lsrc, rsrc)
+        got = dict(zip(left_keys_out, lout))
+        got.update(zip(right_keys_out, rout))
+        lne = left_nonempty()
+        rne = right_nonempty() if lout and lout[0].shape[0] == 0 or rout and rout[0].shape[0] == 0 else True
+        result = []
+        for k in join_output_keys:
+            nonempty = lne if k in tensors_left else rne
+            result.append(got[k] if nonempty else _empty_like_reference())
+        return result
+
+    # pair row ids, then conditions
+    _, _, bidx, pidx = index.probe(right, pmask, [], [], want_index=True)
+    if condition_list and bidx.shape[0] > 0:
+        keep = None
+        for cond in condition_list:
+            m = evaluate_join(dev.type, tensors_left, tensors_right, cond, bidx, pidx)
+            keep = m if keep is None else kernels.mask_and(keep, m)
+        bidx, pidx = kernels.compact(keep, [bidx, pidx])
+
+    nulls = 0
+    if type == 'inner':
+        left_out_idx, right_out_idx = bidx, pidx
+    else:
+        matched = kernels.mark_rows(pidx, nR, dev)
+        if type == 'right-semi':
+            sel = matched if pmask is None else kernels.mask_and(pmask, matched)
+            right_out_idx = kernels.compact_rows(sel)
+            left_out_idx = torch.empty(0, dtype=torch.int64, device=dev)
+        else:
+            unmatched = kernels.mask_and_not(pmask, matched, nR, dev)
+            rest = kernels.compact_rows(unmatched)
+            if type == 'right-anti':
+                right_out_idx = rest
+                left_out_idx = torch.empty(0, dtype=torch.int64, device=dev)
+            else:  # right-outer: matched pairs, then unmatched probe rows with a NULL left side
+                right_out_idx = torch.cat([pidx, rest])
+                left_out_idx = bidx
+                nulls = rest.shape[0]
+
+    null_left = type == 'right-outer' and left_out_idx.numel() > 0
+    if null_left:
+        left_out_idx = torch.cat(
+            (left_out_idx, torch.full((nulls,), constants.null, dtype=torch.int64, device=dev)))
+
+    result = []
+    lne = left_nonempty()
+    rne = right_nonempty()
+    for k in join_output_keys:
+        if k in tensors_left:
+            t = tensors_left[k].tensor
+            result.append(kernels.gather(left_out_idx, [t], null_fill=null_left)[0] if lne
+                          else _empty_like_reference())
+        else:
+            t = tensors_right[k].tensor
+            result.append(kernels.gather(right_out_idx, [t])[0] if rne else _empty_like_reference())
+    if return_global_index:
+        result.extend([kernels.gather(left_out_idx, [left_global_idx.tensor], null_fill=null_left)[0],
+                       kernels.gather(right_out_idx, [right_global_idx.tensor])[0]])
+    return result
+
+
+def join_vortex(gpu_enable, tensor_group, args, leftcol, rightcol, condition_list, type,
+                cpu_mem_pool, gpu_mem_pool, name):
+    perf_logger().start(name)
+    join_key_index = [leftcol, rightcol]
+    tensor_groupings = [args['left group'], args['right group']]
+    join_mask_map = {}
+    join_input = {}
+    join_out_types = []
+    join_output = []
+
+    for side in [LEFT, RIGHT]:
+        tensor_group[join_key_index[side]].normalize()
+        for used_col in args['materialized in']:
+            if used_col in tensor_groupings[side]:
+                join_input[used_col] = tensor_group[used_col]
+                join_output.append(used_col)
+                join_out_types.append(tensor_group[used_col].tensor.dtype)
+                if 'mask map' in args and used_col in args['mask map']:
+                    mask_id = args['mask map'].pop(used_col)
+                    join_mask_map[used_col] = mask_id
+                    join_input[mask_id] = tensor_group[mask_id]
+
+    def join_wrapper(columns):
+        tensor_inputs = [{}, {}]
+        for side in [LEFT, RIGHT]:
+            for k in tensor_groupings[side]:
+                if k in columns:
+                    tensor_inputs[side][k] = columns[k]
+        return join_kernel_new(True, tensor_inputs[LEFT], tensor_inputs[RIGHT],
+                               join_key_index[LEFT], join_key_index[RIGHT], condition_list,
+                               join_output, False, type, torch.tensor([]), torch.tensor([]))
+
+    pipe = InMemoryPipeline(
+        input_columns=join_input,
+        output_columns_type=join_out_types,
+        operator=join_wrapper,
+        mask_map=join_mask_map,
+        cpu_mem_pool=cpu_mem_pool,
+        gpu_mem_pool=gpu_mem_pool,
+        name=f'{name} core pipe',
+        fuse_masks=True,
+    )
+    pipe.do_exchange(20_000_000)
+    join_results = pipe.get_result()
+    message_logger().debug(join_results)
+
+    for i, k in enumerate(join_output):
+        tensor_group[k] = Variable(join_results[i].tensor, tensor_group[k].tensor_type,
+                                   join_results[i].memory_ptr, join_results[i].placement)
+    perf_logger().stop(name)
+    return next(iter(tensor_group.values())).tensor.size(0)
