@@ -302,6 +302,59 @@ int tq_groupby_write(const tq_groupby_desc_t* desc /* host */, int32_t mode, int
                      void* ws, int64_t ngroups, void* const* key_out /* host */,
                      void* const* agg_out /* host */, void* stream);
 
+/* ---------------------------------------------------------------- batched pipelines */
+/* Many filter -> join -> group-by pipelines in one persistent launch.  A pipeline joins its
+ * selected build rows (unique build keys) with its selected probe rows on key equality and
+ * groups the pairs by the build key, reducing aggregate arguments computed over probe columns.
+ * The group table is addressed by build key - kmin over the build key's [min, max] range, so it
+ * is the join directory and the aggregation table at once, and groups come out in key order.
+ * Work is a device queue of (pipeline, tile) items in five phases -- clear the table, insert
+ * build rows, probe + accumulate, count groups per tile, write groups -- where an item waits
+ * (bounded spin) for its pipeline's previous phase to finish.  No host synchronisation occurs;
+ * the launch sequence can be captured into a graph.  A duplicate selected build key sets
+ * flags[0] bit 0 (the caller reruns that pipeline through the per-operator path).
+ * Replaces, per pipeline, the whole chain tqp_filter (operators/filter.py:164-205) x2 ->
+ * join_vortex (operators/hashjoin.py:179-248) -> tqp_hash_aggregate (operators/aggregate.py:
+ * 236-393) grouped by the join key. */
+#define TQ_BATCH_PHASES 5
+
+typedef struct {
+  tq_program_t bprog;  /* build predicate; load slot 0 must be the build key column  */
+  tq_colset_t bcols;
+  tq_program_t pprog;  /* probe predicate; load slot 0 must be the probe key column  */
+  tq_colset_t pcols;
+  tq_program_t aprog;  /* aggregate arguments over probe columns, one segment each   */
+  tq_colset_t acols;
+  int64_t nb, np;      /* build / probe rows                                          */
+  int64_t kmin, range; /* build key range: table slot = key - kmin                    */
+  int32_t naggs;
+  int32_t key_dtype;   /* dtype of the build key (and of the group key output)        */
+  int32_t agg_op[TQ_MAX_SEGS];
+  int32_t agg_dtype[TQ_MAX_SEGS];
+  uint32_t* dir;       /* device [range]: build row + 1, 0 = empty                     */
+  uint32_t* cnt;       /* device [range]: pairs per group                              */
+  uint64_t* acc;       /* device [naggs * range]: accumulators                         */
+  uint32_t* tile_counts; /* device [ntiles[3]]                                         */
+  void* key_out;       /* device [min(nb, range)] group keys                           */
+  void* agg_out[TQ_MAX_SEGS];
+  int64_t* ngroups;    /* device [1]                                                   */
+  int32_t* flags;      /* device [1]                                                   */
+  int32_t ntiles[TQ_BATCH_PHASES];
+  int32_t pad;
+} tq_batch_pipe_t;
+
+/* Counters the launch needs (queue head, per-pipeline phase completion, error word). */
+size_t tq_batch_ws_bytes(int32_t npipes);
+
+/* Run the batch.  pipes: device array of npipes descriptors; items: device int32 pairs
+ * (pipeline, tile), phase-major, with phase_start[0..5] (host) the first item of each phase;
+ * tile_rows: rows (or table slots) per item, a multiple of 1024; grid: persistent workgroups;
+ * stage_loads: the largest nload of any program.  ws is cleared by the call (kernels only), and
+ * ws error word 1 means a wait exceeded spin_limit (the launch then drains without finishing). */
+int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const int32_t* items,
+                 const int64_t* phase_start /* host, 6 */, int32_t tile_rows, int32_t grid,
+                 int32_t stage_loads, void* ws, uint32_t spin_limit, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

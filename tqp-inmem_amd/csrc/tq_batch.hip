@@ -1,0 +1,298 @@
+// Batched filter -> join -> group-by pipelines in one persistent launch.
+//
+// Workgroups pop (pipeline, tile) items from a device queue (one atomic per item) in phase-major
+// order: clear the pipeline's group table, insert selected build rows, probe selected rows and
+// accumulate, count groups per table tile, write groups.  An item of phase k waits until every
+// tile of phase k-1 of its own pipeline has retired (per-pipeline completion counters with
+// agent-scope release/acquire; the wait is bounded and reports an error instead of hanging).
+// Items are popped in order by running workgroups, so the items waited on are already held by
+// resident workgroups and the queue always drains.
+//
+// The group table of a pipeline is addressed by build key - kmin: it is the join directory
+// (build row per key) and the aggregation table (pair count and accumulators per key), so the
+// probe phase does one directory load per selected probe row and groups come out in key order
+// without sorting.  Predicates and aggregate arguments run on the expression interpreter with the
+// same LDS staging as the per-operator kernels.
+#include "tq_agg.h"
+#include "tq_common.h"
+#include "tq_expr.h"
+
+namespace tq {
+
+struct PhaseStarts {
+  int64_t s[TQ_BATCH_PHASES + 1];
+};
+
+struct BatchWs {
+  unsigned long long* head;
+  uint32_t* done;   // [npipes * phases]
+  uint32_t* err;
+};
+
+__device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int64_t slot_of(const tq_batch_pipe_t& B, int64_t key) {
+  const int64_t s = key - B.kmin;
+  return (s >= 0 && s < B.range) ? s : -1;
+}
+
+__device__ __forceinline__ int acc_kind(int op, int dt) {
+  const bool isf = dt == TQ_F64 || dt == TQ_F32;
+  switch (op) {
+    case TQ_AGG_SUM: case TQ_AGG_AVG: return isf ? ACC_FSUM : ACC_ISUM;
+    case TQ_AGG_COUNT: return ACC_COUNT;
+    case TQ_AGG_MIN: return isf ? ACC_FMIN : ACC_IMIN;
+    default: return isf ? ACC_FMAX : ACC_IMAX;
+  }
+}
+
+// Initial accumulator bits (the device twin of agg_kind_init).
+__device__ __forceinline__ uint64_t acc_init(int op, int dt) {
+  if (op != TQ_AGG_MIN) return 0;
+  if (dt == TQ_F64) return f64_bits(1.7976931348623157e308);
+  if (dt == TQ_F32) return f64_bits(3.4028234663852886e38);
+  return (uint64_t)(dt == TQ_I8 ? 127 : dt == TQ_I16 ? 32767 : dt == TQ_I32 ? 2147483647
+                    : dt == TQ_BOOL ? 1 : INT64_MAX);
+}
+
+// Phase 0: clear table slots [s0, s1).
+__device__ void phase_clear(const tq_batch_pipe_t& B, int64_t s0, int64_t s1) {
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
+    B.dir[s] = 0;
+    B.cnt[s] = 0;
+  }
+  for (int a = 0; a < B.naggs; ++a) {
+    const uint64_t init = acc_init(B.agg_op[a], B.agg_dtype[a]);
+    uint64_t* acc = B.acc + (int64_t)a * B.range;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) acc[s] = init;
+  }
+}
+
+// Phase 1: insert selected build rows [r0, r1) into the directory.
+__device__ void phase_build(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, uint64_t* stage) {
+  const int t = threadIdx.x;
+  const tq_program_t& P = B.bprog;
+  for (int64_t sub0 = r0; sub0 < r1; sub0 += kSubRows) {
+    bool live[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) live[j] = sub0 + (int64_t)j * kBlock + t < r1;
+    stage_loads(P, B.bcols, sub0, live, stage);
+#pragma unroll 1
+    for (int j = 0; j < kSub; ++j) {
+      if (!live[j]) continue;
+      const int64_t row = sub0 + (int64_t)j * kBlock + t;
+      Regs R{};
+      run_program(P, B.bcols, 0, P.seg_end[0], j, row, true, stage, R);
+      if (!R.get(P.seg_result[0])) continue;
+      const int64_t key = (int64_t)stage[(0 * kSub + j) * kBlock + t];
+      const int64_t s = slot_of(B, key);
+      if (s < 0) {
+        atomicOr((unsigned int*)B.flags, 2u);  // key outside the declared range
+        continue;
+      }
+      if (atomicCAS(&B.dir[s], 0u, (uint32_t)(row + 1)) != 0u) atomicOr((unsigned int*)B.flags, 1u);
+    }
+  }
+}
+
+// Phase 2: probe selected rows [r0, r1), accumulate matched rows into their group.
+__device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, uint64_t* stage) {
+  const int t = threadIdx.x;
+  const tq_program_t& P = B.pprog;
+  const tq_program_t& A = B.aprog;
+  for (int64_t sub0 = r0; sub0 < r1; sub0 += kSubRows) {
+    bool live[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) live[j] = sub0 + (int64_t)j * kBlock + t < r1;
+    stage_loads(P, B.pcols, sub0, live, stage);
+    bool hit[kSub];
+    int64_t slot[kSub];
+#pragma unroll 1
+    for (int j = 0; j < kSub; ++j) {
+      hit[j] = false;
+      slot[j] = -1;
+      if (!live[j]) continue;
+      const int64_t row = sub0 + (int64_t)j * kBlock + t;
+      Regs R{};
+      run_program(P, B.pcols, 0, P.seg_end[0], j, row, true, stage, R);
+      if (!R.get(P.seg_result[0])) continue;
+      const int64_t s = slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]);
+      if (s >= 0 && B.dir[s] != 0u) {
+        hit[j] = true;
+        slot[j] = s;
+      }
+    }
+    // every thread has read its own staged words; the aggregate stage reuses the buffer
+    stage_loads(A, B.acols, sub0, hit, stage);
+#pragma unroll 1
+    for (int j = 0; j < kSub; ++j) {
+      if (!hit[j]) continue;
+      const int64_t row = sub0 + (int64_t)j * kBlock + t;
+      const int64_t s = slot[j];
+      atomicAdd(&B.cnt[s], 1u);
+      Regs R{};
+      int begin = 0;
+      for (int a = 0; a < B.naggs; ++a) {
+        const int end = A.seg_end[a];
+        run_program(A, B.acols, begin, end, j, row, true, stage, R);
+        begin = end;
+        const int kind = acc_kind(B.agg_op[a], B.agg_dtype[a]);
+        const uint64_t v = to_acc(kind, B.agg_dtype[a], R.get(A.seg_result[a]));
+        acc_apply(&B.acc[(int64_t)a * B.range + s], kind, v);
+      }
+    }
+  }
+}
+
+__device__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
+  v = wave_sum(v);
+  if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) t += sh[w];
+  __syncthreads();
+  return t;
+}
+
+// Phase 3: occupied slots of [s0, s1) -> tile_counts[tile].
+__device__ void phase_count(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, int tile, uint32_t* sh) {
+  uint32_t c = 0;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) c += B.cnt[s] > 0;
+  c = block_sum_u32(c, sh);
+  if (threadIdx.x == 0) B.tile_counts[tile] = c;
+}
+
+// Phase 4: write the groups of [s0, s1) after the groups of earlier tiles.
+__device__ void phase_write(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, int tile,
+                            uint32_t* sh, uint32_t* wsum) {
+  uint32_t pre = 0;
+  for (int i = threadIdx.x; i < tile; i += kBlock) pre += B.tile_counts[i];
+  int64_t run = block_sum_u32(pre, sh);
+  const int t = threadIdx.x, w = t >> 6;
+  for (int64_t base = s0; base < s1; base += kBlock) {
+    const int64_t s = base + t;
+    const bool f = s < s1 && B.cnt[s] > 0;
+    const uint64_t bal = ballot(f);
+    if (lane_id() == 0) wsum[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      tot += wsum[v];
+      before += v < w ? wsum[v] : 0;
+    }
+    if (f) {
+      const int64_t o = run + before + rank_below(bal);
+      store_int(B.key_out, B.key_dtype, o, B.kmin + s);
+      const uint32_t rows = B.cnt[s];
+      for (int a = 0; a < B.naggs; ++a)
+        write_agg(B.agg_out[a], o, B.agg_op[a], acc_kind(B.agg_op[a], B.agg_dtype[a]),
+                  B.agg_dtype[a], B.acc[(int64_t)a * B.range + s], rows);
+    }
+    run += tot;
+    __syncthreads();
+  }
+  if (tile == B.ntiles[4] - 1 && threadIdx.x == 0) *B.ngroups = run;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restrict__ pipes,
+                                                  const int32_t* __restrict__ items,
+                                                  PhaseStarts ps, int32_t tile_rows, BatchWs ws,
+                                                  uint32_t spin_limit) {
+  extern __shared__ uint64_t stage[];
+  __shared__ int64_t s_item;
+  __shared__ int s_abort;
+  __shared__ uint32_t sh[kWaves];
+  __shared__ uint32_t wsum[kWaves];
+  const int64_t nitems = ps.s[TQ_BATCH_PHASES];
+  while (true) {
+    if (threadIdx.x == 0) {
+      s_item = (int64_t)atomicAdd(ws.head, 1ull);
+      s_abort = load_acquire(ws.err) != 0;
+    }
+    __syncthreads();
+    const int64_t it = s_item;
+    const bool abort = s_abort;
+    __syncthreads();
+    if (it >= nitems || abort) break;
+    const int p = __builtin_amdgcn_readfirstlane(items[2 * it]);
+    const int tile = __builtin_amdgcn_readfirstlane(items[2 * it + 1]);
+    int phase = 0;
+    while (phase < TQ_BATCH_PHASES - 1 && it >= ps.s[phase + 1]) ++phase;
+    const tq_batch_pipe_t& B = pipes[p];
+    if (phase > 0) {
+      if (threadIdx.x == 0) {
+        const uint32_t need = (uint32_t)B.ntiles[phase - 1];
+        const uint32_t* d = ws.done + (int64_t)p * TQ_BATCH_PHASES + (phase - 1);
+        uint32_t spins = 0;
+        while (load_acquire(d) < need) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > spin_limit) {
+            atomicOr(ws.err, 1u);
+            break;
+          }
+        }
+        s_abort = spins > spin_limit;
+      }
+      __syncthreads();
+      if (s_abort) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    const int64_t lo = (int64_t)tile * tile_rows;
+    switch (phase) {
+      case 0: phase_clear(B, lo, min(lo + tile_rows, B.range)); break;
+      case 1: phase_build(B, lo, min(lo + tile_rows, B.nb), stage); break;
+      case 2: phase_probe(B, lo, min(lo + tile_rows, B.np), stage); break;
+      case 3: phase_count(B, lo, min(lo + tile_rows, B.range), tile, sh); break;
+      default: phase_write(B, lo, min(lo + tile_rows, B.range), tile, sh, wsum); break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(ws.done + (int64_t)p * TQ_BATCH_PHASES + phase, 1u, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+static BatchWs carve_batch(void* base, int32_t npipes) {
+  char* p = (char*)base;
+  BatchWs w;
+  w.head = (unsigned long long*)p; p += 256;
+  w.err = (uint32_t*)p; p += 256;
+  w.done = (uint32_t*)p;
+  return w;
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" size_t tq_batch_ws_bytes(int32_t npipes) {
+  return 512 + align_up((size_t)(npipes > 0 ? npipes : 1) * TQ_BATCH_PHASES * 4, 256);
+}
+
+extern "C" int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const int32_t* items,
+                            const int64_t* phase_start, int32_t tile_rows, int32_t grid,
+                            int32_t stage_loads, void* ws, uint32_t spin_limit, void* stream) {
+  if (!pipes || !items || !phase_start || !ws) return set_error(TQ_EINVAL, "batch: null argument");
+  if (tile_rows <= 0 || tile_rows % kSubRows) return set_error(TQ_EINVAL, "batch: tile_rows %d not a multiple of %d", tile_rows, kSubRows);
+  if (stage_loads < 1 || stage_loads > TQ_MAX_LOADS) return set_error(TQ_EINVAL, "batch: stage_loads %d", stage_loads);
+  if (grid < 1) return set_error(TQ_EINVAL, "batch: grid %d", grid);
+  PhaseStarts ps;
+  for (int i = 0; i <= TQ_BATCH_PHASES; ++i) {
+    ps.s[i] = phase_start[i];
+    if (i > 0 && ps.s[i] < ps.s[i - 1]) return set_error(TQ_EINVAL, "batch: phase starts not ascending");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  BatchWs w = carve_batch(ws, npipes);
+  TQ_TRY(fill_u64(w.head, 0, 1, s));
+  TQ_TRY(fill_u32(w.err, 0, 1, s));
+  TQ_TRY(fill_u32(w.done, 0, (int64_t)npipes * TQ_BATCH_PHASES, s));
+  if (ps.s[TQ_BATCH_PHASES] == 0) return TQ_OK;
+  const size_t lds = (size_t)stage_loads * kSubRows * 8;
+  k_batch<<<(unsigned)grid, kBlock, lds, s>>>(pipes, items, ps, tile_rows, w, spin_limit);
+  return check_launch("batch");
+}

@@ -16,58 +16,9 @@
 #include "tq_common.h"
 #include "tq_expr.h"
 #include "tq_sort.h"
+#include "tq_agg.h"
 
 namespace tq {
-
-enum AccKind { ACC_FSUM = 0, ACC_ISUM, ACC_COUNT, ACC_FMIN, ACC_FMAX, ACC_IMIN, ACC_IMAX };
-
-constexpr unsigned long long kEmpty = ~0ull;
-constexpr int kLdsBudget = 64 * 1024;
-
-struct GbSpec {
-  int32_t nkeys, naggs;
-  int32_t key_slot[TQ_GB_MAX_KEYS];  // load slot holding key i
-  int64_t key_min[TQ_GB_MAX_KEYS];
-  int64_t key_stride[TQ_GB_MAX_KEYS];
-  int64_t key_range[TQ_GB_MAX_KEYS];
-  int32_t key_dtype[TQ_GB_MAX_KEYS];
-  int32_t kind[TQ_GB_MAX_AGGS];
-  int32_t arg_dtype[TQ_GB_MAX_AGGS];
-  uint64_t init[TQ_GB_MAX_AGGS];
-};
-
-struct GbTable {
-  uint32_t* cnt;
-  unsigned long long* hkey;
-  uint64_t* acc;  // acc[j * slots + slot]
-  int64_t slots;
-  int32_t log2;
-};
-
-__device__ __forceinline__ uint64_t mix(uint64_t p) { return p * 0x9E3779B97F4A7C15ull; }
-
-// Convert an argument value (program register bits) into its accumulator domain.
-__device__ __forceinline__ uint64_t to_acc(int kind, int dt, uint64_t v) {
-  const bool isf = dt == TQ_F64 || dt == TQ_F32;
-  const double d = dt == TQ_F64 ? as_f64(v) : (double)as_f32(v);
-  switch (kind) {
-    case ACC_FSUM: case ACC_FMIN: case ACC_FMAX: return f64_bits(isf ? d : (double)(int64_t)v);
-    case ACC_COUNT: return isf ? (uint64_t)(d != -64.0) : (uint64_t)((int64_t)v != -64);
-    default: return v;  // integer domain
-  }
-}
-
-__device__ __forceinline__ void acc_apply(uint64_t* a, int kind, uint64_t v) {
-  switch (kind) {
-    case ACC_FSUM: atomicAdd((double*)a, as_f64(v)); break;
-    case ACC_ISUM:
-    case ACC_COUNT: atomicAdd((unsigned long long*)a, (unsigned long long)v); break;
-    case ACC_FMIN: atomicMin((double*)a, as_f64(v)); break;
-    case ACC_FMAX: atomicMax((double*)a, as_f64(v)); break;
-    case ACC_IMIN: atomicMin((long long*)a, (long long)v); break;
-    default: atomicMax((long long*)a, (long long)v); break;
-  }
-}
 
 __device__ __forceinline__ int64_t gb_insert(const GbTable& T, uint64_t p) {
   const uint64_t mask = (uint64_t)T.slots - 1;
@@ -277,16 +228,6 @@ struct OutCols {
   int32_t op[TQ_GB_MAX_AGGS];
 };
 
-__device__ __forceinline__ void store_int(void* p, int dt, int64_t i, int64_t v) {
-  switch (dt) {
-    case TQ_BOOL: ((uint8_t*)p)[i] = v != 0; break;
-    case TQ_I8: ((int8_t*)p)[i] = (int8_t)v; break;
-    case TQ_I16: ((int16_t*)p)[i] = (int16_t)v; break;
-    case TQ_I32: ((int32_t*)p)[i] = (int32_t)v; break;
-    default: ((int64_t*)p)[i] = v; break;
-  }
-}
-
 __device__ __forceinline__ void write_group(const GbSpec& G, const OutCols& O, const GbTable& T,
                                             int64_t o, int64_t slot, uint64_t p) {
   for (int k = 0; k < G.nkeys; ++k) {
@@ -294,29 +235,8 @@ __device__ __forceinline__ void write_group(const GbSpec& G, const OutCols& O, c
     store_int(O.key[k], G.key_dtype[k], o, v);
   }
   const uint32_t rows = T.cnt[slot];
-  for (int a = 0; a < G.naggs; ++a) {
-    const uint64_t acc = T.acc[(int64_t)a * T.slots + slot];
-    const int dt = G.arg_dtype[a];
-    switch (O.op[a]) {
-      case TQ_AGG_AVG: {
-        const double sum = G.kind[a] == ACC_FSUM ? as_f64(acc) : (double)(int64_t)acc;
-        const double r = sum / (double)rows;
-        if (dt == TQ_F32) ((float*)O.agg[a])[o] = (float)r; else ((double*)O.agg[a])[o] = r;
-        break;
-      }
-      case TQ_AGG_COUNT: ((int64_t*)O.agg[a])[o] = (int64_t)acc; break;
-      case TQ_AGG_SUM:
-        if (G.kind[a] == ACC_ISUM) ((int64_t*)O.agg[a])[o] = (int64_t)acc;
-        else if (dt == TQ_F32) ((float*)O.agg[a])[o] = (float)as_f64(acc);
-        else ((double*)O.agg[a])[o] = as_f64(acc);
-        break;
-      default:  // MIN / MAX in the argument dtype
-        if (dt == TQ_F64) ((double*)O.agg[a])[o] = as_f64(acc);
-        else if (dt == TQ_F32) ((float*)O.agg[a])[o] = (float)as_f64(acc);
-        else store_int(O.agg[a], dt, o, (int64_t)acc);
-        break;
-    }
-  }
+  for (int a = 0; a < G.naggs; ++a)
+    write_agg(O.agg[a], o, O.op[a], G.kind[a], G.arg_dtype[a], T.acc[(int64_t)a * T.slots + slot], rows);
 }
 
 // Dense tables: groups are the occupied slots in slot (= key) order.
@@ -450,6 +370,26 @@ static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs) {
   return w;
 }
 
+int agg_kind_init(int op, int dt, int32_t* kind, uint64_t* init) {
+  const bool isf = dt == TQ_F64 || dt == TQ_F32;
+  *init = 0;
+  switch (op) {
+    case TQ_AGG_SUM:
+    case TQ_AGG_AVG: *kind = isf ? ACC_FSUM : ACC_ISUM; break;
+    case TQ_AGG_COUNT: *kind = ACC_COUNT; break;
+    case TQ_AGG_MIN:
+      *kind = isf ? ACC_FMIN : ACC_IMIN;
+      if (dt == TQ_F64) { double v = 1.7976931348623157e308; memcpy(init, &v, 8); }
+      else if (dt == TQ_F32) { double v = 3.4028234663852886e38; memcpy(init, &v, 8); }
+      else *init = (uint64_t)(dt == TQ_I8 ? 127 : dt == TQ_I16 ? 32767 : dt == TQ_I32 ? 2147483647
+                              : dt == TQ_BOOL ? 1 : INT64_MAX);
+      break;
+    case TQ_AGG_MAX: *kind = isf ? ACC_FMAX : ACC_IMAX; break;
+    default: return set_error(TQ_EUNSUPPORTED, "aggregate op %d", op);
+  }
+  return TQ_OK;
+}
+
 static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpec* G) {
   *G = GbSpec{};
   if (d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS) return set_error(TQ_EINVAL, "group-by: %d keys", d->nkeys);
@@ -468,26 +408,8 @@ static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpe
       return set_error(TQ_EINVAL, "group-by: load slot %d must hold key %d", k, k);
   }
   for (int a = 0; a < d->naggs; ++a) {
-    const int dt = d->agg_dtype[a];
-    const bool isf = dt == TQ_F64 || dt == TQ_F32;
-    G->arg_dtype[a] = dt;
-    switch (d->agg_op[a]) {
-      case TQ_AGG_SUM:
-      case TQ_AGG_AVG: G->kind[a] = isf ? ACC_FSUM : ACC_ISUM; G->init[a] = 0; break;
-      case TQ_AGG_COUNT: G->kind[a] = ACC_COUNT; G->init[a] = 0; break;
-      case TQ_AGG_MIN:
-        G->kind[a] = isf ? ACC_FMIN : ACC_IMIN;
-        if (dt == TQ_F64) { double v = 1.7976931348623157e308; memcpy(&G->init[a], &v, 8); }
-        else if (dt == TQ_F32) { double v = 3.4028234663852886e38; memcpy(&G->init[a], &v, 8); }
-        else {
-          const int64_t mx = dt == TQ_I8 ? 127 : dt == TQ_I16 ? 32767 : dt == TQ_I32 ? 2147483647
-                           : dt == TQ_BOOL ? 1 : INT64_MAX;
-          G->init[a] = (uint64_t)mx;
-        }
-        break;
-      case TQ_AGG_MAX: G->kind[a] = isf ? ACC_FMAX : ACC_IMAX; G->init[a] = 0; break;
-      default: return set_error(TQ_EUNSUPPORTED, "aggregate op %d", d->agg_op[a]);
-    }
+    G->arg_dtype[a] = d->agg_dtype[a];
+    TQ_TRY(agg_kind_init(d->agg_op[a], d->agg_dtype[a], &G->kind[a], &G->init[a]));
   }
   if (prog && prog->nseg < d->naggs) return set_error(TQ_EINVAL, "group-by: program has %d segments for %d aggregates", prog->nseg, d->naggs);
   return TQ_OK;

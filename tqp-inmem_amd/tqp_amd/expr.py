@@ -62,7 +62,7 @@ class ProgramBuilder:
         self.prog = L.Program()
         self.cols = L.ColSet()
         self.col_tensors = []       # keep tensors alive while the program is in use
-        self.col_ids = {}           # (python id of tensor) -> colset index
+        self.col_ids = {}           # (address, dtype, shape) -> colset index
         self.load_of = {}           # colset index -> load slot
         self.nload = 0
         self.ninstr = 0
@@ -73,7 +73,7 @@ class ProgramBuilder:
 
     # ---------------------------------------------------------------- columns and registers
     def column(self, t: torch.Tensor) -> int:
-        key = id(t)
+        key = (t.data_ptr(), t.dtype, tuple(t.shape))
         if key in self.col_ids:
             return self.col_ids[key]
         idx = len(self.col_tensors)

@@ -144,6 +144,51 @@ def config2_plan():
     ]
 
 
+# ---------------------------------------------------------------- config 3: batched pipelines
+def config3_sizes(n_pipes=256, lo=10_000, hi=6_000_000):
+    """Probe row counts log-uniform in [lo, hi]; build side a quarter of each."""
+    g = rng(3, 0)
+    probe = np.exp(g.uniform(np.log(lo), np.log(hi), n_pipes)).astype(np.int64)
+    build = np.maximum(probe // 4, 1)
+    consts = g.integers(0, 2557, (n_pipes, 2))
+    return probe, build, consts
+
+
+def config3_pipeline(i, n_probe, n_build, c_build, c_probe, device):
+    """Tables (generated on `device` from a per-pipeline seed) and config-1-shaped plan of
+    pipeline i: build filter order_day <= c_build, probe filter ship_day > c_probe, join on
+    key = fk, group by key, sum(price * (1 - discount)) and count(*)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(SEED + 3000 + int(i))
+    nb, np_ = int(n_build), int(n_probe)
+    kw = dict(device=device, generator=gen)
+    group = {
+        101: Variable((torch.randperm(nb, **kw) + 1).to(torch.int32), 'int', None, VariableState.GPU),
+        103: Variable(torch.randint(0, 2557, (nb,), dtype=torch.int32, **kw), 'date', None, VariableState.GPU),
+        201: Variable(torch.randint(1, nb + 1, (np_,), dtype=torch.int32, **kw), 'int', None, VariableState.GPU),
+        202: Variable(torch.rand(np_, dtype=torch.float64, **kw) * 104100.0 + 900.0, 'float', None, VariableState.GPU),
+        203: Variable(torch.randint(0, 11, (np_,), dtype=torch.int64, **kw).to(torch.float64) / 100.0, 'float', None, VariableState.GPU),
+        204: Variable(torch.randint(0, 2557, (np_,), dtype=torch.int32, **kw), 'date', None, VariableState.GPU),
+    }
+    plan = [
+        (1, {'name': 'Filter', 'filter tree': [['order_day#103', '<=', str(int(c_build))]],
+             'materialized in': [103], 'out mask id': 10001}),
+        (2, {'name': 'Filter', 'filter tree': [['ship_day#204', '>', str(int(c_probe))]],
+             'materialized in': [204], 'out mask id': 10002}),
+        (3, {'name': 'Join', 'build': 'left', 'left group': [101, 103],
+             'right group': [201, 202, 203, 204], 'left keys': [101], 'right keys': [201],
+             'mask map': {101: 10001, 201: 10002, 202: 10002, 203: 10002},
+             'materialized in': [101, 201, 202, 203], 'join type': 'Inner'}),
+        (4, {'name': 'Aggregate', 'input ids': [101, 202, 203], 'keys': ['key#101'],
+             'results': ['key#101',
+                         [[[['sum', [['price#202', '*', ['1.0', '-', 'discount#203']]]], '#', '40'],
+                           'AS', 'revenue#301']],
+                         [[[['count', ['1']], '#', '41'], 'AS', 'cnt#302']]],
+             'out agg ids': [301, 302], 'is partial': False}),
+    ]
+    return group, plan
+
+
 # ---------------------------------------------------------------- helpers
 def to_tensor_group(tables, device):
     """{id: (type, ndarray)} -> {id: Variable} on `device`."""
