@@ -204,9 +204,15 @@ typedef struct {
   const void* keys;     /* device: sorted transformed keys */
   const uint32_t* rows; /* device: build row ids */
   const void* sample;   /* device: search sample */
+  /* dense directory, present (dir != NULL) when the selected keys span at most
+   * 4 * nsel + 65536 values: dir[t - dir_base] = first position of transformed key t, or
+   * 0xFFFFFFFF; a probe is then one load instead of a search */
+  const uint32_t* dir;
+  uint64_t dir_base;
+  int64_t dir_range;
 } tq_join_state_t;
 
-/* Workspace for building over nb build rows. */
+/* Workspace for building over nb build rows (room for the dense directory included). */
 size_t tq_join_build_ws_bytes(int64_t nb);
 
 /* Build: sort the keys of the selected build rows (bmask may be NULL) with their row ids.
@@ -303,9 +309,11 @@ int tq_groupby_write(const tq_groupby_desc_t* desc /* host */, int32_t mode, int
                      void* const* agg_out /* host */, void* stream);
 
 /* ---------------------------------------------------------------- batched pipelines */
-/* Many filter -> join -> group-by pipelines in one persistent launch.  A pipeline joins its
- * selected build rows (unique build keys) with its selected probe rows on key equality and
- * groups the pairs by the build key, reducing aggregate arguments computed over probe columns.
+/* Many filter -> join -> group-by pipelines in one persistent launch.  A join pipeline (mode 0)
+ * joins its selected build rows (unique build keys) with its selected probe rows on key equality
+ * and groups the pairs by the build key, reducing aggregate arguments computed over probe
+ * columns.  A group-by pipeline (mode 1) has no build side: it groups its selected rows by up to
+ * 8 integer key columns packed in mixed radix (as tq_groupby_accumulate does).
  * The group table is addressed by build key - kmin over the build key's [min, max] range, so it
  * is the join directory and the aggregation table at once, and groups come out in key order.
  * Work is a device queue of (pipeline, tile) items in five phases -- clear the table, insert
@@ -321,26 +329,33 @@ int tq_groupby_write(const tq_groupby_desc_t* desc /* host */, int32_t mode, int
 typedef struct {
   tq_program_t bprog;  /* build predicate; load slot 0 must be the build key column  */
   tq_colset_t bcols;
-  tq_program_t pprog;  /* probe predicate; load slot 0 must be the probe key column  */
-  tq_colset_t pcols;
+  tq_program_t pprog;  /* probe predicate; load slot 0 must be the probe key column;  */
+  tq_colset_t pcols;   /*   group-by pipelines: slots 0..nkeys-1 are the key columns   */
   tq_program_t aprog;  /* aggregate arguments over probe columns, one segment each   */
   tq_colset_t acols;
   int64_t nb, np;      /* build / probe rows                                          */
-  int64_t kmin, range; /* build key range: table slot = key - kmin                    */
+  int64_t kmin, range; /* table slot = key - kmin (join) or packed key (group-by)     */
   int32_t naggs;
   int32_t key_dtype;   /* dtype of the build key (and of the group key output)        */
   int32_t agg_op[TQ_MAX_SEGS];
   int32_t agg_dtype[TQ_MAX_SEGS];
   uint32_t* dir;       /* device [range]: build row + 1, 0 = empty                     */
-  uint32_t* cnt;       /* device [range]: pairs per group                              */
+  uint32_t* cnt;       /* device [range]: rows per group                               */
   uint64_t* acc;       /* device [naggs * range]: accumulators                         */
   uint32_t* tile_counts; /* device [ntiles[3]]                                         */
-  void* key_out;       /* device [min(nb, range)] group keys                           */
+  void* key_out;       /* device [min(nb, range)] group keys (join pipelines)          */
   void* agg_out[TQ_MAX_SEGS];
   int64_t* ngroups;    /* device [1]                                                   */
   int32_t* flags;      /* device [1]                                                   */
   int32_t ntiles[TQ_BATCH_PHASES];
-  int32_t pad;
+  int32_t mode;        /* 0: join pipeline; 1: group-by pipeline (no build side)        */
+  int32_t nkeys;       /* group-by pipelines: key columns (0 = one global group)        */
+  int32_t lds_table;   /* 1: probe items accumulate in an LDS copy of the table first   */
+  int64_t gkey_min[TQ_GB_MAX_KEYS];    /* group-by pipelines: mixed-radix packing      */
+  int64_t gkey_stride[TQ_GB_MAX_KEYS];
+  int64_t gkey_range[TQ_GB_MAX_KEYS];
+  int32_t gkey_dtype[TQ_GB_MAX_KEYS];
+  void* gkey_out[TQ_GB_MAX_KEYS];
 } tq_batch_pipe_t;
 
 /* Counters the launch needs (queue head, per-pipeline phase completion, error word). */
@@ -349,11 +364,14 @@ size_t tq_batch_ws_bytes(int32_t npipes);
 /* Run the batch.  pipes: device array of npipes descriptors; items: device int32 pairs
  * (pipeline, tile), phase-major, with phase_start[0..5] (host) the first item of each phase;
  * tile_rows: rows (or table slots) per item, a multiple of 1024; grid: persistent workgroups;
- * stage_loads: the largest nload of any program.  ws is cleared by the call (kernels only), and
+ * stage_loads: the largest nload of any program; lds_table_bytes: the largest LDS table
+ * (range * (4 + 8 * naggs)) of the pipelines with lds_table set.  ws is cleared by the call
+ * (kernels only), and
  * ws error word 1 means a wait exceeded spin_limit (the launch then drains without finishing). */
 int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const int32_t* items,
                  const int64_t* phase_start /* host, 6 */, int32_t tile_rows, int32_t grid,
-                 int32_t stage_loads, void* ws, uint32_t spin_limit, void* stream);
+                 int32_t stage_loads, int32_t lds_table_bytes, void* ws, uint32_t spin_limit,
+                 void* stream);
 
 #ifdef __cplusplus
 }

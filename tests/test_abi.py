@@ -48,7 +48,7 @@ def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(_lib.Program) == 64 + 64 * 16 + 128
     assert ctypes.sizeof(_lib.ColSet) == 16 * 8 + 16 * 4
     assert ctypes.sizeof(_lib.SortKey) == 24
-    assert ctypes.sizeof(_lib.JoinState) == 48
+    assert ctypes.sizeof(_lib.JoinState) == 72
 
 
 def test_error_codes_map_to_reference_exceptions():
@@ -57,3 +57,37 @@ def test_error_codes_map_to_reference_exceptions():
     with pytest.raises(ValueError):
         _lib.call("tq_filter_mask", None, None, 10, None, None, None)
     assert "null" in _lib.load().tq_last_error().decode()
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof/offsetof from gcc on include/tqhip.h equal the ctypes mirrors."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    from tqp_amd import _lib, batched
+    checks = {
+        "tq_program_t": (_lib.Program, ["nload", "code", "strpool"]),
+        "tq_colset_t": (_lib.ColSet, ["width"]),
+        "tq_sortkey_t": (_lib.SortKey, ["descending"]),
+        "tq_join_state_t": (_lib.JoinState, ["sample", "dir", "dir_range"]),
+        "tq_groupby_desc_t": (_lib.GroupByDesc, ["key_ptr", "key_stride", "agg_dtype"]),
+        "tq_batch_pipe_t": (batched.BatchPipe, ["pprog", "acols", "dir", "agg_out", "ntiles",
+                                                "gkey_min", "gkey_out"]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void){']
+    for cname, (_, fields) in checks.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in fields:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append('return 0;}')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                   text=True).stdout.splitlines())
+    for cname, (cls, fields) in checks.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f in fields:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"

@@ -78,3 +78,29 @@ def test_batched_rejects_duplicate_build_keys(device):
     eng.run()
     out, rerun = eng.results()
     assert rerun == [0] and 0 not in out
+
+
+def test_batched_groupby_pipelines_match_plain(device):
+    """Group-by pipelines (filter -> aggregate -> key sort, the config-0 shape) batched together
+    with join pipelines give the per-operator results."""
+    from tqp_amd import batched, executor, workloads
+    base = workloads.to_tensor_group(workloads.config0_tables(n=300_000), device)
+    plans = [workloads.config0_plan(ship_day_max=m) for m in (100, 1800, 2556)]
+    specs = [batched.match_groupby_plan(p, base) for p in plans]
+    assert all(s is not None for s in specs)
+    probe, build, consts = workloads.config3_sizes(n_pipes=4, lo=5000, hi=80_000)
+    pipes = [workloads.config3_pipeline(i, probe[i], build[i], consts[i][0], consts[i][1], device)
+             for i in range(4)]
+    specs += [batched.match_plan(p, g) for g, p in pipes]
+    eng = batched.BatchedEngine(specs, tile_rows=2048)
+    eng.run()
+    out, rerun = eng.results()
+    assert rerun == []
+    for i, plan in enumerate(plans):
+        tg = dict(base)
+        executor.execute(copy.deepcopy(plan), tg)
+        for k in (6, 7, 28):
+            assert torch.equal(out[i][k].reshape(tg[k].tensor.shape).cpu(), tg[k].tensor.cpu()), (i, k)
+        for k in (22, 23, 24, 25, 26, 27):
+            np.testing.assert_allclose(out[i][k].cpu().numpy(), tg[k].tensor.cpu().numpy(),
+                                       rtol=1e-12, atol=0, err_msg=f"{i} {k}")

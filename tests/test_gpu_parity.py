@@ -289,3 +289,33 @@ def test_config2_pipeline_reduced(device):
     for k in (322, 313, 402):
         _assert_col(gpu[k].tensor, cpu[k].tensor, f"c2 {k}")
     _assert_col(gpu[401].tensor, cpu[401].tensor, "c2 sum", fp=True)
+
+
+def test_config2_full_size_properties(device):
+    """60M-row star at full size: every fact row joins all three dimensions (keys are drawn from
+    them), so the group counts sum to the fact row count and the group sums add up to the total
+    amount; groups are distinct and ascending; the count of each (c_attr, b_mod) group equals a
+    direct histogram of the fact rows' attribute pairs."""
+    from tqp_amd import executor, workloads
+    tables = workloads.config2_tables()
+    gpu = _gpu_group(tables, device)
+    print("config2 full size: tables resident", flush=True)
+    executor.execute(copy.deepcopy(workloads.config2_plan()), gpu)
+    c = gpu[322].tensor.reshape(-1).cpu().numpy().astype(np.int64)
+    b = gpu[313].tensor.reshape(-1).cpu().numpy().astype(np.int64)
+    cnt = gpu[402].tensor.cpu().numpy()
+    tot = gpu[401].tensor.cpu().numpy()
+    n_fact = tables[331][1].shape[0]
+    assert cnt.sum() == n_fact
+    amount = tables[334][1]
+    assert abs(tot.sum() - amount.sum()) <= 1e-10 * amount.sum()
+    key = c * 1000 + b
+    assert np.all(np.diff(key) > 0)
+    # independent histogram of (c_attr, b_attr % 100) over fact rows via the dimension maps
+    c_attr = np.zeros(26, np.int64)
+    c_attr[tables[321][1]] = tables[322][1]
+    b_mod = np.zeros(tables[311][1].max() + 1, np.int64)
+    b_mod[tables[311][1]] = tables[313][1]
+    fk = c_attr[tables[333][1]] * 1000 + b_mod[tables[332][1]]
+    u, n = np.unique(fk, return_counts=True)
+    assert np.array_equal(u, key) and np.array_equal(n, cnt)

@@ -97,11 +97,36 @@ __device__ void phase_build(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
   }
 }
 
-// Phase 2: probe selected rows [r0, r1), accumulate matched rows into their group.
-__device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, uint64_t* stage) {
+// Packed group key of a group-by pipeline from its staged key slots.
+__device__ __forceinline__ int64_t pack_gkey(const tq_batch_pipe_t& B, const uint64_t* stage, int j) {
+  int64_t p = 0;
+  for (int k = 0; k < B.nkeys; ++k) {
+    const int64_t v = (int64_t)stage[(k * kSub + j) * kBlock + threadIdx.x];
+    p += (v - B.gkey_min[k]) * B.gkey_stride[k];
+  }
+  return p;
+}
+
+// Phase 2: probe / scan selected rows [r0, r1) and accumulate each matched row into its group.
+// With lds_table, the item accumulates into an LDS copy of the (small) table and merges it into
+// the global table once at the end, so few hot groups cost one global atomic per slot per item.
+__device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, uint64_t* stage,
+                            uint64_t* ltab) {
   const int t = threadIdx.x;
   const tq_program_t& P = B.pprog;
   const tq_program_t& A = B.aprog;
+  const bool local = B.lds_table != 0;
+  uint32_t* lcnt = (uint32_t*)(ltab + (int64_t)B.naggs * B.range);
+  if (local) {
+    for (int64_t s = t; s < B.range; s += kBlock) {
+      lcnt[s] = 0;
+      for (int a = 0; a < B.naggs; ++a)
+        ltab[(int64_t)a * B.range + s] = acc_init(B.agg_op[a], B.agg_dtype[a]);
+    }
+    __syncthreads();
+  }
+  uint32_t* cnt = local ? lcnt : B.cnt;
+  uint64_t* acc = local ? ltab : B.acc;
   for (int64_t sub0 = r0; sub0 < r1; sub0 += kSubRows) {
     bool live[kSub];
 #pragma unroll
@@ -118,6 +143,11 @@ __device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
       Regs R{};
       run_program(P, B.pcols, 0, P.seg_end[0], j, row, true, stage, R);
       if (!R.get(P.seg_result[0])) continue;
+      if (B.mode == 1) {
+        hit[j] = true;
+        slot[j] = pack_gkey(B, stage, j);
+        continue;
+      }
       const int64_t s = slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]);
       if (s >= 0 && B.dir[s] != 0u) {
         hit[j] = true;
@@ -131,7 +161,7 @@ __device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
       if (!hit[j]) continue;
       const int64_t row = sub0 + (int64_t)j * kBlock + t;
       const int64_t s = slot[j];
-      atomicAdd(&B.cnt[s], 1u);
+      atomicAdd(&cnt[s], 1u);
       Regs R{};
       int begin = 0;
       for (int a = 0; a < B.naggs; ++a) {
@@ -140,9 +170,21 @@ __device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
         begin = end;
         const int kind = acc_kind(B.agg_op[a], B.agg_dtype[a]);
         const uint64_t v = to_acc(kind, B.agg_dtype[a], R.get(A.seg_result[a]));
-        acc_apply(&B.acc[(int64_t)a * B.range + s], kind, v);
+        acc_apply(&acc[(int64_t)a * B.range + s], kind, v);
       }
     }
+  }
+  if (local) {
+    __syncthreads();
+    for (int64_t s = t; s < B.range; s += kBlock) {
+      const uint32_t c = lcnt[s];
+      if (c == 0) continue;
+      atomicAdd(&B.cnt[s], c);
+      for (int a = 0; a < B.naggs; ++a)
+        acc_apply(&B.acc[(int64_t)a * B.range + s], acc_kind(B.agg_op[a], B.agg_dtype[a]),
+                  ltab[(int64_t)a * B.range + s]);
+    }
+    __syncthreads();
   }
 }
 
@@ -159,8 +201,9 @@ __device__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
 
 // Phase 3: occupied slots of [s0, s1) -> tile_counts[tile].
 __device__ void phase_count(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, int tile, uint32_t* sh) {
+  const bool force0 = B.mode == 1 && B.nkeys == 0;
   uint32_t c = 0;
-  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) c += B.cnt[s] > 0;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) c += (B.cnt[s] > 0) || (force0 && s == 0);
   c = block_sum_u32(c, sh);
   if (threadIdx.x == 0) B.tile_counts[tile] = c;
 }
@@ -172,9 +215,10 @@ __device__ void phase_write(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, in
   for (int i = threadIdx.x; i < tile; i += kBlock) pre += B.tile_counts[i];
   int64_t run = block_sum_u32(pre, sh);
   const int t = threadIdx.x, w = t >> 6;
+  const bool force0 = B.mode == 1 && B.nkeys == 0;
   for (int64_t base = s0; base < s1; base += kBlock) {
     const int64_t s = base + t;
-    const bool f = s < s1 && B.cnt[s] > 0;
+    const bool f = s < s1 && ((B.cnt[s] > 0) || (force0 && s == 0));
     const uint64_t bal = ballot(f);
     if (lane_id() == 0) wsum[w] = (uint32_t)__popcll(bal);
     __syncthreads();
@@ -186,7 +230,13 @@ __device__ void phase_write(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, in
     }
     if (f) {
       const int64_t o = run + before + rank_below(bal);
-      store_int(B.key_out, B.key_dtype, o, B.kmin + s);
+      if (B.mode == 1) {
+        for (int k = 0; k < B.nkeys; ++k)
+          store_int(B.gkey_out[k], B.gkey_dtype[k], o,
+                    (s / B.gkey_stride[k]) % B.gkey_range[k] + B.gkey_min[k]);
+      } else {
+        store_int(B.key_out, B.key_dtype, o, B.kmin + s);
+      }
       const uint32_t rows = B.cnt[s];
       for (int a = 0; a < B.naggs; ++a)
         write_agg(B.agg_out[a], o, B.agg_op[a], acc_kind(B.agg_op[a], B.agg_dtype[a]),
@@ -201,8 +251,9 @@ __device__ void phase_write(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, in
 __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restrict__ pipes,
                                                   const int32_t* __restrict__ items,
                                                   PhaseStarts ps, int32_t tile_rows, BatchWs ws,
-                                                  uint32_t spin_limit) {
+                                                  uint32_t spin_limit, int32_t stage_words) {
   extern __shared__ uint64_t stage[];
+  uint64_t* ltab = stage + stage_words;
   __shared__ int64_t s_item;
   __shared__ int s_abort;
   __shared__ uint32_t sh[kWaves];
@@ -245,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restr
     switch (phase) {
       case 0: phase_clear(B, lo, min(lo + tile_rows, B.range)); break;
       case 1: phase_build(B, lo, min(lo + tile_rows, B.nb), stage); break;
-      case 2: phase_probe(B, lo, min(lo + tile_rows, B.np), stage); break;
+      case 2: phase_probe(B, lo, min(lo + tile_rows, B.np), stage, ltab); break;
       case 3: phase_count(B, lo, min(lo + tile_rows, B.range), tile, sh); break;
       default: phase_write(B, lo, min(lo + tile_rows, B.range), tile, sh, wsum); break;
     }
@@ -276,7 +327,8 @@ extern "C" size_t tq_batch_ws_bytes(int32_t npipes) {
 
 extern "C" int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const int32_t* items,
                             const int64_t* phase_start, int32_t tile_rows, int32_t grid,
-                            int32_t stage_loads, void* ws, uint32_t spin_limit, void* stream) {
+                            int32_t stage_loads, int32_t lds_table_bytes, void* ws,
+                            uint32_t spin_limit, void* stream) {
   if (!pipes || !items || !phase_start || !ws) return set_error(TQ_EINVAL, "batch: null argument");
   if (tile_rows <= 0 || tile_rows % kSubRows) return set_error(TQ_EINVAL, "batch: tile_rows %d not a multiple of %d", tile_rows, kSubRows);
   if (stage_loads < 1 || stage_loads > TQ_MAX_LOADS) return set_error(TQ_EINVAL, "batch: stage_loads %d", stage_loads);
@@ -292,7 +344,12 @@ extern "C" int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const 
   TQ_TRY(fill_u32(w.err, 0, 1, s));
   TQ_TRY(fill_u32(w.done, 0, (int64_t)npipes * TQ_BATCH_PHASES, s));
   if (ps.s[TQ_BATCH_PHASES] == 0) return TQ_OK;
-  const size_t lds = (size_t)stage_loads * kSubRows * 8;
-  k_batch<<<(unsigned)grid, kBlock, lds, s>>>(pipes, items, ps, tile_rows, w, spin_limit);
+  if (lds_table_bytes < 0 || lds_table_bytes > 48 * 1024)
+    return set_error(TQ_EINVAL, "batch: LDS table of %d bytes", lds_table_bytes);
+  const int32_t stage_words = stage_loads * kSubRows;
+  const size_t lds = (size_t)stage_words * 8 + align_up((size_t)lds_table_bytes, 8);
+  if (lds > 64 * 1024) return set_error(TQ_EINVAL, "batch: %zu bytes of LDS", lds);
+  k_batch<<<(unsigned)grid, kBlock, lds, s>>>(pipes, items, ps, tile_rows, w, spin_limit,
+                                              stage_words);
   return check_launch("batch");
 }

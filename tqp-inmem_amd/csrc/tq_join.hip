@@ -2,10 +2,12 @@
 // along, so each equal-key bucket lists build rows ascending); probe = two passes over probe
 // tiles (count, then write at exclusive-scan offsets) that emit pairs in probe-row order.
 //
-// Probe lookups search the sorted keys in two levels: every workgroup first copies an evenly
-// spaced sample of the sorted keys into LDS (the last key of each block of `step` keys), finds
-// the block by binary search in LDS, and only then searches that block in global memory, so a
-// lookup touches one or two cache lines of the (L2-resident) key array instead of log2(nsel).
+// Probe lookups: when the selected build keys are dense (span <= 4*nsel + 65536 values) a
+// directory indexed by key - min key gives the first sorted position in one load.  Otherwise
+// the sorted keys are searched in two levels: every workgroup first copies an evenly spaced
+// sample of the sorted keys into LDS (the last key of each block of `step` keys), finds the
+// block by binary search in LDS, and only then searches that block in global memory, so a
+// lookup touches one or two cache lines of the key array instead of log2(nsel).
 #include "tq_common.h"
 #include "tq_sort.h"
 
@@ -17,10 +19,15 @@ struct JoinView {
   const void* keys;
   const uint32_t* rows;
   const void* sample;
+  const uint32_t* dir;
+  uint64_t dir_base;
+  int64_t dir_range;
   int64_t nsel;
   int64_t step;
   int32_t nsample;
 };
+
+constexpr uint32_t kNoPos = 0xFFFFFFFFu;
 
 template <typename K>
 __device__ __forceinline__ bool probe_key(const void* p, int dt, int64_t r, K* out) {
@@ -61,6 +68,15 @@ template <typename K>
 __device__ __forceinline__ uint32_t bucket(const JoinView& J, const K* __restrict__ ssample, K t,
                                            int64_t* first) {
   const K* keys = (const K*)J.keys;
+  if (J.dir) {
+    const uint64_t off = (uint64_t)t - J.dir_base;  // wraps to a huge value below the base
+    if (off >= (uint64_t)J.dir_range) return 0;
+    const uint32_t p = J.dir[off];
+    *first = p;
+    if (p == kNoPos) return 0;
+    if ((int64_t)p + 1 >= J.nsel || keys[p + 1] != t) return 1;
+    return (uint32_t)(search<K>(J, ssample, keys, t, true) - p);
+  }
   const int64_t p = search<K>(J, ssample, keys, t, false);
   *first = p;
   if (p >= J.nsel || keys[p] != t) return 0;
@@ -126,6 +142,17 @@ __global__ __launch_bounds__(kBlock) void k_join_sample(const K* __restrict__ ke
     int64_t e = (int64_t)(i + 1) * step;
     if (e > nsel) e = nsel;
     sample[i] = keys[e - 1];
+  }
+}
+
+// Dense directory: dir[key - base] = first sorted position of key (dir pre-filled with kNoPos).
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_join_dir(const K* __restrict__ keys, int64_t nsel,
+                                                     uint64_t base, uint32_t* __restrict__ dir) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nsel; i += stride) {
+    const K k = keys[i];
+    if (i == 0 || keys[i - 1] != k) dir[(uint64_t)k - base] = (uint32_t)i;
   }
 }
 
@@ -229,8 +256,11 @@ struct BuildWs {
   void* stats;
   void* radix;
   void* sample;
+  uint32_t* dir;
   size_t bytes;
 };
+
+static int64_t dir_cap(int64_t nsel) { return 4 * nsel + 65536; }
 
 static BuildWs carve_build(void* base, int64_t nb) {
   if (nb < 1) nb = 1;
@@ -248,6 +278,7 @@ static BuildWs carve_build(void* base, int64_t nb) {
   w.stats = p; p += 256;
   w.radix = p; p += radix_ws_bytes(nb);
   w.sample = p; p += kSampleBytes;
+  w.dir = (uint32_t*)p; p += align_up((size_t)dir_cap(nb) * 4, 256);
   w.bytes = (size_t)(p - (char*)base);
   return w;
 }
@@ -278,6 +309,9 @@ static JoinView view_of(const tq_join_state_t* st) {
   J.keys = st->keys;
   J.rows = st->rows;
   J.sample = st->sample;
+  J.dir = st->dir;
+  J.dir_base = st->dir_base;
+  J.dir_range = st->dir_range;
   J.nsel = st->nsel;
   J.step = st->step;
   J.nsample = st->nsample;
@@ -301,7 +335,8 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
   TQ_TRY(check_launch("join pack"));
   if (bmask) TQ_TRY(read_back(w.total, 8, &nsel, s));
   int lo, hi;
-  TQ_TRY(key_bit_range<K>((const K*)w.k0, nsel, w.stats, s, &lo, &hi));
+  uint64_t kmin = 0, kmax = 0;
+  TQ_TRY(key_bit_range<K>((const K*)w.k0, nsel, w.stats, s, &lo, &hi, &kmin, &kmax));
   bool alt;
   TQ_TRY(radix_sort_pairs<K>((K*)w.k0, w.v0, (K*)w.k1, w.v1, nsel, lo, hi, w.radix, s, &alt));
   const K* keys = (const K*)(alt ? w.k1 : w.k0);
@@ -314,6 +349,18 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
     k_join_sample<K><<<(unsigned)ceil_div(nsample_eff, kBlock), kBlock, 0, s>>>(
         keys, nsel, step, nsample_eff, (K*)w.sample);
     TQ_TRY(check_launch("join sample"));
+  }
+  st->dir = nullptr;
+  st->dir_base = 0;
+  st->dir_range = 0;
+  if (nsel > 0 && kmax - kmin < (uint64_t)dir_cap(nsel)) {
+    const int64_t range = (int64_t)(kmax - kmin) + 1;
+    TQ_TRY(fill_u32(w.dir, kNoPos, range, s));
+    k_join_dir<K><<<stream_grid(nsel), kBlock, 0, s>>>(keys, nsel, kmin, w.dir);
+    TQ_TRY(check_launch("join directory"));
+    st->dir = w.dir;
+    st->dir_base = kmin;
+    st->dir_range = range;
   }
   st->nsel = nsel;
   st->key_bits = (int32_t)(8 * sizeof(K));

@@ -16,10 +16,12 @@ template <typename K>
 int radix_sort_pairs(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
                      int lo_bit, int hi_bit, void* ws, hipStream_t s, bool* in_alt);
 
-// Bit range [lo, hi) in which the n keys differ (lo rounded down to a digit boundary); reads
-// the answer back to the host, so it synchronises the stream.
+// Bit range [lo, hi) in which the n keys differ (lo rounded down to a digit boundary) and,
+// optionally, the smallest and largest key; reads the answer back to the host, so it
+// synchronises the stream.  stats_dev holds 4 words.
 template <typename K>
-int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int* lo, int* hi);
+int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int* lo, int* hi,
+                  uint64_t* kmin = nullptr, uint64_t* kmax = nullptr);
 
 // Copy `bytes` (<= 512) of device memory to the host through pinned staging; synchronises.
 int read_back(const void* dev, size_t bytes, int64_t* host_out, hipStream_t s);

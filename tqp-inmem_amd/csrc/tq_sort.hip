@@ -122,20 +122,27 @@ template <typename K>
 __global__ __launch_bounds__(kBlock) void k_key_bits(const K* __restrict__ keys, int64_t n,
                                                      unsigned long long* __restrict__ stats) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  uint64_t o = 0, a = ~0ull;
+  uint64_t o = 0, a = ~0ull, lo = ~0ull, hi = 0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     const uint64_t k = (uint64_t)keys[i];
     o |= k;
     a &= k;
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
   }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     o |= __shfl_xor(o, d, 64);
     a &= __shfl_xor(a, d, 64);
+    const uint64_t l2 = __shfl_xor(lo, d, 64), h2 = __shfl_xor(hi, d, 64);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
   }
   if (lane_id() == 0) {
     atomicOr(&stats[0], (unsigned long long)o);
     atomicAnd(&stats[1], (unsigned long long)a);
+    atomicMin(&stats[2], (unsigned long long)lo);
+    atomicMax(&stats[3], (unsigned long long)hi);
   }
 }
 
@@ -194,18 +201,22 @@ int read_back(const void* dev, size_t bytes, int64_t* host_out, hipStream_t s) {
 }
 
 template <typename K>
-int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int* lo, int* hi) {
+int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int* lo, int* hi,
+                  uint64_t* kmin, uint64_t* kmax) {
   *lo = *hi = 0;
-  if (n <= 1) return TQ_OK;
+  if (n <= 0) return TQ_OK;
   unsigned long long* st = (unsigned long long*)stats_dev;
   TQ_TRY(fill_u64(st, 0ull, 1, s));
-  TQ_TRY(fill_u64(st + 1, ~0ull, 1, s));
+  TQ_TRY(fill_u64(st + 1, ~0ull, 2, s));
+  TQ_TRY(fill_u64(st + 3, 0ull, 1, s));
   int64_t g = ceil_div(n, kBlock);
   if (g > 2048) g = 2048;
   k_key_bits<K><<<(unsigned)g, kBlock, 0, s>>>(keys, n, st);
   TQ_TRY(check_launch("key bits"));
-  int64_t hv[2];
-  TQ_TRY(read_back(st, 16, hv, s));
+  int64_t hv[4];
+  TQ_TRY(read_back(st, 32, hv, s));
+  if (kmin) *kmin = (uint64_t)hv[2];
+  if (kmax) *kmax = (uint64_t)hv[3];
   const uint64_t diff = (uint64_t)hv[0] ^ (uint64_t)hv[1];
   if (diff == 0) return TQ_OK;
   const int low = __builtin_ctzll(diff), high = 64 - __builtin_clzll(diff);
@@ -214,8 +225,10 @@ int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int*
   return TQ_OK;
 }
 
-template int key_bit_range<uint32_t>(const uint32_t*, int64_t, void*, hipStream_t, int*, int*);
-template int key_bit_range<uint64_t>(const uint64_t*, int64_t, void*, hipStream_t, int*, int*);
+template int key_bit_range<uint32_t>(const uint32_t*, int64_t, void*, hipStream_t, int*, int*,
+                                     uint64_t*, uint64_t*);
+template int key_bit_range<uint64_t>(const uint64_t*, int64_t, void*, hipStream_t, int*, int*,
+                                     uint64_t*, uint64_t*);
 
 // ------------------------------------------------------------------ multi-key sort
 // A field is one typed scalar of a key (a 2-D key contributes one int8 field per column).

@@ -83,7 +83,8 @@ class SortKey(ctypes.Structure):
 class JoinState(ctypes.Structure):
     _fields_ = [("nsel", ctypes.c_int64), ("key_bits", ctypes.c_int32),
                 ("nsample", ctypes.c_int32), ("step", ctypes.c_int64),
-                ("keys", ctypes.c_void_p), ("rows", ctypes.c_void_p), ("sample", ctypes.c_void_p)]
+                ("keys", ctypes.c_void_p), ("rows", ctypes.c_void_p), ("sample", ctypes.c_void_p),
+                ("dir", ctypes.c_void_p), ("dir_base", ctypes.c_uint64), ("dir_range", ctypes.c_int64)]
 
 
 class GroupByDesc(ctypes.Structure):
@@ -126,7 +127,7 @@ _SIGNATURES = {
     "tq_groupby_count": ([P, I32, I64, P, P, P], I32),
     "tq_groupby_write": ([P, I32, I64, P, I64, P, P, P], I32),
     "tq_batch_ws_bytes": ([I32], SZ),
-    "tq_batch_run": ([P, I32, P, P, I32, I32, I32, P, ctypes.c_uint32, P], I32),
+    "tq_batch_run": ([P, I32, P, P, I32, I32, I32, I32, P, ctypes.c_uint32, P], I32),
 }
 
 

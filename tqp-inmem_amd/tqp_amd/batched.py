@@ -43,12 +43,21 @@ class BatchPipe(ctypes.Structure):
                 ("tile_counts", ctypes.c_void_p), ("key_out", ctypes.c_void_p),
                 ("agg_out", ctypes.c_void_p * L.TQ_MAX_SEGS),
                 ("ngroups", ctypes.c_void_p), ("flags", ctypes.c_void_p),
-                ("ntiles", ctypes.c_int32 * PHASES), ("pad", ctypes.c_int32)]
+                ("ntiles", ctypes.c_int32 * PHASES), ("mode", ctypes.c_int32),
+                ("nkeys", ctypes.c_int32), ("lds_table", ctypes.c_int32),
+                ("gkey_min", ctypes.c_int64 * L.TQ_GB_MAX_KEYS),
+                ("gkey_stride", ctypes.c_int64 * L.TQ_GB_MAX_KEYS),
+                ("gkey_range", ctypes.c_int64 * L.TQ_GB_MAX_KEYS),
+                ("gkey_dtype", ctypes.c_int32 * L.TQ_GB_MAX_KEYS),
+                ("gkey_out", ctypes.c_void_p * L.TQ_GB_MAX_KEYS)]
+
+LDS_TABLE_MAX = 32 * 1024
 
 
 class PipelineSpec:
-    """One batched pipeline: build/probe columns (id -> tensor), predicates, join keys,
-    aggregates [(output id, op, argument tree)] and the output ids of the group key."""
+    """One batched join pipeline: build/probe columns (id -> tensor), predicates, join keys,
+    aggregates [(output id, op, argument tree)] and the output id of the group key."""
+    mode = 0
 
     def __init__(self, build_cols, probe_cols, build_key, probe_key, build_pred, probe_pred,
                  aggs, key_out_id):
@@ -57,6 +66,16 @@ class PipelineSpec:
         self.build_pred, self.probe_pred = build_pred, probe_pred
         self.aggs = aggs
         self.key_out_id = key_out_id
+
+
+class GroupSpec:
+    """One batched group-by pipeline (no join): columns, predicate, key column ids (output ids
+    equal to the key ids, as the reference's key results are written back), aggregates."""
+    mode = 1
+
+    def __init__(self, cols, pred, keys, aggs, key_out_ids):
+        self.probe_cols, self.probe_pred = cols, pred
+        self.keys, self.aggs, self.key_out_ids = keys, aggs, key_out_ids
 
 
 class _Cols(dict):
@@ -111,6 +130,43 @@ def match_plan(plan, tensor_group):
     return PipelineSpec(bcols, pcols, bkey, pkey, fb['filter tree'], fp['filter tree'], aggs, key_out)
 
 
+def match_groupby_plan(plan, tensor_group):
+    """A GroupSpec for a (Filter, Aggregate over the filtered columns[, Sort by the group keys
+    ascending]) plan, or None."""
+    ops = [a for _, a in plan]
+    names = [a['name'] for a in ops]
+    if names[:2] != ['Filter', 'Aggregate'] or len(ops) > 3:
+        return None
+    f, ag = ops[0], ops[1]
+    mid = f['out mask id']
+    mm = ag.get('mask map', {})
+    if set(mm.values()) != {mid} or set(mm) != set(ag['input ids']) - {mid}:
+        return None
+    keys = [get_id(k) for k in ag['keys']]
+    key_out, aggs = [], []
+    for tree in ag['results']:
+        tree = _unwrap(tree)
+        if _is_key(tree):
+            key_out.append((get_id(tree if isinstance(tree, str) else tree[0]), get_id(right_most(tree))))
+            continue
+        body = _strip_alias(tree[0])
+        if not (isinstance(body, list) and len(body) == 2 and body[0] in E.AGG_FUNCS):
+            return None
+        aggs.append((get_id(right_most(tree)), body[0], body[1]))
+    if [k for k, _ in key_out] != keys:
+        return None
+    if len(ops) == 3:
+        so = ops[2]
+        if so['name'] != 'Sort' or so['key ids'] != [o for _, o in key_out] or any(so['key orders']):
+            return None
+    cols = {c: tensor_group[c].tensor for c in set(ag['input ids']) | set(f['materialized in'])
+            if c in tensor_group and c != mid}
+    for k in keys:
+        if cols[k].dtype.is_floating_point:
+            return None
+    return GroupSpec(cols, f['filter tree'], keys, aggs, [o for _, o in key_out])
+
+
 def _collect_ids(tree, out):
     if isinstance(tree, str):
         if '#' in tree and tree.split('#')[-1].replace('L', '').isdigit():
@@ -147,39 +203,77 @@ class BatchedEngine:
         self._prepare()
 
     # ---------------------------------------------------------------- set-up
+    def _bounds(self, t):
+        key = (t.data_ptr(), t.shape[0], t.dtype)
+        kb = self.key_bounds.get(key)
+        if kb is None:
+            kb = _key_bounds(t) if t.shape[0] > 0 else (0, -1)
+            self.key_bounds[key] = kb
+        return kb
+
+    def _plan_join(self, i, sp):
+        bk = sp.build_cols[sp.build_key].reshape(-1)
+        pk = sp.probe_cols[sp.probe_key].reshape(-1)
+        if bk.dtype.is_floating_point or pk.dtype.is_floating_point:
+            return None
+        nb, np_ = bk.shape[0], pk.shape[0]
+        lo, hi = self._bounds(bk)
+        rng = max(hi - lo + 1, 1)
+        if rng > 4 * nb + 65536:
+            return None
+        bb = E.ProgramBuilder()
+        bb.load_slot(bk)
+        E.compile_predicate(bb, sp.build_pred, _Cols(sp.build_cols))
+        pb = E.ProgramBuilder()
+        pb.load_slot(pk)
+        E.compile_predicate(pb, sp.probe_pred, _Cols(sp.probe_cols))
+        return dict(i=i, bb=bb, pb=pb, lo=lo, rng=rng, cap=min(nb, rng), nb=nb, np=np_,
+                    key_dtype=bk.dtype, gkeys=None)
+
+    def _plan_group(self, i, sp):
+        cols = [sp.probe_cols[k].reshape(-1) for k in sp.keys]
+        n = next(iter(sp.probe_cols.values())).shape[0]
+        mins, ranges = [], []
+        for c in cols:
+            lo, hi = self._bounds(c)
+            mins.append(lo)
+            ranges.append(max(hi - lo + 1, 1))
+        space = 1
+        for r in ranges:
+            space *= r
+        if space > min(1 << 27, 4 * n + 65536):
+            return None
+        strides = []
+        acc = 1
+        for r in reversed(ranges):
+            strides.append(acc)
+            acc *= r
+        strides.reverse()
+        pb = E.ProgramBuilder()
+        for c in cols:
+            pb.load_slot(c)
+        E.compile_predicate(pb, sp.probe_pred, _Cols(sp.probe_cols))
+        bb = E.ProgramBuilder()
+        bb.const(True, torch.bool)
+        bb.end_segment(E.Value("reg", reg=0, dtype=torch.bool))
+        bb.finish()
+        bb.column(cols[0] if cols else next(iter(sp.probe_cols.values())))
+        return dict(i=i, bb=bb, pb=pb, lo=0, rng=space, cap=max(1, min(n, space)), nb=0, np=n,
+                    key_dtype=torch.int64, gkeys=(cols, mins, ranges, strides))
+
     def _prepare(self):
         dev = self.device
         T = self.tile_rows
-        self.pipes = []
-        self.builders = []        # keep compiled programs (and their tensors) alive
         self.outputs = []
-        self.index = []           # spec index of each batched pipe
+        self.builders = []        # keep compiled programs (and their tensors) alive
         stage_loads = 1
-        arena_bytes = 0
+        lds_table = 0
         layout = []
         for i, sp in enumerate(self.specs):
-            bk = sp.build_cols[sp.build_key].reshape(-1)
-            pk = sp.probe_cols[sp.probe_key].reshape(-1)
-            if bk.dtype.is_floating_point or pk.dtype.is_floating_point:
+            lay = self._plan_join(i, sp) if sp.mode == 0 else self._plan_group(i, sp)
+            if lay is None:
                 self.rejected.append(i)
                 continue
-            nb, np_ = bk.shape[0], pk.shape[0]
-            bkey_id = (bk.data_ptr(), nb, bk.dtype)
-            kb = self.key_bounds.get(bkey_id)
-            if kb is None:
-                kb = _key_bounds(bk) if nb > 0 else (0, -1)
-                self.key_bounds[bkey_id] = kb
-            lo, hi = kb
-            rng = max(hi - lo + 1, 1)
-            if rng > 4 * nb + 65536:
-                self.rejected.append(i)
-                continue
-            bb = E.ProgramBuilder()
-            bb.load_slot(bk)
-            E.compile_predicate(bb, sp.build_pred, _Cols(sp.build_cols))
-            pb = E.ProgramBuilder()
-            pb.load_slot(pk)
-            E.compile_predicate(pb, sp.probe_pred, _Cols(sp.probe_cols))
             ab = E.ProgramBuilder()
             arg_dtypes = []
             for _, op, tree in sp.aggs:
@@ -190,43 +284,59 @@ class BatchedEngine:
                 ab.end_segment(v)
             ab.finish()
             if not ab.col_tensors:
-                ab.column(pk)
-            stage_loads = max(stage_loads, bb.nload, pb.nload, ab.nload)
-            cap = min(nb, rng)
-            layout.append((i, bb, pb, ab, arg_dtypes, lo, rng, cap, nb, np_))
+                ab.column(next(iter(sp.probe_cols.values())))
+            lay.update(ab=ab, arg_dtypes=arg_dtypes)
+            tbytes = lay["rng"] * (4 + 8 * len(sp.aggs))
+            lay["lds"] = 1 if tbytes <= LDS_TABLE_MAX else 0
+            if lay["lds"]:
+                lds_table = max(lds_table, tbytes)
+            stage_loads = max(stage_loads, lay["bb"].nload, lay["pb"].nload, ab.nload)
+            layout.append(lay)
         self.stage_loads = stage_loads
-        # one arena for all tables: dir + cnt (u32) and accumulators (8 B each)
-        offs = []
-        for (_, _, _, ab, _, _, rng, _, _, _) in layout:
+        self.lds_table_bytes = lds_table
+        offs, arena_bytes = [], 0
+        for lay in layout:
             offs.append(arena_bytes)
-            arena_bytes += _table_bytes(rng, ab.nseg, T)[-1]
+            arena_bytes += _table_bytes(lay["rng"], lay["ab"].nseg, T)[-1]
         self.arena = torch.empty(max(arena_bytes, 256), dtype=torch.uint8, device=dev)
         self.ngroups = torch.empty(max(len(layout), 1), dtype=torch.int64, device=dev)
         self.flags = torch.zeros(max(len(layout), 1), dtype=torch.int32, device=dev)
         base = self.arena.data_ptr()
         items = [[] for _ in range(PHASES)]
         arr = (BatchPipe * max(len(layout), 1))()
-        for k, ((i, bb, pb, ab, arg_dtypes, lo, rng, cap, nb, np_), off) in enumerate(zip(layout, offs)):
-            sp = self.specs[i]
+        ops = {'sum': L.TQ_AGG_SUM, 'count': L.TQ_AGG_COUNT, 'avg': L.TQ_AGG_AVG,
+               'min': L.TQ_AGG_MIN, 'max': L.TQ_AGG_MAX}
+        for k, (lay, off) in enumerate(zip(layout, offs)):
+            sp = self.specs[lay["i"]]
             d = arr[k]
-            d.bprog, d.bcols = bb.prog, bb.cols
-            d.pprog, d.pcols = pb.prog, pb.cols
-            d.aprog, d.acols = ab.prog, ab.cols
-            d.nb, d.np, d.kmin, d.range = nb, np_, lo, rng
+            d.bprog, d.bcols = lay["bb"].prog, lay["bb"].cols
+            d.pprog, d.pcols = lay["pb"].prog, lay["pb"].cols
+            d.aprog, d.acols = lay["ab"].prog, lay["ab"].cols
+            d.nb, d.np, d.kmin, d.range = lay["nb"], lay["np"], lay["lo"], lay["rng"]
             d.naggs = len(sp.aggs)
-            bk = sp.build_cols[sp.build_key]
-            d.key_dtype = L.dtype_code(bk.dtype)
-            ops = {'sum': L.TQ_AGG_SUM, 'count': L.TQ_AGG_COUNT, 'avg': L.TQ_AGG_AVG,
-                   'min': L.TQ_AGG_MIN, 'max': L.TQ_AGG_MAX}
-            cnt_off, acc_off, tc_off, _ = _table_bytes(rng, d.naggs, T)
+            d.mode = sp.mode
+            d.lds_table = lay["lds"]
+            d.key_dtype = L.dtype_code(lay["key_dtype"])
+            cnt_off, acc_off, tc_off, _ = _table_bytes(lay["rng"], d.naggs, T)
             p = base + off
-            d.dir = p
-            d.cnt = p + cnt_off
-            d.acc = p + acc_off
-            d.tile_counts = p + tc_off
-            key_out = torch.empty(cap, dtype=bk.dtype, device=dev)
+            d.dir, d.cnt, d.acc, d.tile_counts = p, p + cnt_off, p + acc_off, p + tc_off
+            cap = lay["cap"]
+            keys_out = []
+            if sp.mode == 0:
+                key_out = torch.empty(cap, dtype=lay["key_dtype"], device=dev)
+                d.key_out = key_out.data_ptr()
+                keys_out.append(key_out)
+            else:
+                cols, mins, ranges, strides = lay["gkeys"]
+                d.nkeys = len(cols)
+                for j, c in enumerate(cols):
+                    t = torch.empty(cap, dtype=c.dtype, device=dev)
+                    keys_out.append(t)
+                    d.gkey_out[j] = t.data_ptr()
+                    d.gkey_min[j], d.gkey_range[j], d.gkey_stride[j] = mins[j], ranges[j], strides[j]
+                    d.gkey_dtype[j] = L.dtype_code(c.dtype)
             agg_outs = []
-            for a, ((oid, op, _), dt) in enumerate(zip(sp.aggs, arg_dtypes)):
+            for a, ((oid, op, _), dt) in enumerate(zip(sp.aggs, lay["arg_dtypes"])):
                 d.agg_op[a] = ops[op]
                 d.agg_dtype[a] = L.dtype_code(dt)
                 if op == 'count':
@@ -240,18 +350,15 @@ class BatchedEngine:
                 t = torch.empty(cap, dtype=odt, device=dev)
                 agg_outs.append(t)
                 d.agg_out[a] = t.data_ptr()
-            d.key_out = key_out.data_ptr()
             d.ngroups = self.ngroups.data_ptr() + 8 * k
             d.flags = self.flags.data_ptr() + 4 * k
+            rng, nb, np_ = lay["rng"], lay["nb"], lay["np"]
             nt = [_ceil(rng, T), _ceil(nb, T), _ceil(np_, T), _ceil(rng, T), _ceil(rng, T)]
             for ph in range(PHASES):
                 d.ntiles[ph] = nt[ph]
-            self.pipes.append(k)
-            self.builders.append((bb, pb, ab))
-            self.outputs.append((i, key_out, agg_outs))
-            self.index.append(i)
-            for ph in range(PHASES):
                 items[ph].append((nt[ph], k))
+            self.builders.append((lay["bb"], lay["pb"], lay["ab"]))
+            self.outputs.append((lay["i"], keys_out, agg_outs))
         flat = []
         starts = [0]
         for ph in range(PHASES):
@@ -263,8 +370,7 @@ class BatchedEngine:
         self.nitems = len(flat)
         it = torch.tensor(flat if flat else [(0, 0)], dtype=torch.int32).reshape(-1)
         self.items = it.to(dev)
-        raw = bytes(arr)
-        self.desc = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev)
+        self.desc = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
         self._desc_host = arr
         self.ws = torch.empty(max(L.size_call("tq_batch_ws_bytes", len(layout)), 256),
                               dtype=torch.uint8, device=dev)
@@ -276,7 +382,7 @@ class BatchedEngine:
         if self.npipes == 0:
             return
         L.call("tq_batch_run", L.ptr(self.desc), self.npipes, L.ptr(self.items), self.phase_start,
-               self.tile_rows, self.grid, self.stage_loads, L.ptr(self.ws),
+               self.tile_rows, self.grid, self.stage_loads, self.lds_table_bytes, L.ptr(self.ws),
                ctypes.c_uint32(self.spin_limit), L.stream_handle())
 
     def results(self):
@@ -288,13 +394,14 @@ class BatchedEngine:
         ng = self.ngroups.cpu().tolist()
         fl = self.flags.cpu().tolist()
         out, rerun = {}, list(self.rejected)
-        for k, (i, key_out, agg_outs) in enumerate(self.outputs):
+        for k, (i, keys_out, agg_outs) in enumerate(self.outputs):
             if fl[k]:
                 rerun.append(i)
                 continue
             G = ng[k]
             sp = self.specs[i]
-            res = {sp.key_out_id: key_out[:G].reshape(-1, 1)}
+            ids = [sp.key_out_id] if sp.mode == 0 else sp.key_out_ids
+            res = {oid: t[:G].reshape(-1, 1) for oid, t in zip(ids, keys_out)}
             for (oid, _, _), t in zip(sp.aggs, agg_outs):
                 res[oid] = t[:G]
             out[i] = res
