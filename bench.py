@@ -29,8 +29,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-runs", type=int, default=3, help="pipelines timed for the CPU baseline")
+    ap.add_argument("--cpu-runs", type=int, default=6, help="pipelines timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the config-3 batched comparison")
+    ap.add_argument("--batch-pipes", type=int, default=256)
+    ap.add_argument("--serve", action="store_true", help="add the served-form sweep")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied once by a plain copy kernel (PMC traffic calibration)")
     return ap.parse_args()
 
 
@@ -74,6 +79,58 @@ def cpu_baseline(tables, plan, runs):
                       f"torch-CPU restatement of the reference operators, {dt:.1f} s"}
 
 
+def batched_section(pipes_n, dev):
+    """Config 3: the same pipelines as one persistent batched launch and as plain per-operator
+    launches; rows/s of both and whether every result matched."""
+    import numpy as np
+    import torch
+    from tqp_amd import batched, executor, workloads
+    probe, build, consts = workloads.config3_sizes(n_pipes=pipes_n)
+    pipes = [workloads.config3_pipeline(i, probe[i], build[i], consts[i][0], consts[i][1], dev)
+             for i in range(pipes_n)]
+    rows = int(probe.sum() + build.sum())
+    specs = [batched.match_plan(p, g) for g, p in pipes]
+    tune = batched.autotune(specs, key=f"{torch.cuda.get_device_name(0)}|config3|{pipes_n}")
+    eng = batched.BatchedEngine(specs, tile_rows=tune["tile_rows"], grid=tune["grid"])
+    eng.run()
+    out, rerun = eng.results()
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.run()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    # plain per-operator launches of the same pipelines
+    def plain_all():
+        res = []
+        for g, p in pipes:
+            tg = dict(g)
+            executor.execute(copy.deepcopy(p), tg)
+            res.append(tg)
+        return res
+    plain_all()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = plain_all()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    out, rerun = eng.results()
+    same = True
+    for i, tg in enumerate(ref):
+        if i in rerun:
+            continue
+        same &= torch.equal(out[i][101].cpu(), tg[101].tensor.cpu())
+        same &= torch.equal(out[i][302].cpu(), tg[302].tensor.cpu())
+        a, b = out[i][301].cpu().numpy(), tg[301].tensor.cpu().numpy()
+        same &= a.shape == b.shape and bool(np.all(np.abs(a - b) <= 1e-12 * np.abs(b)))
+    return {"pipelines": pipes_n, "input_rows": rows,
+            "batched_rows_per_s": rows / t_batch, "plain_rows_per_s": rows / t_plain,
+            "speedup": t_plain / t_batch, "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
+            "identical_results": bool(same), "rerun_plain": rerun,
+            "tile_rows": tune["tile_rows"], "grid": tune["grid"]}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -95,6 +152,11 @@ def main():
         executor.execute(copy.deepcopy(plan), tg)
         return tg
 
+    if args.calibrate_copy:
+        x = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device=dev)
+        y = x.clone()
+        torch.cuda.synchronize()
+        del x, y
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -133,6 +195,11 @@ def main():
                      "unit": "GB/s", "frac": d["frac"], "traffic": traffic},
         "kernels": kernels,
     }
+    if not args.no_batched:
+        line["batched"] = batched_section(args.batch_pipes, dev)
+    if args.serve:
+        from tqp_amd import serving_bench
+        line["served"] = serving_bench.sweep(dev)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(tables, plan, args.cpu_runs)
     print(json.dumps(line), flush=True)

@@ -21,6 +21,9 @@ class FakeBackend:
     def footprint(self, r):
         return self.fp * (10 if r.shape == 2 else 1)
 
+    def batchable(self, r):
+        return r.shape in (0, 1)
+
     def _start(self, reqs):
         self.live += sum(r.footprint for r in reqs)
         self.max_live_bytes = max(self.max_live_bytes, self.live)

@@ -385,9 +385,11 @@ class BatchedEngine:
                self.tile_rows, self.grid, self.stage_loads, self.lds_table_bytes, L.ptr(self.ws),
                ctypes.c_uint32(self.spin_limit), L.stream_handle())
 
-    def results(self):
-        """Synchronise; returns ({spec index: {out id: tensor}}, [spec indices to rerun])."""
-        torch.cuda.synchronize()
+    def results(self, sync=True):
+        """Returns ({spec index: {out id: tensor}}, [spec indices to rerun]).  With sync=False
+        the caller has already waited for the launch (e.g. on an event of its stream)."""
+        if sync:
+            torch.cuda.synchronize()
         err = int(self.ws[256:260].view(torch.int32).item()) if self.npipes else 0
         if err:
             raise RuntimeError("batched launch aborted: a phase wait exceeded its spin limit")

@@ -11,11 +11,11 @@ The base tables stay resident; a request only carries its constants.  The loop:
   of everything in flight stays within the HBM budget and an execution slot is free.  The estimate
   is `footprint_bytes`, an upper bound on masks, join outputs and group tables.  A request larger
   than the whole budget is rejected with MemoryError.
-* **Continuous batching.** Every pass it moves all admitted shape-1 requests into one persistent
-  batched launch (`batched.BatchedEngine`).  Requests that arrive while that launch runs are
+* **Continuous batching.** Every pass it moves all admitted shape-0 and shape-1 requests into one
+  persistent batched launch (`batched.BatchedEngine`: group-by pipelines and join pipelines).  Requests that arrive while that launch runs are
   admitted as soon as slots and memory free up, into the next launch on another stream.  Batches
   on different streams overlap on the device.
-* **Other shapes.** Shape-0 and shape-2 requests run through the per-operator path on a worker
+* **Other shapes.** Shape-2 requests (three joins) run through the per-operator path on a worker
   thread with its own stream, overlapping the batches.
 * **Results and latency.** Completion is detected with stream events, without blocking.
   Results are stored per request; latency is completion time minus arrival time.
@@ -77,8 +77,8 @@ def validate(req):
 
 class Server:
     """The admission / continuous-batching loop.  `backend` provides:
-    footprint(req) -> bytes, submit_batch(reqs) -> handle, submit_single(req) -> handle,
-    poll(handle) -> bool, collect(handle) -> {rid: result}."""
+    footprint(req) -> bytes, batchable(req) -> bool, submit_batch(reqs) -> handle,
+    submit_single(req) -> handle, poll(handle) -> bool, collect(handle) -> {rid: result}."""
 
     def __init__(self, backend, budget_bytes=200 * 10**9, max_inflight=4, max_batch=64):
         self.backend = backend
@@ -133,7 +133,7 @@ class Server:
                 queue.popleft()
                 used += r.footprint
                 r.admitted = clock() - t0
-                if r.shape == 1 and len(batch) < self.max_batch:
+                if self.backend.batchable(r) and len(batch) < self.max_batch:
                     batch.append(r)
                 else:
                     singles.append(r)
@@ -208,6 +208,9 @@ class GpuBackend:
         n = self.base[2][331].tensor.shape[0]
         return 3 * n * 32 + 16 * 100_000                     # three join outputs + group table
 
+    def batchable(self, req):
+        return req.shape in (0, 1)
+
     def submit_batch(self, reqs):
         from . import batched
         torch = self.torch
@@ -215,7 +218,10 @@ class GpuBackend:
         self._next += 1
         specs = []
         for r in reqs:
-            specs.append(batched.match_plan(self.plan(r), self.base[1]))
+            if r.shape == 1:
+                specs.append(batched.match_plan(self.plan(r), self.base[1]))
+            else:
+                specs.append(batched.match_groupby_plan(self.plan(r), self.base[0]))
         with torch.cuda.stream(stream):
             eng = batched.BatchedEngine(specs, key_bounds=self._bounds)
             eng.run()
@@ -254,13 +260,15 @@ class GpuBackend:
     def collect(self, h):
         if h[0] == "batch":
             eng, reqs = h[1], h[3]
-            out, rerun = eng.results()
+            out, rerun = eng.results(sync=False)
             res = {}
             for k, r in enumerate(reqs):
-                if k in out:
+                if k not in out:
+                    res[r.rid] = RuntimeError(f"request {r.rid} needs the per-operator path")
+                elif r.shape == 1:
                     res[r.rid] = {301: out[k][301], 101: out[k][101]}
                 else:
-                    res[r.rid] = RuntimeError(f"request {r.rid} needs the per-operator path")
+                    res[r.rid] = {c: out[k][c] for c in (6, 7, 22, 23, 24, 25, 26, 27, 28)}
             return res
         return h[3]
 
