@@ -92,12 +92,16 @@ __device__ __forceinline__ void load_sample(const JoinView& J, K* ssample) {
 }
 
 // ------------------------------------------------------------------ build
+// Also reduces OR, AND, min and max of the packed keys into stats[0..3], so the host learns the
+// sort's bit range and the directory's span together with the selected-row count.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_join_pack(const void* __restrict__ bkeys, int dt,
                                                       const uint8_t* __restrict__ mask, int64_t n,
                                                       const int64_t* __restrict__ tile_off,
                                                       K* __restrict__ keys,
-                                                      uint32_t* __restrict__ rows) {
+                                                      uint32_t* __restrict__ rows,
+                                                      unsigned long long* __restrict__ stats) {
+  uint64_t so = 0, sa = ~0ull, slo = ~0ull, shi = 0;
   __shared__ uint32_t wcnt[kItems][kWaves];
   const int t = threadIdx.x, w = t >> 6, l = lane_id();
   const int64_t tile0 = (int64_t)blockIdx.x * kTileRows;
@@ -125,11 +129,30 @@ __global__ __launch_bounds__(kBlock) void k_join_pack(const void* __restrict__ b
       const int64_t r = tile0 + (int64_t)j * kBlock + t;
       const int64_t pos = run + before + rank_below(bal[j]);
       const int64_t v = load_int(bkeys, dt, r);
-      keys[pos] = sizeof(K) == 4 ? (K)((uint32_t)(int32_t)v ^ 0x80000000u)
+      const K k = sizeof(K) == 4 ? (K)((uint32_t)(int32_t)v ^ 0x80000000u)
                                  : (K)((uint64_t)v ^ 0x8000000000000000ull);
+      keys[pos] = k;
       rows[pos] = (uint32_t)r;
+      so |= (uint64_t)k;
+      sa &= (uint64_t)k;
+      slo = (uint64_t)k < slo ? (uint64_t)k : slo;
+      shi = (uint64_t)k > shi ? (uint64_t)k : shi;
     }
     run += tot;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    so |= __shfl_xor(so, d, 64);
+    sa &= __shfl_xor(sa, d, 64);
+    const uint64_t a2 = __shfl_xor(slo, d, 64), b2 = __shfl_xor(shi, d, 64);
+    slo = a2 < slo ? a2 : slo;
+    shi = b2 > shi ? b2 : shi;
+  }
+  if (l == 0 && shi >= slo) {
+    atomicOr(&stats[0], (unsigned long long)so);
+    atomicAnd(&stats[1], (unsigned long long)sa);
+    atomicMin(&stats[2], (unsigned long long)slo);
+    atomicMax(&stats[3], (unsigned long long)shi);
   }
 }
 
@@ -269,13 +292,13 @@ static BuildWs carve_build(void* base, int64_t nb) {
   BuildWs w;
   w.tile_counts = (uint32_t*)p; p += align_up((size_t)nt * 4, 256);
   w.tile_off = (int64_t*)p; p += align_up((size_t)nt * 8, 256);
-  w.total = (int64_t*)p; p += 256;
+  w.total = (int64_t*)p;          // total and the 4 key statistics are read back together
+  w.stats = p + 8; p += 256;
   w.scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(nt);
   w.k0 = p; p += align_up((size_t)nb * 8, 256);
   w.k1 = p; p += align_up((size_t)nb * 8, 256);
   w.v0 = (uint32_t*)p; p += align_up((size_t)nb * 4, 256);
   w.v1 = (uint32_t*)p; p += align_up((size_t)nb * 4, 256);
-  w.stats = p; p += 256;
   w.radix = p; p += radix_ws_bytes(nb);
   w.sample = p; p += kSampleBytes;
   w.dir = (uint32_t*)p; p += align_up((size_t)dir_cap(nb) * 4, 256);
@@ -331,12 +354,20 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
     TQ_TRY(tq_mask_tile_counts(bmask, nb, w.tile_counts, s));
     TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
   }
-  k_join_pack<K><<<(unsigned)nt, kBlock, 0, s>>>(bkeys, dt, bmask, nb, w.tile_off, (K*)w.k0, w.v0);
+  unsigned long long* kst = (unsigned long long*)w.stats;
+  TQ_TRY(fill_u64(kst, 0ull, 1, s));
+  TQ_TRY(fill_u64(kst + 1, ~0ull, 2, s));
+  TQ_TRY(fill_u64(kst + 3, 0ull, 1, s));
+  k_join_pack<K><<<(unsigned)nt, kBlock, 0, s>>>(bkeys, dt, bmask, nb, w.tile_off, (K*)w.k0, w.v0,
+                                                kst);
   TQ_TRY(check_launch("join pack"));
-  if (bmask) TQ_TRY(read_back(w.total, 8, &nsel, s));
-  int lo, hi;
-  uint64_t kmin = 0, kmax = 0;
-  TQ_TRY(key_bit_range<K>((const K*)w.k0, nsel, w.stats, s, &lo, &hi, &kmin, &kmax));
+  int64_t hv[5];
+  if (!bmask) TQ_TRY(fill_u64(w.total, (uint64_t)nb, 1, s));
+  TQ_TRY(read_back(w.total, 40, hv, s));   // nsel, OR, AND, min, max in one synchronisation
+  nsel = hv[0];
+  const uint64_t kmin = (uint64_t)hv[3], kmax = (uint64_t)hv[4];
+  int lo = 0, hi = 0;
+  radix_bit_range((uint64_t)hv[1], (uint64_t)hv[2], &lo, &hi);
   bool alt;
   TQ_TRY(radix_sort_pairs<K>((K*)w.k0, w.v0, (K*)w.k1, w.v1, nsel, lo, hi, w.radix, s, &alt));
   const K* keys = (const K*)(alt ? w.k1 : w.k0);

@@ -23,6 +23,9 @@ template <typename K>
 int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int* lo, int* hi,
                   uint64_t* kmin = nullptr, uint64_t* kmax = nullptr);
 
+// Digit-aligned bit range [lo, hi) in which keys with the given OR and AND differ.
+void radix_bit_range(uint64_t key_or, uint64_t key_and, int* lo, int* hi);
+
 // Copy `bytes` (<= 512) of device memory to the host through pinned staging; synchronises.
 int read_back(const void* dev, size_t bytes, int64_t* host_out, hipStream_t s);
 

@@ -217,12 +217,17 @@ int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int*
   TQ_TRY(read_back(st, 32, hv, s));
   if (kmin) *kmin = (uint64_t)hv[2];
   if (kmax) *kmax = (uint64_t)hv[3];
-  const uint64_t diff = (uint64_t)hv[0] ^ (uint64_t)hv[1];
-  if (diff == 0) return TQ_OK;
+  radix_bit_range((uint64_t)hv[0], (uint64_t)hv[1], lo, hi);
+  return TQ_OK;
+}
+
+void radix_bit_range(uint64_t key_or, uint64_t key_and, int* lo, int* hi) {
+  const uint64_t diff = key_or ^ key_and;
+  *lo = *hi = 0;
+  if (diff == 0) return;
   const int low = __builtin_ctzll(diff), high = 64 - __builtin_clzll(diff);
   *lo = (low / 8) * 8;
   *hi = high;
-  return TQ_OK;
 }
 
 template int key_bit_range<uint32_t>(const uint32_t*, int64_t, void*, hipStream_t, int*, int*,
@@ -274,6 +279,56 @@ __global__ __launch_bounds__(kBlock) void k_pack_keys(FieldPack fp, const uint32
     for (int f = 0; f < fp.n; ++f) k |= field_bits(fp.f[f], r);
     keys[i] = (K)k;
     vals[i] = r;
+  }
+}
+
+// ------------------------------------------------------------------ top-k
+// One tile of up to 4096 (key, row) candidates is bitonic-sorted in LDS by key, then row (so ties
+// keep input order); the first k go to the output at tile * k.  Rounds of this kernel shrink any
+// candidate set to k without a host synchronisation.
+constexpr int kTopTile = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_topk_tile(const uint64_t* __restrict__ kin,
+                                                      const uint32_t* __restrict__ vin, int64_t n,
+                                                      int k, uint64_t* __restrict__ kout,
+                                                      uint32_t* __restrict__ vout) {
+  __shared__ uint64_t sk[kTopTile];
+  __shared__ uint32_t sv[kTopTile];
+  const int64_t base = (int64_t)blockIdx.x * kTopTile;
+  const int64_t rem = n - base;
+  const int cnt = rem < kTopTile ? (int)rem : kTopTile;
+  for (int i = threadIdx.x; i < kTopTile; i += kBlock) {
+    if (i < cnt) {
+      sk[i] = kin[base + i];
+      sv[i] = vin ? vin[base + i] : (uint32_t)(base + i);
+    } else {
+      sk[i] = ~0ull;
+      sv[i] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+  for (int size = 2; size <= kTopTile; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < kTopTile / 2; i += kBlock) {
+        const int lo = 2 * i - (i & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t a = sk[lo], b = sk[hi];
+        const uint32_t va = sv[lo], vb = sv[hi];
+        const bool greater = a > b || (a == b && va > vb);
+        if (greater == up) {
+          sk[lo] = b; sk[hi] = a;
+          sv[lo] = vb; sv[hi] = va;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int m = cnt < k ? cnt : k;
+  for (int i = threadIdx.x; i < k; i += kBlock) {
+    const int64_t o = (int64_t)blockIdx.x * k + i;
+    kout[o] = i < m ? sk[i] : ~0ull;
+    vout[o] = i < m ? sv[i] : 0xFFFFFFFFu;
   }
 }
 
@@ -408,11 +463,59 @@ using namespace tq;
 
 extern "C" size_t tq_sort_ws_bytes(int64_t n) { return sort_ws_bytes(n); }
 
+// Top-k for a limit of at most 256 over keys that pack into one 64-bit sort key: pack, then
+// rounds of per-tile bitonic selection; no host synchronisation.
+static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t limit,
+                     int64_t* perm_out, void* ws, size_t ws_bytes, hipStream_t s, bool* done) {
+  *done = false;
+  FieldPack fp{};
+  int bits = 0;
+  for (int k = 0; k < nkeys; ++k) {
+    if (keys[k].width > 1) return TQ_OK;
+    bits += bits_of(keys[k].dtype);
+  }
+  if (bits > 64 || nkeys > 16 || nkeys < 1) return TQ_OK;
+  if (ws_bytes < sort_ws_bytes(n)) return set_error(TQ_ECAPACITY, "sort workspace too small");
+  int shift = 0;
+  for (int k = nkeys - 1; k >= 0; --k) {
+    fp.f[k] = Field{keys[k].ptr, keys[k].dtype, 1, 0, keys[k].descending ? 1 : 0,
+                    bits_of(keys[k].dtype), shift};
+    shift += bits_of(keys[k].dtype);
+  }
+  fp.n = nkeys;
+  SortWs w = carve_sort_ws(ws, n);
+  k_pack_keys<uint64_t><<<stream_grid(n), kBlock, 0, s>>>(fp, nullptr, n, w.k0, w.v0);
+  TQ_TRY(check_launch("top-k pack"));
+  const int k = (int)limit;
+  uint64_t *ki = w.k0, *ko = w.k1;
+  uint32_t *vi = w.v0, *vo = w.v1;
+  int64_t m = n;
+  while (true) {
+    const int64_t tiles = ceil_div(m, kTopTile);
+    k_topk_tile<<<(unsigned)tiles, kBlock, 0, s>>>(ki, vi, m, k, ko, vo);
+    TQ_TRY(check_launch("top-k tile"));
+    m = tiles * k;
+    uint64_t* tk = ki; ki = ko; ko = tk;
+    uint32_t* tv = vi; vi = vo; vo = tv;
+    if (tiles == 1) break;
+  }
+  const int64_t out = limit < n ? limit : n;
+  k_perm_out<<<stream_grid(out), kBlock, 0, s>>>(vi, out, perm_out);
+  TQ_TRY(check_launch("top-k out"));
+  *done = true;
+  return TQ_OK;
+}
+
 extern "C" int tq_sort_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t limit,
                             int64_t* perm_out, void* ws, size_t ws_bytes, void* stream) {
   if (n <= 0) return TQ_OK;
   if (nkeys < 0 || (nkeys > 0 && !keys)) return set_error(TQ_EINVAL, "sort: bad key list");
   hipStream_t s = (hipStream_t)stream;
+  if (limit >= 0 && limit <= 256 && limit < n && n > kTopTile) {
+    bool done;
+    TQ_TRY(topk_perm(keys, nkeys, n, limit, perm_out, ws, ws_bytes, s, &done));
+    if (done) return TQ_OK;
+  }
   uint32_t* perm;
   TQ_TRY(sort_perm_u32(keys, nkeys, n, ws, ws_bytes, s, &perm));
   const int64_t m = (limit >= 0 && limit < n) ? limit : n;
