@@ -197,9 +197,15 @@ def _agg_case(device, keys, vals, mask, results, key_ids, agg_ids):
         args['mask map'] = {i: 900 for i in input_ids}
     gpu = _gpu_group(tables, device)
     cpu = R.to_cols(tables)
-    aggregate.tqp_hash_aggregate(True, gpu, copy.deepcopy(args), None, None)
-    R.tqp_hash_aggregate(cpu, copy.deepcopy(args))
-    return gpu, cpu
+    errs = []
+    for run, grp in ((lambda g: aggregate.tqp_hash_aggregate(True, g, copy.deepcopy(args), None, None), gpu),
+                     (lambda g: R.tqp_hash_aggregate(g, copy.deepcopy(args)), cpu)):
+        try:
+            run(grp)
+            errs.append(None)
+        except Exception as e:   # the reference's own failure on degenerate inputs
+            errs.append(type(e))
+    return gpu, cpu, errs
 
 
 AGG_RESULTS = lambda: [  # noqa: E731
@@ -214,7 +220,7 @@ AGG_RESULTS = lambda: [  # noqa: E731
 
 
 @pytest.mark.parametrize("case", ["few_groups", "dense_global", "hash_sparse", "skew_front",
-                                  "no_keys", "all_masked"])
+                                  "no_keys", "all_masked", "all_masked_keyed", "all_masked_no_avg"])
 def test_aggregate_parity(device, case):
     g = np.random.default_rng(sum(map(ord, case)) + 1)
     n = 400_000
@@ -225,18 +231,29 @@ def test_aggregate_parity(device, case):
     elif case == "hash_sparse":
         k = {10: ('int', g.integers(0, 1 << 45, 50_000)[g.integers(0, 50_000, n)].astype(np.int64)),
              11: ('int', g.integers(-3, 3, n, dtype=np.int32))}
-    elif case == "skew_front":
+    elif case in ("skew_front", "all_masked_keyed"):
         hot = np.where(g.random(n) < 0.6, 5, g.integers(0, 2000, n))
         k = {10: ('int', hot.astype(np.int32)), 11: ('int', g.integers(0, 100, n, dtype=np.int32))}
     else:
         k = {}
     vals = {20: ('float', g.uniform(900, 105000, n)), 21: ('float', g.integers(0, 11, n) / 100.0),
             22: ('int', g.integers(-1000, 1000, n, dtype=np.int64))}
-    mask = None if case in ("dense_global",) else (g.random(n) < (0.0 if case == "all_masked" else 0.9))
+    masked = case.startswith("all_masked")
+    mask = None if case in ("dense_global",) else (g.random(n) < (0.0 if masked else 0.9))
     key_ids = sorted(k)
-    results = [f"k#{i}" for i in key_ids] + AGG_RESULTS()
+    aggs = AGG_RESULTS()
     agg_ids = [50, 51, 52, 53, 54, 55, 56]
-    gpu, cpu = _agg_case(device, k, vals, mask, results, key_ids, agg_ids)
+    if case == "all_masked_no_avg":
+        aggs = [a for a in aggs if a[0][0][0] != 'avg']
+        agg_ids = [50, 51, 53, 54, 55, 56]
+    results = [f"k#{i}" for i in key_ids] + aggs
+    gpu, cpu, errs = _agg_case(device, k, vals, mask, results, key_ids, agg_ids)
+    # zero selected rows: the reference divides by the group count (keyed) or divides an integer
+    # bincount in place (avg); both must fail the same way, and what it does compute must match
+    expect = {"all_masked": RuntimeError, "all_masked_keyed": ZeroDivisionError}.get(case)
+    assert errs == [expect, expect], (case, errs)
+    if expect is not None:
+        return
     for i in key_ids:
         _assert_col(gpu[i].tensor, cpu[i].tensor, f"{case} key {i}")
     for a in agg_ids:

@@ -219,6 +219,40 @@ def test_aggregate_max_initialised_to_zero():
     assert tg[5].tensor.tolist() == [0, 4]
 
 
+def test_aggregate_zero_selected_rows():
+    # every row masked out: keyed -> 0 groups and cnt_rows / cnt_groups fails (aggregate.py:128);
+    # keyless -> one group over an empty index: torch.bincount of an empty index is int64
+    # whatever the weights, so sum/count are int64 [0], avg's in-place divide fails
+    # (aggregate.py:175-177), min is the dtype's maximum and max is 0 (aggregate.py:154-166)
+    n = 50
+    k = torch.arange(n, dtype=torch.int32) % 3
+    v = torch.linspace(1.0, 2.0, n, dtype=torch.float64)
+    x = torch.arange(n, dtype=torch.int64) - 7
+
+    def group():
+        return {1: R.Col(k.clone(), 'int'), 2: R.Col(v.clone(), 'float'),
+                3: R.Col(x.clone(), 'int'), 9: R.Col(torch.zeros(n, dtype=torch.bool), 'bool')}
+
+    def args(keys, results, out):
+        a = _agg_args(keys, results, out, [1, 2, 3])
+        a['mask map'] = {1: 9, 2: 9, 3: 9}
+        return a
+
+    s = [[['sum', ['v#2']], '#', '1'], 'AS', 's#5']
+    with pytest.raises(ZeroDivisionError):
+        R.tqp_hash_aggregate(group(), args(['k#1'], ['k#1', s], [5]))
+    with pytest.raises(RuntimeError):
+        R.tqp_hash_aggregate(group(), args([], [[[['avg', ['v#2']], '#', '1'], 'AS', 'a#5']], [5]))
+    tg = group()
+    R.tqp_hash_aggregate(tg, args([], [s, [[['count', ['1']], '#', '2'], 'AS', 'c#6'],
+                                       [[['min', ['x#3']], '#', '3'], 'AS', 'n#7'],
+                                       [[['max', ['v#2']], '#', '4'], 'AS', 'm#8']], [5, 6, 7, 8]))
+    assert tg[5].tensor.dtype == torch.int64 and tg[5].tensor.tolist() == [0]
+    assert tg[6].tensor.tolist() == [0]
+    assert tg[7].tensor.tolist() == [torch.iinfo(torch.int64).max]
+    assert tg[8].tensor.dtype == torch.float64 and tg[8].tensor.tolist() == [0.0]
+
+
 # ------------------------------------------------------------------ sort
 @pytest.mark.parametrize("seed", range(5))
 def test_sort_matches_lexicographic_stable(seed):

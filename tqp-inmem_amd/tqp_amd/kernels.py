@@ -269,6 +269,7 @@ class GroupBy:
         L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), s)
         hb = bounds.cpu().tolist()
         nsel = hb[-1]
+        self.nsel = nsel
         for i in range(len(self.keys)):
             lo, hi = hb[2 * i], hb[2 * i + 1]
             d.key_min[i] = lo if nsel > 0 else 0

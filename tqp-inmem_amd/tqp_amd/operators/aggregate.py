@@ -68,6 +68,34 @@ def _strip_alias(tree):
     return tree
 
 
+def _zero_row_aggregates(keys, calls, gb):
+    """Aggregates over an input whose rows are all masked out, as the reference computes them.
+
+    With grouping keys there are no groups, and the reference fails on `cnt_rows / cnt_groups`
+    (aggregate.py:128) with ZeroDivisionError.  Without keys there is one group: `avg` fails
+    when it divides the integer bincount of an empty index in place (aggregate.py:175-177);
+    `sum` and `count` give the int64 bincount of an empty index, [0], for any argument dtype
+    (aggregate.py:190); `min` gives the dtype's maximum and `max` gives 0 (aggregate.py:154-166).
+    """
+    if keys:
+        raise ZeroDivisionError("aggregate with grouping keys over zero selected rows "
+                                "(the group count is zero)")
+    if any(c[0] == 'avg' for c in calls):
+        raise RuntimeError("avg over zero selected rows: the integer row count cannot "
+                           "receive the quotient in place")
+    dev = gb.builder.col_tensors[0].device
+    out = []
+    for c, dt in zip(calls, gb.arg_dtypes):
+        if c[0] in ('sum', 'count'):
+            out.append(torch.zeros(1, dtype=torch.int64, device=dev))
+        elif c[0] == 'min':
+            mx = torch.finfo(dt).max if dt.is_floating_point else torch.iinfo(dt).max
+            out.append(torch.full((1,), mx, dtype=dt, device=dev))
+        else:
+            out.append(torch.zeros(1, dtype=dt, device=dev))
+    return out
+
+
 def agg_group(columns, key_ids, results, mask=None):
     """Group `columns` by `key_ids` and evaluate each result tree (aggregate.py:282-364).
     Returns {output id: tensor}."""
@@ -89,6 +117,8 @@ def agg_group(columns, key_ids, results, mask=None):
             plans.append((out_id, 'agg', body))
     gb = kernels.GroupBy(keys, [(c[0], c[1]) for c in calls], columns, mask=mask, n=n)
     key_out, agg_out = gb.run()
+    if gb.nsel == 0 and calls:
+        agg_out = _zero_row_aggregates(keys, calls, gb)
     by_call = {id(c): t for c, t in zip(calls, agg_out)}
     out = {}
     for out_id, kind, what in plans:
