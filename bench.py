@@ -34,28 +34,39 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the config-3 batched comparison")
     ap.add_argument("--batch-pipes", type=int, default=256)
     ap.add_argument("--serve", action="store_true", help="add the served-form sweep")
+    ap.add_argument("--op-sync", action="store_true",
+                    help="synchronise after every operator, as the reference executor does")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once by a plain copy kernel (PMC traffic calibration)")
     return ap.parse_args()
 
 
-def kernel_table(trace):
-    """Per entry point: launches, mean duration, mean algorithmic bytes, GB/s."""
+def kernel_table(calls, launches):
+    """Per kernel: launches and mean device duration (dispatch-stamped start/stop events); for
+    the principal kernel of every entry point that states its algorithmic bytes, bytes per
+    launch and achieved GB/s against the HBM peak."""
     agg = {}
-    for name, nbytes, e0, e1 in trace:
-        ms = e0.elapsed_time(e1)
-        a = agg.setdefault(name, {"n": 0, "ms": 0.0, "bytes": 0, "has_bytes": nbytes is not None})
+    for name, ms in launches:
+        a = agg.setdefault(name, {"n": 0, "ms": 0.0, "bn": 0, "bms": 0.0, "bytes": 0})
         a["n"] += 1
         a["ms"] += ms
-        a["bytes"] += nbytes or 0
+    for _entry, kern, nbytes, c0, c1 in calls:
+        if nbytes is None or kern is None:
+            continue
+        hits = [i for i in range(c0, c1) if launches[i][0] == kern]
+        if not hits:
+            continue
+        a = agg[kern]
+        a["bn"] += len(hits)
+        a["bms"] += sum(launches[i][1] for i in hits)
+        a["bytes"] += nbytes
     out = {}
-    for name, a in agg.items():
-        avg_ms = a["ms"] / a["n"]
-        row = {"launches": a["n"], "avg_us": round(avg_ms * 1e3, 2), "total_ms": round(a["ms"], 3)}
-        if a["has_bytes"] and avg_ms > 0:
-            avg_b = a["bytes"] / a["n"]
-            gbs = avg_b / (avg_ms * 1e-3) / 1e9
-            row.update({"alg_bytes": int(avg_b), "GBps": round(gbs, 1),
+    for name, a in sorted(agg.items(), key=lambda kv: -kv[1]["ms"]):
+        row = {"launches": a["n"], "avg_us": round(a["ms"] / a["n"] * 1e3, 2),
+               "total_ms": round(a["ms"], 3)}
+        if a["bn"] and a["bms"] > 0:
+            gbs = a["bytes"] / (a["bms"] * 1e-3) / 1e9
+            row.update({"alg_bytes_per_launch": int(a["bytes"] / a["bn"]), "GBps": round(gbs, 1),
                         "frac": round(gbs / HBM_PEAK_GBS, 4)})
         out[name] = row
     return out
@@ -147,9 +158,9 @@ def main():
     base = workloads.to_tensor_group(tables, dev)
     rows = workloads.input_rows(tables, 1)
 
-    def step():
+    def step(op_sync=args.op_sync):
         tg = dict(base)
-        executor.execute(copy.deepcopy(plan), tg)
+        executor.execute(copy.deepcopy(plan), tg, sync=op_sync)
         return tg
 
     if args.calibrate_copy:
@@ -160,17 +171,28 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    _lib.TRACE = []
+    # timed region: every kernel launch carries device-stamped start/stop events
+    _lib.start_trace()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    trace, _lib.TRACE = _lib.TRACE, None
-    kernels = kernel_table(trace)
+    calls, launches = _lib.stop_trace()
+    kernels = kernel_table(calls, launches)
     with_bytes = {k: v for k, v in kernels.items() if "GBps" in v}
     dom = max(with_bytes, key=lambda k: with_bytes[k]["total_ms"])
     d = with_bytes[dom]
+    # the same steps without kernel timing, and with the reference executor's per-operator
+    # synchronisation (main.py:273-274), for comparison
+    side = {}
+    for label, sync in (("untimed", args.op_sync), ("op_sync", True)):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step(sync)
+        torch.cuda.synchronize()
+        side[label] = round((time.perf_counter() - t1) / args.steps * 1e3, 4)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
@@ -190,9 +212,13 @@ def main():
         "data": "synthetic (seeded PCG64 tables of the config-1 shape)",
         "config": {"workload": "filter -> sort-based equi-join -> group-by sum -> top-10; build "
                                "1,500,000 rows, probe 6,000,000 rows, ~264k groups",
-                   "input_rows": rows, "form": "plain per-operator launches"},
+                   "input_rows": rows, "form": "plain per-operator launches",
+                   "op_sync": bool(args.op_sync)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": d["frac"], "traffic": traffic},
+                     "unit": "GB/s", "frac": d["frac"], "traffic": traffic,
+                     "alg_bytes_per_launch": d["alg_bytes_per_launch"], "avg_us": d["avg_us"]},
+        "ms_per_step_untimed": side["untimed"],
+        "ms_per_step_op_sync": side["op_sync"],
         "kernels": kernels,
     }
     if not args.no_batched:

@@ -122,6 +122,24 @@ const char* tq_last_error(void);
 /* Rows covered by one filter/compaction tile (the unit of tile_counts / tile_offsets). */
 int tq_tile_rows(void);
 
+/* Kernel timing.  The reference times operators with torch.cuda.synchronize() and wall clock
+ * around each operator (main.py:273-277); these calls give the per-kernel durations behind such
+ * a time.  While on, every kernel launch of this library carries a start and a stop event that
+ * are stamped when that kernel starts and ends on the device (hipExtLaunchKernel), so host gaps
+ * between launches are excluded.  tq_timing_count returns the number of launches logged so far;
+ * after the caller has synchronised the streams involved, tq_timing_read copies up to cap
+ * records in launch order (name is a static string naming the kernel) and clears the log; at
+ * most 65536 launches are logged between reads, later ones run untimed and are counted in
+ * *dropped.  Not for launches under graph capture. */
+typedef struct {
+  const char* name;
+  float ms;
+  int32_t pad;
+} tq_ktime_t;
+int tq_timing(int32_t on);
+int64_t tq_timing_count(void);
+int tq_timing_read(tq_ktime_t* out, int64_t cap, int64_t* n, int64_t* dropped);
+
 /* ---------------------------------------------------------------- filter + compaction */
 
 /* Evaluate a predicate program over n rows into a bool mask (1 byte/row) and, if tile_counts is
@@ -194,32 +212,39 @@ int tq_sort_perm(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n, 
 
 /* ---------------------------------------------------------------- equi-join */
 /* Host-side handle of a built join index (filled by tq_join_build, read by the probe passes).
- * Build keys are stored order-transformed (sign bit flipped) and sorted, with their build row
- * ids in the same order; ties keep build row order, so every bucket lists rows ascending. */
+ * Keys are order-transformed (sign bit flipped; 32-bit for build keys of up to 32 bits, else
+ * 64-bit).  The index maps a key to its build rows in one of two layouts:
+ *   TQ_JOIN_DENSE: table[key - base] for range consecutive key values;
+ *   TQ_JOIN_HASH:  2^hash_bits slots {u64 key, u32 a, u32 count} of linear probing from
+ *                  (key * 0x9E3779B97F4A7C15) >> (64 - hash_bits); key == empty_key is free.
+ * With unique == 1 every selected key occurs once and an entry holds the build row itself
+ * (dense: u32 row, 0xFFFFFFFF = absent).  Otherwise rows[] lists the selected build rows
+ * grouped by key, ascending within a key, and an entry holds {first position, count} (dense:
+ * two u32, count 0 = absent). */
+#define TQ_JOIN_EMPTY 0
+#define TQ_JOIN_DENSE 1
+#define TQ_JOIN_HASH 2
 typedef struct {
-  int64_t nsel;         /* selected build rows = length of keys/rows */
-  int32_t key_bits;     /* 32 or 64: width of the stored keys */
-  int32_t nsample;      /* entries in the LDS search sample */
-  int64_t step;         /* sample stride: sample[s] = keys[min((s+1)*step, nsel) - 1] */
-  const void* keys;     /* device: sorted transformed keys */
-  const uint32_t* rows; /* device: build row ids */
-  const void* sample;   /* device: search sample */
-  /* dense directory, present (dir != NULL) when the selected keys span at most
-   * 4 * nsel + 65536 values: dir[t - dir_base] = first position of transformed key t, or
-   * 0xFFFFFFFF; a probe is then one load instead of a search */
-  const uint32_t* dir;
-  uint64_t dir_base;
-  int64_t dir_range;
+  int64_t nsel;          /* selected build rows */
+  int32_t key_bits;      /* 32 or 64: width of the transformed keys */
+  int32_t layout;        /* TQ_JOIN_EMPTY / TQ_JOIN_DENSE / TQ_JOIN_HASH */
+  int32_t unique;        /* 1: entries hold build rows; 0: entries index rows[] */
+  int32_t hash_bits;     /* hash layout: log2 of the slot count */
+  uint64_t base;         /* dense layout: transformed key of table[0] */
+  uint64_t empty_key;    /* hash layout: key value of a free slot */
+  int64_t range;         /* dense: table entries; hash: slots */
+  const void* table;     /* device directory or slots */
+  const uint32_t* rows;  /* device: build rows grouped by key (unique == 0) */
 } tq_join_state_t;
 
-/* Workspace for building over nb build rows (room for the dense directory included). */
+/* Workspace for building over nb build rows (room for either layout and for sorting). */
 size_t tq_join_build_ws_bytes(int64_t nb);
 
-/* Build: sort the keys of the selected build rows (bmask may be NULL) with their row ids.
- * Keys are integer columns (TQ_BOOL..TQ_I64).  Synchronises the stream (the row count and key
- * range decide the sort passes) and fills *state; ws must outlive the probe passes.
- * Replaces operators/hashjoin.py:41-52 (arange, torch.sort, unique_consecutive, cumsum) and the
- * build-side mask compaction of IO/vortex_pipeline.py:58. */
+/* Build the index over the selected build rows (bmask may be NULL).  Keys are integer columns
+ * (TQ_BOOL..TQ_I64).  Synchronises the stream twice (the row count and key range pick the
+ * layout; a flag tells whether the keys were unique) and fills *state; ws must outlive the
+ * probe passes.  Replaces operators/hashjoin.py:41-52 (arange, torch.sort, unique_consecutive,
+ * cumsum) and the build-side mask compaction of IO/vortex_pipeline.py:58. */
 int tq_join_build(const void* bkeys, int32_t bdtype, const uint8_t* bmask, int64_t nb, void* ws,
                   size_t ws_bytes, tq_join_state_t* state /* host out */, void* stream);
 

@@ -130,11 +130,33 @@ def _run_join(device, bk, bm, bv, pk, pm, pv, kind='inner', conds=()):
     return gpu, cpu
 
 
-@pytest.mark.parametrize("case", ["unique", "dups", "skew", "empty_build", "empty_probe", "int64"])
+@pytest.mark.parametrize("case", ["unique", "dups", "skew", "empty_build", "empty_probe", "int64",
+                                  "hash_dups", "int64_max", "dense_int64_dups", "narrow_build",
+                                  "wide_probe"])
 def test_join_inner_bit_exact(device, case):
+    # dense directory: unique, dups, skew, dense_int64_dups, narrow_build, wide_probe;
+    # hash slots: int64 (unique keys), hash_dups, int64_max (largest int64 key present)
     g = np.random.default_rng(sum(map(ord, case)))
     nb, np_ = 50_000, 200_000
-    if case == "unique":
+    if case == "hash_dups":
+        pool = g.integers(-(1 << 50), 1 << 50, 5000, dtype=np.int64)
+        bk = pool[g.integers(0, 5000, nb)]
+        pk = np.where(g.random(np_) < 0.7, pool[g.integers(0, 5000, np_)], g.integers(0, 1 << 50, np_))
+    elif case == "int64_max":
+        bk = g.integers(-(1 << 60), 1 << 60, nb, dtype=np.int64)
+        bk[:100] = np.iinfo(np.int64).max
+        pk = np.where(g.random(np_) < 0.5, bk[g.integers(0, nb, np_)], g.integers(0, 1 << 60, np_))
+    elif case == "dense_int64_dups":
+        bk = g.integers(-3000, 3000, nb, dtype=np.int64) + (1 << 40)
+        pk = g.integers(-3500, 3500, np_, dtype=np.int64) + (1 << 40)
+    elif case == "narrow_build":
+        bk = g.integers(-30000, 30000, nb, dtype=np.int16)
+        pk = g.integers(-40000, 40000, np_, dtype=np.int32)
+    elif case == "wide_probe":
+        bk = g.permutation(np.arange(-nb // 2, nb // 2, dtype=np.int32))
+        pk = np.where(g.random(np_) < 0.5, g.integers(-nb // 2, nb // 2, np_),
+                      g.integers(-(1 << 40), 1 << 40, np_)).astype(np.int64)
+    elif case == "unique":
         bk = g.permutation(np.arange(1, nb + 1, dtype=np.int32))
         pk = g.integers(1, nb + 1, np_, dtype=np.int32)
     elif case == "dups":

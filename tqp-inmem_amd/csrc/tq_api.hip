@@ -2,6 +2,10 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <atomic>
+#include <mutex>
+#include <vector>
+
 #include "tq_common.h"
 
 namespace tq {
@@ -24,8 +28,77 @@ int check_launch(const char* what) {
   return TQ_OK;
 }
 
+// ------------------------------------------------------------------ kernel timing log
+// A log of (kernel name, start event, stop event) per timed launch.  Event pairs are created on
+// first use and reused after every read; launches beyond kTimingCap records run untimed and
+// are counted as dropped.
+namespace {
+constexpr size_t kTimingCap = 1 << 16;
+struct TimedLaunch {
+  const char* name;
+  hipEvent_t e0, e1;
+};
+std::atomic<bool> g_timing{false};
+std::mutex g_timing_mu;
+std::vector<TimedLaunch> g_log;     // first g_used entries are live
+size_t g_used = 0;
+int64_t g_dropped = 0;
+}  // namespace
+
+bool timing_on() { return g_timing.load(std::memory_order_relaxed); }
+
+void timing_slot(const char* name, hipEvent_t* e0, hipEvent_t* e1) {
+  std::lock_guard<std::mutex> g(g_timing_mu);
+  if (g_used == g_log.size()) {
+    if (g_log.size() >= kTimingCap) {
+      ++g_dropped;
+      return;
+    }
+    TimedLaunch t{nullptr, nullptr, nullptr};
+    if (hipEventCreate(&t.e0) != hipSuccess || hipEventCreate(&t.e1) != hipSuccess) {
+      ++g_dropped;
+      return;
+    }
+    g_log.push_back(t);
+  }
+  TimedLaunch& t = g_log[g_used++];
+  t.name = name;
+  *e0 = t.e0;
+  *e1 = t.e1;
+}
+
 }  // namespace tq
 
 extern "C" int tq_version(void) { return 10000; }
 extern "C" const char* tq_last_error(void) { return tq::g_last_error.c_str(); }
 extern "C" int tq_tile_rows(void) { return tq::kTileRows; }
+
+extern "C" int tq_timing(int32_t on) {
+  tq::g_timing.store(on != 0);
+  return TQ_OK;
+}
+
+extern "C" int64_t tq_timing_count(void) {
+  std::lock_guard<std::mutex> g(tq::g_timing_mu);
+  return (int64_t)tq::g_used;
+}
+
+extern "C" int tq_timing_read(tq_ktime_t* out, int64_t cap, int64_t* n, int64_t* dropped) {
+  std::lock_guard<std::mutex> g(tq::g_timing_mu);
+  const int64_t m = (int64_t)tq::g_used < cap ? (int64_t)tq::g_used : cap;
+  for (int64_t i = 0; i < m; ++i) {
+    float ms = 0.0f;
+    hipError_t e = hipEventElapsedTime(&ms, tq::g_log[i].e0, tq::g_log[i].e1);
+    if (e != hipSuccess)
+      return tq::set_error(TQ_EHIP, "timing read of %s: %s", tq::g_log[i].name,
+                           hipGetErrorString(e));
+    out[i].name = tq::g_log[i].name;
+    out[i].ms = ms;
+    out[i].pad = 0;
+  }
+  if (n) *n = m;
+  if (dropped) *dropped = tq::g_dropped;
+  tq::g_used = 0;
+  tq::g_dropped = 0;
+  return TQ_OK;
+}

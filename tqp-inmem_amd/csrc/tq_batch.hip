@@ -349,7 +349,5 @@ extern "C" int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const 
   const int32_t stage_words = stage_loads * kSubRows;
   const size_t lds = (size_t)stage_words * 8 + align_up((size_t)lds_table_bytes, 8);
   if (lds > 64 * 1024) return set_error(TQ_EINVAL, "batch: %zu bytes of LDS", lds);
-  k_batch<<<(unsigned)grid, kBlock, lds, s>>>(pipes, items, ps, tile_rows, w, spin_limit,
-                                              stage_words);
-  return check_launch("batch");
+  return launch("k_batch", k_batch, (unsigned)grid, kBlock, lds, s, pipes, items, ps, tile_rows, w, spin_limit, stage_words);
 }

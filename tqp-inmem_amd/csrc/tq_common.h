@@ -3,9 +3,13 @@
 // coalesced (64 consecutive rows per wave instruction) and reductions inside LDS.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
 #include <string>
+#include <tuple>
+#include <utility>
 
 #include "../../include/tqhip.h"
 
@@ -36,6 +40,38 @@ int check_launch(const char* what);
     int r_ = (expr);          \
     if (r_ != TQ_OK) return r_; \
   } while (0)
+
+// ------------------------------------------------------------------ launches
+// Every kernel is launched through `launch`.  While kernel timing is on (tq_timing), the
+// dispatch carries a start and a stop event that are stamped when the kernel itself starts and
+// ends, so a recorded duration is the kernel's alone, excluding host gaps, as a kernel trace
+// measures it.  Timing is meant for eager launches, not for graph capture.
+bool timing_on();
+void timing_slot(const char* name, hipEvent_t* e0, hipEvent_t* e1);
+
+template <typename Tup, size_t... I>
+inline void arg_ptrs(Tup& t, void** out, std::index_sequence<I...>) {
+  ((out[I] = (void*)&std::get<I>(t)), ...);
+}
+
+template <typename... P, typename... A>
+int launch(const char* name, void (*kernel)(P...), dim3 grid, dim3 block, size_t lds,
+           hipStream_t s, A&&... a) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  std::tuple<P...> t(std::forward<A>(a)...);
+  void* argv[sizeof...(P) > 0 ? sizeof...(P) : 1];
+  arg_ptrs(t, argv, std::index_sequence_for<P...>{});
+  hipError_t e;
+  if (timing_on()) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    timing_slot(name, &e0, &e1);
+    e = hipExtLaunchKernel((const void*)kernel, grid, block, argv, lds, s, e0, e1, 0);
+  } else {
+    e = hipLaunchKernel((const void*)kernel, grid, block, argv, lds, s);
+  }
+  if (e != hipSuccess) return set_error(TQ_EHIP, "%s launch: %s", name, hipGetErrorString(e));
+  return TQ_OK;
+}
 
 // ------------------------------------------------------------------ wave64 primitives
 __device__ __forceinline__ int lane_id() {
@@ -109,6 +145,44 @@ __device__ __forceinline__ int64_t load_int(const void* p, int dtype, int64_t i)
     case TQ_I32: return ((const int32_t*)p)[i];
     default: return ((const int64_t*)p)[i];
   }
+}
+
+// Rows r..r+3 of an integer column (r a multiple of 4; rows at or past n read as 0).  One
+// vector load when the column is aligned for it and all four rows exist.
+__device__ __forceinline__ void load_int4(const void* p, int dtype, int64_t r, int64_t n,
+                                          int64_t v[4]) {
+  if (r + 3 < n) {
+    const uintptr_t a = (uintptr_t)p;
+    if (dtype == TQ_I32 && (a & 15) == 0) {
+      const int4 x = *(const int4*)((const int32_t*)p + r);
+      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+      return;
+    }
+    if (dtype == TQ_I64 && (a & 15) == 0) {
+      const longlong2 x = *(const longlong2*)((const int64_t*)p + r);
+      const longlong2 y = *(const longlong2*)((const int64_t*)p + r + 2);
+      v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = (r + i < n) ? load_int(p, dtype, r + i) : 0;
+}
+
+// Selection bits of rows r..r+3 (bit i: row r+i exists and, with a mask, its byte is nonzero).
+__device__ __forceinline__ uint32_t load_sel4(const uint8_t* mask, int64_t r, int64_t n) {
+  const uint32_t exist = r + 3 < n ? 15u : (r >= n ? 0u : ((1u << (n - r)) - 1u));
+  if (!mask) return exist;
+  uint32_t b = 0;
+  if (exist == 15u && ((uintptr_t)(mask + r) & 3) == 0) {
+    const uint32_t w = *(const uint32_t*)(mask + r);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b |= (((w >> (8 * i)) & 0xFFu) != 0u) << i;
+    return b;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b |= ((exist >> i) & 1u) && mask[r + i] != 0 ? (1u << i) : 0u;
+  return b;
 }
 
 inline int dtype_bytes(int dtype) {

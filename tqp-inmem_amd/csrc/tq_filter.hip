@@ -180,14 +180,12 @@ int scan_exclusive(const TIn* in, int64_t n, TOut* out, TOut* total, TOut* ws,
     return TQ_OK;
   }
   if (nchunks == 1) {
-    k_scan_chunk<TIn, TOut><<<1, kBlock, 0, stream>>>(in, n, out, nullptr, total);
-    return check_launch("scan");
+    return launch("k_scan_chunk", (k_scan_chunk<TIn, TOut>), 1, kBlock, 0, stream, in, n, out, nullptr, total);
   }
   if (nchunks > kScanChunk) return set_error(TQ_EINVAL, "scan of %lld elements too large", (long long)n);
-  k_chunk_sums<TIn, TOut><<<(unsigned)nchunks, kBlock, 0, stream>>>(in, n, ws);
-  k_scan_chunk<TOut, TOut><<<1, kBlock, 0, stream>>>(ws, nchunks, ws, nullptr, nullptr);
-  k_scan_chunk<TIn, TOut><<<(unsigned)nchunks, kBlock, 0, stream>>>(in, n, out, ws, total);
-  return check_launch("scan");
+  TQ_TRY(launch("k_chunk_sums", (k_chunk_sums<TIn, TOut>), (unsigned)nchunks, kBlock, 0, stream, in, n, ws));
+  TQ_TRY(launch("k_scan_chunk", (k_scan_chunk<TOut, TOut>), 1, kBlock, 0, stream, ws, nchunks, ws, nullptr, nullptr));
+  return launch("k_scan_chunk", (k_scan_chunk<TIn, TOut>), (unsigned)nchunks, kBlock, 0, stream, in, n, out, ws, total);
 }
 
 template int scan_exclusive<uint32_t, uint32_t>(const uint32_t*, int64_t, uint32_t*, uint32_t*,
@@ -264,14 +262,12 @@ __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, T v, int64_t
 
 int fill_u32(void* p, uint32_t v, int64_t count, hipStream_t s) {
   if (count <= 0) return TQ_OK;
-  k_fill<uint32_t><<<stream_grid(count), kBlock, 0, s>>>((uint32_t*)p, v, count);
-  return check_launch("fill");
+  return launch("k_fill", k_fill<uint32_t>, stream_grid(count), kBlock, 0, s, (uint32_t*)p, v, count);
 }
 
 int fill_u64(void* p, uint64_t v, int64_t count, hipStream_t s) {
   if (count <= 0) return TQ_OK;
-  k_fill<uint64_t><<<stream_grid(count), kBlock, 0, s>>>((uint64_t*)p, v, count);
-  return check_launch("fill");
+  return launch("k_fill", k_fill<uint64_t>, stream_grid(count), kBlock, 0, s, (uint64_t*)p, v, count);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mark(const int64_t* __restrict__ idx, int64_t m,
@@ -302,9 +298,7 @@ extern "C" int tq_filter_mask(const tq_program_t* prog, const tq_colset_t* cols,
   if (n <= 0) return TQ_OK;
   if (prog->nload > TQ_MAX_LOADS) return set_error(TQ_EINVAL, "filter: too many load slots");
   const int64_t tiles = ceil_div(n, kTileRows);
-  k_filter<<<(unsigned)tiles, kBlock, stage_bytes(*prog), (hipStream_t)stream>>>(
-      *prog, *cols, n, mask_out, tile_counts);
-  return check_launch("filter");
+  return launch("k_filter", k_filter, (unsigned)tiles, kBlock, stage_bytes(*prog), (hipStream_t)stream, *prog, *cols, n, mask_out, tile_counts);
 }
 
 extern "C" int tq_eval_column(const tq_program_t* prog, const tq_colset_t* cols, int64_t n,
@@ -317,17 +311,13 @@ extern "C" int tq_eval_column(const tq_program_t* prog, const tq_colset_t* cols,
   if (n <= 0) return TQ_OK;
   int64_t g = ceil_div(n, kSubRows);
   if (g > 4096) g = 4096;
-  k_eval<<<(unsigned)g, kBlock, stage_bytes(*prog), (hipStream_t)stream>>>(*prog, *cols, n, out,
-                                                                          out_bytes);
-  return check_launch("eval");
+  return launch("k_eval", k_eval, (unsigned)g, kBlock, stage_bytes(*prog), (hipStream_t)stream, *prog, *cols, n, out, out_bytes);
 }
 
 extern "C" int tq_mask_tile_counts(const uint8_t* mask, int64_t n, uint32_t* tile_counts,
                                    void* stream) {
   if (n <= 0) return TQ_OK;
-  k_mask_counts<<<(unsigned)ceil_div(n, kTileRows), kBlock, 0, (hipStream_t)stream>>>(
-      mask, n, tile_counts);
-  return check_launch("mask_counts");
+  return launch("k_mask_counts", k_mask_counts, (unsigned)ceil_div(n, kTileRows), kBlock, 0, (hipStream_t)stream, mask, n, tile_counts);
 }
 
 extern "C" size_t tq_scan_ws_bytes(int64_t ntiles) {
@@ -346,20 +336,16 @@ extern "C" int tq_compact(const uint8_t* mask, int64_t n, const int64_t* tile_of
   ColPack cp;
   TQ_TRY(make_colpack(&cp, ncols, src, elem_bytes, nullptr, dst));
   if (n <= 0 || ncols == 0) return TQ_OK;
-  k_compact<<<(unsigned)ceil_div(n, kTileRows), kBlock, 0, (hipStream_t)stream>>>(
-      mask, n, tile_offsets, cp);
-  return check_launch("compact");
+  return launch("k_compact", k_compact, (unsigned)ceil_div(n, kTileRows), kBlock, 0, (hipStream_t)stream, mask, n, tile_offsets, cp);
 }
 
 extern "C" int tq_mark_rows(const int64_t* idx, int64_t m, uint8_t* flags, int64_t n,
                              void* stream) {
   hipStream_t s = (hipStream_t)stream;
   if (n <= 0) return TQ_OK;
-  k_fill<uint8_t><<<stream_grid(n), kBlock, 0, s>>>(flags, (uint8_t)0, n);
-  TQ_TRY(check_launch("mark clear"));
+  TQ_TRY(launch("k_fill", k_fill<uint8_t>, stream_grid(n), kBlock, 0, s, flags, (uint8_t)0, n));
   if (m <= 0) return TQ_OK;
-  k_mark<<<stream_grid(m), kBlock, 0, s>>>(idx, m, flags, n);
-  return check_launch("mark");
+  return launch("k_mark", k_mark, stream_grid(m), kBlock, 0, s, idx, m, flags, n);
 }
 
 extern "C" int tq_gather(const int64_t* idx, int64_t m, int32_t ncols, const void* const* src,
@@ -368,6 +354,5 @@ extern "C" int tq_gather(const int64_t* idx, int64_t m, int32_t ncols, const voi
   ColPack cp;
   TQ_TRY(make_colpack(&cp, ncols, src, elem_bytes, float_kind, dst));
   if (m <= 0 || ncols == 0) return TQ_OK;
-  k_gather<<<stream_grid(m), kBlock, 0, (hipStream_t)stream>>>(idx, m, cp, null_fill);
-  return check_launch("gather");
+  return launch("k_gather", k_gather, stream_grid(m), kBlock, 0, (hipStream_t)stream, idx, m, cp, null_fill);
 }

@@ -54,31 +54,72 @@ struct KeyCols {
   int32_t nkeys;
 };
 
+// Selected-row count and per-key min/max in one pass: each thread reads runs of 4 consecutive
+// rows (vector loads), workgroups reduce through LDS, and thread 0 of each workgroup commits
+// with one atomic per value (the grid is kept small, so the atomics do not contend).
 __global__ __launch_bounds__(kBlock) void k_gb_bounds(KeyCols K, const uint8_t* __restrict__ mask,
                                                       int64_t n, int64_t* __restrict__ out) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  __shared__ int64_t sh[2 * TQ_GB_MAX_KEYS + 1][kWaves];
+  const int64_t stride = (int64_t)gridDim.x * kBlock * 4;
   int64_t cnt = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-    cnt += (!mask || mask[i] != 0);
-  cnt = wave_sum(cnt);
-  if (lane_id() == 0 && cnt) atomicAdd((unsigned long long*)&out[2 * K.nkeys], (unsigned long long)cnt);
-  for (int k = 0; k < K.nkeys; ++k) {
-    int64_t lo = INT64_MAX, hi = INT64_MIN;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-      if (mask && mask[i] == 0) continue;
-      const int64_t v = load_int(K.ptr[k], K.dtype[k], i);
-      lo = v < lo ? v : lo;
-      hi = v > hi ? v : hi;
+  int64_t lo[TQ_GB_MAX_KEYS], hi[TQ_GB_MAX_KEYS];
+#pragma unroll
+  for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
+    lo[k] = INT64_MAX;
+    hi[k] = INT64_MIN;
+  }
+  for (int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4; r < n; r += stride) {
+    const uint32_t sel = load_sel4(mask, r, n);
+    cnt += __popc(sel);
+#pragma unroll
+    for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
+      if (k >= K.nkeys) break;
+      int64_t v[4];
+      load_int4(K.ptr[k], K.dtype[k], r, n, v);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if ((sel >> i) & 1u) {
+          lo[k] = v[i] < lo[k] ? v[i] : lo[k];
+          hi[k] = v[i] > hi[k] ? v[i] : hi[k];
+        }
+      }
     }
+  }
+  cnt = wave_sum(cnt);
+#pragma unroll
+  for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
-      const int64_t a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
-      lo = a < lo ? a : lo;
-      hi = b > hi ? b : hi;
+      const int64_t a = __shfl_xor(lo[k], d, 64), b = __shfl_xor(hi[k], d, 64);
+      lo[k] = a < lo[k] ? a : lo[k];
+      hi[k] = b > hi[k] ? b : hi[k];
     }
-    if (lane_id() == 0) {
-      if (lo != INT64_MAX) atomicMin((long long*)&out[2 * k], (long long)lo);
-      if (hi != INT64_MIN) atomicMax((long long*)&out[2 * k + 1], (long long)hi);
+  }
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sh[0][w] = cnt;
+#pragma unroll
+    for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
+      sh[1 + 2 * k][w] = lo[k];
+      sh[2 + 2 * k][w] = hi[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWaves; ++i) {
+      cnt += sh[0][i];
+#pragma unroll
+      for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
+        lo[k] = sh[1 + 2 * k][i] < lo[k] ? sh[1 + 2 * k][i] : lo[k];
+        hi[k] = sh[2 + 2 * k][i] > hi[k] ? sh[2 + 2 * k][i] : hi[k];
+      }
+    }
+    if (cnt) {
+      atomicAdd((unsigned long long*)&out[2 * K.nkeys], (unsigned long long)cnt);
+      for (int k = 0; k < K.nkeys; ++k) {
+        atomicMin((long long*)&out[2 * k], (long long)lo[k]);
+        atomicMax((long long*)&out[2 * k + 1], (long long)hi[k]);
+      }
     }
   }
 }
@@ -443,13 +484,11 @@ extern "C" int tq_groupby_bounds(const tq_groupby_desc_t* d, const uint8_t* mask
     if (K.dtype[k] < TQ_BOOL || K.dtype[k] > TQ_I64)
       return set_error(TQ_EUNSUPPORTED, "group-by key %d must be an integer column", k);
   }
-  k_gb_init_bounds<<<1, kBlock, 0, s>>>(out, d->nkeys);
-  TQ_TRY(check_launch("bounds init"));
+  TQ_TRY(launch("k_gb_init_bounds", k_gb_init_bounds, 1, kBlock, 0, s, out, d->nkeys));
   if (n <= 0) return TQ_OK;
-  int64_t g = ceil_div(n, kBlock * 8);
-  if (g > 2048) g = 2048;
-  k_gb_bounds<<<(unsigned)g, kBlock, 0, s>>>(K, mask, n, out);
-  return check_launch("bounds");
+  int64_t g = ceil_div(n, kBlock * 16);
+  if (g > 512) g = 512;
+  return launch("k_gb_bounds", k_gb_bounds, (unsigned)g, kBlock, 0, s, K, mask, n, out);
 }
 
 extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload, int32_t* mode,
@@ -506,8 +545,7 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
   hipStream_t s = (hipStream_t)stream;
   const int base = mode & 7;
   const bool front = (mode & TQ_GB_LDS_FRONT) != 0;
-  k_gb_clear<<<stream_grid(slots), kBlock, 0, s>>>(w.T, G, base == TQ_GB_HASH);
-  TQ_TRY(check_launch("group-by clear"));
+  TQ_TRY(launch("k_gb_clear", k_gb_clear, stream_grid(slots), kBlock, 0, s, w.T, G, base == TQ_GB_HASH));
   if (n <= 0) return TQ_OK;
   const int32_t cap = lds_cap(mode, prog->nload, d->naggs, slots);
   const int32_t stage_words = (int32_t)(lds_stage_bytes(prog->nload) / 8);
@@ -518,17 +556,15 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
   int64_t g = ceil_div(n, kSubRows);
   if (g > 1024) g = 1024;
   const unsigned grid = (unsigned)g;
-  if (base == TQ_GB_DENSE_LDS)
-    k_gb_accum<TQ_GB_DENSE_LDS, false><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, cap, stage_words);
-  else if (base == TQ_GB_DENSE_GLOBAL && front)
-    k_gb_accum<TQ_GB_DENSE_GLOBAL, true><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, cap, stage_words);
-  else if (base == TQ_GB_DENSE_GLOBAL)
-    k_gb_accum<TQ_GB_DENSE_GLOBAL, false><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, 1, stage_words);
-  else if (front)
-    k_gb_accum<TQ_GB_HASH, true><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, cap, stage_words);
-  else
-    k_gb_accum<TQ_GB_HASH, false><<<grid, kBlock, lds, s>>>(G, *prog, *cols, mask, n, w.T, 1, stage_words);
-  return check_launch("group-by accumulate");
+#define TQ_ACCUM(BASE, FRONT, CAP)                                                          \
+  launch("k_gb_accum", (k_gb_accum<BASE, FRONT>), grid, kBlock, lds, s, G, *prog, *cols, mask, \
+         n, w.T, CAP, stage_words)
+  if (base == TQ_GB_DENSE_LDS) return TQ_ACCUM(TQ_GB_DENSE_LDS, false, cap);
+  if (base == TQ_GB_DENSE_GLOBAL && front) return TQ_ACCUM(TQ_GB_DENSE_GLOBAL, true, cap);
+  if (base == TQ_GB_DENSE_GLOBAL) return TQ_ACCUM(TQ_GB_DENSE_GLOBAL, false, 1);
+  if (front) return TQ_ACCUM(TQ_GB_HASH, true, cap);
+  return TQ_ACCUM(TQ_GB_HASH, false, 1);
+#undef TQ_ACCUM
 }
 
 extern "C" int tq_groupby_count(const tq_groupby_desc_t* d, int32_t mode, int64_t slots, void* ws,
@@ -537,8 +573,7 @@ extern "C" int tq_groupby_count(const tq_groupby_desc_t* d, int32_t mode, int64_
   GbWs w = carve_gb(ws, mode, slots, d->naggs);
   hipStream_t s = (hipStream_t)stream;
   const int64_t nt = ceil_div(slots, kTileRows);
-  k_gb_occupied<<<(unsigned)nt, kBlock, 0, s>>>(w.T.cnt, slots, d->nkeys == 0, w.tile_counts);
-  TQ_TRY(check_launch("group-by occupied"));
+  TQ_TRY(launch("k_gb_occupied", k_gb_occupied, (unsigned)nt, kBlock, 0, s, w.T.cnt, slots, d->nkeys == 0, w.tile_counts));
   TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.ngroups, w.scan_ws, s));
   if (ngroups_host) TQ_TRY(read_back(w.ngroups, 8, ngroups_host, s));
   return TQ_OK;
@@ -561,16 +596,12 @@ extern "C" int tq_groupby_write(const tq_groupby_desc_t* d, int32_t mode, int64_
   }
   const int64_t nt = ceil_div(slots, kTileRows);
   if ((mode & 7) != TQ_GB_HASH) {
-    k_gb_write_dense<<<(unsigned)nt, kBlock, 0, s>>>(G, O, w.T, w.tile_off, d->nkeys == 0);
-    return check_launch("group-by write");
+    return launch("k_gb_write_dense", k_gb_write_dense, (unsigned)nt, kBlock, 0, s, G, O, w.T, w.tile_off, d->nkeys == 0);
   }
-  k_gb_compact_hash<<<(unsigned)nt, kBlock, 0, s>>>(w.T, w.tile_off, w.k0, w.v0);
-  TQ_TRY(check_launch("group-by compact"));
+  TQ_TRY(launch("k_gb_compact_hash", k_gb_compact_hash, (unsigned)nt, kBlock, 0, s, w.T, w.tile_off, w.k0, w.v0));
   int lo, hi;
   TQ_TRY(key_bit_range<uint64_t>(w.k0, ngroups, w.stats, s, &lo, &hi));
   bool alt;
   TQ_TRY(radix_sort_pairs<uint64_t>(w.k0, w.v0, w.k1, w.v1, ngroups, lo, hi, w.radix, s, &alt));
-  k_gb_write_sorted<<<stream_grid(ngroups), kBlock, 0, s>>>(G, O, w.T, alt ? w.k1 : w.k0,
-                                                            alt ? w.v1 : w.v0, ngroups);
-  return check_launch("group-by write");
+  return launch("k_gb_write_sorted", k_gb_write_sorted, stream_grid(ngroups), kBlock, 0, s, G, O, w.T, alt ? w.k1 : w.k0, alt ? w.v1 : w.v0, ngroups);
 }

@@ -1,209 +1,370 @@
-// Sort-based equi-join: build = compact + radix-sort the selected build keys (row ids ride
-// along, so each equal-key bucket lists build rows ascending); probe = two passes over probe
-// tiles (count, then write at exclusive-scan offsets) that emit pairs in probe-row order.
+// Equi-join index and probe.
 //
-// Probe lookups: when the selected build keys are dense (span <= 4*nsel + 65536 values) a
-// directory indexed by key - min key gives the first sorted position in one load.  Otherwise
-// the sorted keys are searched in two levels: every workgroup first copies an evenly spaced
-// sample of the sorted keys into LDS (the last key of each block of `step` keys), finds the
-// block by binary search in LDS, and only then searches that block in global memory, so a
-// lookup touches one or two cache lines of the key array instead of log2(nsel).
+// Build.  One pass over the build table counts the selected rows of every 4096-row tile and
+// reduces the order-transformed keys' OR, AND, min and max into per-tile partials, which one
+// small kernel combines (no contended atomics).  The host reads the row count and those four
+// numbers (one synchronisation) and picks the index layout:
+//   * dense: a directory indexed by key - min, when the keys span at most 4*nb + 65536 values;
+//   * hash: open addressing with linear probing over a power-of-two slot array of at least
+//     twice the selected rows, multiplicative (Fibonacci) hashing.
+// The build first assumes unique keys: every selected row claims its key's entry with one
+// compare-and-swap and a second claim of a key raises a flag, which the host reads (second
+// synchronisation).  Without duplicates an entry holds its build row and the build is done,
+// with no sort.  With duplicates the selected (key, row) pairs are compacted in row order and
+// radix-sorted by key (stable, so each key's rows stay ascending); an entry then holds the
+// first sorted position of its key and the number of rows with that key.
+//
+// Probe.  Two passes over 4096-row probe tiles.  The count pass looks up every selected probe
+// row and writes the number of pairs per tile; an exclusive scan gives each tile's first output
+// position; the write pass repeats the lookups and writes the pairs at their positions, probe
+// row ascending then build row ascending, gathering the requested build and probe columns
+// straight into the outputs.  A thread owns four runs of four consecutive rows, so mask and key
+// loads are vector loads, and the directory loads of all its rows are issued before any is
+// used.
 #include "tq_common.h"
 #include "tq_sort.h"
 
 namespace tq {
 
-constexpr int kSampleBytes = 32768;
+constexpr uint32_t kNoPos = 0xFFFFFFFFu;
+constexpr int kRuns = kItems / 4;  // runs of 4 consecutive rows per thread and tile
 
-struct JoinView {
-  const void* keys;
-  const uint32_t* rows;
-  const void* sample;
-  const uint32_t* dir;
-  uint64_t dir_base;
-  int64_t dir_range;
-  int64_t nsel;
-  int64_t step;
-  int32_t nsample;
+struct Slot {
+  uint64_t key;
+  uint32_t a;    // build row (unique) or first sorted position
+  uint32_t cnt;  // rows with this key
 };
 
-constexpr uint32_t kNoPos = 0xFFFFFFFFu;
+struct JoinView {
+  const void* table;
+  const uint32_t* rows;
+  uint64_t base;
+  uint64_t empty;
+  int64_t range;
+  int32_t hash_bits;
+};
 
-template <typename K>
-__device__ __forceinline__ bool probe_key(const void* p, int dt, int64_t r, K* out) {
-  const int64_t v = load_int(p, dt, r);
-  if (sizeof(K) == 4) {
-    if (v < INT32_MIN || v > INT32_MAX) return false;
-    *out = (K)((uint32_t)(int32_t)v ^ 0x80000000u);
-  } else {
-    *out = (K)((uint64_t)v ^ 0x8000000000000000ull);
-  }
-  return true;
-}
-
-// First index in [0, nsel) whose key is >= t (strict=false) or > t (strict=true), nsel if none.
-template <typename K>
-__device__ __forceinline__ int64_t search(const JoinView& J, const K* __restrict__ ssample,
-                                          const K* __restrict__ keys, K t, bool strict) {
-  int lo = 0, hi = J.nsample;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const K s = ssample[mid];
-    if (strict ? (s <= t) : (s < t)) lo = mid + 1; else hi = mid;
-  }
-  if (lo == J.nsample) return J.nsel;
-  int64_t b = (int64_t)lo * J.step;
-  int64_t e = b + J.step;
-  if (e > J.nsel) e = J.nsel;
-  while (b < e) {
-    const int64_t mid = (b + e) >> 1;
-    const K k = keys[mid];
-    if (strict ? (k <= t) : (k < t)) b = mid + 1; else e = mid;
-  }
-  return b;
-}
-
-// Bucket of key t: first position and length (0 when absent).
-template <typename K>
-__device__ __forceinline__ uint32_t bucket(const JoinView& J, const K* __restrict__ ssample, K t,
-                                           int64_t* first) {
-  const K* keys = (const K*)J.keys;
-  if (J.dir) {
-    const uint64_t off = (uint64_t)t - J.dir_base;  // wraps to a huge value below the base
-    if (off >= (uint64_t)J.dir_range) return 0;
-    const uint32_t p = J.dir[off];
-    *first = p;
-    if (p == kNoPos) return 0;
-    if ((int64_t)p + 1 >= J.nsel || keys[p + 1] != t) return 1;
-    return (uint32_t)(search<K>(J, ssample, keys, t, true) - p);
-  }
-  const int64_t p = search<K>(J, ssample, keys, t, false);
-  *first = p;
-  if (p >= J.nsel || keys[p] != t) return 0;
-  if (p + 1 >= J.nsel || keys[p + 1] != t) return 1;
-  return (uint32_t)(search<K>(J, ssample, keys, t, true) - p);
+__host__ __device__ inline uint64_t slot_of(uint64_t t, int bits) {
+  return (t * 0x9E3779B97F4A7C15ull) >> (64 - bits);
 }
 
 template <typename K>
-__device__ __forceinline__ void load_sample(const JoinView& J, K* ssample) {
-  const K* src = (const K*)J.sample;
-  for (int i = threadIdx.x; i < J.nsample; i += kBlock) ssample[i] = src[i];
-  __syncthreads();
+__device__ __forceinline__ K order_key(int64_t v) {
+  return sizeof(K) == 4 ? (K)((uint32_t)(int32_t)v ^ 0x80000000u)
+                        : (K)((uint64_t)v ^ 0x8000000000000000ull);
 }
 
-// ------------------------------------------------------------------ build
-// Also reduces OR, AND, min and max of the packed keys into stats[0..3], so the host learns the
-// sort's bit range and the directory's span together with the selected-row count.
+// A probe value can only match 32-bit build keys when it lies in the int32 range.
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_join_pack(const void* __restrict__ bkeys, int dt,
-                                                      const uint8_t* __restrict__ mask, int64_t n,
-                                                      const int64_t* __restrict__ tile_off,
-                                                      K* __restrict__ keys,
-                                                      uint32_t* __restrict__ rows,
-                                                      unsigned long long* __restrict__ stats) {
+__device__ __forceinline__ bool probe_fits(int64_t v) {
+  return sizeof(K) == 8 || (v >= INT32_MIN && v <= INT32_MAX);
+}
+
+// Number of build rows with key t (0 when !valid) and, through *a, the entry's row/position.
+template <typename K, bool HASH, bool UNIQUE>
+__device__ __forceinline__ uint32_t lookup(const JoinView& J, K t, bool valid, uint32_t* a) {
+  if (!HASH) {
+    const uint64_t off = (uint64_t)t - J.base;  // below the base wraps to a huge value
+    valid = valid && off < (uint64_t)J.range;
+    const uint64_t idx = valid ? off : 0;
+    if (UNIQUE) {
+      const uint32_t e = ((const uint32_t*)J.table)[idx];
+      *a = e;
+      return (valid && e != kNoPos) ? 1u : 0u;
+    }
+    const uint2 e = ((const uint2*)J.table)[idx];
+    *a = e.x;
+    return valid ? e.y : 0u;
+  }
+  *a = 0;
+  if (!valid) return 0;
+  const Slot* S = (const Slot*)J.table;
+  const uint64_t mask = (uint64_t)J.range - 1;
+  for (uint64_t h = slot_of((uint64_t)t, J.hash_bits);; h = (h + 1) & mask) {
+    const Slot s = S[h];
+    if (s.key == (uint64_t)t) {
+      *a = s.a;
+      return UNIQUE ? 1u : s.cnt;
+    }
+    if (s.key == J.empty) return 0;
+  }
+}
+
+// Selection bits and values of a thread's 16 rows of a tile: run q covers rows
+// tile*4096 + q*1024 + 4*threadIdx.x + (0..3).
+__device__ __forceinline__ void load_tile_rows(const void* keys, int dt, const uint8_t* mask,
+                                               int64_t n, int64_t tile, uint32_t* sel,
+                                               int64_t v[kItems]) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kRuns; ++q) {
+    const int64_t r = tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * threadIdx.x;
+    s |= load_sel4(mask, r, n) << (4 * q);
+    load_int4(keys, dt, r, n, &v[4 * q]);
+  }
+  *sel = s;
+}
+
+__device__ __forceinline__ int64_t tile_row(int64_t tile, int j) {
+  return tile * kTileRows + (int64_t)(j >> 2) * (4 * kBlock) + 4 * threadIdx.x + (j & 3);
+}
+
+// ------------------------------------------------------------------ build: statistics
+// Per tile: selected rows -> tile_counts[tile]; OR, AND, min, max of the selected transformed
+// keys -> part[4*tile ..] (min > max when the tile selects nothing).
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_join_stats(const void* __restrict__ bkeys, int dt,
+                                                       const uint8_t* __restrict__ mask,
+                                                       int64_t n, uint32_t* __restrict__ tile_counts,
+                                                       uint64_t* __restrict__ part) {
+  __shared__ uint64_t sh[5][kWaves];
+  const int64_t tile = blockIdx.x;
+  uint32_t sel;
+  int64_t v[kItems];
+  load_tile_rows(bkeys, dt, mask, n, tile, &sel, v);
   uint64_t so = 0, sa = ~0ull, slo = ~0ull, shi = 0;
-  __shared__ uint32_t wcnt[kItems][kWaves];
-  const int t = threadIdx.x, w = t >> 6, l = lane_id();
-  const int64_t tile0 = (int64_t)blockIdx.x * kTileRows;
-  bool f[kItems];
-  uint64_t bal[kItems];
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
-    const int64_t r = tile0 + (int64_t)j * kBlock + t;
-    f[j] = (r < n) && (!mask || mask[r] != 0);
-    bal[j] = ballot(f[j]);
-    if (l == 0) wcnt[j][w] = (uint32_t)__popcll(bal[j]);
-  }
-  __syncthreads();
-  int64_t run = mask ? tile_off[blockIdx.x] : tile0;
-#pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    uint32_t before = 0, tot = 0;
-#pragma unroll
-    for (int v = 0; v < kWaves; ++v) {
-      const uint32_t c = wcnt[j][v];
-      tot += c;
-      before += (v < w) ? c : 0;
+    if ((sel >> j) & 1u) {
+      const uint64_t k = (uint64_t)order_key<K>(v[j]);
+      so |= k;
+      sa &= k;
+      slo = k < slo ? k : slo;
+      shi = k > shi ? k : shi;
     }
-    if (f[j]) {
-      const int64_t r = tile0 + (int64_t)j * kBlock + t;
-      const int64_t pos = run + before + rank_below(bal[j]);
-      const int64_t v = load_int(bkeys, dt, r);
-      const K k = sizeof(K) == 4 ? (K)((uint32_t)(int32_t)v ^ 0x80000000u)
-                                 : (K)((uint64_t)v ^ 0x8000000000000000ull);
-      keys[pos] = k;
-      rows[pos] = (uint32_t)r;
-      so |= (uint64_t)k;
-      sa &= (uint64_t)k;
-      slo = (uint64_t)k < slo ? (uint64_t)k : slo;
-      shi = (uint64_t)k > shi ? (uint64_t)k : shi;
-    }
-    run += tot;
   }
+  uint64_t c = (uint64_t)__popc(sel);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
+    c += __shfl_xor(c, d, 64);
     so |= __shfl_xor(so, d, 64);
     sa &= __shfl_xor(sa, d, 64);
     const uint64_t a2 = __shfl_xor(slo, d, 64), b2 = __shfl_xor(shi, d, 64);
     slo = a2 < slo ? a2 : slo;
     shi = b2 > shi ? b2 : shi;
   }
-  if (l == 0 && shi >= slo) {
-    atomicOr(&stats[0], (unsigned long long)so);
-    atomicAnd(&stats[1], (unsigned long long)sa);
-    atomicMin(&stats[2], (unsigned long long)slo);
-    atomicMax(&stats[3], (unsigned long long)shi);
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sh[0][w] = c; sh[1][w] = so; sh[2][w] = sa; sh[3][w] = slo; sh[4][w] = shi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 1; i < kWaves; ++i) {
+      c += sh[0][i]; so |= sh[1][i]; sa &= sh[2][i];
+      slo = sh[3][i] < slo ? sh[3][i] : slo;
+      shi = sh[4][i] > shi ? sh[4][i] : shi;
+    }
+    tile_counts[tile] = (uint32_t)c;
+    part[4 * tile + 0] = so;
+    part[4 * tile + 1] = sa;
+    part[4 * tile + 2] = slo;
+    part[4 * tile + 3] = shi;
   }
 }
 
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_join_sample(const K* __restrict__ keys, int64_t nsel,
-                                                        int64_t step, int32_t nsample,
-                                                        K* __restrict__ sample) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < nsample) {
-    int64_t e = (int64_t)(i + 1) * step;
-    if (e > nsel) e = nsel;
-    sample[i] = keys[e - 1];
+// One workgroup: stats = {selected rows, OR, AND, min, max}; clears the duplicate flag.
+__global__ __launch_bounds__(kBlock) void k_join_stats_final(const uint32_t* __restrict__ counts,
+                                                             const uint64_t* __restrict__ part,
+                                                             int64_t ntiles,
+                                                             uint64_t* __restrict__ stats,
+                                                             uint32_t* __restrict__ dup) {
+  __shared__ uint64_t sh[5][kWaves];
+  uint64_t c = 0, so = 0, sa = ~0ull, slo = ~0ull, shi = 0;
+  for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) {
+    c += counts[i];
+    if (counts[i]) {
+      so |= part[4 * i];
+      sa &= part[4 * i + 1];
+      slo = part[4 * i + 2] < slo ? part[4 * i + 2] : slo;
+      shi = part[4 * i + 3] > shi ? part[4 * i + 3] : shi;
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    c += __shfl_xor(c, d, 64);
+    so |= __shfl_xor(so, d, 64);
+    sa &= __shfl_xor(sa, d, 64);
+    const uint64_t a2 = __shfl_xor(slo, d, 64), b2 = __shfl_xor(shi, d, 64);
+    slo = a2 < slo ? a2 : slo;
+    shi = b2 > shi ? b2 : shi;
+  }
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sh[0][w] = c; sh[1][w] = so; sh[2][w] = sa; sh[3][w] = slo; sh[4][w] = shi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 1; i < kWaves; ++i) {
+      c += sh[0][i]; so |= sh[1][i]; sa &= sh[2][i];
+      slo = sh[3][i] < slo ? sh[3][i] : slo;
+      shi = sh[4][i] > shi ? sh[4][i] : shi;
+    }
+    stats[0] = c; stats[1] = so; stats[2] = sa; stats[3] = slo; stats[4] = shi;
+    *dup = 0;
   }
 }
 
-// Dense directory: dir[key - base] = first sorted position of key (dir pre-filled with kNoPos).
+// ------------------------------------------------------------------ build: unique claim
+template <bool HASH>
+__global__ __launch_bounds__(kBlock) void k_join_clear(void* __restrict__ table, int64_t n,
+                                                       uint64_t empty) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    if (HASH) {
+      Slot s;
+      s.key = empty; s.a = 0; s.cnt = 0;
+      ((Slot*)table)[i] = s;
+    } else {
+      ((uint32_t*)table)[i] = kNoPos;
+    }
+  }
+}
+
+// Every selected build row claims its key's entry; a key claimed twice sets *dup.
+template <typename K, bool HASH>
+__global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ bkeys, int dt,
+                                                       const uint8_t* __restrict__ mask,
+                                                       int64_t n, JoinView J,
+                                                       uint32_t* __restrict__ dup) {
+  const int64_t tile = blockIdx.x;
+  uint32_t sel;
+  int64_t v[kItems];
+  load_tile_rows(bkeys, dt, mask, n, tile, &sel, v);
+  bool seen = false;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    if (!((sel >> j) & 1u)) continue;
+    const uint64_t t = (uint64_t)order_key<K>(v[j]);
+    const uint32_t row = (uint32_t)tile_row(tile, j);
+    if (!HASH) {
+      const uint32_t old = atomicCAS((uint32_t*)J.table + (t - J.base), kNoPos, row);
+      seen |= old != kNoPos;
+    } else {
+      Slot* S = (Slot*)J.table;
+      const uint64_t m = (uint64_t)J.range - 1;
+      for (uint64_t h = slot_of(t, J.hash_bits);; h = (h + 1) & m) {
+        const unsigned long long old =
+            atomicCAS((unsigned long long*)&S[h].key, (unsigned long long)J.empty,
+                      (unsigned long long)t);
+        if (old == J.empty) {
+          S[h].a = row;
+          S[h].cnt = 1;
+          break;
+        }
+        if (old == t) {
+          seen = true;
+          break;
+        }
+      }
+    }
+  }
+  if (seen) *dup = 1;
+}
+
+// ------------------------------------------------------------------ build: duplicate keys
+// Compact the selected (key, row) pairs in row order.
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_join_dir(const K* __restrict__ keys, int64_t nsel,
-                                                     uint64_t base, uint32_t* __restrict__ dir) {
+__global__ __launch_bounds__(kBlock) void k_join_pack(const void* __restrict__ bkeys, int dt,
+                                                      const uint8_t* __restrict__ mask, int64_t n,
+                                                      const int64_t* __restrict__ tile_off,
+                                                      K* __restrict__ keys,
+                                                      uint32_t* __restrict__ rows) {
+  __shared__ uint32_t sh[kWaves];
+  const int64_t tile = blockIdx.x;
+  uint32_t sel;
+  int64_t v[kItems];
+  load_tile_rows(bkeys, dt, mask, n, tile, &sel, v);
+  int64_t base = tile_off[tile];
+#pragma unroll
+  for (int q = 0; q < kRuns; ++q) {
+    const uint32_t s4 = (sel >> (4 * q)) & 15u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_sum((uint32_t)__popc(s4), sh, &tot);
+    int64_t o = base + ex;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if ((s4 >> i) & 1u) {
+        keys[o] = order_key<K>(v[4 * q + i]);
+        rows[o] = (uint32_t)tile_row(tile, 4 * q + i);
+        ++o;
+      }
+    }
+    base += tot;
+  }
+}
+
+// Entries of the sorted keys: at the first position i of each key, count its rows (galloping
+// then binary search for the end of the run) and record {i, count}.
+template <typename K, bool HASH>
+__global__ __launch_bounds__(kBlock) void k_join_runs(const K* __restrict__ keys, int64_t nsel,
+                                                      JoinView J) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nsel; i += stride) {
-    const K k = keys[i];
-    if (i == 0 || keys[i - 1] != k) dir[(uint64_t)k - base] = (uint32_t)i;
+    const K t = keys[i];
+    if (i > 0 && keys[i - 1] == t) continue;
+    int64_t lo = i, step = 1;  // keys[lo] == t
+    while (lo + step < nsel && keys[lo + step] == t) {
+      lo += step;
+      step <<= 1;
+    }
+    int64_t hi = lo + step < nsel ? lo + step : nsel;  // keys[hi] != t or hi == nsel
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (keys[mid] == t) lo = mid; else hi = mid;
+    }
+    const uint32_t cnt = (uint32_t)(hi - i);
+    if (!HASH) {
+      ((uint2*)J.table)[(uint64_t)t - J.base] = make_uint2((uint32_t)i, cnt);
+    } else {
+      Slot* S = (Slot*)J.table;
+      const uint64_t m = (uint64_t)J.range - 1;
+      for (uint64_t h = slot_of((uint64_t)t, J.hash_bits);; h = (h + 1) & m) {
+        const unsigned long long old =
+            atomicCAS((unsigned long long*)&S[h].key, (unsigned long long)J.empty,
+                      (unsigned long long)t);
+        if (old == J.empty) {
+          S[h].a = (uint32_t)i;
+          S[h].cnt = cnt;
+          break;
+        }
+      }
+    }
   }
 }
 
 // ------------------------------------------------------------------ probe
-template <typename K>
+template <typename K, bool HASH, bool UNIQUE>
+__device__ __forceinline__ void probe_rows(const JoinView& J, const void* pkeys, int pdt,
+                                           const uint8_t* pmask, int64_t np, int64_t tile,
+                                           uint32_t c[kItems], uint32_t a[kItems]) {
+  uint32_t sel;
+  int64_t v[kItems];
+  load_tile_rows(pkeys, pdt, pmask, np, tile, &sel, v);
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const bool ok = ((sel >> j) & 1u) && probe_fits<K>(v[j]);
+    c[j] = lookup<K, HASH, UNIQUE>(J, order_key<K>(v[j]), ok, &a[j]);
+  }
+}
+
+template <typename K, bool HASH, bool UNIQUE>
 __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* __restrict__ pkeys,
                                                        int pdt, const uint8_t* __restrict__ pmask,
                                                        int64_t np, int64_t ntiles,
                                                        uint32_t* __restrict__ tile_counts) {
-  extern __shared__ unsigned char smem[];
-  K* ssample = (K*)smem;
   __shared__ uint32_t sh[kWaves];
-  load_sample<K>(J, ssample);
-  const int t = threadIdx.x;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t c[kItems], a[kItems];
+    probe_rows<K, HASH, UNIQUE>(J, pkeys, pdt, pmask, np, tile, c, a);
     uint32_t cnt = 0;
-    for (int j = 0; j < kItems; ++j) {
-      const int64_t r = tile * kTileRows + (int64_t)j * kBlock + t;
-      K key;
-      if (r < np && (!pmask || pmask[r] != 0) && probe_key<K>(pkeys, pdt, r, &key)) {
-        int64_t first;
-        cnt += bucket<K>(J, ssample, key, &first);
-      }
-    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) cnt += c[j];
     cnt = wave_sum(cnt);
-    if (lane_id() == 0) sh[t >> 6] = cnt;
+    if (lane_id() == 0) sh[threadIdx.x >> 6] = cnt;
     __syncthreads();
-    if (t == 0) {
+    if (threadIdx.x == 0) {
       uint32_t s = 0;
 #pragma unroll
       for (int w = 0; w < kWaves; ++w) s += sh[w];
@@ -213,95 +374,84 @@ __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* _
   }
 }
 
-template <typename K>
+template <typename K, bool HASH, bool UNIQUE>
 __global__ __launch_bounds__(kBlock) void k_join_write(
     JoinView J, const void* __restrict__ pkeys, int pdt, const uint8_t* __restrict__ pmask,
     int64_t np, int64_t ntiles, const int64_t* __restrict__ tile_off, ColPack L, ColPack Rc,
     int64_t* __restrict__ build_idx, int64_t* __restrict__ probe_idx) {
-  extern __shared__ unsigned char smem[];
-  K* ssample = (K*)smem;
-  __shared__ uint32_t wtot[kItems][kWaves];
-  load_sample<K>(J, ssample);
-  const int t = threadIdx.x, w = t >> 6, l = lane_id();
+  __shared__ uint32_t sh[kWaves];
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint32_t cnt[kItems], incl[kItems];
-    int64_t first[kItems];
+    uint32_t c[kItems], a[kItems];
+    probe_rows<K, HASH, UNIQUE>(J, pkeys, pdt, pmask, np, tile, c, a);
+    int64_t base = tile_off[tile];
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const int64_t r = tile * kTileRows + (int64_t)j * kBlock + t;
-      K key;
-      cnt[j] = 0;
-      first[j] = 0;
-      if (r < np && (!pmask || pmask[r] != 0) && probe_key<K>(pkeys, pdt, r, &key))
-        cnt[j] = bucket<K>(J, ssample, key, &first[j]);
-      incl[j] = wave_inclusive_sum(cnt[j]);
-      if (l == 63) wtot[j][w] = incl[j];
-    }
-    __syncthreads();
-    int64_t run = tile_off[tile];
+    for (int q = 0; q < kRuns; ++q) {
+      const uint32_t tc = c[4 * q] + c[4 * q + 1] + c[4 * q + 2] + c[4 * q + 3];
+      uint32_t tot;
+      const uint32_t ex = block_exclusive_sum(tc, sh, &tot);
+      int64_t o = base + ex;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      uint32_t before = 0, tot = 0;
-#pragma unroll
-      for (int v = 0; v < kWaves; ++v) {
-        const uint32_t c = wtot[j][v];
-        tot += c;
-        before += (v < w) ? c : 0;
-      }
-      if (cnt[j]) {
-        const int64_t r = tile * kTileRows + (int64_t)j * kBlock + t;
-        const int64_t o0 = run + before + incl[j] - cnt[j];
-        for (uint32_t k = 0; k < cnt[j]; ++k) {
-          const int64_t o = o0 + k;
-          const uint32_t b = J.rows[first[j] + k];
-          for (int c = 0; c < L.n; ++c) copy_row(L.dst[c], o, L.src[c], b, L.bytes[c]);
-          for (int c = 0; c < Rc.n; ++c) copy_row(Rc.dst[c], o, Rc.src[c], r, Rc.bytes[c]);
+      for (int i = 0; i < 4; ++i) {
+        const int j = 4 * q + i;
+        const int64_t r = tile_row(tile, j);
+        for (uint32_t k = 0; k < c[j]; ++k, ++o) {
+          const uint32_t b = UNIQUE ? a[j] : J.rows[a[j] + k];
+          for (int x = 0; x < L.n; ++x) copy_row(L.dst[x], o, L.src[x], b, L.bytes[x]);
+          for (int x = 0; x < Rc.n; ++x) copy_row(Rc.dst[x], o, Rc.src[x], r, Rc.bytes[x]);
           if (build_idx) build_idx[o] = b;
           if (probe_idx) probe_idx[o] = r;
         }
       }
-      run += tot;
+      base += tot;
     }
-    __syncthreads();
   }
 }
 
 // ------------------------------------------------------------------ workspace layouts
+static int64_t dense_cap(int64_t nb) { return 4 * nb + 65536; }
+static int hash_bits_for(int64_t n) {
+  int b = 10;
+  while (((int64_t)1 << b) < 2 * n) ++b;
+  return b;
+}
+
 struct BuildWs {
   uint32_t* tile_counts;
   int64_t* tile_off;
+  uint64_t* stats;   // selected rows, OR, AND, min, max
+  uint32_t* dup;
   int64_t* total;
   int64_t* scan_ws;
+  uint64_t* part;
   void* k0;
   void* k1;
   uint32_t* v0;
   uint32_t* v1;
-  void* stats;
   void* radix;
-  void* sample;
-  uint32_t* dir;
+  void* table;
   size_t bytes;
 };
-
-static int64_t dir_cap(int64_t nsel) { return 4 * nsel + 65536; }
 
 static BuildWs carve_build(void* base, int64_t nb) {
   if (nb < 1) nb = 1;
   const int64_t nt = ceil_div(nb, kTileRows);
+  const size_t dense_bytes = (size_t)dense_cap(nb) * 8;
+  const size_t hash_bytes = ((size_t)1 << hash_bits_for(nb)) * sizeof(Slot);
   char* p = (char*)base;
   BuildWs w;
   w.tile_counts = (uint32_t*)p; p += align_up((size_t)nt * 4, 256);
   w.tile_off = (int64_t*)p; p += align_up((size_t)nt * 8, 256);
-  w.total = (int64_t*)p;          // total and the 4 key statistics are read back together
-  w.stats = p + 8; p += 256;
+  w.stats = (uint64_t*)p;
+  w.dup = (uint32_t*)(p + 64);
+  w.total = (int64_t*)(p + 128); p += 256;
   w.scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(nt);
+  w.part = (uint64_t*)p; p += align_up((size_t)nt * 32, 256);
   w.k0 = p; p += align_up((size_t)nb * 8, 256);
   w.k1 = p; p += align_up((size_t)nb * 8, 256);
   w.v0 = (uint32_t*)p; p += align_up((size_t)nb * 4, 256);
   w.v1 = (uint32_t*)p; p += align_up((size_t)nb * 4, 256);
   w.radix = p; p += radix_ws_bytes(nb);
-  w.sample = p; p += kSampleBytes;
-  w.dir = (uint32_t*)p; p += align_up((size_t)dir_cap(nb) * 4, 256);
+  w.table = p; p += align_up(dense_bytes > hash_bytes ? dense_bytes : hash_bytes, 256);
   w.bytes = (size_t)(p - (char*)base);
   return w;
 }
@@ -329,79 +479,137 @@ static ProbeWs carve_probe(void* base, int64_t np) {
 
 static JoinView view_of(const tq_join_state_t* st) {
   JoinView J;
-  J.keys = st->keys;
+  J.table = st->table;
   J.rows = st->rows;
-  J.sample = st->sample;
-  J.dir = st->dir;
-  J.dir_base = st->dir_base;
-  J.dir_range = st->dir_range;
-  J.nsel = st->nsel;
-  J.step = st->step;
-  J.nsample = st->nsample;
+  J.base = st->base;
+  J.empty = st->empty_key;
+  J.range = st->range;
+  J.hash_bits = st->hash_bits;
   return J;
 }
 
 static unsigned probe_grid(int64_t ntiles) {
-  return (unsigned)(ntiles < 1024 ? (ntiles > 0 ? ntiles : 1) : 1024);
+  return (unsigned)(ntiles < 2048 ? (ntiles > 0 ? ntiles : 1) : 2048);
 }
 
 template <typename K>
 static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64_t nb, BuildWs& w,
                       tq_join_state_t* st, hipStream_t s) {
   const int64_t nt = ceil_div(nb, kTileRows);
-  int64_t nsel = nb;
-  if (bmask) {
-    TQ_TRY(tq_mask_tile_counts(bmask, nb, w.tile_counts, s));
-    TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
-  }
-  unsigned long long* kst = (unsigned long long*)w.stats;
-  TQ_TRY(fill_u64(kst, 0ull, 1, s));
-  TQ_TRY(fill_u64(kst + 1, ~0ull, 2, s));
-  TQ_TRY(fill_u64(kst + 3, 0ull, 1, s));
-  k_join_pack<K><<<(unsigned)nt, kBlock, 0, s>>>(bkeys, dt, bmask, nb, w.tile_off, (K*)w.k0, w.v0,
-                                                kst);
-  TQ_TRY(check_launch("join pack"));
+  TQ_TRY(launch("k_join_stats", k_join_stats<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask,
+                nb, w.tile_counts, w.part));
+  TQ_TRY(launch("k_join_stats_final", k_join_stats_final, 1, kBlock, 0, s,
+                (const uint32_t*)w.tile_counts, (const uint64_t*)w.part, nt, w.stats, w.dup));
   int64_t hv[5];
-  if (!bmask) TQ_TRY(fill_u64(w.total, (uint64_t)nb, 1, s));
-  TQ_TRY(read_back(w.total, 40, hv, s));   // nsel, OR, AND, min, max in one synchronisation
-  nsel = hv[0];
+  TQ_TRY(read_back(w.stats, 40, hv, s));  // one synchronisation: count and key statistics
+  const int64_t nsel = hv[0];
+  st->nsel = nsel;
+  if (nsel == 0) return TQ_OK;
+  const uint64_t kor = (uint64_t)hv[1], kand = (uint64_t)hv[2];
   const uint64_t kmin = (uint64_t)hv[3], kmax = (uint64_t)hv[4];
+  const bool hash = kmax - kmin >= (uint64_t)dense_cap(nb);
+  st->layout = hash ? TQ_JOIN_HASH : TQ_JOIN_DENSE;
+  if (hash) {
+    st->hash_bits = hash_bits_for(nsel);
+    st->range = (int64_t)1 << st->hash_bits;
+    if (sizeof(K) == 4 || kmax != ~0ull) {
+      st->empty_key = sizeof(K) == 4 ? ~0ull : kmax + 1;
+    } else if (kmin != 0) {
+      st->empty_key = kmin - 1;
+    } else {
+      return set_error(TQ_EUNSUPPORTED,
+                       "join build keys include both INT64_MIN and INT64_MAX with a sparse "
+                       "range: no free key value marks an empty hash slot");
+    }
+  } else {
+    st->base = kmin;
+    st->range = (int64_t)(kmax - kmin) + 1;
+  }
+  st->table = w.table;
+  JoinView J = view_of(st);
+  const unsigned cg = stream_grid(st->range);
+  if (hash)
+    TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
+                  st->empty_key));
+  else
+    TQ_TRY(launch("k_join_clear", k_join_clear<false>, cg, kBlock, 0, s, w.table, st->range,
+                  st->empty_key));
+  if (hash)
+    TQ_TRY(launch("k_join_claim", (k_join_claim<K, true>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
+                  bmask, nb, J, w.dup));
+  else
+    TQ_TRY(launch("k_join_claim", (k_join_claim<K, false>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
+                  bmask, nb, J, w.dup));
+  int64_t dup = 0;
+  TQ_TRY(read_back(w.dup, 4, &dup, s));  // second synchronisation: were the keys unique?
+  if ((dup & 0xFFFFFFFF) == 0) {
+    st->unique = 1;
+    return TQ_OK;
+  }
+  // duplicate keys: compact, sort stably by key, record {first position, count} per key
+  st->unique = 0;
+  TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
+  TQ_TRY(launch("k_join_pack", k_join_pack<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask, nb,
+                (const int64_t*)w.tile_off, (K*)w.k0, w.v0));
   int lo = 0, hi = 0;
-  radix_bit_range((uint64_t)hv[1], (uint64_t)hv[2], &lo, &hi);
+  radix_bit_range(kor, kand, &lo, &hi);
   bool alt;
   TQ_TRY(radix_sort_pairs<K>((K*)w.k0, w.v0, (K*)w.k1, w.v1, nsel, lo, hi, w.radix, s, &alt));
   const K* keys = (const K*)(alt ? w.k1 : w.k0);
-  const uint32_t* rows = alt ? w.v1 : w.v0;
-  const int64_t cap = kSampleBytes / (int64_t)sizeof(K);
-  const int32_t nsample = (int32_t)(nsel < cap ? nsel : cap);
-  const int64_t step = nsample > 0 ? ceil_div(nsel, nsample) : 1;
-  const int32_t nsample_eff = nsample > 0 ? (int32_t)ceil_div(nsel, step) : 0;
-  if (nsample_eff > 0) {
-    k_join_sample<K><<<(unsigned)ceil_div(nsample_eff, kBlock), kBlock, 0, s>>>(
-        keys, nsel, step, nsample_eff, (K*)w.sample);
-    TQ_TRY(check_launch("join sample"));
+  st->rows = alt ? w.v1 : w.v0;
+  J.rows = st->rows;
+  if (hash) {
+    TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
+                  st->empty_key));
+    TQ_TRY(launch("k_join_runs", (k_join_runs<K, true>), stream_grid(nsel), kBlock, 0, s, keys,
+                  nsel, J));
+  } else {
+    TQ_TRY(fill_u64(w.table, 0ull, st->range, s));  // {first, count} = {0, 0}: absent
+    TQ_TRY(launch("k_join_runs", (k_join_runs<K, false>), stream_grid(nsel), kBlock, 0, s, keys,
+                  nsel, J));
   }
-  st->dir = nullptr;
-  st->dir_base = 0;
-  st->dir_range = 0;
-  if (nsel > 0 && kmax - kmin < (uint64_t)dir_cap(nsel)) {
-    const int64_t range = (int64_t)(kmax - kmin) + 1;
-    TQ_TRY(fill_u32(w.dir, kNoPos, range, s));
-    k_join_dir<K><<<stream_grid(nsel), kBlock, 0, s>>>(keys, nsel, kmin, w.dir);
-    TQ_TRY(check_launch("join directory"));
-    st->dir = w.dir;
-    st->dir_base = kmin;
-    st->dir_range = range;
-  }
-  st->nsel = nsel;
-  st->key_bits = (int32_t)(8 * sizeof(K));
-  st->nsample = nsample_eff;
-  st->step = step;
-  st->keys = keys;
-  st->rows = rows;
-  st->sample = w.sample;
   return TQ_OK;
 }
+
+template <typename K, bool HASH, bool UNIQUE>
+static int count_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
+                      const uint8_t* pmask, int64_t np, int64_t nt, ProbeWs& w, hipStream_t s) {
+  return launch("k_join_count", (k_join_count<K, HASH, UNIQUE>), probe_grid(nt), kBlock, 0, s,
+                view_of(st), pkeys, pdt, pmask, np, nt, w.tile_counts);
+}
+
+template <typename K, bool HASH, bool UNIQUE>
+static int write_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
+                      const uint8_t* pmask, int64_t np, int64_t nt, const ProbeWs& w,
+                      const ColPack& L, const ColPack& R, int64_t* bidx, int64_t* pidx,
+                      hipStream_t s) {
+  return launch("k_join_write", (k_join_write<K, HASH, UNIQUE>), probe_grid(nt), kBlock, 0, s,
+                view_of(st), pkeys, pdt, pmask, np, nt, (const int64_t*)w.tile_off, L, R, bidx,
+                pidx);
+}
+
+// Calls F<K, HASH, UNIQUE>::run(args...) for the state's key width and layout.
+template <template <typename, bool, bool> class F, typename... A>
+static int dispatch(const tq_join_state_t* st, A&&... a) {
+  const bool h = st->layout == TQ_JOIN_HASH, u = st->unique != 0;
+  if (st->key_bits == 32) {
+    if (h) return u ? F<uint32_t, true, true>::run(a...) : F<uint32_t, true, false>::run(a...);
+    return u ? F<uint32_t, false, true>::run(a...) : F<uint32_t, false, false>::run(a...);
+  }
+  if (h) return u ? F<uint64_t, true, true>::run(a...) : F<uint64_t, true, false>::run(a...);
+  return u ? F<uint64_t, false, true>::run(a...) : F<uint64_t, false, false>::run(a...);
+}
+
+template <typename K, bool H, bool U>
+struct CountF {
+  template <typename... A>
+  static int run(A&&... a) { return count_impl<K, H, U>(a...); }
+};
+template <typename K, bool H, bool U>
+struct WriteF {
+  template <typename... A>
+  static int run(A&&... a) { return write_impl<K, H, U>(a...); }
+};
 
 }  // namespace tq
 
@@ -415,11 +623,12 @@ extern "C" int tq_join_build(const void* bkeys, int32_t bdtype, const uint8_t* b
   if (!state) return set_error(TQ_EINVAL, "join build: null state");
   if (bdtype < TQ_BOOL || bdtype > TQ_I64)
     return set_error(TQ_EUNSUPPORTED, "join keys must be integer columns (dtype %d)", bdtype);
-  if (nb >= ((int64_t)1 << 32)) return set_error(TQ_EINVAL, "build side too large");
+  if (nb >= ((int64_t)1 << 32) - 1) return set_error(TQ_EINVAL, "build side too large");
   BuildWs w = carve_build(ws, nb);
   if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "join build workspace too small");
   *state = tq_join_state_t{};
   state->key_bits = bdtype == TQ_I64 ? 64 : 32;
+  state->layout = TQ_JOIN_EMPTY;
   if (nb <= 0) return TQ_OK;
   hipStream_t s = (hipStream_t)stream;
   return bdtype == TQ_I64 ? build_impl<uint64_t>(bkeys, bdtype, bmask, nb, w, state, s)
@@ -435,22 +644,14 @@ extern "C" int tq_join_count(const tq_join_state_t* st, const void* pkeys, int32
   hipStream_t s = (hipStream_t)stream;
   ProbeWs w = carve_probe(pws, np);
   const int64_t nt = np > 0 ? ceil_div(np, kTileRows) : 0;
-  if (np <= 0 || st->nsel <= 0) {
+  if (np <= 0 || st->nsel <= 0 || st->layout == TQ_JOIN_EMPTY) {
     TQ_TRY(fill_u32(w.tile_counts, 0, nt, s));
     TQ_TRY(fill_u64(w.tile_off, 0, nt, s));
     TQ_TRY(fill_u64(w.total, 0, 1, s));
     if (total_host) *total_host = 0;
     return TQ_OK;
   }
-  JoinView J = view_of(st);
-  const size_t lds = (size_t)st->nsample * (st->key_bits / 8);
-  if (st->key_bits == 32)
-    k_join_count<uint32_t><<<probe_grid(nt), kBlock, lds, s>>>(J, pkeys, pdtype, pmask, np, nt,
-                                                                w.tile_counts);
-  else
-    k_join_count<uint64_t><<<probe_grid(nt), kBlock, lds, s>>>(J, pkeys, pdtype, pmask, np, nt,
-                                                                w.tile_counts);
-  TQ_TRY(check_launch("join count"));
+  TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
   TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
   if (total_host) TQ_TRY(read_back(w.total, 8, total_host, s));
   return TQ_OK;
@@ -466,19 +667,9 @@ extern "C" int tq_join_write(const tq_join_state_t* st, const void* pkeys, int32
   ColPack L, R;
   TQ_TRY(make_colpack(&L, nleft, lsrc, lbytes, nullptr, ldst));
   TQ_TRY(make_colpack(&R, nright, rsrc, rbytes, nullptr, rdst));
-  if (np <= 0 || st->nsel <= 0) return TQ_OK;
+  if (np <= 0 || st->nsel <= 0 || st->layout == TQ_JOIN_EMPTY) return TQ_OK;
   hipStream_t s = (hipStream_t)stream;
   ProbeWs w = carve_probe((void*)pws, np);
   const int64_t nt = ceil_div(np, kTileRows);
-  JoinView J = view_of(st);
-  const size_t lds = (size_t)st->nsample * (st->key_bits / 8);
-  if (st->key_bits == 32)
-    k_join_write<uint32_t><<<probe_grid(nt), kBlock, lds, s>>>(J, pkeys, pdtype, pmask, np, nt,
-                                                                w.tile_off, L, R, build_idx,
-                                                                probe_idx);
-  else
-    k_join_write<uint64_t><<<probe_grid(nt), kBlock, lds, s>>>(J, pkeys, pdtype, pmask, np, nt,
-                                                                w.tile_off, L, R, build_idx,
-                                                                probe_idx);
-  return check_launch("join write");
+  return dispatch<WriteF>(st, st, pkeys, pdtype, pmask, np, nt, w, L, R, build_idx, probe_idx, s);
 }

@@ -118,9 +118,12 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
   }
 }
 
+// OR, AND, min and max of n keys: per-thread, then per-workgroup through LDS, then one atomic
+// per statistic and workgroup (the grid is kept small, so the atomics do not contend).
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_key_bits(const K* __restrict__ keys, int64_t n,
                                                      unsigned long long* __restrict__ stats) {
+  __shared__ uint64_t sh[4][kWaves];
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   uint64_t o = 0, a = ~0ull, lo = ~0ull, hi = 0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
@@ -138,11 +141,25 @@ __global__ __launch_bounds__(kBlock) void k_key_bits(const K* __restrict__ keys,
     lo = l2 < lo ? l2 : lo;
     hi = h2 > hi ? h2 : hi;
   }
+  const int w = threadIdx.x >> 6;
   if (lane_id() == 0) {
-    atomicOr(&stats[0], (unsigned long long)o);
-    atomicAnd(&stats[1], (unsigned long long)a);
-    atomicMin(&stats[2], (unsigned long long)lo);
-    atomicMax(&stats[3], (unsigned long long)hi);
+    sh[0][w] = o; sh[1][w] = a; sh[2][w] = lo; sh[3][w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 1; i < kWaves; ++i) {
+      o |= sh[0][i];
+      a &= sh[1][i];
+      lo = sh[2][i] < lo ? sh[2][i] : lo;
+      hi = sh[3][i] > hi ? sh[3][i] : hi;
+    }
+    if (lo <= hi) {
+      atomicOr(&stats[0], (unsigned long long)o);
+      atomicAnd(&stats[1], (unsigned long long)a);
+      atomicMin(&stats[2], (unsigned long long)lo);
+      atomicMax(&stats[3], (unsigned long long)hi);
+    }
   }
 }
 
@@ -166,10 +183,9 @@ int radix_sort_pairs(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, i
   K *ki = keys, *ko = keys_alt;
   uint32_t *vi = vals, *vo = vals_alt;
   for (int shift = lo_bit; shift < hi_bit; shift += 8) {
-    k_radix_hist<K><<<(unsigned)ntiles, kBlock, 0, s>>>(ki, n, shift, hist, ntiles);
+    TQ_TRY(launch("k_radix_hist", k_radix_hist<K>, (unsigned)ntiles, kBlock, 0, s, ki, n, shift, hist, ntiles));
     TQ_TRY((scan_exclusive<uint32_t, uint32_t>(hist, h, offs, nullptr, sws, s)));
-    k_radix_scatter<K><<<(unsigned)ntiles, kBlock, 0, s>>>(ki, vi, n, shift, offs, ntiles, ko, vo);
-    TQ_TRY(check_launch("radix pass"));
+    TQ_TRY(launch("k_radix_scatter", k_radix_scatter<K>, (unsigned)ntiles, kBlock, 0, s, ki, vi, n, shift, offs, ntiles, ko, vo));
     K* tk = ki; ki = ko; ko = tk;
     uint32_t* tv = vi; vi = vo; vo = tv;
     *in_alt = !*in_alt;
@@ -209,10 +225,9 @@ int key_bit_range(const K* keys, int64_t n, void* stats_dev, hipStream_t s, int*
   TQ_TRY(fill_u64(st, 0ull, 1, s));
   TQ_TRY(fill_u64(st + 1, ~0ull, 2, s));
   TQ_TRY(fill_u64(st + 3, 0ull, 1, s));
-  int64_t g = ceil_div(n, kBlock);
-  if (g > 2048) g = 2048;
-  k_key_bits<K><<<(unsigned)g, kBlock, 0, s>>>(keys, n, st);
-  TQ_TRY(check_launch("key bits"));
+  int64_t g = ceil_div(n, kBlock * 8);
+  if (g > 512) g = 512;
+  TQ_TRY(launch("k_key_bits", k_key_bits<K>, (unsigned)g, kBlock, 0, s, keys, n, st));
   int64_t hv[4];
   TQ_TRY(read_back(st, 32, hv, s));
   if (kmin) *kmin = (uint64_t)hv[2];
@@ -283,52 +298,124 @@ __global__ __launch_bounds__(kBlock) void k_pack_keys(FieldPack fp, const uint32
 }
 
 // ------------------------------------------------------------------ top-k
-// One tile of up to 4096 (key, row) candidates is bitonic-sorted in LDS by key, then row (so ties
-// keep input order); the first k go to the output at tile * k.  Rounds of this kernel shrink any
-// candidate set to k without a host synchronisation.
+// The k smallest (key, row) pairs of a tile of up to 4096 candidates, without sorting the tile:
+// a radix select in LDS finds the k-th smallest pair T, digit by digit from the most
+// significant (8 key bytes, then 4 row bytes; each pass histograms the candidates whose higher
+// digits equal T's digits found so far, and a pass whose bin holds exactly the remaining rank
+// ends the search).  Every real pair <= T is kept, at most k; the tile's k output slots are
+// filled up with sentinels (all-ones key and row).  Rounds of this kernel shrink any candidate
+// set to one tile without a host synchronisation; the last round also ranks its k pairs and
+// writes their rows, smallest first, as the permutation.  Rows are distinct, so ordering by
+// (key, row) is the stable order of the keys.
 constexpr int kTopTile = 4096;
+constexpr int kTopPer = kTopTile / kBlock;
 
-__global__ __launch_bounds__(kBlock) void k_topk_tile(const uint64_t* __restrict__ kin,
-                                                      const uint32_t* __restrict__ vin, int64_t n,
-                                                      int k, uint64_t* __restrict__ kout,
-                                                      uint32_t* __restrict__ vout) {
-  __shared__ uint64_t sk[kTopTile];
-  __shared__ uint32_t sv[kTopTile];
+__device__ __forceinline__ uint32_t digit_of(uint64_t key, uint32_t row, int d) {
+  return d >= 4 ? (uint32_t)(key >> (8 * (d - 4))) & 0xFFu : (row >> (8 * d)) & 0xFFu;
+}
+
+__global__ __launch_bounds__(kBlock) void k_topk_select(const uint64_t* __restrict__ kin,
+                                                        const uint32_t* __restrict__ vin,
+                                                        int64_t n, int k,
+                                                        uint64_t* __restrict__ kout,
+                                                        uint32_t* __restrict__ vout,
+                                                        int64_t* __restrict__ perm_out,
+                                                        int64_t nout) {
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t scan_sh[kWaves];
+  __shared__ uint32_t found[3];  // digit, remaining rank, bin count
+  __shared__ uint32_t nsel;
+  __shared__ uint64_t fk[256];
+  __shared__ uint32_t fv[256];
+  const int t = threadIdx.x, l = lane_id();
   const int64_t base = (int64_t)blockIdx.x * kTopTile;
-  const int64_t rem = n - base;
-  const int cnt = rem < kTopTile ? (int)rem : kTopTile;
-  for (int i = threadIdx.x; i < kTopTile; i += kBlock) {
-    if (i < cnt) {
-      sk[i] = kin[base + i];
-      sv[i] = vin ? vin[base + i] : (uint32_t)(base + i);
+  uint64_t key[kTopPer];
+  uint32_t row[kTopPer];
+#pragma unroll
+  for (int j = 0; j < kTopPer; ++j) {
+    const int64_t i = base + (int64_t)j * kBlock + t;
+    const bool v = i < n;
+    key[j] = v ? kin[i] : ~0ull;
+    row[j] = v ? (vin ? vin[i] : (uint32_t)i) : 0xFFFFFFFFu;
+  }
+  uint64_t pk = 0, mk = 0;  // digits of T found so far (key part) and their mask
+  uint32_t pr = 0, mr = 0;  // (row part)
+  uint32_t kk = (uint32_t)k;
+  for (int d = 11; d >= 0; --d) {
+    hist[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kTopPer; ++j) {
+      const bool cand = (key[j] & mk) == pk && (row[j] & mr) == pr;
+      const uint32_t dg = digit_of(key[j], row[j], d);
+      // lanes holding the same digit add once, through their lowest lane
+      uint64_t peers = ballot(cand);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint64_t m = ballot(cand && ((dg >> b) & 1u));
+        peers &= ((dg >> b) & 1u) ? m : ~m;
+      }
+      if (cand && (int)__builtin_ctzll(peers) == l) atomicAdd(&hist[dg], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    const uint32_t h = hist[t];
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_sum(h, scan_sh, &tot);
+    if (ex < kk && kk <= ex + h) {
+      found[0] = (uint32_t)t;
+      found[1] = kk - ex;
+      found[2] = h;
+    }
+    __syncthreads();
+    const uint32_t dg = found[0];
+    kk = found[1];
+    const bool whole = found[2] == kk;
+    if (d >= 4) {
+      pk |= (uint64_t)dg << (8 * (d - 4));
+      mk |= 0xFFull << (8 * (d - 4));
     } else {
-      sk[i] = ~0ull;
-      sv[i] = 0xFFFFFFFFu;
+      pr |= dg << (8 * d);
+      mr |= 0xFFu << (8 * d);
+    }
+    __syncthreads();
+    if (whole) break;
+  }
+  // keep the real pairs whose digits so far are <= T's
+  if (t == 0) nsel = 0;
+  __syncthreads();
+  const int64_t obase = (int64_t)blockIdx.x * k;
+#pragma unroll
+  for (int j = 0; j < kTopPer; ++j) {
+    const uint64_t km = key[j] & mk;
+    const bool keep = row[j] != 0xFFFFFFFFu &&
+                      (km < pk || (km == pk && (row[j] & mr) <= pr));
+    if (keep) {
+      const uint32_t slot = atomicAdd(&nsel, 1u);
+      if (perm_out) {
+        fk[slot] = key[j];
+        fv[slot] = row[j];
+      } else {
+        kout[obase + slot] = key[j];
+        vout[obase + slot] = row[j];
+      }
     }
   }
   __syncthreads();
-  for (int size = 2; size <= kTopTile; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = threadIdx.x; i < kTopTile / 2; i += kBlock) {
-        const int lo = 2 * i - (i & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const uint64_t a = sk[lo], b = sk[hi];
-        const uint32_t va = sv[lo], vb = sv[hi];
-        const bool greater = a > b || (a == b && va > vb);
-        if (greater == up) {
-          sk[lo] = b; sk[hi] = a;
-          sv[lo] = vb; sv[hi] = va;
-        }
-      }
-      __syncthreads();
+  const uint32_t m = nsel;
+  if (!perm_out) {
+    for (int i = (int)m + t; i < k; i += kBlock) {
+      kout[obase + i] = ~0ull;
+      vout[obase + i] = 0xFFFFFFFFu;
     }
+    return;
   }
-  const int m = cnt < k ? cnt : k;
-  for (int i = threadIdx.x; i < k; i += kBlock) {
-    const int64_t o = (int64_t)blockIdx.x * k + i;
-    kout[o] = i < m ? sk[i] : ~0ull;
-    vout[o] = i < m ? sv[i] : 0xFFFFFFFFu;
+  // last round: rank the kept pairs by (key, row)
+  for (int i = t; i < (int)m; i += kBlock) {
+    const uint64_t a = fk[i];
+    const uint32_t va = fv[i];
+    int rank = 0;
+    for (int j = 0; j < (int)m; ++j) rank += (fk[j] < a || (fk[j] == a && fv[j] < va)) ? 1 : 0;
+    if (rank < nout) perm_out[rank] = va;
   }
 }
 
@@ -385,8 +472,7 @@ static int sort_group(const FieldPack& fp, int total_bits, const uint32_t* perm,
     uint32_t* k1 = (uint32_t*)w.k1;
     uint32_t* v0 = perm == w.v0 ? w.v1 : w.v0;
     uint32_t* v1 = perm == w.v0 ? w.v0 : w.v1;
-    k_pack_keys<uint32_t><<<g, kBlock, 0, s>>>(fp, perm, n, k0, v0);
-    TQ_TRY(check_launch("pack keys"));
+    TQ_TRY(launch("k_pack_keys", k_pack_keys<uint32_t>, g, kBlock, 0, s, fp, perm, n, k0, v0));
     int lo, hi;
     TQ_TRY(key_bit_range<uint32_t>(k0, n, w.stats, s, &lo, &hi));
     bool alt;
@@ -395,8 +481,7 @@ static int sort_group(const FieldPack& fp, int total_bits, const uint32_t* perm,
   } else {
     uint32_t* v0 = perm == w.v0 ? w.v1 : w.v0;
     uint32_t* v1 = perm == w.v0 ? w.v0 : w.v1;
-    k_pack_keys<uint64_t><<<g, kBlock, 0, s>>>(fp, perm, n, w.k0, v0);
-    TQ_TRY(check_launch("pack keys"));
+    TQ_TRY(launch("k_pack_keys", k_pack_keys<uint64_t>, g, kBlock, 0, s, fp, perm, n, w.k0, v0));
     int lo, hi;
     TQ_TRY(key_bit_range<uint64_t>(w.k0, n, w.stats, s, &lo, &hi));
     bool alt;
@@ -449,8 +534,7 @@ int sort_perm_u32(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, void* ws, 
   }
   if (!perm) {  // no keys: identity
     FieldPack fp{};
-    k_pack_keys<uint32_t><<<stream_grid(n), kBlock, 0, s>>>(fp, nullptr, n, (uint32_t*)w.k0, w.v0);
-    TQ_TRY(check_launch("identity"));
+    TQ_TRY(launch("k_pack_keys", k_pack_keys<uint32_t>, stream_grid(n), kBlock, 0, s, fp, nullptr, n, (uint32_t*)w.k0, w.v0));
     perm = w.v0;
   }
   *perm_out = perm;
@@ -484,24 +568,25 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
   }
   fp.n = nkeys;
   SortWs w = carve_sort_ws(ws, n);
-  k_pack_keys<uint64_t><<<stream_grid(n), kBlock, 0, s>>>(fp, nullptr, n, w.k0, w.v0);
-  TQ_TRY(check_launch("top-k pack"));
+  TQ_TRY(launch("k_pack_keys", k_pack_keys<uint64_t>, stream_grid(n), kBlock, 0, s, fp, nullptr, n, w.k0, w.v0));
   const int k = (int)limit;
+  const int64_t out = limit < n ? limit : n;
   uint64_t *ki = w.k0, *ko = w.k1;
-  uint32_t *vi = w.v0, *vo = w.v1;
+  uint32_t *vi = nullptr, *vo = w.v1;
   int64_t m = n;
   while (true) {
     const int64_t tiles = ceil_div(m, kTopTile);
-    k_topk_tile<<<(unsigned)tiles, kBlock, 0, s>>>(ki, vi, m, k, ko, vo);
-    TQ_TRY(check_launch("top-k tile"));
+    const bool last = tiles == 1;
+    TQ_TRY(launch("k_topk_select", k_topk_select, (unsigned)tiles, kBlock, 0, s,
+                  (const uint64_t*)ki, (const uint32_t*)vi, m, k, ko, vo,
+                  last ? perm_out : (int64_t*)nullptr, out));
+    if (last) break;
     m = tiles * k;
-    uint64_t* tk = ki; ki = ko; ko = tk;
-    uint32_t* tv = vi; vi = vo; vo = tv;
-    if (tiles == 1) break;
+    ki = ko;  // this round's output is the next round's input
+    vi = vo;
+    ko = ki == w.k0 ? w.k1 : w.k0;
+    vo = vi == w.v0 ? w.v1 : w.v0;
   }
-  const int64_t out = limit < n ? limit : n;
-  k_perm_out<<<stream_grid(out), kBlock, 0, s>>>(vi, out, perm_out);
-  TQ_TRY(check_launch("top-k out"));
   *done = true;
   return TQ_OK;
 }
@@ -519,6 +604,5 @@ extern "C" int tq_sort_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, 
   uint32_t* perm;
   TQ_TRY(sort_perm_u32(keys, nkeys, n, ws, ws_bytes, s, &perm));
   const int64_t m = (limit >= 0 && limit < n) ? limit : n;
-  k_perm_out<<<stream_grid(m), kBlock, 0, s>>>(perm, m, perm_out);
-  return check_launch("sort perm");
+  return launch("k_perm_out", k_perm_out, stream_grid(m), kBlock, 0, s, perm, m, perm_out);
 }

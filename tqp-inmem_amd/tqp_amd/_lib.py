@@ -75,6 +75,10 @@ class ColSet(ctypes.Structure):
     _fields_ = [("ptr", ctypes.c_void_p * TQ_MAX_COLS), ("width", ctypes.c_int32 * TQ_MAX_COLS)]
 
 
+class KTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("ms", ctypes.c_float), ("pad", ctypes.c_int32)]
+
+
 class SortKey(ctypes.Structure):
     _fields_ = [("ptr", ctypes.c_void_p), ("dtype", ctypes.c_int32), ("width", ctypes.c_int32),
                 ("descending", ctypes.c_int32), ("pad", ctypes.c_int32)]
@@ -82,9 +86,10 @@ class SortKey(ctypes.Structure):
 
 class JoinState(ctypes.Structure):
     _fields_ = [("nsel", ctypes.c_int64), ("key_bits", ctypes.c_int32),
-                ("nsample", ctypes.c_int32), ("step", ctypes.c_int64),
-                ("keys", ctypes.c_void_p), ("rows", ctypes.c_void_p), ("sample", ctypes.c_void_p),
-                ("dir", ctypes.c_void_p), ("dir_base", ctypes.c_uint64), ("dir_range", ctypes.c_int64)]
+                ("layout", ctypes.c_int32), ("unique", ctypes.c_int32),
+                ("hash_bits", ctypes.c_int32), ("base", ctypes.c_uint64),
+                ("empty_key", ctypes.c_uint64), ("range", ctypes.c_int64),
+                ("table", ctypes.c_void_p), ("rows", ctypes.c_void_p)]
 
 
 class GroupByDesc(ctypes.Structure):
@@ -106,6 +111,9 @@ _SIGNATURES = {
     "tq_version": ([], I32),
     "tq_last_error": ([], ctypes.c_char_p),
     "tq_tile_rows": ([], I32),
+    "tq_timing": ([I32], I32),
+    "tq_timing_count": ([], I64),
+    "tq_timing_read": ([P, I64, P, P], I32),
     "tq_filter_mask": ([P, P, I64, P, P, P], I32),
     "tq_mask_tile_counts": ([P, I64, P, P], I32),
     "tq_scan_ws_bytes": ([I64], SZ),
@@ -149,27 +157,58 @@ def load():
 _ERRORS = {TQ_EINVAL: ValueError, TQ_EUNSUPPORTED: NotImplementedError, TQ_EHIP: RuntimeError,
            TQ_ECAPACITY: MemoryError}
 
-# When a list, every call appends (name, algorithmic bytes, start event, end event); the events
-# are recorded on torch's current stream, the stream the kernels are launched on.
+# Kernel timing.  While TRACE is a list, the library stamps every kernel launch on the device
+# (tq_timing) and every call appends (entry point, principal kernel, algorithmic bytes, first,
+# end): launches first..end-1 of the library's timing log are the ones this call made.
 TRACE = None
 
 
-def call(name, *args, alg_bytes=None):
+def call(name, *args, alg_bytes=None, kernel=None):
     lib = load()
     fn = getattr(lib, name)
     if TRACE is None:
         rc = fn(*args)
     else:
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
+        c0 = lib.tq_timing_count()
         rc = fn(*args)
-        e1.record()
-        TRACE.append((name, alg_bytes, e0, e1))
+        TRACE.append((name, kernel, alg_bytes, c0, lib.tq_timing_count()))
     if rc != TQ_OK:
         msg = lib.tq_last_error().decode(errors="replace")
         raise _ERRORS.get(rc, RuntimeError)(f"{name}: {msg}")
     return rc
+
+
+def start_trace():
+    """Clear the timing log and start timing kernel launches."""
+    global TRACE
+    lib = load()
+    torch.cuda.synchronize()
+    _read_log(lib)
+    TRACE = []
+    lib.tq_timing(1)
+
+
+def stop_trace():
+    """Stop timing; returns (calls, launches) where launches is [(kernel name, ms)] in launch
+    order and calls index into it.  Synchronises the device first."""
+    global TRACE
+    lib = load()
+    torch.cuda.synchronize()
+    lib.tq_timing(0)
+    calls, TRACE = TRACE, None
+    return calls, _read_log(lib)
+
+
+def _read_log(lib):
+    cap = max(int(lib.tq_timing_count()), 1)
+    buf = (KTime * cap)()
+    n, dropped = ctypes.c_int64(), ctypes.c_int64()
+    rc = lib.tq_timing_read(buf, cap, ctypes.byref(n), ctypes.byref(dropped))
+    if rc != TQ_OK:
+        raise RuntimeError(lib.tq_last_error().decode(errors="replace"))
+    if dropped.value:
+        raise RuntimeError(f"kernel timing log overflowed ({dropped.value} launches untimed)")
+    return [(buf[i].name.decode(), float(buf[i].ms)) for i in range(n.value)]
 
 
 def size_call(name, *args) -> int:

@@ -43,7 +43,7 @@ def filter_mask(tree, tensors, subquery_result=None, n=None):
     # read each loaded column once, write one mask byte per row
     alg = rows * (1 + b.load_bytes())
     L.call("tq_filter_mask", E.program_ptr(b), E.colset_ptr(b), rows, L.ptr(mask), L.ptr(counts),
-           L.stream_handle(), alg_bytes=alg)
+           L.stream_handle(), alg_bytes=alg, kernel="k_filter")
     return mask, counts
 
 
@@ -89,7 +89,7 @@ def compact(mask: torch.Tensor, columns, tile_counts=None):
         rb = sum(L.row_bytes(c) for c in src)
         L.call("tq_compact", L.ptr(mask), n, L.ptr(offs), len(src), L.ptr_array(src),
                L.int_array([L.row_bytes(c) for c in src]), L.ptr_array(dst), L.stream_handle(),
-               alg_bytes=n + 2 * m * rb)
+               alg_bytes=n + 2 * m * rb, kernel="k_compact")
     return outs
 
 
@@ -179,7 +179,8 @@ class JoinIndex:
         pm = L.ptr(pmask) if pmask is not None else None
         L.call("tq_join_count", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype), pm,
                np_, L.ptr(pws), ctypes.byref(total), L.stream_handle(),
-               alg_bytes=np_ * (pkey.element_size() + (1 if pmask is not None else 0)))
+               alg_bytes=np_ * (pkey.element_size() + (1 if pmask is not None else 0)),
+               kernel="k_join_count")
         m = int(total.value)
         lout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in left_cols]
         rout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in right_cols]
@@ -198,7 +199,8 @@ class JoinIndex:
                    pm, np_, L.ptr(pws), len(lsrc), L.ptr_array(lsrc),
                    L.int_array([L.row_bytes(c) for c in lsrc]), L.ptr_array(lout), len(rsrc),
                    L.ptr_array(rsrc), L.int_array([L.row_bytes(c) for c in rsrc]),
-                   L.ptr_array(rout), L.ptr(bidx), L.ptr(pidx), L.stream_handle(), alg_bytes=alg)
+                   L.ptr_array(rout), L.ptr(bidx), L.ptr(pidx), L.stream_handle(), alg_bytes=alg,
+                   kernel="k_join_write")
         return lout, rout, bidx, pidx
 
 
@@ -284,7 +286,7 @@ class GroupBy:
         # every selected row's loaded columns (keys included) and the mask byte of every row
         alg = nsel * b.load_bytes() + (self.n if self.mask is not None else 0)
         L.call("tq_groupby_accumulate", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr,
-               self.n, mode, slots, L.ptr(ws), wsb.value, s, alg_bytes=alg)
+               self.n, mode, slots, L.ptr(ws), wsb.value, s, alg_bytes=alg, kernel="k_gb_accum")
         ng = ctypes.c_int64()
         L.call("tq_groupby_count", ctypes.byref(d), mode, slots, L.ptr(ws), ctypes.byref(ng), s)
         G = int(ng.value)
