@@ -44,7 +44,30 @@ PREDICATES = [
     [['flag_a#6', '=', '2'], 'OR', ['NOT', ['tax#5', '<=', '0.04']]],
     ['flag_a#6', 'IN', ['0', '2']],
     [['quantity#2', '<', '25'], 'AND', ['ship_day#1', '>=', '1995-01-01']],
+    [['ship_day#1', '>=', '100'], 'AND', ['ship_day#1', '<', '2000']],
+    [['flag_b#7', '=', '1'], 'AND', ['price#3', '<', '60000.0']],
+    ['tax#5', '>=', '0.04'],
+    [[['isnotnull', ['flag_a#6']], 'AND', ['flag_a#6', '>', '0']], 'AND', ['quantity#2', '>=', '10']],
 ]
+
+
+def test_filter_f32_narrow_int_columns(device):
+    """Comparison conjunctions over float32, int16 and bool columns, with a ragged tail."""
+    from tqp_amd import kernels
+    g = np.random.default_rng(11)
+    n = 70_003
+    tables = {1: ('float', g.uniform(-5, 5, n).astype(np.float32)),
+              2: ('int', g.integers(-300, 300, n, dtype=np.int16)),
+              3: ('bool', g.random(n) < 0.5)}
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    for tree in ([['f#1', '<', '1.5'], 'AND', ['s#2', '>=', '-20']],
+                 [['f#1', '>', '-2'], 'AND', [['s#2', '>', '7'], 'AND', ['s#2', '<=', '250']]],
+                 ['s#2', '=', '3']):
+        mask, counts = kernels.filter_mask(tree, gpu, None)
+        want = R.filter_evaluate(cpu, tree, None, None)
+        _assert_col(mask, want, f"mask {tree}")
+        assert int(counts.sum().item()) == int(want.sum().item())
 
 
 @pytest.mark.parametrize("pi", range(len(PREDICATES)))

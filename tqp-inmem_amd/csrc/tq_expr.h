@@ -199,4 +199,161 @@ __device__ __forceinline__ void run_program(const tq_program_t& P, const tq_cols
   }
 }
 
+// The kSub rows of a thread's sub-tile together: one Regs per row (each a set of named scalars
+// selected by switch), reached only with compile-time row indices, so nothing goes to scratch.
+struct V4 {
+  uint64_t v[kSub];
+};
+// One row's eight registers as an 8-lane vector: a register index that is uniform across the
+// wave selects an element with the hardware's register-indexed moves, not through scratch.
+static_assert(kSub == 4, "RegsN holds one register set per sub-tile row");
+// Registers are read and written with selects on the (wave-uniform) register index: plain
+// VALU work, where a switch would let the compiler turn the register file into an indexed
+// scratch array.
+struct RegRow {
+  uint64_t r0, r1, r2, r3, r4, r5, r6, r7;
+  // a mux tree on the index bits
+  __device__ __forceinline__ uint64_t get(int i) const {
+    const bool b0 = i & 1, b1 = i & 2, b2 = i & 4;
+    const uint64_t x01 = b0 ? r1 : r0, x23 = b0 ? r3 : r2;
+    const uint64_t x45 = b0 ? r5 : r4, x67 = b0 ? r7 : r6;
+    const uint64_t x03 = b1 ? x23 : x01, x47 = b1 ? x67 : x45;
+    return b2 ? x47 : x03;
+  }
+  __device__ __forceinline__ void put(int i, uint64_t v) {
+    const uint32_t hot = 1u << (i & 7);
+    r0 = (hot & 1u) ? v : r0;
+    r1 = (hot & 2u) ? v : r1;
+    r2 = (hot & 4u) ? v : r2;
+    r3 = (hot & 8u) ? v : r3;
+    r4 = (hot & 16u) ? v : r4;
+    r5 = (hot & 32u) ? v : r5;
+    r6 = (hot & 64u) ? v : r6;
+    r7 = (hot & 128u) ? v : r7;
+  }
+};
+
+struct RegsN {
+  RegRow q0, q1, q2, q3;
+  __device__ __forceinline__ V4 get(int i) const {
+    const int k = i & 7;
+    V4 x;
+    x.v[0] = q0.get(k);
+    x.v[1] = q1.get(k);
+    x.v[2] = q2.get(k);
+    x.v[3] = q3.get(k);
+    return x;
+  }
+  __device__ __forceinline__ void put(int i, const V4& x) {
+    const int k = i & 7;
+    q0.put(k, x.v[0]);
+    q1.put(k, x.v[1]);
+    q2.put(k, x.v[2]);
+    q3.put(k, x.v[3]);
+  }
+};
+
+// Instructions [begin, end) for all kSub rows of the sub-tile at once: each instruction is
+// fetched and dispatched once and then runs as kSub independent operations.  row[j] is the
+// global row of sub-tile row j; rows with valid[j] == false compute garbage that callers drop.
+__device__ __forceinline__ void run_rows(const tq_program_t& P, const tq_colset_t& C, int begin,
+                                         int end, const int64_t (&row)[kSub],
+                                         const bool (&valid)[kSub],
+                                         const uint64_t* __restrict__ stage, RegsN& R) {
+  const int t = threadIdx.x;
+  for (int pc = begin; pc < end; ++pc) {
+    const tq_instr_t I = P.code[pc];
+    const V4 A = R.get(I.a), B = R.get(I.b);
+    const uint64_t(&a)[kSub] = A.v;
+    const uint64_t(&b)[kSub] = B.v;
+    V4 O;
+    uint64_t(&o)[kSub] = O.v;
+#define TQ_EACH(EXPR) _Pragma("unroll") for (int j = 0; j < kSub; ++j) o[j] = (uint64_t)(EXPR);
+    switch (I.op) {
+      case TQ_OP_LOAD: TQ_EACH(stage[(I.a * kSub + j) * kBlock + t]) break;
+      case TQ_OP_CONST: TQ_EACH(I.imm) break;
+      case TQ_OP_STR_EQ:
+      case TQ_OP_STR_PREFIX: {
+        const int8_t* p = (const int8_t*)col_ptr(C, I.a);
+        const int w = col_width(C, I.a);
+        const int off = (int)(I.imm & 0xffffffff);
+        const int len = (int)(I.imm >> 32);
+        const int upto = (I.op == TQ_OP_STR_EQ) ? w : len;
+#pragma unroll
+        for (int j = 0; j < kSub; ++j) {
+          bool eq = valid[j];
+          if (eq) {
+            const int8_t* sp = p + row[j] * (int64_t)w;
+            for (int c = 0; c < upto; ++c) {
+              const int8_t want = c < len ? (int8_t)P.strpool[off + c] : (int8_t)0;
+              eq = eq && (sp[c] == want);
+            }
+          }
+          o[j] = eq;
+        }
+        break;
+      }
+      case TQ_OP_CVT_I64_F64: TQ_EACH(f64_bits((double)(int64_t)a[j])) break;
+      case TQ_OP_CVT_I64_F32: TQ_EACH(f32_bits((float)(int64_t)a[j])) break;
+      case TQ_OP_CVT_F32_F64: TQ_EACH(f64_bits((double)as_f32(a[j]))) break;
+      case TQ_OP_CVT_F64_F32: TQ_EACH(f32_bits((float)as_f64(a[j]))) break;
+      case TQ_OP_WRAP_I8: TQ_EACH((int64_t)(int8_t)a[j]) break;
+      case TQ_OP_WRAP_I16: TQ_EACH((int64_t)(int16_t)a[j]) break;
+      case TQ_OP_WRAP_I32: TQ_EACH((int64_t)(int32_t)a[j]) break;
+      case TQ_OP_ADD_I64: TQ_EACH(a[j] + b[j]) break;
+      case TQ_OP_SUB_I64: TQ_EACH(a[j] - b[j]) break;
+      case TQ_OP_MUL_I64: TQ_EACH(a[j] * b[j]) break;
+      case TQ_OP_ADD_F64: TQ_EACH(f64_bits(as_f64(a[j]) + as_f64(b[j]))) break;
+      case TQ_OP_SUB_F64: TQ_EACH(f64_bits(as_f64(a[j]) - as_f64(b[j]))) break;
+      case TQ_OP_MUL_F64: TQ_EACH(f64_bits(as_f64(a[j]) * as_f64(b[j]))) break;
+      case TQ_OP_DIV_F64: TQ_EACH(f64_bits(as_f64(a[j]) / as_f64(b[j]))) break;
+      case TQ_OP_ADD_F32: TQ_EACH(f32_bits(as_f32(a[j]) + as_f32(b[j]))) break;
+      case TQ_OP_SUB_F32: TQ_EACH(f32_bits(as_f32(a[j]) - as_f32(b[j]))) break;
+      case TQ_OP_MUL_F32: TQ_EACH(f32_bits(as_f32(a[j]) * as_f32(b[j]))) break;
+      case TQ_OP_DIV_F32: TQ_EACH(f32_bits(as_f32(a[j]) / as_f32(b[j]))) break;
+      case TQ_OP_LT_I64: TQ_EACH((int64_t)a[j] < (int64_t)b[j]) break;
+      case TQ_OP_LE_I64: TQ_EACH((int64_t)a[j] <= (int64_t)b[j]) break;
+      case TQ_OP_GT_I64: TQ_EACH((int64_t)a[j] > (int64_t)b[j]) break;
+      case TQ_OP_GE_I64: TQ_EACH((int64_t)a[j] >= (int64_t)b[j]) break;
+      case TQ_OP_EQ_I64: TQ_EACH(a[j] == b[j]) break;
+      case TQ_OP_NE_I64: TQ_EACH(a[j] != b[j]) break;
+      case TQ_OP_LT_F64: TQ_EACH(as_f64(a[j]) < as_f64(b[j])) break;
+      case TQ_OP_LE_F64: TQ_EACH(as_f64(a[j]) <= as_f64(b[j])) break;
+      case TQ_OP_GT_F64: TQ_EACH(as_f64(a[j]) > as_f64(b[j])) break;
+      case TQ_OP_GE_F64: TQ_EACH(as_f64(a[j]) >= as_f64(b[j])) break;
+      case TQ_OP_EQ_F64: TQ_EACH(as_f64(a[j]) == as_f64(b[j])) break;
+      case TQ_OP_NE_F64: TQ_EACH(as_f64(a[j]) != as_f64(b[j])) break;
+      case TQ_OP_LT_F32: TQ_EACH(as_f32(a[j]) < as_f32(b[j])) break;
+      case TQ_OP_LE_F32: TQ_EACH(as_f32(a[j]) <= as_f32(b[j])) break;
+      case TQ_OP_GT_F32: TQ_EACH(as_f32(a[j]) > as_f32(b[j])) break;
+      case TQ_OP_GE_F32: TQ_EACH(as_f32(a[j]) >= as_f32(b[j])) break;
+      case TQ_OP_EQ_F32: TQ_EACH(as_f32(a[j]) == as_f32(b[j])) break;
+      case TQ_OP_NE_F32: TQ_EACH(as_f32(a[j]) != as_f32(b[j])) break;
+      case TQ_OP_ISCLOSE_F64: {
+        // torch.isclose: a == b, or |a-b| finite and <= atol + |rtol * b| with rtol = 1e-5
+        const double atol = __longlong_as_double(I.imm);
+#pragma unroll
+        for (int j = 0; j < kSub; ++j) {
+          const double x = as_f64(a[j]), y = as_f64(b[j]);
+          const double err = fabs(x - y);
+          o[j] = (x == y) || (isfinite(err) && err <= atol + fabs(1e-5 * y));
+        }
+        break;
+      }
+      case TQ_OP_AND: TQ_EACH((a[j] != 0) && (b[j] != 0)) break;
+      case TQ_OP_OR: TQ_EACH((a[j] != 0) || (b[j] != 0)) break;
+      case TQ_OP_NOT: TQ_EACH(a[j] == 0) break;
+      case TQ_OP_WHERE: {
+        const V4 Cv = R.get(I.c);
+        TQ_EACH(Cv.v[j] ? a[j] : b[j])
+        break;
+      }
+      case TQ_OP_YEAR: TQ_EACH(year_of_day((int64_t)a[j])) break;
+      default: TQ_EACH(0) break;
+    }
+#undef TQ_EACH
+    R.put(I.dst, O);
+  }
+}
+
 }  // namespace tq

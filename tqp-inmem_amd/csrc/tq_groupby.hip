@@ -143,24 +143,6 @@ __device__ __forceinline__ uint64_t pack_key(const GbSpec& G, const uint64_t* st
   return p;
 }
 
-// Accumulate one row (sub-tile row j) into slot `slot` of a table whose accumulators live at
-// acc[j * stride + slot].
-__device__ __forceinline__ void accumulate_row(const GbSpec& G, const tq_program_t& P,
-                                               const tq_colset_t& C, int j, int64_t row,
-                                               const uint64_t* stage, uint32_t* cnt,
-                                               uint64_t* acc, int64_t stride, int64_t slot) {
-  atomicAdd(&cnt[slot], 1u);
-  Regs R{};
-  int begin = 0;
-  for (int a = 0; a < G.naggs; ++a) {
-    const int end = P.seg_end[a];
-    run_program(P, C, begin, end, j, row, true, stage, R);
-    begin = end;
-    const uint64_t v = to_acc(G.kind[a], G.arg_dtype[a], R.get(P.seg_result[a]));
-    acc_apply(&acc[(int64_t)a * stride + slot], G.kind[a], v);
-  }
-}
-
 template <int BASE, bool FRONT>
 __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, tq_colset_t C,
                                                      const uint8_t* __restrict__ mask, int64_t n,
@@ -193,13 +175,21 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
       live[j] = r < n && (!mask || mask[r] != 0);
     }
     stage_loads(P, C, sub0, live, stage);
-#pragma unroll 1
+    // the table slot of every live row: LDS (dense-LDS tables and the hash front) or global
+    uint64_t* abase[kSub];
+    int64_t astride[kSub], aslot[kSub];
+#pragma unroll
     for (int j = 0; j < kSub; ++j) {
+      abase[j] = T.acc;
+      astride[j] = T.slots;
+      aslot[j] = 0;
       if (!live[j]) continue;
-      const int64_t row = sub0 + (int64_t)j * kBlock + t;
       const uint64_t p = pack_key(G, stage, j);
       if (BASE == TQ_GB_DENSE_LDS) {
-        accumulate_row(G, P, C, j, row, stage, lcnt, lacc, cap, (int64_t)p);
+        abase[j] = lacc;
+        astride[j] = cap;
+        aslot[j] = (int64_t)p;
+        atomicAdd(&lcnt[p], 1u);
         continue;
       }
       if (FRONT) {
@@ -218,12 +208,33 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
           h = (h + 1) & m;
         }
         if (ls >= 0) {
-          accumulate_row(G, P, C, j, row, stage, lcnt, lacc, cap, ls);
+          abase[j] = lacc;
+          astride[j] = cap;
+          aslot[j] = ls;
+          atomicAdd(&lcnt[ls], 1u);
           continue;
         }
       }
-      const int64_t gs = global_slot(BASE, T, p);
-      accumulate_row(G, P, C, j, row, stage, T.cnt, T.acc, T.slots, gs);
+      aslot[j] = global_slot(BASE, T, p);
+      atomicAdd(&T.cnt[aslot[j]], 1u);
+    }
+    // every aggregate argument for the sub-tile's rows at once, then one atomic per live row
+    int64_t rows[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) rows[j] = sub0 + (int64_t)j * kBlock + t;
+    RegsN R{};
+    int begin = 0;
+    for (int a = 0; a < G.naggs; ++a) {
+      const int end = P.seg_end[a];
+      run_rows(P, C, begin, end, rows, live, stage, R);
+      begin = end;
+      const V4 V = R.get(P.seg_result[a]);
+      const uint64_t* v = V.v;
+#pragma unroll
+      for (int j = 0; j < kSub; ++j)
+        if (live[j])
+          acc_apply(&abase[j][(int64_t)a * astride[j] + aslot[j]], G.kind[a],
+                    to_acc(G.kind[a], G.arg_dtype[a], v[j]));
     }
   }
   if (use_lds) {

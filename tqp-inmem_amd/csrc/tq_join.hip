@@ -374,16 +374,81 @@ __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* _
   }
 }
 
+// Output column x of the pairs listed in LDS: element i of the tile's output (at obase + i)
+// takes src row from_build[i] (build side) or tile0 + from_probe[i] (probe side).  Consecutive
+// threads write consecutive outputs.
+template <typename T, bool PROBE>
+__device__ __forceinline__ void gather_list(void* dst, const void* src, int64_t obase,
+                                            uint32_t total, const uint32_t* from_build,
+                                            const uint16_t* from_probe, int64_t tile0) {
+  for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
+    const int64_t from = PROBE ? tile0 + from_probe[i] : (int64_t)from_build[i];
+    ((T*)dst)[obase + i] = ((const T*)src)[from];
+  }
+}
+
+template <bool PROBE>
+__device__ __forceinline__ void gather_col(void* dst, const void* src, int bytes, int64_t obase,
+                                           uint32_t total, const uint32_t* fb, const uint16_t* fp,
+                                           int64_t tile0) {
+  switch (bytes) {
+    case 1: gather_list<uint8_t, PROBE>(dst, src, obase, total, fb, fp, tile0); break;
+    case 2: gather_list<uint16_t, PROBE>(dst, src, obase, total, fb, fp, tile0); break;
+    case 4: gather_list<uint32_t, PROBE>(dst, src, obase, total, fb, fp, tile0); break;
+    case 8: gather_list<uint64_t, PROBE>(dst, src, obase, total, fb, fp, tile0); break;
+    default:
+      for (uint32_t i = threadIdx.x; i < total; i += kBlock)
+        copy_row(dst, obase + i, src, PROBE ? tile0 + fp[i] : (int64_t)fb[i], bytes);
+  }
+}
+
 template <typename K, bool HASH, bool UNIQUE>
 __global__ __launch_bounds__(kBlock) void k_join_write(
     JoinView J, const void* __restrict__ pkeys, int pdt, const uint8_t* __restrict__ pmask,
     int64_t np, int64_t ntiles, const int64_t* __restrict__ tile_off, ColPack L, ColPack Rc,
     int64_t* __restrict__ build_idx, int64_t* __restrict__ probe_idx) {
   __shared__ uint32_t sh[kWaves];
+  __shared__ uint32_t fb[UNIQUE ? kTileRows : 1];  // build row of the tile's i-th pair
+  __shared__ uint16_t fp[UNIQUE ? kTileRows : 1];  // probe row - tile0 of the i-th pair
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t c[kItems], a[kItems];
     probe_rows<K, HASH, UNIQUE>(J, pkeys, pdt, pmask, np, tile, c, a);
     int64_t base = tile_off[tile];
+    if (UNIQUE) {
+      // at most one pair per probe row: list the tile's pairs in output order in LDS, then
+      // write each output column with consecutive threads on consecutive outputs
+      uint32_t run = 0;
+#pragma unroll
+      for (int q = 0; q < kRuns; ++q) {
+        uint32_t h4 = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h4 |= c[4 * q + i] << i;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_sum((uint32_t)__popc(h4), sh, &tot);
+        uint32_t o = run + ex;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if ((h4 >> i) & 1u) {
+            fb[o] = a[4 * q + i];
+            fp[o] = (uint16_t)(q * (4 * kBlock) + 4 * threadIdx.x + i);
+            ++o;
+          }
+        }
+        run += tot;
+      }
+      __syncthreads();
+      const int64_t tile0 = tile * kTileRows;
+      for (int x = 0; x < L.n; ++x)
+        gather_col<false>(L.dst[x], L.src[x], L.bytes[x], base, run, fb, fp, tile0);
+      for (int x = 0; x < Rc.n; ++x)
+        gather_col<true>(Rc.dst[x], Rc.src[x], Rc.bytes[x], base, run, fb, fp, tile0);
+      for (uint32_t i = threadIdx.x; i < run; i += kBlock) {
+        if (build_idx) build_idx[base + i] = fb[i];
+        if (probe_idx) probe_idx[base + i] = tile0 + fp[i];
+      }
+      __syncthreads();
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < kRuns; ++q) {
       const uint32_t tc = c[4 * q] + c[4 * q + 1] + c[4 * q + 2] + c[4 * q + 3];
@@ -395,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_join_write(
         const int j = 4 * q + i;
         const int64_t r = tile_row(tile, j);
         for (uint32_t k = 0; k < c[j]; ++k, ++o) {
-          const uint32_t b = UNIQUE ? a[j] : J.rows[a[j] + k];
+          const uint32_t b = J.rows[a[j] + k];
           for (int x = 0; x < L.n; ++x) copy_row(L.dst[x], o, L.src[x], b, L.bytes[x]);
           for (int x = 0; x < Rc.n; ++x) copy_row(Rc.dst[x], o, Rc.src[x], r, Rc.bytes[x]);
           if (build_idx) build_idx[o] = b;
