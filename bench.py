@@ -27,8 +27,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-runs", type=int, default=6, help="pipelines timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the config-3 batched comparison")
@@ -158,9 +158,16 @@ def main():
     base = workloads.to_tensor_group(tables, dev)
     rows = workloads.input_rows(tables, 1)
 
+    # the executor edits plan arguments in place (as main.py's operators do), so every step gets
+    # its own copy of the plan, made before the timed region
+    copies = []
+
+    def fresh_plans(k):
+        copies[:] = [copy.deepcopy(plan) for _ in range(k)]
+
     def step(op_sync=args.op_sync):
         tg = dict(base)
-        executor.execute(copy.deepcopy(plan), tg, sync=op_sync)
+        executor.execute(copies.pop() if copies else copy.deepcopy(plan), tg, sync=op_sync)
         return tg
 
     if args.calibrate_copy:
@@ -172,6 +179,8 @@ def main():
         step()
     torch.cuda.synchronize()
     # timed region: the whole pipeline, kernels untimed
+    fresh_plans(args.steps)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
@@ -180,6 +189,7 @@ def main():
     # the same steps again with every kernel launch carrying device-stamped start/stop events
     # (the kernel durations and achieved bandwidths), then with the reference executor's
     # per-operator synchronisation (main.py:273-274), for comparison
+    fresh_plans(args.steps)
     _lib.start_trace()
     t1 = time.perf_counter()
     for _ in range(args.steps):
@@ -191,6 +201,8 @@ def main():
     with_bytes = {k: v for k, v in kernels.items() if "GBps" in v}
     dom = max(with_bytes, key=lambda k: with_bytes[k]["total_ms"])
     d = with_bytes[dom]
+    fresh_plans(args.steps)
+    torch.cuda.synchronize()
     t2 = time.perf_counter()
     for _ in range(args.steps):
         step(True)

@@ -234,6 +234,7 @@ typedef struct {
   uint64_t empty_key;    /* hash layout: key value of a free slot */
   int64_t range;         /* dense: table entries; hash: slots */
   const void* table;     /* device directory or slots */
+  const uint32_t* bits;  /* dense layout: presence bit per table entry (bit e%32 of word e/32) */
   const uint32_t* rows;  /* device: build rows grouped by key (unique == 0) */
 } tq_join_state_t;
 
@@ -286,6 +287,9 @@ int tq_join_write(const tq_join_state_t* state /* host */, const void* pkeys, in
 #define TQ_GB_DENSE_LDS 0    /* whole key space in one LDS array per workgroup, then merged  */
 #define TQ_GB_DENSE_GLOBAL 1 /* key space addressed directly in one global array            */
 #define TQ_GB_HASH 2         /* global open-addressing hash table on the packed key         */
+#define TQ_GB_DENSE_PART 3   /* dense key space in partitions of 2^k slots: rows are scattered
+                                by partition, then one workgroup reduces each partition in LDS
+                                and stores its slots (no global atomics)                     */
 #define TQ_GB_LDS_FRONT 8    /* flag: an LDS hash table absorbs repeated keys first         */
 
 typedef struct {
@@ -298,12 +302,17 @@ typedef struct {
   int64_t key_stride[TQ_GB_MAX_KEYS]; /* mixed-radix weights, filled by tq_groupby_plan    */
   int32_t agg_op[TQ_GB_MAX_AGGS];     /* TQ_AGG_*                                          */
   int32_t agg_dtype[TQ_GB_MAX_AGGS];  /* dtype of the argument segment's value            */
+  int64_t nsel;                       /* selected rows; set by tq_groupby_plan             */
 } tq_groupby_desc_t;
 
 /* Pass 1: per-key-column min and max over selected rows (mask may be NULL) and the selected row
- * count.  out (device): int64[2*nkeys + 1] = {min0, max0, min1, max1, ..., nsel}. */
+ * count.  out (device): int64[2*nkeys + 1] = {min0, max0, min1, max1, ..., nsel} (min > max
+ * when no row is selected).  ws: tq_groupby_bounds_ws_bytes(n) bytes of per-tile partials.
+ * Replaces the key stacking of operators/aggregate.py:297-312 (unsqueeze/torch.cat of the
+ * key columns that precedes the hashed grouping). */
+size_t tq_groupby_bounds_ws_bytes(int64_t n);
 int tq_groupby_bounds(const tq_groupby_desc_t* desc /* host */, const uint8_t* mask, int64_t n,
-                      int64_t* out, void* stream);
+                      int64_t* out, void* ws, void* stream);
 
 /* Host planning from the bounds: fills key_stride, picks the table mode and slot count and
  * returns the workspace size.  The packed key is sum((k_i - min_i) * stride_i), so packed order

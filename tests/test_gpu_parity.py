@@ -305,6 +305,24 @@ def test_aggregate_parity(device, case):
         _assert_col(gpu[a].tensor, cpu[a].tensor, f"{case} agg {a}", fp=True)
 
 
+def test_aggregate_many_columns_global_table(device):
+    """Eight loaded columns fill the LDS stage, so the dense key space takes the global table
+    with atomics rather than LDS partitions; 600k rows over a 250k-key range."""
+    g = np.random.default_rng(77)
+    n = 600_000
+    k = {10: ('int', g.integers(0, 250_000, n, dtype=np.int32))}
+    vals = {20 + i: ('float', g.uniform(1, 100, n)) for i in range(6)}
+    vals[26] = ('int', g.integers(-9, 9, n, dtype=np.int64))
+    results = ["k#10"] + [[[['sum', [f'v#{20 + i}']], '#', str(i)], 'AS', f's#{50 + i}']
+                          for i in range(6)] + [[[['max', ['v#26']], '#', '6'], 'AS', 'm#56']]
+    agg_ids = [50, 51, 52, 53, 54, 55, 56]
+    gpu, cpu, errs = _agg_case(device, k, vals, None, results, [10], agg_ids)
+    assert errs == [None, None]
+    _assert_col(gpu[10].tensor, cpu[10].tensor, "wide key")
+    for a in agg_ids:
+        _assert_col(gpu[a].tensor, cpu[a].tensor, f"wide agg {a}", fp=True)
+
+
 # ------------------------------------------------------------------ full pipelines
 def _run_pipeline(device, tables, plan):
     from tqp_amd import executor

@@ -39,49 +39,42 @@ __device__ __forceinline__ int64_t global_slot(int base_mode, const GbTable& T, 
 }
 
 // ------------------------------------------------------------------ bounds
-__global__ __launch_bounds__(kBlock) void k_gb_init_bounds(int64_t* out, int nkeys) {
-  const int i = threadIdx.x;
-  if (i < nkeys) {
-    out[2 * i] = INT64_MAX;
-    out[2 * i + 1] = INT64_MIN;
-  }
-  if (i == 0) out[2 * nkeys] = 0;
-}
-
 struct KeyCols {
   const void* ptr[TQ_GB_MAX_KEYS];
   int32_t dtype[TQ_GB_MAX_KEYS];
   int32_t nkeys;
 };
 
-// Selected-row count and per-key min/max in one pass: each thread reads runs of 4 consecutive
-// rows (vector loads), workgroups reduce through LDS, and thread 0 of each workgroup commits
-// with one atomic per value (the grid is kept small, so the atomics do not contend).
+constexpr int kBoundWords = 1 + 2 * TQ_GB_MAX_KEYS;  // per tile: count, then (min, max) per key
+
+// Per 4096-row tile: selected rows and each key's min and max -> part[tile * kBoundWords ..].
+// A thread reads four runs of four consecutive rows (vector loads, all in flight together).
 __global__ __launch_bounds__(kBlock) void k_gb_bounds(KeyCols K, const uint8_t* __restrict__ mask,
-                                                      int64_t n, int64_t* __restrict__ out) {
-  __shared__ int64_t sh[2 * TQ_GB_MAX_KEYS + 1][kWaves];
-  const int64_t stride = (int64_t)gridDim.x * kBlock * 4;
-  int64_t cnt = 0;
+                                                      int64_t n, int64_t* __restrict__ part) {
+  __shared__ int64_t sh[kBoundWords][kWaves];
+  const int64_t tile = blockIdx.x;
+  const int t = threadIdx.x;
+  uint32_t sel = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    sel |= load_sel4(mask, tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * t, n) << (4 * q);
+  int64_t cnt = __popc(sel);
   int64_t lo[TQ_GB_MAX_KEYS], hi[TQ_GB_MAX_KEYS];
 #pragma unroll
   for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
     lo[k] = INT64_MAX;
     hi[k] = INT64_MIN;
-  }
-  for (int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4; r < n; r += stride) {
-    const uint32_t sel = load_sel4(mask, r, n);
-    cnt += __popc(sel);
+    if (k >= K.nkeys) continue;
+    int64_t v[kItems];
 #pragma unroll
-    for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
-      if (k >= K.nkeys) break;
-      int64_t v[4];
-      load_int4(K.ptr[k], K.dtype[k], r, n, v);
+    for (int q = 0; q < 4; ++q)
+      load_int4(K.ptr[k], K.dtype[k], tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * t, n,
+                &v[4 * q]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if ((sel >> i) & 1u) {
-          lo[k] = v[i] < lo[k] ? v[i] : lo[k];
-          hi[k] = v[i] > hi[k] ? v[i] : hi[k];
-        }
+    for (int j = 0; j < kItems; ++j) {
+      if ((sel >> j) & 1u) {
+        lo[k] = v[j] < lo[k] ? v[j] : lo[k];
+        hi[k] = v[j] > hi[k] ? v[j] : hi[k];
       }
     }
   }
@@ -90,12 +83,12 @@ __global__ __launch_bounds__(kBlock) void k_gb_bounds(KeyCols K, const uint8_t* 
   for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
-      const int64_t a = __shfl_xor(lo[k], d, 64), b = __shfl_xor(hi[k], d, 64);
-      lo[k] = a < lo[k] ? a : lo[k];
-      hi[k] = b > hi[k] ? b : hi[k];
+      const int64_t x = __shfl_xor(lo[k], d, 64), y = __shfl_xor(hi[k], d, 64);
+      lo[k] = x < lo[k] ? x : lo[k];
+      hi[k] = y > hi[k] ? y : hi[k];
     }
   }
-  const int w = threadIdx.x >> 6;
+  const int w = t >> 6;
   if (lane_id() == 0) {
     sh[0][w] = cnt;
 #pragma unroll
@@ -105,22 +98,61 @@ __global__ __launch_bounds__(kBlock) void k_gb_bounds(KeyCols K, const uint8_t* 
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (t < kBoundWords) {
+    int64_t r = sh[t][0];
     for (int i = 1; i < kWaves; ++i) {
-      cnt += sh[0][i];
+      const int64_t x = sh[t][i];
+      r = t == 0 ? r + x : ((t & 1) ? (x < r ? x : r) : (x > r ? x : r));
+    }
+    part[tile * kBoundWords + t] = r;
+  }
+}
+
+// One workgroup: out = {min0, max0, ..., min_{k-1}, max_{k-1}, selected rows}.
+__global__ __launch_bounds__(kBlock) void k_gb_bounds_final(const int64_t* __restrict__ part,
+                                                            int64_t ntiles, int nkeys,
+                                                            int64_t* __restrict__ out) {
+  __shared__ int64_t sh[kBoundWords][kWaves];
+  int64_t acc[kBoundWords];
 #pragma unroll
-      for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
-        lo[k] = sh[1 + 2 * k][i] < lo[k] ? sh[1 + 2 * k][i] : lo[k];
-        hi[k] = sh[2 + 2 * k][i] > hi[k] ? sh[2 + 2 * k][i] : hi[k];
+  for (int i = 0; i < kBoundWords; ++i) acc[i] = i == 0 ? 0 : ((i & 1) ? INT64_MAX : INT64_MIN);
+  for (int64_t tl = threadIdx.x; tl < ntiles; tl += kBlock) {
+#pragma unroll
+    for (int i = 0; i < kBoundWords; ++i) {
+      const int64_t x = part[tl * kBoundWords + i];
+      acc[i] = i == 0 ? acc[i] + x : ((i & 1) ? (x < acc[i] ? x : acc[i]) : (x > acc[i] ? x : acc[i]));
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kBoundWords; ++i) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int64_t x = __shfl_xor(acc[i], d, 64);
+      acc[i] = i == 0 ? acc[i] + x : ((i & 1) ? (x < acc[i] ? x : acc[i]) : (x > acc[i] ? x : acc[i]));
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int i = 0; i < kBoundWords; ++i) sh[i][w] = acc[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 1; v < kWaves; ++v) {
+#pragma unroll
+      for (int i = 0; i < kBoundWords; ++i) {
+        const int64_t x = sh[i][v];
+        acc[i] = i == 0 ? acc[i] + x : ((i & 1) ? (x < acc[i] ? x : acc[i]) : (x > acc[i] ? x : acc[i]));
       }
     }
-    if (cnt) {
-      atomicAdd((unsigned long long*)&out[2 * K.nkeys], (unsigned long long)cnt);
-      for (int k = 0; k < K.nkeys; ++k) {
-        atomicMin((long long*)&out[2 * k], (long long)lo[k]);
-        atomicMax((long long*)&out[2 * k + 1], (long long)hi[k]);
+#pragma unroll
+    for (int k = 0; k < TQ_GB_MAX_KEYS; ++k) {
+      if (k < nkeys) {
+        out[2 * k] = acc[1 + 2 * k];
+        out[2 * k + 1] = acc[2 + 2 * k];
       }
     }
+    out[2 * nkeys] = acc[0];
   }
 }
 
@@ -251,6 +283,152 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
   }
 }
 
+// ------------------------------------------------------------------ partitioned dense tables
+// Packed keys of a thread's 16 rows of a 4096-row tile (four runs of four consecutive rows) and
+// their selection bits.
+__device__ __forceinline__ uint32_t tile_packed_keys(const KeyCols& K, const GbSpec& G,
+                                                     const uint8_t* mask, int64_t n, int64_t tile,
+                                                     uint64_t (&p)[kItems]) {
+  uint32_t sel = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    sel |= load_sel4(mask, tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * threadIdx.x, n)
+           << (4 * q);
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) p[j] = 0;
+  for (int k = 0; k < G.nkeys; ++k) {
+    int64_t v[kItems];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      load_int4(K.ptr[k], K.dtype[k], tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * threadIdx.x,
+                n, &v[4 * q]);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      p[j] += (uint64_t)(v[j] - G.key_min[k]) * (uint64_t)G.key_stride[k];
+  }
+  return sel;
+}
+
+// Rows per partition: an LDS histogram per tile, added to part_cnt once per non-empty partition.
+__global__ __launch_bounds__(kBlock) void k_gbp_count(KeyCols K, GbSpec G,
+                                                      const uint8_t* __restrict__ mask, int64_t n,
+                                                      int pb, int nparts,
+                                                      uint32_t* __restrict__ part_cnt) {
+  extern __shared__ uint32_t lh[];
+  for (int i = threadIdx.x; i < nparts; i += kBlock) lh[i] = 0;
+  __syncthreads();
+  uint64_t p[kItems];
+  const uint32_t sel = tile_packed_keys(K, G, mask, n, blockIdx.x, p);
+#pragma unroll
+  for (int j = 0; j < kItems; ++j)
+    if ((sel >> j) & 1u) atomicAdd(&lh[p[j] >> pb], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nparts; i += kBlock)
+    if (lh[i]) atomicAdd(&part_cnt[i], lh[i]);
+}
+
+// Scatter a tile's rows to their partitions: the tile reserves one range per partition it
+// touches, then every row takes the next position of its range and stores its slot within the
+// partition and its aggregate arguments (evaluated here, in the accumulator domain).
+__global__ __launch_bounds__(kBlock) void k_gbp_scatter(
+    GbSpec G, tq_program_t P, tq_colset_t C, KeyCols K, const uint8_t* __restrict__ mask,
+    int64_t n, int pb, int nparts, const int64_t* __restrict__ part_off,
+    uint32_t* __restrict__ part_cur, uint32_t* __restrict__ rec_key,
+    uint64_t* __restrict__ rec_val, int64_t cap, int32_t stage_words) {
+  extern __shared__ uint64_t smem[];
+  uint64_t* stage = smem;
+  int64_t* lbase = (int64_t*)(smem + stage_words);
+  uint32_t* lh = (uint32_t*)(lbase + nparts);
+  const int t = threadIdx.x;
+  const int64_t tile = blockIdx.x;
+  for (int i = t; i < nparts; i += kBlock) lh[i] = 0;
+  __syncthreads();
+  {
+    uint64_t p[kItems];
+    const uint32_t sel = tile_packed_keys(K, G, mask, n, tile, p);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      if ((sel >> j) & 1u) atomicAdd(&lh[p[j] >> pb], 1u);
+  }
+  __syncthreads();
+  for (int i = t; i < nparts; i += kBlock) {
+    const uint32_t c = lh[i];
+    if (c) lbase[i] = part_off[i] + atomicAdd(&part_cur[i], c);
+    lh[i] = 0;
+  }
+  __syncthreads();
+  const uint64_t pmask = ((uint64_t)1 << pb) - 1;
+  for (int sub = 0; sub < kTileRows / kSubRows; ++sub) {
+    const int64_t sub0 = tile * kTileRows + (int64_t)sub * kSubRows;
+    bool live[kSub];
+    int64_t rows[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      rows[j] = sub0 + (int64_t)j * kBlock + t;
+      live[j] = rows[j] < n && (!mask || mask[rows[j]] != 0);
+    }
+    stage_loads(P, C, sub0, live, stage);
+    int64_t pos[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      pos[j] = 0;
+      if (!live[j]) continue;
+      const uint64_t pk = pack_key(G, stage, j);
+      const uint64_t b = pk >> pb;
+      pos[j] = lbase[b] + atomicAdd(&lh[b], 1u);
+      rec_key[pos[j]] = (uint32_t)(pk & pmask);
+    }
+    RegsN R{};
+    int begin = 0;
+    for (int a = 0; a < G.naggs; ++a) {
+      const int end = P.seg_end[a];
+      run_rows(P, C, begin, end, rows, live, stage, R);
+      begin = end;
+      const V4 V = R.get(P.seg_result[a]);
+#pragma unroll
+      for (int j = 0; j < kSub; ++j)
+        if (live[j]) rec_val[(int64_t)a * cap + pos[j]] = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+    }
+  }
+}
+
+// One workgroup per partition: reduce its rows into an LDS table, then store every slot of the
+// partition (empty ones included) into the dense global table.
+__global__ __launch_bounds__(kBlock) void k_gbp_reduce(GbSpec G, GbTable T, int pb,
+                                                       const int64_t* __restrict__ part_off,
+                                                       const uint32_t* __restrict__ part_cnt,
+                                                       const uint32_t* __restrict__ rec_key,
+                                                       const uint64_t* __restrict__ rec_val,
+                                                       int64_t cap) {
+  extern __shared__ uint64_t smem[];
+  const int ps = 1 << pb;
+  uint64_t* lacc = smem;                                   // naggs * ps
+  uint32_t* lcnt = (uint32_t*)(smem + (int64_t)G.naggs * ps);
+  const int t = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  for (int i = t; i < ps; i += kBlock) {
+    lcnt[i] = 0;
+    for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
+  }
+  __syncthreads();
+  const int64_t r0 = part_off[b];
+  const uint32_t m = part_cnt[b];
+  for (uint32_t r = t; r < m; r += kBlock) {
+    const uint32_t slot = rec_key[r0 + r];
+    atomicAdd(&lcnt[slot], 1u);
+    for (int a = 0; a < G.naggs; ++a)
+      acc_apply(&lacc[a * ps + slot], G.kind[a], rec_val[(int64_t)a * cap + r0 + r]);
+  }
+  __syncthreads();
+  const int64_t g0 = b << pb;
+  for (int i = t; i < ps; i += kBlock) {
+    const int64_t g = g0 + i;
+    if (g >= T.slots) break;
+    T.cnt[g] = lcnt[i];
+    for (int a = 0; a < G.naggs; ++a) T.acc[(int64_t)a * T.slots + g] = lacc[a * ps + i];
+  }
+}
+
 // ------------------------------------------------------------------ count / write
 __global__ __launch_bounds__(kBlock) void k_gb_occupied(const uint32_t* __restrict__ cnt,
                                                         int64_t slots, int force0,
@@ -280,36 +458,48 @@ struct OutCols {
   int32_t op[TQ_GB_MAX_AGGS];
 };
 
+// Key k of packed key p (32-bit division when the packed space allows it).
+__device__ __forceinline__ int64_t key_of(const GbSpec& G, int k, uint64_t p) {
+  if (G.nkeys == 1) return (int64_t)p + G.key_min[0];
+  uint64_t q;
+  if (p <= 0xFFFFFFFFull && G.key_stride[k] <= 0xFFFFFFFFll && G.key_range[k] <= 0xFFFFFFFFll)
+    q = ((uint32_t)p / (uint32_t)G.key_stride[k]) % (uint32_t)G.key_range[k];
+  else
+    q = (p / (uint64_t)G.key_stride[k]) % (uint64_t)G.key_range[k];
+  return (int64_t)q + G.key_min[k];
+}
+
 __device__ __forceinline__ void write_group(const GbSpec& G, const OutCols& O, const GbTable& T,
                                             int64_t o, int64_t slot, uint64_t p) {
-  for (int k = 0; k < G.nkeys; ++k) {
-    const int64_t v = (int64_t)((p / (uint64_t)G.key_stride[k]) % (uint64_t)G.key_range[k]) + G.key_min[k];
-    store_int(O.key[k], G.key_dtype[k], o, v);
-  }
+  for (int k = 0; k < G.nkeys; ++k) store_int(O.key[k], G.key_dtype[k], o, key_of(G, k, p));
   const uint32_t rows = T.cnt[slot];
   for (int a = 0; a < G.naggs; ++a)
     write_agg(O.agg[a], o, O.op[a], G.kind[a], G.arg_dtype[a], T.acc[(int64_t)a * T.slots + slot], rows);
 }
 
-// Dense tables: groups are the occupied slots in slot (= key) order.
+// Dense tables: groups are the occupied slots in slot (= key) order.  A thread ranks its 16
+// slots of a 4096-slot tile first, then writes each output column with all 16 loads in flight.
 __global__ __launch_bounds__(kBlock) void k_gb_write_dense(GbSpec G, OutCols O, GbTable T,
                                                            const int64_t* __restrict__ tile_off,
                                                            int force0) {
   __shared__ uint32_t wcnt[kItems][kWaves];
   const int t = threadIdx.x, w = t >> 6, l = lane_id();
   const int64_t tile0 = (int64_t)blockIdx.x * kTileRows;
-  bool f[kItems];
-  uint64_t bal[kItems];
+  uint32_t cnt[kItems], rel[kItems];
+  uint32_t occ = 0;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int64_t s = tile0 + (int64_t)j * kBlock + t;
-    f[j] = s < T.slots && ((T.cnt[s] > 0) || (force0 && s == 0));
-    bal[j] = ballot(f[j]);
-    if (l == 0) wcnt[j][w] = (uint32_t)__popcll(bal[j]);
+    cnt[j] = s < T.slots ? T.cnt[s] : 0u;
+    const bool f = s < T.slots && (cnt[j] > 0 || (force0 && s == 0));
+    const uint64_t bal = ballot(f);
+    if (l == 0) wcnt[j][w] = (uint32_t)__popcll(bal);
+    rel[j] = (uint32_t)rank_below(bal);
+    occ |= (uint32_t)f << j;
   }
   __syncthreads();
-  int64_t run = tile_off[blockIdx.x];
-#pragma unroll 1
+  uint32_t run = 0;
+#pragma unroll
   for (int j = 0; j < kItems; ++j) {
     uint32_t before = 0, tot = 0;
 #pragma unroll
@@ -318,11 +508,27 @@ __global__ __launch_bounds__(kBlock) void k_gb_write_dense(GbSpec G, OutCols O, 
       tot += c;
       before += (v < w) ? c : 0;
     }
-    if (f[j]) {
-      const int64_t s = tile0 + (int64_t)j * kBlock + t;
-      write_group(G, O, T, run + before + rank_below(bal[j]), s, (uint64_t)s);
-    }
+    rel[j] += run + before;
     run += tot;
+  }
+  const int64_t base = tile_off[blockIdx.x];
+  for (int k = 0; k < G.nkeys; ++k) {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      if ((occ >> j) & 1u)
+        store_int(O.key[k], G.key_dtype[k], base + rel[j],
+                  key_of(G, k, (uint64_t)(tile0 + (int64_t)j * kBlock + t)));
+  }
+  for (int a = 0; a < G.naggs; ++a) {
+    const uint64_t* acc = T.acc + (int64_t)a * T.slots;
+    uint64_t v[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      v[j] = ((occ >> j) & 1u) ? acc[tile0 + (int64_t)j * kBlock + t] : 0ull;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      if ((occ >> j) & 1u)
+        write_agg(O.agg[a], base + rel[j], O.op[a], G.kind[a], G.arg_dtype[a], v[j], cnt[j]);
   }
 }
 
@@ -387,6 +593,15 @@ struct GbWs {
   uint32_t* v1;
   void* stats;
   void* radix;
+  // partitioned dense mode
+  int32_t part_bits, nparts;
+  uint32_t* part_cnt;  // rows per partition, then (after it) the scatter cursors
+  uint32_t* part_cur;
+  int64_t* part_off;
+  int64_t* part_scan_ws;
+  uint32_t* rec_key;   // slot within the partition, per scattered row
+  uint64_t* rec_val;   // rec_val[a * rows_cap + i]: aggregate a's argument (accumulator domain)
+  int64_t rows_cap;
   size_t bytes;
 };
 
@@ -396,7 +611,18 @@ static int ilog2_ceil(int64_t v) {
   return l;
 }
 
-static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs) {
+// Partitions of the partitioned dense mode: the largest power-of-two slot count whose LDS table
+// (u32 count + one u64 per aggregate per slot) fits kPartLds.
+constexpr size_t kPartLds = 48 * 1024;
+constexpr int kPartMax = 2048;
+static int part_bits_for(int naggs) {
+  const size_t per = 4 + 8 * (size_t)(naggs > 0 ? naggs : 1);
+  int b = 0;
+  while (((size_t)2 << b) * per <= kPartLds) ++b;
+  return b;
+}
+
+static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t rows_cap) {
   const int bm = mode & 7;
   char* p = (char*)base;
   const int64_t nt = ceil_div(slots, kTileRows);
@@ -417,6 +643,18 @@ static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs) {
     w.v1 = (uint32_t*)p; p += align_up((size_t)slots * 4, 256);
     w.stats = p; p += 256;
     w.radix = p; p += radix_ws_bytes(slots);
+  }
+  if (bm == TQ_GB_DENSE_PART) {
+    const int na = naggs > 0 ? naggs : 1;
+    w.part_bits = part_bits_for(naggs);
+    w.nparts = (int32_t)ceil_div(slots, (int64_t)1 << w.part_bits);
+    w.rows_cap = rows_cap > 0 ? rows_cap : 1;
+    w.part_cnt = (uint32_t*)p;
+    w.part_cur = w.part_cnt + w.nparts; p += align_up((size_t)w.nparts * 8, 256);
+    w.part_off = (int64_t*)p; p += align_up((size_t)w.nparts * 8, 256);
+    w.part_scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(w.nparts);
+    w.rec_key = (uint32_t*)p; p += align_up((size_t)w.rows_cap * 4, 256);
+    w.rec_val = (uint64_t*)p; p += align_up((size_t)w.rows_cap * 8 * na, 256);
   }
   w.bytes = (size_t)(p - (char*)base);
   return w;
@@ -483,9 +721,15 @@ static int32_t lds_cap(int mode, int32_t nload, int naggs, int64_t slots) {
 
 using namespace tq;
 
+extern "C" size_t tq_groupby_bounds_ws_bytes(int64_t n) {
+  return align_up((size_t)ceil_div(n > 0 ? n : 1, kTileRows) * kBoundWords * sizeof(int64_t), 256);
+}
+
 extern "C" int tq_groupby_bounds(const tq_groupby_desc_t* d, const uint8_t* mask, int64_t n,
-                                 int64_t* out, void* stream) {
-  if (!d || d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS) return set_error(TQ_EINVAL, "group-by bounds: bad descriptor");
+                                 int64_t* out, void* ws, void* stream) {
+  if (!d || d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS)
+    return set_error(TQ_EINVAL, "group-by bounds: bad descriptor");
+  if (!out || (n > 0 && !ws)) return set_error(TQ_EINVAL, "group-by bounds: null argument");
   hipStream_t s = (hipStream_t)stream;
   KeyCols K{};
   K.nkeys = d->nkeys;
@@ -495,11 +739,12 @@ extern "C" int tq_groupby_bounds(const tq_groupby_desc_t* d, const uint8_t* mask
     if (K.dtype[k] < TQ_BOOL || K.dtype[k] > TQ_I64)
       return set_error(TQ_EUNSUPPORTED, "group-by key %d must be an integer column", k);
   }
-  TQ_TRY(launch("k_gb_init_bounds", k_gb_init_bounds, 1, kBlock, 0, s, out, d->nkeys));
-  if (n <= 0) return TQ_OK;
-  int64_t g = ceil_div(n, kBlock * 16);
-  if (g > 512) g = 512;
-  return launch("k_gb_bounds", k_gb_bounds, (unsigned)g, kBlock, 0, s, K, mask, n, out);
+  const int64_t nt = n > 0 ? ceil_div(n, kTileRows) : 0;
+  if (nt > 0)
+    TQ_TRY(launch("k_gb_bounds", k_gb_bounds, (unsigned)nt, kBlock, 0, s, K, mask, n,
+                  (int64_t*)ws));
+  return launch("k_gb_bounds_final", k_gb_bounds_final, 1, kBlock, 0, s, (const int64_t*)ws, nt,
+                (int)d->nkeys, out);
 }
 
 extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload, int32_t* mode,
@@ -517,6 +762,7 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
   }
   const int64_t S = (int64_t)space;
   const int64_t ns = nsel > 0 ? nsel : 0;
+  d->nsel = ns;
   const size_t stage = lds_stage_bytes(nload);
   const size_t dense_per = 4 + 8 * (size_t)d->naggs;
   int32_t m;
@@ -526,7 +772,12 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
     sl = S;
   } else {
     const int64_t distinct = ns < S ? ns : S;
-    if (S <= ((int64_t)1 << 27) && S <= (4 * ns > 65536 ? 4 * ns : 65536)) {
+    const int64_t nparts = ceil_div(S, (int64_t)1 << part_bits_for(d->naggs));
+    const bool dense_ok = S <= ((int64_t)1 << 27) && S <= (4 * ns > 65536 ? 4 * ns : 65536);
+    if (dense_ok && nparts <= kPartMax && stage + (size_t)nparts * 12 <= (size_t)kLdsBudget) {
+      m = TQ_GB_DENSE_PART;
+      sl = S;
+    } else if (dense_ok) {
       m = TQ_GB_DENSE_GLOBAL;
       sl = S;
     } else {
@@ -535,11 +786,11 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
     }
     const size_t per = 12 + 8 * (size_t)d->naggs;
     const bool room = stage + 1024 * per <= (size_t)kLdsBudget;
-    if (room && distinct > 0 && ns >= 4 * distinct) m |= TQ_GB_LDS_FRONT;
+    if (m != TQ_GB_DENSE_PART && room && distinct > 0 && ns >= 4 * distinct) m |= TQ_GB_LDS_FRONT;
   }
   *mode = m;
   *slots = sl;
-  *ws_bytes = carve_gb(nullptr, m, sl, d->naggs).bytes;
+  *ws_bytes = carve_gb(nullptr, m, sl, d->naggs, ns).bytes;
   return TQ_OK;
 }
 
@@ -551,11 +802,37 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
   GbSpec G;
   TQ_TRY(make_spec(d, prog, &G));
   if (prog->nload > TQ_MAX_LOADS || prog->n_instr > TQ_MAX_INSTR) return set_error(TQ_EINVAL, "group-by: malformed program");
-  GbWs w = carve_gb(ws, mode, slots, d->naggs);
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel);
   if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "group-by workspace too small");
   hipStream_t s = (hipStream_t)stream;
   const int base = mode & 7;
   const bool front = (mode & TQ_GB_LDS_FRONT) != 0;
+  if (base == TQ_GB_DENSE_PART && n > 0 && d->nsel > 0) {
+    KeyCols K{};
+    K.nkeys = d->nkeys;
+    for (int k = 0; k < d->nkeys; ++k) {
+      K.ptr[k] = d->key_ptr[k];
+      K.dtype[k] = d->key_dtype[k];
+    }
+    const int64_t nt = ceil_div(n, kTileRows);
+    const int np = w.nparts;
+    const int32_t stage_words = (int32_t)(lds_stage_bytes(prog->nload) / 8);
+    const size_t lds_scatter = lds_stage_bytes(prog->nload) + (size_t)np * 12;
+    const size_t lds_reduce = ((size_t)1 << w.part_bits) * (4 + 8 * (size_t)d->naggs);
+    if (lds_scatter > (size_t)kLdsBudget || lds_reduce > (size_t)kLdsBudget)
+      return set_error(TQ_ECAPACITY, "group-by: partition tables exceed LDS");
+    TQ_TRY(fill_u32(w.part_cnt, 0u, 2 * (int64_t)np, s));  // counts and cursors
+    TQ_TRY(launch("k_gbp_count", k_gbp_count, (unsigned)nt, kBlock, (size_t)np * 4, s, K, G, mask,
+                  n, w.part_bits, np, w.part_cnt));
+    TQ_TRY((scan_exclusive<uint32_t, int64_t>(w.part_cnt, np, w.part_off, nullptr, w.part_scan_ws,
+                                              s)));
+    TQ_TRY(launch("k_gbp_scatter", k_gbp_scatter, (unsigned)nt, kBlock, lds_scatter, s, G, *prog,
+                  *cols, K, mask, n, w.part_bits, np, (const int64_t*)w.part_off, w.part_cur,
+                  w.rec_key, w.rec_val, w.rows_cap, stage_words));
+    return launch("k_gbp_reduce", k_gbp_reduce, (unsigned)np, kBlock, lds_reduce, s, G, w.T,
+                  w.part_bits, (const int64_t*)w.part_off, (const uint32_t*)w.part_cnt,
+                  (const uint32_t*)w.rec_key, (const uint64_t*)w.rec_val, w.rows_cap);
+  }
   TQ_TRY(launch("k_gb_clear", k_gb_clear, stream_grid(slots), kBlock, 0, s, w.T, G, base == TQ_GB_HASH));
   if (n <= 0) return TQ_OK;
   const int32_t cap = lds_cap(mode, prog->nload, d->naggs, slots);
@@ -581,7 +858,7 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
 extern "C" int tq_groupby_count(const tq_groupby_desc_t* d, int32_t mode, int64_t slots, void* ws,
                                 int64_t* ngroups_host, void* stream) {
   if (!d) return set_error(TQ_EINVAL, "group-by count: null descriptor");
-  GbWs w = carve_gb(ws, mode, slots, d->naggs);
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel);
   hipStream_t s = (hipStream_t)stream;
   const int64_t nt = ceil_div(slots, kTileRows);
   TQ_TRY(launch("k_gb_occupied", k_gb_occupied, (unsigned)nt, kBlock, 0, s, w.T.cnt, slots, d->nkeys == 0, w.tile_counts));
@@ -596,7 +873,7 @@ extern "C" int tq_groupby_write(const tq_groupby_desc_t* d, int32_t mode, int64_
   if (!d) return set_error(TQ_EINVAL, "group-by write: null descriptor");
   GbSpec G;
   TQ_TRY(make_spec(d, nullptr, &G));
-  GbWs w = carve_gb(ws, mode, slots, d->naggs);
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel);
   hipStream_t s = (hipStream_t)stream;
   if (ngroups <= 0) return TQ_OK;
   OutCols O{};

@@ -7,9 +7,10 @@
 //   * dense: a directory indexed by key - min, when the keys span at most 4*nb + 65536 values;
 //   * hash: open addressing with linear probing over a power-of-two slot array of at least
 //     twice the selected rows, multiplicative (Fibonacci) hashing.
-// The build first assumes unique keys: every selected row claims its key's entry with one
-// compare-and-swap and a second claim of a key raises a flag, which the host reads (second
-// synchronisation).  Without duplicates an entry holds its build row and the build is done,
+// The build first assumes unique keys: every selected row claims its key's entry (a plain store
+// into the dense directory followed by a pass in which every row checks that it owns its entry;
+// a compare-and-swap on the hash slot's key) and a key claimed twice raises a flag, which the
+// host reads (second synchronisation).  Without duplicates an entry holds its build row and the build is done,
 // with no sort.  With duplicates the selected (key, row) pairs are compacted in row order and
 // radix-sorted by key (stable, so each key's rows stay ascending); an entry then holds the
 // first sorted position of its key and the number of rows with that key.
@@ -37,6 +38,7 @@ struct Slot {
 
 struct JoinView {
   const void* table;
+  const uint32_t* bits;  // dense layouts: presence bit per directory entry
   const uint32_t* rows;
   uint64_t base;
   uint64_t empty;
@@ -61,22 +63,26 @@ __device__ __forceinline__ bool probe_fits(int64_t v) {
 }
 
 // Number of build rows with key t (0 when !valid) and, through *a, the entry's row/position.
-template <typename K, bool HASH, bool UNIQUE>
+// Dense layouts test the presence bitmap first (range/8 bytes, cache resident) and read the
+// directory only for present keys, and only when the caller needs the row (NEED_ROW) or the
+// count of a repeated key.
+template <typename K, bool HASH, bool UNIQUE, bool NEED_ROW>
 __device__ __forceinline__ uint32_t lookup(const JoinView& J, K t, bool valid, uint32_t* a) {
+  *a = 0;
   if (!HASH) {
     const uint64_t off = (uint64_t)t - J.base;  // below the base wraps to a huge value
     valid = valid && off < (uint64_t)J.range;
-    const uint64_t idx = valid ? off : 0;
+    const uint32_t word = J.bits[valid ? (off >> 5) : 0];
+    const bool present = valid && ((word >> (off & 31)) & 1u);
     if (UNIQUE) {
-      const uint32_t e = ((const uint32_t*)J.table)[idx];
-      *a = e;
-      return (valid && e != kNoPos) ? 1u : 0u;
+      if (NEED_ROW && present) *a = ((const uint32_t*)J.table)[off];
+      return present ? 1u : 0u;
     }
-    const uint2 e = ((const uint2*)J.table)[idx];
+    if (!present) return 0;
+    const uint2 e = ((const uint2*)J.table)[off];
     *a = e.x;
-    return valid ? e.y : 0u;
+    return e.y;
   }
-  *a = 0;
   if (!valid) return 0;
   const Slot* S = (const Slot*)J.table;
   const uint64_t mask = (uint64_t)J.range - 1;
@@ -222,7 +228,9 @@ __global__ __launch_bounds__(kBlock) void k_join_clear(void* __restrict__ table,
   }
 }
 
-// Every selected build row claims its key's entry; a key claimed twice sets *dup.
+// Every selected build row claims its key's entry; a key claimed twice sets *dup.  Dense
+// directories take plain stores here (one of several rows with the same key wins) and
+// k_join_verify then flags any row that does not find itself in its entry.
 template <typename K, bool HASH>
 __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ bkeys, int dt,
                                                        const uint8_t* __restrict__ mask,
@@ -239,8 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ 
     const uint64_t t = (uint64_t)order_key<K>(v[j]);
     const uint32_t row = (uint32_t)tile_row(tile, j);
     if (!HASH) {
-      const uint32_t old = atomicCAS((uint32_t*)J.table + (t - J.base), kNoPos, row);
-      seen |= old != kNoPos;
+      ((uint32_t*)J.table)[t - J.base] = row;
     } else {
       Slot* S = (Slot*)J.table;
       const uint64_t m = (uint64_t)J.range - 1;
@@ -261,6 +268,47 @@ __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ 
     }
   }
   if (seen) *dup = 1;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_join_verify(const void* __restrict__ bkeys, int dt,
+                                                        const uint8_t* __restrict__ mask,
+                                                        int64_t n, JoinView J,
+                                                        uint32_t* __restrict__ dup) {
+  const int64_t tile = blockIdx.x;
+  uint32_t sel;
+  int64_t v[kItems];
+  load_tile_rows(bkeys, dt, mask, n, tile, &sel, v);
+  uint32_t e[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint64_t off = ((sel >> j) & 1u) ? (uint64_t)order_key<K>(v[j]) - J.base : 0;
+    e[j] = ((const uint32_t*)J.table)[off];
+  }
+  bool seen = false;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j)
+    seen |= ((sel >> j) & 1u) && e[j] != (uint32_t)tile_row(tile, j);
+  if (seen) *dup = 1;
+}
+
+// Presence bitmap of a dense directory: bit i of bits[w] is set when entry 32*w + i holds a key.
+// A wave tests 64 consecutive entries and its ballot is two bitmap words.
+template <bool UNIQUE>
+__global__ __launch_bounds__(kBlock) void k_join_bits(const void* __restrict__ table,
+                                                      int64_t range, uint32_t* __restrict__ bits) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const int64_t nw = (range + 31) >> 5;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; (e & ~63ll) < range; e += stride) {
+    const bool present = e < range && (UNIQUE ? ((const uint32_t*)table)[e] != kNoPos
+                                              : ((const uint2*)table)[e].y != 0u);
+    const uint64_t m = ballot(present);
+    if (lane_id() == 0) {
+      const int64_t w = e >> 5;
+      bits[w] = (uint32_t)m;
+      if (w + 1 < nw) bits[w + 1] = (uint32_t)(m >> 32);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ build: duplicate keys
@@ -335,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_join_runs(const K* __restrict__ keys
 }
 
 // ------------------------------------------------------------------ probe
-template <typename K, bool HASH, bool UNIQUE>
+template <typename K, bool HASH, bool UNIQUE, bool NEED_ROW>
 __device__ __forceinline__ void probe_rows(const JoinView& J, const void* pkeys, int pdt,
                                            const uint8_t* pmask, int64_t np, int64_t tile,
                                            uint32_t c[kItems], uint32_t a[kItems]) {
@@ -345,7 +393,7 @@ __device__ __forceinline__ void probe_rows(const JoinView& J, const void* pkeys,
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const bool ok = ((sel >> j) & 1u) && probe_fits<K>(v[j]);
-    c[j] = lookup<K, HASH, UNIQUE>(J, order_key<K>(v[j]), ok, &a[j]);
+    c[j] = lookup<K, HASH, UNIQUE, NEED_ROW>(J, order_key<K>(v[j]), ok, &a[j]);
   }
 }
 
@@ -357,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* _
   __shared__ uint32_t sh[kWaves];
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t c[kItems], a[kItems];
-    probe_rows<K, HASH, UNIQUE>(J, pkeys, pdt, pmask, np, tile, c, a);
+    probe_rows<K, HASH, UNIQUE, false>(J, pkeys, pdt, pmask, np, tile, c, a);
     uint32_t cnt = 0;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) cnt += c[j];
@@ -402,7 +450,7 @@ __device__ __forceinline__ void gather_col(void* dst, const void* src, int bytes
   }
 }
 
-template <typename K, bool HASH, bool UNIQUE>
+template <typename K, bool HASH, bool UNIQUE, bool NEED_ROW>
 __global__ __launch_bounds__(kBlock) void k_join_write(
     JoinView J, const void* __restrict__ pkeys, int pdt, const uint8_t* __restrict__ pmask,
     int64_t np, int64_t ntiles, const int64_t* __restrict__ tile_off, ColPack L, ColPack Rc,
@@ -412,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_join_write(
   __shared__ uint16_t fp[UNIQUE ? kTileRows : 1];  // probe row - tile0 of the i-th pair
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t c[kItems], a[kItems];
-    probe_rows<K, HASH, UNIQUE>(J, pkeys, pdt, pmask, np, tile, c, a);
+    probe_rows<K, HASH, UNIQUE, NEED_ROW || !UNIQUE>(J, pkeys, pdt, pmask, np, tile, c, a);
     int64_t base = tile_off[tile];
     if (UNIQUE) {
       // at most one pair per probe row: list the tile's pairs in output order in LDS, then
@@ -494,6 +542,7 @@ struct BuildWs {
   uint32_t* v1;
   void* radix;
   void* table;
+  uint32_t* bits;
   size_t bytes;
 };
 
@@ -517,6 +566,7 @@ static BuildWs carve_build(void* base, int64_t nb) {
   w.v1 = (uint32_t*)p; p += align_up((size_t)nb * 4, 256);
   w.radix = p; p += radix_ws_bytes(nb);
   w.table = p; p += align_up(dense_bytes > hash_bytes ? dense_bytes : hash_bytes, 256);
+  w.bits = (uint32_t*)p; p += align_up((size_t)ceil_div(dense_cap(nb), 32) * 4, 256);
   w.bytes = (size_t)(p - (char*)base);
   return w;
 }
@@ -545,6 +595,7 @@ static ProbeWs carve_probe(void* base, int64_t np) {
 static JoinView view_of(const tq_join_state_t* st) {
   JoinView J;
   J.table = st->table;
+  J.bits = st->bits;
   J.rows = st->rows;
   J.base = st->base;
   J.empty = st->empty_key;
@@ -602,13 +653,21 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
   if (hash)
     TQ_TRY(launch("k_join_claim", (k_join_claim<K, true>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
                   bmask, nb, J, w.dup));
-  else
+  else {
     TQ_TRY(launch("k_join_claim", (k_join_claim<K, false>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
                   bmask, nb, J, w.dup));
+    TQ_TRY(launch("k_join_verify", k_join_verify<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask,
+                  nb, J, w.dup));
+  }
   int64_t dup = 0;
   TQ_TRY(read_back(w.dup, 4, &dup, s));  // second synchronisation: were the keys unique?
   if ((dup & 0xFFFFFFFF) == 0) {
     st->unique = 1;
+    if (!hash) {
+      st->bits = w.bits;
+      TQ_TRY(launch("k_join_bits", k_join_bits<true>, stream_grid(st->range), kBlock,
+                    0, s, (const void*)w.table, st->range, w.bits));
+    }
     return TQ_OK;
   }
   // duplicate keys: compact, sort stably by key, record {first position, count} per key
@@ -632,6 +691,9 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
     TQ_TRY(fill_u64(w.table, 0ull, st->range, s));  // {first, count} = {0, 0}: absent
     TQ_TRY(launch("k_join_runs", (k_join_runs<K, false>), stream_grid(nsel), kBlock, 0, s, keys,
                   nsel, J));
+    st->bits = w.bits;
+    TQ_TRY(launch("k_join_bits", k_join_bits<false>, stream_grid(st->range), kBlock,
+                  0, s, (const void*)w.table, st->range, w.bits));
   }
   return TQ_OK;
 }
@@ -648,8 +710,12 @@ static int write_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
                       const uint8_t* pmask, int64_t np, int64_t nt, const ProbeWs& w,
                       const ColPack& L, const ColPack& R, int64_t* bidx, int64_t* pidx,
                       hipStream_t s) {
-  return launch("k_join_write", (k_join_write<K, HASH, UNIQUE>), probe_grid(nt), kBlock, 0, s,
-                view_of(st), pkeys, pdt, pmask, np, nt, (const int64_t*)w.tile_off, L, R, bidx,
+  if (L.n > 0 || bidx)
+    return launch("k_join_write", (k_join_write<K, HASH, UNIQUE, true>), probe_grid(nt), kBlock,
+                  0, s, view_of(st), pkeys, pdt, pmask, np, nt, (const int64_t*)w.tile_off, L, R,
+                  bidx, pidx);
+  return launch("k_join_write", (k_join_write<K, HASH, UNIQUE, false>), probe_grid(nt), kBlock, 0,
+                s, view_of(st), pkeys, pdt, pmask, np, nt, (const int64_t*)w.tile_off, L, R, bidx,
                 pidx);
 }
 

@@ -348,8 +348,10 @@ __global__ __launch_bounds__(kBlock) void k_topk_select(const uint64_t* __restri
     for (int j = 0; j < kTopPer; ++j) {
       const bool cand = (key[j] & mk) == pk && (row[j] & mr) == pr;
       const uint32_t dg = digit_of(key[j], row[j], d);
-      // lanes holding the same digit add once, through their lowest lane
+      // lanes holding the same digit add once, through their lowest lane; waves without a
+      // candidate (most of them once the first digits are fixed) skip the matching
       uint64_t peers = ballot(cand);
+      if (peers == 0) continue;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
         const uint64_t m = ballot(cand && ((dg >> b) & 1u));

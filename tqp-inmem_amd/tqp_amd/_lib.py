@@ -44,7 +44,7 @@ OP = {name: i for i, name in enumerate(OPCODES)}
 
 TQ_AGG_SUM, TQ_AGG_COUNT, TQ_AGG_AVG, TQ_AGG_MIN, TQ_AGG_MAX = range(5)
 TQ_GB_MAX_KEYS = 8
-TQ_GB_DENSE_LDS, TQ_GB_DENSE_GLOBAL, TQ_GB_HASH, TQ_GB_LDS_FRONT = 0, 1, 2, 8
+TQ_GB_DENSE_LDS, TQ_GB_DENSE_GLOBAL, TQ_GB_HASH, TQ_GB_DENSE_PART, TQ_GB_LDS_FRONT = 0, 1, 2, 3, 8
 
 DTYPE_CODE = {
     torch.bool: TQ_BOOL, torch.int8: TQ_I8, torch.int16: TQ_I16, torch.int32: TQ_I32,
@@ -89,7 +89,7 @@ class JoinState(ctypes.Structure):
                 ("layout", ctypes.c_int32), ("unique", ctypes.c_int32),
                 ("hash_bits", ctypes.c_int32), ("base", ctypes.c_uint64),
                 ("empty_key", ctypes.c_uint64), ("range", ctypes.c_int64),
-                ("table", ctypes.c_void_p), ("rows", ctypes.c_void_p)]
+                ("table", ctypes.c_void_p), ("bits", ctypes.c_void_p), ("rows", ctypes.c_void_p)]
 
 
 class GroupByDesc(ctypes.Structure):
@@ -100,7 +100,8 @@ class GroupByDesc(ctypes.Structure):
                 ("key_range", ctypes.c_int64 * TQ_GB_MAX_KEYS),
                 ("key_stride", ctypes.c_int64 * TQ_GB_MAX_KEYS),
                 ("agg_op", ctypes.c_int32 * TQ_MAX_SEGS),
-                ("agg_dtype", ctypes.c_int32 * TQ_MAX_SEGS)]
+                ("agg_dtype", ctypes.c_int32 * TQ_MAX_SEGS),
+                ("nsel", ctypes.c_int64)]
 
 
 # ---------------------------------------------------------------- loading
@@ -129,7 +130,8 @@ _SIGNATURES = {
     "tq_join_probe_ws_bytes": ([I64], SZ),
     "tq_join_count": ([P, P, I32, P, I64, P, P, P], I32),
     "tq_join_write": ([P, P, I32, P, I64, P, I32, P, P, P, I32, P, P, P, P, P, P], I32),
-    "tq_groupby_bounds": ([P, P, I64, P, P], I32),
+    "tq_groupby_bounds_ws_bytes": ([I64], SZ),
+    "tq_groupby_bounds": ([P, P, I64, P, P, P], I32),
     "tq_groupby_plan": ([P, I64, I32, P, P, P], I32),
     "tq_groupby_accumulate": ([P, P, P, P, I64, I32, I64, P, SZ, P], I32),
     "tq_groupby_count": ([P, I32, I64, P, P, P], I32),

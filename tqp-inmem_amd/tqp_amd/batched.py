@@ -183,7 +183,9 @@ def _key_bounds(t: torch.Tensor):
     d.key_dtype[0] = L.dtype_code(t.dtype)
     d.key_ptr[0] = t.data_ptr()
     out = torch.empty(3, dtype=torch.int64, device=t.device)
-    L.call("tq_groupby_bounds", ctypes.byref(d), None, t.shape[0], L.ptr(out), L.stream_handle())
+    ws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", t.shape[0]), t.device)
+    L.call("tq_groupby_bounds", ctypes.byref(d), None, t.shape[0], L.ptr(out), L.ptr(ws),
+           L.stream_handle())
     lo, hi, _ = out.cpu().tolist()
     return lo, hi
 

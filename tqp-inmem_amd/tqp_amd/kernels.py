@@ -268,7 +268,8 @@ class GroupBy:
         s = L.stream_handle()
         mptr = L.ptr(self.mask) if self.mask is not None else None
         bounds = torch.empty(2 * len(self.keys) + 1, dtype=torch.int64, device=dev)
-        L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), s)
+        bws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", self.n), dev)
+        L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), L.ptr(bws), s)
         hb = bounds.cpu().tolist()
         nsel = hb[-1]
         self.nsel = nsel

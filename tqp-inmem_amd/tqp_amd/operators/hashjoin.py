@@ -99,11 +99,16 @@ def join_kernel_new(gpu_enable, tensors_left, tensors_right, left_key, right_key
 
     fast = type == 'inner' and not condition_list and not return_global_index
     if fast:
-        lsrc = [tensors_left[k].tensor for k in left_keys_out]
-        rsrc = [tensors_right[k].tensor for k in right_keys_out]
+        # a pair's build key equals its probe key, so an output of the build key column of the
+        # probe key's dtype is copied from the probe side (no build-row lookup needed for it)
+        same = left.dtype == right.dtype and left.dim() == right.dim() == 1
+        via_probe = [k for k in left_keys_out if k == left_key and same]
+        via_build = [k for k in left_keys_out if k not in via_probe]
+        lsrc = [tensors_left[k].tensor for k in via_build]
+        rsrc = [tensors_right[k].tensor for k in right_keys_out] + [right for _ in via_probe]
         lout, rout, _, _ = index.probe(right, pmask, lsrc, rsrc)
-        got = dict(zip(left_keys_out, lout))
-        got.update(zip(right_keys_out, rout))
+        got = dict(zip(via_build, lout))
+        got.update(zip(right_keys_out + via_probe, rout))
         lne = left_nonempty()
         rne = right_nonempty() if lout and lout[0].shape[0] == 0 or rout and rout[0].shape[0] == 0 else True
         result = []
