@@ -236,15 +236,25 @@ typedef struct {
   const void* table;     /* device directory or slots */
   const uint32_t* bits;  /* dense layout: presence bit per table entry (bit e%32 of word e/32) */
   const uint32_t* rows;  /* device: build rows grouped by key (unique == 0) */
+  /* Build inputs, kept while `pending`: the build assumed unique keys without waiting for its
+   * duplicate check; the first tq_join_count reads the check together with its pair count and,
+   * for repeated keys, rebuilds the index (sorted layout) before counting again. */
+  int32_t pending;
+  int32_t bdtype;
+  const void* bkeys;
+  const uint8_t* bmask;
+  int64_t nb;
+  void* ws;
+  uint64_t key_or, key_and; /* OR / AND of the selected transformed keys */
 } tq_join_state_t;
 
 /* Workspace for building over nb build rows (room for either layout and for sorting). */
 size_t tq_join_build_ws_bytes(int64_t nb);
 
 /* Build the index over the selected build rows (bmask may be NULL).  Keys are integer columns
- * (TQ_BOOL..TQ_I64).  Synchronises the stream twice (the row count and key range pick the
- * layout; a flag tells whether the keys were unique) and fills *state; ws must outlive the
- * probe passes.  Replaces operators/hashjoin.py:41-52 (arange, torch.sort, unique_consecutive,
+ * (TQ_BOOL..TQ_I64).  Synchronises the stream once (the row count and key range pick the
+ * layout) and fills *state; bkeys, bmask and ws must stay valid until the probe passes are
+ * done.  Replaces operators/hashjoin.py:41-52 (arange, torch.sort, unique_consecutive,
  * cumsum) and the build-side mask compaction of IO/vortex_pipeline.py:58. */
 int tq_join_build(const void* bkeys, int32_t bdtype, const uint8_t* bmask, int64_t nb, void* ws,
                   size_t ws_bytes, tq_join_state_t* state /* host out */, void* stream);
@@ -255,9 +265,11 @@ size_t tq_join_probe_ws_bytes(int64_t np);
 /* Probe, count pass: number of (build, probe) pairs with equal keys among selected probe rows
  * (pmask may be NULL), per probe tile and in total.  The total is left in device memory inside
  * pws and, when total_host is non-NULL, also returned there (which synchronises the stream).
+ * While the state is `pending`, the call always synchronises: it settles the build's
+ * uniqueness (updating *state) before its count is final.
  * Replaces operators/hashjoin.py:57-98 (searchsorted, eq, nonzero, lenR.max(), masked_select,
  * repeat_interleave) up to the pair count. */
-int tq_join_count(const tq_join_state_t* state /* host */, const void* pkeys, int32_t pdtype,
+int tq_join_count(tq_join_state_t* state /* host, in/out */, const void* pkeys, int32_t pdtype,
                   const uint8_t* pmask, int64_t np, void* pws, int64_t* total_host,
                   void* stream);
 
@@ -288,8 +300,8 @@ int tq_join_write(const tq_join_state_t* state /* host */, const void* pkeys, in
 #define TQ_GB_DENSE_GLOBAL 1 /* key space addressed directly in one global array            */
 #define TQ_GB_HASH 2         /* global open-addressing hash table on the packed key         */
 #define TQ_GB_DENSE_PART 3   /* dense key space in partitions of 2^k slots: rows are scattered
-                                by partition, then one workgroup reduces each partition in LDS
-                                and stores its slots (no global atomics)                     */
+                                by partition, one workgroup reduces each partition in LDS and
+                                keeps its groups in key order (no global atomics)             */
 #define TQ_GB_LDS_FRONT 8    /* flag: an LDS hash table absorbs repeated keys first         */
 
 typedef struct {
@@ -303,6 +315,7 @@ typedef struct {
   int32_t agg_op[TQ_GB_MAX_AGGS];     /* TQ_AGG_*                                          */
   int32_t agg_dtype[TQ_GB_MAX_AGGS];  /* dtype of the argument segment's value            */
   int64_t nsel;                       /* selected rows; set by tq_groupby_plan             */
+  int64_t nrows;                      /* rows the accumulate pass reads (set before plan)  */
 } tq_groupby_desc_t;
 
 /* Pass 1: per-key-column min and max over selected rows (mask may be NULL) and the selected row

@@ -48,7 +48,7 @@ def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(_lib.Program) == 64 + 64 * 16 + 128
     assert ctypes.sizeof(_lib.ColSet) == 16 * 8 + 16 * 4
     assert ctypes.sizeof(_lib.SortKey) == 24
-    assert ctypes.sizeof(_lib.JoinState) == 72
+    assert ctypes.sizeof(_lib.JoinState) == 128
     assert ctypes.sizeof(_lib.KTime) == 16
 
 
@@ -71,9 +71,9 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         "tq_program_t": (_lib.Program, ["nload", "code", "strpool"]),
         "tq_colset_t": (_lib.ColSet, ["width"]),
         "tq_sortkey_t": (_lib.SortKey, ["descending"]),
-        "tq_join_state_t": (_lib.JoinState, ["layout", "base", "range", "bits", "rows"]),
+        "tq_join_state_t": (_lib.JoinState, ["layout", "base", "range", "bits", "rows", "pending", "nb", "key_and"]),
         "tq_ktime_t": (_lib.KTime, ["ms"]),
-        "tq_groupby_desc_t": (_lib.GroupByDesc, ["key_ptr", "key_stride", "agg_dtype", "nsel"]),
+        "tq_groupby_desc_t": (_lib.GroupByDesc, ["key_ptr", "key_stride", "agg_dtype", "nsel", "nrows"]),
         "tq_batch_pipe_t": (batched.BatchPipe, ["pprog", "acols", "dir", "agg_out", "ntiles",
                                                 "gkey_min", "gkey_out"]),
     }

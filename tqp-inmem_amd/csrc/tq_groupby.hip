@@ -283,152 +283,6 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
   }
 }
 
-// ------------------------------------------------------------------ partitioned dense tables
-// Packed keys of a thread's 16 rows of a 4096-row tile (four runs of four consecutive rows) and
-// their selection bits.
-__device__ __forceinline__ uint32_t tile_packed_keys(const KeyCols& K, const GbSpec& G,
-                                                     const uint8_t* mask, int64_t n, int64_t tile,
-                                                     uint64_t (&p)[kItems]) {
-  uint32_t sel = 0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    sel |= load_sel4(mask, tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * threadIdx.x, n)
-           << (4 * q);
-#pragma unroll
-  for (int j = 0; j < kItems; ++j) p[j] = 0;
-  for (int k = 0; k < G.nkeys; ++k) {
-    int64_t v[kItems];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      load_int4(K.ptr[k], K.dtype[k], tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * threadIdx.x,
-                n, &v[4 * q]);
-#pragma unroll
-    for (int j = 0; j < kItems; ++j)
-      p[j] += (uint64_t)(v[j] - G.key_min[k]) * (uint64_t)G.key_stride[k];
-  }
-  return sel;
-}
-
-// Rows per partition: an LDS histogram per tile, added to part_cnt once per non-empty partition.
-__global__ __launch_bounds__(kBlock) void k_gbp_count(KeyCols K, GbSpec G,
-                                                      const uint8_t* __restrict__ mask, int64_t n,
-                                                      int pb, int nparts,
-                                                      uint32_t* __restrict__ part_cnt) {
-  extern __shared__ uint32_t lh[];
-  for (int i = threadIdx.x; i < nparts; i += kBlock) lh[i] = 0;
-  __syncthreads();
-  uint64_t p[kItems];
-  const uint32_t sel = tile_packed_keys(K, G, mask, n, blockIdx.x, p);
-#pragma unroll
-  for (int j = 0; j < kItems; ++j)
-    if ((sel >> j) & 1u) atomicAdd(&lh[p[j] >> pb], 1u);
-  __syncthreads();
-  for (int i = threadIdx.x; i < nparts; i += kBlock)
-    if (lh[i]) atomicAdd(&part_cnt[i], lh[i]);
-}
-
-// Scatter a tile's rows to their partitions: the tile reserves one range per partition it
-// touches, then every row takes the next position of its range and stores its slot within the
-// partition and its aggregate arguments (evaluated here, in the accumulator domain).
-__global__ __launch_bounds__(kBlock) void k_gbp_scatter(
-    GbSpec G, tq_program_t P, tq_colset_t C, KeyCols K, const uint8_t* __restrict__ mask,
-    int64_t n, int pb, int nparts, const int64_t* __restrict__ part_off,
-    uint32_t* __restrict__ part_cur, uint32_t* __restrict__ rec_key,
-    uint64_t* __restrict__ rec_val, int64_t cap, int32_t stage_words) {
-  extern __shared__ uint64_t smem[];
-  uint64_t* stage = smem;
-  int64_t* lbase = (int64_t*)(smem + stage_words);
-  uint32_t* lh = (uint32_t*)(lbase + nparts);
-  const int t = threadIdx.x;
-  const int64_t tile = blockIdx.x;
-  for (int i = t; i < nparts; i += kBlock) lh[i] = 0;
-  __syncthreads();
-  {
-    uint64_t p[kItems];
-    const uint32_t sel = tile_packed_keys(K, G, mask, n, tile, p);
-#pragma unroll
-    for (int j = 0; j < kItems; ++j)
-      if ((sel >> j) & 1u) atomicAdd(&lh[p[j] >> pb], 1u);
-  }
-  __syncthreads();
-  for (int i = t; i < nparts; i += kBlock) {
-    const uint32_t c = lh[i];
-    if (c) lbase[i] = part_off[i] + atomicAdd(&part_cur[i], c);
-    lh[i] = 0;
-  }
-  __syncthreads();
-  const uint64_t pmask = ((uint64_t)1 << pb) - 1;
-  for (int sub = 0; sub < kTileRows / kSubRows; ++sub) {
-    const int64_t sub0 = tile * kTileRows + (int64_t)sub * kSubRows;
-    bool live[kSub];
-    int64_t rows[kSub];
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      rows[j] = sub0 + (int64_t)j * kBlock + t;
-      live[j] = rows[j] < n && (!mask || mask[rows[j]] != 0);
-    }
-    stage_loads(P, C, sub0, live, stage);
-    int64_t pos[kSub];
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      pos[j] = 0;
-      if (!live[j]) continue;
-      const uint64_t pk = pack_key(G, stage, j);
-      const uint64_t b = pk >> pb;
-      pos[j] = lbase[b] + atomicAdd(&lh[b], 1u);
-      rec_key[pos[j]] = (uint32_t)(pk & pmask);
-    }
-    RegsN R{};
-    int begin = 0;
-    for (int a = 0; a < G.naggs; ++a) {
-      const int end = P.seg_end[a];
-      run_rows(P, C, begin, end, rows, live, stage, R);
-      begin = end;
-      const V4 V = R.get(P.seg_result[a]);
-#pragma unroll
-      for (int j = 0; j < kSub; ++j)
-        if (live[j]) rec_val[(int64_t)a * cap + pos[j]] = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
-    }
-  }
-}
-
-// One workgroup per partition: reduce its rows into an LDS table, then store every slot of the
-// partition (empty ones included) into the dense global table.
-__global__ __launch_bounds__(kBlock) void k_gbp_reduce(GbSpec G, GbTable T, int pb,
-                                                       const int64_t* __restrict__ part_off,
-                                                       const uint32_t* __restrict__ part_cnt,
-                                                       const uint32_t* __restrict__ rec_key,
-                                                       const uint64_t* __restrict__ rec_val,
-                                                       int64_t cap) {
-  extern __shared__ uint64_t smem[];
-  const int ps = 1 << pb;
-  uint64_t* lacc = smem;                                   // naggs * ps
-  uint32_t* lcnt = (uint32_t*)(smem + (int64_t)G.naggs * ps);
-  const int t = threadIdx.x;
-  const int64_t b = blockIdx.x;
-  for (int i = t; i < ps; i += kBlock) {
-    lcnt[i] = 0;
-    for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
-  }
-  __syncthreads();
-  const int64_t r0 = part_off[b];
-  const uint32_t m = part_cnt[b];
-  for (uint32_t r = t; r < m; r += kBlock) {
-    const uint32_t slot = rec_key[r0 + r];
-    atomicAdd(&lcnt[slot], 1u);
-    for (int a = 0; a < G.naggs; ++a)
-      acc_apply(&lacc[a * ps + slot], G.kind[a], rec_val[(int64_t)a * cap + r0 + r]);
-  }
-  __syncthreads();
-  const int64_t g0 = b << pb;
-  for (int i = t; i < ps; i += kBlock) {
-    const int64_t g = g0 + i;
-    if (g >= T.slots) break;
-    T.cnt[g] = lcnt[i];
-    for (int a = 0; a < G.naggs; ++a) T.acc[(int64_t)a * T.slots + g] = lacc[a * ps + i];
-  }
-}
-
 // ------------------------------------------------------------------ count / write
 __global__ __launch_bounds__(kBlock) void k_gb_occupied(const uint32_t* __restrict__ cnt,
                                                         int64_t slots, int force0,
@@ -532,6 +386,177 @@ __global__ __launch_bounds__(kBlock) void k_gb_write_dense(GbSpec G, OutCols O, 
   }
 }
 
+// ------------------------------------------------------------------ partitioned dense tables
+// The key space [0, S) is cut into partitions of ps = 2^pb slots.  A scatter tile is tsub
+// sub-tiles of 1024 rows; hist[b * ptiles + tile] counts the tile's selected rows in partition
+// b, and its exclusive scan (partition-major) is where the tile's rows of partition b go, so
+// the scatter needs no global atomics.
+
+// Packed key of row j of the sub-tile starting at sub0 (rows sub0 + j*256 + t), read from the
+// key columns.
+__device__ __forceinline__ uint64_t packed_key_at(const KeyCols& K, const GbSpec& G, int64_t row) {
+  uint64_t p = 0;
+  for (int k = 0; k < G.nkeys; ++k)
+    p += (uint64_t)(load_int(K.ptr[k], K.dtype[k], row) - G.key_min[k]) * (uint64_t)G.key_stride[k];
+  return p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
+                                                     const uint8_t* __restrict__ mask, int64_t n,
+                                                     int pb, int nparts, int tsub, int64_t ptiles,
+                                                     uint32_t* __restrict__ hist) {
+  extern __shared__ uint32_t lh[];
+  const int t = threadIdx.x;
+  for (int i = t; i < nparts; i += kBlock) lh[i] = 0;
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+  for (int sub = 0; sub < tsub; ++sub) {
+    const int64_t sub0 = (tile * tsub + sub) * kSubRows;
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      const int64_t r = sub0 + (int64_t)j * kBlock + t;
+      if (r < n && (!mask || mask[r] != 0)) atomicAdd(&lh[packed_key_at(K, G, r) >> pb], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < nparts; i += kBlock) hist[(int64_t)i * ptiles + tile] = lh[i];
+}
+
+// Scatter a tile's selected rows: each takes the next position of its partition's range and
+// stores its slot within the partition and its aggregate arguments (accumulator domain).
+__global__ __launch_bounds__(kBlock) void k_gbp_scatter(
+    GbSpec G, tq_program_t P, tq_colset_t C, const uint8_t* __restrict__ mask, int64_t n, int pb,
+    int nparts, int tsub, int64_t ptiles, const uint32_t* __restrict__ hoff,
+    uint32_t* __restrict__ rec_key, uint64_t* __restrict__ rec_val, int64_t cap,
+    int32_t stage_words) {
+  extern __shared__ uint64_t smem[];
+  uint64_t* stage = smem;
+  uint32_t* lh = (uint32_t*)(smem + stage_words);
+  const int t = threadIdx.x;
+  const int64_t tile = blockIdx.x;
+  for (int i = t; i < nparts; i += kBlock) lh[i] = hoff[(int64_t)i * ptiles + tile];
+  __syncthreads();
+  const uint64_t pmask = ((uint64_t)1 << pb) - 1;
+  for (int sub = 0; sub < tsub; ++sub) {
+    const int64_t sub0 = (tile * tsub + sub) * kSubRows;
+    bool live[kSub];
+    int64_t rows[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      rows[j] = sub0 + (int64_t)j * kBlock + t;
+      live[j] = rows[j] < n && (!mask || mask[rows[j]] != 0);
+    }
+    stage_loads(P, C, sub0, live, stage);
+    uint32_t pos[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      pos[j] = 0;
+      if (!live[j]) continue;
+      const uint64_t pk = pack_key(G, stage, j);
+      pos[j] = atomicAdd(&lh[pk >> pb], 1u);
+      rec_key[pos[j]] = (uint32_t)(pk & pmask);
+    }
+    RegsN R{};
+    int begin = 0;
+    for (int a = 0; a < G.naggs; ++a) {
+      const int end = P.seg_end[a];
+      run_rows(P, C, begin, end, rows, live, stage, R);
+      begin = end;
+      const V4 V = R.get(P.seg_result[a]);
+#pragma unroll
+      for (int j = 0; j < kSub; ++j)
+        if (live[j]) rec_val[(int64_t)a * cap + pos[j]] = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+    }
+  }
+}
+
+// One workgroup per partition: reduce its rows into an LDS table, then keep its occupied slots,
+// in slot order, at [b << pb, ...) of gslot/gcnt/gacc and their number in grp_cnt[b].
+__global__ __launch_bounds__(kBlock) void k_gbp_reduce(
+    GbSpec G, int pb, int nparts, int64_t ptiles, const uint32_t* __restrict__ hoff,
+    const uint32_t* __restrict__ htotal, const uint32_t* __restrict__ rec_key,
+    const uint64_t* __restrict__ rec_val, int64_t cap, uint32_t* __restrict__ grp_cnt,
+    uint32_t* __restrict__ gslot, uint32_t* __restrict__ gcnt, uint64_t* __restrict__ gacc) {
+  extern __shared__ uint64_t smem[];
+  __shared__ uint32_t wocc[kItems][kWaves];
+  const int ps = 1 << pb;
+  uint64_t* lacc = smem;                                   // naggs * ps
+  uint32_t* lcnt = (uint32_t*)(smem + (int64_t)G.naggs * ps);
+  const int t = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  for (int i = t; i < ps; i += kBlock) {
+    lcnt[i] = 0;
+    for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
+  }
+  __syncthreads();
+  const uint32_t r0 = hoff[b * ptiles];
+  const uint32_t r1 = b + 1 < nparts ? hoff[(b + 1) * ptiles] : *htotal;
+  for (uint32_t r = r0 + t; r < r1; r += kBlock) {
+    const uint32_t slot = rec_key[r];
+    atomicAdd(&lcnt[slot], 1u);
+    for (int a = 0; a < G.naggs; ++a)
+      acc_apply(&lacc[a * ps + slot], G.kind[a], rec_val[(int64_t)a * cap + r]);
+  }
+  __syncthreads();
+  // stable compaction of the occupied slots: slot i = j*256 + t, ranked by wave ballots and one
+  // exchange of the per-(j, wave) counts
+  const int64_t gbase = b << pb;
+  const int64_t gstride = (int64_t)nparts << pb;
+  const int w = t >> 6;
+  uint64_t bal[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int i = j * kBlock + t;
+    bal[j] = ballot(i < ps && lcnt[i] > 0);
+    if (lane_id() == 0) wocc[j][w] = (uint32_t)__popcll(bal[j]);
+  }
+  __syncthreads();
+  uint32_t run = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      const uint32_t c = wocc[j][v];
+      tot += c;
+      before += (v < w) ? c : 0u;
+    }
+    if ((bal[j] >> lane_id()) & 1u) {
+      const int i = j * kBlock + t;
+      const int64_t o = gbase + run + before + rank_below(bal[j]);
+      gslot[o] = (uint32_t)i;
+      gcnt[o] = lcnt[i];
+      for (int a = 0; a < G.naggs; ++a) gacc[(int64_t)a * gstride + o] = lacc[a * ps + i];
+    }
+    run += tot;
+  }
+  if (t == 0) grp_cnt[b] = run;
+}
+
+// Write partition b's groups at grp_off[b]...: keys decoded from the packed key, aggregates
+// finished in their output dtypes.
+__global__ __launch_bounds__(kBlock) void k_gbp_write(GbSpec G, OutCols O, int pb, int nparts,
+                                                      const uint32_t* __restrict__ grp_cnt,
+                                                      const int64_t* __restrict__ grp_off,
+                                                      const uint32_t* __restrict__ gslot,
+                                                      const uint32_t* __restrict__ gcnt,
+                                                      const uint64_t* __restrict__ gacc) {
+  const int64_t b = blockIdx.x;
+  const uint32_t m = grp_cnt[b];
+  const int64_t o0 = grp_off[b];
+  const int64_t gbase = b << pb;
+  const int64_t gstride = (int64_t)nparts << pb;
+  for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
+    const int64_t g = gbase + i;
+    const uint64_t pk = (uint64_t)gbase + gslot[g];
+    for (int k = 0; k < G.nkeys; ++k) store_int(O.key[k], G.key_dtype[k], o0 + i, key_of(G, k, pk));
+    const uint32_t rows = gcnt[g];
+    for (int a = 0; a < G.naggs; ++a)
+      write_agg(O.agg[a], o0 + i, O.op[a], G.kind[a], G.arg_dtype[a], gacc[(int64_t)a * gstride + g],
+                rows);
+  }
+}
+
 // Hash tables, step 1: compact (packed key, slot) of occupied slots.
 __global__ __launch_bounds__(kBlock) void k_gb_compact_hash(GbTable T,
                                                             const int64_t* __restrict__ tile_off,
@@ -594,14 +619,22 @@ struct GbWs {
   void* stats;
   void* radix;
   // partitioned dense mode
-  int32_t part_bits, nparts;
-  uint32_t* part_cnt;  // rows per partition, then (after it) the scatter cursors
-  uint32_t* part_cur;
-  int64_t* part_off;
-  int64_t* part_scan_ws;
-  uint32_t* rec_key;   // slot within the partition, per scattered row
-  uint64_t* rec_val;   // rec_val[a * rows_cap + i]: aggregate a's argument (accumulator domain)
+  int32_t part_bits, nparts;  // partitions of 2^part_bits slots
+  int32_t tsub;               // 1024-row sub-tiles per scatter tile
+  int64_t ptiles;             // scatter tiles over the input rows
+  uint32_t* hist;             // rows of tile t in partition b at hist[b * ptiles + t]
+  uint32_t* hoff;             // exclusive scan of hist: first scattered position of (b, t)
+  uint32_t* htotal;
+  uint32_t* hscan_ws;
+  uint32_t* rec_key;          // slot within the partition, per scattered row
+  uint64_t* rec_val;          // rec_val[a * rows_cap + i]: argument of aggregate a
   int64_t rows_cap;
+  uint32_t* grp_cnt;          // groups per partition
+  int64_t* grp_off;           // exclusive scan of grp_cnt
+  int64_t* grp_scan_ws;
+  uint32_t* gslot;            // partition b's groups in slot order at [b << part_bits, ...)
+  uint32_t* gcnt;
+  uint64_t* gacc;             // gacc[a * (nparts << part_bits) + i]
   size_t bytes;
 };
 
@@ -618,17 +651,52 @@ constexpr int kPartMax = 2048;
 static int part_bits_for(int naggs) {
   const size_t per = 4 + 8 * (size_t)(naggs > 0 ? naggs : 1);
   int b = 0;
-  while (((size_t)2 << b) * per <= kPartLds) ++b;
-  return b;
+  while (((size_t)2 << b) * per <= kPartLds && ((int64_t)2 << b) <= kTileRows) ++b;
+  return b;  // 2^b <= 4096 slots: the reduce kernel ranks 16 slots per thread
 }
 
-static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t rows_cap) {
+// Sub-tiles per scatter tile: enough that the per-tile partition histogram (nparts x tiles)
+// stays within the two-level scan's 2^24 elements.
+static int part_tsub(int64_t nrows, int nparts) {
+  const int64_t subs = ceil_div(nrows > 0 ? nrows : 1, kSubRows);
+  int t = 1;
+  while (ceil_div(subs, t) * nparts > ((int64_t)1 << 24)) t *= 2;
+  return t;
+}
+
+static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t rows_cap,
+                     int64_t nrows) {
   const int bm = mode & 7;
   char* p = (char*)base;
   const int64_t nt = ceil_div(slots, kTileRows);
   GbWs w{};
   w.T.slots = slots;
   w.T.log2 = ilog2_ceil(slots);
+  if (bm == TQ_GB_DENSE_PART) {
+    const int na = naggs > 0 ? naggs : 1;
+    w.part_bits = part_bits_for(naggs);
+    w.nparts = (int32_t)ceil_div(slots, (int64_t)1 << w.part_bits);
+    w.tsub = part_tsub(nrows, w.nparts);
+    w.ptiles = ceil_div(nrows > 0 ? nrows : 1, (int64_t)kSubRows * w.tsub);
+    const int64_t nh = (int64_t)w.nparts * w.ptiles;
+    const int64_t gcap = (int64_t)w.nparts << w.part_bits;
+    w.rows_cap = rows_cap > 0 ? rows_cap : 1;
+    w.hist = (uint32_t*)p; p += align_up((size_t)nh * 4, 256);
+    w.hoff = (uint32_t*)p; p += align_up((size_t)nh * 4, 256);
+    w.htotal = (uint32_t*)p; p += 256;
+    w.hscan_ws = (uint32_t*)p; p += tq_scan_ws_bytes(nh);
+    w.rec_key = (uint32_t*)p; p += align_up((size_t)w.rows_cap * 4, 256);
+    w.rec_val = (uint64_t*)p; p += align_up((size_t)w.rows_cap * 8 * na, 256);
+    w.grp_cnt = (uint32_t*)p; p += align_up((size_t)w.nparts * 4, 256);
+    w.grp_off = (int64_t*)p; p += align_up((size_t)w.nparts * 8, 256);
+    w.ngroups = (int64_t*)p; p += 256;
+    w.grp_scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(w.nparts);
+    w.gslot = (uint32_t*)p; p += align_up((size_t)gcap * 4, 256);
+    w.gcnt = (uint32_t*)p; p += align_up((size_t)gcap * 4, 256);
+    w.gacc = (uint64_t*)p; p += align_up((size_t)gcap * 8 * na, 256);
+    w.bytes = (size_t)(p - (char*)base);
+    return w;
+  }
   w.T.cnt = (uint32_t*)p; p += align_up((size_t)slots * 4, 256);
   w.T.hkey = (unsigned long long*)p; p += bm == TQ_GB_HASH ? align_up((size_t)slots * 8, 256) : 0;
   w.T.acc = (uint64_t*)p; p += align_up((size_t)slots * 8 * (naggs > 0 ? naggs : 1), 256);
@@ -643,18 +711,6 @@ static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t row
     w.v1 = (uint32_t*)p; p += align_up((size_t)slots * 4, 256);
     w.stats = p; p += 256;
     w.radix = p; p += radix_ws_bytes(slots);
-  }
-  if (bm == TQ_GB_DENSE_PART) {
-    const int na = naggs > 0 ? naggs : 1;
-    w.part_bits = part_bits_for(naggs);
-    w.nparts = (int32_t)ceil_div(slots, (int64_t)1 << w.part_bits);
-    w.rows_cap = rows_cap > 0 ? rows_cap : 1;
-    w.part_cnt = (uint32_t*)p;
-    w.part_cur = w.part_cnt + w.nparts; p += align_up((size_t)w.nparts * 8, 256);
-    w.part_off = (int64_t*)p; p += align_up((size_t)w.nparts * 8, 256);
-    w.part_scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(w.nparts);
-    w.rec_key = (uint32_t*)p; p += align_up((size_t)w.rows_cap * 4, 256);
-    w.rec_val = (uint64_t*)p; p += align_up((size_t)w.rows_cap * 8 * na, 256);
   }
   w.bytes = (size_t)(p - (char*)base);
   return w;
@@ -774,7 +830,8 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
     const int64_t distinct = ns < S ? ns : S;
     const int64_t nparts = ceil_div(S, (int64_t)1 << part_bits_for(d->naggs));
     const bool dense_ok = S <= ((int64_t)1 << 27) && S <= (4 * ns > 65536 ? 4 * ns : 65536);
-    if (dense_ok && nparts <= kPartMax && stage + (size_t)nparts * 12 <= (size_t)kLdsBudget) {
+    if (dense_ok && nparts <= kPartMax && stage + (size_t)nparts * 4 <= (size_t)kLdsBudget &&
+        d->nrows > 0 && d->nrows < ((int64_t)1 << 32)) {
       m = TQ_GB_DENSE_PART;
       sl = S;
     } else if (dense_ok) {
@@ -790,7 +847,7 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
   }
   *mode = m;
   *slots = sl;
-  *ws_bytes = carve_gb(nullptr, m, sl, d->naggs, ns).bytes;
+  *ws_bytes = carve_gb(nullptr, m, sl, d->naggs, ns, d->nrows).bytes;
   return TQ_OK;
 }
 
@@ -802,36 +859,38 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
   GbSpec G;
   TQ_TRY(make_spec(d, prog, &G));
   if (prog->nload > TQ_MAX_LOADS || prog->n_instr > TQ_MAX_INSTR) return set_error(TQ_EINVAL, "group-by: malformed program");
-  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel);
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel, d->nrows);
   if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "group-by workspace too small");
   hipStream_t s = (hipStream_t)stream;
   const int base = mode & 7;
   const bool front = (mode & TQ_GB_LDS_FRONT) != 0;
-  if (base == TQ_GB_DENSE_PART && n > 0 && d->nsel > 0) {
+  if (base == TQ_GB_DENSE_PART) {
+    const int np = w.nparts;
+    const int64_t nh = (int64_t)np * w.ptiles;
+    if (n <= 0 || d->nsel <= 0) return fill_u32(w.grp_cnt, 0u, np, s);
+    if (n != d->nrows) return set_error(TQ_EINVAL, "group-by: %lld rows, planned for %lld",
+                                        (long long)n, (long long)d->nrows);
     KeyCols K{};
     K.nkeys = d->nkeys;
     for (int k = 0; k < d->nkeys; ++k) {
       K.ptr[k] = d->key_ptr[k];
       K.dtype[k] = d->key_dtype[k];
     }
-    const int64_t nt = ceil_div(n, kTileRows);
-    const int np = w.nparts;
     const int32_t stage_words = (int32_t)(lds_stage_bytes(prog->nload) / 8);
-    const size_t lds_scatter = lds_stage_bytes(prog->nload) + (size_t)np * 12;
+    const size_t lds_scatter = lds_stage_bytes(prog->nload) + (size_t)np * 4;
     const size_t lds_reduce = ((size_t)1 << w.part_bits) * (4 + 8 * (size_t)d->naggs);
     if (lds_scatter > (size_t)kLdsBudget || lds_reduce > (size_t)kLdsBudget)
       return set_error(TQ_ECAPACITY, "group-by: partition tables exceed LDS");
-    TQ_TRY(fill_u32(w.part_cnt, 0u, 2 * (int64_t)np, s));  // counts and cursors
-    TQ_TRY(launch("k_gbp_count", k_gbp_count, (unsigned)nt, kBlock, (size_t)np * 4, s, K, G, mask,
-                  n, w.part_bits, np, w.part_cnt));
-    TQ_TRY((scan_exclusive<uint32_t, int64_t>(w.part_cnt, np, w.part_off, nullptr, w.part_scan_ws,
-                                              s)));
-    TQ_TRY(launch("k_gbp_scatter", k_gbp_scatter, (unsigned)nt, kBlock, lds_scatter, s, G, *prog,
-                  *cols, K, mask, n, w.part_bits, np, (const int64_t*)w.part_off, w.part_cur,
-                  w.rec_key, w.rec_val, w.rows_cap, stage_words));
-    return launch("k_gbp_reduce", k_gbp_reduce, (unsigned)np, kBlock, lds_reduce, s, G, w.T,
-                  w.part_bits, (const int64_t*)w.part_off, (const uint32_t*)w.part_cnt,
-                  (const uint32_t*)w.rec_key, (const uint64_t*)w.rec_val, w.rows_cap);
+    TQ_TRY(launch("k_gbp_hist", k_gbp_hist, (unsigned)w.ptiles, kBlock, (size_t)np * 4, s, K, G,
+                  mask, n, w.part_bits, np, w.tsub, w.ptiles, w.hist));
+    TQ_TRY((scan_exclusive<uint32_t, uint32_t>(w.hist, nh, w.hoff, w.htotal, w.hscan_ws, s)));
+    TQ_TRY(launch("k_gbp_scatter", k_gbp_scatter, (unsigned)w.ptiles, kBlock, lds_scatter, s, G,
+                  *prog, *cols, mask, n, w.part_bits, np, w.tsub, w.ptiles,
+                  (const uint32_t*)w.hoff, w.rec_key, w.rec_val, w.rows_cap, stage_words));
+    return launch("k_gbp_reduce", k_gbp_reduce, (unsigned)np, kBlock, lds_reduce, s, G,
+                  w.part_bits, np, w.ptiles, (const uint32_t*)w.hoff, (const uint32_t*)w.htotal,
+                  (const uint32_t*)w.rec_key, (const uint64_t*)w.rec_val, w.rows_cap, w.grp_cnt,
+                  w.gslot, w.gcnt, w.gacc);
   }
   TQ_TRY(launch("k_gb_clear", k_gb_clear, stream_grid(slots), kBlock, 0, s, w.T, G, base == TQ_GB_HASH));
   if (n <= 0) return TQ_OK;
@@ -858,8 +917,14 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
 extern "C" int tq_groupby_count(const tq_groupby_desc_t* d, int32_t mode, int64_t slots, void* ws,
                                 int64_t* ngroups_host, void* stream) {
   if (!d) return set_error(TQ_EINVAL, "group-by count: null descriptor");
-  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel);
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel, d->nrows);
   hipStream_t s = (hipStream_t)stream;
+  if ((mode & 7) == TQ_GB_DENSE_PART) {
+    TQ_TRY((scan_exclusive<uint32_t, int64_t>(w.grp_cnt, w.nparts, w.grp_off, w.ngroups,
+                                              w.grp_scan_ws, s)));
+    if (ngroups_host) TQ_TRY(read_back(w.ngroups, 8, ngroups_host, s));
+    return TQ_OK;
+  }
   const int64_t nt = ceil_div(slots, kTileRows);
   TQ_TRY(launch("k_gb_occupied", k_gb_occupied, (unsigned)nt, kBlock, 0, s, w.T.cnt, slots, d->nkeys == 0, w.tile_counts));
   TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.ngroups, w.scan_ws, s));
@@ -873,7 +938,7 @@ extern "C" int tq_groupby_write(const tq_groupby_desc_t* d, int32_t mode, int64_
   if (!d) return set_error(TQ_EINVAL, "group-by write: null descriptor");
   GbSpec G;
   TQ_TRY(make_spec(d, nullptr, &G));
-  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel);
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel, d->nrows);
   hipStream_t s = (hipStream_t)stream;
   if (ngroups <= 0) return TQ_OK;
   OutCols O{};
@@ -882,6 +947,10 @@ extern "C" int tq_groupby_write(const tq_groupby_desc_t* d, int32_t mode, int64_
     O.agg[a] = agg_out[a];
     O.op[a] = d->agg_op[a];
   }
+  if ((mode & 7) == TQ_GB_DENSE_PART)
+    return launch("k_gbp_write", k_gbp_write, (unsigned)w.nparts, kBlock, 0, s, G, O, w.part_bits,
+                  w.nparts, (const uint32_t*)w.grp_cnt, (const int64_t*)w.grp_off,
+                  (const uint32_t*)w.gslot, (const uint32_t*)w.gcnt, (const uint64_t*)w.gacc);
   const int64_t nt = ceil_div(slots, kTileRows);
   if ((mode & 7) != TQ_GB_HASH) {
     return launch("k_gb_write_dense", k_gb_write_dense, (unsigned)nt, kBlock, 0, s, G, O, w.T, w.tile_off, d->nkeys == 0);

@@ -8,9 +8,9 @@
 //   * hash: open addressing with linear probing over a power-of-two slot array of at least
 //     twice the selected rows, multiplicative (Fibonacci) hashing.
 // The build first assumes unique keys: every selected row claims its key's entry (a plain store
-// into the dense directory followed by a pass in which every row checks that it owns its entry;
-// a compare-and-swap on the hash slot's key) and a key claimed twice raises a flag, which the
-// host reads (second synchronisation).  Without duplicates an entry holds its build row and the build is done,
+// into the dense directory, whose occupied entries are then counted; a compare-and-swap on the
+// hash slot's key, which flags a key claimed twice).  The check is read back together with the
+// first probe's pair count, so the build itself synchronises once.  Without duplicates an entry holds its build row and the build is done,
 // with no sort.  With duplicates the selected (key, row) pairs are compacted in row order and
 // radix-sorted by key (stable, so each key's rows stay ascending); an entry then holds the
 // first sorted position of its key and the number of rows with that key.
@@ -169,13 +169,18 @@ __global__ __launch_bounds__(kBlock) void k_join_stats(const void* __restrict__ 
   }
 }
 
-// One workgroup: stats = {selected rows, OR, AND, min, max}; clears the duplicate flag.
+// One workgroup: stats = {selected rows, OR, AND, min, max}; clears the duplicate flag and the
+// occupied-entry counters.
+constexpr int kOccSlots = 64;  // occupied-entry counters, spread so their atomics rarely meet
+
 __global__ __launch_bounds__(kBlock) void k_join_stats_final(const uint32_t* __restrict__ counts,
                                                              const uint64_t* __restrict__ part,
                                                              int64_t ntiles,
                                                              uint64_t* __restrict__ stats,
-                                                             uint32_t* __restrict__ dup) {
+                                                             uint32_t* __restrict__ dup,
+                                                             uint32_t* __restrict__ occ) {
   __shared__ uint64_t sh[5][kWaves];
+  if (threadIdx.x < kOccSlots) occ[threadIdx.x] = 0;
   uint64_t c = 0, so = 0, sa = ~0ull, slo = ~0ull, shi = 0;
   for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) {
     c += counts[i];
@@ -228,9 +233,10 @@ __global__ __launch_bounds__(kBlock) void k_join_clear(void* __restrict__ table,
   }
 }
 
-// Every selected build row claims its key's entry; a key claimed twice sets *dup.  Dense
-// directories take plain stores here (one of several rows with the same key wins) and
-// k_join_verify then flags any row that does not find itself in its entry.
+// Every selected build row claims its key's entry; in a hash table a key claimed twice sets
+// *dup.  Dense directories take plain stores (one of several rows with the same key wins);
+// k_join_bits then counts the occupied entries, which equals the selected row count exactly
+// when the keys are unique.
 template <typename K, bool HASH>
 __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ bkeys, int dt,
                                                        const uint8_t* __restrict__ mask,
@@ -270,45 +276,30 @@ __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ 
   if (seen) *dup = 1;
 }
 
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_join_verify(const void* __restrict__ bkeys, int dt,
-                                                        const uint8_t* __restrict__ mask,
-                                                        int64_t n, JoinView J,
-                                                        uint32_t* __restrict__ dup) {
-  const int64_t tile = blockIdx.x;
-  uint32_t sel;
-  int64_t v[kItems];
-  load_tile_rows(bkeys, dt, mask, n, tile, &sel, v);
-  uint32_t e[kItems];
-#pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const uint64_t off = ((sel >> j) & 1u) ? (uint64_t)order_key<K>(v[j]) - J.base : 0;
-    e[j] = ((const uint32_t*)J.table)[off];
-  }
-  bool seen = false;
-#pragma unroll
-  for (int j = 0; j < kItems; ++j)
-    seen |= ((sel >> j) & 1u) && e[j] != (uint32_t)tile_row(tile, j);
-  if (seen) *dup = 1;
-}
-
 // Presence bitmap of a dense directory: bit i of bits[w] is set when entry 32*w + i holds a key.
-// A wave tests 64 consecutive entries and its ballot is two bitmap words.
+// A wave tests 64 consecutive entries and its ballot is two bitmap words.  With occ, the number
+// of occupied entries is added to occ[wave % kOccSlots]: after the plain-store claim, the keys
+// were unique exactly when that number equals the selected row count.
 template <bool UNIQUE>
 __global__ __launch_bounds__(kBlock) void k_join_bits(const void* __restrict__ table,
-                                                      int64_t range, uint32_t* __restrict__ bits) {
+                                                      int64_t range, uint32_t* __restrict__ bits,
+                                                      uint32_t* __restrict__ occ) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   const int64_t nw = (range + 31) >> 5;
+  uint32_t c = 0;
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; (e & ~63ll) < range; e += stride) {
     const bool present = e < range && (UNIQUE ? ((const uint32_t*)table)[e] != kNoPos
                                               : ((const uint2*)table)[e].y != 0u);
     const uint64_t m = ballot(present);
+    c += (uint32_t)__popcll(m);
     if (lane_id() == 0) {
       const int64_t w = e >> 5;
       bits[w] = (uint32_t)m;
       if (w + 1 < nw) bits[w + 1] = (uint32_t)(m >> 32);
     }
   }
+  if (occ && lane_id() == 0 && c)
+    atomicAdd(&occ[((int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6)) % kOccSlots], c);
 }
 
 // ------------------------------------------------------------------ build: duplicate keys
@@ -533,6 +524,7 @@ struct BuildWs {
   int64_t* tile_off;
   uint64_t* stats;   // selected rows, OR, AND, min, max
   uint32_t* dup;
+  uint32_t* occ;     // kOccSlots counters of occupied dense entries
   int64_t* total;
   int64_t* scan_ws;
   uint64_t* part;
@@ -558,6 +550,7 @@ static BuildWs carve_build(void* base, int64_t nb) {
   w.stats = (uint64_t*)p;
   w.dup = (uint32_t*)(p + 64);
   w.total = (int64_t*)(p + 128); p += 256;
+  w.occ = (uint32_t*)p; p += 256;
   w.scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(nt);
   w.part = (uint64_t*)p; p += align_up((size_t)nt * 32, 256);
   w.k0 = p; p += align_up((size_t)nb * 8, 256);
@@ -608,6 +601,42 @@ static unsigned probe_grid(int64_t ntiles) {
   return (unsigned)(ntiles < 2048 ? (ntiles > 0 ? ntiles : 1) : 2048);
 }
 
+// Sorted layout for repeated keys: compact the selected (key, row) pairs in row order (the
+// statistics pass left the per-tile counts in the workspace), sort them stably by key and record
+// {first position, count} per key.
+template <typename K>
+static int build_multi(tq_join_state_t* st, BuildWs& w, hipStream_t s) {
+  const int64_t nb = st->nb, nsel = st->nsel;
+  const int64_t nt = ceil_div(nb, kTileRows);
+  const bool hash = st->layout == TQ_JOIN_HASH;
+  st->unique = 0;
+  TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
+  TQ_TRY(launch("k_join_pack", k_join_pack<K>, (unsigned)nt, kBlock, 0, s, st->bkeys,
+                (int)st->bdtype, st->bmask, nb, (const int64_t*)w.tile_off, (K*)w.k0, w.v0));
+  int lo = 0, hi = 0;
+  radix_bit_range(st->key_or, st->key_and, &lo, &hi);
+  bool alt;
+  TQ_TRY(radix_sort_pairs<K>((K*)w.k0, w.v0, (K*)w.k1, w.v1, nsel, lo, hi, w.radix, s, &alt));
+  const K* keys = (const K*)(alt ? w.k1 : w.k0);
+  st->rows = alt ? w.v1 : w.v0;
+  JoinView J = view_of(st);
+  const unsigned cg = stream_grid(st->range);
+  if (hash) {
+    TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
+                  st->empty_key));
+    TQ_TRY(launch("k_join_runs", (k_join_runs<K, true>), stream_grid(nsel), kBlock, 0, s, keys,
+                  nsel, J));
+  } else {
+    TQ_TRY(fill_u64(w.table, 0ull, st->range, s));  // {first, count} = {0, 0}: absent
+    TQ_TRY(launch("k_join_runs", (k_join_runs<K, false>), stream_grid(nsel), kBlock, 0, s, keys,
+                  nsel, J));
+    st->bits = w.bits;
+    TQ_TRY(launch("k_join_bits", k_join_bits<false>, stream_grid(st->range), kBlock, 0, s,
+                  (const void*)w.table, st->range, w.bits, (uint32_t*)nullptr));
+  }
+  return TQ_OK;
+}
+
 template <typename K>
 static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64_t nb, BuildWs& w,
                       tq_join_state_t* st, hipStream_t s) {
@@ -615,13 +644,15 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
   TQ_TRY(launch("k_join_stats", k_join_stats<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask,
                 nb, w.tile_counts, w.part));
   TQ_TRY(launch("k_join_stats_final", k_join_stats_final, 1, kBlock, 0, s,
-                (const uint32_t*)w.tile_counts, (const uint64_t*)w.part, nt, w.stats, w.dup));
+                (const uint32_t*)w.tile_counts, (const uint64_t*)w.part, nt, w.stats, w.dup,
+                w.occ));
   int64_t hv[5];
-  TQ_TRY(read_back(w.stats, 40, hv, s));  // one synchronisation: count and key statistics
+  TQ_TRY(read_back(w.stats, 40, hv, s));  // the one synchronisation: count and key statistics
   const int64_t nsel = hv[0];
   st->nsel = nsel;
   if (nsel == 0) return TQ_OK;
-  const uint64_t kor = (uint64_t)hv[1], kand = (uint64_t)hv[2];
+  st->key_or = (uint64_t)hv[1];
+  st->key_and = (uint64_t)hv[2];
   const uint64_t kmin = (uint64_t)hv[3], kmax = (uint64_t)hv[4];
   const bool hash = kmax - kmin >= (uint64_t)dense_cap(nb);
   st->layout = hash ? TQ_JOIN_HASH : TQ_JOIN_DENSE;
@@ -644,57 +675,27 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
   st->table = w.table;
   JoinView J = view_of(st);
   const unsigned cg = stream_grid(st->range);
-  if (hash)
-    TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
-                  st->empty_key));
-  else
-    TQ_TRY(launch("k_join_clear", k_join_clear<false>, cg, kBlock, 0, s, w.table, st->range,
-                  st->empty_key));
-  if (hash)
-    TQ_TRY(launch("k_join_claim", (k_join_claim<K, true>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
-                  bmask, nb, J, w.dup));
-  else {
-    TQ_TRY(launch("k_join_claim", (k_join_claim<K, false>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
-                  bmask, nb, J, w.dup));
-    TQ_TRY(launch("k_join_verify", k_join_verify<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask,
-                  nb, J, w.dup));
-  }
-  int64_t dup = 0;
-  TQ_TRY(read_back(w.dup, 4, &dup, s));  // second synchronisation: were the keys unique?
-  if ((dup & 0xFFFFFFFF) == 0) {
-    st->unique = 1;
-    if (!hash) {
-      st->bits = w.bits;
-      TQ_TRY(launch("k_join_bits", k_join_bits<true>, stream_grid(st->range), kBlock,
-                    0, s, (const void*)w.table, st->range, w.bits));
-    }
-    return TQ_OK;
-  }
-  // duplicate keys: compact, sort stably by key, record {first position, count} per key
-  st->unique = 0;
-  TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
-  TQ_TRY(launch("k_join_pack", k_join_pack<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask, nb,
-                (const int64_t*)w.tile_off, (K*)w.k0, w.v0));
-  int lo = 0, hi = 0;
-  radix_bit_range(kor, kand, &lo, &hi);
-  bool alt;
-  TQ_TRY(radix_sort_pairs<K>((K*)w.k0, w.v0, (K*)w.k1, w.v1, nsel, lo, hi, w.radix, s, &alt));
-  const K* keys = (const K*)(alt ? w.k1 : w.k0);
-  st->rows = alt ? w.v1 : w.v0;
-  J.rows = st->rows;
+  // assume unique keys; the duplicate flag is read with the first pair count
   if (hash) {
     TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
                   st->empty_key));
-    TQ_TRY(launch("k_join_runs", (k_join_runs<K, true>), stream_grid(nsel), kBlock, 0, s, keys,
-                  nsel, J));
+    TQ_TRY(launch("k_join_claim", (k_join_claim<K, true>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
+                  bmask, nb, J, w.dup));
   } else {
-    TQ_TRY(fill_u64(w.table, 0ull, st->range, s));  // {first, count} = {0, 0}: absent
-    TQ_TRY(launch("k_join_runs", (k_join_runs<K, false>), stream_grid(nsel), kBlock, 0, s, keys,
-                  nsel, J));
+    TQ_TRY(launch("k_join_clear", k_join_clear<false>, cg, kBlock, 0, s, w.table, st->range,
+                  st->empty_key));
+    TQ_TRY(launch("k_join_claim", (k_join_claim<K, false>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
+                  bmask, nb, J, w.dup));
     st->bits = w.bits;
-    TQ_TRY(launch("k_join_bits", k_join_bits<false>, stream_grid(st->range), kBlock,
-                  0, s, (const void*)w.table, st->range, w.bits));
+    TQ_TRY(launch("k_join_bits", k_join_bits<true>, cg, kBlock, 0, s, (const void*)w.table,
+                  st->range, w.bits, w.occ));
   }
+  st->unique = 1;
+  st->pending = 1;
+  st->bdtype = dt;
+  st->bkeys = bkeys;
+  st->bmask = bmask;
+  st->nb = nb;
   return TQ_OK;
 }
 
@@ -762,11 +763,12 @@ extern "C" int tq_join_build(const void* bkeys, int32_t bdtype, const uint8_t* b
   state->layout = TQ_JOIN_EMPTY;
   if (nb <= 0) return TQ_OK;
   hipStream_t s = (hipStream_t)stream;
+  state->ws = ws;
   return bdtype == TQ_I64 ? build_impl<uint64_t>(bkeys, bdtype, bmask, nb, w, state, s)
                           : build_impl<uint32_t>(bkeys, bdtype, bmask, nb, w, state, s);
 }
 
-extern "C" int tq_join_count(const tq_join_state_t* st, const void* pkeys, int32_t pdtype,
+extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdtype,
                              const uint8_t* pmask, int64_t np, void* pws, int64_t* total_host,
                              void* stream) {
   if (!st) return set_error(TQ_EINVAL, "join count: null state");
@@ -775,6 +777,38 @@ extern "C" int tq_join_count(const tq_join_state_t* st, const void* pkeys, int32
   hipStream_t s = (hipStream_t)stream;
   ProbeWs w = carve_probe(pws, np);
   const int64_t nt = np > 0 ? ceil_div(np, kTileRows) : 0;
+  if (st->pending) {
+    // settle the build's uniqueness with the same synchronisation as this count
+    BuildWs bw = carve_build(st->ws, st->nb);
+    const bool dense = st->layout == TQ_JOIN_DENSE;
+    const void* flag = dense ? (const void*)bw.occ : (const void*)bw.dup;
+    const size_t flag_bytes = dense ? kOccSlots * 4 : 4;
+    int64_t total = 0, fl[kOccSlots / 2] = {};
+    if (np > 0) {
+      TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
+      TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
+      TQ_TRY(read_back2(w.total, 8, &total, flag, flag_bytes, fl, s));
+    } else {
+      TQ_TRY(read_back(flag, flag_bytes, fl, s));
+    }
+    st->pending = 0;
+    bool dup;
+    if (dense) {
+      // unique exactly when every selected row occupies its own directory entry
+      const uint32_t* c = (const uint32_t*)fl;
+      int64_t occupied = 0;
+      for (int i = 0; i < kOccSlots; ++i) occupied += c[i];
+      dup = occupied != st->nsel;
+    } else {
+      dup = (fl[0] & 0xFFFFFFFF) != 0;
+    }
+    if (!dup) {
+      if (np <= 0) return tq_join_count(st, pkeys, pdtype, pmask, np, pws, total_host, stream);
+      if (total_host) *total_host = total;
+      return TQ_OK;
+    }
+    TQ_TRY(st->key_bits == 64 ? build_multi<uint64_t>(st, bw, s) : build_multi<uint32_t>(st, bw, s));
+  }
   if (np <= 0 || st->nsel <= 0 || st->layout == TQ_JOIN_EMPTY) {
     TQ_TRY(fill_u32(w.tile_counts, 0, nt, s));
     TQ_TRY(fill_u64(w.tile_off, 0, nt, s));
@@ -795,6 +829,7 @@ extern "C" int tq_join_write(const tq_join_state_t* st, const void* pkeys, int32
                              void* const* rdst, int64_t* build_idx, int64_t* probe_idx,
                              void* stream) {
   if (!st) return set_error(TQ_EINVAL, "join write: null state");
+  if (st->pending) return set_error(TQ_EINVAL, "join write: run tq_join_count first");
   ColPack L, R;
   TQ_TRY(make_colpack(&L, nleft, lsrc, lbytes, nullptr, ldst));
   TQ_TRY(make_colpack(&R, nright, rsrc, rbytes, nullptr, rdst));

@@ -207,6 +207,19 @@ int64_t* host_scratch() {
   return p;
 }
 
+int read_back2(const void* dev0, size_t bytes0, int64_t* host0, const void* dev1, size_t bytes1,
+               int64_t* host1, hipStream_t s) {
+  int64_t* h = host_scratch();
+  if (!h) return set_error(TQ_EHIP, "pinned staging allocation failed");
+  if (bytes0 > 256 || bytes1 > 256) return set_error(TQ_EINVAL, "read_back2: copy too large");
+  TQ_TRY_HIP(hipMemcpyAsync(h, dev0, bytes0, hipMemcpyDeviceToHost, s));
+  TQ_TRY_HIP(hipMemcpyAsync(h + 32, dev1, bytes1, hipMemcpyDeviceToHost, s));
+  TQ_TRY_HIP(hipStreamSynchronize(s));
+  memcpy(host0, h, bytes0);
+  memcpy(host1, h + 32, bytes1);
+  return TQ_OK;
+}
+
 int read_back(const void* dev, size_t bytes, int64_t* host_out, hipStream_t s) {
   int64_t* h = host_scratch();
   if (!h) return set_error(TQ_EHIP, "pinned staging allocation failed");

@@ -45,6 +45,7 @@ OP = {name: i for i, name in enumerate(OPCODES)}
 TQ_AGG_SUM, TQ_AGG_COUNT, TQ_AGG_AVG, TQ_AGG_MIN, TQ_AGG_MAX = range(5)
 TQ_GB_MAX_KEYS = 8
 TQ_GB_DENSE_LDS, TQ_GB_DENSE_GLOBAL, TQ_GB_HASH, TQ_GB_DENSE_PART, TQ_GB_LDS_FRONT = 0, 1, 2, 3, 8
+TQ_JOIN_EMPTY, TQ_JOIN_DENSE, TQ_JOIN_HASH = 0, 1, 2
 
 DTYPE_CODE = {
     torch.bool: TQ_BOOL, torch.int8: TQ_I8, torch.int16: TQ_I16, torch.int32: TQ_I32,
@@ -89,7 +90,10 @@ class JoinState(ctypes.Structure):
                 ("layout", ctypes.c_int32), ("unique", ctypes.c_int32),
                 ("hash_bits", ctypes.c_int32), ("base", ctypes.c_uint64),
                 ("empty_key", ctypes.c_uint64), ("range", ctypes.c_int64),
-                ("table", ctypes.c_void_p), ("bits", ctypes.c_void_p), ("rows", ctypes.c_void_p)]
+                ("table", ctypes.c_void_p), ("bits", ctypes.c_void_p), ("rows", ctypes.c_void_p),
+                ("pending", ctypes.c_int32), ("bdtype", ctypes.c_int32),
+                ("bkeys", ctypes.c_void_p), ("bmask", ctypes.c_void_p), ("nb", ctypes.c_int64),
+                ("ws", ctypes.c_void_p), ("key_or", ctypes.c_uint64), ("key_and", ctypes.c_uint64)]
 
 
 class GroupByDesc(ctypes.Structure):
@@ -101,7 +105,7 @@ class GroupByDesc(ctypes.Structure):
                 ("key_stride", ctypes.c_int64 * TQ_GB_MAX_KEYS),
                 ("agg_op", ctypes.c_int32 * TQ_MAX_SEGS),
                 ("agg_dtype", ctypes.c_int32 * TQ_MAX_SEGS),
-                ("nsel", ctypes.c_int64)]
+                ("nsel", ctypes.c_int64), ("nrows", ctypes.c_int64)]
 
 
 # ---------------------------------------------------------------- loading
@@ -219,7 +223,9 @@ def size_call(name, *args) -> int:
 
 # ---------------------------------------------------------------- helpers
 def stream_handle():
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    """torch's current stream on the current device, as a raw hipStream_t (the C calls below
+    skip torch.cuda.current_stream's Python-level device lookup)."""
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
 
 
 def ptr(t: torch.Tensor):
@@ -258,6 +264,20 @@ def int_array(vals):
     return arr
 
 
-def workspace(nbytes: int, device) -> torch.Tensor:
-    """Raw device workspace from torch's caching allocator."""
-    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+_WORKSPACES = {}
+
+
+def workspace(nbytes: int, device, role=None) -> torch.Tensor:
+    """Raw device workspace.  With a role, the buffer is kept per (role, device, stream) and
+    handed out again to the next call of that role on the same stream, which runs after the
+    previous one's kernels in stream order; a holder must not keep using its buffer once
+    another call of its role has been issued."""
+    nbytes = max(int(nbytes), 256)
+    if role is None:
+        return torch.empty(nbytes, dtype=torch.uint8, device=device)
+    key = (role, str(device), torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
+    buf = _WORKSPACES.get(key)
+    if buf is None or buf.numel() < nbytes:
+        buf = torch.empty(nbytes + nbytes // 4, dtype=torch.uint8, device=device)
+        _WORKSPACES[key] = buf
+    return buf[:nbytes]

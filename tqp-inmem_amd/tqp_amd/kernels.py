@@ -6,6 +6,7 @@ Outputs are allocated here with torch's caching allocator, workspaces likewise; 
 launches kernels on the current stream.
 """
 import ctypes
+import weakref
 
 import torch
 
@@ -59,7 +60,7 @@ def scan_tiles(counts: torch.Tensor, nt: int):
     dev = counts.device
     offs = torch.empty(max(nt, 1), dtype=torch.int64, device=dev)
     total = torch.empty(1, dtype=torch.int64, device=dev)
-    ws = L.workspace(L.size_call("tq_scan_ws_bytes", nt), dev)
+    ws = L.workspace(L.size_call("tq_scan_ws_bytes", nt), dev, "scan")
     L.call("tq_scan_tiles", L.ptr(counts), nt, L.ptr(offs), L.ptr(total), L.ptr(ws),
            L.stream_handle())
     return offs, total
@@ -141,10 +142,39 @@ def sort_perm(keys, n: int, limit=None, device=None):
         arr[i].width = t.shape[1] if t.dim() == 2 else 1
         arr[i].descending = 1 if desc else 0
     ws_bytes = L.size_call("tq_sort_ws_bytes", n)
-    ws = L.workspace(ws_bytes, dev)
+    ws = L.workspace(ws_bytes, dev, "sort")
     L.call("tq_sort_perm", arr, len(keys), n, -1 if limit is None else int(limit), L.ptr(perm),
            L.ptr(ws), ws_bytes, L.stream_handle())
     return perm
+
+
+# ---------------------------------------------------------------- value bounds
+# Host-known value bounds of integer columns: a join writes its key outputs from keys whose
+# range its build already measured, so a group-by over those columns can plan without its own
+# bounds pass and synchronisation.  An entry is only trusted while the tensor's in-place
+# version counter is unchanged.
+_BOUNDS = {}   # id(tensor) -> (weak reference, version, lo, hi); tensors compare elementwise
+
+
+def note_bounds(t: torch.Tensor, lo: int, hi: int):
+    k = id(t)
+    _BOUNDS[k] = (weakref.ref(t, lambda _r, k=k: _BOUNDS.pop(k, None)), t._version, int(lo), int(hi))
+
+
+def known_bounds(t: torch.Tensor):
+    """(lo, hi) recorded for t, or None."""
+    e = _BOUNDS.get(id(t))
+    if e is None or e[0]() is not t or e[1] != t._version:
+        return None
+    return e[2], e[3]
+
+
+def _untransform(bits: int, key_bits: int) -> int:
+    if key_bits == 32:
+        v = (bits ^ 0x80000000) & 0xFFFFFFFF
+        return v - (1 << 32) if v >= 1 << 31 else v
+    v = (bits ^ 0x8000000000000000) & 0xFFFFFFFFFFFFFFFF
+    return v - (1 << 64) if v >= 1 << 63 else v
 
 
 # ---------------------------------------------------------------- join
@@ -157,7 +187,7 @@ class JoinIndex:
         self.bmask = bmask
         nb = self.bkey.shape[0]
         ws_bytes = L.size_call("tq_join_build_ws_bytes", nb)
-        self.ws = L.workspace(ws_bytes, bkey.device)
+        self.ws = L.workspace(ws_bytes, bkey.device, "join_build")
         self.state = L.JoinState()
         L.call("tq_join_build", L.ptr(self.bkey), L.dtype_code(self.bkey.dtype),
                L.ptr(bmask) if bmask is not None else None, nb, L.ptr(self.ws), ws_bytes,
@@ -167,6 +197,15 @@ class JoinIndex:
     def nsel(self):
         return int(self.state.nsel)
 
+    def key_bounds(self):
+        """(min, max) of the selected build keys when the build measured them (dense layout),
+        else None."""
+        st = self.state
+        if st.nsel <= 0 or st.layout != L.TQ_JOIN_DENSE:
+            return None
+        return (_untransform(st.base, st.key_bits),
+                _untransform(st.base + st.range - 1, st.key_bits))
+
     def probe(self, pkey: torch.Tensor, pmask, left_cols, right_cols, want_index=False):
         """All (build, probe) pairs with equal keys, probe row ascending then build row
         ascending.  Returns (left outputs, right outputs, build_idx, probe_idx)."""
@@ -174,7 +213,7 @@ class JoinIndex:
         pkey = pkey.contiguous()
         np_ = pkey.shape[0]
         dev = pkey.device
-        pws = L.workspace(L.size_call("tq_join_probe_ws_bytes", np_), dev)
+        pws = L.workspace(L.size_call("tq_join_probe_ws_bytes", np_), dev, "join_probe")
         total = ctypes.c_int64(0)
         pm = L.ptr(pmask) if pmask is not None else None
         L.call("tq_join_count", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype), pm,
@@ -259,6 +298,7 @@ class GroupBy:
         d = L.GroupByDesc()
         d.nkeys = len(self.keys)
         d.naggs = len(self.ops)
+        d.nrows = self.n
         for i, k in enumerate(self.keys):
             d.key_dtype[i] = L.dtype_code(k.dtype)
             d.key_ptr[i] = k.data_ptr()
@@ -267,10 +307,15 @@ class GroupBy:
             d.agg_dtype[j] = L.dtype_code(dt)
         s = L.stream_handle()
         mptr = L.ptr(self.mask) if self.mask is not None else None
-        bounds = torch.empty(2 * len(self.keys) + 1, dtype=torch.int64, device=dev)
-        bws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", self.n), dev)
-        L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), L.ptr(bws), s)
-        hb = bounds.cpu().tolist()
+        known = [known_bounds(k) for k in self.keys] if self.mask is None else [None]
+        if self.keys and all(b is not None for b in known):
+            # every row is selected and the key ranges are already known: no bounds pass
+            hb = [v for b in known for v in b] + [self.n]
+        else:
+            bounds = torch.empty(2 * len(self.keys) + 1, dtype=torch.int64, device=dev)
+            bws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", self.n), dev, "gb_bounds")
+            L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), L.ptr(bws), s)
+            hb = bounds.cpu().tolist()
         nsel = hb[-1]
         self.nsel = nsel
         for i in range(len(self.keys)):
@@ -283,7 +328,7 @@ class GroupBy:
         L.call("tq_groupby_plan", ctypes.byref(d), nsel, b.nload, ctypes.byref(mode),
                ctypes.byref(slots), ctypes.byref(wsb))
         self.mode, self.slots = mode.value, slots.value
-        ws = L.workspace(wsb.value, dev)
+        ws = L.workspace(wsb.value, dev, "groupby")
         # every selected row's loaded columns (keys included) and the mask byte of every row
         alg = nsel * b.load_bytes() + (self.n if self.mask is not None else 0)
         L.call("tq_groupby_accumulate", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr,

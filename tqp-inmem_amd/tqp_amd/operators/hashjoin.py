@@ -109,6 +109,12 @@ def join_kernel_new(gpu_enable, tensors_left, tensors_right, left_key, right_key
         lout, rout, _, _ = index.probe(right, pmask, lsrc, rsrc)
         got = dict(zip(via_build, lout))
         got.update(zip(right_keys_out + via_probe, rout))
+        kb = index.key_bounds()
+        if kb is not None:
+            # every pair's keys equal a selected build key
+            for k in (left_key, right_key):
+                if k in got and got[k].dim() == 1:
+                    kernels.note_bounds(got[k], *kb)
         lne = left_nonempty()
         rne = right_nonempty() if lout and lout[0].shape[0] == 0 or rout and rout[0].shape[0] == 0 else True
         result = []
