@@ -29,3 +29,29 @@ def test_untransform_inverts_the_order_transform():
         assert kernels._untransform((v & 0xFFFFFFFF) ^ 0x80000000, 32) == v
     for v in (0, 5, -5, 2**63 - 1, -2**63):
         assert kernels._untransform((v & 0xFFFFFFFFFFFFFFFF) ^ 0x8000000000000000, 64) == v
+
+
+def test_program_cache_rebinds_columns_and_recompiles_on_dtype_change(monkeypatch):
+    from tqp_amd import _lib as L
+    from tqp_amd import expr as E
+    monkeypatch.setattr(L, "require_gpu", lambda *a, **k: None)
+    E._PROGRAMS.clear()
+
+    class Col:
+        def __init__(self, t):
+            self.tensor = t
+
+    tree = [['d#5', '>', '10'], 'AND', ['d#5', '<', '20']]
+    key = ("t", repr(tree))
+    a = {5: Col(torch.arange(100, dtype=torch.int32))}
+    b1 = E.cached_program(key, a, lambda pb: E.compile_predicate(pb, tree, a))
+    assert isinstance(b1, E.ProgramBuilder) and key in E._PROGRAMS
+    c = {5: Col(torch.arange(50, dtype=torch.int32))}
+    b2 = E.cached_program(key, c, lambda pb: E.compile_predicate(pb, tree, c))
+    assert isinstance(b2, E.BoundProgram)
+    assert b2.prog is b1.prog and b2.n == 50
+    assert b2.cols.ptr[0] == c[5].tensor.data_ptr()
+    # same tree over an int64 column compiles again (the compare dtype may change)
+    d = {5: Col(torch.arange(30, dtype=torch.int64))}
+    b3 = E.cached_program(key, d, lambda pb: E.compile_predicate(pb, tree, d))
+    assert isinstance(b3, E.ProgramBuilder) and b3.cols.ptr[0] == d[5].tensor.data_ptr()

@@ -171,7 +171,8 @@ __global__ __launch_bounds__(kBlock) void k_join_stats(const void* __restrict__ 
 
 // One workgroup: stats = {selected rows, OR, AND, min, max}; clears the duplicate flag and the
 // occupied-entry counters.
-constexpr int kOccSlots = 64;  // occupied-entry counters, spread so their atomics rarely meet
+constexpr int kOccSlots = 64;   // occupied-entry counters, spread so their atomics rarely meet
+constexpr int kOccStride = 32;  // u32 words between counters: one 128-byte line each
 
 __global__ __launch_bounds__(kBlock) void k_join_stats_final(const uint32_t* __restrict__ counts,
                                                              const uint64_t* __restrict__ part,
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stats_final(const uint32_t* __r
                                                              uint32_t* __restrict__ dup,
                                                              uint32_t* __restrict__ occ) {
   __shared__ uint64_t sh[5][kWaves];
-  if (threadIdx.x < kOccSlots) occ[threadIdx.x] = 0;
+  if (threadIdx.x < kOccSlots) occ[threadIdx.x * kOccStride] = 0;
   uint64_t c = 0, so = 0, sa = ~0ull, slo = ~0ull, shi = 0;
   for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) {
     c += counts[i];
@@ -298,8 +299,25 @@ __global__ __launch_bounds__(kBlock) void k_join_bits(const void* __restrict__ t
       if (w + 1 < nw) bits[w + 1] = (uint32_t)(m >> 32);
     }
   }
-  if (occ && lane_id() == 0 && c)
-    atomicAdd(&occ[((int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6)) % kOccSlots], c);
+  if (!occ) return;
+  __shared__ uint32_t wc[kWaves];
+  if (lane_id() == 0) wc[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) b += wc[i];
+    // one add per workgroup, on a counter in its own 128-byte line
+    if (b) atomicAdd(&occ[(blockIdx.x % kOccSlots) * kOccStride], b);
+  }
+}
+
+// Sum of the occupied-entry counters into *out (one workgroup).
+__global__ __launch_bounds__(kBlock) void k_join_occ_sum(const uint32_t* __restrict__ occ,
+                                                         int64_t* __restrict__ out) {
+  int64_t v = threadIdx.x < kOccSlots ? occ[threadIdx.x * kOccStride] : 0;
+  v = wave_sum(v);
+  if (threadIdx.x == 0) *out = v;  // the counters are in the first wave
 }
 
 // ------------------------------------------------------------------ build: duplicate keys
@@ -388,11 +406,14 @@ __device__ __forceinline__ void probe_rows(const JoinView& J, const void* pkeys,
   }
 }
 
+// Pairs per probe tile.  With unique build keys a probe row has at most one pair, and the
+// thread's 16 match bits go to hits[tile * 256 + t] for the write pass.
 template <typename K, bool HASH, bool UNIQUE>
 __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* __restrict__ pkeys,
                                                        int pdt, const uint8_t* __restrict__ pmask,
                                                        int64_t np, int64_t ntiles,
-                                                       uint32_t* __restrict__ tile_counts) {
+                                                       uint32_t* __restrict__ tile_counts,
+                                                       uint16_t* __restrict__ hits) {
   __shared__ uint32_t sh[kWaves];
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t c[kItems], a[kItems];
@@ -400,6 +421,12 @@ __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* _
     uint32_t cnt = 0;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) cnt += c[j];
+    if (UNIQUE) {
+      uint32_t h = 0;
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) h |= c[j] << j;
+      hits[tile * kBlock + threadIdx.x] = (uint16_t)h;
+    }
     cnt = wave_sum(cnt);
     if (lane_id() == 0) sh[threadIdx.x >> 6] = cnt;
     __syncthreads();
@@ -441,53 +468,73 @@ __device__ __forceinline__ void gather_col(void* dst, const void* src, int bytes
   }
 }
 
+// Write pass for unique build keys: the count pass's match bits say which probe rows have their
+// (single) pair, so the probe keys and mask are not read again; the build row is looked up
+// only when build columns or build row ids are written (NEED_ROW).  The tile's pairs are listed
+// in output order in LDS, then each output column is written with consecutive threads on
+// consecutive outputs.
+template <typename K, bool HASH, bool NEED_ROW>
+__global__ __launch_bounds__(kBlock) void k_join_write_unique(
+    JoinView J, const void* __restrict__ pkeys, int pdt, int64_t ntiles,
+    const uint16_t* __restrict__ hits, const int64_t* __restrict__ tile_off, ColPack L,
+    ColPack Rc, int64_t* __restrict__ build_idx, int64_t* __restrict__ probe_idx) {
+  __shared__ uint32_t sh[kWaves];
+  __shared__ uint32_t fb[kTileRows];  // build row of the tile's i-th pair
+  __shared__ uint16_t fp[kTileRows];  // probe row - tile0 of the i-th pair
+  const int t = threadIdx.x;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t h16 = hits[tile * kBlock + t];
+    uint32_t a[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      a[j] = 0;
+      if (NEED_ROW && ((h16 >> j) & 1u))
+        lookup<K, HASH, true, true>(J, order_key<K>(load_int(pkeys, pdt, tile_row(tile, j))), true,
+                                    &a[j]);
+    }
+    const int64_t base = tile_off[tile];
+    uint32_t run = 0;
+#pragma unroll
+    for (int q = 0; q < kRuns; ++q) {
+      const uint32_t h4 = (h16 >> (4 * q)) & 15u;
+      uint32_t tot;
+      const uint32_t ex = block_exclusive_sum((uint32_t)__popc(h4), sh, &tot);
+      uint32_t o = run + ex;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if ((h4 >> i) & 1u) {
+          fb[o] = a[4 * q + i];
+          fp[o] = (uint16_t)(q * (4 * kBlock) + 4 * t + i);
+          ++o;
+        }
+      }
+      run += tot;
+    }
+    __syncthreads();
+    const int64_t tile0 = tile * kTileRows;
+    for (int x = 0; x < L.n; ++x)
+      gather_col<false>(L.dst[x], L.src[x], L.bytes[x], base, run, fb, fp, tile0);
+    for (int x = 0; x < Rc.n; ++x)
+      gather_col<true>(Rc.dst[x], Rc.src[x], Rc.bytes[x], base, run, fb, fp, tile0);
+    for (uint32_t i = t; i < run; i += kBlock) {
+      if (build_idx) build_idx[base + i] = fb[i];
+      if (probe_idx) probe_idx[base + i] = tile0 + fp[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Write pass for repeated build keys: repeats the lookups and expands each probe row's bucket.
 template <typename K, bool HASH, bool UNIQUE, bool NEED_ROW>
 __global__ __launch_bounds__(kBlock) void k_join_write(
     JoinView J, const void* __restrict__ pkeys, int pdt, const uint8_t* __restrict__ pmask,
     int64_t np, int64_t ntiles, const int64_t* __restrict__ tile_off, ColPack L, ColPack Rc,
     int64_t* __restrict__ build_idx, int64_t* __restrict__ probe_idx) {
   __shared__ uint32_t sh[kWaves];
-  __shared__ uint32_t fb[UNIQUE ? kTileRows : 1];  // build row of the tile's i-th pair
-  __shared__ uint16_t fp[UNIQUE ? kTileRows : 1];  // probe row - tile0 of the i-th pair
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t c[kItems], a[kItems];
-    probe_rows<K, HASH, UNIQUE, NEED_ROW || !UNIQUE>(J, pkeys, pdt, pmask, np, tile, c, a);
+    probe_rows<K, HASH, UNIQUE, true>(J, pkeys, pdt, pmask, np, tile, c, a);
     int64_t base = tile_off[tile];
-    if (UNIQUE) {
-      // at most one pair per probe row: list the tile's pairs in output order in LDS, then
-      // write each output column with consecutive threads on consecutive outputs
-      uint32_t run = 0;
-#pragma unroll
-      for (int q = 0; q < kRuns; ++q) {
-        uint32_t h4 = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h4 |= c[4 * q + i] << i;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_sum((uint32_t)__popc(h4), sh, &tot);
-        uint32_t o = run + ex;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if ((h4 >> i) & 1u) {
-            fb[o] = a[4 * q + i];
-            fp[o] = (uint16_t)(q * (4 * kBlock) + 4 * threadIdx.x + i);
-            ++o;
-          }
-        }
-        run += tot;
-      }
-      __syncthreads();
-      const int64_t tile0 = tile * kTileRows;
-      for (int x = 0; x < L.n; ++x)
-        gather_col<false>(L.dst[x], L.src[x], L.bytes[x], base, run, fb, fp, tile0);
-      for (int x = 0; x < Rc.n; ++x)
-        gather_col<true>(Rc.dst[x], Rc.src[x], Rc.bytes[x], base, run, fb, fp, tile0);
-      for (uint32_t i = threadIdx.x; i < run; i += kBlock) {
-        if (build_idx) build_idx[base + i] = fb[i];
-        if (probe_idx) probe_idx[base + i] = tile0 + fp[i];
-      }
-      __syncthreads();
-      continue;
-    }
 #pragma unroll
     for (int q = 0; q < kRuns; ++q) {
       const uint32_t tc = c[4 * q] + c[4 * q + 1] + c[4 * q + 2] + c[4 * q + 3];
@@ -550,7 +597,7 @@ static BuildWs carve_build(void* base, int64_t nb) {
   w.stats = (uint64_t*)p;
   w.dup = (uint32_t*)(p + 64);
   w.total = (int64_t*)(p + 128); p += 256;
-  w.occ = (uint32_t*)p; p += 256;
+  w.occ = (uint32_t*)p; p += kOccSlots * kOccStride * 4;
   w.scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(nt);
   w.part = (uint64_t*)p; p += align_up((size_t)nt * 32, 256);
   w.k0 = p; p += align_up((size_t)nb * 8, 256);
@@ -569,6 +616,7 @@ struct ProbeWs {
   int64_t* tile_off;
   int64_t* total;
   int64_t* scan_ws;
+  uint16_t* hits;  // unique build keys: per (tile, thread) match bits of the thread's 16 rows
   size_t bytes;
 };
 
@@ -581,6 +629,7 @@ static ProbeWs carve_probe(void* base, int64_t np) {
   w.tile_off = (int64_t*)p; p += align_up((size_t)nt * 8, 256);
   w.total = (int64_t*)p; p += 256;
   w.scan_ws = (int64_t*)p; p += tq_scan_ws_bytes(nt);
+  w.hits = (uint16_t*)p; p += align_up((size_t)nt * kBlock * 2, 256);
   w.bytes = (size_t)(p - (char*)base);
   return w;
 }
@@ -703,7 +752,8 @@ template <typename K, bool HASH, bool UNIQUE>
 static int count_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
                       const uint8_t* pmask, int64_t np, int64_t nt, ProbeWs& w, hipStream_t s) {
   return launch("k_join_count", (k_join_count<K, HASH, UNIQUE>), probe_grid(nt), kBlock, 0, s,
-                view_of(st), pkeys, pdt, pmask, np, nt, w.tile_counts);
+                view_of(st), pkeys, pdt, pmask, np, nt, w.tile_counts,
+                UNIQUE ? w.hits : (uint16_t*)nullptr);
 }
 
 template <typename K, bool HASH, bool UNIQUE>
@@ -711,13 +761,16 @@ static int write_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
                       const uint8_t* pmask, int64_t np, int64_t nt, const ProbeWs& w,
                       const ColPack& L, const ColPack& R, int64_t* bidx, int64_t* pidx,
                       hipStream_t s) {
-  if (L.n > 0 || bidx)
-    return launch("k_join_write", (k_join_write<K, HASH, UNIQUE, true>), probe_grid(nt), kBlock,
-                  0, s, view_of(st), pkeys, pdt, pmask, np, nt, (const int64_t*)w.tile_off, L, R,
-                  bidx, pidx);
-  return launch("k_join_write", (k_join_write<K, HASH, UNIQUE, false>), probe_grid(nt), kBlock, 0,
-                s, view_of(st), pkeys, pdt, pmask, np, nt, (const int64_t*)w.tile_off, L, R, bidx,
-                pidx);
+  const JoinView J = view_of(st);
+  const int64_t* off = (const int64_t*)w.tile_off;
+  if (UNIQUE && (L.n > 0 || bidx))
+    return launch("k_join_write", (k_join_write_unique<K, HASH, true>), probe_grid(nt), kBlock, 0,
+                  s, J, pkeys, pdt, nt, (const uint16_t*)w.hits, off, L, R, bidx, pidx);
+  if (UNIQUE)
+    return launch("k_join_write", (k_join_write_unique<K, HASH, false>), probe_grid(nt), kBlock,
+                  0, s, J, pkeys, pdt, nt, (const uint16_t*)w.hits, off, L, R, bidx, pidx);
+  return launch("k_join_write", (k_join_write<K, HASH, false, true>), probe_grid(nt), kBlock, 0, s,
+                J, pkeys, pdt, pmask, np, nt, off, L, R, bidx, pidx);
 }
 
 // Calls F<K, HASH, UNIQUE>::run(args...) for the state's key width and layout.
@@ -781,27 +834,23 @@ extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdt
     // settle the build's uniqueness with the same synchronisation as this count
     BuildWs bw = carve_build(st->ws, st->nb);
     const bool dense = st->layout == TQ_JOIN_DENSE;
-    const void* flag = dense ? (const void*)bw.occ : (const void*)bw.dup;
-    const size_t flag_bytes = dense ? kOccSlots * 4 : 4;
-    int64_t total = 0, fl[kOccSlots / 2] = {};
+    if (dense) {
+      // gather the spread counters into one word next to the duplicate flag
+      TQ_TRY(launch("k_join_occ_sum", k_join_occ_sum, 1, kBlock, 0, s, (const uint32_t*)bw.occ,
+                    (int64_t*)(bw.dup + 2)));
+    }
+    const void* flag = (const void*)bw.dup;  // dup (u32), pad, occupied-entry sum (i64)
+    int64_t total = 0, fl[2] = {};
     if (np > 0) {
       TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
       TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
-      TQ_TRY(read_back2(w.total, 8, &total, flag, flag_bytes, fl, s));
+      TQ_TRY(read_back2(w.total, 8, &total, flag, 16, fl, s));
     } else {
-      TQ_TRY(read_back(flag, flag_bytes, fl, s));
+      TQ_TRY(read_back(flag, 16, fl, s));
     }
     st->pending = 0;
-    bool dup;
-    if (dense) {
-      // unique exactly when every selected row occupies its own directory entry
-      const uint32_t* c = (const uint32_t*)fl;
-      int64_t occupied = 0;
-      for (int i = 0; i < kOccSlots; ++i) occupied += c[i];
-      dup = occupied != st->nsel;
-    } else {
-      dup = (fl[0] & 0xFFFFFFFF) != 0;
-    }
+    // dense: unique exactly when every selected row occupies its own directory entry
+    const bool dup = dense ? fl[1] != st->nsel : (fl[0] & 0xFFFFFFFF) != 0;
     if (!dup) {
       if (np <= 0) return tq_join_count(st, pkeys, pdtype, pmask, np, pws, total_host, stream);
       if (total_host) *total_host = total;

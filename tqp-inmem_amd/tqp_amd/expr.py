@@ -55,6 +55,21 @@ def _proto(v: Value):
     return torch.empty(1, dtype=v.dtype) if v.kind == "reg" else v.value
 
 
+_RESULT_TYPES = {}
+
+
+def _result_type(a: Value, b: Value):
+    """torch.result_type of two operands, memoised: it depends only on the tensors' dtypes and
+    the Python types of scalar literals."""
+    key = (a.dtype if a.kind == "reg" else type(a.value), a.kind == "reg",
+           b.dtype if b.kind == "reg" else type(b.value), b.kind == "reg")
+    dt = _RESULT_TYPES.get(key)
+    if dt is None:
+        dt = torch.result_type(_proto(a), _proto(b))
+        _RESULT_TYPES[key] = dt
+    return dt
+
+
 class ProgramBuilder:
     """Accumulates instructions, load slots and the column set of one program."""
 
@@ -70,6 +85,7 @@ class ProgramBuilder:
         self.strpool = bytearray()
         self.free = list(range(L.TQ_NREG))
         self.n = None
+        self.n_col = None
 
     # ---------------------------------------------------------------- columns and registers
     def column(self, t: torch.Tensor) -> int:
@@ -149,6 +165,7 @@ class ProgramBuilder:
             return Value("col2d", col=self.column(t), width=t.shape[1], dtype=t.dtype)
         if self.n is None:
             self.n = t.shape[0]
+            self.n_col = self.column(t)
         slot = self.load_slot(t)
         r = self.alloc()
         self.emit("LOAD", dst=r, a=slot)
@@ -205,7 +222,7 @@ class ProgramBuilder:
         name = {"<": "LT", "<=": "LE", ">": "GT", ">=": "GE", "=": "EQ", "<>": "NE", "!=": "NE"}[op]
         if a.is_lit and b.is_lit:
             raise NotImplementedError("comparison of two literals")
-        dt = torch.result_type(_proto(a), _proto(b))
+        dt = _result_type(a, b)
         if dt == torch.bool:
             dt = torch.int64
         return self.binary(name, a, b, dt, torch.bool)
@@ -213,7 +230,7 @@ class ProgramBuilder:
     def arith(self, op: str, a: Value, b: Value) -> Value:
         if a.is_lit and b.is_lit:
             raise NotImplementedError("arithmetic on two literals")
-        dt = torch.result_type(_proto(a), _proto(b))
+        dt = _result_type(a, b)
         if op == "/" and not dt.is_floating_point:
             dt = torch.get_default_dtype()
         if dt == torch.bool:
@@ -393,7 +410,7 @@ def compile_row_expr(b: ProgramBuilder, tree, tensors, agg_ref=None) -> Value:
             cond = b.truth(ev(tree[2]))
             then = ev(tree[4])
             other = ev(tree[6])
-            dt = torch.result_type(_proto(then), _proto(other))
+            dt = _result_type(then, other)
             rt = b.to_dtype(then, dt)
             ro = b.to_dtype(other, dt)
             b.emit("WHERE", dst=rt.reg, a=rt.reg, b=ro.reg, c=cond.reg)
@@ -458,6 +475,68 @@ def find_aggregates(tree, out):
 
 def literal_dtype(value):
     return torch.int64 if isinstance(value, int) and not isinstance(value, bool) else torch.float64
+
+
+# ---------------------------------------------------------------- compiled-program cache
+class BoundProgram:
+    """A cached program bound to the columns of this call: the program is shared (read-only once
+    finished), the column set is this call's."""
+
+    def __init__(self, prog, cols, col_tensors, n, load_bytes, extra):
+        self.prog = prog
+        self.cols = cols
+        self.col_tensors = col_tensors
+        self.n = n
+        self.nload = prog.nload
+        self.extra = extra
+        self._load_bytes = load_bytes
+
+    def load_bytes(self) -> int:
+        return self._load_bytes
+
+
+_PROGRAMS = {}
+_PROGRAMS_MAX = 512
+
+
+def cached_program(key, tensors, compile_fn):
+    """Compile with compile_fn(builder) -> extra once per key and bind later calls' columns to
+    the same program.  The key must fix everything the program text depends on except the
+    referenced columns themselves; those are found again by column id in `tensors` ({id: object
+    with .tensor}) and must keep their dtype and row shape, else the program is compiled anew.
+    Programs reading a column that is not one of `tensors` (a derived or reshaped tensor) are
+    not cached."""
+    ent = _PROGRAMS.get(key)
+    if ent is not None:
+        prog, refs, n_col, lb, extra = ent
+        cols = L.ColSet()
+        col_tensors = []
+        for i, (cid, dt, tail, width) in enumerate(refs):
+            v = tensors.get(cid)
+            t = v.tensor if v is not None else None
+            if t is None or t.dtype != dt or tuple(t.shape[1:]) != tail or not t.is_contiguous():
+                break
+            cols.ptr[i] = t.data_ptr()
+            cols.width[i] = width
+            col_tensors.append(t)
+        else:
+            n = col_tensors[n_col].shape[0] if n_col is not None else None
+            return BoundProgram(prog, cols, col_tensors, n, lb, extra)
+    b = ProgramBuilder()
+    extra = compile_fn(b)
+    b.finish()
+    b.extra = extra
+    ids = {id(v.tensor): cid for cid, v in tensors.items()}
+    refs = []
+    for i, t in enumerate(b.col_tensors):
+        cid = ids.get(id(t))
+        if cid is None:
+            return b
+        refs.append((cid, t.dtype, tuple(t.shape[1:]), int(b.cols.width[i])))
+    if len(_PROGRAMS) >= _PROGRAMS_MAX:
+        _PROGRAMS.clear()
+    _PROGRAMS[key] = (b.prog, refs, b.n_col, b.load_bytes(), extra)
+    return b
 
 
 def colset_ptr(b: ProgramBuilder):

@@ -30,8 +30,8 @@ def ntiles(n: int) -> int:
 
 def filter_mask(tree, tensors, subquery_result=None, n=None):
     """Bool mask of a predicate tree (operators/filter.py:14-162) plus per-tile counts."""
-    b = E.ProgramBuilder()
-    E.compile_predicate(b, tree, tensors, subquery_result)
+    b = E.cached_program(("filter", repr(tree), repr(subquery_result)), tensors,
+                         lambda pb: E.compile_predicate(pb, tree, tensors, subquery_result))
     rows = b.n if b.n is not None else n
     if rows is None:
         rows = next(iter(tensors.values())).tensor.shape[0]
@@ -39,8 +39,10 @@ def filter_mask(tree, tensors, subquery_result=None, n=None):
     mask = torch.empty(rows, dtype=torch.bool, device=dev)
     counts = torch.empty(max(ntiles(rows), 1), dtype=torch.int32, device=dev)
     if not b.col_tensors:
-        # no column referenced (e.g. only isnotnull): give the kernel a dummy column
-        b.column(mask)
+        # no column referenced (e.g. only isnotnull): the kernel's column set is never read,
+        # point it at the mask
+        b.cols.ptr[0] = mask.data_ptr()
+        b.cols.width[0] = 1
     # read each loaded column once, write one mask byte per row
     alg = rows * (1 + b.load_bytes())
     L.call("tq_filter_mask", E.program_ptr(b), E.colset_ptr(b), rows, L.ptr(mask), L.ptr(counts),
@@ -267,28 +269,38 @@ class GroupBy:
         if len(self.keys) > L.TQ_GB_MAX_KEYS:
             raise NotImplementedError("more than 8 group keys")
         self.mask = mask
-        b = E.ProgramBuilder()
-        for k in self.keys:          # key columns take load slots 0..nkeys-1
-            b.load_slot(k)
-        self.arg_dtypes = []
-        for op, tree in aggs:
-            v = E.compile_row_expr(b, tree, tensors)
-            if v.kind == "lit":
-                v = b.const(v.value, E.literal_dtype(v.value))
-            if v.kind != "reg":
-                raise NotImplementedError("string-valued aggregate argument")
-            self.arg_dtypes.append(v.dtype)
-            b.end_segment(v)
-        b.finish()
+
+        def compile_fn(b):
+            for k in self.keys:          # key columns take load slots 0..nkeys-1
+                b.load_slot(k)
+            dtypes = []
+            for op, tree in aggs:
+                v = E.compile_row_expr(b, tree, tensors)
+                if v.kind == "lit":
+                    v = b.const(v.value, E.literal_dtype(v.value))
+                if v.kind != "reg":
+                    raise NotImplementedError("string-valued aggregate argument")
+                dtypes.append(v.dtype)
+                b.end_segment(v)
+            return dtypes
+
+        # the program depends on which columns are the keys, the aggregate trees and dtypes
+        ids = {id(v.tensor): cid for cid, v in tensors.items()}
+        key_cids = tuple(ids.get(id(k)) for k in self.keys)
+        b = E.cached_program(("groupby", key_cids, repr(list(aggs))), tensors, compile_fn)
+        self.arg_dtypes = list(b.extra)
         self.builder = b
         rows = n
         if rows is None:
             rows = self.keys[0].shape[0] if self.keys else (b.n if b.n is not None else 0)
         self.n = rows
         if not b.col_tensors:
-            dummy = torch.zeros(max(rows, 1), dtype=torch.int8,
-                                device=mask.device if mask is not None else 'cuda')
-            b.column(dummy)
+            # no column read (e.g. count(*) without keys): the column set is never read
+            self._dummy = torch.zeros(max(rows, 1), dtype=torch.int8,
+                                      device=mask.device if mask is not None else 'cuda')
+            b.col_tensors = [self._dummy]
+            b.cols.ptr[0] = self._dummy.data_ptr()
+            b.cols.width[0] = 1
         self.ops = [self.OPS[op] for op, _ in aggs]
 
     def run(self):
