@@ -259,6 +259,16 @@ size_t tq_join_build_ws_bytes(int64_t nb);
 int tq_join_build(const void* bkeys, int32_t bdtype, const uint8_t* bmask, int64_t nb, void* ws,
                   size_t ws_bytes, tq_join_state_t* state /* host out */, void* stream);
 
+/* tq_join_build for a caller that knows bounds of the build key column (key_lo <= every key
+ * <= key_hi, e.g. column statistics recorded when the table was loaded): the bounds pick the
+ * layout, so the build launches without synchronising.  state->nsel is -1 until the first
+ * tq_join_count settles it.  A key outside the bounds is never trusted: that count detects it
+ * and builds again as tq_join_build does.  Same reference interface as tq_join_build
+ * (operators/hashjoin.py:41-52). */
+int tq_join_build_ranged(const void* bkeys, int32_t bdtype, const uint8_t* bmask, int64_t nb,
+                         int64_t key_lo, int64_t key_hi, void* ws, size_t ws_bytes,
+                         tq_join_state_t* state /* host out */, void* stream);
+
 /* Workspace for probing np rows. */
 size_t tq_join_probe_ws_bytes(int64_t np);
 

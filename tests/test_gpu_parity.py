@@ -18,9 +18,9 @@ pytestmark = pytest.mark.gpu
 FP_RTOL = 1e-12
 
 
-def _gpu_group(tables, dev):
+def _gpu_group(tables, dev, stats=True):
     from tqp_amd import workloads
-    return workloads.to_tensor_group(tables, dev)
+    return workloads.to_tensor_group(tables, dev, stats=stats)
 
 
 def _assert_col(got, want, name, fp=False):
@@ -128,6 +128,25 @@ def test_sort_multikey_bit_exact(device, n):
     assert np.array_equal(perm.cpu().numpy(), CF.sort_perm([f], [True])[:10])
 
 
+@pytest.mark.parametrize("n,k", [(4097, 1), (70_000, 10), (70_000, 64), (1_000_000, 64),
+                                 (1_000_000, 10), (300_000, 65), (300_000, 256)])
+def test_sort_limit_bit_exact(device, n, k):
+    # limits up to 64 take the wave-list rounds, 65..256 the radix-select rounds; heavy ties,
+    # NaN/-0.0 and two packed keys check the stable (key, row) order of the first k
+    from tqp_amd import kernels
+    g = np.random.default_rng(n + k)
+    f = np.round(g.normal(size=n), 1)
+    f[g.random(n) < 0.05] = -0.0
+    f[g.random(n) < 0.02] = np.nan
+    a = g.integers(-3, 3, n, dtype=np.int32)
+    b = g.integers(0, 2, n, dtype=np.int8)
+    for keys, descs in (([f], [True]), ([f], [False]), ([a, b], [True, False]),
+                        ([b, a], [False, False])):
+        dk = [(torch.from_numpy(x).to(device), d) for x, d in zip(keys, descs)]
+        perm = kernels.sort_perm(dk, n, limit=k)
+        assert np.array_equal(perm.cpu().numpy(), CF.sort_perm(keys, descs)[:k]), (descs, k)
+
+
 def test_sort_2d_string_and_int32_keys(device):
     from tqp_amd import kernels
     g = np.random.default_rng(7)
@@ -140,23 +159,29 @@ def test_sort_2d_string_and_int32_keys(device):
 
 
 # ------------------------------------------------------------------ join
-def _run_join(device, bk, bm, bv, pk, pm, pv, kind='inner', conds=()):
+def _run_join(device, bk, bm, bv, pk, pm, pv, kind='inner', conds=(), stats=True, bounds=None):
+    # stats: load-time column bounds (the build skips its statistics pass); bounds: override
+    # the build key's recorded bounds
+    from tqp_amd import kernels
     from tqp_amd.operators import hashjoin
     tables = {1: ('int', bk), 2: ('int', bv), 3: ('int', pk), 4: ('float', pv),
               901: ('bool', bm), 902: ('bool', pm)}
     args = {'left group': [1, 2], 'right group': [3, 4], 'materialized in': [1, 2, 3, 4],
             'mask map': {1: 901, 2: 901, 3: 902, 4: 902}}
-    gpu = _gpu_group(tables, device)
+    gpu = _gpu_group(tables, device, stats=stats)
+    if bounds is not None:
+        kernels.note_bounds(gpu[1].tensor, *bounds)
     cpu = R.to_cols(tables)
     hashjoin.join_vortex(True, gpu, copy.deepcopy(args), 1, 3, list(conds), kind, None, None, "j")
     R.join_vortex(cpu, copy.deepcopy(args), 1, 3, list(conds), kind)
     return gpu, cpu
 
 
+@pytest.mark.parametrize("stats", [True, False], ids=["bounded", "stats_pass"])
 @pytest.mark.parametrize("case", ["unique", "dups", "skew", "empty_build", "empty_probe", "int64",
                                   "hash_dups", "int64_max", "dense_int64_dups", "narrow_build",
                                   "wide_probe"])
-def test_join_inner_bit_exact(device, case):
+def test_join_inner_bit_exact(device, case, stats):
     # dense directory: unique, dups, skew, dense_int64_dups, narrow_build, wide_probe;
     # hash slots: int64 (unique keys), hash_dups, int64_max (largest int64 key present)
     g = np.random.default_rng(sum(map(ord, case)))
@@ -198,7 +223,35 @@ def test_join_inner_bit_exact(device, case):
     pm = g.random(np_) < (0.0 if case == "empty_probe" else 0.53)
     bv = g.integers(0, 1 << 40, nb, dtype=np.int64)
     pv = g.random(np_)
-    gpu, cpu = _run_join(device, bk, bm, bv, pk, pm, pv)
+    gpu, cpu = _run_join(device, bk, bm, bv, pk, pm, pv, stats=stats)
+    for k in (1, 2, 3, 4):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"{case} col {k}")
+
+
+@pytest.mark.parametrize("case", ["narrow_unique", "narrow_dups", "narrow_int64", "wide_hash"])
+def test_join_bounds_that_do_not_hold(device, case):
+    # recorded bounds that miss some build keys: the first count finds the keys outside them,
+    # and the index is built again from the keys' own statistics
+    g = np.random.default_rng(sum(map(ord, case)) + 5)
+    nb, np_ = 40_000, 120_000
+    if case == "narrow_unique":
+        bk = g.permutation(np.arange(1, nb + 1, dtype=np.int32))
+        bounds = (100, nb - 100)
+    elif case == "narrow_dups":
+        bk = g.integers(0, 3000, nb, dtype=np.int32)
+        bounds = (10, 2000)
+    elif case == "narrow_int64":
+        bk = g.integers(-(1 << 50), 1 << 50, nb, dtype=np.int64)
+        bounds = (-(1 << 49), 1 << 49)
+    else:
+        bk = g.permutation(np.arange(1, nb + 1, dtype=np.int32)) * 1000
+        bounds = (0, 500 * nb)
+    pk = np.where(g.random(np_) < 0.6, bk[g.integers(0, nb, np_)], g.integers(0, int(bk.max()), np_))
+    pk = pk.astype(bk.dtype)
+    bm = g.random(nb) < 0.5
+    pm = g.random(np_) < 0.5
+    gpu, cpu = _run_join(device, bk, bm, g.integers(0, 99, nb, dtype=np.int64), pk, pm,
+                         g.random(np_), bounds=bounds)
     for k in (1, 2, 3, 4):
         _assert_col(gpu[k].tensor, cpu[k].tensor, f"{case} col {k}")
 

@@ -51,7 +51,7 @@ __host__ __device__ inline uint64_t slot_of(uint64_t t, int bits) {
 }
 
 template <typename K>
-__device__ __forceinline__ K order_key(int64_t v) {
+__host__ __device__ __forceinline__ K order_key(int64_t v) {
   return sizeof(K) == 4 ? (K)((uint32_t)(int32_t)v ^ 0x80000000u)
                         : (K)((uint64_t)v ^ 0x8000000000000000ull);
 }
@@ -219,9 +219,16 @@ __global__ __launch_bounds__(kBlock) void k_join_stats_final(const uint32_t* __r
 }
 
 // ------------------------------------------------------------------ build: unique claim
+// Also zeroes the uniqueness-check words (flags[0..5]) and the occupied-entry counters when
+// given: a build with host-known key bounds has no statistics pass to do it.
 template <bool HASH>
 __global__ __launch_bounds__(kBlock) void k_join_clear(void* __restrict__ table, int64_t n,
-                                                       uint64_t empty) {
+                                                       uint64_t empty, uint32_t* __restrict__ flags,
+                                                       uint32_t* __restrict__ occ) {
+  if (flags && blockIdx.x == 0) {
+    if (threadIdx.x < 6) flags[threadIdx.x] = 0;
+    if (threadIdx.x < kOccSlots) occ[threadIdx.x * kOccStride] = 0;
+  }
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     if (HASH) {
@@ -242,17 +249,36 @@ template <typename K, bool HASH>
 __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ bkeys, int dt,
                                                        const uint8_t* __restrict__ mask,
                                                        int64_t n, JoinView J,
-                                                       uint32_t* __restrict__ dup) {
+                                                       uint32_t* __restrict__ dup,
+                                                       unsigned long long* __restrict__ nsel,
+                                                       uint64_t kmin, uint64_t kspan,
+                                                       uint32_t* __restrict__ oob) {
   const int64_t tile = blockIdx.x;
   uint32_t sel;
   int64_t v[kItems];
   load_tile_rows(bkeys, dt, mask, n, tile, &sel, v);
-  bool seen = false;
+  if (nsel) {  // selected rows: one add per workgroup
+    __shared__ uint32_t wc[kWaves];
+    const uint32_t c = wave_sum((uint32_t)__popc(sel));
+    if (lane_id() == 0) wc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t b = 0;
+#pragma unroll
+      for (int i = 0; i < kWaves; ++i) b += wc[i];
+      if (b) atomicAdd(nsel, (unsigned long long)b);
+    }
+  }
+  bool seen = false, out = false;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     if (!((sel >> j) & 1u)) continue;
     const uint64_t t = (uint64_t)order_key<K>(v[j]);
     const uint32_t row = (uint32_t)tile_row(tile, j);
+    if (oob && t - kmin > kspan) {  // outside the bounds the caller promised: not claimed
+      out = true;
+      continue;
+    }
     if (!HASH) {
       ((uint32_t*)J.table)[t - J.base] = row;
     } else {
@@ -275,6 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ 
     }
   }
   if (seen) *dup = 1;
+  if (out) *oob = 1;
 }
 
 // Presence bitmap of a dense directory: bit i of bits[w] is set when entry 32*w + i holds a key.
@@ -672,7 +699,7 @@ static int build_multi(tq_join_state_t* st, BuildWs& w, hipStream_t s) {
   const unsigned cg = stream_grid(st->range);
   if (hash) {
     TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
-                  st->empty_key));
+                  st->empty_key, (uint32_t*)nullptr, (uint32_t*)nullptr));
     TQ_TRY(launch("k_join_runs", (k_join_runs<K, true>), stream_grid(nsel), kBlock, 0, s, keys,
                   nsel, J));
   } else {
@@ -686,28 +713,17 @@ static int build_multi(tq_join_state_t* st, BuildWs& w, hipStream_t s) {
   return TQ_OK;
 }
 
+// Layout for selected transformed keys within [kmin, kmax]; a hash table gets at least twice
+// cap_rows slots.
 template <typename K>
-static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64_t nb, BuildWs& w,
-                      tq_join_state_t* st, hipStream_t s) {
-  const int64_t nt = ceil_div(nb, kTileRows);
-  TQ_TRY(launch("k_join_stats", k_join_stats<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask,
-                nb, w.tile_counts, w.part));
-  TQ_TRY(launch("k_join_stats_final", k_join_stats_final, 1, kBlock, 0, s,
-                (const uint32_t*)w.tile_counts, (const uint64_t*)w.part, nt, w.stats, w.dup,
-                w.occ));
-  int64_t hv[5];
-  TQ_TRY(read_back(w.stats, 40, hv, s));  // the one synchronisation: count and key statistics
-  const int64_t nsel = hv[0];
-  st->nsel = nsel;
-  if (nsel == 0) return TQ_OK;
-  st->key_or = (uint64_t)hv[1];
-  st->key_and = (uint64_t)hv[2];
-  const uint64_t kmin = (uint64_t)hv[3], kmax = (uint64_t)hv[4];
+static int set_layout(tq_join_state_t* st, uint64_t kmin, uint64_t kmax, int64_t cap_rows,
+                      int64_t nb) {
   const bool hash = kmax - kmin >= (uint64_t)dense_cap(nb);
   st->layout = hash ? TQ_JOIN_HASH : TQ_JOIN_DENSE;
   if (hash) {
-    st->hash_bits = hash_bits_for(nsel);
+    st->hash_bits = hash_bits_for(cap_rows);
     st->range = (int64_t)1 << st->hash_bits;
+    st->base = 0;
     if (sizeof(K) == 4 || kmax != ~0ull) {
       st->empty_key = sizeof(K) == 4 ? ~0ull : kmax + 1;
     } else if (kmin != 0) {
@@ -721,20 +737,55 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
     st->base = kmin;
     st->range = (int64_t)(kmax - kmin) + 1;
   }
+  return TQ_OK;
+}
+
+// Statistics pass: selected rows, OR/AND and range of their keys (one synchronisation).
+template <typename K>
+static int build_stats(const void* bkeys, int32_t dt, const uint8_t* bmask, int64_t nb,
+                       BuildWs& w, tq_join_state_t* st, hipStream_t s, uint64_t* kmin,
+                       uint64_t* kmax) {
+  const int64_t nt = ceil_div(nb, kTileRows);
+  TQ_TRY(launch("k_join_stats", k_join_stats<K>, (unsigned)nt, kBlock, 0, s, bkeys, dt, bmask,
+                nb, w.tile_counts, w.part));
+  TQ_TRY(launch("k_join_stats_final", k_join_stats_final, 1, kBlock, 0, s,
+                (const uint32_t*)w.tile_counts, (const uint64_t*)w.part, nt, w.stats, w.dup,
+                w.occ));
+  int64_t hv[5];
+  TQ_TRY(read_back(w.stats, 40, hv, s));
+  st->nsel = hv[0];
+  st->key_or = (uint64_t)hv[1];
+  st->key_and = (uint64_t)hv[2];
+  *kmin = (uint64_t)hv[3];
+  *kmax = (uint64_t)hv[4];
+  return TQ_OK;
+}
+
+// Claim pass assuming unique keys; the check is read with the first pair count.  A ranged
+// build (key bounds from the caller, no statistics pass) also clears the check words, counts
+// the selected rows and flags keys outside the bounds.
+template <typename K>
+static int claim_unique(const void* bkeys, int32_t dt, const uint8_t* bmask, int64_t nb,
+                        BuildWs& w, tq_join_state_t* st, bool ranged, uint64_t kmin,
+                        uint64_t kmax, hipStream_t s) {
+  const int64_t nt = ceil_div(nb, kTileRows);
   st->table = w.table;
   JoinView J = view_of(st);
   const unsigned cg = stream_grid(st->range);
-  // assume unique keys; the duplicate flag is read with the first pair count
+  uint32_t* clear_flags = ranged ? w.dup : nullptr;
+  unsigned long long* nsel_dev = ranged ? (unsigned long long*)(w.dup + 4) : nullptr;
+  uint32_t* oob = ranged ? w.dup + 1 : nullptr;
+  const bool hash = st->layout == TQ_JOIN_HASH;
   if (hash) {
     TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
-                  st->empty_key));
+                  st->empty_key, clear_flags, w.occ));
     TQ_TRY(launch("k_join_claim", (k_join_claim<K, true>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
-                  bmask, nb, J, w.dup));
+                  bmask, nb, J, w.dup, nsel_dev, kmin, kmax - kmin, oob));
   } else {
     TQ_TRY(launch("k_join_clear", k_join_clear<false>, cg, kBlock, 0, s, w.table, st->range,
-                  st->empty_key));
+                  st->empty_key, clear_flags, w.occ));
     TQ_TRY(launch("k_join_claim", (k_join_claim<K, false>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
-                  bmask, nb, J, w.dup));
+                  bmask, nb, J, w.dup, nsel_dev, kmin, kmax - kmin, oob));
     st->bits = w.bits;
     TQ_TRY(launch("k_join_bits", k_join_bits<true>, cg, kBlock, 0, s, (const void*)w.table,
                   st->range, w.bits, w.occ));
@@ -746,6 +797,48 @@ static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64
   st->bmask = bmask;
   st->nb = nb;
   return TQ_OK;
+}
+
+template <typename K>
+static int build_impl(const void* bkeys, int32_t dt, const uint8_t* bmask, int64_t nb, BuildWs& w,
+                      tq_join_state_t* st, hipStream_t s) {
+  uint64_t kmin, kmax;
+  TQ_TRY(build_stats<K>(bkeys, dt, bmask, nb, w, st, s, &kmin, &kmax));
+  if (st->nsel == 0) {
+    st->layout = TQ_JOIN_EMPTY;
+    return TQ_OK;
+  }
+  TQ_TRY(set_layout<K>(st, kmin, kmax, st->nsel, nb));
+  return claim_unique<K>(bkeys, dt, bmask, nb, w, st, false, 0, 0, s);
+}
+
+// Build with caller-given bounds lo <= every build key <= hi: no statistics pass, so no
+// synchronisation; the selected row count is settled by the first pair count (nsel = -1 until
+// then).
+template <typename K>
+static int build_ranged(const void* bkeys, int32_t dt, const uint8_t* bmask, int64_t nb,
+                        int64_t lo, int64_t hi, BuildWs& w, tq_join_state_t* st, hipStream_t s) {
+  const uint64_t kmin = (uint64_t)order_key<K>(lo), kmax = (uint64_t)order_key<K>(hi);
+  st->nsel = -1;
+  st->key_or = 0;
+  st->key_and = ~0ull;
+  TQ_TRY(set_layout<K>(st, kmin, kmax, nb, nb));
+  return claim_unique<K>(bkeys, dt, bmask, nb, w, st, true, kmin, kmax, s);
+}
+
+// The build's statistics again (after a ranged build's check failed), then the layout for
+// repeated keys or, when the check failed on the bounds, the unique claim.
+template <typename K>
+static int rebuild_from_stats(tq_join_state_t* st, BuildWs& w, bool repeated, hipStream_t s) {
+  uint64_t kmin, kmax;
+  TQ_TRY(build_stats<K>(st->bkeys, st->bdtype, st->bmask, st->nb, w, st, s, &kmin, &kmax));
+  if (st->nsel == 0) {
+    st->layout = TQ_JOIN_EMPTY;
+    return TQ_OK;
+  }
+  TQ_TRY(set_layout<K>(st, kmin, kmax, st->nsel, st->nb));
+  if (repeated) return build_multi<K>(st, w, s);
+  return claim_unique<K>(st->bkeys, st->bdtype, st->bmask, st->nb, w, st, false, 0, 0, s);
 }
 
 template <typename K, bool HASH, bool UNIQUE>
@@ -821,6 +914,32 @@ extern "C" int tq_join_build(const void* bkeys, int32_t bdtype, const uint8_t* b
                           : build_impl<uint32_t>(bkeys, bdtype, bmask, nb, w, state, s);
 }
 
+extern "C" int tq_join_build_ranged(const void* bkeys, int32_t bdtype, const uint8_t* bmask,
+                                    int64_t nb, int64_t key_lo, int64_t key_hi, void* ws,
+                                    size_t ws_bytes, tq_join_state_t* state, void* stream) {
+  if (!state) return set_error(TQ_EINVAL, "join build: null state");
+  if (bdtype < TQ_BOOL || bdtype > TQ_I64)
+    return set_error(TQ_EUNSUPPORTED, "join keys must be integer columns (dtype %d)", bdtype);
+  if (nb >= ((int64_t)1 << 32) - 1) return set_error(TQ_EINVAL, "build side too large");
+  if (key_lo > key_hi) return set_error(TQ_EINVAL, "join build: key bounds %lld > %lld",
+                                        (long long)key_lo, (long long)key_hi);
+  BuildWs w = carve_build(ws, nb);
+  if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "join build workspace too small");
+  *state = tq_join_state_t{};
+  state->key_bits = bdtype == TQ_I64 ? 64 : 32;
+  state->layout = TQ_JOIN_EMPTY;
+  if (nb <= 0) return TQ_OK;
+  hipStream_t s = (hipStream_t)stream;
+  state->ws = ws;
+  if (bdtype != TQ_I64) {  // bounds beyond the 32-bit key range cannot hold a key
+    key_lo = key_lo < INT32_MIN ? INT32_MIN : key_lo;
+    key_hi = key_hi > INT32_MAX ? INT32_MAX : key_hi;
+    if (key_lo > key_hi) return build_impl<uint32_t>(bkeys, bdtype, bmask, nb, w, state, s);
+    return build_ranged<uint32_t>(bkeys, bdtype, bmask, nb, key_lo, key_hi, w, state, s);
+  }
+  return build_ranged<uint64_t>(bkeys, bdtype, bmask, nb, key_lo, key_hi, w, state, s);
+}
+
 extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdtype,
                              const uint8_t* pmask, int64_t np, void* pws, int64_t* total_host,
                              void* stream) {
@@ -839,24 +958,43 @@ extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdt
       TQ_TRY(launch("k_join_occ_sum", k_join_occ_sum, 1, kBlock, 0, s, (const uint32_t*)bw.occ,
                     (int64_t*)(bw.dup + 2)));
     }
-    const void* flag = (const void*)bw.dup;  // dup (u32), pad, occupied-entry sum (i64)
-    int64_t total = 0, fl[2] = {};
+    // dup (u32), out-of-bounds (u32), occupied-entry sum (i64), selected rows (i64, ranged)
+    const void* flag = (const void*)bw.dup;
+    int64_t total = 0, fl[3] = {};
     if (np > 0) {
       TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
       TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
-      TQ_TRY(read_back2(w.total, 8, &total, flag, 16, fl, s));
+      TQ_TRY(read_back2(w.total, 8, &total, flag, 24, fl, s));
     } else {
-      TQ_TRY(read_back(flag, 16, fl, s));
+      TQ_TRY(read_back(flag, 24, fl, s));
     }
     st->pending = 0;
+    const bool ranged = st->nsel < 0;
+    const bool is64 = st->key_bits == 64;
+    if (ranged) {
+      st->nsel = fl[2];
+      if ((fl[0] >> 32) != 0) {
+        // a key outside the bounds given to the build: build again from the keys' statistics
+        TQ_TRY(is64 ? rebuild_from_stats<uint64_t>(st, bw, false, s)
+                    : rebuild_from_stats<uint32_t>(st, bw, false, s));
+        return tq_join_count(st, pkeys, pdtype, pmask, np, pws, total_host, stream);
+      }
+      if (st->nsel == 0) st->layout = TQ_JOIN_EMPTY;  // nothing claimed: the count was 0
+    }
     // dense: unique exactly when every selected row occupies its own directory entry
-    const bool dup = dense ? fl[1] != st->nsel : (fl[0] & 0xFFFFFFFF) != 0;
+    const bool dup = st->nsel > 0 && (dense ? fl[1] != st->nsel : (fl[0] & 0xFFFFFFFF) != 0);
     if (!dup) {
       if (np <= 0) return tq_join_count(st, pkeys, pdtype, pmask, np, pws, total_host, stream);
       if (total_host) *total_host = total;
       return TQ_OK;
     }
-    TQ_TRY(st->key_bits == 64 ? build_multi<uint64_t>(st, bw, s) : build_multi<uint32_t>(st, bw, s));
+    if (ranged) {
+      // repeated keys need the statistics (per-tile counts, key bits) for the sorted layout
+      TQ_TRY(is64 ? rebuild_from_stats<uint64_t>(st, bw, true, s)
+                  : rebuild_from_stats<uint32_t>(st, bw, true, s));
+    } else {
+      TQ_TRY(is64 ? build_multi<uint64_t>(st, bw, s) : build_multi<uint32_t>(st, bw, s));
+    }
   }
   if (np <= 0 || st->nsel <= 0 || st->layout == TQ_JOIN_EMPTY) {
     TQ_TRY(fill_u32(w.tile_counts, 0, nt, s));

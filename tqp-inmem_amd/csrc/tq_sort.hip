@@ -434,6 +434,148 @@ __global__ __launch_bounds__(kBlock) void k_topk_select(const uint64_t* __restri
   }
 }
 
+// ------------------------------------------------------------------ small top-k (k <= 64)
+// A workgroup reduces 4096 candidates (16 per thread) to its k smallest (key, row) pairs, in
+// order, without sorting them:
+//   1. every thread takes the smallest of its 16;
+//   2. T = the k-th smallest of those 256 minima (each minimum's rank among them, by counting).
+//      k minima are <= T, so the workgroup's k smallest are all <= T, and only the (at most k)
+//      threads whose minimum is <= T hold pairs <= T: at most 16k survivors;
+//   3. the survivors go to LDS and each one's rank among them, by counting, is its output slot.
+// A round turns m candidates into k per workgroup; the last round (one workgroup) writes the
+// rows of its first nout pairs as the permutation.  Rows are distinct, so (key, row) order is
+// the stable key order and every rank is unique.
+constexpr int kSmallK = 64;
+constexpr int kListPer = 16;                    // candidates per thread and round
+constexpr int kListTile = kBlock * kListPer;    // candidates per workgroup and round
+constexpr int kSurvMax = kSmallK * kListPer;    // survivors of step 2
+
+__device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t ra, uint64_t kb, uint32_t rb) {
+  return ka < kb || (ka == kb && ra < rb);
+}
+
+// Sort key bits of rows base + j*kBlock (j < kListPer) from the key fields; rows at or past n
+// are flagged invalid through *valid.
+__device__ __forceinline__ void field_keys(const FieldPack& fp, int64_t base, int64_t n,
+                                           uint64_t key[kListPer], uint32_t* valid) {
+  uint32_t v = 0;
+#pragma unroll
+  for (int j = 0; j < kListPer; ++j) {
+    key[j] = 0;
+    v |= (base + (int64_t)j * kBlock < n ? 1u : 0u) << j;
+  }
+  *valid = v;
+  for (int f = 0; f < fp.n; ++f) {
+    const Field F = fp.f[f];
+#pragma unroll
+    for (int j = 0; j < kListPer; ++j)
+      if ((v >> j) & 1u) key[j] |= field_bits(F, base + (int64_t)j * kBlock);
+  }
+}
+
+// One round.  Source: the key fields of rows 0..m-1 (kin == null) or m (key, row) pairs.
+// Writes the workgroup's k smallest in order to kout/vout[blockIdx.x*k ..] (sentinels when it
+// has fewer), or, in the last round (perm_out != null, one workgroup), the rows of the first
+// nout as the permutation.
+__global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint64_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin, int64_t m,
+                                                       int k, uint64_t* __restrict__ kout,
+                                                       uint32_t* __restrict__ vout,
+                                                       int64_t* __restrict__ perm_out, int64_t nout) {
+  __shared__ uint64_t mk[kBlock];
+  __shared__ uint32_t mr[kBlock];
+  __shared__ uint64_t sk[kSurvMax];
+  __shared__ uint32_t sr[kSurvMax];
+  __shared__ uint64_t tk;
+  __shared__ uint32_t tr, ns;
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kListTile + t;
+  uint64_t key[kListPer];
+  uint32_t row[kListPer];
+  uint32_t valid = 0;
+  if (kin) {
+#pragma unroll
+    for (int j = 0; j < kListPer; ++j) {
+      const int64_t i = base + (int64_t)j * kBlock;
+      const bool ok = i < m;
+      valid |= (ok ? 1u : 0u) << j;
+      key[j] = ok ? kin[i] : ~0ull;
+      row[j] = ok ? vin[i] : 0xFFFFFFFFu;
+    }
+  } else {
+    field_keys(fp, base, m, key, &valid);
+#pragma unroll
+    for (int j = 0; j < kListPer; ++j) row[j] = (uint32_t)(base + (int64_t)j * kBlock);
+  }
+#pragma unroll
+  for (int j = 0; j < kListPer; ++j) {  // invalid candidates become sentinels
+    if (!((valid >> j) & 1u)) {
+      key[j] = ~0ull;
+      row[j] = 0xFFFFFFFFu;
+    }
+  }
+  // 1. the thread's minimum
+  uint64_t ak = key[0];
+  uint32_t ar = row[0];
+#pragma unroll
+  for (int j = 1; j < kListPer; ++j) {
+    if (pair_less(key[j], row[j], ak, ar)) {
+      ak = key[j];
+      ar = row[j];
+    }
+  }
+  mk[t] = ak;
+  mr[t] = ar;
+  if (t == 0) {
+    tk = ~0ull;
+    tr = 0xFFFFFFFFu;
+    ns = 0;
+  }
+  __syncthreads();
+  // 2. T: the minimum whose rank among the minima is k-1 (stays the sentinel when fewer than k
+  //    threads hold a real candidate)
+  if (ar != 0xFFFFFFFFu) {
+    int rank = 0;
+    for (int i = 0; i < kBlock; ++i) rank += pair_less(mk[i], mr[i], ak, ar) ? 1 : 0;
+    if (rank == k - 1) {
+      tk = ak;
+      tr = ar;
+    }
+  }
+  __syncthreads();
+  const uint64_t Tk = tk;
+  const uint32_t Tr = tr;
+  // 3. survivors <= T into LDS, then each one's rank among them
+#pragma unroll
+  for (int j = 0; j < kListPer; ++j) {
+    if (row[j] != 0xFFFFFFFFu && !pair_less(Tk, Tr, key[j], row[j])) {
+      const uint32_t p = atomicAdd(&ns, 1u);
+      sk[p] = key[j];
+      sr[p] = row[j];
+    }
+  }
+  __syncthreads();
+  const int s = (int)ns;
+  for (int i = t; i < s; i += kBlock) {
+    const uint64_t xk = sk[i];
+    const uint32_t xr = sr[i];
+    int rank = 0;
+    for (int q = 0; q < s; ++q) rank += pair_less(sk[q], sr[q], xk, xr) ? 1 : 0;
+    if (perm_out) {
+      if (rank < nout) perm_out[rank] = xr;
+    } else if (rank < k) {
+      kout[(int64_t)blockIdx.x * k + rank] = xk;
+      vout[(int64_t)blockIdx.x * k + rank] = xr;
+    }
+  }
+  if (!perm_out) {
+    for (int i = s + t; i < k; i += kBlock) {
+      kout[(int64_t)blockIdx.x * k + i] = ~0ull;
+      vout[(int64_t)blockIdx.x * k + i] = 0xFFFFFFFFu;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_perm_out(const uint32_t* __restrict__ v, int64_t m,
                                                      int64_t* __restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -562,8 +704,9 @@ using namespace tq;
 
 extern "C" size_t tq_sort_ws_bytes(int64_t n) { return sort_ws_bytes(n); }
 
-// Top-k for a limit of at most 256 over keys that pack into one 64-bit sort key: pack, then
-// rounds of per-tile bitonic selection; no host synchronisation.
+// Top-k for a limit of at most 256 over keys that pack into one 64-bit sort key, with no host
+// synchronisation: rounds of per-workgroup threshold selection for a limit of at most 64, else
+// packing and rounds of per-tile radix selection.
 static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t limit,
                      int64_t* perm_out, void* ws, size_t ws_bytes, hipStream_t s, bool* done) {
   *done = false;
@@ -574,6 +717,7 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
     bits += bits_of(keys[k].dtype);
   }
   if (bits > 64 || nkeys > 16 || nkeys < 1) return TQ_OK;
+  if (n >= (int64_t)0xFFFFFFFF) return TQ_OK;  // rows must stay below the sentinel row
   if (ws_bytes < sort_ws_bytes(n)) return set_error(TQ_ECAPACITY, "sort workspace too small");
   int shift = 0;
   for (int k = nkeys - 1; k >= 0; --k) {
@@ -583,9 +727,31 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
   }
   fp.n = nkeys;
   SortWs w = carve_sort_ws(ws, n);
-  TQ_TRY(launch("k_pack_keys", k_pack_keys<uint64_t>, stream_grid(n), kBlock, 0, s, fp, nullptr, n, w.k0, w.v0));
   const int k = (int)limit;
   const int64_t out = limit < n ? limit : n;
+  if (k >= 1 && k <= kSmallK) {
+    // rounds of per-workgroup selection straight from the key columns; each keeps k per 4096
+    const uint64_t* ki = nullptr;
+    const uint32_t* vi = nullptr;
+    uint64_t* ko = w.k0;
+    uint32_t* vo = w.v0;
+    int64_t m = n;
+    while (true) {
+      const int64_t blocks = ceil_div(m, kListTile);
+      const bool last = blocks == 1;
+      TQ_TRY(launch("k_topk_block", k_topk_block, (unsigned)blocks, kBlock, 0, s, fp, ki, vi, m,
+                    k, ko, vo, last ? perm_out : (int64_t*)nullptr, out));
+      if (last) break;
+      m = blocks * k;
+      ki = ko;
+      vi = vo;
+      ko = ki == w.k0 ? w.k1 : w.k0;
+      vo = vi == w.v0 ? w.v1 : w.v0;
+    }
+    *done = true;
+    return TQ_OK;
+  }
+  TQ_TRY(launch("k_pack_keys", k_pack_keys<uint64_t>, stream_grid(n), kBlock, 0, s, fp, nullptr, n, w.k0, w.v0));
   uint64_t *ki = w.k0, *ko = w.k1;
   uint32_t *vi = nullptr, *vo = w.v1;
   int64_t m = n;

@@ -131,6 +131,7 @@ _SIGNATURES = {
     "tq_sort_perm": ([P, I32, I64, I64, P, P, SZ, P], I32),
     "tq_join_build_ws_bytes": ([I64], SZ),
     "tq_join_build": ([P, I32, P, I64, P, SZ, P, P], I32),
+    "tq_join_build_ranged": ([P, I32, P, I64, I64, I64, P, SZ, P, P], I32),
     "tq_join_probe_ws_bytes": ([I64], SZ),
     "tq_join_count": ([P, P, I32, P, I64, P, P, P], I32),
     "tq_join_write": ([P, P, I32, P, I64, P, I32, P, P, P, I32, P, P, P, P, P, P], I32),

@@ -191,12 +191,20 @@ class JoinIndex:
         ws_bytes = L.size_call("tq_join_build_ws_bytes", nb)
         self.ws = L.workspace(ws_bytes, bkey.device, "join_build")
         self.state = L.JoinState()
-        L.call("tq_join_build", L.ptr(self.bkey), L.dtype_code(self.bkey.dtype),
-               L.ptr(bmask) if bmask is not None else None, nb, L.ptr(self.ws), ws_bytes,
-               ctypes.byref(self.state), L.stream_handle())
+        bm = L.ptr(bmask) if bmask is not None else None
+        kb = known_bounds(self.bkey)
+        if kb is not None:
+            # bounds of the whole key column pick the layout: no statistics pass, no wait
+            L.call("tq_join_build_ranged", L.ptr(self.bkey), L.dtype_code(self.bkey.dtype), bm, nb,
+                   kb[0], kb[1], L.ptr(self.ws), ws_bytes, ctypes.byref(self.state),
+                   L.stream_handle())
+        else:
+            L.call("tq_join_build", L.ptr(self.bkey), L.dtype_code(self.bkey.dtype), bm, nb,
+                   L.ptr(self.ws), ws_bytes, ctypes.byref(self.state), L.stream_handle())
 
     @property
     def nsel(self):
+        """Selected build rows (-1 after a bounded build until the first probe count)."""
         return int(self.state.nsel)
 
     def key_bounds(self):
@@ -232,10 +240,15 @@ class JoinIndex:
                 raise NotImplementedError("join with more than 16 columns per side")
             lsrc = [c.contiguous() for c in left_cols]
             rsrc = [c.contiguous() for c in right_cols]
-            # probe keys + mask once; per pair: its build row id, the gathered inputs and the
-            # written outputs (and the index pairs when requested)
-            per_pair = 4 + 2 * sum(L.row_bytes(c) for c in lsrc + rsrc) + (16 if want_index else 0)
-            alg = np_ * (pkey.element_size() + (1 if pmask is not None else 0)) + m * per_pair
+            # the join's boundary bytes less what the build and count passes read: every other
+            # gathered input column read once over its table, every output written once
+            seen = {self.bkey.data_ptr(), pkey.data_ptr()}
+            reads = 0
+            for c in lsrc + rsrc:
+                if c.data_ptr() not in seen:
+                    seen.add(c.data_ptr())
+                    reads += c.shape[0] * L.row_bytes(c)
+            alg = reads + m * (sum(L.row_bytes(c) for c in lsrc + rsrc) + (16 if want_index else 0))
             L.call("tq_join_write", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype),
                    pm, np_, L.ptr(pws), len(lsrc), L.ptr_array(lsrc),
                    L.int_array([L.row_bytes(c) for c in lsrc]), L.ptr_array(lout), len(rsrc),

@@ -190,11 +190,18 @@ def config3_pipeline(i, n_probe, n_build, c_build, c_probe, device):
 
 
 # ---------------------------------------------------------------- helpers
-def to_tensor_group(tables, device):
-    """{id: (type, ndarray)} -> {id: Variable} on `device`."""
+def to_tensor_group(tables, device, stats=True):
+    """{id: (type, ndarray)} -> {id: Variable} on `device`.  With stats, the minimum and maximum
+    of every 1-D integer column are taken on the host while it is loaded and recorded as the
+    tensor's value bounds (column statistics, as a columnar file's footer carries them), which
+    lets a join build over that column pick its layout without a device round trip."""
+    from . import kernels
     group = {}
     for cid, (ty, arr) in tables.items():
-        t = torch.from_numpy(np.ascontiguousarray(arr)).to(device)
+        a = np.ascontiguousarray(arr)
+        t = torch.from_numpy(a).to(device)
+        if stats and a.ndim == 1 and a.size and a.dtype.kind in "iu" and t.is_cuda:
+            kernels.note_bounds(t, int(a.min()), int(a.max()))
         group[cid] = Variable(t, ty, None, VariableState.GPU if t.is_cuda else VariableState.CPU)
     return group
 
