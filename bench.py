@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the config-3 batched comparison")
     ap.add_argument("--batch-pipes", type=int, default=256)
     ap.add_argument("--serve", action="store_true", help="add the served-form sweep")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the config-0 and config-2 plain-form lines")
     ap.add_argument("--op-sync", action="store_true",
                     help="synchronise after every operator, as the reference executor does")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -88,6 +90,53 @@ def cpu_baseline(tables, plan, runs):
             "kind": "port",
             "sample": f"{runs} full config-1 pipelines (7.5M input rows each) through the "
                       f"torch-CPU restatement of the reference operators, {dt:.1f} s"}
+
+
+CONFIG_WORKLOADS = {
+    0: "filter -> 6-group aggregate (7 aggregates) -> sort; 1,000,000 rows",
+    2: "3-join star (fact 60,000,000 rows, Zipf fk_b) -> ~2,500-group sum/count -> full sort",
+}
+
+
+def configs_section(dev, steps=20, warmup=5):
+    """Configs 0 and 2 in the plain per-operator form: input rows/s, ms per pipeline and the
+    dominant kernel's achieved HBM bandwidth (dispatch-stamped events, as for config 1)."""
+    import torch
+    from tqp_amd import _lib, executor, workloads
+    out = {}
+    for cfg in (0, 2):
+        tables = getattr(workloads, f"config{cfg}_tables")()
+        plan = getattr(workloads, f"config{cfg}_plan")()
+        base = workloads.to_tensor_group(tables, dev)
+        rows = workloads.input_rows(tables, cfg)
+        del tables
+        plans = [copy.deepcopy(plan) for _ in range(steps + warmup)]
+        for _ in range(warmup):
+            executor.execute(plans.pop(), dict(base), sync=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            executor.execute(plans.pop(), dict(base), sync=False)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        plans = [copy.deepcopy(plan) for _ in range(steps)]
+        _lib.start_trace()
+        for _ in range(steps):
+            executor.execute(plans.pop(), dict(base), sync=False)
+        calls, launches = _lib.stop_trace()
+        kt = kernel_table(calls, launches)
+        with_bytes = {k: v for k, v in kt.items() if "GBps" in v}
+        dom = max(with_bytes, key=lambda k: with_bytes[k]["total_ms"]) if with_bytes else None
+        out[f"config{cfg}"] = {
+            "workload": CONFIG_WORKLOADS[cfg], "input_rows": rows, "steps": steps,
+            "rows_per_s": rows / dt, "ms_per_step": dt * 1e3,
+            "kernel_ms_per_step": round(sum(v["total_ms"] for v in kt.values()) / steps, 4),
+            "dominant_kernel": dict(with_bytes[dom], kernel=dom) if dom else None,
+            "kernels": dict(list(kt.items())[:8])}
+        del base
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return out
 
 
 def batched_section(pipes_n, dev):
@@ -237,6 +286,8 @@ def main():
         "ms_per_step_op_sync": side["op_sync"],
         "kernels": kernels,
     }
+    if not args.no_configs:
+        line["configs"] = configs_section(dev)
     if not args.no_batched:
         line["batched"] = batched_section(args.batch_pipes, dev)
     if args.serve:

@@ -100,6 +100,18 @@ def test_compaction_bit_exact_and_ordered(device, n):
     assert np.array_equal(rows.cpu().numpy(), np.flatnonzero(m))
 
 
+@pytest.mark.parametrize("n", [1, 63, 4096, 4097, 300_001, 5_000_000])
+def test_tile_scan_exact(device, n):
+    # one chunk, two-level and ragged scans of per-tile counts into int64 offsets
+    from tqp_amd import kernels
+    g = np.random.default_rng(n)
+    c = g.integers(0, 4097, n, dtype=np.int64).astype(np.int32)
+    offs, total = kernels.scan_tiles(torch.from_numpy(c).to(device), n)
+    want = np.concatenate([[0], np.cumsum(c.astype(np.int64))])
+    assert np.array_equal(offs.cpu().numpy(), want[:-1])
+    assert int(total.item()) == int(want[-1])
+
+
 def test_compaction_empty_and_full(device):
     from tqp_amd import kernels
     for m in (np.zeros(10000, bool), np.ones(10000, bool)):

@@ -363,18 +363,54 @@ __global__ __launch_bounds__(kBlock) void k_mask_counts(const uint8_t* __restric
 }
 
 // ------------------------------------------------------------------ scans
-// A scan chunk is 4096 inputs: thread t owns the 16 consecutive inputs [16t, 16t+16).
+// A scan chunk is 4096 u32 inputs: thread t owns the 16 consecutive inputs [16t, 16t + 16),
+// read with four 16-byte loads and written back with 16-byte stores.
 constexpr int kScanChunk = kBlock * kItems;
 
-template <typename TIn, typename TOut>
-__global__ __launch_bounds__(kBlock) void k_chunk_sums(const TIn* __restrict__ in, int64_t n,
+__device__ __forceinline__ void load16_u32(const uint32_t* in, int64_t b0, int64_t n,
+                                           uint32_t (&v)[kItems]) {
+  if (b0 + kItems <= n && ((uintptr_t)(in + b0) & 15) == 0) {
+#pragma unroll
+    for (int q = 0; q < kItems / 4; ++q) {
+      const uint4 x = *(const uint4*)(in + b0 + 4 * q);
+      v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) v[j] = b0 + j < n ? in[b0 + j] : 0u;
+  }
+}
+
+template <typename TOut>
+__device__ __forceinline__ void store16(TOut* out, int64_t b0, int64_t n, const TOut (&v)[kItems]) {
+  if (b0 + kItems <= n && ((uintptr_t)(out + b0) & 15) == 0) {
+    if (sizeof(TOut) == 4) {
+#pragma unroll
+      for (int q = 0; q < kItems / 4; ++q)
+        *(uint4*)(out + b0 + 4 * q) = make_uint4((uint32_t)v[4 * q], (uint32_t)v[4 * q + 1],
+                                                 (uint32_t)v[4 * q + 2], (uint32_t)v[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < kItems / 2; ++q)
+        *(longlong2*)(out + b0 + 2 * q) = make_longlong2((long long)v[2 * q], (long long)v[2 * q + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      if (b0 + j < n) out[b0 + j] = v[j];
+  }
+}
+
+template <typename TOut>
+__global__ __launch_bounds__(kBlock) void k_chunk_sums(const uint32_t* __restrict__ in, int64_t n,
                                                        TOut* __restrict__ sums) {
   __shared__ TOut sh[kWaves];
   const int64_t b0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kItems;
+  uint32_t v[kItems];
+  load16_u32(in, b0, n, v);
   TOut s = 0;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j)
-    if (b0 + j < n) s += (TOut)in[b0 + j];
+  for (int j = 0; j < kItems; ++j) s += (TOut)v[j];
   s = wave_sum(s);
   if (lane_id() == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -386,31 +422,39 @@ __global__ __launch_bounds__(kBlock) void k_chunk_sums(const TIn* __restrict__ i
   }
 }
 
-// Exclusive scan of one chunk; adds chunk_base[blockIdx.x] when given.  The last chunk writes
-// the grand total to *total when total is non-null.
-template <typename TIn, typename TOut>
-__global__ __launch_bounds__(kBlock) void k_scan_chunk(const TIn* __restrict__ in, int64_t n,
+// Exclusive scan of one chunk; with chunk_sums (the sums of all chunks), the chunk first adds
+// up the sums of the chunks before it.  The last chunk writes the grand total to *total when
+// total is non-null.
+template <typename TOut>
+__global__ __launch_bounds__(kBlock) void k_scan_chunk(const uint32_t* __restrict__ in, int64_t n,
                                                        TOut* __restrict__ out,
-                                                       const TOut* __restrict__ chunk_base,
+                                                       const TOut* __restrict__ chunk_sums,
                                                        TOut* __restrict__ total) {
   __shared__ TOut sh[kWaves];
   const int64_t b0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kItems;
-  TOut v[kItems];
+  TOut prior = 0;
+  if (chunk_sums) {  // at most kScanChunk chunks: 16 sums per thread
+    TOut p = 0;
+    for (int i = threadIdx.x; i < (int)blockIdx.x; i += kBlock) p += chunk_sums[i];
+    TOut tp;
+    block_exclusive_sum(p, sh, &tp);
+    prior = tp;
+  }
+  uint32_t v[kItems];
+  load16_u32(in, b0, n, v);
   TOut s = 0;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    v[j] = (b0 + j < n) ? (TOut)in[b0 + j] : (TOut)0;
-    s += v[j];
-  }
+  for (int j = 0; j < kItems; ++j) s += (TOut)v[j];
   TOut tot;
-  TOut run = block_exclusive_sum(s, sh, &tot);
-  if (chunk_base) run += chunk_base[blockIdx.x];
+  TOut run = prior + block_exclusive_sum(s, sh, &tot);
+  TOut o[kItems];
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
-    if (b0 + j < n) out[b0 + j] = run;
-    run += v[j];
+    o[j] = run;
+    run += (TOut)v[j];
   }
-  if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total = run;
+  store16(out, b0, n, o);
+  if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = prior + tot;
 }
 
 template <typename TIn, typename TOut>
@@ -421,13 +465,16 @@ int scan_exclusive(const TIn* in, int64_t n, TOut* out, TOut* total, TOut* ws,
     if (total) return sizeof(TOut) == 8 ? fill_u64(total, 0, 1, stream) : fill_u32(total, 0, 1, stream);
     return TQ_OK;
   }
+  static_assert(sizeof(TIn) == 4, "scans read u32 counts");
   if (nchunks == 1) {
-    return launch("k_scan_chunk", (k_scan_chunk<TIn, TOut>), 1, kBlock, 0, stream, in, n, out, nullptr, total);
+    return launch("k_scan_chunk", (k_scan_chunk<TOut>), 1, kBlock, 0, stream, (const uint32_t*)in,
+                  n, out, (const TOut*)nullptr, total);
   }
   if (nchunks > kScanChunk) return set_error(TQ_EINVAL, "scan of %lld elements too large", (long long)n);
-  TQ_TRY(launch("k_chunk_sums", (k_chunk_sums<TIn, TOut>), (unsigned)nchunks, kBlock, 0, stream, in, n, ws));
-  TQ_TRY(launch("k_scan_chunk", (k_scan_chunk<TOut, TOut>), 1, kBlock, 0, stream, ws, nchunks, ws, nullptr, nullptr));
-  return launch("k_scan_chunk", (k_scan_chunk<TIn, TOut>), (unsigned)nchunks, kBlock, 0, stream, in, n, out, ws, total);
+  TQ_TRY(launch("k_chunk_sums", (k_chunk_sums<TOut>), (unsigned)nchunks, kBlock, 0, stream,
+                (const uint32_t*)in, n, ws));
+  return launch("k_scan_chunk", (k_scan_chunk<TOut>), (unsigned)nchunks, kBlock, 0, stream,
+                (const uint32_t*)in, n, out, (const TOut*)ws, total);
 }
 
 template int scan_exclusive<uint32_t, uint32_t>(const uint32_t*, int64_t, uint32_t*, uint32_t*,

@@ -392,13 +392,33 @@ __global__ __launch_bounds__(kBlock) void k_gb_write_dense(GbSpec G, OutCols O, 
 // b, and its exclusive scan (partition-major) is where the tile's rows of partition b go, so
 // the scatter needs no global atomics.
 
-// Packed key of row j of the sub-tile starting at sub0 (rows sub0 + j*256 + t), read from the
-// key columns.
-__device__ __forceinline__ uint64_t packed_key_at(const KeyCols& K, const GbSpec& G, int64_t row) {
-  uint64_t p = 0;
-  for (int k = 0; k < G.nkeys; ++k)
-    p += (uint64_t)(load_int(K.ptr[k], K.dtype[k], row) - G.key_min[k]) * (uint64_t)G.key_stride[k];
-  return p;
+// Packed keys of rows sub0 + j*256 + t (j < kSub; rows not live read row 0 and are ignored):
+// per key column the dtype is dispatched once and the kSub loads are issued together.
+template <typename T>
+__device__ __forceinline__ void add_key4(const void* col, int64_t sub0, const bool (&live)[kSub],
+                                         int64_t kmin, int64_t stride, uint64_t (&pk)[kSub]) {
+  const T* p = (const T*)col;
+  T x[kSub];
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) x[j] = p[live[j] ? sub0 + (int64_t)j * kBlock + threadIdx.x : 0];
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) pk[j] += (uint64_t)((int64_t)x[j] - kmin) * (uint64_t)stride;
+}
+
+__device__ __forceinline__ void packed_keys4(const KeyCols& K, const GbSpec& G, int64_t sub0,
+                                             const bool (&live)[kSub], uint64_t (&pk)[kSub]) {
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) pk[j] = 0;
+  for (int k = 0; k < G.nkeys; ++k) {
+    const int64_t lo = G.key_min[k], st = G.key_stride[k];
+    switch (K.dtype[k]) {
+      case TQ_BOOL: add_key4<uint8_t>(K.ptr[k], sub0, live, lo, st, pk); break;  // 0 or 1
+      case TQ_I8: add_key4<int8_t>(K.ptr[k], sub0, live, lo, st, pk); break;
+      case TQ_I16: add_key4<int16_t>(K.ptr[k], sub0, live, lo, st, pk); break;
+      case TQ_I32: add_key4<int32_t>(K.ptr[k], sub0, live, lo, st, pk); break;
+      default: add_key4<int64_t>(K.ptr[k], sub0, live, lo, st, pk); break;
+    }
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
@@ -412,11 +432,17 @@ __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
   const int64_t tile = blockIdx.x;
   for (int sub = 0; sub < tsub; ++sub) {
     const int64_t sub0 = (tile * tsub + sub) * kSubRows;
+    bool live[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) {
       const int64_t r = sub0 + (int64_t)j * kBlock + t;
-      if (r < n && (!mask || mask[r] != 0)) atomicAdd(&lh[packed_key_at(K, G, r) >> pb], 1u);
+      live[j] = r < n && (!mask || mask[r] != 0);
     }
+    uint64_t pk[kSub];
+    packed_keys4(K, G, sub0, live, pk);
+#pragma unroll
+    for (int j = 0; j < kSub; ++j)
+      if (live[j]) atomicAdd(&lh[pk[j] >> pb], 1u);
   }
   __syncthreads();
   for (int i = t; i < nparts; i += kBlock) hist[(int64_t)i * ptiles + tile] = lh[i];
@@ -491,11 +517,30 @@ __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
   __syncthreads();
   const uint32_t r0 = hoff[b * ptiles];
   const uint32_t r1 = b + 1 < nparts ? hoff[(b + 1) * ptiles] : *htotal;
-  for (uint32_t r = r0 + t; r < r1; r += kBlock) {
-    const uint32_t slot = rec_key[r];
-    atomicAdd(&lcnt[slot], 1u);
-    for (int a = 0; a < G.naggs; ++a)
-      acc_apply(&lacc[a * ps + slot], G.kind[a], rec_val[(int64_t)a * cap + r]);
+  constexpr int U = 8;  // records per thread whose loads are issued together
+  for (uint32_t base = r0 + t; base < r1; base += U * kBlock) {
+    uint32_t sl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t r = base + u * kBlock;
+      sl[u] = r < r1 ? rec_key[r] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + u * kBlock < r1) atomicAdd(&lcnt[sl[u]], 1u);
+    for (int a = 0; a < G.naggs; ++a) {
+      const uint64_t* rv = rec_val + (int64_t)a * cap;
+      uint64_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t r = base + u * kBlock;
+        v[u] = r < r1 ? rv[r] : 0ull;
+      }
+      const int kind = G.kind[a];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (base + u * kBlock < r1) acc_apply(&lacc[a * ps + sl[u]], kind, v[u]);
+    }
   }
   __syncthreads();
   // stable compaction of the occupied slots: slot i = j*256 + t, ranked by wave ballots and one
@@ -546,14 +591,35 @@ __global__ __launch_bounds__(kBlock) void k_gbp_write(GbSpec G, OutCols O, int p
   const int64_t o0 = grp_off[b];
   const int64_t gbase = b << pb;
   const int64_t gstride = (int64_t)nparts << pb;
-  for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
-    const int64_t g = gbase + i;
-    const uint64_t pk = (uint64_t)gbase + gslot[g];
-    for (int k = 0; k < G.nkeys; ++k) store_int(O.key[k], G.key_dtype[k], o0 + i, key_of(G, k, pk));
-    const uint32_t rows = gcnt[g];
-    for (int a = 0; a < G.naggs; ++a)
-      write_agg(O.agg[a], o0 + i, O.op[a], G.kind[a], G.arg_dtype[a], gacc[(int64_t)a * gstride + g],
-                rows);
+  constexpr int U = 4;  // groups per thread whose loads are issued together
+  for (uint32_t i0 = threadIdx.x; i0 < m; i0 += U * kBlock) {
+    uint32_t sl[U], rows[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * kBlock;
+      sl[u] = i < m ? gslot[gbase + i] : 0u;
+      rows[u] = i < m ? gcnt[gbase + i] : 0u;
+    }
+    for (int k = 0; k < G.nkeys; ++k) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t i = i0 + u * kBlock;
+        if (i < m) store_int(O.key[k], G.key_dtype[k], o0 + i, key_of(G, k, (uint64_t)gbase + sl[u]));
+      }
+    }
+    for (int a = 0; a < G.naggs; ++a) {
+      uint64_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t i = i0 + u * kBlock;
+        v[u] = i < m ? gacc[(int64_t)a * gstride + gbase + i] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t i = i0 + u * kBlock;
+        if (i < m) write_agg(O.agg[a], o0 + i, O.op[a], G.kind[a], G.arg_dtype[a], v[u], rows[u]);
+      }
+    }
   }
 }
 

@@ -220,7 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_join_stats_final(const uint32_t* __r
 
 // ------------------------------------------------------------------ build: unique claim
 // Also zeroes the uniqueness-check words (flags[0..5]) and the occupied-entry counters when
-// given: a build with host-known key bounds has no statistics pass to do it.
+// given: a build with host-known key bounds has no statistics pass to do it.  A dense
+// directory is filled four entries per thread (16-byte stores).
 template <bool HASH>
 __global__ __launch_bounds__(kBlock) void k_join_clear(void* __restrict__ table, int64_t n,
                                                        uint64_t empty, uint32_t* __restrict__ flags,
@@ -230,15 +231,19 @@ __global__ __launch_bounds__(kBlock) void k_join_clear(void* __restrict__ table,
     if (threadIdx.x < kOccSlots) occ[threadIdx.x * kOccStride] = 0;
   }
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    if (HASH) {
+  if (HASH) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
       Slot s;
       s.key = empty; s.a = 0; s.cnt = 0;
       ((Slot*)table)[i] = s;
-    } else {
-      ((uint32_t*)table)[i] = kNoPos;
     }
+    return;
   }
+  const int64_t n4 = n >> 2;  // the directory is 256-byte aligned
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += stride)
+    ((uint4*)table)[i] = make_uint4(kNoPos, kNoPos, kNoPos, kNoPos);
+  for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    ((uint32_t*)table)[i] = kNoPos;
 }
 
 // Every selected build row claims its key's entry; in a hash table a key claimed twice sets
@@ -250,14 +255,14 @@ __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ 
                                                        const uint8_t* __restrict__ mask,
                                                        int64_t n, JoinView J,
                                                        uint32_t* __restrict__ dup,
-                                                       unsigned long long* __restrict__ nsel,
+                                                       uint32_t* __restrict__ tile_counts,
                                                        uint64_t kmin, uint64_t kspan,
                                                        uint32_t* __restrict__ oob) {
   const int64_t tile = blockIdx.x;
   uint32_t sel;
   int64_t v[kItems];
   load_tile_rows(bkeys, dt, mask, n, tile, &sel, v);
-  if (nsel) {  // selected rows: one add per workgroup
+  if (tile_counts) {  // selected rows of the tile
     __shared__ uint32_t wc[kWaves];
     const uint32_t c = wave_sum((uint32_t)__popc(sel));
     if (lane_id() == 0) wc[threadIdx.x >> 6] = c;
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ 
       uint32_t b = 0;
 #pragma unroll
       for (int i = 0; i < kWaves; ++i) b += wc[i];
-      if (b) atomicAdd(nsel, (unsigned long long)b);
+      tile_counts[tile] = b;
     }
   }
   bool seen = false, out = false;
@@ -305,30 +310,45 @@ __global__ __launch_bounds__(kBlock) void k_join_claim(const void* __restrict__ 
 }
 
 // Presence bitmap of a dense directory: bit i of bits[w] is set when entry 32*w + i holds a key.
-// A wave tests 64 consecutive entries and its ballot is two bitmap words.  With occ, the number
-// of occupied entries is added to occ[wave % kOccSlots]: after the plain-store claim, the keys
+// A thread tests four consecutive entries (one 16-byte load for unique directories), so a wave
+// covers 256 entries = 8 bitmap words, each assembled from 8 lanes.  With occ, the number of
+// occupied entries is added to occ[block % kOccSlots]: after the plain-store claim, the keys
 // were unique exactly when that number equals the selected row count.
 template <bool UNIQUE>
 __global__ __launch_bounds__(kBlock) void k_join_bits(const void* __restrict__ table,
                                                       int64_t range, uint32_t* __restrict__ bits,
                                                       uint32_t* __restrict__ occ) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  const int64_t nw = (range + 31) >> 5;
+  const int64_t stride = (int64_t)gridDim.x * kBlock * 4;
+  const int l = lane_id();
   uint32_t c = 0;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; (e & ~63ll) < range; e += stride) {
-    const bool present = e < range && (UNIQUE ? ((const uint32_t*)table)[e] != kNoPos
-                                              : ((const uint2*)table)[e].y != 0u);
-    const uint64_t m = ballot(present);
-    c += (uint32_t)__popcll(m);
-    if (lane_id() == 0) {
-      const int64_t w = e >> 5;
-      bits[w] = (uint32_t)m;
-      if (w + 1 < nw) bits[w + 1] = (uint32_t)(m >> 32);
+  for (int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4; (e & ~255ll) < range;
+       e += stride) {
+    uint32_t nib = 0;
+    if (UNIQUE && e + 3 < range) {
+      const uint4 q = *(const uint4*)((const uint32_t*)table + e);
+      nib = (q.x != kNoPos ? 1u : 0u) | (q.y != kNoPos ? 2u : 0u) | (q.z != kNoPos ? 4u : 0u) |
+            (q.w != kNoPos ? 8u : 0u);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t x = e + i;
+        const bool p = x < range && (UNIQUE ? ((const uint32_t*)table)[x] != kNoPos
+                                            : ((const uint2*)table)[x].y != 0u);
+        nib |= (p ? 1u : 0u) << i;
+      }
     }
+    c += (uint32_t)__popc(nib);
+    // word (e / 32) takes the nibbles of lanes 8*(l/8) .. 8*(l/8)+7
+    uint32_t word = nib << (4 * (l & 7));
+    word |= __shfl_xor(word, 1, 64);
+    word |= __shfl_xor(word, 2, 64);
+    word |= __shfl_xor(word, 4, 64);
+    if ((l & 7) == 0 && e < range) bits[e >> 5] = word;
   }
   if (!occ) return;
   __shared__ uint32_t wc[kWaves];
-  if (lane_id() == 0) wc[threadIdx.x >> 6] = c;
+  c = wave_sum(c);
+  if (l == 0) wc[threadIdx.x >> 6] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t b = 0;
@@ -339,12 +359,27 @@ __global__ __launch_bounds__(kBlock) void k_join_bits(const void* __restrict__ t
   }
 }
 
-// Sum of the occupied-entry counters into *out (one workgroup).
-__global__ __launch_bounds__(kBlock) void k_join_occ_sum(const uint32_t* __restrict__ occ,
-                                                         int64_t* __restrict__ out) {
-  int64_t v = threadIdx.x < kOccSlots ? occ[threadIdx.x * kOccStride] : 0;
-  v = wave_sum(v);
-  if (threadIdx.x == 0) *out = v;  // the counters are in the first wave
+// The uniqueness check's numbers next to the duplicate flag (one workgroup): the occupied-entry
+// sum (dense directories) and, after a ranged build, the selected rows (sum of tile counts).
+__global__ __launch_bounds__(kBlock) void k_join_check(const uint32_t* __restrict__ occ,
+                                                       const uint32_t* __restrict__ tile_counts,
+                                                       int64_t ntiles, int64_t* __restrict__ out) {
+  __shared__ int64_t sh[kWaves];
+  int64_t v = (occ && threadIdx.x < kOccSlots) ? occ[threadIdx.x * kOccStride] : 0;
+  v = wave_sum(v);  // the counters are in the first wave
+  if (threadIdx.x == 0) out[0] = v;
+  if (!tile_counts) return;
+  int64_t c = 0;
+  for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) c += tile_counts[i];
+  c = wave_sum(c);
+  if (lane_id() == 0) sh[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) t += sh[i];
+    out[1] = t;
+  }
 }
 
 // ------------------------------------------------------------------ build: duplicate keys
@@ -707,7 +742,7 @@ static int build_multi(tq_join_state_t* st, BuildWs& w, hipStream_t s) {
     TQ_TRY(launch("k_join_runs", (k_join_runs<K, false>), stream_grid(nsel), kBlock, 0, s, keys,
                   nsel, J));
     st->bits = w.bits;
-    TQ_TRY(launch("k_join_bits", k_join_bits<false>, stream_grid(st->range), kBlock, 0, s,
+    TQ_TRY(launch("k_join_bits", k_join_bits<false>, stream_grid(ceil_div(st->range, 4)), kBlock, 0, s,
                   (const void*)w.table, st->range, w.bits, (uint32_t*)nullptr));
   }
   return TQ_OK;
@@ -773,21 +808,22 @@ static int claim_unique(const void* bkeys, int32_t dt, const uint8_t* bmask, int
   JoinView J = view_of(st);
   const unsigned cg = stream_grid(st->range);
   uint32_t* clear_flags = ranged ? w.dup : nullptr;
-  unsigned long long* nsel_dev = ranged ? (unsigned long long*)(w.dup + 4) : nullptr;
+  uint32_t* tcounts = ranged ? w.tile_counts : nullptr;
   uint32_t* oob = ranged ? w.dup + 1 : nullptr;
+  const unsigned cg4 = stream_grid(ceil_div(st->range, 4));
   const bool hash = st->layout == TQ_JOIN_HASH;
   if (hash) {
     TQ_TRY(launch("k_join_clear", k_join_clear<true>, cg, kBlock, 0, s, w.table, st->range,
                   st->empty_key, clear_flags, w.occ));
     TQ_TRY(launch("k_join_claim", (k_join_claim<K, true>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
-                  bmask, nb, J, w.dup, nsel_dev, kmin, kmax - kmin, oob));
+                  bmask, nb, J, w.dup, tcounts, kmin, kmax - kmin, oob));
   } else {
-    TQ_TRY(launch("k_join_clear", k_join_clear<false>, cg, kBlock, 0, s, w.table, st->range,
+    TQ_TRY(launch("k_join_clear", k_join_clear<false>, cg4, kBlock, 0, s, w.table, st->range,
                   st->empty_key, clear_flags, w.occ));
     TQ_TRY(launch("k_join_claim", (k_join_claim<K, false>), (unsigned)nt, kBlock, 0, s, bkeys, dt,
-                  bmask, nb, J, w.dup, nsel_dev, kmin, kmax - kmin, oob));
+                  bmask, nb, J, w.dup, tcounts, kmin, kmax - kmin, oob));
     st->bits = w.bits;
-    TQ_TRY(launch("k_join_bits", k_join_bits<true>, cg, kBlock, 0, s, (const void*)w.table,
+    TQ_TRY(launch("k_join_bits", k_join_bits<true>, cg4, kBlock, 0, s, (const void*)w.table,
                   st->range, w.bits, w.occ));
   }
   st->unique = 1;
@@ -953,10 +989,13 @@ extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdt
     // settle the build's uniqueness with the same synchronisation as this count
     BuildWs bw = carve_build(st->ws, st->nb);
     const bool dense = st->layout == TQ_JOIN_DENSE;
-    if (dense) {
-      // gather the spread counters into one word next to the duplicate flag
-      TQ_TRY(launch("k_join_occ_sum", k_join_occ_sum, 1, kBlock, 0, s, (const uint32_t*)bw.occ,
-                    (int64_t*)(bw.dup + 2)));
+    if (dense || st->nsel < 0) {
+      // the spread occupied-entry counters and (ranged build) the per-tile selected counts,
+      // summed into the two words after the duplicate flag
+      TQ_TRY(launch("k_join_check", k_join_check, 1, kBlock, 0, s,
+                    dense ? (const uint32_t*)bw.occ : (const uint32_t*)nullptr,
+                    st->nsel < 0 ? (const uint32_t*)bw.tile_counts : (const uint32_t*)nullptr,
+                    ceil_div(st->nb, kTileRows), (int64_t*)(bw.dup + 2)));
     }
     // dup (u32), out-of-bounds (u32), occupied-entry sum (i64), selected rows (i64, ranged)
     const void* flag = (const void*)bw.dup;

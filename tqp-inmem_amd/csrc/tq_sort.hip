@@ -437,11 +437,14 @@ __global__ __launch_bounds__(kBlock) void k_topk_select(const uint64_t* __restri
 // ------------------------------------------------------------------ small top-k (k <= 64)
 // A workgroup reduces 4096 candidates (16 per thread) to its k smallest (key, row) pairs, in
 // order, without sorting them:
-//   1. every thread takes the smallest of its 16;
-//   2. T = the k-th smallest of those 256 minima (each minimum's rank among them, by counting).
-//      k minima are <= T, so the workgroup's k smallest are all <= T, and only the (at most k)
-//      threads whose minimum is <= T hold pairs <= T: at most 16k survivors;
-//   3. the survivors go to LDS and each one's rank among them, by counting, is its output slot.
+//   1. every thread takes the smallest of its 16, and each wave sorts its 64 minima across its
+//      lanes (bitonic network of shuffles);
+//   2. T = the k-th smallest of the 256 minima: lane l < k of a wave ranks its minimum by l plus
+//      a binary search in each other wave's first k.  k minima are <= T, so the workgroup's k
+//      smallest are all <= T, and only the (at most k) threads whose minimum is <= T hold pairs
+//      <= T: at most 16k survivors;
+//   3. the survivors go to LDS (one atomic per wave) and each one's rank among them, by
+//      counting, is its output slot.
 // A round turns m candidates into k per workgroup; the last round (one workgroup) writes the
 // rows of its first nout pairs as the permutation.  Rows are distinct, so (key, row) order is
 // the stable key order and every rank is unique.
@@ -454,8 +457,27 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t ra, uint64_t kb,
   return ka < kb || (ka == kb && ra < rb);
 }
 
-// Sort key bits of rows base + j*kBlock (j < kListPer) from the key fields; rows at or past n
-// are flagged invalid through *valid.
+// Order bits of one key field for rows base + j*kBlock (j < kListPer), ORed into key[]: the
+// dtype is dispatched once, then all loads are issued before any is used (rows past the end
+// read row base instead and are dropped by the caller).
+template <typename T, typename Bits>
+__device__ __forceinline__ void or_field(const Field& F, int64_t base, uint32_t valid,
+                                         uint64_t key[kListPer], Bits bits) {
+  const T* p = (const T*)F.ptr;
+  T x[kListPer];
+#pragma unroll
+  for (int j = 0; j < kListPer; ++j) x[j] = p[((valid >> j) & 1u) ? base + (int64_t)j * kBlock : base];
+#pragma unroll
+  for (int j = 0; j < kListPer; ++j) {
+    uint64_t v = bits(x[j]);
+    if (F.desc) v = ~v;
+    if (F.bits < 64) v &= (1ull << F.bits) - 1;
+    key[j] |= v << F.shift;
+  }
+}
+
+// Sort key bits of rows base + j*kBlock (j < kListPer) from the key fields (1-D keys); rows at
+// or past n are flagged invalid through *valid.
 __device__ __forceinline__ void field_keys(const FieldPack& fp, int64_t base, int64_t n,
                                            uint64_t key[kListPer], uint32_t* valid) {
   uint32_t v = 0;
@@ -465,11 +487,35 @@ __device__ __forceinline__ void field_keys(const FieldPack& fp, int64_t base, in
     v |= (base + (int64_t)j * kBlock < n ? 1u : 0u) << j;
   }
   *valid = v;
+  if (!v) return;
   for (int f = 0; f < fp.n; ++f) {
     const Field F = fp.f[f];
-#pragma unroll
-    for (int j = 0; j < kListPer; ++j)
-      if ((v >> j) & 1u) key[j] |= field_bits(F, base + (int64_t)j * kBlock);
+    switch (F.dtype) {
+      case TQ_BOOL:
+        or_field<uint8_t>(F, base, v, key, [](uint8_t x) { return (uint64_t)(x != 0); });
+        break;
+      case TQ_I8:
+        or_field<int8_t>(F, base, v, key, [](int8_t x) { return (uint64_t)((uint8_t)x ^ 0x80u); });
+        break;
+      case TQ_I16:
+        or_field<int16_t>(F, base, v, key,
+                          [](int16_t x) { return (uint64_t)((uint16_t)x ^ 0x8000u); });
+        break;
+      case TQ_I32:
+        or_field<int32_t>(F, base, v, key,
+                          [](int32_t x) { return (uint64_t)((uint32_t)x ^ 0x80000000u); });
+        break;
+      case TQ_I64:
+        or_field<int64_t>(F, base, v, key,
+                          [](int64_t x) { return (uint64_t)x ^ 0x8000000000000000ull; });
+        break;
+      case TQ_F32:
+        or_field<float>(F, base, v, key, [](float x) { return (uint64_t)f32_order_bits(x); });
+        break;
+      default:
+        or_field<double>(F, base, v, key, [](double x) { return f64_order_bits(x); });
+        break;
+    }
   }
 }
 
@@ -482,10 +528,9 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
                                                        int k, uint64_t* __restrict__ kout,
                                                        uint32_t* __restrict__ vout,
                                                        int64_t* __restrict__ perm_out, int64_t nout) {
-  __shared__ uint64_t mk[kBlock];
-  __shared__ uint32_t mr[kBlock];
-  __shared__ uint64_t sk[kSurvMax];
-  __shared__ uint32_t sr[kSurvMax];
+  // (key, row) pairs as 16-byte LDS entries: one broadcast read per pair in the counting loops
+  __shared__ ulonglong2 mins[kBlock];
+  __shared__ ulonglong2 surv[kSurvMax];
   __shared__ uint64_t tk;
   __shared__ uint32_t tr, ns;
   const int t = threadIdx.x;
@@ -514,7 +559,8 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
       row[j] = 0xFFFFFFFFu;
     }
   }
-  // 1. the thread's minimum
+  // 1. the thread's minimum, then the wave's 64 minima sorted across lanes
+  const int l = lane_id(), w = t >> 6;
   uint64_t ak = key[0];
   uint32_t ar = row[0];
 #pragma unroll
@@ -524,43 +570,81 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
       ar = row[j];
     }
   }
-  mk[t] = ak;
-  mr[t] = ar;
+  uint64_t sk_ = ak;
+  uint32_t sr_ = ar;
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int j = size >> 1; j > 0; j >>= 1) {
+      const uint64_t ok = __shfl_xor(sk_, j, 64);
+      const uint32_t orr = __shfl_xor(sr_, j, 64);
+      const bool up = (l & size) == 0;          // this pair of the stage sorts ascending
+      const bool low = (l & j) == 0;            // this lane keeps the smaller of the pair
+      const bool other_less = pair_less(ok, orr, sk_, sr_);
+      if (other_less == (low == up)) {
+        sk_ = ok;
+        sr_ = orr;
+      }
+    }
+  }
+  mins[t] = make_ulonglong2(sk_, sr_);  // mins[w * 64 + l]: wave w's l-th smallest
   if (t == 0) {
     tk = ~0ull;
     tr = 0xFFFFFFFFu;
     ns = 0;
   }
   __syncthreads();
-  // 2. T: the minimum whose rank among the minima is k-1 (stays the sentinel when fewer than k
-  //    threads hold a real candidate)
-  if (ar != 0xFFFFFFFFu) {
-    int rank = 0;
-    for (int i = 0; i < kBlock; ++i) rank += pair_less(mk[i], mr[i], ak, ar) ? 1 : 0;
+  // 2. T (stays the sentinel when fewer than k threads hold a real candidate)
+  if (l < k && sr_ != 0xFFFFFFFFu) {
+    int rank = l;
+#pragma unroll
+    for (int o = 1; o < kWaves; ++o) {
+      const ulonglong2* L = mins + ((w + o) & (kWaves - 1)) * 64;
+      int lo = 0, hi = k;  // entries of L[0, k) less than (sk_, sr_)
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const ulonglong2 e = L[mid];
+        if (pair_less(e.x, (uint32_t)e.y, sk_, sr_)) lo = mid + 1;
+        else hi = mid;
+      }
+      rank += lo;
+    }
     if (rank == k - 1) {
-      tk = ak;
-      tr = ar;
+      tk = sk_;
+      tr = sr_;
     }
   }
   __syncthreads();
   const uint64_t Tk = tk;
   const uint32_t Tr = tr;
-  // 3. survivors <= T into LDS, then each one's rank among them
+  // 3. survivors <= T into LDS (a lane's survivors after those of lower lanes; one atomic per
+  //    wave), then each one's rank among them
+  uint32_t keep = 0;
+#pragma unroll
+  for (int j = 0; j < kListPer; ++j)
+    keep |= (row[j] != 0xFFFFFFFFu && !pair_less(Tk, Tr, key[j], row[j]) ? 1u : 0u) << j;
+  const uint32_t c = (uint32_t)__popc(keep);
+  const uint32_t inc = wave_inclusive_sum(c);
+  uint32_t wbase = 0;
+  if (l == 63 && inc) wbase = atomicAdd(&ns, inc);
+  wbase = __shfl(wbase, 63, 64);
+  uint32_t p = wbase + inc - c;
 #pragma unroll
   for (int j = 0; j < kListPer; ++j) {
-    if (row[j] != 0xFFFFFFFFu && !pair_less(Tk, Tr, key[j], row[j])) {
-      const uint32_t p = atomicAdd(&ns, 1u);
-      sk[p] = key[j];
-      sr[p] = row[j];
-    }
+    if (((keep >> j) & 1u) && p < kSurvMax) surv[p++] = make_ulonglong2(key[j], row[j]);
   }
   __syncthreads();
-  const int s = (int)ns;
+  const int s = (int)ns < kSurvMax ? (int)ns : kSurvMax;  // at most 16k by construction
   for (int i = t; i < s; i += kBlock) {
-    const uint64_t xk = sk[i];
-    const uint32_t xr = sr[i];
+    const ulonglong2 x = surv[i];
+    const uint64_t xk = x.x;
+    const uint32_t xr = (uint32_t)x.y;
     int rank = 0;
-    for (int q = 0; q < s; ++q) rank += pair_less(sk[q], sr[q], xk, xr) ? 1 : 0;
+#pragma unroll 8
+    for (int q = 0; q < s; ++q) {
+      const ulonglong2 e = surv[q];
+      rank += pair_less(e.x, (uint32_t)e.y, xk, xr) ? 1 : 0;
+    }
     if (perm_out) {
       if (rank < nout) perm_out[rank] = xr;
     } else if (rank < k) {

@@ -223,37 +223,42 @@ class JoinIndex:
         pkey = pkey.contiguous()
         np_ = pkey.shape[0]
         dev = pkey.device
+        if len(left_cols) > L.TQ_MAX_COLS or len(right_cols) > L.TQ_MAX_COLS:
+            raise NotImplementedError("join with more than 16 columns per side")
         pws = L.workspace(L.size_call("tq_join_probe_ws_bytes", np_), dev, "join_probe")
         total = ctypes.c_int64(0)
         pm = L.ptr(pmask) if pmask is not None else None
+        s = L.stream_handle()
+        # everything the write pass needs but the pair count is prepared before the count's
+        # synchronisation, so the device waits as little as possible for the write launch
+        lsrc = [c.contiguous() for c in left_cols]
+        rsrc = [c.contiguous() for c in right_cols]
+        lbytes = [L.row_bytes(c) for c in lsrc]
+        rbytes = [L.row_bytes(c) for c in rsrc]
+        lsp, rsp = L.ptr_array(lsrc), L.ptr_array(rsrc)
+        lba, rba = L.int_array(lbytes), L.int_array(rbytes)
+        # the join's boundary bytes less what the build and count passes read: every other
+        # gathered input column read once over its table, every output written once
+        seen = {self.bkey.data_ptr(), pkey.data_ptr()}
+        reads = 0
+        for c, rb in zip(lsrc + rsrc, lbytes + rbytes):
+            if c.data_ptr() not in seen:
+                seen.add(c.data_ptr())
+                reads += c.shape[0] * rb
+        per_pair = sum(lbytes) + sum(rbytes) + (16 if want_index else 0)
         L.call("tq_join_count", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype), pm,
-               np_, L.ptr(pws), ctypes.byref(total), L.stream_handle(),
+               np_, L.ptr(pws), ctypes.byref(total), s,
                alg_bytes=np_ * (pkey.element_size() + (1 if pmask is not None else 0)),
                kernel="k_join_count")
         m = int(total.value)
-        lout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in left_cols]
-        rout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in right_cols]
+        lout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in lsrc]
+        rout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in rsrc]
         bidx = torch.empty(m, dtype=torch.int64, device=dev) if want_index else None
         pidx = torch.empty(m, dtype=torch.int64, device=dev) if want_index else None
         if m > 0:
-            if len(left_cols) > L.TQ_MAX_COLS or len(right_cols) > L.TQ_MAX_COLS:
-                raise NotImplementedError("join with more than 16 columns per side")
-            lsrc = [c.contiguous() for c in left_cols]
-            rsrc = [c.contiguous() for c in right_cols]
-            # the join's boundary bytes less what the build and count passes read: every other
-            # gathered input column read once over its table, every output written once
-            seen = {self.bkey.data_ptr(), pkey.data_ptr()}
-            reads = 0
-            for c in lsrc + rsrc:
-                if c.data_ptr() not in seen:
-                    seen.add(c.data_ptr())
-                    reads += c.shape[0] * L.row_bytes(c)
-            alg = reads + m * (sum(L.row_bytes(c) for c in lsrc + rsrc) + (16 if want_index else 0))
             L.call("tq_join_write", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype),
-                   pm, np_, L.ptr(pws), len(lsrc), L.ptr_array(lsrc),
-                   L.int_array([L.row_bytes(c) for c in lsrc]), L.ptr_array(lout), len(rsrc),
-                   L.ptr_array(rsrc), L.int_array([L.row_bytes(c) for c in rsrc]),
-                   L.ptr_array(rout), L.ptr(bidx), L.ptr(pidx), L.stream_handle(), alg_bytes=alg,
+                   pm, np_, L.ptr(pws), len(lsrc), lsp, lba, L.ptr_array(lout), len(rsrc), rsp, rba,
+                   L.ptr_array(rout), L.ptr(bidx), L.ptr(pidx), s, alg_bytes=reads + m * per_pair,
                    kernel="k_join_write")
         return lout, rout, bidx, pidx
 
@@ -358,11 +363,7 @@ class GroupBy:
         alg = nsel * b.load_bytes() + (self.n if self.mask is not None else 0)
         L.call("tq_groupby_accumulate", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr,
                self.n, mode, slots, L.ptr(ws), wsb.value, s, alg_bytes=alg, kernel="k_gb_accum")
-        ng = ctypes.c_int64()
-        L.call("tq_groupby_count", ctypes.byref(d), mode, slots, L.ptr(ws), ctypes.byref(ng), s)
-        G = int(ng.value)
-        key_out = [torch.empty(G, dtype=k.dtype, device=dev) for k in self.keys]
-        agg_out = []
+        out_dtypes = []   # decided before the count's synchronisation
         for op, dt in zip(self.ops, self.arg_dtypes):
             if op == L.TQ_AGG_COUNT:
                 odt = torch.int64
@@ -372,7 +373,12 @@ class GroupBy:
                 odt = dt if dt.is_floating_point else torch.int64
             else:
                 odt = dt
-            agg_out.append(torch.empty(G, dtype=odt, device=dev))
+            out_dtypes.append(odt)
+        ng = ctypes.c_int64()
+        L.call("tq_groupby_count", ctypes.byref(d), mode, slots, L.ptr(ws), ctypes.byref(ng), s)
+        G = int(ng.value)
+        key_out = [torch.empty(G, dtype=k.dtype, device=dev) for k in self.keys]
+        agg_out = [torch.empty(G, dtype=odt, device=dev) for odt in out_dtypes]
         if G > 0:
             L.call("tq_groupby_write", ctypes.byref(d), mode, slots, L.ptr(ws), G,
                    L.ptr_array(key_out), L.ptr_array(agg_out), s)

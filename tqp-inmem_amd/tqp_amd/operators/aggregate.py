@@ -96,13 +96,21 @@ def _zero_row_aggregates(keys, calls, gb):
     return out
 
 
-def agg_group(columns, key_ids, results, mask=None):
-    """Group `columns` by `key_ids` and evaluate each result tree (aggregate.py:282-364).
-    Returns {output id: tensor}."""
-    keys = [columns[k].tensor for k in key_ids]
-    n = next(iter(columns.values())).tensor.shape[0]
-    calls = []               # aggregate call nodes in order
-    plans = []               # per result: ('key', key_id) or ('agg', tree)
+# Parsed result lists, keyed by their text: the plan arguments of an operator are plain nested
+# lists that the executor hands over afresh for every query, so the parse is reused by value.
+_PARSED = {}
+
+
+def _parse_results(results):
+    """(calls, plans) of a result list: the aggregate call nodes in evaluation order, and per
+    result ('key', key id) or ('agg', tree).  The returned trees are a cached parse, shared by
+    every call with equal text."""
+    key = repr(results)
+    hit = _PARSED.get(key)
+    if hit is not None:
+        return hit
+    calls = []
+    plans = []
     for tree in results:
         tree = _unwrap(tree)
         out_id = get_id(right_most(tree))
@@ -115,6 +123,18 @@ def agg_group(columns, key_ids, results, mask=None):
                 raise ValueError(f"aggregate result without an aggregate function: {tree}")
             calls.extend(found)
             plans.append((out_id, 'agg', body))
+    if len(_PARSED) > 256:
+        _PARSED.clear()
+    _PARSED[key] = (calls, plans)
+    return calls, plans
+
+
+def agg_group(columns, key_ids, results, mask=None):
+    """Group `columns` by `key_ids` and evaluate each result tree (aggregate.py:282-364).
+    Returns {output id: tensor}."""
+    keys = [columns[k].tensor for k in key_ids]
+    n = next(iter(columns.values())).tensor.shape[0]
+    calls, plans = _parse_results(results)
     gb = kernels.GroupBy(keys, [(c[0], c[1]) for c in calls], columns, mask=mask, n=n)
     key_out, agg_out = gb.run()
     if gb.nsel == 0 and calls:
@@ -147,6 +167,9 @@ def evaluate(device_name, tensors, tree, cnt_rows, cnt_groups, inverted_index):
     raise NotImplementedError("use agg_group: aggregates are reduced in one fused launch")
 
 
+_STATIC = {}
+
+
 def tqp_hash_aggregate(gpu_enable, tensor_group, args, gpu_mem_pool, cpu_mem_pool,
                        name="aggregation"):
     perf_logger().start(f"{name}")
@@ -167,17 +190,26 @@ def tqp_hash_aggregate(gpu_enable, tensor_group, args, gpu_mem_pool, cpu_mem_poo
                 if v not in transfer_set:
                     transfer_set.append(v)
 
-    key_ids, key_ids_new, key_types, key_types_name = [], [], [], []
-    agg_types = []
-    for tree in args['results']:
-        tree = _unwrap(tree)
-        if _is_key(tree):
-            key_ids.append(get_id(tree if isinstance(tree, str) else tree[0]))
-            key_ids_new.append(get_id(tree if isinstance(tree, str) else tree[-1]))
-            key_types.extend(tensor_group[key_ids[-1]].get_type_torch())
-            key_types_name.append(tensor_group[key_ids[-1]].tensor_type)
-        else:
-            agg_types.append(get_result_type(tree, transfer_set, transfer_types))
+    # the result list's keys and declared dtypes depend on its text and the input column types
+    ck = (repr(args['results']), tuple(transfer_set), tuple(transfer_types),
+          tuple(tensor_group[i].tensor_type for i in transfer_set))
+    static = _STATIC.get(ck)
+    if static is None:
+        key_ids, key_ids_new, key_types, key_types_name = [], [], [], []
+        agg_types = []
+        for tree in args['results']:
+            tree = _unwrap(tree)
+            if _is_key(tree):
+                key_ids.append(get_id(tree if isinstance(tree, str) else tree[0]))
+                key_ids_new.append(get_id(tree if isinstance(tree, str) else tree[-1]))
+                key_types.extend(tensor_group[key_ids[-1]].get_type_torch())
+                key_types_name.append(tensor_group[key_ids[-1]].tensor_type)
+            else:
+                agg_types.append(get_result_type(tree, transfer_set, transfer_types))
+        if len(_STATIC) > 256:
+            _STATIC.clear()
+        static = _STATIC[ck] = (key_ids, key_ids_new, key_types, key_types_name, agg_types)
+    key_ids, key_ids_new, key_types, key_types_name, agg_types = static
     agg_ids = args['out agg ids']
 
     def agg_wrapper(columns):

@@ -28,6 +28,7 @@ from ..variable import Variable, VariableState
 
 LEFT = 0
 RIGHT = 1
+_NO_GLOBAL_INDEX = torch.tensor([])   # join_vortex's unused global-index arguments
 
 
 def _side_mask(tensors):
@@ -204,7 +205,7 @@ def join_vortex(gpu_enable, tensor_group, args, leftcol, rightcol, condition_lis
                     tensor_inputs[side][k] = columns[k]
         return join_kernel_new(True, tensor_inputs[LEFT], tensor_inputs[RIGHT],
                                join_key_index[LEFT], join_key_index[RIGHT], condition_list,
-                               join_output, False, type, torch.tensor([]), torch.tensor([]))
+                               join_output, False, type, _NO_GLOBAL_INDEX, _NO_GLOBAL_INDEX)
 
     pipe = InMemoryPipeline(
         input_columns=join_input,
