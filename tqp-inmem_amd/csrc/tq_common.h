@@ -147,42 +147,120 @@ __device__ __forceinline__ int64_t load_int(const void* p, int dtype, int64_t i)
   }
 }
 
-// Rows r..r+3 of an integer column (r a multiple of 4; rows at or past n read as 0).  One
-// vector load when the column is aligned for it and all four rows exist.
-__device__ __forceinline__ void load_int4(const void* p, int dtype, int64_t r, int64_t n,
-                                          int64_t v[4]) {
-  if (r + 3 < n) {
-    const uintptr_t a = (uintptr_t)p;
-    if (dtype == TQ_I32 && (a & 15) == 0) {
-      const int4 x = *(const int4*)((const int32_t*)p + r);
-      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-      return;
-    }
-    if (dtype == TQ_I64 && (a & 15) == 0) {
-      const longlong2 x = *(const longlong2*)((const int64_t*)p + r);
-      const longlong2 y = *(const longlong2*)((const int64_t*)p + r + 2);
-      v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
-      return;
+// Selection of rows base + j*kBlock + threadIdx.x (j < N): the row exists and, with a mask,
+// its byte is set.  The mask bytes are loaded unconditionally (a row past n reads row 0, so
+// n > 0 is required), so all N loads are in flight together rather than one per branch.
+template <int N>
+__device__ __forceinline__ void select_rows(const uint8_t* mask, int64_t base, int64_t n,
+                                            bool (&live)[N]) {
+  uint8_t m[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) m[j] = 1;
+  if (mask) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const int64_t r = base + (int64_t)j * blockDim.x + threadIdx.x;
+      m[j] = mask[r < n ? r : 0];
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = (r + i < n) ? load_int(p, dtype, r + i) : 0;
+  for (int j = 0; j < N; ++j)
+    live[j] = base + (int64_t)j * blockDim.x + threadIdx.x < n && m[j] != 0;
 }
 
-// Selection bits of rows r..r+3 (bit i: row r+i exists and, with a mask, its byte is nonzero).
-__device__ __forceinline__ uint32_t load_sel4(const uint8_t* mask, int64_t r, int64_t n) {
-  const uint32_t exist = r + 3 < n ? 15u : (r >= n ? 0u : ((1u << (n - r)) - 1u));
-  if (!mask) return exist;
-  uint32_t b = 0;
-  if (exist == 15u && ((uintptr_t)(mask + r) & 3) == 0) {
-    const uint32_t w = *(const uint32_t*)(mask + r);
+// ------------------------------------------------------------------ tile row loads
+// A thread's 16 rows of a 4096-row tile: item j is row tile0 + (j/4)*1024 + 4*threadIdx.x +
+// j%4 (four runs of four consecutive rows).  The loaders issue every load before using any:
+// a full tile with aligned columns takes one vector load per run, any other tile clamped
+// scalar loads (rows past n read row 0, so n > 0 is required).  Branches depend only on the
+// tile, the pointer and the dtype, so they are uniform across the workgroup.
+__device__ __forceinline__ int64_t run_row(int64_t tile0, int j) {
+  return tile0 + (int64_t)(j >> 2) * (4 * kBlock) + 4 * threadIdx.x + (j & 3);
+}
+
+template <typename T, typename Conv>
+__device__ __forceinline__ void load_rows16(const T* p, int64_t tile0, int64_t n,
+                                            uint64_t (&v)[kItems], Conv conv) {
+  T x[kItems];
+  const bool full = tile0 + kTileRows <= n;
+  const bool vec = (sizeof(T) <= 8) && ((uintptr_t)p % (sizeof(T) * 4 < 16 ? sizeof(T) * 4 : 16)) == 0;
+  if (full && vec) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) b |= (((w >> (8 * i)) & 0xFFu) != 0u) << i;
-    return b;
+    for (int q = 0; q < 4; ++q) {
+      const T* r = p + run_row(tile0, 4 * q);
+      if constexpr (sizeof(T) == 8) {
+        const longlong2 a = *(const longlong2*)r, b = *(const longlong2*)(r + 2);
+        x[4 * q] = __builtin_bit_cast(T, a.x); x[4 * q + 1] = __builtin_bit_cast(T, a.y);
+        x[4 * q + 2] = __builtin_bit_cast(T, b.x); x[4 * q + 3] = __builtin_bit_cast(T, b.y);
+      } else if constexpr (sizeof(T) == 4) {
+        const uint4 a = *(const uint4*)r;
+        x[4 * q] = __builtin_bit_cast(T, a.x); x[4 * q + 1] = __builtin_bit_cast(T, a.y);
+        x[4 * q + 2] = __builtin_bit_cast(T, a.z); x[4 * q + 3] = __builtin_bit_cast(T, a.w);
+      } else {
+        // 1- and 2-byte values: one 4- or 8-byte load of the run
+        T tmp[4];
+        if constexpr (sizeof(T) == 2) {
+          const uint2 a = *(const uint2*)r;
+          __builtin_memcpy(tmp, &a, 8);
+        } else {
+          const uint32_t a = *(const uint32_t*)r;
+          __builtin_memcpy(tmp, &a, 4);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[4 * q + i] = tmp[i];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int64_t r = run_row(tile0, j);
+      x[j] = p[r < n ? r : 0];
+    }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) b |= ((exist >> i) & 1u) && mask[r + i] != 0 ? (1u << i) : 0u;
-  return b;
+  for (int j = 0; j < kItems; ++j) v[j] = run_row(tile0, j) < n ? conv(x[j]) : 0ull;
+}
+
+// Column values of the thread's 16 tile rows as the expression programs hold them: integers
+// sign-extended, bool as 0/1, f64 as its bits, f32 as its bits in the low half; rows past n
+// read as 0.
+__device__ __forceinline__ void load_raw16(const void* p, int dt, int64_t tile0, int64_t n,
+                                           uint64_t (&v)[kItems]) {
+  switch (dt) {
+    case TQ_BOOL:
+      load_rows16((const uint8_t*)p, tile0, n, v, [](uint8_t x) { return (uint64_t)(x != 0); });
+      break;
+    case TQ_I8:
+      load_rows16((const int8_t*)p, tile0, n, v, [](int8_t x) { return (uint64_t)(int64_t)x; });
+      break;
+    case TQ_I16:
+      load_rows16((const int16_t*)p, tile0, n, v, [](int16_t x) { return (uint64_t)(int64_t)x; });
+      break;
+    case TQ_I32:
+      load_rows16((const int32_t*)p, tile0, n, v, [](int32_t x) { return (uint64_t)(int64_t)x; });
+      break;
+    case TQ_F32:
+      load_rows16((const uint32_t*)p, tile0, n, v, [](uint32_t x) { return (uint64_t)x; });
+      break;
+    default:
+      load_rows16((const uint64_t*)p, tile0, n, v, [](uint64_t x) { return x; });
+      break;
+  }
+}
+
+// Selection bits of the thread's 16 tile rows (bit j: the row exists and, with a mask, its
+// byte is set).
+__device__ __forceinline__ uint32_t load_sel16(const uint8_t* mask, int64_t tile0, int64_t n) {
+  uint32_t exist = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) exist |= (run_row(tile0, j) < n ? 1u : 0u) << j;
+  if (!mask) return exist;
+  uint64_t m[kItems];
+  load_rows16(mask, tile0, n, m, [](uint8_t x) { return (uint64_t)(x != 0); });
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) s |= (m[j] ? 1u : 0u) << j;
+  return s & exist;
 }
 
 inline int dtype_bytes(int dtype) {

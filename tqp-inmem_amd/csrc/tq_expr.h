@@ -61,13 +61,17 @@ __device__ __forceinline__ void stage_loads(const tq_program_t& P, const tq_cols
     const void* col = col_ptr(C, P.load_col[s]);
     const int dt = P.load_dtype[s];
     uint64_t v[kSub];
+// Rows that are not live read row 0 of the column (which exists whenever a sub-tile does) and
+// are zeroed afterwards: unconditional loads are all in flight together.
 #define TQ_STAGE(T, EXPR)                                          \
   {                                                                \
     const T* p = (const T*)col;                                    \
     _Pragma("unroll") for (int j = 0; j < kSub; ++j) {             \
-      const int64_t r = sub0 + (int64_t)j * kBlock + t;            \
-      v[j] = live[j] ? (EXPR) : 0;                                 \
+      const int64_t r = live[j] ? sub0 + (int64_t)j * kBlock + t : 0; \
+      v[j] = (EXPR);                                               \
     }                                                              \
+    _Pragma("unroll") for (int j = 0; j < kSub; ++j)               \
+      if (!live[j]) v[j] = 0;                                      \
   }
     switch (dt) {
       case TQ_BOOL: TQ_STAGE(uint8_t, (uint64_t)(p[r] != 0)) break;

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(tq_program_t P, tq_colset_t C
 // Most predicates are conjunctions of column-versus-constant comparisons (isnotnull compiles to
 // a constant true).  tq_filter_mask recognises such a program by following its registers and
 // runs k_filter_conj instead of the interpreter: per load slot, a thread reads its 16 rows of a
-// 4096-row tile (four runs of four consecutive rows, vector loads) and tests them against
+// 4096-row tile (four runs of four consecutive rows, all loads in flight together) and tests them against
 // every term on that slot; the mask bytes of a run go out as one 32-bit store.
 constexpr int kMaxTerms = 8;
 
@@ -89,49 +89,6 @@ struct Conj {
   int32_t term_op[kMaxTerms];
   int64_t term_imm[kMaxTerms];
 };
-
-// Rows r..r+3 of a column as the interpreter's load slot holds them: integers sign-extended,
-// bool as 0/1, f64 as its bits, f32 as its bits in the low half.
-__device__ __forceinline__ void load_raw4(const void* p, int dt, int64_t r, int64_t n,
-                                          uint64_t v[4]) {
-  const bool full = r + 3 < n;
-  if (dt == TQ_F64 || dt == TQ_I64) {
-    int64_t w[4];
-    load_int4(p, TQ_I64, r, n, w);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = (uint64_t)w[i];
-    return;
-  }
-  if (dt == TQ_F32) {
-    if (full && ((uintptr_t)p & 15) == 0) {
-      const uint4 x = *(const uint4*)((const uint32_t*)p + r);
-      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = (r + i < n) ? ((const uint32_t*)p)[r + i] : 0u;
-    }
-    return;
-  }
-  if (dt == TQ_BOOL || dt == TQ_I8) {
-    int8_t b[4] = {0, 0, 0, 0};
-    if (full && ((uintptr_t)((const int8_t*)p + r) & 3) == 0) {
-      const uint32_t w = *(const uint32_t*)((const int8_t*)p + r);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = (int8_t)(w >> (8 * i));
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = (r + i < n) ? ((const int8_t*)p)[r + i] : 0;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      v[i] = dt == TQ_BOOL ? (uint64_t)(b[i] != 0) : (uint64_t)(int64_t)b[i];
-    return;
-  }
-  int64_t w[4];
-  load_int4(p, dt, r, n, w);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = (uint64_t)w[i];
-}
 
 template <typename T>
 __device__ __forceinline__ uint32_t cmp_bits(int op, const T (&x)[kItems], T c) {
@@ -158,19 +115,10 @@ __global__ __launch_bounds__(kBlock) void k_filter_conj(Conj C, int64_t n,
   __shared__ uint32_t sh[kWaves];
   const int64_t tile = blockIdx.x;
   const int t = threadIdx.x;
-  uint32_t acc = 0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int64_t r = tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * t;
-    acc |= load_sel4(nullptr, r, n) << (4 * q);
-  }
+  uint32_t acc = load_sel16(nullptr, tile * kTileRows, n);
   for (int s = 0; s < C.nslot; ++s) {
     uint64_t v[kItems];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t r = tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * t;
-      load_raw4(C.slot_col[s], C.slot_dtype[s], r, n, &v[4 * q]);
-    }
+    load_raw16(C.slot_col[s], C.slot_dtype[s], tile * kTileRows, n, v);
     for (int k = 0; k < C.nterm; ++k) {
       if (C.term_slot[k] != s) continue;
       const int op = C.term_op[k];
@@ -344,13 +292,11 @@ __global__ __launch_bounds__(kBlock) void k_mask_counts(const uint8_t* __restric
                                                         int64_t n,
                                                         uint32_t* __restrict__ tile_counts) {
   __shared__ uint32_t sh[kWaves];
-  const int64_t base = (int64_t)blockIdx.x * kTileRows + threadIdx.x;
+  bool f[kItems];
+  select_rows(mask, (int64_t)blockIdx.x * kTileRows, n, f);
   uint32_t cnt = 0;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const int64_t r = base + (int64_t)j * kBlock;
-    if (r < n) cnt += mask[r] != 0;
-  }
+  for (int j = 0; j < kItems; ++j) cnt += f[j] ? 1u : 0u;
   cnt = wave_sum(cnt);
   if (lane_id() == 0) sh[threadIdx.x >> 6] = cnt;
   __syncthreads();
@@ -491,10 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_compact(const uint8_t* __restrict__ 
   const int64_t tile0 = (int64_t)blockIdx.x * kTileRows;
   bool f[kItems];
   uint64_t bal[kItems];
+  select_rows(mask, tile0, n, f);
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
-    const int64_t r = tile0 + (int64_t)j * kBlock + t;
-    f[j] = (r < n) && mask[r] != 0;
     bal[j] = ballot(f[j]);
     if (l == 0) wcnt[j][w] = (uint32_t)__popcll(bal[j]);
   }

@@ -54,10 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_bounds(KeyCols K, const uint8_t* 
   __shared__ int64_t sh[kBoundWords][kWaves];
   const int64_t tile = blockIdx.x;
   const int t = threadIdx.x;
-  uint32_t sel = 0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    sel |= load_sel4(mask, tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * t, n) << (4 * q);
+  const uint32_t sel = load_sel16(mask, tile * kTileRows, n);
   int64_t cnt = __popc(sel);
   int64_t lo[TQ_GB_MAX_KEYS], hi[TQ_GB_MAX_KEYS];
 #pragma unroll
@@ -65,16 +62,14 @@ __global__ __launch_bounds__(kBlock) void k_gb_bounds(KeyCols K, const uint8_t* 
     lo[k] = INT64_MAX;
     hi[k] = INT64_MIN;
     if (k >= K.nkeys) continue;
-    int64_t v[kItems];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      load_int4(K.ptr[k], K.dtype[k], tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * t, n,
-                &v[4 * q]);
+    uint64_t raw[kItems];
+    load_raw16(K.ptr[k], K.dtype[k], tile * kTileRows, n, raw);
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
+      const int64_t vj = (int64_t)raw[j];
       if ((sel >> j) & 1u) {
-        lo[k] = v[j] < lo[k] ? v[j] : lo[k];
-        hi[k] = v[j] > hi[k] ? v[j] : hi[k];
+        lo[k] = vj < lo[k] ? vj : lo[k];
+        hi[k] = vj > hi[k] ? vj : hi[k];
       }
     }
   }
@@ -201,11 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
   for (int64_t sub = blockIdx.x; sub < nsub; sub += gridDim.x) {
     const int64_t sub0 = sub * kSubRows;
     bool live[kSub];
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      const int64_t r = sub0 + (int64_t)j * kBlock + t;
-      live[j] = r < n && (!mask || mask[r] != 0);
-    }
+    select_rows(mask, sub0, n, live);
     stage_loads(P, C, sub0, live, stage);
     // the table slot of every live row: LDS (dense-LDS tables and the hash front) or global
     uint64_t* abase[kSub];
@@ -433,11 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
   for (int sub = 0; sub < tsub; ++sub) {
     const int64_t sub0 = (tile * tsub + sub) * kSubRows;
     bool live[kSub];
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      const int64_t r = sub0 + (int64_t)j * kBlock + t;
-      live[j] = r < n && (!mask || mask[r] != 0);
-    }
+    select_rows(mask, sub0, n, live);
     uint64_t pk[kSub];
     packed_keys4(K, G, sub0, live, pk);
 #pragma unroll
@@ -468,10 +455,8 @@ __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
     bool live[kSub];
     int64_t rows[kSub];
 #pragma unroll
-    for (int j = 0; j < kSub; ++j) {
-      rows[j] = sub0 + (int64_t)j * kBlock + t;
-      live[j] = rows[j] < n && (!mask || mask[rows[j]] != 0);
-    }
+    for (int j = 0; j < kSub; ++j) rows[j] = sub0 + (int64_t)j * kBlock + t;
+    select_rows(mask, sub0, n, live);
     stage_loads(P, C, sub0, live, stage);
     uint32_t pos[kSub];
 #pragma unroll

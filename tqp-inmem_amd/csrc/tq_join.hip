@@ -101,14 +101,11 @@ __device__ __forceinline__ uint32_t lookup(const JoinView& J, K t, bool valid, u
 __device__ __forceinline__ void load_tile_rows(const void* keys, int dt, const uint8_t* mask,
                                                int64_t n, int64_t tile, uint32_t* sel,
                                                int64_t v[kItems]) {
-  uint32_t s = 0;
+  uint64_t raw[kItems];
+  load_raw16(keys, dt, tile * kTileRows, n, raw);
+  *sel = load_sel16(mask, tile * kTileRows, n);
 #pragma unroll
-  for (int q = 0; q < kRuns; ++q) {
-    const int64_t r = tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * threadIdx.x;
-    s |= load_sel4(mask, r, n) << (4 * q);
-    load_int4(keys, dt, r, n, &v[4 * q]);
-  }
-  *sel = s;
+  for (int j = 0; j < kItems; ++j) v[j] = (int64_t)raw[j];
 }
 
 __device__ __forceinline__ int64_t tile_row(int64_t tile, int j) {

@@ -31,10 +31,16 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ key
   for (int i = t; i < kWaves * kBins; i += kBlock) (&cnt[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+  K k[kSortItems];  // every load in flight before the first atomic (rows past n read row 0)
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const int64_t e = tile0 + (int64_t)j * kBlock + t;
-    if (e < n) atomicAdd(&cnt[w][(uint32_t)(keys[e] >> shift) & (kBins - 1)], 1u);
+    k[j] = keys[e < n ? e : 0];
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = tile0 + (int64_t)j * kBlock + t;
+    if (e < n) atomicAdd(&cnt[w][(uint32_t)(k[j] >> shift) & (kBins - 1)], 1u);
   }
   __syncthreads();
   uint32_t s = 0;
@@ -63,11 +69,19 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
   K key[kSortItems];
   uint32_t val[kSortItems], dig[kSortItems], rnk[kSortItems];
 #pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {  // every load in flight first (rows past n read row 0)
+    const int64_t e = wave0 + (int64_t)j * 64 + l;
+    key[j] = kin[e < n ? e : 0];
+    val[j] = vin[e < n ? e : 0];
+  }
+#pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const int64_t e = wave0 + (int64_t)j * 64 + l;
     const bool valid = e < n;
-    key[j] = valid ? kin[e] : (K)0;
-    val[j] = valid ? vin[e] : 0u;
+    if (!valid) {
+      key[j] = (K)0;
+      val[j] = 0u;
+    }
     const uint32_t d = (uint32_t)(key[j] >> shift) & (kBins - 1);
     uint64_t peers = ballot(valid);
 #pragma unroll
