@@ -313,6 +313,8 @@ int tq_join_write(const tq_join_state_t* state /* host */, const void* pkeys, in
                                 by partition, one workgroup reduces each partition in LDS and
                                 keeps its groups in key order (no global atomics)             */
 #define TQ_GB_LDS_FRONT 8    /* flag: an LDS hash table absorbs repeated keys first         */
+#define TQ_GB_LDS_LANES 16   /* flag (DENSE_LDS): one table copy per lane, so rows of a few hot
+                              * groups never meet on an LDS address                         */
 
 typedef struct {
   int32_t nkeys;

@@ -12,6 +12,7 @@ enum AccKind { ACC_FSUM = 0, ACC_ISUM, ACC_COUNT, ACC_FMIN, ACC_FMAX, ACC_IMIN, 
 
 constexpr unsigned long long kEmpty = ~0ull;
 constexpr int kLdsBudget = 64 * 1024;
+constexpr int kLdsBig = 152 * 1024;  // gfx950 gives a workgroup up to 160 KB of LDS on request
 
 struct GbSpec {
   int32_t nkeys, naggs;
@@ -43,6 +44,19 @@ __device__ __forceinline__ uint64_t to_acc(int kind, int dt, uint64_t v) {
     case ACC_FSUM: case ACC_FMIN: case ACC_FMAX: return f64_bits(isf ? d : (double)(int64_t)v);
     case ACC_COUNT: return isf ? (uint64_t)(d != -64.0) : (uint64_t)((int64_t)v != -64);
     default: return v;  // integer domain
+  }
+}
+
+// Two accumulators of one kind combined (no atomics).
+__device__ __forceinline__ uint64_t acc_combine(int kind, uint64_t a, uint64_t b) {
+  switch (kind) {
+    case ACC_FSUM: return f64_bits(as_f64(a) + as_f64(b));
+    case ACC_ISUM:
+    case ACC_COUNT: return a + b;
+    case ACC_FMIN: return as_f64(b) < as_f64(a) ? b : a;
+    case ACC_FMAX: return as_f64(b) > as_f64(a) ? b : a;
+    case ACC_IMIN: return (int64_t)b < (int64_t)a ? b : a;
+    default: return (int64_t)b > (int64_t)a ? b : a;
   }
 }
 

@@ -170,10 +170,17 @@ __device__ __forceinline__ uint64_t pack_key(const GbSpec& G, const uint64_t* st
   return p;
 }
 
-template <int BASE, bool FRONT>
+// BASE == DENSE_LDS: the workgroup's table is the whole key space in LDS (with LANES one copy
+// per lane: entry s of lane l at s * 64 + l, so the rows of a few hot groups never meet on an
+// address); at the end the workgroup writes its table to part[blockIdx.x] and k_gb_merge
+// combines the partials, so no global atomics are contended.  FRONT: an LDS hash table of cap
+// entries (filled to 3/4) absorbs repeated keys; rows whose key finds no place go to the
+// global table, and the LDS entries are added into it at the end.
+template <int BASE, bool FRONT, bool LANES>
 __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, tq_colset_t C,
                                                      const uint8_t* __restrict__ mask, int64_t n,
-                                                     GbTable T, int32_t cap, int32_t stage_words) {
+                                                     GbTable T, int32_t cap, int32_t stage_words,
+                                                     uint64_t* __restrict__ part) {
   extern __shared__ uint64_t smem[];
   uint64_t* stage = smem;
   uint64_t* lacc = smem + stage_words;           // naggs * cap accumulators
@@ -182,6 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
   __shared__ uint32_t fill;
   const bool use_lds = BASE == TQ_GB_DENSE_LDS || FRONT;
   const int t = threadIdx.x;
+  const int lane = lane_id();
   if (use_lds) {
     for (int s = t; s < cap; s += kBlock) {
       lcnt[s] = 0;
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
     __syncthreads();
   }
   const int64_t nsub = ceil_div(n, kSubRows);
-  const uint32_t limit = (uint32_t)cap / 2;
+  const uint32_t limit = (uint32_t)cap / 4 * 3;
   for (int64_t sub = blockIdx.x; sub < nsub; sub += gridDim.x) {
     const int64_t sub0 = sub * kSubRows;
     bool live[kSub];
@@ -211,8 +219,8 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
       if (BASE == TQ_GB_DENSE_LDS) {
         abase[j] = lacc;
         astride[j] = cap;
-        aslot[j] = (int64_t)p;
-        atomicAdd(&lcnt[p], 1u);
+        aslot[j] = LANES ? (int64_t)p * 64 + lane : (int64_t)p;
+        atomicAdd(&lcnt[aslot[j]], 1u);
         continue;
       }
       if (FRONT) {
@@ -260,17 +268,60 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
                     to_acc(G.kind[a], G.arg_dtype[a], v[j]));
     }
   }
-  if (use_lds) {
+  if (BASE == TQ_GB_DENSE_LDS) {
+    // this workgroup's table -> part[blockIdx.x][slot][count, aggregates...]
+    __syncthreads();
+    const int64_t S = LANES ? cap / 64 : cap;
+    const int nf = 1 + G.naggs;
+    uint64_t* mine = part + (int64_t)blockIdx.x * S * nf;
+    if (LANES) {
+      // wave w combines the 64 lane copies of slots w, w + 4, ...
+      for (int64_t s = t >> 6; s < S; s += kWaves) {
+        uint64_t c = lcnt[s * 64 + lane];
+        c = wave_sum(c);
+        if (lane == 0) mine[s * nf] = c;
+        for (int a = 0; a < G.naggs; ++a) {
+          uint64_t v = lacc[(int64_t)a * cap + s * 64 + lane];
+          const int kind = G.kind[a];
+#pragma unroll
+          for (int d = 32; d >= 1; d >>= 1) v = acc_combine(kind, v, __shfl_xor(v, d, 64));
+          if (lane == 0) mine[s * nf + 1 + a] = v;
+        }
+      }
+    } else {
+      for (int64_t s = t; s < S; s += kBlock) {
+        mine[s * nf] = lcnt[s];
+        for (int a = 0; a < G.naggs; ++a) mine[s * nf + 1 + a] = lacc[(int64_t)a * cap + s];
+      }
+    }
+  } else if (FRONT) {
     __syncthreads();
     for (int s = t; s < cap; s += kBlock) {
       const uint32_t c = lcnt[s];
       if (c == 0) continue;
-      const uint64_t p = FRONT ? (uint64_t)lkey[s] : (uint64_t)s;
+      const uint64_t p = (uint64_t)lkey[s];
       const int64_t gs = global_slot(BASE, T, p);
       atomicAdd(&T.cnt[gs], c);
       for (int a = 0; a < G.naggs; ++a)
         acc_apply(&T.acc[(int64_t)a * T.slots + gs], G.kind[a], lacc[(int64_t)a * cap + s]);
     }
+  }
+}
+
+// Dense-LDS partials of nblocks workgroups combined into the table (count, then aggregates).
+__global__ __launch_bounds__(kBlock) void k_gb_merge(GbSpec G, const uint64_t* __restrict__ part,
+                                                     int32_t nblocks, GbTable T) {
+  const int nf = 1 + G.naggs;
+  const int64_t S = T.slots, total = S * nf;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += stride) {
+    const int64_t s = e / nf;
+    const int f = (int)(e - s * nf);
+    const int kind = f == 0 ? ACC_COUNT : G.kind[f - 1];
+    uint64_t v = f == 0 ? 0ull : G.init[f - 1];
+    for (int b = 0; b < nblocks; ++b) v = acc_combine(kind, v, part[(int64_t)b * total + e]);
+    if (f == 0) T.cnt[s] = (uint32_t)v;
+    else T.acc[(int64_t)(f - 1) * S + s] = v;
   }
 }
 
@@ -436,12 +487,13 @@ __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
 }
 
 // Scatter a tile's selected rows: each takes the next position of its partition's range and
-// stores its slot within the partition and its aggregate arguments (accumulator domain).
+// stores one record there, {slot within the partition, aggregate arguments (accumulator
+// domain)...} in 1 + naggs words (one 16-byte store for a single aggregate), so a row costs
+// one scattered write rather than one per field.
 __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
     GbSpec G, tq_program_t P, tq_colset_t C, const uint8_t* __restrict__ mask, int64_t n, int pb,
     int nparts, int tsub, int64_t ptiles, const uint32_t* __restrict__ hoff,
-    uint32_t* __restrict__ rec_key, uint64_t* __restrict__ rec_val, int64_t cap,
-    int32_t stage_words) {
+    uint64_t* __restrict__ rec, int32_t stage_words) {
   extern __shared__ uint64_t smem[];
   uint64_t* stage = smem;
   uint32_t* lh = (uint32_t*)(smem + stage_words);
@@ -459,14 +511,17 @@ __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
     select_rows(mask, sub0, n, live);
     stage_loads(P, C, sub0, live, stage);
     uint32_t pos[kSub];
+    uint64_t slot[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) {
       pos[j] = 0;
+      slot[j] = 0;
       if (!live[j]) continue;
       const uint64_t pk = pack_key(G, stage, j);
       pos[j] = atomicAdd(&lh[pk >> pb], 1u);
-      rec_key[pos[j]] = (uint32_t)(pk & pmask);
+      slot[j] = pk & pmask;
     }
+    const int rw = 1 + G.naggs;
     RegsN R{};
     int begin = 0;
     for (int a = 0; a < G.naggs; ++a) {
@@ -475,8 +530,19 @@ __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
       begin = end;
       const V4 V = R.get(P.seg_result[a]);
 #pragma unroll
+      for (int j = 0; j < kSub; ++j) {
+        if (!live[j]) continue;
+        const uint64_t v = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+        if (G.naggs == 1)
+          *(ulonglong2*)(rec + (int64_t)pos[j] * 2) = make_ulonglong2(slot[j], v);
+        else
+          rec[(int64_t)pos[j] * rw + 1 + a] = v;
+      }
+    }
+    if (G.naggs != 1) {
+#pragma unroll
       for (int j = 0; j < kSub; ++j)
-        if (live[j]) rec_val[(int64_t)a * cap + pos[j]] = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+        if (live[j]) rec[(int64_t)pos[j] * rw] = slot[j];
     }
   }
 }
@@ -485,9 +551,9 @@ __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
 // in slot order, at [b << pb, ...) of gslot/gcnt/gacc and their number in grp_cnt[b].
 __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
     GbSpec G, int pb, int nparts, int64_t ptiles, const uint32_t* __restrict__ hoff,
-    const uint32_t* __restrict__ htotal, const uint32_t* __restrict__ rec_key,
-    const uint64_t* __restrict__ rec_val, int64_t cap, uint32_t* __restrict__ grp_cnt,
-    uint32_t* __restrict__ gslot, uint32_t* __restrict__ gcnt, uint64_t* __restrict__ gacc) {
+    const uint32_t* __restrict__ htotal, const uint64_t* __restrict__ rec,
+    uint32_t* __restrict__ grp_cnt, uint32_t* __restrict__ gslot, uint32_t* __restrict__ gcnt,
+    uint64_t* __restrict__ gacc) {
   extern __shared__ uint64_t smem[];
   __shared__ uint32_t wocc[kItems][kWaves];
   const int ps = 1 << pb;
@@ -503,23 +569,41 @@ __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
   const uint32_t r0 = hoff[b * ptiles];
   const uint32_t r1 = b + 1 < nparts ? hoff[(b + 1) * ptiles] : *htotal;
   constexpr int U = 8;  // records per thread whose loads are issued together
+  const int rw = 1 + G.naggs;
   for (uint32_t base = r0 + t; base < r1; base += U * kBlock) {
+    // records past r1 read record r0 (the partition has rows when the loop runs) and are dropped
     uint32_t sl[U];
+    if (G.naggs == 1) {
+      ulonglong2 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t r = base + u * kBlock;
+        x[u] = *(const ulonglong2*)(rec + (int64_t)(r < r1 ? r : r0) * 2);
+      }
+      const int kind = G.kind[0];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (base + u * kBlock >= r1) continue;
+        sl[u] = (uint32_t)x[u].x;
+        atomicAdd(&lcnt[sl[u]], 1u);
+        acc_apply(&lacc[sl[u]], kind, x[u].y);
+      }
+      continue;
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t r = base + u * kBlock;
-      sl[u] = r < r1 ? rec_key[r] : 0u;
+      sl[u] = (uint32_t)rec[(int64_t)(r < r1 ? r : r0) * rw];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (base + u * kBlock < r1) atomicAdd(&lcnt[sl[u]], 1u);
     for (int a = 0; a < G.naggs; ++a) {
-      const uint64_t* rv = rec_val + (int64_t)a * cap;
       uint64_t v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const uint32_t r = base + u * kBlock;
-        v[u] = r < r1 ? rv[r] : 0ull;
+        v[u] = rec[(int64_t)(r < r1 ? r : r0) * rw + 1 + a];
       }
       const int kind = G.kind[a];
 #pragma unroll
@@ -669,6 +753,9 @@ struct GbWs {
   uint32_t* v1;
   void* stats;
   void* radix;
+  // dense-LDS mode: per-workgroup tables, combined by k_gb_merge
+  uint64_t* part;             // part[block][slot][count, aggregates...]
+  int32_t pblocks;            // workgroups of the accumulate launch
   // partitioned dense mode
   int32_t part_bits, nparts;  // partitions of 2^part_bits slots
   int32_t tsub;               // 1024-row sub-tiles per scatter tile
@@ -677,8 +764,7 @@ struct GbWs {
   uint32_t* hoff;             // exclusive scan of hist: first scattered position of (b, t)
   uint32_t* htotal;
   uint32_t* hscan_ws;
-  uint32_t* rec_key;          // slot within the partition, per scattered row
-  uint64_t* rec_val;          // rec_val[a * rows_cap + i]: argument of aggregate a
+  uint64_t* rec;              // per scattered row: {slot within the partition, arguments...}
   int64_t rows_cap;
   uint32_t* grp_cnt;          // groups per partition
   int64_t* grp_off;           // exclusive scan of grp_cnt
@@ -699,6 +785,7 @@ static int ilog2_ceil(int64_t v) {
 // (u32 count + one u64 per aggregate per slot) fits kPartLds.
 constexpr size_t kPartLds = 48 * 1024;
 constexpr int kPartMax = 2048;
+constexpr int64_t kPartRows = 65536;  // average input rows per partition at most
 static int part_bits_for(int naggs) {
   const size_t per = 4 + 8 * (size_t)(naggs > 0 ? naggs : 1);
   int b = 0;
@@ -713,6 +800,17 @@ static int part_tsub(int64_t nrows, int nparts) {
   int t = 1;
   while (ceil_div(subs, t) * nparts > ((int64_t)1 << 24)) t *= 2;
   return t;
+}
+
+// Workgroups of a dense-LDS accumulate launch: one per 1024-row sub-tile, at most 1024, and few
+// enough that their partial tables stay within 32 MB.
+static int32_t dense_lds_grid(int64_t nrows, int64_t slots, int naggs) {
+  int64_t g = ceil_div(nrows > 0 ? nrows : 1, kSubRows);
+  if (g > 1024) g = 1024;
+  const int64_t per = slots * (1 + naggs) * 8;
+  const int64_t cap = ((int64_t)32 << 20) / (per > 0 ? per : 1);
+  if (g > cap) g = cap;
+  return (int32_t)(g < 1 ? 1 : g);
 }
 
 static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t rows_cap,
@@ -736,8 +834,7 @@ static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t row
     w.hoff = (uint32_t*)p; p += align_up((size_t)nh * 4, 256);
     w.htotal = (uint32_t*)p; p += 256;
     w.hscan_ws = (uint32_t*)p; p += tq_scan_ws_bytes(nh);
-    w.rec_key = (uint32_t*)p; p += align_up((size_t)w.rows_cap * 4, 256);
-    w.rec_val = (uint64_t*)p; p += align_up((size_t)w.rows_cap * 8 * na, 256);
+    w.rec = (uint64_t*)p; p += align_up((size_t)w.rows_cap * 8 * (1 + na), 256);
     w.grp_cnt = (uint32_t*)p; p += align_up((size_t)w.nparts * 4, 256);
     w.grp_off = (int64_t*)p; p += align_up((size_t)w.nparts * 8, 256);
     w.ngroups = (int64_t*)p; p += 256;
@@ -747,6 +844,11 @@ static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t row
     w.gacc = (uint64_t*)p; p += align_up((size_t)gcap * 8 * na, 256);
     w.bytes = (size_t)(p - (char*)base);
     return w;
+  }
+  if (bm == TQ_GB_DENSE_LDS) {
+    w.pblocks = dense_lds_grid(nrows, slots, naggs);
+    w.part = (uint64_t*)p;
+    p += align_up((size_t)w.pblocks * slots * (1 + naggs) * 8, 256);
   }
   w.T.cnt = (uint32_t*)p; p += align_up((size_t)slots * 4, 256);
   w.T.hkey = (unsigned long long*)p; p += bm == TQ_GB_HASH ? align_up((size_t)slots * 8, 256) : 0;
@@ -812,16 +914,38 @@ static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpe
   return TQ_OK;
 }
 
+// Launch with `lds` bytes of dynamic LDS; above 64 KB the kernel is first allowed that much
+// (gfx950 gives a workgroup up to 160 KB).
+template <typename... P, typename... A>
+static int launch_lds(const char* name, void (*kernel)(P...), unsigned grid, size_t lds,
+                      hipStream_t s, A&&... a) {
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void*)kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return set_error(TQ_EHIP, "%s: %zu bytes of LDS: %s", name, lds,
+                                          hipGetErrorString(e));
+  }
+  return launch(name, kernel, grid, kBlock, lds, s, std::forward<A>(a)...);
+}
+
 static size_t lds_stage_bytes(int32_t nload) { return (size_t)(nload > 0 ? nload : 1) * kSubRows * 8; }
 
 static int32_t lds_cap(int mode, int32_t nload, int naggs, int64_t slots) {
-  const size_t avail = kLdsBudget - lds_stage_bytes(nload);
-  if ((mode & 7) == TQ_GB_DENSE_LDS) return (int32_t)slots;
+  if ((mode & 7) == TQ_GB_DENSE_LDS) return (int32_t)(slots * ((mode & TQ_GB_LDS_LANES) ? 64 : 1));
   if (!(mode & TQ_GB_LDS_FRONT)) return 0;
+  const size_t avail = kLdsBig - lds_stage_bytes(nload);
   const size_t per = 12 + 8 * (size_t)naggs;
   int32_t cap = 1;
-  while ((size_t)cap * 2 * per <= avail) cap *= 2;
+  while ((size_t)cap * 2 * per <= avail && cap < 8192) cap *= 2;
   return cap;
+}
+
+// LDS bytes of an accumulate launch.
+static size_t accum_lds(int mode, int32_t nload, int naggs, int32_t cap) {
+  size_t lds = lds_stage_bytes(nload);
+  if ((mode & 7) == TQ_GB_DENSE_LDS) lds += (size_t)cap * (8 * naggs + 4);
+  else if (mode & TQ_GB_LDS_FRONT) lds += (size_t)cap * (8 * naggs + 12);
+  return lds;
 }
 
 }  // namespace tq
@@ -874,15 +998,20 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
   const size_t dense_per = 4 + 8 * (size_t)d->naggs;
   int32_t m;
   int64_t sl;
-  if ((size_t)S * dense_per + stage <= (size_t)kLdsBudget) {
+  if (S <= 64 && (size_t)S * 64 * dense_per + stage <= (size_t)kLdsBig) {
+    // a handful of groups: one table copy per lane keeps their rows off shared LDS addresses
+    m = TQ_GB_DENSE_LDS | TQ_GB_LDS_LANES;
+    sl = S;
+  } else if ((size_t)S * dense_per + stage <= (size_t)kLdsBudget) {
     m = TQ_GB_DENSE_LDS;
     sl = S;
   } else {
     const int64_t distinct = ns < S ? ns : S;
     const int64_t nparts = ceil_div(S, (int64_t)1 << part_bits_for(d->naggs));
     const bool dense_ok = S <= ((int64_t)1 << 27) && S <= (4 * ns > 65536 ? 4 * ns : 65536);
+    // partitions are reduced one workgroup each, so each must get a modest share of the rows
     if (dense_ok && nparts <= kPartMax && stage + (size_t)nparts * 4 <= (size_t)kLdsBudget &&
-        d->nrows > 0 && d->nrows < ((int64_t)1 << 32)) {
+        d->nrows > 0 && d->nrows < ((int64_t)1 << 32) && d->nrows <= nparts * kPartRows) {
       m = TQ_GB_DENSE_PART;
       sl = S;
     } else if (dense_ok) {
@@ -893,7 +1022,7 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
       sl = (int64_t)1 << ilog2_ceil(2 * (distinct > 2048 ? distinct : 2048));
     }
     const size_t per = 12 + 8 * (size_t)d->naggs;
-    const bool room = stage + 1024 * per <= (size_t)kLdsBudget;
+    const bool room = stage + 1024 * per <= (size_t)kLdsBig;
     if (m != TQ_GB_DENSE_PART && room && distinct > 0 && ns >= 4 * distinct) m |= TQ_GB_LDS_FRONT;
   }
   *mode = m;
@@ -937,31 +1066,41 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
     TQ_TRY((scan_exclusive<uint32_t, uint32_t>(w.hist, nh, w.hoff, w.htotal, w.hscan_ws, s)));
     TQ_TRY(launch("k_gbp_scatter", k_gbp_scatter, (unsigned)w.ptiles, kBlock, lds_scatter, s, G,
                   *prog, *cols, mask, n, w.part_bits, np, w.tsub, w.ptiles,
-                  (const uint32_t*)w.hoff, w.rec_key, w.rec_val, w.rows_cap, stage_words));
+                  (const uint32_t*)w.hoff, w.rec, stage_words));
     return launch("k_gbp_reduce", k_gbp_reduce, (unsigned)np, kBlock, lds_reduce, s, G,
                   w.part_bits, np, w.ptiles, (const uint32_t*)w.hoff, (const uint32_t*)w.htotal,
-                  (const uint32_t*)w.rec_key, (const uint64_t*)w.rec_val, w.rows_cap, w.grp_cnt,
+                  (const uint64_t*)w.rec, w.grp_cnt,
                   w.gslot, w.gcnt, w.gacc);
   }
-  TQ_TRY(launch("k_gb_clear", k_gb_clear, stream_grid(slots), kBlock, 0, s, w.T, G, base == TQ_GB_HASH));
+  const bool dense_lds = base == TQ_GB_DENSE_LDS;
+  const bool lanes = dense_lds && (mode & TQ_GB_LDS_LANES) != 0;
+  // a dense-LDS table is written whole by the merge of the workgroups' partials
+  if (!dense_lds || n <= 0)
+    TQ_TRY(launch("k_gb_clear", k_gb_clear, stream_grid(slots), kBlock, 0, s, w.T, G, base == TQ_GB_HASH));
   if (n <= 0) return TQ_OK;
   const int32_t cap = lds_cap(mode, prog->nload, d->naggs, slots);
   const int32_t stage_words = (int32_t)(lds_stage_bytes(prog->nload) / 8);
-  size_t lds = lds_stage_bytes(prog->nload);
-  if (base == TQ_GB_DENSE_LDS) lds += (size_t)cap * (8 * d->naggs + 4);
-  else if (front) lds += (size_t)cap * (8 * d->naggs + 12);
-  if (lds > (size_t)kLdsBudget) return set_error(TQ_ECAPACITY, "group-by: LDS table too large");
+  const size_t lds = accum_lds(mode, prog->nload, d->naggs, cap);
+  if (lds > (size_t)kLdsBig) return set_error(TQ_ECAPACITY, "group-by: LDS table too large");
+  if (dense_lds && n != d->nrows)
+    return set_error(TQ_EINVAL, "group-by: %lld rows, planned for %lld", (long long)n,
+                     (long long)d->nrows);
   int64_t g = ceil_div(n, kSubRows);
-  if (g > 1024) g = 1024;
-  const unsigned grid = (unsigned)g;
-#define TQ_ACCUM(BASE, FRONT, CAP)                                                          \
-  launch("k_gb_accum", (k_gb_accum<BASE, FRONT>), grid, kBlock, lds, s, G, *prog, *cols, mask, \
-         n, w.T, CAP, stage_words)
-  if (base == TQ_GB_DENSE_LDS) return TQ_ACCUM(TQ_GB_DENSE_LDS, false, cap);
-  if (base == TQ_GB_DENSE_GLOBAL && front) return TQ_ACCUM(TQ_GB_DENSE_GLOBAL, true, cap);
-  if (base == TQ_GB_DENSE_GLOBAL) return TQ_ACCUM(TQ_GB_DENSE_GLOBAL, false, 1);
-  if (front) return TQ_ACCUM(TQ_GB_HASH, true, cap);
-  return TQ_ACCUM(TQ_GB_HASH, false, 1);
+  if (g > (front ? 512 : 1024)) g = front ? 512 : 1024;
+  const unsigned grid = dense_lds ? (unsigned)w.pblocks : (unsigned)g;
+#define TQ_ACCUM(BASE, FRONT, LANES, CAP)                                                     \
+  launch_lds("k_gb_accum", (k_gb_accum<BASE, FRONT, LANES>), grid, lds, s, G, *prog, *cols,  \
+             mask, n, w.T, CAP, stage_words, w.part)
+  if (dense_lds) {
+    TQ_TRY(lanes ? TQ_ACCUM(TQ_GB_DENSE_LDS, false, true, cap)
+                 : TQ_ACCUM(TQ_GB_DENSE_LDS, false, false, cap));
+    return launch("k_gb_merge", k_gb_merge, stream_grid(slots * (1 + d->naggs)), kBlock, 0, s, G,
+                  (const uint64_t*)w.part, w.pblocks, w.T);
+  }
+  if (base == TQ_GB_DENSE_GLOBAL && front) return TQ_ACCUM(TQ_GB_DENSE_GLOBAL, true, false, cap);
+  if (base == TQ_GB_DENSE_GLOBAL) return TQ_ACCUM(TQ_GB_DENSE_GLOBAL, false, false, 1);
+  if (front) return TQ_ACCUM(TQ_GB_HASH, true, false, cap);
+  return TQ_ACCUM(TQ_GB_HASH, false, false, 1);
 #undef TQ_ACCUM
 }
 

@@ -45,6 +45,7 @@ OP = {name: i for i, name in enumerate(OPCODES)}
 TQ_AGG_SUM, TQ_AGG_COUNT, TQ_AGG_AVG, TQ_AGG_MIN, TQ_AGG_MAX = range(5)
 TQ_GB_MAX_KEYS = 8
 TQ_GB_DENSE_LDS, TQ_GB_DENSE_GLOBAL, TQ_GB_HASH, TQ_GB_DENSE_PART, TQ_GB_LDS_FRONT = 0, 1, 2, 3, 8
+TQ_GB_LDS_LANES = 16
 TQ_JOIN_EMPTY, TQ_JOIN_DENSE, TQ_JOIN_HASH = 0, 1, 2
 
 DTYPE_CODE = {
