@@ -367,6 +367,18 @@ int tq_groupby_write(const tq_groupby_desc_t* desc /* host */, int32_t mode, int
                      void* ws, int64_t ngroups, void* const* key_out /* host */,
                      void* const* agg_out /* host */, void* stream);
 
+/* Passes 2-4 in one call with one synchronisation: the caller allocates the outputs for
+ * out_cap groups (min(selected rows, slots), at least 1, always suffices) and receives the
+ * group count in *ngroups_host; groups as tq_groupby_write writes them.  In the partitioned
+ * mode each partition's workgroup adds up the group counts the partitions before it publish
+ * and writes its groups at that offset itself, so no separate numbering or write pass runs.
+ * Same reference interface as passes 2-4 (operators/aggregate.py:127-193, 302-327). */
+int tq_groupby_run(const tq_groupby_desc_t* desc /* host */, const tq_program_t* prog /* host */,
+                   const tq_colset_t* cols /* host */, const uint8_t* mask, int64_t n,
+                   int32_t mode, int64_t slots, void* ws, size_t ws_bytes,
+                   void* const* key_out /* host */, void* const* agg_out /* host */,
+                   int64_t out_cap, int64_t* ngroups_host /* host out */, void* stream);
+
 /* ---------------------------------------------------------------- batched pipelines */
 /* Many filter -> join -> group-by pipelines in one persistent launch.  A join pipeline (mode 0)
  * joins its selected build rows (unique build keys) with its selected probe rows on key equality

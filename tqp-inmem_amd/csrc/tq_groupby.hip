@@ -13,10 +13,13 @@
 // Dense tables come out in key order by compacting occupied slots; hash tables are ordered by a
 // radix sort of their packed keys.  Sums use float64 atomics (hardware global/LDS f64 add), so
 // their summation order varies between runs; counts and integer sums are exact.
+#include <map>
+#include <mutex>
+
+#include "tq_agg.h"
 #include "tq_common.h"
 #include "tq_expr.h"
 #include "tq_sort.h"
-#include "tq_agg.h"
 
 namespace tq {
 
@@ -308,20 +311,36 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
   }
 }
 
-// Dense-LDS partials of nblocks workgroups combined into the table (count, then aggregates).
+// Dense-LDS partials of nblocks workgroups combined into the table (count, then aggregates):
+// one wave per field, its lanes splitting the partials and combining them by shuffles.
 __global__ __launch_bounds__(kBlock) void k_gb_merge(GbSpec G, const uint64_t* __restrict__ part,
                                                      int32_t nblocks, GbTable T) {
   const int nf = 1 + G.naggs;
   const int64_t S = T.slots, total = S * nf;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += stride) {
+  const int l = lane_id();
+  const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+  for (int64_t e = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6); e < total; e += nwaves) {
     const int64_t s = e / nf;
     const int f = (int)(e - s * nf);
     const int kind = f == 0 ? ACC_COUNT : G.kind[f - 1];
-    uint64_t v = f == 0 ? 0ull : G.init[f - 1];
-    for (int b = 0; b < nblocks; ++b) v = acc_combine(kind, v, part[(int64_t)b * total + e]);
-    if (f == 0) T.cnt[s] = (uint32_t)v;
-    else T.acc[(int64_t)(f - 1) * S + s] = v;
+    const uint64_t init = f == 0 ? 0ull : G.init[f - 1];
+    uint64_t v = init;
+    for (int b0 = l; b0 < nblocks; b0 += 4 * 64) {
+      uint64_t x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int b = b0 + u * 64;
+        x[u] = b < nblocks ? part[(int64_t)b * total + e] : init;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v = acc_combine(kind, v, x[u]);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = acc_combine(kind, v, __shfl_xor(v, d, 64));
+    if (l == 0) {
+      if (f == 0) T.cnt[s] = (uint32_t)v;
+      else T.acc[(int64_t)(f - 1) * S + s] = v;
+    }
   }
 }
 
@@ -463,12 +482,16 @@ __device__ __forceinline__ void packed_keys4(const KeyCols& K, const GbSpec& G, 
   }
 }
 
+// Workgroup 0 also clears the look-back words of the fused reduce (flags[0..nparts + 1)).
 __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
                                                      const uint8_t* __restrict__ mask, int64_t n,
                                                      int pb, int nparts, int tsub, int64_t ptiles,
-                                                     uint32_t* __restrict__ hist) {
+                                                     uint32_t* __restrict__ hist,
+                                                     uint64_t* __restrict__ flags) {
   extern __shared__ uint32_t lh[];
   const int t = threadIdx.x;
+  if (flags && blockIdx.x == 0)
+    for (int i = t; i <= nparts; i += kBlock) flags[i] = 0;
   for (int i = t; i < nparts; i += kBlock) lh[i] = 0;
   __syncthreads();
   const int64_t tile = blockIdx.x;
@@ -547,15 +570,41 @@ __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
   }
 }
 
-// One workgroup per partition: reduce its rows into an LDS table, then keep its occupied slots,
-// in slot order, at [b << pb, ...) of gslot/gcnt/gacc and their number in grp_cnt[b].
+// Look-back word of a partition: (groups << 2) | 1 once its own group count is known (0 until
+// then).
+struct LookBack {
+  uint64_t* flags;   // [nparts] look-back words, then [nparts] = error flag
+  int64_t* ngroups;  // total groups (written by the last partition)
+  int64_t out_cap;   // rows of the outputs
+  uint32_t spin_limit;
+};
+
+// The words carry only their own values (no other data is published through them), so relaxed
+// device-scope atomics suffice: they reach every XCD without the cache write-back and
+// invalidation that release/acquire ordering costs.
+__device__ __forceinline__ uint64_t ld_word64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_word64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One workgroup per partition: reduce its rows into an LDS table, then compact its occupied
+// slots in slot order.  Unfused: they go to [b << pb, ...) of gslot/gcnt/gacc and their number
+// to grp_cnt[b], for a scan and k_gbp_write.  FUSED: the workgroup publishes its group count
+// and adds up those of the partitions before it to find its output offset (workgroups start in
+// index order, so every one waited on is running or done; the wait is bounded and a timeout
+// sets the error word, after which the host takes the unfused path), then writes the final key
+// and aggregate columns itself.
+template <bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
     GbSpec G, int pb, int nparts, int64_t ptiles, const uint32_t* __restrict__ hoff,
     const uint32_t* __restrict__ htotal, const uint64_t* __restrict__ rec,
     uint32_t* __restrict__ grp_cnt, uint32_t* __restrict__ gslot, uint32_t* __restrict__ gcnt,
-    uint64_t* __restrict__ gacc) {
+    uint64_t* __restrict__ gacc, OutCols O, LookBack LB) {
   extern __shared__ uint64_t smem[];
   __shared__ uint32_t wocc[kItems][kWaves];
+  __shared__ int64_t s_off;
   const int ps = 1 << pb;
   uint64_t* lacc = smem;                                   // naggs * ps
   uint32_t* lcnt = (uint32_t*)(smem + (int64_t)G.naggs * ps);
@@ -625,6 +674,45 @@ __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
     if (lane_id() == 0) wocc[j][w] = (uint32_t)__popcll(bal[j]);
   }
   __syncthreads();
+  int64_t obase = 0;
+  if (FUSED) {
+    if (t < 64) {
+      // wave 0: publish this partition's group count, then add up the counts of every partition
+      // before it, 64 per load round (all partitions reduce at once, so nothing waits on a
+      // chain of offsets)
+      const int l = lane_id();
+      uint64_t own = 0;
+#pragma unroll
+      for (int j = 0; j < kItems; ++j)
+#pragma unroll
+        for (int v = 0; v < kWaves; ++v) own += wocc[j][v];
+      if (l == 0) st_word64(LB.flags + b, (own << 2) | 1ull);
+      uint64_t before = 0;
+      bool ok = true;
+      for (int64_t q0 = 0; q0 < b && ok; q0 += 64) {
+        const int64_t q = q0 + l;
+        uint64_t f = 1ull;  // lanes past b - 1 add nothing
+        uint32_t spins = 0;
+        while (true) {
+          if (q < b) f = ld_word64(LB.flags + q);
+          if (ballot(f == 0) == 0) break;
+          if (++spins > LB.spin_limit) {
+            ok = false;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        before += wave_sum(q < b ? (f >> 2) : 0ull);
+      }
+      if (l == 0) {
+        if (!ok) st_word64(LB.flags + nparts, 1ull);
+        if (b == nparts - 1) *LB.ngroups = (int64_t)(before + own);
+        s_off = ok ? (int64_t)before : -1;
+      }
+    }
+    __syncthreads();
+    obase = s_off;
+  }
   uint32_t run = 0;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
@@ -637,14 +725,27 @@ __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
     }
     if ((bal[j] >> lane_id()) & 1u) {
       const int i = j * kBlock + t;
-      const int64_t o = gbase + run + before + rank_below(bal[j]);
-      gslot[o] = (uint32_t)i;
-      gcnt[o] = lcnt[i];
-      for (int a = 0; a < G.naggs; ++a) gacc[(int64_t)a * gstride + o] = lacc[a * ps + i];
+      const uint32_t local = run + before + rank_below(bal[j]);
+      if (FUSED) {
+        const int64_t o = obase + local;
+        if (obase >= 0 && o < LB.out_cap) {
+          const uint64_t pk = (uint64_t)gbase + (uint64_t)i;
+          for (int k = 0; k < G.nkeys; ++k) store_int(O.key[k], G.key_dtype[k], o, key_of(G, k, pk));
+          for (int a = 0; a < G.naggs; ++a)
+            write_agg(O.agg[a], o, O.op[a], G.kind[a], G.arg_dtype[a], lacc[a * ps + i], lcnt[i]);
+        } else if (obase >= 0) {
+          st_word64(LB.flags + nparts, 1ull);  // more groups than the outputs hold
+        }
+      } else {
+        const int64_t o = gbase + local;
+        gslot[o] = (uint32_t)i;
+        gcnt[o] = lcnt[i];
+        for (int a = 0; a < G.naggs; ++a) gacc[(int64_t)a * gstride + o] = lacc[a * ps + i];
+      }
     }
     run += tot;
   }
-  if (t == 0) grp_cnt[b] = run;
+  if (!FUSED && t == 0) grp_cnt[b] = run;
 }
 
 // Write partition b's groups at grp_off[b]...: keys decoded from the packed key, aggregates
@@ -766,6 +867,7 @@ struct GbWs {
   uint32_t* hscan_ws;
   uint64_t* rec;              // per scattered row: {slot within the partition, arguments...}
   int64_t rows_cap;
+  uint64_t* flags;            // fused reduce: look-back words [nparts], error word
   uint32_t* grp_cnt;          // groups per partition
   int64_t* grp_off;           // exclusive scan of grp_cnt
   int64_t* grp_scan_ws;
@@ -835,6 +937,7 @@ static GbWs carve_gb(void* base, int mode, int64_t slots, int naggs, int64_t row
     w.htotal = (uint32_t*)p; p += 256;
     w.hscan_ws = (uint32_t*)p; p += tq_scan_ws_bytes(nh);
     w.rec = (uint64_t*)p; p += align_up((size_t)w.rows_cap * 8 * (1 + na), 256);
+    w.flags = (uint64_t*)p; p += align_up((size_t)(w.nparts + 1) * 8, 256);
     w.grp_cnt = (uint32_t*)p; p += align_up((size_t)w.nparts * 4, 256);
     w.grp_off = (int64_t*)p; p += align_up((size_t)w.nparts * 8, 256);
     w.ngroups = (int64_t*)p; p += 256;
@@ -916,14 +1019,25 @@ static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpe
 
 // Launch with `lds` bytes of dynamic LDS; above 64 KB the kernel is first allowed that much
 // (gfx950 gives a workgroup up to 160 KB).
+// The allowance is set once per kernel (per device: a kernel's attribute is per device, and the
+// cache below is keyed by both), raised when a larger table is asked for.
 template <typename... P, typename... A>
 static int launch_lds(const char* name, void (*kernel)(P...), unsigned grid, size_t lds,
                       hipStream_t s, A&&... a) {
   if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute((const void*)kernel,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return set_error(TQ_EHIP, "%s: %zu bytes of LDS: %s", name, lds,
-                                          hipGetErrorString(e));
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> allowed;
+    int dev = 0;
+    TQ_TRY_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    size_t& have = allowed[{dev, (const void*)kernel}];
+    if (have < lds) {
+      const hipError_t e = hipFuncSetAttribute(
+          (const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig);
+      if (e != hipSuccess)
+        return set_error(TQ_EHIP, "%s: %zu bytes of LDS: %s", name, lds, hipGetErrorString(e));
+      have = kLdsBig;
+    }
   }
   return launch(name, kernel, grid, kBlock, lds, s, std::forward<A>(a)...);
 }
@@ -1031,17 +1145,15 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
   return TQ_OK;
 }
 
-extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_program_t* prog,
-                                     const tq_colset_t* cols, const uint8_t* mask, int64_t n,
-                                     int32_t mode, int64_t slots, void* ws, size_t ws_bytes,
-                                     void* stream) {
-  if (!d || !prog || !cols) return set_error(TQ_EINVAL, "group-by accumulate: null argument");
+// Accumulate; with fused (partitioned mode only), the partition reduce writes the final groups
+// into *fused through the look-back words.
+static int accumulate_impl(const tq_groupby_desc_t* d, const tq_program_t* prog,
+                           const tq_colset_t* cols, const uint8_t* mask, int64_t n, int32_t mode,
+                           int64_t slots, GbWs& w, hipStream_t s, const OutCols* fused,
+                           LookBack lb) {
   GbSpec G;
   TQ_TRY(make_spec(d, prog, &G));
   if (prog->nload > TQ_MAX_LOADS || prog->n_instr > TQ_MAX_INSTR) return set_error(TQ_EINVAL, "group-by: malformed program");
-  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel, d->nrows);
-  if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "group-by workspace too small");
-  hipStream_t s = (hipStream_t)stream;
   const int base = mode & 7;
   const bool front = (mode & TQ_GB_LDS_FRONT) != 0;
   if (base == TQ_GB_DENSE_PART) {
@@ -1062,15 +1174,20 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
     if (lds_scatter > (size_t)kLdsBudget || lds_reduce > (size_t)kLdsBudget)
       return set_error(TQ_ECAPACITY, "group-by: partition tables exceed LDS");
     TQ_TRY(launch("k_gbp_hist", k_gbp_hist, (unsigned)w.ptiles, kBlock, (size_t)np * 4, s, K, G,
-                  mask, n, w.part_bits, np, w.tsub, w.ptiles, w.hist));
+                  mask, n, w.part_bits, np, w.tsub, w.ptiles, w.hist,
+                  fused ? w.flags : (uint64_t*)nullptr));
     TQ_TRY((scan_exclusive<uint32_t, uint32_t>(w.hist, nh, w.hoff, w.htotal, w.hscan_ws, s)));
     TQ_TRY(launch("k_gbp_scatter", k_gbp_scatter, (unsigned)w.ptiles, kBlock, lds_scatter, s, G,
                   *prog, *cols, mask, n, w.part_bits, np, w.tsub, w.ptiles,
                   (const uint32_t*)w.hoff, w.rec, stage_words));
-    return launch("k_gbp_reduce", k_gbp_reduce, (unsigned)np, kBlock, lds_reduce, s, G,
+    if (fused)
+      return launch("k_gbp_reduce", k_gbp_reduce<true>, (unsigned)np, kBlock, lds_reduce, s, G,
+                    w.part_bits, np, w.ptiles, (const uint32_t*)w.hoff,
+                    (const uint32_t*)w.htotal, (const uint64_t*)w.rec, w.grp_cnt, w.gslot, w.gcnt,
+                    w.gacc, *fused, lb);
+    return launch("k_gbp_reduce", k_gbp_reduce<false>, (unsigned)np, kBlock, lds_reduce, s, G,
                   w.part_bits, np, w.ptiles, (const uint32_t*)w.hoff, (const uint32_t*)w.htotal,
-                  (const uint64_t*)w.rec, w.grp_cnt,
-                  w.gslot, w.gcnt, w.gacc);
+                  (const uint64_t*)w.rec, w.grp_cnt, w.gslot, w.gcnt, w.gacc, OutCols{}, lb);
   }
   const bool dense_lds = base == TQ_GB_DENSE_LDS;
   const bool lanes = dense_lds && (mode & TQ_GB_LDS_LANES) != 0;
@@ -1094,7 +1211,9 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
   if (dense_lds) {
     TQ_TRY(lanes ? TQ_ACCUM(TQ_GB_DENSE_LDS, false, true, cap)
                  : TQ_ACCUM(TQ_GB_DENSE_LDS, false, false, cap));
-    return launch("k_gb_merge", k_gb_merge, stream_grid(slots * (1 + d->naggs)), kBlock, 0, s, G,
+    int64_t mg = ceil_div(slots * (1 + d->naggs), kWaves);
+    if (mg > 4096) mg = 4096;
+    return launch("k_gb_merge", k_gb_merge, (unsigned)mg, kBlock, 0, s, G,
                   (const uint64_t*)w.part, w.pblocks, w.T);
   }
   if (base == TQ_GB_DENSE_GLOBAL && front) return TQ_ACCUM(TQ_GB_DENSE_GLOBAL, true, false, cap);
@@ -1102,6 +1221,17 @@ extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_progra
   if (front) return TQ_ACCUM(TQ_GB_HASH, true, false, cap);
   return TQ_ACCUM(TQ_GB_HASH, false, false, 1);
 #undef TQ_ACCUM
+}
+
+extern "C" int tq_groupby_accumulate(const tq_groupby_desc_t* d, const tq_program_t* prog,
+                                     const tq_colset_t* cols, const uint8_t* mask, int64_t n,
+                                     int32_t mode, int64_t slots, void* ws, size_t ws_bytes,
+                                     void* stream) {
+  if (!d || !prog || !cols) return set_error(TQ_EINVAL, "group-by accumulate: null argument");
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel, d->nrows);
+  if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "group-by workspace too small");
+  return accumulate_impl(d, prog, cols, mask, n, mode, slots, w, (hipStream_t)stream, nullptr,
+                         LookBack{});
 }
 
 extern "C" int tq_groupby_count(const tq_groupby_desc_t* d, int32_t mode, int64_t slots, void* ws,
@@ -1151,4 +1281,48 @@ extern "C" int tq_groupby_write(const tq_groupby_desc_t* d, int32_t mode, int64_
   bool alt;
   TQ_TRY(radix_sort_pairs<uint64_t>(w.k0, w.v0, w.k1, w.v1, ngroups, lo, hi, w.radix, s, &alt));
   return launch("k_gb_write_sorted", k_gb_write_sorted, stream_grid(ngroups), kBlock, 0, s, G, O, w.T, alt ? w.k1 : w.k0, alt ? w.v1 : w.v0, ngroups);
+}
+
+extern "C" int tq_groupby_run(const tq_groupby_desc_t* d, const tq_program_t* prog,
+                              const tq_colset_t* cols, const uint8_t* mask, int64_t n,
+                              int32_t mode, int64_t slots, void* ws, size_t ws_bytes,
+                              void* const* key_out, void* const* agg_out, int64_t out_cap,
+                              int64_t* ngroups_host, void* stream) {
+  if (!d || !prog || !cols || !ngroups_host)
+    return set_error(TQ_EINVAL, "group-by run: null argument");
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel, d->nrows);
+  if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "group-by workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  if ((mode & 7) == TQ_GB_DENSE_PART && n > 0 && d->nsel > 0) {
+    OutCols O{};
+    for (int k = 0; k < d->nkeys; ++k) O.key[k] = key_out[k];
+    for (int a = 0; a < d->naggs; ++a) {
+      O.agg[a] = agg_out[a];
+      O.op[a] = d->agg_op[a];
+    }
+    const LookBack lb{w.flags, w.ngroups, out_cap, 1u << 20};
+    TQ_TRY(accumulate_impl(d, prog, cols, mask, n, mode, slots, w, s, &O, lb));
+    int64_t g = 0, err = 0;
+    TQ_TRY(read_back2(w.ngroups, 8, &g, w.flags + w.nparts, 8, &err, s));
+    if (!err) {
+      *ngroups_host = g;
+      return TQ_OK;
+    }
+    // the look-back gave up: reduce again without it (the scattered records are intact)
+    GbSpec G;
+    TQ_TRY(make_spec(d, prog, &G));
+    const size_t lds_reduce = ((size_t)1 << w.part_bits) * (4 + 8 * (size_t)d->naggs);
+    TQ_TRY(launch("k_gbp_reduce", k_gbp_reduce<false>, (unsigned)w.nparts, kBlock, lds_reduce, s,
+                  G, w.part_bits, w.nparts, w.ptiles, (const uint32_t*)w.hoff,
+                  (const uint32_t*)w.htotal, (const uint64_t*)w.rec, w.grp_cnt, w.gslot, w.gcnt,
+                  w.gacc, OutCols{}, LookBack{}));
+  } else {
+    TQ_TRY(accumulate_impl(d, prog, cols, mask, n, mode, slots, w, s, nullptr, LookBack{}));
+  }
+  int64_t g = 0;
+  TQ_TRY(tq_groupby_count(d, mode, slots, ws, &g, stream));
+  if (g > out_cap) return set_error(TQ_ECAPACITY, "group-by: %lld groups, outputs hold %lld",
+                                    (long long)g, (long long)out_cap);
+  *ngroups_host = g;
+  return tq_groupby_write(d, mode, slots, ws, g, key_out, agg_out, stream);
 }

@@ -142,6 +142,7 @@ _SIGNATURES = {
     "tq_groupby_accumulate": ([P, P, P, P, I64, I32, I64, P, SZ, P], I32),
     "tq_groupby_count": ([P, I32, I64, P, P, P], I32),
     "tq_groupby_write": ([P, I32, I64, P, I64, P, P, P], I32),
+    "tq_groupby_run": ([P, P, P, P, I64, I32, I64, P, SZ, P, P, I64, P, P], I32),
     "tq_batch_ws_bytes": ([I32], SZ),
     "tq_batch_run": ([P, I32, P, P, I32, I32, I32, I32, P, ctypes.c_uint32, P], I32),
 }

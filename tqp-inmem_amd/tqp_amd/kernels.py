@@ -361,9 +361,7 @@ class GroupBy:
         ws = L.workspace(wsb.value, dev, "groupby")
         # every selected row's loaded columns (keys included) and the mask byte of every row
         alg = nsel * b.load_bytes() + (self.n if self.mask is not None else 0)
-        L.call("tq_groupby_accumulate", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr,
-               self.n, mode, slots, L.ptr(ws), wsb.value, s, alg_bytes=alg, kernel="k_gb_accum")
-        out_dtypes = []   # decided before the count's synchronisation
+        out_dtypes = []
         for op, dt in zip(self.ops, self.arg_dtypes):
             if op == L.TQ_AGG_COUNT:
                 odt = torch.int64
@@ -374,14 +372,17 @@ class GroupBy:
             else:
                 odt = dt
             out_dtypes.append(odt)
+        # outputs sized for the most groups there can be; the groups written are a prefix
+        cap = max(1, min(nsel, self.slots))
+        key_full = [torch.empty(cap, dtype=k.dtype, device=dev) for k in self.keys]
+        agg_full = [torch.empty(cap, dtype=odt, device=dev) for odt in out_dtypes]
         ng = ctypes.c_int64()
-        L.call("tq_groupby_count", ctypes.byref(d), mode, slots, L.ptr(ws), ctypes.byref(ng), s)
+        L.call("tq_groupby_run", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr, self.n,
+               mode, slots, L.ptr(ws), wsb.value, L.ptr_array(key_full), L.ptr_array(agg_full), cap,
+               ctypes.byref(ng), s, alg_bytes=alg, kernel="k_gb_accum")
         G = int(ng.value)
-        key_out = [torch.empty(G, dtype=k.dtype, device=dev) for k in self.keys]
-        agg_out = [torch.empty(G, dtype=odt, device=dev) for odt in out_dtypes]
-        if G > 0:
-            L.call("tq_groupby_write", ctypes.byref(d), mode, slots, L.ptr(ws), G,
-                   L.ptr_array(key_out), L.ptr_array(agg_out), s)
+        key_out = [o[:G] for o in key_full]
+        agg_out = [o[:G] for o in agg_full]
         return key_out, agg_out
 
 
