@@ -501,15 +501,40 @@ __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* _
 
 // Output column x of the pairs listed in LDS: element i of the tile's output (at obase + i)
 // takes src row from_build[i] (build side) or tile0 + from_probe[i] (probe side).  Consecutive
-// threads write consecutive outputs.
+// threads write consecutive outputs.  Thread t owns elements t, t + kBlock, ... (R rounds, R *
+// kBlock >= total) and issues all R loads before its first store, so R loads per thread are in
+// flight instead of one.  The code is straight-line: lanes past the list's end load the last
+// listed pair again and store that same value to the last output, a duplicate of the same
+// bytes, so no branch separates the loads and the stores.
+template <typename T, bool PROBE, int R>
+__device__ __forceinline__ void gather_rounds(T* __restrict__ d, const T* __restrict__ s,
+                                              uint32_t total, const uint32_t* from_build,
+                                              const uint16_t* from_probe, int64_t tile0) {
+  T v[R];
+  uint32_t at[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t i = r * kBlock + threadIdx.x;
+    at[r] = i < total ? i : total - 1;
+    const int64_t from = PROBE ? tile0 + from_probe[at[r]] : (int64_t)from_build[at[r]];
+    v[r] = s[from];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) d[at[r]] = v[r];
+}
+
 template <typename T, bool PROBE>
 __device__ __forceinline__ void gather_list(void* dst, const void* src, int64_t obase,
                                             uint32_t total, const uint32_t* from_build,
                                             const uint16_t* from_probe, int64_t tile0) {
-  for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
-    const int64_t from = PROBE ? tile0 + from_probe[i] : (int64_t)from_build[i];
-    ((T*)dst)[obase + i] = ((const T*)src)[from];
-  }
+  static_assert(kItems * kBlock == kTileRows, "a tile lists at most kItems rounds of pairs");
+  if (total == 0) return;
+  T* d = (T*)dst + obase;
+  const T* s = (const T*)src;
+  if (total <= 4 * kBlock)
+    gather_rounds<T, PROBE, 4>(d, s, total, from_build, from_probe, tile0);
+  else
+    gather_rounds<T, PROBE, kItems>(d, s, total, from_build, from_probe, tile0);
 }
 
 template <bool PROBE>
