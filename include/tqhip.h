@@ -210,6 +210,12 @@ size_t tq_sort_ws_bytes(int64_t n);
 int tq_sort_perm(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n, int64_t limit,
                  int64_t* perm_out, void* ws, size_t ws_bytes, void* stream);
 
+/* Copy n small device words (bytes[i] each, 512 bytes in all at most) to host_out, packed at
+ * 8-byte-aligned offsets, with one synchronisation of the stream.  Replaces the `.item()` /
+ * `.cpu()` reads the reference makes of device scalars (e.g. operators/hashjoin.py:76). */
+int tq_read_words(int32_t n, const void* const* dev /* host */, const size_t* bytes /* host */,
+                  int64_t* host_out /* host out */, void* stream);
+
 /* ---------------------------------------------------------------- equi-join */
 /* Host-side handle of a built join index (filled by tq_join_build, read by the probe passes).
  * Keys are order-transformed (sign bit flipped; 32-bit for build keys of up to 32 bits, else
@@ -295,6 +301,33 @@ int tq_join_write(const tq_join_state_t* state /* host */, const void* pkeys, in
                   const int32_t* rbytes /* host */, void* const* rdst /* host */,
                   int64_t* build_idx, int64_t* probe_idx, void* stream);
 
+/* Count, scan and write in one call with nothing read back, for a caller that goes on
+ * launching work that consumes the pairs before it synchronises.  Applies (*applied = 1) when
+ * the state's entries hold build rows (a unique build, settled or still pending) and np > 0;
+ * otherwise *applied = 0 and nothing is launched.  The outputs must hold np rows (a probe row
+ * has at most one pair); the pair count m is left in device memory at *total_dev and the first
+ * m output rows are the pairs, as tq_join_write writes them.  While the state is pending,
+ * *check_dev points at the build's 24 check bytes (read them back with the count and pass them
+ * to tq_join_settle); the pairs are only valid once tq_join_settle accepts them.  Same
+ * reference interface as tq_join_count + tq_join_write (operators/hashjoin.py:57-98). */
+int tq_join_probe_deferred(const tq_join_state_t* state /* host */, const void* pkeys,
+                           int32_t pdtype, const uint8_t* pmask, int64_t np, void* pws,
+                           int32_t nleft, const void* const* lsrc, const int32_t* lbytes,
+                           void* const* ldst, int32_t nright, const void* const* rsrc,
+                           const int32_t* rbytes, void* const* rdst,
+                           const int64_t** total_dev /* host out */,
+                           const void** check_dev /* host out */, int32_t* applied /* host out */,
+                           void* stream);
+
+/* Settle a pending build from its check words as read back after tq_join_probe_deferred:
+ * *ok = 1 when the build's keys were unique (and inside the bounds of a ranged build), which
+ * makes the deferred pairs final and the state settled; *ok = 0 leaves the state pending, and
+ * the caller discards the deferred pairs and probes with tq_join_count / tq_join_write, which
+ * build again.  Host only.  No counterpart in the reference (its join sorts the build side
+ * every time, operators/hashjoin.py:41-52). */
+int tq_join_settle(tq_join_state_t* state /* host, in/out */, const int64_t* words /* host */,
+                   int32_t* ok /* host out */);
+
 /* ---------------------------------------------------------------- group-by aggregate */
 #define TQ_AGG_SUM 0   /* f64 sum for f64 args, f32 for f32, exact int64 for integer/bool args */
 #define TQ_AGG_COUNT 1 /* int64 count of rows whose argument != -64                            */
@@ -378,6 +411,20 @@ int tq_groupby_run(const tq_groupby_desc_t* desc /* host */, const tq_program_t*
                    int32_t mode, int64_t slots, void* ws, size_t ws_bytes,
                    void* const* key_out /* host */, void* const* agg_out /* host */,
                    int64_t out_cap, int64_t* ngroups_host /* host out */, void* stream);
+
+/* tq_groupby_run over rows whose count a producer earlier on the stream left in device memory:
+ * the rows are the first min(n, *n_dev) of the columns, n being the capacity the call is
+ * planned for (desc->nrows == n, partitioned mode only).  Nothing is read back: the group
+ * count is left at *ngroups_dev and a look-back failure flag at *err_dev (non-zero: the
+ * outputs are not valid; run tq_groupby_run instead).  Same reference interface as
+ * tq_groupby_run (operators/aggregate.py:127-193, 302-327). */
+int tq_groupby_run_deferred(const tq_groupby_desc_t* desc /* host */,
+                            const tq_program_t* prog /* host */, const tq_colset_t* cols /* host */,
+                            const uint8_t* mask, int64_t n, const int64_t* n_dev, int32_t mode,
+                            int64_t slots, void* ws, size_t ws_bytes,
+                            void* const* key_out /* host */, void* const* agg_out /* host */,
+                            int64_t out_cap, const int64_t** ngroups_dev /* host out */,
+                            const int64_t** err_dev /* host out */, void* stream);
 
 /* ---------------------------------------------------------------- batched pipelines */
 /* Many filter -> join -> group-by pipelines in one persistent launch.  A join pipeline (mode 0)

@@ -483,18 +483,37 @@ __device__ __forceinline__ void packed_keys4(const KeyCols& K, const GbSpec& G, 
 }
 
 // Workgroup 0 also clears the look-back words of the fused reduce (flags[0..nparts + 1)).
+// Rows and histogram stride of a partitioned pass.  With n_dev, a producer earlier on the stream
+// left the row count in device memory and the rows are the first min(n, *n_dev), n being the
+// capacity the launches were sized for: only the tiles holding rows take part, and the
+// histogram is laid out with that many tiles per partition, so its first nparts * tiles entries
+// are the whole histogram (the scan runs over the capacity; what follows is never used).
+__device__ __forceinline__ int64_t part_rows(int64_t n, const int64_t* n_dev) {
+  if (!n_dev) return n;
+  const int64_t m = *n_dev;
+  return m < n ? (m > 0 ? m : 0) : n;
+}
+__device__ __forceinline__ int64_t part_stride(int64_t rows, const int64_t* n_dev, int tsub,
+                                               int64_t ptiles) {
+  return n_dev ? ceil_div(rows, (int64_t)tsub * kSubRows) : ptiles;
+}
+
 __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
                                                      const uint8_t* __restrict__ mask, int64_t n,
+                                                     const int64_t* __restrict__ n_dev,
                                                      int pb, int nparts, int tsub, int64_t ptiles,
                                                      uint32_t* __restrict__ hist,
                                                      uint64_t* __restrict__ flags) {
   extern __shared__ uint32_t lh[];
   const int t = threadIdx.x;
+  n = part_rows(n, n_dev);
+  const int64_t pt = part_stride(n, n_dev, tsub, ptiles);
   if (flags && blockIdx.x == 0)
     for (int i = t; i <= nparts; i += kBlock) flags[i] = 0;
+  const int64_t tile = blockIdx.x;
+  if (tile >= pt) return;
   for (int i = t; i < nparts; i += kBlock) lh[i] = 0;
   __syncthreads();
-  const int64_t tile = blockIdx.x;
   for (int sub = 0; sub < tsub; ++sub) {
     const int64_t sub0 = (tile * tsub + sub) * kSubRows;
     bool live[kSub];
@@ -506,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
       if (live[j]) atomicAdd(&lh[pk[j] >> pb], 1u);
   }
   __syncthreads();
-  for (int i = t; i < nparts; i += kBlock) hist[(int64_t)i * ptiles + tile] = lh[i];
+  for (int i = t; i < nparts; i += kBlock) hist[(int64_t)i * pt + tile] = lh[i];
 }
 
 // Scatter a tile's selected rows: each takes the next position of its partition's range and
@@ -514,15 +533,18 @@ __global__ __launch_bounds__(kBlock) void k_gbp_hist(KeyCols K, GbSpec G,
 // domain)...} in 1 + naggs words (one 16-byte store for a single aggregate), so a row costs
 // one scattered write rather than one per field.
 __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
-    GbSpec G, tq_program_t P, tq_colset_t C, const uint8_t* __restrict__ mask, int64_t n, int pb,
-    int nparts, int tsub, int64_t ptiles, const uint32_t* __restrict__ hoff,
-    uint64_t* __restrict__ rec, int32_t stage_words) {
+    GbSpec G, tq_program_t P, tq_colset_t C, const uint8_t* __restrict__ mask, int64_t n,
+    const int64_t* __restrict__ n_dev, int pb, int nparts, int tsub, int64_t ptiles,
+    const uint32_t* __restrict__ hoff, uint64_t* __restrict__ rec, int32_t stage_words) {
   extern __shared__ uint64_t smem[];
   uint64_t* stage = smem;
   uint32_t* lh = (uint32_t*)(smem + stage_words);
   const int t = threadIdx.x;
   const int64_t tile = blockIdx.x;
-  for (int i = t; i < nparts; i += kBlock) lh[i] = hoff[(int64_t)i * ptiles + tile];
+  n = part_rows(n, n_dev);  // rows and stride as in k_gbp_hist
+  const int64_t pt = part_stride(n, n_dev, tsub, ptiles);
+  if (tile >= pt) return;
+  for (int i = t; i < nparts; i += kBlock) lh[i] = hoff[(int64_t)i * pt + tile];
   __syncthreads();
   const uint64_t pmask = ((uint64_t)1 << pb) - 1;
   for (int sub = 0; sub < tsub; ++sub) {
@@ -598,7 +620,8 @@ __device__ __forceinline__ void st_word64(uint64_t* p, uint64_t v) {
 // and aggregate columns itself.
 template <bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
-    GbSpec G, int pb, int nparts, int64_t ptiles, const uint32_t* __restrict__ hoff,
+    GbSpec G, int pb, int nparts, int64_t ptiles, int64_t n, const int64_t* __restrict__ n_dev,
+    int tsub, const uint32_t* __restrict__ hoff,
     const uint32_t* __restrict__ htotal, const uint64_t* __restrict__ rec,
     uint32_t* __restrict__ grp_cnt, uint32_t* __restrict__ gslot, uint32_t* __restrict__ gcnt,
     uint64_t* __restrict__ gacc, OutCols O, LookBack LB) {
@@ -615,8 +638,11 @@ __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
     for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
   }
   __syncthreads();
-  const uint32_t r0 = hoff[b * ptiles];
-  const uint32_t r1 = b + 1 < nparts ? hoff[(b + 1) * ptiles] : *htotal;
+  // partition b's records: from its first histogram entry to the next partition's (the grand
+  // total after the last partition of a full-capacity histogram)
+  const int64_t pt = part_stride(part_rows(n, n_dev), n_dev, tsub, ptiles);
+  const uint32_t r0 = hoff[b * pt];
+  const uint32_t r1 = (b + 1) * pt < (int64_t)nparts * ptiles ? hoff[(b + 1) * pt] : *htotal;
   constexpr int U = 8;  // records per thread whose loads are issued together
   const int rw = 1 + G.naggs;
   for (uint32_t base = r0 + t; base < r1; base += U * kBlock) {
@@ -1150,7 +1176,7 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
 static int accumulate_impl(const tq_groupby_desc_t* d, const tq_program_t* prog,
                            const tq_colset_t* cols, const uint8_t* mask, int64_t n, int32_t mode,
                            int64_t slots, GbWs& w, hipStream_t s, const OutCols* fused,
-                           LookBack lb) {
+                           LookBack lb, const int64_t* n_dev = nullptr) {
   GbSpec G;
   TQ_TRY(make_spec(d, prog, &G));
   if (prog->nload > TQ_MAX_LOADS || prog->n_instr > TQ_MAX_INSTR) return set_error(TQ_EINVAL, "group-by: malformed program");
@@ -1174,19 +1200,19 @@ static int accumulate_impl(const tq_groupby_desc_t* d, const tq_program_t* prog,
     if (lds_scatter > (size_t)kLdsBudget || lds_reduce > (size_t)kLdsBudget)
       return set_error(TQ_ECAPACITY, "group-by: partition tables exceed LDS");
     TQ_TRY(launch("k_gbp_hist", k_gbp_hist, (unsigned)w.ptiles, kBlock, (size_t)np * 4, s, K, G,
-                  mask, n, w.part_bits, np, w.tsub, w.ptiles, w.hist,
+                  mask, n, n_dev, w.part_bits, np, w.tsub, w.ptiles, w.hist,
                   fused ? w.flags : (uint64_t*)nullptr));
     TQ_TRY((scan_exclusive<uint32_t, uint32_t>(w.hist, nh, w.hoff, w.htotal, w.hscan_ws, s)));
     TQ_TRY(launch("k_gbp_scatter", k_gbp_scatter, (unsigned)w.ptiles, kBlock, lds_scatter, s, G,
-                  *prog, *cols, mask, n, w.part_bits, np, w.tsub, w.ptiles,
+                  *prog, *cols, mask, n, n_dev, w.part_bits, np, w.tsub, w.ptiles,
                   (const uint32_t*)w.hoff, w.rec, stage_words));
     if (fused)
       return launch("k_gbp_reduce", k_gbp_reduce<true>, (unsigned)np, kBlock, lds_reduce, s, G,
-                    w.part_bits, np, w.ptiles, (const uint32_t*)w.hoff,
+                    w.part_bits, np, w.ptiles, n, n_dev, w.tsub, (const uint32_t*)w.hoff,
                     (const uint32_t*)w.htotal, (const uint64_t*)w.rec, w.grp_cnt, w.gslot, w.gcnt,
                     w.gacc, *fused, lb);
     return launch("k_gbp_reduce", k_gbp_reduce<false>, (unsigned)np, kBlock, lds_reduce, s, G,
-                  w.part_bits, np, w.ptiles, (const uint32_t*)w.hoff, (const uint32_t*)w.htotal,
+                  w.part_bits, np, w.ptiles, n, n_dev, w.tsub, (const uint32_t*)w.hoff, (const uint32_t*)w.htotal,
                   (const uint64_t*)w.rec, w.grp_cnt, w.gslot, w.gcnt, w.gacc, OutCols{}, lb);
   }
   const bool dense_lds = base == TQ_GB_DENSE_LDS;
@@ -1283,6 +1309,34 @@ extern "C" int tq_groupby_write(const tq_groupby_desc_t* d, int32_t mode, int64_
   return launch("k_gb_write_sorted", k_gb_write_sorted, stream_grid(ngroups), kBlock, 0, s, G, O, w.T, alt ? w.k1 : w.k0, alt ? w.v1 : w.v0, ngroups);
 }
 
+extern "C" int tq_groupby_run_deferred(const tq_groupby_desc_t* d, const tq_program_t* prog,
+                                       const tq_colset_t* cols, const uint8_t* mask, int64_t n,
+                                       const int64_t* n_dev, int32_t mode, int64_t slots,
+                                       void* ws, size_t ws_bytes, void* const* key_out,
+                                       void* const* agg_out, int64_t out_cap,
+                                       const int64_t** ngroups_dev, const int64_t** err_dev,
+                                       void* stream) {
+  if (!d || !prog || !cols || !n_dev || !ngroups_dev || !err_dev)
+    return set_error(TQ_EINVAL, "group-by run (deferred): null argument");
+  if ((mode & 7) != TQ_GB_DENSE_PART || n <= 0 || d->nsel <= 0 || n != d->nrows)
+    return set_error(TQ_EINVAL, "group-by run (deferred): needs the partitioned mode planned for "
+                                "the %lld-row capacity", (long long)n);
+  GbWs w = carve_gb(ws, mode, slots, d->naggs, d->nsel, d->nrows);
+  if (ws_bytes < w.bytes) return set_error(TQ_ECAPACITY, "group-by workspace too small");
+  OutCols O{};
+  for (int k = 0; k < d->nkeys; ++k) O.key[k] = key_out[k];
+  for (int a = 0; a < d->naggs; ++a) {
+    O.agg[a] = agg_out[a];
+    O.op[a] = d->agg_op[a];
+  }
+  const LookBack lb{w.flags, w.ngroups, out_cap, 1u << 20};
+  TQ_TRY(accumulate_impl(d, prog, cols, mask, n, mode, slots, w, (hipStream_t)stream, &O, lb,
+                         n_dev));
+  *ngroups_dev = w.ngroups;
+  *err_dev = (const int64_t*)(w.flags + w.nparts);
+  return TQ_OK;
+}
+
 extern "C" int tq_groupby_run(const tq_groupby_desc_t* d, const tq_program_t* prog,
                               const tq_colset_t* cols, const uint8_t* mask, int64_t n,
                               int32_t mode, int64_t slots, void* ws, size_t ws_bytes,
@@ -1313,9 +1367,9 @@ extern "C" int tq_groupby_run(const tq_groupby_desc_t* d, const tq_program_t* pr
     TQ_TRY(make_spec(d, prog, &G));
     const size_t lds_reduce = ((size_t)1 << w.part_bits) * (4 + 8 * (size_t)d->naggs);
     TQ_TRY(launch("k_gbp_reduce", k_gbp_reduce<false>, (unsigned)w.nparts, kBlock, lds_reduce, s,
-                  G, w.part_bits, w.nparts, w.ptiles, (const uint32_t*)w.hoff,
-                  (const uint32_t*)w.htotal, (const uint64_t*)w.rec, w.grp_cnt, w.gslot, w.gcnt,
-                  w.gacc, OutCols{}, LookBack{}));
+                  G, w.part_bits, w.nparts, w.ptiles, n, (const int64_t*)nullptr, w.tsub,
+                  (const uint32_t*)w.hoff, (const uint32_t*)w.htotal, (const uint64_t*)w.rec,
+                  w.grp_cnt, w.gslot, w.gcnt, w.gacc, OutCols{}, LookBack{}));
   } else {
     TQ_TRY(accumulate_impl(d, prog, cols, mask, n, mode, slots, w, s, nullptr, LookBack{}));
   }

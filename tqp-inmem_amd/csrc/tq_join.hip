@@ -1070,6 +1070,68 @@ extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdt
   return TQ_OK;
 }
 
+extern "C" int tq_join_probe_deferred(const tq_join_state_t* st, const void* pkeys,
+                                      int32_t pdtype, const uint8_t* pmask, int64_t np, void* pws,
+                                      int32_t nleft, const void* const* lsrc,
+                                      const int32_t* lbytes, void* const* ldst, int32_t nright,
+                                      const void* const* rsrc, const int32_t* rbytes,
+                                      void* const* rdst, const int64_t** total_dev,
+                                      const void** check_dev, int32_t* applied, void* stream) {
+  if (!st || !total_dev || !check_dev || !applied)
+    return set_error(TQ_EINVAL, "join probe (deferred): null argument");
+  if (pdtype < TQ_BOOL || pdtype > TQ_I64)
+    return set_error(TQ_EUNSUPPORTED, "join keys must be integer columns (dtype %d)", pdtype);
+  *applied = 0;
+  *total_dev = nullptr;
+  *check_dev = nullptr;
+  if (np <= 0 || !st->unique || st->layout == TQ_JOIN_EMPTY || st->nsel == 0) return TQ_OK;
+  ColPack L, R;
+  TQ_TRY(make_colpack(&L, nleft, lsrc, lbytes, nullptr, ldst));
+  TQ_TRY(make_colpack(&R, nright, rsrc, rbytes, nullptr, rdst));
+  hipStream_t s = (hipStream_t)stream;
+  ProbeWs w = carve_probe(pws, np);
+  const int64_t nt = ceil_div(np, kTileRows);
+  if (st->pending) {
+    // the same check words tq_join_count reads, left for the caller's read-back
+    BuildWs bw = carve_build(st->ws, st->nb);
+    const bool dense = st->layout == TQ_JOIN_DENSE;
+    if (dense || st->nsel < 0)
+      TQ_TRY(launch("k_join_check", k_join_check, 1, kBlock, 0, s,
+                    dense ? (const uint32_t*)bw.occ : (const uint32_t*)nullptr,
+                    st->nsel < 0 ? (const uint32_t*)bw.tile_counts : (const uint32_t*)nullptr,
+                    ceil_div(st->nb, kTileRows), (int64_t*)(bw.dup + 2)));
+    *check_dev = (const void*)bw.dup;
+  }
+  TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
+  TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
+  TQ_TRY(dispatch<WriteF>(st, st, pkeys, pdtype, pmask, np, nt, w, L, R, (int64_t*)nullptr,
+                          (int64_t*)nullptr, s));
+  *total_dev = w.total;
+  *applied = 1;
+  return TQ_OK;
+}
+
+extern "C" int tq_join_settle(tq_join_state_t* st, const int64_t* words, int32_t* ok) {
+  if (!st || !ok) return set_error(TQ_EINVAL, "join settle: null argument");
+  *ok = 1;
+  if (!st->pending) return TQ_OK;
+  if (!words) return set_error(TQ_EINVAL, "join settle: the build's check words are needed");
+  const bool ranged = st->nsel < 0;
+  const bool dense = st->layout == TQ_JOIN_DENSE;
+  const int64_t nsel = ranged ? words[2] : st->nsel;
+  const uint64_t w0 = (uint64_t)words[0];
+  const bool oob = ranged && (w0 >> 32) != 0;
+  const bool dup = nsel > 0 && (dense ? words[1] != nsel : (w0 & 0xFFFFFFFFull) != 0);
+  if (oob || dup) {
+    *ok = 0;  // the state stays pending: tq_join_count settles it and builds again
+    return TQ_OK;
+  }
+  st->nsel = nsel;
+  st->pending = 0;
+  if (nsel == 0) st->layout = TQ_JOIN_EMPTY;
+  return TQ_OK;
+}
+
 extern "C" int tq_join_write(const tq_join_state_t* st, const void* pkeys, int32_t pdtype,
                              const uint8_t* pmask, int64_t np, const void* pws, int32_t nleft,
                              const void* const* lsrc, const int32_t* lbytes, void* const* ldst,

@@ -28,6 +28,10 @@ void radix_bit_range(uint64_t key_or, uint64_t key_and, int* lo, int* hi);
 
 // Copy `bytes` (<= 512) of device memory to the host through pinned staging; synchronises.
 int read_back(const void* dev, size_t bytes, int64_t* host_out, hipStream_t s);
+// n copies with one synchronisation, packed into host_out at 8-byte-aligned offsets (at most
+// 512 bytes in all).
+int read_back_n(int n, const void* const* dev, const size_t* bytes, int64_t* host_out,
+                hipStream_t s);
 // Two such copies (each <= 256 bytes) with one synchronisation.
 int read_back2(const void* dev0, size_t bytes0, int64_t* host0, const void* dev1, size_t bytes1,
                int64_t* host1, hipStream_t s);

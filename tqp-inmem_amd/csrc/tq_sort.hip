@@ -234,6 +234,23 @@ int read_back2(const void* dev0, size_t bytes0, int64_t* host0, const void* dev1
   return TQ_OK;
 }
 
+int read_back_n(int n, const void* const* dev, const size_t* bytes, int64_t* host_out,
+                hipStream_t s) {
+  int64_t* h = host_scratch();
+  if (!h) return set_error(TQ_EHIP, "pinned staging allocation failed");
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) off += align_up(bytes[i], 8);
+  if (off > 512) return set_error(TQ_EINVAL, "read_back_n: %zu bytes", off);
+  off = 0;
+  for (int i = 0; i < n; ++i) {
+    TQ_TRY_HIP(hipMemcpyAsync((char*)h + off, dev[i], bytes[i], hipMemcpyDeviceToHost, s));
+    off += align_up(bytes[i], 8);
+  }
+  TQ_TRY_HIP(hipStreamSynchronize(s));
+  memcpy(host_out, h, off);
+  return TQ_OK;
+}
+
 int read_back(const void* dev, size_t bytes, int64_t* host_out, hipStream_t s) {
   int64_t* h = host_scratch();
   if (!h) return set_error(TQ_EHIP, "pinned staging allocation failed");
@@ -801,6 +818,13 @@ int sort_perm_u32(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, void* ws, 
 using namespace tq;
 
 extern "C" size_t tq_sort_ws_bytes(int64_t n) { return sort_ws_bytes(n); }
+
+extern "C" int tq_read_words(int32_t n, const void* const* dev, const size_t* bytes,
+                             int64_t* host_out, void* stream) {
+  if (n < 0 || (n > 0 && (!dev || !bytes || !host_out)))
+    return set_error(TQ_EINVAL, "read words: bad argument");
+  return read_back_n(n, dev, bytes, host_out, (hipStream_t)stream);
+}
 
 // Top-k for a limit of at most 256 over keys that pack into one 64-bit sort key, with no host
 // synchronisation: rounds of per-workgroup threshold selection for a limit of at most 64, else

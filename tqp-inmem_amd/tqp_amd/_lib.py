@@ -143,6 +143,11 @@ _SIGNATURES = {
     "tq_groupby_count": ([P, I32, I64, P, P, P], I32),
     "tq_groupby_write": ([P, I32, I64, P, I64, P, P, P], I32),
     "tq_groupby_run": ([P, P, P, P, I64, I32, I64, P, SZ, P, P, I64, P, P], I32),
+    "tq_groupby_run_deferred": ([P, P, P, P, I64, P, I32, I64, P, SZ, P, P, I64, P, P, P], I32),
+    "tq_join_probe_deferred": ([P, P, I32, P, I64, P, I32, P, P, P, I32, P, P, P, P, P, P, P],
+                               I32),
+    "tq_join_settle": ([P, P, P], I32),
+    "tq_read_words": ([I32, P, P, P, P], I32),
     "tq_batch_ws_bytes": ([I32], SZ),
     "tq_batch_run": ([P, I32, P, P, I32, I32, I32, I32, P, ctypes.c_uint32, P], I32),
 }
@@ -180,11 +185,33 @@ def call(name, *args, alg_bytes=None, kernel=None):
     else:
         c0 = lib.tq_timing_count()
         rc = fn(*args)
-        TRACE.append((name, kernel, alg_bytes, c0, lib.tq_timing_count()))
+        c1 = lib.tq_timing_count()
+        if isinstance(kernel, tuple):   # several principal kernels, bytes listed alike
+            for k, b in zip(kernel, alg_bytes):
+                TRACE.append((name, k, b, c0, c1))
+        else:
+            TRACE.append((name, kernel, alg_bytes, c0, c1))
     if rc != TQ_OK:
         msg = lib.tq_last_error().decode(errors="replace")
         raise _ERRORS.get(rc, RuntimeError)(f"{name}: {msg}")
     return rc
+
+
+def trace_len():
+    """Entries in the call log so far (None while not timing)."""
+    return None if TRACE is None else len(TRACE)
+
+
+def trace_set_bytes(at, kernel, alg_bytes):
+    """Set the algorithmic bytes of the logged entry for `kernel` at or after index `at` (for a
+    call whose bytes depend on a count read back later)."""
+    if TRACE is None or at is None:
+        return
+    for i in range(at, len(TRACE)):
+        e = TRACE[i]
+        if e[1] == kernel:
+            TRACE[i] = (e[0], e[1], alg_bytes, e[3], e[4])
+            return
 
 
 def start_trace():

@@ -6,6 +6,7 @@ Outputs are allocated here with torch's caching allocator, workspaces likewise; 
 launches kernels on the current stream.
 """
 import ctypes
+import threading
 import weakref
 
 import torch
@@ -191,6 +192,7 @@ class JoinIndex:
         ws_bytes = L.size_call("tq_join_build_ws_bytes", nb)
         self.ws = L.workspace(ws_bytes, bkey.device, "join_build")
         self.state = L.JoinState()
+        self.deferred = False   # pairs written by a deferred probe, not yet settled
         bm = L.ptr(bmask) if bmask is not None else None
         kb = known_bounds(self.bkey)
         if kb is not None:
@@ -208,10 +210,11 @@ class JoinIndex:
         return int(self.state.nsel)
 
     def key_bounds(self):
-        """(min, max) of the selected build keys when the build measured them (dense layout),
-        else None."""
+        """Bounds of every key that can have a pair: the dense directory's key range (the
+        selected build keys' measured range, or the bounds a ranged build was given), else
+        None."""
         st = self.state
-        if st.nsel <= 0 or st.layout != L.TQ_JOIN_DENSE:
+        if st.nsel == 0 or st.layout != L.TQ_JOIN_DENSE:
             return None
         return (_untransform(st.base, st.key_bits),
                 _untransform(st.base + st.range - 1, st.key_bits))
@@ -246,9 +249,30 @@ class JoinIndex:
                 seen.add(c.data_ptr())
                 reads += c.shape[0] * rb
         per_pair = sum(lbytes) + sum(rbytes) + (16 if want_index else 0)
+        count_bytes = np_ * (pkey.element_size() + (1 if pmask is not None else 0))
+        d = _deferred()
+        if d is not None and d.join is None and not want_index and np_ > 0:
+            # outputs for every probe row (a unique build gives a probe row one pair at most);
+            # the pair count stays on the device until the deferral resolves
+            lout = [torch.empty((np_,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev)
+                    for c in lsrc]
+            rout = [torch.empty((np_,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev)
+                    for c in rsrc]
+            tot, chk, applied = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int32()
+            at = L.trace_len()
+            L.call("tq_join_probe_deferred", ctypes.byref(self.state), L.ptr(pkey),
+                   L.dtype_code(pkey.dtype), pm, np_, L.ptr(pws), len(lsrc), lsp, lba,
+                   L.ptr_array(lout), len(rsrc), rsp, rba, L.ptr_array(rout), ctypes.byref(tot),
+                   ctypes.byref(chk), ctypes.byref(applied), s,
+                   alg_bytes=(count_bytes, 0), kernel=("k_join_count", "k_join_write"))
+            if applied.value:
+                self.deferred = True
+                d.join = {"index": self, "total": tot.value, "check": chk.value,
+                          "outs": lout + rout, "cap": np_, "ws": pws, "trace": at,
+                          "write_bytes": (reads, per_pair)}
+                return lout, rout, None, None
         L.call("tq_join_count", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype), pm,
-               np_, L.ptr(pws), ctypes.byref(total), s,
-               alg_bytes=np_ * (pkey.element_size() + (1 if pmask is not None else 0)),
+               np_, L.ptr(pws), ctypes.byref(total), s, alg_bytes=count_bytes,
                kernel="k_join_count")
         m = int(total.value)
         lout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in lsrc]
@@ -261,6 +285,86 @@ class JoinIndex:
                    L.ptr_array(rout), L.ptr(bidx), L.ptr(pidx), s, alg_bytes=reads + m * per_pair,
                    kernel="k_join_write")
         return lout, rout, bidx, pidx
+
+
+# ---------------------------------------------------------------- deferred row counts
+class DeferredJoinFailed(Exception):
+    """A deferred join's build keys were not unique (or left the bounds its build was given):
+    its pairs are void and the operators run again without deferral."""
+
+
+class Deferred:
+    """A unique-key join whose pairs feed a group-by, run with one synchronisation.
+
+    While a Deferred is active (`deferring`), JoinIndex.probe writes its pairs into outputs
+    sized for every probe row and leaves the pair count m on the device; a partitioned group-by
+    whose columns are those outputs reads m there and leaves its group count G on the device.
+    `resolve` then reads m, G and the build's uniqueness check back together; any other
+    consumer of the pairs calls `resolve_join` first, which reads m and the check."""
+
+    def __init__(self):
+        self.join = None
+        self.group = None
+        self.m = None
+        self.G = None
+
+    def _read(self, with_group):
+        j = self.join
+        ptrs, sizes = [j["total"]], [8]
+        if j["check"]:
+            ptrs.append(j["check"])
+            sizes.append(24)
+        if with_group:
+            ptrs += [self.group["ngroups"], self.group["err"]]
+            sizes += [8, 8]
+        host = (ctypes.c_int64 * 64)()
+        L.call("tq_read_words", len(ptrs), (ctypes.c_void_p * len(ptrs))(*ptrs),
+               (ctypes.c_size_t * len(ptrs))(*sizes), host, L.stream_handle())
+        m = int(host[0])
+        ok = ctypes.c_int32(1)
+        if j["check"]:
+            L.call("tq_join_settle", ctypes.byref(j["index"].state),
+                   (ctypes.c_int64 * 3)(host[1], host[2], host[3]), ctypes.byref(ok))
+        if not ok.value or m <= 0:
+            # repeated or out-of-bounds build keys void the pairs; no pairs at all take the
+            # plain path, which reproduces the reference's empty-side results
+            raise DeferredJoinFailed()
+        reads, per_pair = j["write_bytes"]
+        L.trace_set_bytes(j["trace"], "k_join_write", reads + m * per_pair)
+        j["index"].deferred = False
+        self.m = m
+        k = 4 if j["check"] else 1
+        return host, k
+
+    def resolve_join(self):
+        """Settle the pairs alone (for a consumer that needs m on the host)."""
+        if self.m is None:
+            self._read(False)
+        return self.m
+
+    def resolve(self):
+        """Read everything back; returns (m, G), G None when no group-by was deferred."""
+        if self.group is None:
+            return self.resolve_join(), None
+        host, k = self._read(True)
+        g, err = int(host[k]), int(host[k + 1])
+        if err:
+            raise DeferredJoinFailed()   # the fused reduce gave up: run the plain path
+        self.G = g
+        L.trace_set_bytes(self.group["trace"], "k_gb_accum", self.m * self.group["row_bytes"])
+        return self.m, g
+
+
+_TLS = threading.local()
+
+
+def set_deferred(df):
+    """Make `df` (a Deferred, or None) the calling thread's active deferral."""
+    _TLS.df = df
+
+
+def _deferred():
+    return getattr(_TLS, "df", None)
 
 
 # ---------------------------------------------------------------- group-by
@@ -321,6 +425,15 @@ class GroupBy:
             b.cols.width[0] = 1
         self.ops = [self.OPS[op] for op, _ in aggs]
 
+    def _reads_pairs(self, df):
+        """Whether every column this group-by reads is an output of the deferred join (sized
+        for the join's capacity, unmasked)."""
+        if self.mask is not None or self.n != df.join["cap"]:
+            return False
+        outs = {t.data_ptr() for t in df.join["outs"]}
+        cols = list(self.keys) + [t for t in self.builder.col_tensors if t is not getattr(self, "_dummy", None)]
+        return bool(cols) and all(t.data_ptr() in outs for t in cols)
+
     def run(self):
         """Returns (key columns, aggregate columns), groups in ascending key order."""
         b = self.builder
@@ -337,7 +450,16 @@ class GroupBy:
             d.agg_dtype[j] = L.dtype_code(dt)
         s = L.stream_handle()
         mptr = L.ptr(self.mask) if self.mask is not None else None
+        df = _deferred()
+        pend = df is not None and df.join is not None and df.m is None
+        if pend and not self._reads_pairs(df):
+            if self.n == df.join["cap"]:
+                raise DeferredJoinFailed()   # reads the pairs alongside other columns
+            pend = False
         known = [known_bounds(k) for k in self.keys] if self.mask is None else [None]
+        if pend and not (self.keys and all(x is not None for x in known)):
+            self.n = df.resolve_join()    # the bounds pass needs the real row count
+            return self.run()
         if self.keys and all(b is not None for b in known):
             # every row is selected and the key ranges are already known: no bounds pass
             hb = [v for b in known for v in b] + [self.n]
@@ -358,6 +480,11 @@ class GroupBy:
         L.call("tq_groupby_plan", ctypes.byref(d), nsel, b.nload, ctypes.byref(mode),
                ctypes.byref(slots), ctypes.byref(wsb))
         self.mode, self.slots = mode.value, slots.value
+        if pend and (self.mode & 7) != L.TQ_GB_DENSE_PART:
+            # only the partitioned mode reads its row count on the device: settle the pairs
+            # and plan again for the real count
+            self.n = df.resolve_join()
+            return self.run()
         ws = L.workspace(wsb.value, dev, "groupby")
         # every selected row's loaded columns (keys included) and the mask byte of every row
         alg = nsel * b.load_bytes() + (self.n if self.mask is not None else 0)
@@ -376,6 +503,16 @@ class GroupBy:
         cap = max(1, min(nsel, self.slots))
         key_full = [torch.empty(cap, dtype=k.dtype, device=dev) for k in self.keys]
         agg_full = [torch.empty(cap, dtype=odt, device=dev) for odt in out_dtypes]
+        if pend:
+            ngd, errd = ctypes.c_void_p(), ctypes.c_void_p()
+            at = L.trace_len()
+            L.call("tq_groupby_run_deferred", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b),
+                   None, self.n, ctypes.c_void_p(df.join["total"]), mode, slots, L.ptr(ws),
+                   wsb.value, L.ptr_array(key_full), L.ptr_array(agg_full), cap,
+                   ctypes.byref(ngd), ctypes.byref(errd), s, alg_bytes=0, kernel="k_gb_accum")
+            df.group = {"ngroups": ngd.value, "err": errd.value, "outs": key_full + agg_full,
+                        "cap": cap, "trace": at, "row_bytes": b.load_bytes(), "ws": ws}
+            return key_full, agg_full
         ng = ctypes.c_int64()
         L.call("tq_groupby_run", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr, self.n,
                mode, slots, L.ptr(ws), wsb.value, L.ptr_array(key_full), L.ptr_array(agg_full), cap,

@@ -129,6 +129,14 @@ def _parse_results(results):
     return calls, plans
 
 
+def results_are_direct(results):
+    """Whether every result is a grouping key or one aggregate call as it stands (no
+    expression combines aggregates, which would be evaluated over the group rows)."""
+    calls, plans = _parse_results(results)
+    ids = {id(c) for c in calls}
+    return all(kind == 'key' or id(_strip_alias(what)) in ids for _, kind, what in plans)
+
+
 def agg_group(columns, key_ids, results, mask=None):
     """Group `columns` by `key_ids` and evaluate each result tree (aggregate.py:282-364).
     Returns {output id: tensor}."""

@@ -93,7 +93,8 @@ def join_kernel_new(gpu_enable, tensors_left, tensors_right, left_key, right_key
     right_keys_out = [k for k in join_output_keys if k not in tensors_left]
 
     def left_nonempty():
-        return index.nsel > 0
+        # a deferred probe's build is settled later (its pairs are void if it selected nothing)
+        return index.nsel > 0 or index.deferred
 
     def right_nonempty():
         return nR > 0 and (pmask is None or kernels.count_true(pmask) > 0)
