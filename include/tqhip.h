@@ -210,6 +210,16 @@ size_t tq_sort_ws_bytes(int64_t n);
 int tq_sort_perm(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n, int64_t limit,
                  int64_t* perm_out, void* ws, size_t ws_bytes, void* stream);
 
+/* tq_sort_perm with a limit of 1..64 over rows whose count a producer earlier on the stream left
+ * in device memory: the rows are the first min(n, *n_dev) of the key columns, n being the
+ * capacity; perm_out holds min(limit, n) entries, of which the first min(limit, rows) are the
+ * permutation and the rest row 0.  Applies (*applied = 1) when the keys pack into 64 bits;
+ * otherwise nothing is launched.  Nothing is read back.  Same reference interface as
+ * tq_sort_perm (operators/sort.py:6-58). */
+int tq_sort_perm_deferred(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n,
+                          const int64_t* n_dev, int64_t limit, int64_t* perm_out, void* ws,
+                          size_t ws_bytes, int32_t* applied /* host out */, void* stream);
+
 /* Copy n small device words (bytes[i] each, 512 bytes in all at most) to host_out, packed at
  * 8-byte-aligned offsets, with one synchronisation of the stream.  Replaces the `.item()` /
  * `.cpu()` reads the reference makes of device scalars (e.g. operators/hashjoin.py:76). */

@@ -86,10 +86,34 @@ def test_deferred_join_aggregate_matches_oracle(device, monkeypatch, upto):
     assert seen.seen == ["held"], seen.seen
     cpu, _ = _oracle(tables, plan)
     _check(gpu, cpu, n, plan)
+    if upto == 5:
+        want = R.to_cols(tables)
+        R.execute(copy.deepcopy(plan[:4]), want)
+        assert n == want[101].tensor.shape[0]     # the sort's rows before its limit
     if upto == 4:
         # the pair columns the aggregate did not replace are cut to the pair count
         for k in (201, 202, 203):
             _assert_col(gpu[k].tensor, cpu[k].tensor, f"pair col {k}", fp=True)
+
+
+def test_deferred_top_rows_fewer_than_limit(device, monkeypatch):
+    """Five selected build rows: fewer groups than the top-k limit, so the deferred top-k's
+    permutation is cut to the group count."""
+    from tqp_amd import workloads
+    tables = workloads.config1_tables(n_build=200_000, n_probe=800_000)
+    seg = np.zeros(200_000, dtype=np.int8)
+    seg[[7, 1000, 50_000, 123_456, 199_999]] = 2
+    tables[102] = ('string', seg)
+    plan = workloads.config1_plan()
+    seen = _Outcomes(monkeypatch)
+    gpu, n = _run(tables, plan, device)
+    assert seen.seen == ["held"], seen.seen
+    cpu, _ = _oracle(tables, plan)
+    _check(gpu, cpu, n, plan)
+    assert gpu[101].tensor.shape[0] < 10
+    want = R.to_cols(tables)
+    R.execute(copy.deepcopy(plan[:4]), want)
+    assert n == want[101].tensor.shape[0]     # the sort's rows before its limit
 
 
 def test_deferred_same_as_per_operator_path(device):

@@ -550,12 +550,15 @@ __device__ __forceinline__ void field_keys(const FieldPack& fp, int64_t base, in
   }
 }
 
-// One round.  Source: the key fields of rows 0..m-1 (kin == null) or m (key, row) pairs.
-// Writes the workgroup's k smallest in order to kout/vout[blockIdx.x*k ..] (sentinels when it
-// has fewer), or, in the last round (perm_out != null, one workgroup), the rows of the first
-// nout as the permutation.
+// One round.  Source: the key fields of rows 0..m-1 (kin == null) or m (key, row) pairs; with
+// n_dev (key fields only) the rows are the first min(m, *n_dev), a count a producer earlier on
+// the stream left in device memory.  Writes the workgroup's k smallest in order to
+// kout/vout[blockIdx.x*k ..] (sentinels when it has fewer), or, in the last round (perm_out !=
+// null, one workgroup), the rows of the first nout as the permutation; permutation entries past
+// the candidates (only when n_dev cut the rows below nout) get row 0.
 __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint64_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin, int64_t m,
+                                                       const int64_t* __restrict__ n_dev,
                                                        int k, uint64_t* __restrict__ kout,
                                                        uint32_t* __restrict__ vout,
                                                        int64_t* __restrict__ perm_out, int64_t nout) {
@@ -566,6 +569,10 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
   __shared__ uint32_t tr, ns;
   const int t = threadIdx.x;
   const int64_t base = (int64_t)blockIdx.x * kListTile + t;
+  if (n_dev && !kin) {
+    const int64_t c = *n_dev;
+    m = c < m ? (c > 0 ? c : 0) : m;
+  }
   uint64_t key[kListPer];
   uint32_t row[kListPer];
   uint32_t valid = 0;
@@ -688,6 +695,8 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
       kout[(int64_t)blockIdx.x * k + i] = ~0ull;
       vout[(int64_t)blockIdx.x * k + i] = 0xFFFFFFFFu;
     }
+  } else {
+    for (int64_t i = s + t; i < nout; i += kBlock) perm_out[i] = 0;
   }
 }
 
@@ -830,7 +839,8 @@ extern "C" int tq_read_words(int32_t n, const void* const* dev, const size_t* by
 // synchronisation: rounds of per-workgroup threshold selection for a limit of at most 64, else
 // packing and rounds of per-tile radix selection.
 static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t limit,
-                     int64_t* perm_out, void* ws, size_t ws_bytes, hipStream_t s, bool* done) {
+                     int64_t* perm_out, void* ws, size_t ws_bytes, hipStream_t s, bool* done,
+                     const int64_t* n_dev = nullptr) {
   *done = false;
   FieldPack fp{};
   int bits = 0;
@@ -862,7 +872,8 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
       const int64_t blocks = ceil_div(m, kListTile);
       const bool last = blocks == 1;
       TQ_TRY(launch("k_topk_block", k_topk_block, (unsigned)blocks, kBlock, 0, s, fp, ki, vi, m,
-                    k, ko, vo, last ? perm_out : (int64_t*)nullptr, out));
+                    ki ? (const int64_t*)nullptr : n_dev, k, ko, vo,
+                    last ? perm_out : (int64_t*)nullptr, out));
       if (last) break;
       m = blocks * k;
       ki = ko;
@@ -873,6 +884,7 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
     *done = true;
     return TQ_OK;
   }
+  if (n_dev) return TQ_OK;  // a device row count is read by the small-limit rounds only
   TQ_TRY(launch("k_pack_keys", k_pack_keys<uint64_t>, stream_grid(n), kBlock, 0, s, fp, nullptr, n, w.k0, w.v0));
   uint64_t *ki = w.k0, *ko = w.k1;
   uint32_t *vi = nullptr, *vo = w.v1;
@@ -891,6 +903,20 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
     vo = vi == w.v0 ? w.v1 : w.v0;
   }
   *done = true;
+  return TQ_OK;
+}
+
+extern "C" int tq_sort_perm_deferred(const tq_sortkey_t* keys, int32_t nkeys, int64_t n,
+                                     const int64_t* n_dev, int64_t limit, int64_t* perm_out,
+                                     void* ws, size_t ws_bytes, int32_t* applied, void* stream) {
+  if (!applied || !n_dev) return set_error(TQ_EINVAL, "sort (deferred): null argument");
+  *applied = 0;
+  if (n <= 0 || limit < 1 || limit > kSmallK) return TQ_OK;
+  if (nkeys < 1 || !keys) return set_error(TQ_EINVAL, "sort: bad key list");
+  bool done = false;
+  TQ_TRY(topk_perm(keys, nkeys, n, limit, perm_out, ws, ws_bytes, (hipStream_t)stream, &done,
+                   n_dev));
+  *applied = done ? 1 : 0;
   return TQ_OK;
 }
 

@@ -13,7 +13,7 @@ import time
 import torch
 
 from . import kernels
-from .operators.aggregate import results_are_direct, tqp_hash_aggregate
+from .operators.aggregate import result_ids, results_are_direct, tqp_hash_aggregate
 from .operators.filter import tqp_filter
 from .operators.hashjoin import join_vortex
 from .operators.sort import tqp_sort
@@ -42,44 +42,67 @@ def _run_join(tensor_group, args, u, cpu_mem_pool, gpu_mem_pool, gpu_enable):
                        cpu_mem_pool=cpu_mem_pool, gpu_mem_pool=gpu_mem_pool, name=f"join {u}")
 
 
-def _pairs_feed_aggregate(plan, i):
-    """Whether plan[i] is an inner single-key Join without extra conditions whose pairs only
-    feed the Aggregate at plan[i + 1] (unmasked inputs, every result a key or one aggregate
-    call): the two can run with one synchronisation."""
+def _deferrable(plan, i):
+    """How many nodes from plan[i] run with one synchronisation: 2 when plan[i] is an inner
+    single-key Join without extra conditions whose pairs only feed the Aggregate at plan[i + 1]
+    (unmasked inputs, every result a key or one aggregate call), 3 when a 'take ordered' with a
+    limit of at most 64 over that aggregate's outputs follows; else 0."""
     if i + 1 >= len(plan):
-        return False
+        return 0
     ja, aa = plan[i][1], plan[i + 1][1]
     if ja['name'] != 'Join' or aa['name'] != 'Aggregate' or aa.get('is partial'):
-        return False
+        return 0
     if ja.get('join type') != 'Inner' or 'join condition' in ja or len(ja['left keys']) != 1:
-        return False
+        return 0
     ids = aa['input ids']
     if any(k in ids for k in aa.get('mask map', {})):
-        return False
-    return set(ids) <= set(ja['materialized in']) and results_are_direct(aa['results'])
+        return 0
+    if not (set(ids) <= set(ja['materialized in']) and results_are_direct(aa['results'])):
+        return 0
+    if i + 2 < len(plan):
+        sa = plan[i + 2][1]
+        outs = set(result_ids(aa['results']))
+        if sa['name'] == 'take ordered' and 1 <= sa.get('limit', 0) <= 64 \
+                and set(sa['input ids']) <= outs and set(sa['key ids']) <= outs:
+            return 3
+    return 2
 
 
-def _join_then_aggregate(plan, i, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu_enable,
-                         op_times):
-    """plan[i] (Join) and plan[i + 1] (Aggregate) with one synchronisation.  The join leaves its
-    pair count on the device and the group-by reads it there (kernels.Deferred); the counts and
-    the build's uniqueness check are read back together at the end, and the outputs sized for
-    the capacities are cut to the real counts.  When the check fails (repeated or out-of-bounds
-    build keys, no pairs, a fused reduce that gave up) both operators run again as usual, from
-    the state before the join."""
-    (uj, ja), (ua, aa) = plan[i], plan[i + 1]
+def _run_node(tensor_group, u, args, cpu_mem_pool, gpu_mem_pool, gpu_enable):
+    name = args['name']
+    if name == 'Join':
+        return _run_join(tensor_group, args, u, cpu_mem_pool, gpu_mem_pool, gpu_enable)
+    if name == 'Aggregate':
+        return tqp_hash_aggregate(gpu_enable, tensor_group, args, cpu_mem_pool=cpu_mem_pool,
+                                  gpu_mem_pool=gpu_mem_pool, name=f"aggregation {u}")
+    return tqp_sort(gpu_enable, tensor_group, args, with_limit=True, name=f"sort lim {u}")
+
+
+def _run_deferred(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu_enable,
+                  op_times):
+    """plan[i:i + count] (Join, Aggregate[, take ordered]) with one synchronisation.  The join
+    leaves its pair count on the device, the group-by reads it there and leaves its group count,
+    the top-k reads that (kernels.Deferred); the counts and the build's uniqueness check are read
+    back together at the end, and the outputs sized for the capacities are cut to the real
+    counts.  When the check fails (repeated or out-of-bounds build keys, no pairs, a fused reduce
+    that gave up) the nodes run again as usual, from the state before the join."""
+    nodes = plan[i:i + count]
     saved_group = dict(tensor_group)
+    ja = nodes[0][1]
     saved_mask_map = dict(ja['mask map']) if 'mask map' in ja else None
-    saved_ids = list(aa['input ids'])
+    saved_ids = list(nodes[1][1]['input ids'])
     df = kernels.Deferred()
-    t0 = time.perf_counter()
+    stages = []          # tensor_group after each node
+    times = []
+    rets = []
     kernels.set_deferred(df)
     try:
-        _run_join(tensor_group, ja, uj, cpu_mem_pool, gpu_mem_pool, gpu_enable)
-        after_join = dict(tensor_group)
-        t1 = time.perf_counter()
-        tqp_hash_aggregate(gpu_enable, tensor_group, aa, cpu_mem_pool=cpu_mem_pool,
-                           gpu_mem_pool=gpu_mem_pool, name=f"aggregation {ua}")
+        for u, args in nodes:
+            t0 = time.perf_counter()
+            rets.append(_run_node(tensor_group, u, args, cpu_mem_pool, gpu_mem_pool, gpu_enable))
+            stages.append(dict(tensor_group))
+            times.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
         m, g = df.resolve() if df.join is not None else (None, None)
         ok = True
     except kernels.DeferredJoinFailed:
@@ -88,30 +111,37 @@ def _join_then_aggregate(plan, i, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu_
         kernels.set_deferred(None)
     if ok:
         if m is not None:
-            _cut_to_counts(tensor_group, saved_group, after_join, df, m, g)
-        t2 = time.perf_counter()
+            _cut_to_counts(tensor_group, saved_group, stages, df, m, g)
+        times[-1] += time.perf_counter() - t0
+        if count == 3:
+            # the sort's row count before its limit: the group count, which the sort saw as
+            # a capacity when the group-by was deferred
+            num_rows = g if g is not None else rets[-1]
+        else:
+            num_rows = next(iter(tensor_group.values())).tensor.size(0)
     else:
         tensor_group.clear()
         tensor_group.update(saved_group)
         if saved_mask_map is not None:
             ja['mask map'] = saved_mask_map
-        aa['input ids'][:] = saved_ids
-        t0 = time.perf_counter()
-        _run_join(tensor_group, ja, uj, cpu_mem_pool, gpu_mem_pool, gpu_enable)
-        t1 = time.perf_counter()
-        tqp_hash_aggregate(gpu_enable, tensor_group, aa, cpu_mem_pool=cpu_mem_pool,
-                           gpu_mem_pool=gpu_mem_pool, name=f"aggregation {ua}")
-        t2 = time.perf_counter()
+        nodes[1][1]['input ids'][:] = saved_ids
+        times = []
+        for u, args in nodes:
+            t0 = time.perf_counter()
+            num_rows = _run_node(tensor_group, u, args, cpu_mem_pool, gpu_mem_pool, gpu_enable)
+            times.append(time.perf_counter() - t0)
     if op_times is not None:
-        op_times.setdefault(f"Join {uj}", []).append(t1 - t0)
-        op_times.setdefault(f"Aggregate {ua}", []).append(t2 - t1)
-    return next(iter(tensor_group.values())).tensor.size(0)
+        for (u, args), dt in zip(nodes, times):
+            op_times.setdefault(f"{args['name']} {u}", []).append(dt)
+    return num_rows
 
 
-def _cut_to_counts(tensor_group, before, after_join, df, m, g):
-    """Replace every column the two operators left at a capacity size by its first m (pairs)
-    or g (groups) rows."""
+def _cut_to_counts(tensor_group, before, stages, df, m, g):
+    """Replace every column the deferred nodes left at a capacity size by its real rows: the
+    first m (pairs), g (groups) or min(g, limit) (top rows)."""
     def cut(v, n):
+        if n == v.tensor.shape[0]:
+            return
         t = v.tensor[:n]
         kb = kernels.known_bounds(v.tensor)
         if kb is not None:
@@ -124,11 +154,14 @@ def _cut_to_counts(tensor_group, before, after_join, df, m, g):
         if before.get(k) is v:
             continue
         t = v.tensor
-        if after_join.get(k) is v:
+        if stages[0].get(k) is v:            # written by the join
             if t.data_ptr() in join_outs and t.shape[0] == df.join["cap"]:
                 cut(v, m)
-        elif g is not None and t.data_ptr() in group_outs and t.shape[0] == df.group["cap"]:
-            cut(v, g)
+        elif stages[1].get(k) is v:          # written by the aggregate
+            if g is not None and t.data_ptr() in group_outs and t.shape[0] == df.group["cap"]:
+                cut(v, g)
+        elif df.sorted is not None and g is not None and t.shape[0] == df.sorted["cap"]:
+            cut(v, min(g, df.sorted["limit"]))   # written by the top-k
 
 
 def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem_pool=None,
@@ -143,10 +176,11 @@ def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem
     while i < len(plan):
         u, args = plan[i]
         name = args['name']
-        if not sync and gpu_enable and _pairs_feed_aggregate(plan, i):
-            num_rows = _join_then_aggregate(plan, i, tensor_group, cpu_mem_pool, gpu_mem_pool,
-                                            gpu_enable, op_times)
-            i += 2
+        count = _deferrable(plan, i) if not sync and gpu_enable else 0
+        if count:
+            num_rows = _run_deferred(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool,
+                                     gpu_enable, op_times)
+            i += count
             continue
         i += 1
         t0 = time.perf_counter()

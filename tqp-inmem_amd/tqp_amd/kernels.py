@@ -126,7 +126,22 @@ def eval_program(b: E.ProgramBuilder, n: int, out_dtype, device):
 # ---------------------------------------------------------------- sort
 def sort_perm(keys, n: int, limit=None, device=None):
     """Stable multi-key sort permutation (int64[min(n, limit)]).  keys: list of
-    (tensor, descending) from most to least significant; 2-D int8 keys compare row-wise."""
+    (tensor, descending) from most to least significant; 2-D int8 keys compare row-wise.
+
+    Under a deferral whose group-by left its group count on the device, a sort of those groups
+    with a limit of at most 64 reads the count there too (the permutation then has
+    min(n, limit) entries, of which the first min(groups, limit) are valid); any other sort of
+    them reads the count back first and sorts that many rows."""
+    df = _deferred()
+    pending = (df is not None and df.group is not None and df.G is None
+               and n == df.group["cap"])
+    if pending:
+        outs = {t.data_ptr() for t in df.group["outs"]}
+        if not all(t.data_ptr() in outs for t, _ in keys):
+            pending = False
+        elif limit is None or limit < 1 or limit > 64:
+            n = df.resolve()[1]
+            pending = False
     m = n if limit is None or limit < 0 or limit > n else int(limit)
     dev = device if device is not None else keys[0][0].device
     perm = torch.empty(m, dtype=torch.int64, device=dev)
@@ -146,6 +161,15 @@ def sort_perm(keys, n: int, limit=None, device=None):
         arr[i].descending = 1 if desc else 0
     ws_bytes = L.size_call("tq_sort_ws_bytes", n)
     ws = L.workspace(ws_bytes, dev, "sort")
+    if pending:
+        applied = ctypes.c_int32()
+        L.call("tq_sort_perm_deferred", arr, len(keys), n, ctypes.c_void_p(df.group["ngroups"]),
+               int(limit), L.ptr(perm), L.ptr(ws), ws_bytes, ctypes.byref(applied),
+               L.stream_handle())
+        if applied.value:
+            df.sorted = {"cap": m, "limit": int(limit)}
+            return perm
+        return sort_perm(keys, df.resolve()[1], limit, device)
     L.call("tq_sort_perm", arr, len(keys), n, -1 if limit is None else int(limit), L.ptr(perm),
            L.ptr(ws), ws_bytes, L.stream_handle())
     return perm
@@ -305,6 +329,7 @@ class Deferred:
     def __init__(self):
         self.join = None
         self.group = None
+        self.sorted = None
         self.m = None
         self.G = None
 
@@ -343,9 +368,12 @@ class Deferred:
         return self.m
 
     def resolve(self):
-        """Read everything back; returns (m, G), G None when no group-by was deferred."""
+        """Read everything back (once); returns (m, G), G None when no group-by was
+        deferred."""
         if self.group is None:
             return self.resolve_join(), None
+        if self.G is not None:
+            return self.m, self.G
         host, k = self._read(True)
         g, err = int(host[k]), int(host[k + 1])
         if err:

@@ -137,6 +137,11 @@ def results_are_direct(results):
     return all(kind == 'key' or id(_strip_alias(what)) in ids for _, kind, what in plans)
 
 
+def result_ids(results):
+    """Output ids of a result list, keys and aggregates, in order."""
+    return [out_id for out_id, _, _ in _parse_results(results)[1]]
+
+
 def agg_group(columns, key_ids, results, mask=None):
     """Group `columns` by `key_ids` and evaluate each result tree (aggregate.py:282-364).
     Returns {output id: tensor}."""
