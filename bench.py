@@ -74,6 +74,38 @@ def kernel_table(calls, launches):
     return out
 
 
+JOIN_AGG_ENTRIES = {"tq_join_build", "tq_join_build_ranged", "tq_join_count", "tq_join_write",
+                    "tq_join_probe_deferred", "tq_groupby_bounds", "tq_groupby_run",
+                    "tq_groupby_run_deferred", "tq_groupby_accumulate", "tq_groupby_count",
+                    "tq_groupby_write"}
+
+
+def join_aggregate_span(calls, launches, steps, tables, pairs, groups):
+    """The config-1 join + aggregate against the HBM roofline: the two operators' boundary bytes
+    (every input column they read once, every output written once) over the summed device
+    durations of the kernels their entry points launched."""
+    seen = set()
+    ms = 0.0
+    for entry, _kern, _b, c0, c1 in calls:
+        if entry in JOIN_AGG_ENTRIES and (c0, c1) not in seen:
+            seen.add((c0, c1))
+            ms += sum(launches[i][1] for i in range(c0, c1))
+    nb, np_ = tables[101][1].shape[0], tables[201][1].shape[0]
+    join_bytes = (nb * (4 + 1)            # build key + build mask
+                  + np_ * (4 + 1)         # probe key + probe mask
+                  + np_ * (8 + 8)         # price, discount gathered from the probe table
+                  + pairs * (4 + 4 + 8 + 8))   # key, fk, price, discount written per pair
+    agg_bytes = pairs * (4 + 8 + 8) + groups * (4 + 8)   # key, price, discount in; key, sum out
+    total = join_bytes + agg_bytes
+    us = ms / steps * 1e3
+    gbs = total / (us * 1e-6) / 1e9
+    return {"boundary_bytes": total, "join_bytes": join_bytes, "aggregate_bytes": agg_bytes,
+            "pairs": pairs, "groups": groups, "kernel_us_per_step": round(us, 2),
+            "launches_per_step": round(sum(c1 - c0 for c0, c1 in seen) / steps, 2),
+            "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "target_frac": 0.5}
+
+
 def cpu_baseline(tables, plan, runs):
     import torch
     from oracle import ref_ops as R
@@ -247,6 +279,13 @@ def main():
     dt_traced = time.perf_counter() - t1
     calls, launches = _lib.stop_trace()
     kernels = kernel_table(calls, launches)
+    # pair and group counts of one step, for the join + aggregate boundary bytes
+    ja_plan = copy.deepcopy(plan[:4])
+    ja_group = dict(base)
+    executor.execute(ja_plan, ja_group, sync=True)
+    pairs = int(ja_group[201].tensor.shape[0])
+    groups = int(ja_group[301].tensor.shape[0])
+    ja = join_aggregate_span(calls, launches, args.steps, tables, pairs, groups)
     with_bytes = {k: v for k, v in kernels.items() if "GBps" in v}
     dom = max(with_bytes, key=lambda k: with_bytes[k]["total_ms"])
     d = with_bytes[dom]
@@ -282,6 +321,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": d["frac"], "traffic": traffic,
                      "alg_bytes_per_launch": d["alg_bytes_per_launch"], "avg_us": d["avg_us"]},
+        "join_aggregate": ja,
         "ms_per_step_kernel_timed": side["kernel_timed"],
         "ms_per_step_op_sync": side["op_sync"],
         "kernels": kernels,

@@ -321,8 +321,24 @@ __device__ __forceinline__ void write_null(void* dst, int64_t di, int bytes, int
 int fill_u32(void* p, uint32_t v, int64_t count, hipStream_t s);
 int fill_u64(void* p, uint64_t v, int64_t count, hipStream_t s);
 unsigned stream_grid(int64_t m);
+// Length of a scan whose element count a producer earlier on the stream left in device memory:
+// with n_dev, the elements are the first mult * ceil(min(*n_dev, cap) / per) of the n the launch
+// was sized for (a histogram of per-tile counts whose tile count follows a device row count).
+struct ScanLen {
+  const int64_t* n_dev = nullptr;
+  int64_t cap = 0, per = 1, mult = 1;
+  __device__ __forceinline__ int64_t of(int64_t n) const {
+    if (!n_dev) return n;
+    int64_t r = *n_dev;
+    r = r < 0 ? 0 : (r < cap ? r : cap);
+    const int64_t e = mult * ((r + per - 1) / per);
+    return e < n ? e : n;
+  }
+};
+
 template <typename TIn, typename TOut>
-int scan_exclusive(const TIn* in, int64_t n, TOut* out, TOut* total, TOut* ws, hipStream_t stream);
+int scan_exclusive(const TIn* in, int64_t n, TOut* out, TOut* total, TOut* ws, hipStream_t stream,
+                   ScanLen len = ScanLen{});
 
 // Fill a host-side ColPack from caller arrays; returns TQ_EINVAL text on bad widths.
 int make_colpack(ColPack* cp, int32_t ncols, const void* const* src, const int32_t* bytes,

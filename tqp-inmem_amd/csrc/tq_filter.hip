@@ -349,8 +349,10 @@ __device__ __forceinline__ void store16(TOut* out, int64_t b0, int64_t n, const 
 
 template <typename TOut>
 __global__ __launch_bounds__(kBlock) void k_chunk_sums(const uint32_t* __restrict__ in, int64_t n,
-                                                       TOut* __restrict__ sums) {
+                                                       TOut* __restrict__ sums, ScanLen len) {
   __shared__ TOut sh[kWaves];
+  n = len.of(n);
+  if ((int64_t)blockIdx.x * kScanChunk >= n) return;  // past a device length: never read
   const int64_t b0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kItems;
   uint32_t v[kItems];
   load16_u32(in, b0, n, v);
@@ -375,8 +377,14 @@ template <typename TOut>
 __global__ __launch_bounds__(kBlock) void k_scan_chunk(const uint32_t* __restrict__ in, int64_t n,
                                                        TOut* __restrict__ out,
                                                        const TOut* __restrict__ chunk_sums,
-                                                       TOut* __restrict__ total) {
+                                                       TOut* __restrict__ total, ScanLen len) {
   __shared__ TOut sh[kWaves];
+  n = len.of(n);
+  if ((int64_t)blockIdx.x * kScanChunk >= n) {  // past a device length: never read
+    if (n == 0 && total && blockIdx.x == 0 && threadIdx.x == 0) *total = 0;
+    return;
+  }
+  const int64_t last = (n - 1) / kScanChunk;  // the last chunk with elements
   const int64_t b0 = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kItems;
   TOut prior = 0;
   if (chunk_sums) {  // at most kScanChunk chunks: 16 sums per thread
@@ -400,12 +408,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_chunk(const uint32_t* __restric
     run += (TOut)v[j];
   }
   store16(out, b0, n, o);
-  if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = prior + tot;
+  if (total && (int64_t)blockIdx.x == last && threadIdx.x == 0) *total = prior + tot;
 }
 
 template <typename TIn, typename TOut>
 int scan_exclusive(const TIn* in, int64_t n, TOut* out, TOut* total, TOut* ws,
-                   hipStream_t stream) {
+                   hipStream_t stream, ScanLen len) {
   const int64_t nchunks = ceil_div(n, kScanChunk);
   if (n <= 0) {
     if (total) return sizeof(TOut) == 8 ? fill_u64(total, 0, 1, stream) : fill_u32(total, 0, 1, stream);
@@ -414,19 +422,19 @@ int scan_exclusive(const TIn* in, int64_t n, TOut* out, TOut* total, TOut* ws,
   static_assert(sizeof(TIn) == 4, "scans read u32 counts");
   if (nchunks == 1) {
     return launch("k_scan_chunk", (k_scan_chunk<TOut>), 1, kBlock, 0, stream, (const uint32_t*)in,
-                  n, out, (const TOut*)nullptr, total);
+                  n, out, (const TOut*)nullptr, total, len);
   }
   if (nchunks > kScanChunk) return set_error(TQ_EINVAL, "scan of %lld elements too large", (long long)n);
   TQ_TRY(launch("k_chunk_sums", (k_chunk_sums<TOut>), (unsigned)nchunks, kBlock, 0, stream,
-                (const uint32_t*)in, n, ws));
+                (const uint32_t*)in, n, ws, len));
   return launch("k_scan_chunk", (k_scan_chunk<TOut>), (unsigned)nchunks, kBlock, 0, stream,
-                (const uint32_t*)in, n, out, (const TOut*)ws, total);
+                (const uint32_t*)in, n, out, (const TOut*)ws, total, len);
 }
 
 template int scan_exclusive<uint32_t, uint32_t>(const uint32_t*, int64_t, uint32_t*, uint32_t*,
-                                                uint32_t*, hipStream_t);
+                                                uint32_t*, hipStream_t, ScanLen);
 template int scan_exclusive<uint32_t, int64_t>(const uint32_t*, int64_t, int64_t*, int64_t*,
-                                               int64_t*, hipStream_t);
+                                               int64_t*, hipStream_t, ScanLen);
 
 // ------------------------------------------------------------------ compaction
 __global__ __launch_bounds__(kBlock) void k_compact(const uint8_t* __restrict__ mask, int64_t n,

@@ -639,10 +639,10 @@ __global__ __launch_bounds__(kBlock) void k_gbp_reduce(
   }
   __syncthreads();
   // partition b's records: from its first histogram entry to the next partition's (the grand
-  // total after the last partition of a full-capacity histogram)
+  // total after the last partition); no rows, no records
   const int64_t pt = part_stride(part_rows(n, n_dev), n_dev, tsub, ptiles);
-  const uint32_t r0 = hoff[b * pt];
-  const uint32_t r1 = (b + 1) * pt < (int64_t)nparts * ptiles ? hoff[(b + 1) * pt] : *htotal;
+  const uint32_t r0 = pt > 0 ? hoff[b * pt] : 0u;
+  const uint32_t r1 = pt > 0 ? (b + 1 < nparts ? hoff[(b + 1) * pt] : *htotal) : 0u;
   constexpr int U = 8;  // records per thread whose loads are issued together
   const int rw = 1 + G.naggs;
   for (uint32_t base = r0 + t; base < r1; base += U * kBlock) {
@@ -1202,7 +1202,10 @@ static int accumulate_impl(const tq_groupby_desc_t* d, const tq_program_t* prog,
     TQ_TRY(launch("k_gbp_hist", k_gbp_hist, (unsigned)w.ptiles, kBlock, (size_t)np * 4, s, K, G,
                   mask, n, n_dev, w.part_bits, np, w.tsub, w.ptiles, w.hist,
                   fused ? w.flags : (uint64_t*)nullptr));
-    TQ_TRY((scan_exclusive<uint32_t, uint32_t>(w.hist, nh, w.hoff, w.htotal, w.hscan_ws, s)));
+    // with a device row count, only the histogram's first np * (tiles holding rows) entries
+    ScanLen hl{};
+    if (n_dev) hl = ScanLen{n_dev, n, (int64_t)w.tsub * kSubRows, (int64_t)np};
+    TQ_TRY((scan_exclusive<uint32_t, uint32_t>(w.hist, nh, w.hoff, w.htotal, w.hscan_ws, s, hl)));
     TQ_TRY(launch("k_gbp_scatter", k_gbp_scatter, (unsigned)w.ptiles, kBlock, lds_scatter, s, G,
                   *prog, *cols, mask, n, n_dev, w.part_bits, np, w.tsub, w.ptiles,
                   (const uint32_t*)w.hoff, w.rec, stage_words));

@@ -358,16 +358,24 @@ __global__ __launch_bounds__(kBlock) void k_join_bits(const void* __restrict__ t
 
 // The uniqueness check's numbers next to the duplicate flag (one workgroup): the occupied-entry
 // sum (dense directories) and, after a ranged build, the selected rows (sum of tile counts).
-__global__ __launch_bounds__(kBlock) void k_join_check(const uint32_t* __restrict__ occ,
-                                                       const uint32_t* __restrict__ tile_counts,
-                                                       int64_t ntiles, int64_t* __restrict__ out) {
+// The build's uniqueness check, summed where the first probe reads it: the spread occupied-entry
+// counters (dense layouts) into out[0] and, for a ranged build, the per-tile selected counts into
+// out[1].  Run by one workgroup: its own kernel, or the first workgroup of the count pass.
+struct BuildCheck {
+  const uint32_t* occ;          // null: no occupied-entry counters
+  const uint32_t* tile_counts;  // null: the build counted its rows already
+  int64_t ntiles;
+  int64_t* out;                 // null: nothing to check
+};
+
+__device__ __forceinline__ void build_check(const BuildCheck& B) {
   __shared__ int64_t sh[kWaves];
-  int64_t v = (occ && threadIdx.x < kOccSlots) ? occ[threadIdx.x * kOccStride] : 0;
+  int64_t v = (B.occ && threadIdx.x < kOccSlots) ? B.occ[threadIdx.x * kOccStride] : 0;
   v = wave_sum(v);  // the counters are in the first wave
-  if (threadIdx.x == 0) out[0] = v;
-  if (!tile_counts) return;
+  if (threadIdx.x == 0) B.out[0] = v;
+  if (!B.tile_counts) return;
   int64_t c = 0;
-  for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) c += tile_counts[i];
+  for (int64_t i = threadIdx.x; i < B.ntiles; i += kBlock) c += B.tile_counts[i];
   c = wave_sum(c);
   if (lane_id() == 0) sh[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -375,9 +383,11 @@ __global__ __launch_bounds__(kBlock) void k_join_check(const uint32_t* __restric
     int64_t t = 0;
 #pragma unroll
     for (int i = 0; i < kWaves; ++i) t += sh[i];
-    out[1] = t;
+    B.out[1] = t;
   }
 }
+
+__global__ __launch_bounds__(kBlock) void k_join_check(BuildCheck B) { build_check(B); }
 
 // ------------------------------------------------------------------ build: duplicate keys
 // Compact the selected (key, row) pairs in row order.
@@ -472,8 +482,13 @@ __global__ __launch_bounds__(kBlock) void k_join_count(JoinView J, const void* _
                                                        int pdt, const uint8_t* __restrict__ pmask,
                                                        int64_t np, int64_t ntiles,
                                                        uint32_t* __restrict__ tile_counts,
-                                                       uint16_t* __restrict__ hits) {
+                                                       uint16_t* __restrict__ hits,
+                                                       BuildCheck check) {
   __shared__ uint32_t sh[kWaves];
+  if (check.out && blockIdx.x == 0) {  // a pending build's check rides on the first workgroup
+    build_check(check);
+    __syncthreads();
+  }
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t c[kItems], a[kItems];
     probe_rows<K, HASH, UNIQUE, false>(J, pkeys, pdt, pmask, np, tile, c, a);
@@ -901,10 +916,26 @@ static int rebuild_from_stats(tq_join_state_t* st, BuildWs& w, bool repeated, hi
 
 template <typename K, bool HASH, bool UNIQUE>
 static int count_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
-                      const uint8_t* pmask, int64_t np, int64_t nt, ProbeWs& w, hipStream_t s) {
+                      const uint8_t* pmask, int64_t np, int64_t nt, ProbeWs& w,
+                      const BuildCheck& check, hipStream_t s) {
   return launch("k_join_count", (k_join_count<K, HASH, UNIQUE>), probe_grid(nt), kBlock, 0, s,
                 view_of(st), pkeys, pdt, pmask, np, nt, w.tile_counts,
-                UNIQUE ? w.hits : (uint16_t*)nullptr);
+                UNIQUE ? w.hits : (uint16_t*)nullptr, check);
+}
+
+// The check words of a pending build (dup u32, out-of-bounds u32, occupied entries i64,
+// selected rows i64 for a ranged build), summed by the workgroup that runs the check.
+static BuildCheck pending_check(const tq_join_state_t* st) {
+  BuildCheck c{};
+  if (!st->pending) return c;
+  BuildWs bw = carve_build(st->ws, st->nb);
+  const bool dense = st->layout == TQ_JOIN_DENSE;
+  if (!dense && st->nsel >= 0) return c;  // hash, not ranged: the duplicate flag says it all
+  c.occ = dense ? (const uint32_t*)bw.occ : nullptr;
+  c.tile_counts = st->nsel < 0 ? (const uint32_t*)bw.tile_counts : nullptr;
+  c.ntiles = ceil_div(st->nb, kTileRows);
+  c.out = (int64_t*)(bw.dup + 2);
+  return c;
 }
 
 template <typename K, bool HASH, bool UNIQUE>
@@ -1011,22 +1042,16 @@ extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdt
     // settle the build's uniqueness with the same synchronisation as this count
     BuildWs bw = carve_build(st->ws, st->nb);
     const bool dense = st->layout == TQ_JOIN_DENSE;
-    if (dense || st->nsel < 0) {
-      // the spread occupied-entry counters and (ranged build) the per-tile selected counts,
-      // summed into the two words after the duplicate flag
-      TQ_TRY(launch("k_join_check", k_join_check, 1, kBlock, 0, s,
-                    dense ? (const uint32_t*)bw.occ : (const uint32_t*)nullptr,
-                    st->nsel < 0 ? (const uint32_t*)bw.tile_counts : (const uint32_t*)nullptr,
-                    ceil_div(st->nb, kTileRows), (int64_t*)(bw.dup + 2)));
-    }
+    const BuildCheck check = pending_check(st);
     // dup (u32), out-of-bounds (u32), occupied-entry sum (i64), selected rows (i64, ranged)
     const void* flag = (const void*)bw.dup;
     int64_t total = 0, fl[3] = {};
     if (np > 0) {
-      TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
+      TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, check, s));
       TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
       TQ_TRY(read_back2(w.total, 8, &total, flag, 24, fl, s));
     } else {
+      if (check.out) TQ_TRY(launch("k_join_check", k_join_check, 1, kBlock, 0, s, check));
       TQ_TRY(read_back(flag, 24, fl, s));
     }
     st->pending = 0;
@@ -1064,7 +1089,7 @@ extern "C" int tq_join_count(tq_join_state_t* st, const void* pkeys, int32_t pdt
     if (total_host) *total_host = 0;
     return TQ_OK;
   }
-  TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
+  TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, BuildCheck{}, s));
   TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
   if (total_host) TQ_TRY(read_back(w.total, 8, total_host, s));
   return TQ_OK;
@@ -1091,18 +1116,9 @@ extern "C" int tq_join_probe_deferred(const tq_join_state_t* st, const void* pke
   hipStream_t s = (hipStream_t)stream;
   ProbeWs w = carve_probe(pws, np);
   const int64_t nt = ceil_div(np, kTileRows);
-  if (st->pending) {
-    // the same check words tq_join_count reads, left for the caller's read-back
-    BuildWs bw = carve_build(st->ws, st->nb);
-    const bool dense = st->layout == TQ_JOIN_DENSE;
-    if (dense || st->nsel < 0)
-      TQ_TRY(launch("k_join_check", k_join_check, 1, kBlock, 0, s,
-                    dense ? (const uint32_t*)bw.occ : (const uint32_t*)nullptr,
-                    st->nsel < 0 ? (const uint32_t*)bw.tile_counts : (const uint32_t*)nullptr,
-                    ceil_div(st->nb, kTileRows), (int64_t*)(bw.dup + 2)));
-    *check_dev = (const void*)bw.dup;
-  }
-  TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, s));
+  if (st->pending)  // the same check words tq_join_count reads, left for the caller's read-back
+    *check_dev = (const void*)carve_build(st->ws, st->nb).dup;
+  TQ_TRY(dispatch<CountF>(st, st, pkeys, pdtype, pmask, np, nt, w, pending_check(st), s));
   TQ_TRY(tq_scan_tiles(w.tile_counts, nt, w.tile_off, w.total, w.scan_ws, s));
   TQ_TRY(dispatch<WriteF>(st, st, pkeys, pdtype, pmask, np, nt, w, L, R, (int64_t*)nullptr,
                           (int64_t*)nullptr, s));
