@@ -223,24 +223,74 @@ int64_t* host_scratch() {
 
 int read_back2(const void* dev0, size_t bytes0, int64_t* host0, const void* dev1, size_t bytes1,
                int64_t* host1, hipStream_t s) {
-  int64_t* h = host_scratch();
-  if (!h) return set_error(TQ_EHIP, "pinned staging allocation failed");
   if (bytes0 > 256 || bytes1 > 256) return set_error(TQ_EINVAL, "read_back2: copy too large");
-  TQ_TRY_HIP(hipMemcpyAsync(h, dev0, bytes0, hipMemcpyDeviceToHost, s));
-  TQ_TRY_HIP(hipMemcpyAsync(h + 32, dev1, bytes1, hipMemcpyDeviceToHost, s));
-  TQ_TRY_HIP(hipStreamSynchronize(s));
+  const void* dev[2] = {dev0, dev1};
+  const size_t bytes[2] = {bytes0, bytes1};
+  int64_t h[64];
+  TQ_TRY(read_back_n(2, dev, bytes, h, s));
   memcpy(host0, h, bytes0);
-  memcpy(host1, h + 32, bytes1);
+  memcpy(host1, (const char*)h + align_up(bytes0, 8), bytes1);
   return TQ_OK;
+}
+
+// Up to 8 runs of 8-byte words gathered by one small kernel straight into host memory.
+constexpr int kMaxWordRuns = 8;
+struct WordRuns {
+  const uint64_t* src[kMaxWordRuns];
+  int32_t nw[kMaxWordRuns];
+  int32_t off[kMaxWordRuns];
+  int32_t n;
+};
+
+__global__ __launch_bounds__(64) void k_collect_words(WordRuns W, uint64_t* __restrict__ dst) {
+  for (int p = 0; p < W.n; ++p)
+    for (int w = threadIdx.x; w < W.nw[p]; w += 64) dst[W.off[p] + w] = W.src[p][w];
+}
+
+// Pinned, coherent host words that kernels write directly (device view in *dev).
+static int64_t* host_words(uint64_t** dev) {
+  static thread_local int64_t* p = nullptr;
+  static thread_local uint64_t* d = nullptr;
+  if (!p) {
+    if (hipHostMalloc((void**)&p, 64 * sizeof(int64_t), hipHostMallocCoherent) != hipSuccess) {
+      p = nullptr;
+      return nullptr;
+    }
+    if (hipHostGetDevicePointer((void**)&d, p, 0) != hipSuccess) d = nullptr;
+  }
+  *dev = d;
+  return d ? p : nullptr;
 }
 
 int read_back_n(int n, const void* const* dev, const size_t* bytes, int64_t* host_out,
                 hipStream_t s) {
+  size_t off = 0;
+  bool words = n <= kMaxWordRuns;
+  for (int i = 0; i < n; ++i) {
+    off += align_up(bytes[i], 8);
+    words = words && bytes[i] % 8 == 0 && ((uintptr_t)dev[i] & 7) == 0;
+  }
+  if (off > 512) return set_error(TQ_EINVAL, "read_back_n: %zu bytes", off);
+  uint64_t* hd = nullptr;
+  int64_t* hw = words ? host_words(&hd) : nullptr;
+  if (hw) {
+    // one launch instead of one copy per run
+    WordRuns W{};
+    W.n = n;
+    int32_t o = 0;
+    for (int i = 0; i < n; ++i) {
+      W.src[i] = (const uint64_t*)dev[i];
+      W.nw[i] = (int32_t)(bytes[i] / 8);
+      W.off[i] = o;
+      o += W.nw[i];
+    }
+    TQ_TRY(launch("k_collect_words", k_collect_words, 1, 64, 0, s, W, hd));
+    TQ_TRY_HIP(hipStreamSynchronize(s));
+    memcpy(host_out, hw, off);
+    return TQ_OK;
+  }
   int64_t* h = host_scratch();
   if (!h) return set_error(TQ_EHIP, "pinned staging allocation failed");
-  size_t off = 0;
-  for (int i = 0; i < n; ++i) off += align_up(bytes[i], 8);
-  if (off > 512) return set_error(TQ_EINVAL, "read_back_n: %zu bytes", off);
   off = 0;
   for (int i = 0; i < n; ++i) {
     TQ_TRY_HIP(hipMemcpyAsync((char*)h + off, dev[i], bytes[i], hipMemcpyDeviceToHost, s));
