@@ -371,6 +371,11 @@ typedef struct {
   int32_t agg_dtype[TQ_GB_MAX_AGGS];  /* dtype of the argument segment's value            */
   int64_t nsel;                       /* selected rows; set by tq_groupby_plan             */
   int64_t nrows;                      /* rows the accumulate pass reads (set before plan)  */
+  /* Key dictionaries (all NULL: none).  key_dict[k] non-NULL: key k's packed digit is the rank
+   * of its value among the distinct selected values, which tq_groupby_dict found over the
+   * value range [key_min, key_min + key_dict_range - 1]; key_range is then their count. */
+  const void* key_dict[TQ_GB_MAX_KEYS];
+  int64_t key_dict_range[TQ_GB_MAX_KEYS];
 } tq_groupby_desc_t;
 
 /* Pass 1: per-key-column min and max over selected rows (mask may be NULL) and the selected row
@@ -381,6 +386,22 @@ typedef struct {
 size_t tq_groupby_bounds_ws_bytes(int64_t n);
 int tq_groupby_bounds(const tq_groupby_desc_t* desc /* host */, const uint8_t* mask, int64_t n,
                       int64_t* out, void* ws, void* stream);
+
+/* Key dictionaries, for keys whose value range is much wider than their distinct values (e.g.
+ * attribute codes spaced apart): over the selected rows, one pass marks every present value of
+ * every key in a presence bitmap over [key_min, key_min + key_range - 1] (ranges as
+ * tq_groupby_bounds gives them, each at most TQ_GB_DICT_MAX_RANGE), then each key's bitmap is
+ * ranked and its distinct values listed in ascending order.  dict_out[k] receives key k's
+ * dictionary (inside ws) and ndistinct_host[k] its distinct count (one synchronisation); the
+ * caller that adopts them sets key_dict[k] = dict_out[k], key_dict_range[k] = key_range[k] and
+ * key_range[k] = ndistinct_host[k] before tq_groupby_plan.  Groups and their order are
+ * unchanged; only the packed key space shrinks.  No counterpart in the reference (its group ids
+ * come from a hash sort, operators/aggregate.py:320-327). */
+#define TQ_GB_DICT_MAX_RANGE (1 << 16)
+size_t tq_groupby_dict_ws_bytes(const tq_groupby_desc_t* desc /* host */);
+int tq_groupby_dict(const tq_groupby_desc_t* desc /* host */, const uint8_t* mask, int64_t n,
+                    void* ws, size_t ws_bytes, void** dict_out /* host out, [nkeys] */,
+                    int64_t* ndistinct_host /* host out, [nkeys] */, void* stream);
 
 /* Host planning from the bounds: fills key_stride, picks the table mode and slot count and
  * returns the workspace size.  The packed key is sum((k_i - min_i) * stride_i), so packed order

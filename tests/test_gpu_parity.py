@@ -370,6 +370,41 @@ def test_aggregate_parity(device, case):
         _assert_col(gpu[a].tensor, cpu[a].tensor, f"{case} agg {a}", fp=True)
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_aggregate_key_dictionaries(device, monkeypatch, masked):
+    """Keys whose values are spaced apart (10 codes 100 apart, 20 codes 7 apart): the raw key
+    space (901 x 134 values) needs a global table, the dictionaries shrink it to at most 200
+    slots in LDS.  Groups, counts and sums must match the oracle's, and the dictionaries must
+    have been taken (with a mask, only the selected rows' values count)."""
+    from tqp_amd import kernels
+    taken = []
+    orig = kernels.GroupBy._adopt_dictionaries
+
+    def adopt(self, d, *a):
+        orig(self, d, *a)
+        taken.append(d.key_dict[0] is not None and d.key_dict[1] is not None)
+
+    monkeypatch.setattr(kernels.GroupBy, "_adopt_dictionaries", adopt)
+    g = np.random.default_rng(4242 + masked)
+    n = 1_600_000   # 1.12M selected rows with the mask: above the dictionaries' row threshold
+    k = {10: ('int', (g.integers(0, 10, n) * 100).astype(np.int32)),
+         11: ('int', (g.integers(0, 20, n) * 7 + 3).astype(np.int32))}
+    vals = {20: ('float', g.uniform(900, 105000, n)), 21: ('float', g.integers(0, 11, n) / 100.0),
+            22: ('int', g.integers(-1000, 1000, n, dtype=np.int64))}
+    mask = (g.random(n) < 0.7) if masked else None
+    key_ids = sorted(k)
+    agg_ids = [50, 51, 52, 53, 54, 55, 56]
+    results = [f"k#{i}" for i in key_ids] + AGG_RESULTS()
+    print("dictionary case, masked", masked, flush=True)
+    gpu, cpu, errs = _agg_case(device, k, vals, mask, results, key_ids, agg_ids)
+    assert errs == [None, None], errs
+    assert taken == [True], taken
+    for i in key_ids:
+        _assert_col(gpu[i].tensor, cpu[i].tensor, f"dict key {i}")
+    for a in agg_ids:
+        _assert_col(gpu[a].tensor, cpu[a].tensor, f"dict agg {a}", fp=True)
+
+
 def test_aggregate_many_columns_global_table(device):
     """Eight loaded columns fill the LDS stage, so the dense key space takes the global table
     with atomics rather than LDS partitions; 600k rows over a 250k-key range."""

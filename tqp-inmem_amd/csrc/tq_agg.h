@@ -24,7 +24,21 @@ struct GbSpec {
   int32_t kind[TQ_GB_MAX_AGGS];
   int32_t arg_dtype[TQ_GB_MAX_AGGS];
   uint64_t init[TQ_GB_MAX_AGGS];
+  // key dictionaries (null: the digit is value - key_min): presence bits and exclusive ranks
+  // per 32-value word over the value range, and the distinct values in ascending order
+  const uint32_t* dict_bits[TQ_GB_MAX_KEYS];
+  const uint32_t* dict_rank[TQ_GB_MAX_KEYS];
+  const int64_t* dict_val[TQ_GB_MAX_KEYS];
 };
+
+// Packed-key digit of key k's value v: its offset in the key's range, or its rank among the
+// key's distinct values when the key has a dictionary.
+__device__ __forceinline__ uint64_t key_digit(const GbSpec& G, int k, int64_t v) {
+  const uint64_t off = (uint64_t)(v - G.key_min[k]);
+  if (!G.dict_bits[k]) return off;
+  const uint32_t w = (uint32_t)(off >> 5), b = (uint32_t)(off & 31);
+  return G.dict_rank[k][w] + (uint32_t)__popc(G.dict_bits[k][w] & ((1u << b) - 1u));
+}
 
 struct GbTable {
   uint32_t* cnt;

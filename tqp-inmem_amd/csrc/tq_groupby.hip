@@ -168,7 +168,7 @@ __device__ __forceinline__ uint64_t pack_key(const GbSpec& G, const uint64_t* st
   uint64_t p = 0;
   for (int k = 0; k < G.nkeys; ++k) {
     const int64_t v = (int64_t)stage[(G.key_slot[k] * kSub + j) * kBlock + threadIdx.x];
-    p += (uint64_t)(v - G.key_min[k]) * (uint64_t)G.key_stride[k];
+    p += key_digit(G, k, v) * (uint64_t)G.key_stride[k];
   }
   return p;
 }
@@ -373,15 +373,17 @@ struct OutCols {
   int32_t op[TQ_GB_MAX_AGGS];
 };
 
-// Key k of packed key p (32-bit division when the packed space allows it).
+// Key k of packed key p (32-bit division when the packed space allows it); a dictionary digit
+// is looked up.
 __device__ __forceinline__ int64_t key_of(const GbSpec& G, int k, uint64_t p) {
-  if (G.nkeys == 1) return (int64_t)p + G.key_min[0];
   uint64_t q;
-  if (p <= 0xFFFFFFFFull && G.key_stride[k] <= 0xFFFFFFFFll && G.key_range[k] <= 0xFFFFFFFFll)
+  if (G.nkeys == 1)
+    q = p;
+  else if (p <= 0xFFFFFFFFull && G.key_stride[k] <= 0xFFFFFFFFll && G.key_range[k] <= 0xFFFFFFFFll)
     q = ((uint32_t)p / (uint32_t)G.key_stride[k]) % (uint32_t)G.key_range[k];
   else
     q = (p / (uint64_t)G.key_stride[k]) % (uint64_t)G.key_range[k];
-  return (int64_t)q + G.key_min[k];
+  return G.dict_val[k] ? G.dict_val[k][q] : (int64_t)q + G.key_min[k];
 }
 
 __device__ __forceinline__ void write_group(const GbSpec& G, const OutCols& O, const GbTable& T,
@@ -456,14 +458,16 @@ __global__ __launch_bounds__(kBlock) void k_gb_write_dense(GbSpec G, OutCols O, 
 // Packed keys of rows sub0 + j*256 + t (j < kSub; rows not live read row 0 and are ignored):
 // per key column the dtype is dispatched once and the kSub loads are issued together.
 template <typename T>
-__device__ __forceinline__ void add_key4(const void* col, int64_t sub0, const bool (&live)[kSub],
-                                         int64_t kmin, int64_t stride, uint64_t (&pk)[kSub]) {
+__device__ __forceinline__ void add_key4(const GbSpec& G, int k, const void* col, int64_t sub0,
+                                         const bool (&live)[kSub], uint64_t (&pk)[kSub]) {
   const T* p = (const T*)col;
   T x[kSub];
 #pragma unroll
   for (int j = 0; j < kSub; ++j) x[j] = p[live[j] ? sub0 + (int64_t)j * kBlock + threadIdx.x : 0];
+  const uint64_t stride = (uint64_t)G.key_stride[k];
 #pragma unroll
-  for (int j = 0; j < kSub; ++j) pk[j] += (uint64_t)((int64_t)x[j] - kmin) * (uint64_t)stride;
+  for (int j = 0; j < kSub; ++j)  // rows that are not live read row 0, which may lie outside
+    pk[j] += live[j] ? key_digit(G, k, (int64_t)x[j]) * stride : 0;  // the key's range
 }
 
 __device__ __forceinline__ void packed_keys4(const KeyCols& K, const GbSpec& G, int64_t sub0,
@@ -471,13 +475,12 @@ __device__ __forceinline__ void packed_keys4(const KeyCols& K, const GbSpec& G, 
 #pragma unroll
   for (int j = 0; j < kSub; ++j) pk[j] = 0;
   for (int k = 0; k < G.nkeys; ++k) {
-    const int64_t lo = G.key_min[k], st = G.key_stride[k];
     switch (K.dtype[k]) {
-      case TQ_BOOL: add_key4<uint8_t>(K.ptr[k], sub0, live, lo, st, pk); break;  // 0 or 1
-      case TQ_I8: add_key4<int8_t>(K.ptr[k], sub0, live, lo, st, pk); break;
-      case TQ_I16: add_key4<int16_t>(K.ptr[k], sub0, live, lo, st, pk); break;
-      case TQ_I32: add_key4<int32_t>(K.ptr[k], sub0, live, lo, st, pk); break;
-      default: add_key4<int64_t>(K.ptr[k], sub0, live, lo, st, pk); break;
+      case TQ_BOOL: add_key4<uint8_t>(G, k, K.ptr[k], sub0, live, pk); break;  // 0 or 1
+      case TQ_I8: add_key4<int8_t>(G, k, K.ptr[k], sub0, live, pk); break;
+      case TQ_I16: add_key4<int16_t>(G, k, K.ptr[k], sub0, live, pk); break;
+      case TQ_I32: add_key4<int32_t>(G, k, K.ptr[k], sub0, live, pk); break;
+      default: add_key4<int64_t>(G, k, K.ptr[k], sub0, live, pk); break;
     }
   }
 }
@@ -1018,6 +1021,20 @@ int agg_kind_init(int op, int dt, int32_t* kind, uint64_t* init) {
   return TQ_OK;
 }
 
+// A key dictionary over a value range of `range` values: presence bits and exclusive ranks per
+// 32-value word, then the distinct values (room for all of them).
+struct DictLayout {
+  size_t rank_off, val_off, bytes;
+};
+static DictLayout dict_layout(int64_t range) {
+  const size_t w = (size_t)ceil_div(range > 0 ? range : 1, 32);
+  DictLayout L;
+  L.rank_off = w * 4;
+  L.val_off = align_up(w * 8, 8);
+  L.bytes = align_up(L.val_off + (size_t)(range > 0 ? range : 1) * 8, 256);
+  return L;
+}
+
 static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpec* G) {
   *G = GbSpec{};
   if (d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS) return set_error(TQ_EINVAL, "group-by: %d keys", d->nkeys);
@@ -1030,6 +1047,13 @@ static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpe
     G->key_stride[k] = d->key_stride[k] > 0 ? d->key_stride[k] : 1;
     G->key_range[k] = d->key_range[k] > 0 ? d->key_range[k] : 1;
     G->key_dtype[k] = d->key_dtype[k];
+    if (d->key_dict[k]) {
+      const DictLayout L = dict_layout(d->key_dict_range[k]);
+      const char* base = (const char*)d->key_dict[k];
+      G->dict_bits[k] = (const uint32_t*)base;
+      G->dict_rank[k] = (const uint32_t*)(base + L.rank_off);
+      G->dict_val[k] = (const int64_t*)(base + L.val_off);
+    }
     if (d->key_dtype[k] < TQ_BOOL || d->key_dtype[k] > TQ_I64)
       return set_error(TQ_EUNSUPPORTED, "group-by key %d must be an integer column", k);
     if (prog && (prog->nload <= k || prog->load_dtype[k] != d->key_dtype[k]))
@@ -1118,6 +1142,136 @@ extern "C" int tq_groupby_bounds(const tq_groupby_desc_t* d, const uint8_t* mask
                 (int)d->nkeys, out);
 }
 
+// ------------------------------------------------------------------ key dictionaries
+struct DictPtrs {
+  uint32_t* bits[TQ_GB_MAX_KEYS];
+  uint32_t* rank[TQ_GB_MAX_KEYS];
+  int64_t* val[TQ_GB_MAX_KEYS];
+  int64_t lo[TQ_GB_MAX_KEYS];
+  int32_t words[TQ_GB_MAX_KEYS];
+  int32_t lds_off[TQ_GB_MAX_KEYS + 1];  // word offsets of the keys' bitmaps in LDS; [nkeys]: total
+  int64_t* ndistinct;                     // [nkeys]
+};
+constexpr int kDictLdsWords = 16384;  // 64 KB of LDS bitmaps per workgroup
+
+// Presence bits of every key's selected values: each workgroup marks a grid-stride share of the
+// 4096-row tiles in LDS copies of the bitmaps, then ORs its non-zero words into the global ones.
+// Selected values lie in [lo, lo + 32 * words) (the bounds were taken over the same rows).
+__global__ __launch_bounds__(kBlock) void k_gb_presence(KeyCols K, const uint8_t* __restrict__ mask,
+                                                        int64_t n, DictPtrs D) {
+  extern __shared__ uint32_t lb[];
+  const int t = threadIdx.x;
+  const int total = D.lds_off[K.nkeys];
+  for (int i = t; i < total; i += kBlock) lb[i] = 0;
+  __syncthreads();
+  const int64_t nt = ceil_div(n, kTileRows);
+  for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    const int64_t tile0 = tile * kTileRows;
+    const uint32_t sel = load_sel16(mask, tile0, n);
+    for (int k = 0; k < K.nkeys; ++k) {
+      uint64_t v[kItems];
+      load_raw16(K.ptr[k], K.dtype[k], tile0, n, v);
+      uint32_t* bk = lb + D.lds_off[k];
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) {
+        if ((sel >> j) & 1u) {
+          const uint64_t off = (uint64_t)((int64_t)v[j] - D.lo[k]);
+          atomicOr(&bk[off >> 5], 1u << (off & 31));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < K.nkeys; ++k) {
+    const int w0 = D.lds_off[k], nw = D.lds_off[k + 1] - w0;
+    for (int i = t; i < nw; i += kBlock) {
+      const uint32_t w = lb[w0 + i];
+      if (w) atomicOr(&D.bits[k][i], w);
+    }
+  }
+}
+
+// One workgroup per key: exclusive ranks of the presence words and the distinct values listed
+// in ascending order; the count goes to ndistinct[k].
+__global__ __launch_bounds__(kBlock) void k_gb_dict_rank(DictPtrs D) {
+  __shared__ uint32_t sh[kWaves];
+  const int k = blockIdx.x, t = threadIdx.x;
+  const int nw = D.words[k];
+  uint32_t run = 0;
+  for (int w0 = 0; w0 < nw; w0 += kBlock) {
+    const int i = w0 + t;
+    const uint32_t bits = i < nw ? D.bits[k][i] : 0u;
+    uint32_t tot;
+    const uint32_t r = run + block_exclusive_sum((uint32_t)__popc(bits), sh, &tot);
+    if (i < nw) {
+      D.rank[k][i] = r;
+      uint32_t b = bits, o = r;
+      while (b) {
+        const int bit = __ffs(b) - 1;
+        D.val[k][o++] = D.lo[k] + (int64_t)i * 32 + bit;
+        b &= b - 1;
+      }
+    }
+    run += tot;
+  }
+  if (t == 0) D.ndistinct[k] = run;
+}
+
+extern "C" size_t tq_groupby_dict_ws_bytes(const tq_groupby_desc_t* d) {
+  if (!d || d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS) return 0;
+  size_t b = 256;  // the distinct counts
+  for (int k = 0; k < d->nkeys; ++k) b += dict_layout(d->key_range[k]).bytes;
+  return b;
+}
+
+extern "C" int tq_groupby_dict(const tq_groupby_desc_t* d, const uint8_t* mask, int64_t n,
+                               void* ws, size_t ws_bytes, void** dict_out, int64_t* ndistinct_host,
+                               void* stream) {
+  if (!d || d->nkeys < 1 || d->nkeys > TQ_GB_MAX_KEYS || !dict_out || !ndistinct_host)
+    return set_error(TQ_EINVAL, "group-by dictionary: bad argument");
+  if (ws_bytes < tq_groupby_dict_ws_bytes(d))
+    return set_error(TQ_ECAPACITY, "group-by dictionary: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  KeyCols K{};
+  K.nkeys = d->nkeys;
+  DictPtrs D{};
+  char* p = (char*)ws;
+  D.ndistinct = (int64_t*)p;
+  p += 256;
+  int lds = 0;
+  for (int k = 0; k < d->nkeys; ++k) {
+    const int64_t range = d->key_range[k];
+    if (range < 1 || range > TQ_GB_DICT_MAX_RANGE)
+      return set_error(TQ_EINVAL, "group-by dictionary: key %d spans %lld values", k,
+                       (long long)range);
+    if (d->key_dtype[k] < TQ_BOOL || d->key_dtype[k] > TQ_I64)
+      return set_error(TQ_EUNSUPPORTED, "group-by key %d must be an integer column", k);
+    K.ptr[k] = d->key_ptr[k];
+    K.dtype[k] = d->key_dtype[k];
+    const DictLayout L = dict_layout(range);
+    D.bits[k] = (uint32_t*)p;
+    D.rank[k] = (uint32_t*)(p + L.rank_off);
+    D.val[k] = (int64_t*)(p + L.val_off);
+    D.lo[k] = d->key_min[k];
+    D.words[k] = (int32_t)ceil_div(range, 32);
+    D.lds_off[k] = lds;
+    lds += D.words[k];
+    dict_out[k] = (void*)p;
+    p += L.bytes;
+  }
+  D.lds_off[d->nkeys] = lds;
+  if (lds > kDictLdsWords)
+    return set_error(TQ_EINVAL, "group-by dictionary: %d bitmap words exceed LDS", lds);
+  TQ_TRY(fill_u64(ws, 0, (int64_t)((p - (char*)ws) / 8), s));
+  if (n > 0) {
+    const int64_t nt = ceil_div(n, kTileRows);
+    TQ_TRY(launch("k_gb_presence", k_gb_presence, (unsigned)(nt < 1024 ? nt : 1024), kBlock,
+                  (size_t)lds * 4, s, K, mask, n, D));
+    TQ_TRY(launch("k_gb_dict_rank", k_gb_dict_rank, (unsigned)d->nkeys, kBlock, 0, s, D));
+  }
+  return read_back(D.ndistinct, (size_t)d->nkeys * 8, ndistinct_host, s);
+}
+
 extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload, int32_t* mode,
                                int64_t* slots, size_t* ws_bytes) {
   if (!d || d->nkeys < 0 || d->nkeys > TQ_GB_MAX_KEYS || d->naggs < 0 || d->naggs > TQ_GB_MAX_AGGS)
@@ -1136,13 +1290,18 @@ extern "C" int tq_groupby_plan(tq_groupby_desc_t* d, int64_t nsel, int32_t nload
   d->nsel = ns;
   const size_t stage = lds_stage_bytes(nload);
   const size_t dense_per = 4 + 8 * (size_t)d->naggs;
+  // a key space shrunk by dictionaries may use up to half of a CU's LDS per workgroup (two
+  // workgroups per CU), where the dense-LDS table otherwise stays within kLdsBudget
+  bool dict = false;
+  for (int k = 0; k < d->nkeys; ++k) dict = dict || d->key_dict[k] != nullptr;
+  const size_t dense_budget = dict ? (size_t)80 * 1024 : (size_t)kLdsBudget;
   int32_t m;
   int64_t sl;
   if (S <= 64 && (size_t)S * 64 * dense_per + stage <= (size_t)kLdsBig) {
     // a handful of groups: one table copy per lane keeps their rows off shared LDS addresses
     m = TQ_GB_DENSE_LDS | TQ_GB_LDS_LANES;
     sl = S;
-  } else if ((size_t)S * dense_per + stage <= (size_t)kLdsBudget) {
+  } else if ((size_t)S * dense_per + stage <= (size_t)dense_budget) {
     m = TQ_GB_DENSE_LDS;
     sl = S;
   } else {

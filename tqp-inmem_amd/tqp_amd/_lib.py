@@ -45,6 +45,7 @@ OP = {name: i for i, name in enumerate(OPCODES)}
 TQ_AGG_SUM, TQ_AGG_COUNT, TQ_AGG_AVG, TQ_AGG_MIN, TQ_AGG_MAX = range(5)
 TQ_GB_MAX_KEYS = 8
 TQ_GB_DENSE_LDS, TQ_GB_DENSE_GLOBAL, TQ_GB_HASH, TQ_GB_DENSE_PART, TQ_GB_LDS_FRONT = 0, 1, 2, 3, 8
+TQ_GB_DICT_MAX_RANGE = 1 << 16
 TQ_GB_LDS_LANES = 16
 TQ_JOIN_EMPTY, TQ_JOIN_DENSE, TQ_JOIN_HASH = 0, 1, 2
 
@@ -106,7 +107,9 @@ class GroupByDesc(ctypes.Structure):
                 ("key_stride", ctypes.c_int64 * TQ_GB_MAX_KEYS),
                 ("agg_op", ctypes.c_int32 * TQ_MAX_SEGS),
                 ("agg_dtype", ctypes.c_int32 * TQ_MAX_SEGS),
-                ("nsel", ctypes.c_int64), ("nrows", ctypes.c_int64)]
+                ("nsel", ctypes.c_int64), ("nrows", ctypes.c_int64),
+                ("key_dict", ctypes.c_void_p * TQ_GB_MAX_KEYS),
+                ("key_dict_range", ctypes.c_int64 * TQ_GB_MAX_KEYS)]
 
 
 # ---------------------------------------------------------------- loading
@@ -140,6 +143,8 @@ _SIGNATURES = {
     "tq_groupby_bounds_ws_bytes": ([I64], SZ),
     "tq_groupby_bounds": ([P, P, I64, P, P, P], I32),
     "tq_groupby_plan": ([P, I64, I32, P, P, P], I32),
+    "tq_groupby_dict_ws_bytes": ([P], SZ),
+    "tq_groupby_dict": ([P, P, I64, P, SZ, P, P, P], I32),
     "tq_groupby_accumulate": ([P, P, P, P, I64, I32, I64, P, SZ, P], I32),
     "tq_groupby_count": ([P, I32, I64, P, P, P], I32),
     "tq_groupby_write": ([P, I32, I64, P, I64, P, P, P], I32),

@@ -453,6 +453,47 @@ class GroupBy:
             b.cols.width[0] = 1
         self.ops = [self.OPS[op] for op, _ in aggs]
 
+    # a dense LDS table of at most this many bytes (with the staging rows) runs two workgroups
+    # per CU; above it the group-by takes a global or hash table
+    DENSE_LDS_BYTES = 64 * 1024
+    DICT_LDS_BYTES = 80 * 1024
+
+    def _dictionaries_pay(self, d, nsel, nload):
+        """Key dictionaries are worth a presence pass and a read-back when the raw key space is
+        too large for a dense LDS table, every key's value range is narrow enough for a bitmap,
+        and there are enough rows to amortise the pass."""
+        if not self.keys or nsel < (1 << 20):
+            return False
+        ranges = [d.key_range[i] for i in range(len(self.keys))]
+        if any(r > L.TQ_GB_DICT_MAX_RANGE for r in ranges) or \
+                sum((r + 31) // 32 for r in ranges) > 16384:
+            return False
+        space = 1
+        for r in ranges:
+            space *= r
+        per = 4 + 8 * len(self.ops)
+        return space * per + max(nload, 1) * 8192 > self.DENSE_LDS_BYTES
+
+    def _adopt_dictionaries(self, d, mptr, s, dev, nload):
+        """Build the keys' dictionaries and adopt them when the shrunk key space fits a dense
+        LDS table."""
+        wsb = L.size_call("tq_groupby_dict_ws_bytes", ctypes.byref(d))
+        ws = L.workspace(wsb, dev, "gb_dict")
+        out = (ctypes.c_void_p * L.TQ_GB_MAX_KEYS)()
+        nd = (ctypes.c_int64 * L.TQ_GB_MAX_KEYS)()
+        L.call("tq_groupby_dict", ctypes.byref(d), mptr, self.n, L.ptr(ws), wsb, out, nd, s)
+        k = len(self.keys)
+        space = 1
+        for i in range(k):
+            space *= max(int(nd[i]), 1)
+        if space * (4 + 8 * len(self.ops)) + max(nload, 1) * 8192 > self.DICT_LDS_BYTES:
+            return
+        for i in range(k):
+            d.key_dict[i] = out[i]
+            d.key_dict_range[i] = d.key_range[i]
+            d.key_range[i] = max(int(nd[i]), 1)
+        self._dict_ws = ws
+
     def _reads_pairs(self, df):
         """Whether every column this group-by reads is an output of the deferred join (sized
         for the join's capacity, unmasked)."""
@@ -502,6 +543,8 @@ class GroupBy:
             lo, hi = hb[2 * i], hb[2 * i + 1]
             d.key_min[i] = lo if nsel > 0 else 0
             d.key_range[i] = (hi - lo + 1) if nsel > 0 else 1
+        if not pend and self._dictionaries_pay(d, nsel, b.nload):
+            self._adopt_dictionaries(d, mptr, s, dev, b.nload)
         mode = ctypes.c_int32()
         slots = ctypes.c_int64()
         wsb = ctypes.c_size_t()
