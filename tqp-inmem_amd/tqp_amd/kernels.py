@@ -318,13 +318,15 @@ class DeferredJoinFailed(Exception):
 
 
 class Deferred:
-    """A unique-key join whose pairs feed a group-by, run with one synchronisation.
+    """A unique-key join whose pairs feed a group-by (and a top-k of its groups), run with one
+    synchronisation.
 
-    While a Deferred is active (`deferring`), JoinIndex.probe writes its pairs into outputs
-    sized for every probe row and leaves the pair count m on the device; a partitioned group-by
-    whose columns are those outputs reads m there and leaves its group count G on the device.
-    `resolve` then reads m, G and the build's uniqueness check back together; any other
-    consumer of the pairs calls `resolve_join` first, which reads m and the check."""
+    While a Deferred is the thread's active one (`set_deferred`), JoinIndex.probe writes its
+    pairs into outputs sized for every probe row and leaves the pair count m on the device; a
+    partitioned group-by whose columns are those outputs reads m there and leaves its group
+    count G on the device; a top-k of those groups reads G there (`sort_perm`).  `resolve`
+    then reads m, G and the build's uniqueness check back together; a consumer that needs m on
+    the host sooner calls `resolve_join`, which reads m and the check."""
 
     def __init__(self):
         self.join = None
