@@ -933,11 +933,14 @@ static int part_tsub(int64_t nrows, int nparts) {
   return t;
 }
 
-// Workgroups of a dense-LDS accumulate launch: one per 1024-row sub-tile, at most 1024, and few
+// Workgroups of a dense-LDS accumulate launch: one per 1024-row sub-tile, at most 512, and few
 // enough that their partial tables stay within 32 MB.
 static int32_t dense_lds_grid(int64_t nrows, int64_t slots, int naggs) {
   int64_t g = ceil_div(nrows > 0 ? nrows : 1, kSubRows);
-  if (g > 1024) g = 1024;
+  // one round of resident workgroups (a dense-LDS table with its stage takes up to 80 KB: two
+  // per CU), each over several sub-tiles, so the table set-up and the partial write are paid
+  // once per workgroup and no second round trails
+  if (g > 512) g = 512;
   const int64_t per = slots * (1 + naggs) * 8;
   const int64_t cap = ((int64_t)32 << 20) / (per > 0 ? per : 1);
   if (g > cap) g = cap;
