@@ -3,6 +3,8 @@
 #include <stdio.h>
 
 #include <atomic>
+#include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -102,3 +104,33 @@ extern "C" int tq_timing_read(tq_ktime_t* out, int64_t cap, int64_t* n, int64_t*
   tq::g_dropped = 0;
   return TQ_OK;
 }
+
+namespace tq {
+
+int64_t resident_blocks(const void* kernel, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void*, size_t>, int64_t> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1024;
+  }
+  std::lock_guard<std::mutex> g(mu);
+  const auto key = std::make_tuple(dev, kernel, lds);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, lds) != hipSuccess ||
+      per < 1) {
+    (void)hipGetLastError();
+    per = 1;  // e.g. more LDS than a workgroup gets before it is allowed: one per CU
+  }
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1) {
+    (void)hipGetLastError();
+    cus = 256;
+  }
+  return cache[key] = (int64_t)per * cus;
+}
+
+}  // namespace tq

@@ -321,6 +321,19 @@ __device__ __forceinline__ void write_null(void* dst, int64_t di, int bytes, int
 int fill_u32(void* p, uint32_t v, int64_t count, hipStream_t s);
 int fill_u64(void* p, uint64_t v, int64_t count, hipStream_t s);
 unsigned stream_grid(int64_t m);
+
+// Workgroups of `kernel` (kBlock threads, `lds` bytes of dynamic LDS) resident on the current
+// device at once: occupancy per CU times the CUs (cached per device, kernel and LDS size).
+int64_t resident_blocks(const void* kernel, size_t lds);
+
+// Grid of a kernel whose workgroups stride over `items` work items: one resident round, so no
+// workgroup starts only after others have finished all of theirs.
+template <typename... P>
+unsigned resident_grid(void (*kernel)(P...), size_t lds, int64_t items) {
+  const int64_t r = resident_blocks((const void*)kernel, lds);
+  const int64_t g = items < r ? items : r;
+  return (unsigned)(g < 1 ? 1 : g);
+}
 // Length of a scan whose element count a producer earlier on the stream left in device memory:
 // with n_dev, the elements are the first mult * ceil(min(*n_dev, cap) / per) of the n the launch
 // was sized for (a histogram of per-tile counts whose tile count follows a device row count).

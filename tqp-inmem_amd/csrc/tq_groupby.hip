@@ -1393,12 +1393,13 @@ static int accumulate_impl(const tq_groupby_desc_t* d, const tq_program_t* prog,
   if (dense_lds && n != d->nrows)
     return set_error(TQ_EINVAL, "group-by: %lld rows, planned for %lld", (long long)n,
                      (long long)d->nrows);
-  int64_t g = ceil_div(n, kSubRows);
-  if (g > (front ? 512 : 1024)) g = front ? 512 : 1024;
-  const unsigned grid = dense_lds ? (unsigned)w.pblocks : (unsigned)g;
+  // global and hash tables: one resident round of workgroups striding over the sub-tiles
+  const int64_t g = ceil_div(n, kSubRows);
 #define TQ_ACCUM(BASE, FRONT, LANES, CAP)                                                     \
-  launch_lds("k_gb_accum", (k_gb_accum<BASE, FRONT, LANES>), grid, lds, s, G, *prog, *cols,  \
-             mask, n, w.T, CAP, stage_words, w.part)
+  launch_lds("k_gb_accum", (k_gb_accum<BASE, FRONT, LANES>),                                 \
+             dense_lds ? (unsigned)w.pblocks                                                  \
+                       : resident_grid(k_gb_accum<BASE, FRONT, LANES>, lds, g),               \
+             lds, s, G, *prog, *cols, mask, n, w.T, CAP, stage_words, w.part)
   if (dense_lds) {
     TQ_TRY(lanes ? TQ_ACCUM(TQ_GB_DENSE_LDS, false, true, cap)
                  : TQ_ACCUM(TQ_GB_DENSE_LDS, false, false, cap));

@@ -745,9 +745,7 @@ static JoinView view_of(const tq_join_state_t* st) {
   return J;
 }
 
-static unsigned probe_grid(int64_t ntiles) {
-  return (unsigned)(ntiles < 2048 ? (ntiles > 0 ? ntiles : 1) : 2048);
-}
+
 
 // Sorted layout for repeated keys: compact the selected (key, row) pairs in row order (the
 // statistics pass left the per-tile counts in the workspace), sort them stably by key and record
@@ -918,7 +916,8 @@ template <typename K, bool HASH, bool UNIQUE>
 static int count_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
                       const uint8_t* pmask, int64_t np, int64_t nt, ProbeWs& w,
                       const BuildCheck& check, hipStream_t s) {
-  return launch("k_join_count", (k_join_count<K, HASH, UNIQUE>), probe_grid(nt), kBlock, 0, s,
+  return launch("k_join_count", (k_join_count<K, HASH, UNIQUE>),
+                resident_grid(k_join_count<K, HASH, UNIQUE>, 0, nt), kBlock, 0, s,
                 view_of(st), pkeys, pdt, pmask, np, nt, w.tile_counts,
                 UNIQUE ? w.hits : (uint16_t*)nullptr, check);
 }
@@ -946,13 +945,16 @@ static int write_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
   const JoinView J = view_of(st);
   const int64_t* off = (const int64_t*)w.tile_off;
   if (UNIQUE && (L.n > 0 || bidx))
-    return launch("k_join_write", (k_join_write_unique<K, HASH, true>), probe_grid(nt), kBlock, 0,
-                  s, J, pkeys, pdt, nt, (const uint16_t*)w.hits, off, L, R, bidx, pidx);
+    return launch("k_join_write", (k_join_write_unique<K, HASH, true>),
+                  resident_grid(k_join_write_unique<K, HASH, true>, 0, nt), kBlock, 0, s, J,
+                  pkeys, pdt, nt, (const uint16_t*)w.hits, off, L, R, bidx, pidx);
   if (UNIQUE)
-    return launch("k_join_write", (k_join_write_unique<K, HASH, false>), probe_grid(nt), kBlock,
-                  0, s, J, pkeys, pdt, nt, (const uint16_t*)w.hits, off, L, R, bidx, pidx);
-  return launch("k_join_write", (k_join_write<K, HASH, false, true>), probe_grid(nt), kBlock, 0, s,
-                J, pkeys, pdt, pmask, np, nt, off, L, R, bidx, pidx);
+    return launch("k_join_write", (k_join_write_unique<K, HASH, false>),
+                  resident_grid(k_join_write_unique<K, HASH, false>, 0, nt), kBlock, 0, s, J,
+                  pkeys, pdt, nt, (const uint16_t*)w.hits, off, L, R, bidx, pidx);
+  return launch("k_join_write", (k_join_write<K, HASH, false, true>),
+                resident_grid(k_join_write<K, HASH, false, true>, 0, nt), kBlock, 0, s, J, pkeys,
+                pdt, pmask, np, nt, off, L, R, bidx, pidx);
 }
 
 // Calls F<K, HASH, UNIQUE>::run(args...) for the state's key width and layout.
