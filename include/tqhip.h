@@ -84,6 +84,11 @@ enum tq_opcode {
   TQ_OP_STR_EQ, TQ_OP_STR_PREFIX,
   /* dst <- calendar year of int64 day count since 1990-01-01 (bucketize on year edges) */
   TQ_OP_YEAR,
+  /* dst <- string column a matches '%w1%w2%': w1 occurs at some position q and w2 at some
+     position >= q + len(w1), both inside the column's width (padding bytes included), as the
+     reference's like_mask scans (operators/like.py:15-25, filter.py:121-127);
+     imm = off1 | len1 << 16 | off2 << 32 | len2 << 48 in the string pool */
+  TQ_OP_STR_LIKE2,
   TQ_OP_COUNT_
 };
 

@@ -4,6 +4,7 @@ This module is the parity oracle and the CPU baseline.  It restates, on torch CP
 algorithms of the reference's Python operators, each function citing the file:line it follows:
 
   filter_evaluate / tqp_filter    operators/filter.py:14-162 / 164-205
+  like_mask                       operators/like.py:13-25
   apply_masks                     IO/vortex_pipeline.py:51-70
   join_kernel / join_vortex       operators/hashjoin.py:24-177 / 179-248
   agg_evaluate / tqp_hash_aggregate  operators/aggregate.py:58-233 / 236-393
@@ -134,6 +135,11 @@ def filter_evaluate(tensors, tree, rows, subquery_result, leaf_function=None):
         for r in vals:
             mask = torch.logical_or(mask, torch.eq(lhs, r))
         return mask
+    if op == 'LIKE':
+        lhs = filter_evaluate(tensors, tree[0], rows, subquery_result, leaf_function)
+        assert tree[-1].count('%') == 3
+        _, w1, w2, _ = tree[-1].split('%')
+        return like_mask(lhs, w1, w2)
     ops = {'<=': torch.le, '>=': torch.ge, '<': torch.lt, '>': torch.gt, '=': torch.eq,
            'AND': torch.logical_and, 'OR': torch.logical_or}
     if op not in ops:
@@ -153,6 +159,29 @@ def filter_evaluate(tensors, tree, rows, subquery_result, leaf_function=None):
         other = torch.full((lhs.shape[0],), rhs, dtype=lhs.dtype) if isinstance(rhs, float) else rhs
         return torch.isclose(lhs, other.to(lhs.dtype) if torch.is_tensor(other) else other, atol=EPS)
     return ops[op](lhs, rhs)
+
+
+def like_mask(a, w1, w2):
+    """operators/like.py:13-25 ('%w1%w2%', filter.py:121-127): rows of the [n, m] string tensor
+    with w1 at some position q and w2 at a position >= q + len(w1), both windows inside the m
+    bytes (padding included).  Restated with window matches over every start position and a
+    running OR of the w1 hits."""
+    x = a.numpy()
+    n, m = x.shape
+    n1, n2 = len(w1), len(w2)
+    if m - n1 - n2 + 1 <= 0:
+        return torch.zeros(n, dtype=torch.bool)
+
+    def hits(w):
+        if not w:
+            return np.ones((n, m + 1), dtype=bool)
+        win = np.lib.stride_tricks.sliding_window_view(x, len(w), axis=1)
+        return (win == str_codes(w, len(w)).numpy()).all(axis=-1)
+
+    npos = m - n1 - n2 + 1
+    seen1 = np.logical_or.accumulate(hits(w1)[:, :npos], axis=1)
+    at2 = hits(w2)[:, n1:n1 + npos]
+    return torch.from_numpy((seen1 & at2).any(axis=1))
 
 
 def tqp_filter(tensor_group, args, subquery_result=None):

@@ -70,6 +70,37 @@ def test_filter_f32_narrow_int_columns(device):
         assert int(counts.sum().item()) == int(want.sum().item())
 
 
+def test_filter_like_bit_exact(device):
+    """'%w1%w2%' over a 2-D string column (ragged row count), alone, negated and in a
+    conjunction, with words planted in both orders and overlapping."""
+    from tqp_amd import kernels
+    g = np.random.default_rng(23)
+    n, w = 200_003, 40
+    s = np.array([ord(c) for c in "ab q"], dtype=np.int8)[g.integers(0, 4, (n, w))]
+    s[g.random((n, w)) < 0.05] = 0
+    for word, frac in (("special", 0.3), ("requests", 0.3), ("specialrequests", 0.05)):
+        rows = np.flatnonzero(g.random(n) < frac)
+        at = g.integers(0, w - len(word) + 1, rows.shape[0])
+        for r, p in zip(rows, at):
+            s[r, p:p + len(word)] = [ord(c) for c in word]
+    tables = {4: ('string', s), 5: ('int', g.integers(0, 100, n, dtype=np.int32))}
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    sums = []
+    for tree in (['c#4', 'LIKE', '%special%requests%'],
+                 ['NOT', ['c#4', 'LIKE', '%special%requests%']],
+                 [['c#4', 'LIKE', '%q%ab%'], 'AND', ['k#5', '<', '50']],
+                 ['c#4', 'LIKE', '%%aa%'],
+                 ['c#4', 'LIKE', '%a%' + 'b' * 40 + '%']):
+        print("like:", tree, flush=True)
+        mask, counts = kernels.filter_mask(tree, gpu, None)
+        want = R.filter_evaluate(cpu, tree, None, None)
+        _assert_col(mask, want, f"mask {tree}")
+        assert int(counts.sum().item()) == int(want.sum().item())
+        sums.append(int(want.sum()))
+    assert 0 < sums[0] < n and 0 < sums[3] < n and sums[4] == 0
+
+
 @pytest.mark.parametrize("pi", range(len(PREDICATES)))
 def test_filter_mask_bit_exact(device, pi):
     from tqp_amd import kernels, workloads

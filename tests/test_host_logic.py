@@ -55,3 +55,30 @@ def test_program_cache_rebinds_columns_and_recompiles_on_dtype_change(monkeypatc
     d = {5: Col(torch.arange(30, dtype=torch.int64))}
     b3 = E.cached_program(key, d, lambda pb: E.compile_predicate(pb, tree, d))
     assert isinstance(b3, E.ProgramBuilder) and b3.cols.ptr[0] == d[5].tensor.data_ptr()
+
+
+def test_like_compiles_to_one_string_instruction(monkeypatch):
+    """'%w1%w2%' becomes one STR_LIKE2 over the 2-D column, both words in the string pool and
+    their offsets and lengths packed into the immediate; other % counts are refused."""
+    import pytest
+
+    from tqp_amd import _lib as L
+    from tqp_amd import expr as E
+    monkeypatch.setattr(L, "require_gpu", lambda *a, **k: None)
+
+    class Col:
+        def __init__(self, t):
+            self.tensor = t
+
+    cols = {7: Col(torch.zeros((10, 16), dtype=torch.int8))}
+    pb = E.ProgramBuilder()
+    E.compile_predicate(pb, ['c#7', 'LIKE', '%special%requests%'], cols)
+    ins = [pb.prog.code[i] for i in range(pb.ninstr)]
+    like = [i for i in ins if i.op == L.OP["STR_LIKE2"]]
+    assert len(like) == 1
+    imm = like[0].imm
+    off1, len1, off2, len2 = imm & 0xFFFF, (imm >> 16) & 0xFFFF, (imm >> 32) & 0xFFFF, imm >> 48
+    pool = bytes(pb.prog.strpool[i] for i in range(L.TQ_STRPOOL))
+    assert pool[off1:off1 + len1] == b"special" and pool[off2:off2 + len2] == b"requests"
+    with pytest.raises(AssertionError):
+        E.compile_predicate(E.ProgramBuilder(), ['c#7', 'LIKE', '%special%'], cols)

@@ -6,6 +6,8 @@ available), so each oracle function is checked against an independent statement 
 (oracle/closed_form.py) on random small inputs, plus hand-worked cases.  The planted-mistake
 tests show the checks have teeth: a plausible bug injected into the oracle makes them fail.
 """
+import re
+
 import numpy as np
 import pytest
 import torch
@@ -66,6 +68,37 @@ def test_filter_string_equality_rows():
     t = {5: R.Col(torch.from_numpy(s), 'string')}
     assert R.filter_evaluate(t, ['seg#5', '=', 'BUILDING'], None, None).tolist() == [True, False, False, True]
     assert R.filter_evaluate(t, ['StartsWith', ['seg#5', 'BUILD']], None, None).tolist() == [True, True, False, True]
+
+
+def _like_regex(s, w1, w2):
+    # '%w1%w2%' over the full row bytes, padding included: a regular-expression search
+    text = bytes(int(c) & 0xFF for c in s).decode('latin-1')
+    return re.search(re.escape(w1) + '.*' + re.escape(w2), text, re.DOTALL) is not None
+
+
+def test_like_matches_regex_search():
+    g = np.random.default_rng(5)
+    rows = ["special requests", "requests special", "specialrequests", "specialrequest",
+            "xspecial yrequestsz", "special", "", "requests", "spec requests special requests",
+            "special%requests", "sspecialrequestss"]
+    s = np.zeros((len(rows) + 400, 32), dtype=np.int8)
+    for i, w in enumerate(rows):
+        s[i, :len(w)] = [ord(c) for c in w]
+    # random rows over a small alphabet, so both words occur by chance in many orders
+    rnd = g.integers(0, 4, (400, 32))
+    s[len(rows):] = np.array([ord(c) for c in "abc "], dtype=np.int8)[rnd]
+    s[len(rows):][g.random((400, 32)) < 0.1] = 0
+    t = {5: R.Col(torch.from_numpy(s), 'string')}
+    for pat in ('%special%requests%', '%ab%ca%', '%a%a%', '%abc%%', '%%cab%', '%%%',
+                '%special requests%s%', '%a%' + 'b' * 31 + '%'):
+        _, w1, w2, _ = pat.split('%')
+        got = R.filter_evaluate(t, ['c#5', 'LIKE', pat], None, None).tolist()
+        want = [_like_regex(r, w1, w2) for r in s]
+        assert got == want, pat
+    assert R.filter_evaluate(t, ['c#5', 'LIKE', '%special%requests%'], None, None).tolist()[:11] == \
+        [True, False, True, False, True, False, False, False, True, True, True]
+    with pytest.raises(AssertionError):
+        R.filter_evaluate(t, ['c#5', 'LIKE', '%special%'], None, None)
 
 
 def test_compaction_order_preserving():

@@ -87,6 +87,28 @@ __device__ __forceinline__ void stage_loads(const tq_program_t& P, const tq_cols
   }
 }
 
+// '%w1%w2%' over one row's string bytes sp[0, w): the positions scanned as like_mask does
+// (operators/like.py:15-25): at each pos, w1 seen at or before pos, then w2 at pos + len1.
+__device__ __forceinline__ bool str_like2(const int8_t* sp, int w, const uint8_t* pool,
+                                          int64_t imm) {
+  const int off1 = (int)(imm & 0xffff), len1 = (int)((imm >> 16) & 0xffff);
+  const int off2 = (int)((imm >> 32) & 0xffff), len2 = (int)((imm >> 48) & 0xffff);
+  bool appear = false;
+  for (int pos = 0; pos + len1 + len2 <= w; ++pos) {
+    if (!appear) {
+      bool m = true;
+      for (int c = 0; c < len1 && m; ++c) m = sp[pos + c] == (int8_t)pool[off1 + c];
+      appear = m;
+    }
+    if (appear) {
+      bool m = true;
+      for (int c = 0; c < len2 && m; ++c) m = sp[pos + len1 + c] == (int8_t)pool[off2 + c];
+      if (m) return true;
+    }
+  }
+  return false;
+}
+
 // One row's eight registers, kept as named scalars and selected by switch: an indexable array
 // would be placed in private scratch memory by the compiler.
 struct Regs {
@@ -130,6 +152,10 @@ __device__ __forceinline__ void run_program(const tq_program_t& P, const tq_cols
     switch (I.op) {
       case TQ_OP_LOAD: o = stage[(I.a * kSub + j) * kBlock + t]; break;
       case TQ_OP_CONST: o = (uint64_t)I.imm; break;
+      case TQ_OP_STR_LIKE2:
+        o = valid && str_like2((const int8_t*)col_ptr(C, I.a) + row * (int64_t)col_width(C, I.a),
+                               col_width(C, I.a), P.strpool, I.imm);
+        break;
       case TQ_OP_STR_EQ:
       case TQ_OP_STR_PREFIX: {
         const int8_t* p = (const int8_t*)col_ptr(C, I.a);
@@ -276,6 +302,14 @@ __device__ __forceinline__ void run_rows(const tq_program_t& P, const tq_colset_
     switch (I.op) {
       case TQ_OP_LOAD: TQ_EACH(stage[(I.a * kSub + j) * kBlock + t]) break;
       case TQ_OP_CONST: TQ_EACH(I.imm) break;
+      case TQ_OP_STR_LIKE2: {
+        const int8_t* p = (const int8_t*)col_ptr(C, I.a);
+        const int w = col_width(C, I.a);
+#pragma unroll
+        for (int j = 0; j < kSub; ++j)
+          o[j] = valid[j] && str_like2(p + row[j] * (int64_t)w, w, P.strpool, I.imm);
+        break;
+      }
       case TQ_OP_STR_EQ:
       case TQ_OP_STR_PREFIX: {
         const int8_t* p = (const int8_t*)col_ptr(C, I.a);

@@ -39,6 +39,7 @@ OPCODES = [
     "WHERE",
     "STR_EQ", "STR_PREFIX",
     "YEAR",
+    "STR_LIKE2",
 ]
 OP = {name: i for i, name in enumerate(OPCODES)}
 

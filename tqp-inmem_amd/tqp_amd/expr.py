@@ -273,6 +273,16 @@ class ProgramBuilder:
         self.emit("STR_PREFIX" if prefix else "STR_EQ", dst=r, a=col.col, imm=self.pool_string(s))
         return Value("reg", reg=r, dtype=torch.bool)
 
+    def str_like2(self, col: Value, w1: str, w2: str) -> Value:
+        """Rows of a 2-D string column that hold w1 and, after it, w2 (like.py:13-25)."""
+        if col.kind != "col2d":
+            raise NotImplementedError("LIKE needs a 2-D string column")
+        p1, p2 = self.pool_string(w1), self.pool_string(w2)
+        imm = ((p1 & 0xFFFF) | ((p1 >> 32) << 16) | ((p2 & 0xFFFF) << 32) | ((p2 >> 32) << 48))
+        r = self.alloc()
+        self.emit("STR_LIKE2", dst=r, a=col.col, imm=imm)
+        return Value("reg", reg=r, dtype=torch.bool)
+
     def load_bytes(self) -> int:
         """Bytes per row the load table reads."""
         return sum(self.col_tensors[self.prog.load_col[s]].element_size() for s in range(self.nload))
@@ -346,7 +356,11 @@ def compile_predicate(b: ProgramBuilder, tree, tensors, subquery_result=None, le
             b.release(lhs)
             return acc if acc is not None else b.const(False, torch.bool)
         if op == 'LIKE':
-            raise NotImplementedError("LIKE predicates")
+            # '%w1%w2%' only, as the reference asserts (filter.py:121-127)
+            lhs = ev(tree[0])
+            assert tree[-1].count('%') == 3
+            _, w1, w2, _ = tree[-1].split('%')
+            return b.str_like2(lhs, w1, w2)
         if op in ('AND', 'OR'):
             return b.logic(op, ev(tree[0]), ev(tree[-1]))
         if op not in ('<=', '>=', '<', '>', '='):
