@@ -226,8 +226,8 @@ def batched_section(pipes_n, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              f"(every box has one MI355X; the sharded design is described in DESIGN.md)",
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              f"(the path runs on one MI355X; sharding across GPUs is a design, not built)",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -187,7 +187,7 @@ def tqp_hash_aggregate(gpu_enable, tensor_group, args, gpu_mem_pool, cpu_mem_poo
                        name="aggregation"):
     perf_logger().start(f"{name}")
     if not gpu_enable:
-        raise NotImplementedError("this build executes operators on the GPU only")
+        raise NotImplementedError("tqp_amd executes operators on the GPU only (gpu_enable=False has no CPU path)")
     assert not args['is partial']
 
     transfer_set = args['input ids']

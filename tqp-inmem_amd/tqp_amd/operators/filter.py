@@ -27,7 +27,7 @@ def tqp_filter(gpu_enable, tensor_group, args, subquery_result, cpu_mem_pool, gp
     perf_logger().start(name)
     assert 'mask map' not in args, "Filter not accepting masks"
     if not gpu_enable:
-        raise NotImplementedError("this build executes operators on the GPU only")
+        raise NotImplementedError("tqp_amd executes operators on the GPU only (gpu_enable=False has no CPU path)")
     tree = args['filter tree']
     input_columns = {k: tensor_group[k] for k in args['materialized in']}
 

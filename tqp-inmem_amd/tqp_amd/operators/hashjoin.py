@@ -77,7 +77,7 @@ def join_kernel_new(gpu_enable, tensors_left, tensors_right, left_key, right_key
                     left_global_idx, right_global_idx):
     assert type in {'inner', 'right-semi', 'right-anti', 'right-outer'}, "Invalid Join type."
     if not gpu_enable:
-        raise NotImplementedError("this build executes operators on the GPU only")
+        raise NotImplementedError("tqp_amd executes operators on the GPU only (gpu_enable=False has no CPU path)")
 
     tensors_left[left_key].normalize()
     tensors_right[right_key].normalize()

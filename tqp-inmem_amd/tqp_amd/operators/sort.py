@@ -16,7 +16,7 @@ from ..variable import Variable
 def tqp_sort(gpu_enable, tensor_group, args, with_limit, name="sort"):
     perf_logger().start(name)
     if not gpu_enable:
-        raise NotImplementedError("this build executes operators on the GPU only")
+        raise NotImplementedError("tqp_amd executes operators on the GPU only (gpu_enable=False has no CPU path)")
     limit = args['limit'] if with_limit else None
 
     input_ids = args['input ids']
