@@ -623,6 +623,19 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
     const int64_t c = *n_dev;
     m = c < m ? (c > 0 ? c : 0) : m;
   }
+  if ((int64_t)blockIdx.x * kListTile >= m) {
+    // no candidate in this workgroup (a grid sized for a capacity): only the sentinels, or
+    // the zero permutation, that the full pass would write with no survivors
+    if (perm_out) {
+      for (int64_t i = t; i < nout; i += kBlock) perm_out[i] = 0;
+    } else {
+      for (int i = t; i < k; i += kBlock) {
+        kout[(int64_t)blockIdx.x * k + i] = ~0ull;
+        vout[(int64_t)blockIdx.x * k + i] = 0xFFFFFFFFu;
+      }
+    }
+    return;
+  }
   uint64_t key[kListPer];
   uint32_t row[kListPer];
   uint32_t valid = 0;
