@@ -106,22 +106,45 @@ def join_aggregate_span(calls, launches, steps, tables, pairs, groups):
             "target_frac": 0.5}
 
 
+def cpu_model():
+    """The host CPU's model name (what `lscpu` prints as 'Model name')."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(tables, plan, runs):
+    """The torch-CPU restatement of the reference operators on this host: one thread per core
+    of the process's CPU share (OMP_NUM_THREADS when the host sets one, else every core)."""
     import torch
     from oracle import ref_ops as R
     from tqp_amd import workloads
-    torch.set_num_threads(torch.get_num_threads())
+    ncpu = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", ncpu) or ncpu)
+    threads = max(1, min(ncpu, share))
+    torch.set_num_threads(threads)
     rows = workloads.input_rows(tables, 1)
     cols = R.to_cols(tables)
     R.execute(copy.deepcopy(plan), dict(cols))          # warm-up
-    t0 = time.perf_counter()
+    times = []
     for _ in range(runs):
+        t0 = time.perf_counter()
         R.execute(copy.deepcopy(plan), dict(cols))
-    dt = time.perf_counter() - t0
-    return {"value": rows * runs / dt, "unit": "input rows/s", "cores": torch.get_num_threads(),
-            "kind": "port",
-            "sample": f"{runs} full config-1 pipelines (7.5M input rows each) through the "
-                      f"torch-CPU restatement of the reference operators, {dt:.1f} s"}
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": rows / med, "unit": "input rows/s", "cores": torch.get_num_threads(),
+            "kind": "port", "os_cpu_count": ncpu, "torch_threads": torch.get_num_threads(),
+            "cpu_model": cpu_model(),
+            "sample": f"median of {runs} full config-1 pipelines (7.5M input rows each) through "
+                      f"the torch-CPU restatement of the reference operators, after one warm-up; "
+                      f"{sum(times):.1f} s in all, {torch.get_num_threads()} threads of "
+                      f"{ncpu} cores"}
 
 
 CONFIG_WORKLOADS = {

@@ -484,6 +484,25 @@ def test_config1_pipeline(device):
     _assert_col(gpu[301].tensor, cpu[301].tensor, "c1 top revenue", fp=True)
 
 
+@pytest.mark.parametrize("sync", [True, False])
+def test_config1_pipeline_under_inference_mode(device, sync):
+    """main.py:341-342 runs the plan inside torch.inference_mode(): the operators' outputs are
+    then inference tensors (no version counter), including the tables a scan would load there."""
+    from tqp_amd import executor, workloads
+    tables = workloads.config1_tables(n_build=200_000, n_probe=800_000)
+    plan = workloads.config1_plan()
+    cpu = R.to_cols(tables)
+    R.execute(copy.deepcopy(plan), cpu)
+    print(f"config1 under inference_mode, sync={sync}", flush=True)
+    with torch.inference_mode():
+        gpu = _gpu_group(tables, device)
+        executor.execute(copy.deepcopy(plan), gpu, sync=sync)
+        torch.cuda.synchronize()
+    assert gpu[301].tensor.is_inference()
+    _assert_col(gpu[101].tensor, cpu[101].tensor, "c1 top keys (inference)")
+    _assert_col(gpu[301].tensor, cpu[301].tensor, "c1 top revenue (inference)", fp=True)
+
+
 def test_config1_join_aggregate_all_groups(device):
     """Every group of the 264k-group aggregate (not only the top 10)."""
     from tqp_amd import workloads

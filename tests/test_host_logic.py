@@ -15,6 +15,25 @@ def test_value_bounds_follow_the_tensor_and_its_version():
     assert kernels.known_bounds(t) is None
 
 
+def test_value_bounds_skip_inference_tensors():
+    """main.py runs every query under torch.inference_mode(); tensors made there carry no
+    version counter, so no bounds are recorded or trusted for them (and no call raises)."""
+    with torch.inference_mode():
+        t = torch.arange(10)
+        kernels.note_bounds(t, 0, 9)
+        assert kernels.known_bounds(t) is None
+        t.add_(1)
+        assert kernels.known_bounds(t) is None
+    assert kernels.known_bounds(t) is None
+    u = torch.arange(4)                       # an ordinary tensor read inside inference mode
+    kernels.note_bounds(u, 0, 3)
+    with torch.inference_mode():
+        assert kernels.known_bounds(u) == (0, 3)
+        v = u[:2]                             # a view of an ordinary tensor stays versioned
+        kernels.note_bounds(v, 0, 1)
+        assert kernels.known_bounds(v) == (0, 1)
+
+
 def test_value_bounds_entry_dies_with_the_tensor():
     u = torch.arange(5)
     kernels.note_bounds(u, 0, 4)

@@ -179,11 +179,16 @@ def sort_perm(keys, n: int, limit=None, device=None):
 # Host-known value bounds of integer columns: a join writes its key outputs from keys whose
 # range its build already measured, so a group-by over those columns can plan without its own
 # bounds pass and synchronisation.  An entry is only trusted while the tensor's in-place
-# version counter is unchanged.
+# version counter is unchanged.  Tensors created under torch.inference_mode() (the reference's
+# main.py runs every query inside it) have no version counter, so an in-place write to them
+# could not be noticed: no bounds are recorded or trusted for them, and the consumers measure
+# the bounds on the device instead.
 _BOUNDS = {}   # id(tensor) -> (weak reference, version, lo, hi); tensors compare elementwise
 
 
 def note_bounds(t: torch.Tensor, lo: int, hi: int):
+    if t.is_inference():
+        return
     k = id(t)
     _BOUNDS[k] = (weakref.ref(t, lambda _r, k=k: _BOUNDS.pop(k, None)), t._version, int(lo), int(hi))
 
@@ -191,7 +196,7 @@ def note_bounds(t: torch.Tensor, lo: int, hi: int):
 def known_bounds(t: torch.Tensor):
     """(lo, hi) recorded for t, or None."""
     e = _BOUNDS.get(id(t))
-    if e is None or e[0]() is not t or e[1] != t._version:
+    if e is None or e[0]() is not t or t.is_inference() or e[1] != t._version:
         return None
     return e[2], e[3]
 
