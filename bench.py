@@ -77,7 +77,7 @@ def kernel_table(calls, launches):
 JOIN_AGG_ENTRIES = {"tq_join_build", "tq_join_build_ranged", "tq_join_count", "tq_join_write",
                     "tq_join_probe_deferred", "tq_groupby_bounds", "tq_groupby_run",
                     "tq_groupby_run_deferred", "tq_groupby_accumulate", "tq_groupby_count",
-                    "tq_groupby_write"}
+                    "tq_groupby_write", "tq_groupjoin_run"}
 
 
 def join_aggregate_span(calls, launches, steps, tables, pairs, groups):

@@ -462,6 +462,60 @@ int tq_groupby_run_deferred(const tq_groupby_desc_t* desc /* host */,
                             int64_t out_cap, const int64_t** ngroups_dev /* host out */,
                             const int64_t** err_dev /* host out */, void* stream);
 
+/* ---------------------------------------------------------------- join fused with group-by */
+/* An inner equi-join whose pairs feed only a group-by on the join key, with unique selected
+ * build keys, computed without materialising the pairs: every group is a selected build key
+ * that some selected probe row matches, and each matching probe row contributes its aggregate
+ * arguments (segment j of `prog`, evaluated over probe columns at the probe row) once.  Groups
+ * come out in ascending key order: out_key[g] = the key (in out_key_dtype), out_agg[j][g] as
+ * listed at TQ_AGG_*.  Three launches, no synchronisation.
+ *
+ * Key bounds: every build key (selected or not) lies in [key_lo, key_hi]; the slots are
+ * key - key_lo.  `pres` is a byte array of tq_groupjoin_pres_bytes(hi - lo + 1) bytes that the
+ * caller zeroes once and passes again with a new epoch (1..255) each call: a byte holds the
+ * epoch of the last call that saw its key, so it never needs clearing between calls; when the
+ * epochs are used up the caller zeroes it again.  Calls sharing a `pres` array must be ordered
+ * (one stream).
+ *
+ * *words (host out) receives the device address of int64 result words, valid once the stream
+ * has synchronised: [0] groups, [1] pairs (matching probe rows), [2] selected build rows, [3]
+ * distinct selected build keys, [4] error bits (1: a build key outside the bounds, 2: a
+ * look-back wait timed out, 4: more groups than out_cap).  The outputs are valid only when
+ * words[4] == 0 and words[2] == words[3] (unique keys) and words[1] > 0; otherwise the caller
+ * runs the join and the group-by separately.  Fails with TQ_EUNSUPPORTED when the key range
+ * needs more than 2048 reduce slices (2^12 keys each for one aggregate).
+ * Replaces operators/hashjoin.py:24-177 (join_kernel_new: sort, searchsorted, bucket expansion
+ * and gathers of the pair columns) followed by operators/aggregate.py:282-364 (agg_wrapper's
+ * hashed grouping and the per-aggregate bincount/scatter_reduce of aggregate.py:127-193) when
+ * the aggregate's only key is the join key. */
+typedef struct {
+  const void* bkey;          /* build key column (TQ_BOOL..TQ_I64), nb rows                  */
+  int32_t bkey_dtype;
+  const uint8_t* bmask;      /* selected build rows (NULL: all)                              */
+  int64_t nb;
+  const void* pkey;          /* probe key column, np rows                                    */
+  int32_t pkey_dtype;
+  const uint8_t* pmask;      /* selected probe rows (NULL: all)                              */
+  int64_t np;
+  int64_t key_lo, key_hi;    /* bounds of the build key column                               */
+  uint32_t epoch;            /* 1..255, see above                                            */
+  int32_t naggs;
+  int32_t agg_op[TQ_GB_MAX_AGGS];
+  int32_t agg_dtype[TQ_GB_MAX_AGGS];  /* dtype of each argument segment's value             */
+  void* out_key;
+  int32_t out_key_dtype;     /* TQ_BOOL..TQ_I64                                              */
+  void* out_agg[TQ_GB_MAX_AGGS];
+  int64_t out_cap;           /* rows of the outputs: min(hi - lo + 1, np) always suffices    */
+} tq_groupjoin_desc_t;
+
+size_t tq_groupjoin_pres_bytes(int64_t key_range);
+/* Workspace bytes (0 when the shape is unsupported); nload = prog->nload. */
+size_t tq_groupjoin_ws_bytes(int64_t nb, int64_t np, int64_t key_range, int32_t naggs,
+                             int32_t nload);
+int tq_groupjoin_run(const tq_groupjoin_desc_t* desc /* host */, const tq_program_t* prog /* host */,
+                     const tq_colset_t* cols /* host */, void* pres, size_t pres_bytes, void* ws,
+                     size_t ws_bytes, int64_t** words /* host out */, void* stream);
+
 /* ---------------------------------------------------------------- batched pipelines */
 /* Many filter -> join -> group-by pipelines in one persistent launch.  A join pipeline (mode 0)
  * joins its selected build rows (unique build keys) with its selected probe rows on key equality

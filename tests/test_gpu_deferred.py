@@ -2,7 +2,8 @@
 with sync=False): the results must equal the oracle's and the per-operator path's, whether the
 deferral holds (unique build keys inside their bounds), is settled early (a group-by mode that
 needs the row count on the host) or is voided and rerun (repeated build keys, keys outside the
-bounds the build was given, no pairs at all)."""
+bounds the build was given, no pairs at all).  These run the join and the group-by as separate
+operators (fuse=False); tests/test_gpu_groupjoin.py covers them fused."""
 import copy
 
 import numpy as np
@@ -55,7 +56,7 @@ def _run(tables, plan, device, stats=True, bounds=None):
     for cid, (lo, hi) in (bounds or {}).items():
         kernels.note_bounds(gpu[cid].tensor, lo, hi)
     print("deferred run:", [a['name'] for _, a in plan], flush=True)
-    n = executor.execute(copy.deepcopy(plan), gpu, sync=False)
+    n = executor.execute(copy.deepcopy(plan), gpu, sync=False, fuse=False)
     torch.cuda.synchronize()
     return gpu, n
 
@@ -122,7 +123,7 @@ def test_deferred_same_as_per_operator_path(device):
     plan = workloads.config1_plan()
     a = workloads.to_tensor_group(tables, device)
     b = workloads.to_tensor_group(tables, device)
-    na = executor.execute(copy.deepcopy(plan), a, sync=False)
+    na = executor.execute(copy.deepcopy(plan), a, sync=False, fuse=False)
     nb = executor.execute(copy.deepcopy(plan), b, sync=True)
     assert na == nb
     assert sorted(a) == sorted(b)
@@ -193,7 +194,7 @@ def test_deferred_without_pairs_matches_per_operator_path(device, monkeypatch):
     na = executor.execute(copy.deepcopy(plan), a, sync=True)
     seen = _Outcomes(monkeypatch)
     b = workloads.to_tensor_group(tables, device)
-    nb = executor.execute(copy.deepcopy(plan), b, sync=False)
+    nb = executor.execute(copy.deepcopy(plan), b, sync=False, fuse=False)
     assert seen.seen == ["voided"], seen.seen
     assert na == nb
     assert sorted(a) == sorted(b)

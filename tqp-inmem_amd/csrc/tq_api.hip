@@ -57,7 +57,10 @@ void timing_slot(const char* name, hipEvent_t* e0, hipEvent_t* e1) {
       return;
     }
     TimedLaunch t{nullptr, nullptr, nullptr};
-    if (hipEventCreate(&t.e0) != hipSuccess || hipEventCreate(&t.e1) != hipSuccess) {
+    // timing-only events: no system-scope fence when stamped (the log is read after a full
+    // device synchronisation), so a stamp adds no cache write-back to the kernel it brackets
+    if (hipEventCreateWithFlags(&t.e0, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&t.e1, hipEventDisableSystemFence) != hipSuccess) {
       ++g_dropped;
       return;
     }

@@ -1,0 +1,742 @@
+// Equi-join fused with the group-by of its output on the join key ("groupjoin").
+//
+// When a join's pairs only feed an aggregate that groups by the join key, and the build keys are
+// unique, every group is one build key and every pair one probe row: the aggregate is a
+// reduction of probe-row values into the build keys' slots.  Three launches, no pair columns, no
+// global atomics on the row path:
+//
+//  1. build: each selected build row stamps its key's byte, pres[key - lo] = epoch.  The bytes
+//     are never cleared: a call's epoch (1..255) differs from every stamp left since the byte
+//     array was last zeroed, so only this call's keys read as present.  Per 4096-row tile the
+//     selected rows are counted (and a key outside [lo, hi] is flagged in the count's top bit).
+//  2. probe: a workgroup takes a chunk of 8192 probe rows.  It loads every key and mask byte of
+//     the chunk, then tests every selected row's presence byte (all loads in flight before any
+//     is used), counts the matches per key slice (slices of 2^sb consecutive keys) and lists
+//     them in LDS; the slice counts' exclusive scan places the chunk's records, slice by slice,
+//     in the chunk's own region of the record array (no global scan, no atomics).  Then the
+//     matched rows' aggregate arguments are gathered and evaluated by the expression program,
+//     and each match becomes one record {slot, argument values...}.
+//  3. reduce: a workgroup per key slice gathers its records from every chunk (the chunks' slice
+//     offsets are a row of the offset table), reduces them into an LDS table, compacts the
+//     occupied slots in key order and writes the final key and aggregate columns at the offset
+//     it gets from the group counts of the slices before it (decoupled look-back with a
+//     wall-clock bound; a timeout sets the error word and the host takes the per-operator
+//     path).  Each slice also counts its keys' stamps, so the last slice, which reads every
+//     slice's look-back word, holds the distinct build keys and the pairs for the uniqueness
+//     check.
+//
+// Groups come out in ascending key order.  f64 sums are added in LDS in varying order.
+#include <map>
+#include <mutex>
+
+#include "tq_agg.h"
+#include "tq_common.h"
+#include "tq_expr.h"
+
+namespace tq {
+
+constexpr int kGjTiles = 2;                     // probe tiles per chunk
+constexpr int kGjChunk = kGjTiles * kTileRows;  // probe rows (and record capacity) per chunk
+constexpr int kGjMaxSlices = 2048;
+constexpr int kGjBatch = 1024;                  // chunks gathered per reduce round
+constexpr size_t kGjTableLds = 48 * 1024;       // reduce table: count + aggregates per slot
+
+// Words the host reads back (int64, in the workspace): groups, pairs (matched probe rows),
+// selected build rows, distinct selected build keys, error bits.
+enum { GJ_GROUPS = 0, GJ_PAIRS, GJ_NSEL, GJ_NSET, GJ_ERR, GJ_NWORDS = 8 };
+enum { GJ_ERR_BOUNDS = 1, GJ_ERR_WAIT = 2, GJ_ERR_CAP = 4 };
+constexpr uint32_t kOobBit = 0x80000000u;       // in a build tile count: a key outside [lo, hi]
+
+// A slice's look-back word: bit 0 published, then its groups, its stamped keys (each at most
+// 2^sb <= 4096 < 2^14) and its records (the pairs whose key lies in the slice).
+__host__ __device__ inline uint64_t lb_word(uint64_t groups, uint64_t keys, uint64_t recs) {
+  return 1ull | (groups << 1) | (keys << 15) | (recs << 29);
+}
+__device__ __forceinline__ uint64_t lb_groups(uint64_t w) { return (w >> 1) & 0x3FFF; }
+__device__ __forceinline__ uint64_t lb_keys(uint64_t w) { return (w >> 15) & 0x3FFF; }
+__device__ __forceinline__ uint64_t lb_recs(uint64_t w) { return w >> 29; }
+
+struct GjSpec {
+  int32_t naggs;
+  int32_t kind[TQ_GB_MAX_AGGS];
+  int32_t arg_dtype[TQ_GB_MAX_AGGS];
+  int32_t op[TQ_GB_MAX_AGGS];
+  uint64_t init[TQ_GB_MAX_AGGS];
+};
+
+struct GjOut {
+  void* key;
+  int32_t key_dtype;
+  void* agg[TQ_GB_MAX_AGGS];
+  int64_t cap;
+};
+
+// ------------------------------------------------------------------ 1. build
+// Workgroup 0 also zeroes the reduce's look-back words (flags[0..nflags)) and the words the
+// host reads back (the later kernels of the call only add to or OR into them).
+__global__ __launch_bounds__(kBlock) void k_gj_build(const void* __restrict__ bkeys, int dt,
+                                                     const uint8_t* __restrict__ bmask,
+                                                     int64_t nb, int64_t lo, uint64_t range,
+                                                     uint8_t* __restrict__ pres, uint32_t epoch,
+                                                     uint32_t* __restrict__ tile_counts,
+                                                     uint64_t* __restrict__ flags, int nflags,
+                                                     int64_t* __restrict__ words) {
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0) {
+    for (int i = t; i < nflags; i += kBlock) flags[i] = 0;
+    if (t < GJ_NWORDS) words[t] = 0;
+  }
+  const int64_t tile = blockIdx.x;
+  const int64_t tile0 = tile * kTileRows;
+  if (tile0 >= nb) return;  // nb == 0: the launch only clears the words
+  uint64_t raw[kItems];
+  load_raw16(bkeys, dt, tile0, nb, raw);
+  const uint32_t sel = load_sel16(bmask, tile0, nb);
+  bool oob = false;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    if (!((sel >> j) & 1u)) continue;
+    const uint64_t off = raw[j] - (uint64_t)lo;  // below lo wraps to a huge offset
+    if (off < range)
+      pres[off] = (uint8_t)epoch;
+    else
+      oob = true;
+  }
+  __shared__ uint32_t wc[kWaves];
+  const uint32_t c = wave_sum((uint32_t)__popc(sel)) | (ballot(oob) ? kOobBit : 0u);
+  if (lane_id() == 0) wc[t >> 6] = c;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t n = 0, f = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) {
+      n += wc[i] & ~kOobBit;
+      f |= wc[i] & kOobBit;
+    }
+    tile_counts[tile] = n | f;
+  }
+}
+
+// ------------------------------------------------------------------ 2. probe
+// Gather the program's load slots at rows[j] (live rows only) into the stage, in the layout
+// stage_loads uses, so run_rows can evaluate them.
+__device__ __forceinline__ void gather_loads(const tq_program_t& P, const tq_colset_t& C,
+                                             const int64_t (&rows)[kSub],
+                                             const bool (&live)[kSub],
+                                             uint64_t* __restrict__ stage) {
+  const int t = threadIdx.x;
+  for (int s = 0; s < P.nload; ++s) {
+    const void* col = col_ptr(C, P.load_col[s]);
+    const int dt = P.load_dtype[s];
+    uint64_t v[kSub];
+#define TQ_GATHER(T, EXPR)                                 \
+  {                                                        \
+    const T* p = (const T*)col;                            \
+    _Pragma("unroll") for (int j = 0; j < kSub; ++j) {     \
+      const int64_t r = rows[j];                           \
+      v[j] = live[j] ? (EXPR) : 0;                         \
+    }                                                      \
+  }
+    switch (dt) {
+      case TQ_BOOL: TQ_GATHER(uint8_t, (uint64_t)(p[r] != 0)) break;
+      case TQ_I8: TQ_GATHER(int8_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_I16: TQ_GATHER(int16_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_I32: TQ_GATHER(int32_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_F32: TQ_GATHER(uint32_t, (uint64_t)p[r]) break;
+      default: TQ_GATHER(uint64_t, p[r]) break;
+    }
+#undef TQ_GATHER
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) stage[(s * kSub + j) * kBlock + t] = v[j];
+  }
+}
+
+// Dynamic LDS: stage (nload * kSubRows u64), hist[S] u32, the chunk's matches in slice order
+// u32[kGjChunk] as (local row << sb) | slot within the slice.
+// loff[s * W + c] = first record of slice s in chunk c's region (s < S), loff[S * W + c] =
+// the chunk's matches.  A record is 1 + naggs words: {slot within the slice, accumulator-domain
+// values}; a chunk's records are written in slice order by consecutive threads.
+__global__ __launch_bounds__(kBlock) void k_gj_probe(
+    GjSpec G, tq_program_t P, tq_colset_t C, const void* __restrict__ pkeys, int pdt,
+    const uint8_t* __restrict__ pmask, int64_t np, int64_t lo, uint64_t range,
+    const uint8_t* __restrict__ pres, uint32_t epoch, int sb, int S, int64_t W,
+    uint32_t* __restrict__ loff, uint64_t* __restrict__ rec, int32_t stage_words) {
+  extern __shared__ uint64_t smem[];
+  uint64_t* stage = smem;
+  uint32_t* hist = (uint32_t*)(smem + stage_words);
+  uint32_t* srt = hist + ((S + 1) & ~1);
+  __shared__ uint32_t sh[kWaves];
+  const int t = threadIdx.x;
+  const int64_t c = blockIdx.x;
+  const int64_t base = c * kGjChunk;
+  const uint32_t smask = (1u << sb) - 1u;
+  for (int i = t; i < S; i += kBlock) hist[i] = 0;
+  // every key and mask byte of the chunk, then every presence byte, before any is used
+  uint64_t raw[kGjTiles][kItems];
+  uint32_t sel[kGjTiles];
+#pragma unroll
+  for (int q = 0; q < kGjTiles; ++q) {
+    const int64_t tile0 = base + (int64_t)q * kTileRows;
+    load_raw16(pkeys, pdt, tile0, np, raw[q]);  // rows past np read row 0 (unselected)
+    sel[q] = load_sel16(pmask, tile0, np);
+  }
+  uint32_t hit[kGjTiles];
+#pragma unroll
+  for (int q = 0; q < kGjTiles; ++q) {
+    uint8_t b[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const uint64_t off = raw[q][j] - (uint64_t)lo;
+      const bool ok = ((sel[q] >> j) & 1u) && off < range;
+      b[j] = ok ? pres[off] : (uint8_t)0;
+    }
+    hit[q] = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) hit[q] |= (b[j] == (uint8_t)epoch ? 1u : 0u) << j;
+    hit[q] &= sel[q];
+  }
+  __syncthreads();  // hist cleared
+#pragma unroll
+  for (int q = 0; q < kGjTiles; ++q)
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      if ((hit[q] >> j) & 1u) atomicAdd(&hist[(raw[q][j] - (uint64_t)lo) >> sb], 1u);
+  __syncthreads();
+  // exclusive scan of the slice counts (in place: hist becomes the slices' write cursors)
+  uint32_t M;
+  {
+    constexpr int kPer = kGjMaxSlices / kBlock;
+    uint32_t v[kPer], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = t * kPer + k;
+      v[k] = i < S ? hist[i] : 0u;
+      sum += v[k];
+    }
+    uint32_t run = block_exclusive_sum(sum, sh, &M);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = t * kPer + k;
+      if (i < S) {
+        hist[i] = run;
+        loff[(int64_t)i * W + c] = run;
+      }
+      run += v[k];
+    }
+    if (t == 0) loff[(int64_t)S * W + c] = M;
+  }
+  __syncthreads();
+  // the matches in slice order
+#pragma unroll
+  for (int q = 0; q < kGjTiles; ++q)
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+      if ((hit[q] >> j) & 1u) {
+        const uint32_t slot = (uint32_t)(raw[q][j] - (uint64_t)lo);
+        const uint32_t lrow = (uint32_t)(q * kTileRows + (j >> 2) * (4 * kBlock) + 4 * t + (j & 3));
+        srt[atomicAdd(&hist[slot >> sb], 1u)] = (lrow << sb) | (slot & smask);
+      }
+  __syncthreads();
+  const int rw = 1 + G.naggs;
+  uint64_t* crec = rec + base * rw;
+  for (uint32_t e0 = 0; e0 < M; e0 += kSubRows) {
+    bool live[kSub];
+    int64_t rows[kSub];
+    uint32_t slot[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      const uint32_t e = e0 + j * kBlock + t;
+      live[j] = e < M;
+      const uint32_t v = live[j] ? srt[e] : 0u;
+      rows[j] = live[j] ? base + (v >> sb) : 0;
+      slot[j] = v & smask;
+    }
+    gather_loads(P, C, rows, live, stage);
+    RegsN R{};
+    int begin = 0;
+    for (int a = 0; a < G.naggs; ++a) {
+      const int end = P.seg_end[a];
+      run_rows(P, C, begin, end, rows, live, stage, R);
+      begin = end;
+      const V4 V = R.get(P.seg_result[a]);
+#pragma unroll
+      for (int j = 0; j < kSub; ++j) {
+        if (!live[j]) continue;
+        const uint64_t v = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+        const int64_t e = e0 + j * kBlock + t;
+        if (G.naggs == 1)
+          *(ulonglong2*)(crec + e * 2) = make_ulonglong2(slot[j], v);
+        else
+          crec[e * rw + 1 + a] = v;
+      }
+    }
+    if (G.naggs != 1) {
+#pragma unroll
+      for (int j = 0; j < kSub; ++j)
+        if (live[j]) crec[(int64_t)(e0 + j * kBlock + t) * rw] = slot[j];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ 3. reduce
+__device__ __forceinline__ uint64_t gj_ld(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gj_st(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct GjCheck {
+  const uint8_t* pres;      // presence bytes of this call's keys
+  uint32_t epoch;
+  uint64_t range;
+  const uint32_t* btile;    // build tile counts (top bit: a key outside the bounds)
+  int64_t nbt;
+  int64_t* words;           // GJ_* words
+  int64_t spin_ticks;       // look-back bound in wall-clock ticks
+};
+
+// Stamped keys of slice s: presence bytes [s << sb, (s + 1) << sb) that hold the epoch.  A
+// thread tests 16 consecutive bytes with one load (the byte array is padded to 256 bytes).
+__device__ __forceinline__ uint32_t slice_keys(const GjCheck& K, int s, int sb) {
+  const int ps = 1 << sb;
+  const uint64_t first = (uint64_t)s << sb;
+  const uint32_t rep = K.epoch * 0x01010101u;
+  uint32_t c = 0;
+  for (int i = threadIdx.x * 16; i < ps; i += kBlock * 16) {
+    if (first + (uint64_t)i >= K.range) break;
+    const uint4 q = *(const uint4*)(K.pres + first + i);
+    const uint32_t d[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t x = d[k] ^ rep;  // bytes equal to the epoch become zero
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        c += (((x >> (8 * b)) & 0xFFu) == 0u && first + i + 4 * k + b < K.range) ? 1u : 0u;
+    }
+  }
+  return c;
+}
+
+// Dynamic LDS: acc[naggs][ps] u64, cnt[ps] u32.  Static: the current batch of chunks.
+// With xper > 0 the grid is 8 * xper workgroups and workgroup b (which the dispatcher places on
+// XCD b % 8) reduces slice (b % 8) * xper + b / 8: consecutive slices run on one XCD, so the
+// record lines one slice fetches beyond its own run (the neighbouring slices' records of the same
+// chunk) are found in that XCD's L2 by the next slice.  The host takes this mapping only when
+// every workgroup of the grid is resident at once, which the look-back (a slice waits for every
+// slice below it, whatever its workgroup index) then needs.
+__global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, int64_t W,
+                                                      const uint32_t* __restrict__ loff,
+                                                      const uint64_t* __restrict__ rec,
+                                                      int64_t lo, GjOut O,
+                                                      uint64_t* __restrict__ flags, GjCheck K,
+                                                      int xper) {
+  extern __shared__ uint64_t smem[];
+  __shared__ uint32_t pre[kGjBatch + 1];
+  __shared__ uint32_t st[kGjBatch];
+  __shared__ uint32_t sh[kWaves];
+  __shared__ uint32_t wocc[kItems][kWaves];
+  __shared__ uint32_t s_keys;
+  __shared__ int64_t s_off;
+  const int ps = 1 << sb;
+  uint64_t* lacc = smem;
+  uint32_t* lcnt = (uint32_t*)(smem + (int64_t)G.naggs * ps);
+  const int t = threadIdx.x;
+  const int s = xper > 0 ? (int)(blockIdx.x % 8) * xper + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  if (s >= S) return;  // the remapped grid's unused places
+  const int rw = 1 + G.naggs;
+  for (int i = t; i < ps; i += kBlock) {
+    lcnt[i] = 0;
+    for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
+  }
+  if (t == 0) s_keys = 0;
+  __syncthreads();
+  {
+    const uint32_t kc = wave_sum(slice_keys(K, s, sb));
+    if (lane_id() == 0 && kc) atomicAdd(&s_keys, kc);
+  }
+  const uint64_t smask = (uint64_t)ps - 1;
+  uint32_t nrec = 0;  // this slice's records (uniform)
+  for (int64_t wb = 0; wb < W; wb += kGjBatch) {
+    const int nbw = (int)(W - wb < kGjBatch ? W - wb : kGjBatch);
+    // this batch's chunks: first record of slice s and how many
+    constexpr int kPer = kGjBatch / kBlock;
+    uint32_t cnt[kPer], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = t * kPer + k;
+      cnt[k] = 0;
+      if (i < nbw) {
+        const uint32_t a = loff[(int64_t)s * W + wb + i];
+        const uint32_t b = loff[(int64_t)(s + 1) * W + wb + i];
+        st[i] = a;
+        cnt[k] = b - a;
+      }
+      sum += cnt[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_sum(sum, sh, &tot);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = t * kPer + k;
+      if (i < nbw) pre[i] = run;
+      run += cnt[k];
+    }
+    if (t == 0) pre[nbw] = tot;
+    nrec += tot;
+    __syncthreads();
+    constexpr int U = 8;  // records per thread whose loads are issued together
+    for (uint32_t i0 = t; i0 < tot; i0 += U * kBlock) {
+      int64_t at[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t idx = i0 + u * kBlock;
+        ok[u] = idx < tot;
+        // the chunk holding record idx: the last i with pre[i] <= idx
+        int lo_i = 0, hi_i = nbw;  // pre[lo_i] <= idx < pre[hi_i]
+        while (hi_i - lo_i > 1) {
+          const int mid = (lo_i + hi_i) >> 1;
+          if (pre[mid] <= idx) lo_i = mid; else hi_i = mid;
+        }
+        at[u] = ok[u] ? ((wb + lo_i) * kGjChunk + st[lo_i] + (idx - pre[lo_i])) : 0;
+      }
+      if (G.naggs == 1) {
+        ulonglong2 x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          x[u] = ok[u] ? *(const ulonglong2*)(rec + at[u] * 2) : make_ulonglong2(0, 0);
+        const int kind = G.kind[0];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!ok[u]) continue;
+          const uint32_t sl = (uint32_t)(x[u].x & smask);
+          atomicAdd(&lcnt[sl], 1u);
+          acc_apply(&lacc[sl], kind, x[u].y);
+        }
+      } else {
+        uint32_t sl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sl[u] = ok[u] ? (uint32_t)(rec[at[u] * rw] & smask) : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (ok[u]) atomicAdd(&lcnt[sl[u]], 1u);
+        for (int a = 0; a < G.naggs; ++a) {
+          uint64_t v[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) v[u] = ok[u] ? rec[at[u] * rw + 1 + a] : 0;
+          const int kind = G.kind[a];
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (ok[u]) acc_apply(&lacc[a * ps + sl[u]], kind, v[u]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // occupied slots in slot order: slot i = j*256 + t, ranked by wave ballots
+  const int w = t >> 6;
+  uint64_t bal[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int i = j * kBlock + t;
+    bal[j] = ballot(i < ps && lcnt[i] > 0);
+    if (lane_id() == 0) wocc[j][w] = (uint32_t)__popcll(bal[j]);
+  }
+  __syncthreads();
+  // publish this slice's counts, then add up those of the slices below it, 256 per round (every
+  // slice waited on belongs to a workgroup that is resident or done)
+  uint64_t own = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j)
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) own += wocc[j][v];
+  if (t == 0) gj_st(flags + s, lb_word(own, s_keys, nrec));
+  {
+    uint64_t before = 0, keys = 0, recs = 0;
+    bool okw = true;
+    const int64_t t0 = wall_clock64();
+    for (int q0 = 0; q0 < s && okw; q0 += kBlock) {
+      const int q = q0 + t;
+      uint64_t f = 1ull;  // threads past s - 1 add nothing
+      while (true) {
+        if (q < s) f = gj_ld(flags + q);
+        if (ballot(f == 0) == 0) break;
+        if (wall_clock64() - t0 > K.spin_ticks) {
+          okw = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (q < s) {
+        before += lb_groups(f);
+        keys += lb_keys(f);
+        recs += lb_recs(f);
+      }
+    }
+    __shared__ uint64_t lbs[3][kWaves];
+    __shared__ uint32_t lbok[kWaves];
+    before = wave_sum(before);
+    keys = wave_sum(keys);
+    recs = wave_sum(recs);
+    if (lane_id() == 0) {
+      lbs[0][w] = before;
+      lbs[1][w] = keys;
+      lbs[2][w] = recs;
+      lbok[w] = okw ? 1u : 0u;
+    }
+    __syncthreads();
+    if (t == 0) {
+      uint64_t b = 0, k = 0, r = 0;
+      bool ok = true;
+#pragma unroll
+      for (int v = 0; v < kWaves; ++v) {
+        b += lbs[0][v];
+        k += lbs[1][v];
+        r += lbs[2][v];
+        ok = ok && lbok[v];
+      }
+      if (!ok) atomicOr((unsigned long long*)&K.words[GJ_ERR], (unsigned long long)GJ_ERR_WAIT);
+      if (s == S - 1) {
+        K.words[GJ_GROUPS] = (int64_t)(b + own);
+        K.words[GJ_NSET] = (int64_t)(k + s_keys);
+        K.words[GJ_PAIRS] = (int64_t)(r + nrec);
+      }
+      s_off = ok ? (int64_t)b : -1;
+    }
+  }
+  __syncthreads();
+  const int64_t obase = s_off;
+  const int64_t kbase = lo + ((int64_t)s << sb);
+  uint32_t run = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      const uint32_t cc = wocc[j][v];
+      tot += cc;
+      before += (v < w) ? cc : 0u;
+    }
+    if ((bal[j] >> lane_id()) & 1u) {
+      const int i = j * kBlock + t;
+      const int64_t o = obase + run + before + rank_below(bal[j]);
+      if (obase >= 0 && o < O.cap) {
+        store_int(O.key, O.key_dtype, o, kbase + i);
+        for (int a = 0; a < G.naggs; ++a)
+          write_agg(O.agg[a], o, G.op[a], G.kind[a], G.arg_dtype[a], lacc[a * ps + i], lcnt[i]);
+      } else if (obase >= 0) {  // more groups than the outputs hold
+        atomicOr((unsigned long long*)&K.words[GJ_ERR], (unsigned long long)GJ_ERR_CAP);
+      }
+    }
+    run += tot;
+  }
+  if (s == S - 1) {
+    // the last slice (which no workgroup waits on) also sums the build's tile counts: the
+    // selected build rows, and whether a key fell outside the bounds
+    int64_t nsel = 0;
+    uint32_t oob = 0;
+    for (int64_t i = t; i < K.nbt; i += kBlock) {
+      const uint32_t v = K.btile[i];
+      nsel += v & ~kOobBit;
+      oob |= v & kOobBit;
+    }
+    nsel = wave_sum(nsel);
+    const bool any_oob = ballot(oob != 0) != 0;
+    __shared__ int64_t red[kWaves];
+    __shared__ uint32_t roob[kWaves];
+    if (lane_id() == 0) {
+      red[w] = nsel;
+      roob[w] = any_oob ? 1u : 0u;
+    }
+    __syncthreads();
+    if (t == 0) {
+      int64_t a = 0;
+      uint32_t e = 0;
+#pragma unroll
+      for (int i = 0; i < kWaves; ++i) {
+        a += red[i];
+        e |= roob[i];
+      }
+      K.words[GJ_NSEL] = a;
+      if (e) atomicOr((unsigned long long*)&K.words[GJ_ERR], (unsigned long long)GJ_ERR_BOUNDS);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host side
+struct GjPlan {
+  int sb, S;
+  int64_t range, nbt, W;
+  int32_t stage_words;
+  size_t off_btile, off_loff, off_flags, off_words, off_rec, bytes;
+};
+
+static int gj_plan(int64_t nb, int64_t np, int64_t range, int32_t naggs, int32_t nload,
+                   GjPlan* p) {
+  if (naggs < 1 || naggs > TQ_GB_MAX_AGGS) return set_error(TQ_EINVAL, "groupjoin: %d aggregates", naggs);
+  if (range < 1) return set_error(TQ_EINVAL, "groupjoin: empty key range");
+  const size_t per = 4 + 8 * (size_t)naggs;
+  int sb = 0;
+  while (((size_t)2 << sb) * per <= kGjTableLds && ((int64_t)2 << sb) <= kTileRows) ++sb;
+  const int64_t S = ceil_div(range, (int64_t)1 << sb);
+  if (S > kGjMaxSlices)
+    return set_error(TQ_EUNSUPPORTED, "groupjoin: key range %lld needs %lld slices (at most %d)",
+                     (long long)range, (long long)S, kGjMaxSlices);
+  if (np > ((int64_t)1 << 31) - 1 || nb > ((int64_t)1 << 31) - 1)
+    return set_error(TQ_EUNSUPPORTED, "groupjoin: more than 2^31 - 1 rows");
+  p->sb = sb;
+  p->S = (int)S;
+  p->range = range;
+  p->nbt = ceil_div(nb > 0 ? nb : 1, kTileRows);
+  p->W = ceil_div(np > 0 ? np : 1, kGjChunk);
+  p->stage_words = (int32_t)(nload > 0 ? nload : 1) * kSubRows;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = align_up(o + bytes, 256);
+    return at;
+  };
+  p->off_words = take(GJ_NWORDS * sizeof(int64_t));
+  p->off_flags = take((size_t)S * sizeof(uint64_t));
+  p->off_btile = take((size_t)p->nbt * sizeof(uint32_t));
+  p->off_loff = take((size_t)(S + 1) * p->W * sizeof(uint32_t));
+  p->off_rec = take((size_t)p->W * kGjChunk * (1 + naggs) * sizeof(uint64_t));
+  p->bytes = o;
+  return TQ_OK;
+}
+
+static int gj_check_dtype(int dt, const char* what) {
+  if (dt < TQ_BOOL || dt > TQ_I64)
+    return set_error(TQ_EUNSUPPORTED, "groupjoin: %s must be an integer column", what);
+  return TQ_OK;
+}
+
+template <typename... P, typename... A>
+static int gj_launch_lds(const char* name, void (*kernel)(P...), unsigned grid, size_t lds,
+                         hipStream_t s, A&&... a) {
+  if (lds > (size_t)kLdsBig) return set_error(TQ_ECAPACITY, "%s: %zu bytes of LDS", name, lds);
+  if (lds > 64 * 1024) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, bool> allowed;
+    int dev = 0;
+    TQ_TRY_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    bool& have = allowed[{dev, (const void*)kernel}];
+    if (!have) {
+      const hipError_t e = hipFuncSetAttribute(
+          (const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig);
+      if (e != hipSuccess)
+        return set_error(TQ_EHIP, "%s: %zu bytes of LDS: %s", name, lds, hipGetErrorString(e));
+      have = true;
+    }
+  }
+  return launch(name, kernel, grid, kBlock, lds, s, std::forward<A>(a)...);
+}
+
+static int64_t gj_spin_ticks() {
+  // a look-back wait gives up after 100 ms of wall-clock time
+  static int64_t ticks = 0;
+  if (ticks == 0) {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0) {
+      (void)hipGetLastError();
+      khz = 100000;  // 100 MHz
+    }
+    ticks = (int64_t)khz * 100;
+  }
+  return ticks;
+}
+
+}  // namespace tq
+
+using namespace tq;
+
+extern "C" size_t tq_groupjoin_ws_bytes(int64_t nb, int64_t np, int64_t key_range, int32_t naggs,
+                                        int32_t nload) {
+  GjPlan p;
+  if (gj_plan(nb, np, key_range, naggs, nload, &p) != TQ_OK) return 0;
+  return p.bytes;
+}
+
+extern "C" size_t tq_groupjoin_pres_bytes(int64_t key_range) {
+  return align_up((size_t)(key_range > 0 ? key_range : 1), 256);
+}
+
+extern "C" int tq_groupjoin_run(const tq_groupjoin_desc_t* d, const tq_program_t* prog,
+                                const tq_colset_t* cols, void* pres, size_t pres_bytes, void* ws,
+                                size_t ws_bytes, int64_t** words, void* stream) {
+  if (!d || !prog || !cols || !words) return set_error(TQ_EINVAL, "groupjoin: null argument");
+  if (d->nb < 0 || d->np < 0) return set_error(TQ_EINVAL, "groupjoin: negative row count");
+  if ((d->nb > 0 && !d->bkey) || (d->np > 0 && !d->pkey))
+    return set_error(TQ_EINVAL, "groupjoin: null key column");
+  TQ_TRY(gj_check_dtype(d->bkey_dtype, "the build key"));
+  TQ_TRY(gj_check_dtype(d->pkey_dtype, "the probe key"));
+  TQ_TRY(gj_check_dtype(d->out_key_dtype, "the group key output"));
+  if (d->key_hi < d->key_lo) return set_error(TQ_EINVAL, "groupjoin: key bounds hi < lo");
+  const uint64_t urange = (uint64_t)d->key_hi - (uint64_t)d->key_lo + 1;
+  if (urange == 0 || urange > ((uint64_t)1 << 40))
+    return set_error(TQ_EUNSUPPORTED, "groupjoin: key range too wide");
+  const int64_t range = (int64_t)urange;
+  if (prog->nseg != d->naggs)
+    return set_error(TQ_EINVAL, "groupjoin: %d program segments for %d aggregates", prog->nseg,
+                     d->naggs);
+  if (d->epoch < 1 || d->epoch > 255) return set_error(TQ_EINVAL, "groupjoin: epoch %u", d->epoch);
+  GjPlan p;
+  TQ_TRY(gj_plan(d->nb, d->np, range, d->naggs, prog->nload, &p));
+  if (!ws || ws_bytes < p.bytes)
+    return set_error(TQ_ECAPACITY, "groupjoin: workspace of %zu bytes, %zu needed", ws_bytes, p.bytes);
+  if (!pres || pres_bytes < tq_groupjoin_pres_bytes(range))
+    return set_error(TQ_ECAPACITY, "groupjoin: presence array of %zu bytes, %zu needed",
+                     pres_bytes, tq_groupjoin_pres_bytes(range));
+  if (d->out_cap > 0 && !d->out_key) return set_error(TQ_EINVAL, "groupjoin: null key output");
+  GjSpec G{};
+  GjOut O{};
+  G.naggs = d->naggs;
+  for (int a = 0; a < d->naggs; ++a) {
+    TQ_TRY(agg_kind_init(d->agg_op[a], d->agg_dtype[a], &G.kind[a], &G.init[a]));
+    G.arg_dtype[a] = d->agg_dtype[a];
+    G.op[a] = d->agg_op[a];
+    O.agg[a] = d->out_agg[a];
+    if (d->out_cap > 0 && !O.agg[a]) return set_error(TQ_EINVAL, "groupjoin: null aggregate output");
+  }
+  O.key = d->out_key;
+  O.key_dtype = d->out_key_dtype;
+  O.cap = d->out_cap;
+  hipStream_t s = (hipStream_t)stream;
+  char* w = (char*)ws;
+  int64_t* wd = (int64_t*)(w + p.off_words);
+  uint64_t* flags = (uint64_t*)(w + p.off_flags);
+  uint32_t* btile = (uint32_t*)(w + p.off_btile);
+  uint32_t* loff = (uint32_t*)(w + p.off_loff);
+  uint64_t* rec = (uint64_t*)(w + p.off_rec);
+  *words = wd;
+  const unsigned bgrid = (unsigned)(d->nb > 0 ? p.nbt : 1);
+  TQ_TRY(launch("k_gj_build", k_gj_build, bgrid, kBlock, 0, s, d->bkey, (int)d->bkey_dtype,
+                d->bmask, d->nb, d->key_lo, (uint64_t)range, (uint8_t*)pres, (uint32_t)d->epoch,
+                btile, flags, p.S, wd));
+  if (d->np > 0) {
+    const size_t lds = (size_t)p.stage_words * 8 + (size_t)((p.S + 1) & ~1) * 4 +
+                       (size_t)kGjChunk * 4;
+    TQ_TRY(gj_launch_lds("k_gj_probe", k_gj_probe, (unsigned)p.W, lds, s, G, *prog, *cols,
+                         d->pkey, (int)d->pkey_dtype, d->pmask, d->np, d->key_lo,
+                         (uint64_t)range, (const uint8_t*)pres, (uint32_t)d->epoch, p.sb, p.S,
+                         p.W, loff, rec, p.stage_words));
+  } else {
+    // no probe rows: every chunk's slice offsets are zero
+    TQ_TRY(fill_u32(loff, 0u, (int64_t)(p.S + 1) * p.W, s));
+  }
+  GjCheck K{(const uint8_t*)pres, (uint32_t)d->epoch, (uint64_t)range, btile,
+            d->nb > 0 ? p.nbt : 0, wd, gj_spin_ticks()};
+  const size_t rlds = ((size_t)8 * d->naggs + 4) << p.sb;
+  // consecutive slices on one XCD when the remapped grid is resident all at once
+  const int xper = (int)ceil_div(p.S, 8);
+  const bool remap = p.S > 8 && rlds <= 64 * 1024 &&
+                     (int64_t)8 * xper <= resident_blocks((const void*)k_gj_reduce, rlds);
+  const unsigned rgrid = remap ? (unsigned)(8 * xper) : (unsigned)p.S;
+  return gj_launch_lds("k_gj_reduce", k_gj_reduce, rgrid, rlds, s, G, p.sb, p.S, p.W,
+                       (const uint32_t*)loff, (const uint64_t*)rec, d->key_lo, O, flags, K,
+                       remap ? xper : 0);
+}

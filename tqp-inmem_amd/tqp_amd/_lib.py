@@ -113,6 +113,22 @@ class GroupByDesc(ctypes.Structure):
                 ("key_dict_range", ctypes.c_int64 * TQ_GB_MAX_KEYS)]
 
 
+class GroupJoinDesc(ctypes.Structure):
+    _fields_ = [("bkey", ctypes.c_void_p), ("bkey_dtype", ctypes.c_int32),
+                ("bmask", ctypes.c_void_p), ("nb", ctypes.c_int64),
+                ("pkey", ctypes.c_void_p), ("pkey_dtype", ctypes.c_int32),
+                ("pmask", ctypes.c_void_p), ("np", ctypes.c_int64),
+                ("key_lo", ctypes.c_int64), ("key_hi", ctypes.c_int64),
+                ("epoch", ctypes.c_uint32), ("naggs", ctypes.c_int32),
+                ("agg_op", ctypes.c_int32 * TQ_MAX_SEGS),
+                ("agg_dtype", ctypes.c_int32 * TQ_MAX_SEGS),
+                ("out_key", ctypes.c_void_p), ("out_key_dtype", ctypes.c_int32),
+                ("out_agg", ctypes.c_void_p * TQ_MAX_SEGS), ("out_cap", ctypes.c_int64)]
+
+
+GJ_GROUPS, GJ_PAIRS, GJ_NSEL, GJ_NSET, GJ_ERR = range(5)
+
+
 # ---------------------------------------------------------------- loading
 _lib = None
 
@@ -155,6 +171,9 @@ _SIGNATURES = {
                                I32),
     "tq_join_settle": ([P, P, P], I32),
     "tq_read_words": ([I32, P, P, P, P], I32),
+    "tq_groupjoin_pres_bytes": ([I64], SZ),
+    "tq_groupjoin_ws_bytes": ([I64, I64, I64, I32, I32], SZ),
+    "tq_groupjoin_run": ([P, P, P, P, SZ, P, SZ, P, P], I32),
     "tq_batch_ws_bytes": ([I32], SZ),
     "tq_batch_run": ([P, I32, P, P, I32, I32, I32, I32, P, ctypes.c_uint32, P], I32),
 }

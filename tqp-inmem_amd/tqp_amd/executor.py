@@ -12,7 +12,7 @@ import time
 
 import torch
 
-from . import kernels
+from . import groupjoin, kernels
 from .operators.aggregate import result_ids, results_are_direct, tqp_hash_aggregate
 from .operators.filter import tqp_filter
 from .operators.hashjoin import join_vortex
@@ -136,6 +136,84 @@ def _run_deferred(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu_
     return num_rows
 
 
+def _run_groupjoin(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu_enable,
+                   op_times):
+    """plan[i:i + count] (Join, Aggregate[, take ordered]) with the join and the aggregate fused
+    (groupjoin.py) when the aggregate groups by the join key alone; returns None, with nothing
+    changed, when they cannot be fused.  The counts are read back once at the end; a void
+    fused result (repeated or out-of-bounds build keys, no pairs, a look-back that gave up)
+    reruns the nodes per operator from the state before the join."""
+    (uj, ja), (ua, aa) = plan[i], plan[i + 1]
+    p = groupjoin.plan(tensor_group, ja, aa)
+    if p is None:
+        return None
+    saved_group = dict(tensor_group)
+    saved_mask_map = dict(ja['mask map']) if 'mask map' in ja else None
+    saved_ids = list(aa['input ids'])
+
+    def run_join(tg, args):
+        return _run_join(tg, args, uj, cpu_mem_pool, gpu_mem_pool, gpu_enable)
+
+    t0 = time.perf_counter()
+    try:
+        df = groupjoin.run(tensor_group, ja, aa, p, run_join)
+    except kernels.DeferredJoinFailed:
+        return None
+    times = [time.perf_counter() - t0]
+    after_agg = dict(tensor_group)
+    ok = True
+    rows_sorted = None
+    try:
+        if count == 3:
+            us, sa = plan[i + 2]
+            t1 = time.perf_counter()
+            kernels.set_deferred(df)
+            try:
+                rows_sorted = tqp_sort(gpu_enable, tensor_group, sa, with_limit=True,
+                                       name=f"sort lim {us}")
+            finally:
+                kernels.set_deferred(None)
+            times.append(time.perf_counter() - t1)
+        t2 = time.perf_counter()
+        m, g = df.resolve()
+        times[-1] += time.perf_counter() - t2
+    except kernels.DeferredJoinFailed:
+        ok = False
+    nodes = plan[i:i + count]
+    if ok:
+        for v in df.lazy:
+            v.rows = m
+        for k, v in tensor_group.items():
+            if after_agg.get(k) is v and k in df.written:
+                v.tensor = v.tensor[:g]                       # written by the aggregate
+            elif count == 3 and saved_group.get(k) is not v and after_agg.get(k) is not v \
+                    and df.sorted is not None and v.tensor.shape[0] == df.sorted["cap"]:
+                v.tensor = v.tensor[:min(g, df.sorted["limit"])]   # written by the top-k
+        if count == 3:
+            num_rows = g if rows_sorted == df.group["cap"] else rows_sorted
+        else:
+            first = next(iter(tensor_group.values()))
+            num_rows = first.rows if isinstance(first, groupjoin.LazyJoinColumn) \
+                and not first.materialized else first.tensor.size(0)
+        if op_times is not None:
+            op_times.setdefault(f"Join+Aggregate {uj}+{ua}", []).append(times[0])
+            if count == 3:
+                op_times.setdefault(f"take ordered {plan[i + 2][0]}", []).append(times[1])
+        return num_rows
+    tensor_group.clear()
+    tensor_group.update(saved_group)
+    if saved_mask_map is not None:
+        ja['mask map'] = saved_mask_map
+    aa['input ids'][:] = saved_ids
+    num_rows = None
+    for u, args in nodes:
+        t0 = time.perf_counter()
+        num_rows = _run_node(tensor_group, u, args, cpu_mem_pool, gpu_mem_pool, gpu_enable)
+        if op_times is not None:
+            op_times.setdefault(f"{args['name']} {u}", []).append(time.perf_counter() - t0)
+    return num_rows
+
+
 def _cut_to_counts(tensor_group, before, stages, df, m, g):
     """Replace every column the deferred nodes left at a capacity size by its real rows: the
     first m (pairs), g (groups) or min(g, limit) (top rows)."""
@@ -165,12 +243,14 @@ def _cut_to_counts(tensor_group, before, stages, df, m, g):
 
 
 def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem_pool=None,
-            gpu_enable=True, op_times=None, sync=True):
+            gpu_enable=True, op_times=None, sync=True, fuse=True):
     """Run `plan` (list of (u, args)) over `tensor_group` in place; returns the last row count.
 
     Without per-operator synchronisation (sync=False), a Join whose pairs only feed the next
-    Aggregate runs together with it (one synchronisation instead of two); the results are the
-    same."""
+    Aggregate runs together with it (one synchronisation instead of two); when the aggregate
+    groups by the join key alone and `fuse` is set, the two run as one fused groupjoin whose
+    pairs are never written (the join's output columns are computed if something reads them).
+    The results are the same."""
     num_rows = None
     i = 0
     while i < len(plan):
@@ -178,8 +258,11 @@ def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem
         name = args['name']
         count = _deferrable(plan, i) if not sync and gpu_enable else 0
         if count:
-            num_rows = _run_deferred(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool,
-                                     gpu_enable, op_times)
+            num_rows = _run_groupjoin(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool,
+                                      gpu_enable, op_times) if fuse else None
+            if num_rows is None:
+                num_rows = _run_deferred(plan, i, count, tensor_group, cpu_mem_pool,
+                                         gpu_mem_pool, gpu_enable, op_times)
             i += count
             continue
         i += 1
