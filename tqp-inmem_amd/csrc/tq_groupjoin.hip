@@ -296,24 +296,26 @@ struct GjCheck {
   int64_t spin_ticks;       // look-back bound in wall-clock ticks
 };
 
-// Stamped keys of slice s: presence bytes [s << sb, (s + 1) << sb) that hold the epoch.  A
-// thread tests 16 consecutive bytes with one load (the byte array is padded to 256 bytes).
-__device__ __forceinline__ uint32_t slice_keys(const GjCheck& K, int s, int sb) {
-  const int ps = 1 << sb;
-  const uint64_t first = (uint64_t)s << sb;
+// Stamped keys of slice s: presence bytes [s << sb, (s + 1) << sb) that hold the epoch.  Thread
+// t tests the 16 bytes at t * 16 with one load (2^sb <= 4096 = 16 * 256 bytes; the byte array
+// is padded to 256 bytes): slice_bytes issues the load, slice_keys counts.
+__device__ __forceinline__ uint4 slice_bytes(const GjCheck& K, int s, int sb) {
+  const uint64_t at = ((uint64_t)s << sb) + (uint64_t)threadIdx.x * 16;
+  if ((int)threadIdx.x * 16 >= (1 << sb) || at >= K.range) return make_uint4(0, 0, 0, 0);
+  return *(const uint4*)(K.pres + at);
+}
+__device__ __forceinline__ uint32_t slice_keys(const GjCheck& K, int s, int sb, uint4 q) {
+  const uint64_t at = ((uint64_t)s << sb) + (uint64_t)threadIdx.x * 16;
+  if ((int)threadIdx.x * 16 >= (1 << sb) || at >= K.range) return 0;
   const uint32_t rep = K.epoch * 0x01010101u;
+  const uint32_t d[4] = {q.x, q.y, q.z, q.w};
   uint32_t c = 0;
-  for (int i = threadIdx.x * 16; i < ps; i += kBlock * 16) {
-    if (first + (uint64_t)i >= K.range) break;
-    const uint4 q = *(const uint4*)(K.pres + first + i);
-    const uint32_t d[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t x = d[k] ^ rep;  // bytes equal to the epoch become zero
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t x = d[k] ^ rep;  // bytes equal to the epoch become zero
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
-        c += (((x >> (8 * b)) & 0xFFu) == 0u && first + i + 4 * k + b < K.range) ? 1u : 0u;
-    }
+    for (int b = 0; b < 4; ++b)
+      c += (((x >> (8 * b)) & 0xFFu) == 0u && at + 4 * k + b < K.range) ? 1u : 0u;
   }
   return c;
 }
@@ -345,6 +347,20 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   const int s = xper > 0 ? (int)(blockIdx.x % 8) * xper + (int)(blockIdx.x / 8) : (int)blockIdx.x;
   if (s >= S) return;  // the remapped grid's unused places
   const int rw = 1 + G.naggs;
+  // the presence bytes and the first batch's chunk offsets are in flight while the table clears
+  const uint4 kq = slice_bytes(K, s, sb);
+  constexpr int kPer = kGjBatch / kBlock;
+  uint32_t la[kPer], lb[kPer];
+  auto load_batch = [&](int64_t wb) {
+    const int nbw = (int)(W - wb < kGjBatch ? W - wb : kGjBatch);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = t * kPer + k;
+      la[k] = i < nbw ? loff[(int64_t)s * W + wb + i] : 0u;
+      lb[k] = i < nbw ? loff[(int64_t)(s + 1) * W + wb + i] : 0u;
+    }
+  };
+  load_batch(0);
   for (int i = t; i < ps; i += kBlock) {
     lcnt[i] = 0;
     for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
@@ -352,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   if (t == 0) s_keys = 0;
   __syncthreads();
   {
-    const uint32_t kc = wave_sum(slice_keys(K, s, sb));
+    const uint32_t kc = wave_sum(slice_keys(K, s, sb, kq));
     if (lane_id() == 0 && kc) atomicAdd(&s_keys, kc);
   }
   const uint64_t smask = (uint64_t)ps - 1;
@@ -360,17 +376,14 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   for (int64_t wb = 0; wb < W; wb += kGjBatch) {
     const int nbw = (int)(W - wb < kGjBatch ? W - wb : kGjBatch);
     // this batch's chunks: first record of slice s and how many
-    constexpr int kPer = kGjBatch / kBlock;
     uint32_t cnt[kPer], sum = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int i = t * kPer + k;
       cnt[k] = 0;
       if (i < nbw) {
-        const uint32_t a = loff[(int64_t)s * W + wb + i];
-        const uint32_t b = loff[(int64_t)(s + 1) * W + wb + i];
-        st[i] = a;
-        cnt[k] = b - a;
+        st[i] = la[k];
+        cnt[k] = lb[k] - la[k];
       }
       sum += cnt[k];
     }
@@ -433,6 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
       }
     }
     __syncthreads();
+    if (wb + kGjBatch < W) load_batch(wb + kGjBatch);
   }
   // occupied slots in slot order: slot i = j*256 + t, ranked by wave ballots
   const int w = t >> 6;
@@ -508,29 +522,64 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   __syncthreads();
   const int64_t obase = s_off;
   const int64_t kbase = lo + ((int64_t)s << sb);
-  uint32_t run = 0;
+  // output row of slot j * 256 + t (-1: unoccupied, or no valid offset)
+  int64_t opos[kItems];
+  {
+    uint32_t run = 0;
+    bool over = false;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    uint32_t before = 0, tot = 0;
+    for (int j = 0; j < kItems; ++j) {
+      uint32_t before = 0, tot = 0;
 #pragma unroll
-    for (int v = 0; v < kWaves; ++v) {
-      const uint32_t cc = wocc[j][v];
-      tot += cc;
-      before += (v < w) ? cc : 0u;
-    }
-    if ((bal[j] >> lane_id()) & 1u) {
-      const int i = j * kBlock + t;
+      for (int v = 0; v < kWaves; ++v) {
+        const uint32_t cc = wocc[j][v];
+        tot += cc;
+        before += (v < w) ? cc : 0u;
+      }
       const int64_t o = obase + run + before + rank_below(bal[j]);
-      if (obase >= 0 && o < O.cap) {
-        store_int(O.key, O.key_dtype, o, kbase + i);
-        for (int a = 0; a < G.naggs; ++a)
-          write_agg(O.agg[a], o, G.op[a], G.kind[a], G.arg_dtype[a], lacc[a * ps + i], lcnt[i]);
-      } else if (obase >= 0) {  // more groups than the outputs hold
-        atomicOr((unsigned long long*)&K.words[GJ_ERR], (unsigned long long)GJ_ERR_CAP);
+      const bool mine = obase >= 0 && ((bal[j] >> lane_id()) & 1u);
+      over = over || (mine && o >= O.cap);  // more groups than the outputs hold
+      opos[j] = mine && o < O.cap ? o : -1;
+      run += tot;
+    }
+    if (over) atomicOr((unsigned long long*)&K.words[GJ_ERR], (unsigned long long)GJ_ERR_CAP);
+  }
+  // one tight loop per output column, the dtype dispatched once per column
+#define TQ_PUT(T, VAL)                                     \
+  _Pragma("unroll") for (int j = 0; j < kItems; ++j) {     \
+    if (opos[j] < 0) continue;                             \
+    const int i = j * kBlock + t;                          \
+    ((T*)dst)[opos[j]] = (VAL);                            \
+  }
+  {
+    void* dst = O.key;
+    switch (O.key_dtype) {
+      case TQ_I32: TQ_PUT(int32_t, (int32_t)(kbase + i)) break;
+      case TQ_I64: TQ_PUT(int64_t, kbase + i) break;
+      case TQ_I16: TQ_PUT(int16_t, (int16_t)(kbase + i)) break;
+      default: TQ_PUT(int8_t, (int8_t)(kbase + i)) break;
+    }
+  }
+  for (int a = 0; a < G.naggs; ++a) {
+    void* dst = O.agg[a];
+    const uint64_t* acc = lacc + (int64_t)a * ps;
+    if ((G.op[a] == TQ_AGG_SUM || G.op[a] == TQ_AGG_MIN || G.op[a] == TQ_AGG_MAX) &&
+        G.arg_dtype[a] == TQ_F64) {
+      TQ_PUT(double, as_f64(acc[i]))          // f64 accumulators are the f64 results
+    } else if (G.op[a] == TQ_AGG_COUNT ||
+               (G.op[a] == TQ_AGG_SUM && G.kind[a] == ACC_ISUM)) {
+      TQ_PUT(int64_t, (int64_t)acc[i])        // counts and exact integer sums
+    } else {
+      const int op = G.op[a], kind = G.kind[a], dt = G.arg_dtype[a];
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) {
+        if (opos[j] < 0) continue;
+        const int i = j * kBlock + t;
+        write_agg(dst, opos[j], op, kind, dt, acc[i], lcnt[i]);
       }
     }
-    run += tot;
   }
+#undef TQ_PUT
   if (s == S - 1) {
     // the last slice (which no workgroup waits on) also sums the build's tile counts: the
     // selected build rows, and whether a key fell outside the bounds
