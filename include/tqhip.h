@@ -526,7 +526,7 @@ int tq_groupjoin_run(const tq_groupjoin_desc_t* desc /* host */, const tq_progra
  * is the join directory and the aggregation table at once, and groups come out in key order.
  * Work is a device queue of (pipeline, tile) items in five phases -- clear the table, insert
  * build rows, probe + accumulate, count groups per tile, write groups -- where an item waits
- * (bounded spin) for its pipeline's previous phase to finish.  No host synchronisation occurs;
+ * (bounded wait) for its pipeline's previous phase to finish.  No host synchronisation occurs;
  * the launch sequence can be captured into a graph.  A duplicate selected build key sets
  * flags[0] bit 0 (the caller reruns that pipeline through the per-operator path).
  * Replaces, per pipeline, the whole chain tqp_filter (operators/filter.py:164-205) x2 ->
@@ -574,11 +574,13 @@ size_t tq_batch_ws_bytes(int32_t npipes);
  * tile_rows: rows (or table slots) per item, a multiple of 1024; grid: persistent workgroups;
  * stage_loads: the largest nload of any program; lds_table_bytes: the largest LDS table
  * (range * (4 + 8 * naggs)) of the pipelines with lds_table set.  ws is cleared by the call
- * (kernels only), and
- * ws error word 1 means a wait exceeded spin_limit (the launch then drains without finishing). */
+ * (kernels only).  An item's wait for its pipeline's previous phase polls with relaxed loads
+ * and gives up after wait_us microseconds of wall-clock time (the device's constant-rate
+ * clock): ws error word 1 then means a wait timed out (the launch drains without finishing and
+ * the caller reruns the pipelines per operator). */
 int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const int32_t* items,
                  const int64_t* phase_start /* host, 6 */, int32_t tile_rows, int32_t grid,
-                 int32_t stage_loads, int32_t lds_table_bytes, void* ws, uint32_t spin_limit,
+                 int32_t stage_loads, int32_t lds_table_bytes, void* ws, uint32_t wait_us,
                  void* stream);
 
 #ifdef __cplusplus

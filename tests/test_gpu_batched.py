@@ -104,3 +104,47 @@ def test_batched_groupby_pipelines_match_plain(device):
         for k in (22, 23, 24, 25, 26, 27):
             np.testing.assert_allclose(out[i][k].cpu().numpy(), tg[k].tensor.cpu().numpy(),
                                        rtol=1e-12, atol=0, err_msg=f"{i} {k}")
+
+
+def _host_tables(group):
+    """The pipeline's device columns as the oracle's {id: (type, ndarray)} tables."""
+    return {cid: (v.tensor_type, v.tensor.cpu().numpy()) for cid, v in group.items()}
+
+
+def test_config3_full_size_batched_plain_and_oracle(device):
+    """The 256 pipelines at their stated sizes (probe rows log-uniform in [10,000, 6,000,000],
+    build a quarter of probe, random filter constants): every pipeline's batched result equals
+    its per-operator result (keys and counts bit-exact, sums within 1e-12), and eight sampled
+    pipelines, the largest among them, equal the oracle's."""
+    from oracle import ref_ops as R
+    from tqp_amd import batched, workloads
+    probe, build, consts = workloads.config3_sizes(n_pipes=256)
+    print("config3 full size:", int(probe.sum() + build.sum()), "input rows", flush=True)
+    pipes = [workloads.config3_pipeline(i, probe[i], build[i], consts[i][0], consts[i][1], device)
+             for i in range(len(probe))]
+    specs = [batched.match_plan(plan, group) for group, plan in pipes]
+    assert all(s is not None for s in specs)
+    eng = batched.BatchedEngine(specs, tile_rows=8192)
+    eng.run()
+    out, rerun = eng.results()
+    assert rerun == []
+    for i, (group, plan) in enumerate(pipes):
+        ref = _plain(group, plan)
+        got = out[i]
+        assert torch.equal(got[101].cpu(), ref[101].tensor.cpu()), f"keys of pipeline {i}"
+        assert torch.equal(got[302].cpu(), ref[302].tensor.cpu()), f"counts of pipeline {i}"
+        np.testing.assert_allclose(got[301].cpu().numpy(), ref[301].tensor.cpu().numpy(),
+                                   rtol=1e-12, atol=0, err_msg=f"sums of pipeline {i}")
+    print("config3: all 256 batched == per-operator", flush=True)
+    g = np.random.default_rng(33)
+    sample = sorted({int(np.argmax(probe))} | set(g.choice(len(probe), 7, replace=False).tolist()))
+    for i in sample:
+        group, plan = pipes[i]
+        cpu = R.to_cols(_host_tables(group))
+        R.execute(copy.deepcopy(plan), cpu)
+        got = out[i]
+        print("config3 oracle: pipeline", i, "probe rows", int(probe[i]), flush=True)
+        assert torch.equal(got[101].cpu().reshape(cpu[101].tensor.shape), cpu[101].tensor), i
+        assert torch.equal(got[302].cpu(), cpu[302].tensor), i
+        np.testing.assert_allclose(got[301].cpu().numpy(), cpu[301].tensor.numpy(), rtol=1e-12,
+                                   atol=0, err_msg=f"oracle sums of pipeline {i}")

@@ -14,6 +14,9 @@
 // without sorting.  Predicates and aggregate arguments run on the expression interpreter with the
 // same LDS staging as the per-operator kernels.
 #include "tq_agg.h"
+#include <map>
+#include <mutex>
+
 #include "tq_common.h"
 #include "tq_expr.h"
 
@@ -31,6 +34,12 @@ struct BatchWs {
 
 __device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A poll of a phase counter: relaxed (an acquire load also invalidates the CU's caches on every
+// iteration, which with hundreds of workgroups polling slows the workgroups they wait on); the
+// waiter takes one acquire fence when the count is reached.
+__device__ __forceinline__ uint32_t load_relaxed(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ int64_t slot_of(const tq_batch_pipe_t& B, int64_t key) {
@@ -251,7 +260,7 @@ __device__ void phase_write(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, in
 __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restrict__ pipes,
                                                   const int32_t* __restrict__ items,
                                                   PhaseStarts ps, int32_t tile_rows, BatchWs ws,
-                                                  uint32_t spin_limit, int32_t stage_words) {
+                                                  int64_t wait_ticks, int32_t stage_words) {
   extern __shared__ uint64_t stage[];
   uint64_t* ltab = stage + stage_words;
   __shared__ int64_t s_item;
@@ -276,17 +285,20 @@ __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restr
     const tq_batch_pipe_t& B = pipes[p];
     if (phase > 0) {
       if (threadIdx.x == 0) {
+        // wait for the pipeline's previous phase, at most wait_ticks of wall-clock time
         const uint32_t need = (uint32_t)B.ntiles[phase - 1];
         const uint32_t* d = ws.done + (int64_t)p * TQ_BATCH_PHASES + (phase - 1);
-        uint32_t spins = 0;
-        while (load_acquire(d) < need) {
+        const int64_t t0 = wall_clock64();
+        bool late = false;
+        while (load_relaxed(d) < need) {
           __builtin_amdgcn_s_sleep(2);
-          if (++spins > spin_limit) {
+          if (wall_clock64() - t0 > wait_ticks) {
             atomicOr(ws.err, 1u);
+            late = true;
             break;
           }
         }
-        s_abort = spins > spin_limit;
+        s_abort = late;
       }
       __syncthreads();
       if (s_abort) break;
@@ -328,7 +340,7 @@ extern "C" size_t tq_batch_ws_bytes(int32_t npipes) {
 extern "C" int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const int32_t* items,
                             const int64_t* phase_start, int32_t tile_rows, int32_t grid,
                             int32_t stage_loads, int32_t lds_table_bytes, void* ws,
-                            uint32_t spin_limit, void* stream) {
+                            uint32_t wait_us, void* stream) {
   if (!pipes || !items || !phase_start || !ws) return set_error(TQ_EINVAL, "batch: null argument");
   if (tile_rows <= 0 || tile_rows % kSubRows) return set_error(TQ_EINVAL, "batch: tile_rows %d not a multiple of %d", tile_rows, kSubRows);
   if (stage_loads < 1 || stage_loads > TQ_MAX_LOADS) return set_error(TQ_EINVAL, "batch: stage_loads %d", stage_loads);
@@ -349,5 +361,26 @@ extern "C" int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const 
   const int32_t stage_words = stage_loads * kSubRows;
   const size_t lds = (size_t)stage_words * 8 + align_up((size_t)lds_table_bytes, 8);
   if (lds > 64 * 1024) return set_error(TQ_EINVAL, "batch: %zu bytes of LDS", lds);
-  return launch("k_batch", k_batch, (unsigned)grid, kBlock, lds, s, pipes, items, ps, tile_rows, w, spin_limit, stage_words);
+  // the wall clock's rate, once per device (kHz: ticks per millisecond)
+  static std::mutex mu;
+  static std::map<int, int64_t> khz_of;
+  int dev = 0;
+  TQ_TRY_HIP(hipGetDevice(&dev));
+  int64_t khz;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = khz_of.find(dev);
+    if (it == khz_of.end()) {
+      int r = 0;
+      if (hipDeviceGetAttribute(&r, hipDeviceAttributeWallClockRate, dev) != hipSuccess || r <= 0) {
+        (void)hipGetLastError();
+        r = 100000;  // 100 MHz
+      }
+      it = khz_of.emplace(dev, (int64_t)r).first;
+    }
+    khz = it->second;
+  }
+  const int64_t wait_ticks = (int64_t)wait_us * khz / 1000;
+  return launch("k_batch", k_batch, (unsigned)grid, kBlock, lds, s, pipes, items, ps, tile_rows, w,
+                wait_ticks, stage_words);
 }

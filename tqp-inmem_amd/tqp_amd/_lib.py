@@ -212,7 +212,10 @@ def call(name, *args, alg_bytes=None, kernel=None):
         c0 = lib.tq_timing_count()
         rc = fn(*args)
         c1 = lib.tq_timing_count()
-        if isinstance(kernel, tuple):   # several principal kernels, bytes listed alike
+        if isinstance(kernel, dict):    # {kernel: algorithmic bytes of this call}
+            for k, b in kernel.items():
+                TRACE.append((name, k, b, c0, c1))
+        elif isinstance(kernel, tuple):   # several principal kernels, bytes listed alike
             for k, b in zip(kernel, alg_bytes):
                 TRACE.append((name, k, b, c0, c1))
         else:

@@ -193,13 +193,13 @@ def _key_bounds(t: torch.Tensor):
 class BatchedEngine:
     """Prepare once, then run() (graph-capturable) and results()."""
 
-    def __init__(self, specs, tile_rows=4096, grid=None, spin_limit=1 << 22, key_bounds=None):
+    def __init__(self, specs, tile_rows=4096, grid=None, wait_us=100_000, key_bounds=None):
         self.specs = list(specs)
         self.tile_rows = int(tile_rows)
         self.device = torch.device("cuda:0")
         props = torch.cuda.get_device_properties(self.device)
         self.grid = grid or props.multi_processor_count * 4
-        self.spin_limit = spin_limit
+        self.wait_us = wait_us
         self.key_bounds = key_bounds or {}
         self.rejected = []        # indices of specs the batch cannot run
         self._prepare()
@@ -385,7 +385,7 @@ class BatchedEngine:
             return
         L.call("tq_batch_run", L.ptr(self.desc), self.npipes, L.ptr(self.items), self.phase_start,
                self.tile_rows, self.grid, self.stage_loads, self.lds_table_bytes, L.ptr(self.ws),
-               ctypes.c_uint32(self.spin_limit), L.stream_handle())
+               ctypes.c_uint32(self.wait_us), L.stream_handle())
 
     def results(self, sync=True):
         """Returns ({spec index: {out id: tensor}}, [spec indices to rerun]).  With sync=False
@@ -394,7 +394,7 @@ class BatchedEngine:
             torch.cuda.synchronize()
         err = int(self.ws[256:260].view(torch.int32).item()) if self.npipes else 0
         if err:
-            raise RuntimeError("batched launch aborted: a phase wait exceeded its spin limit")
+            raise RuntimeError("batched launch aborted: a phase wait exceeded its time limit")
         ng = self.ngroups.cpu().tolist()
         fl = self.flags.cpu().tolist()
         out, rerun = {}, list(self.rejected)

@@ -171,13 +171,14 @@ class GroupJoinDeferred(kernels.Deferred):
             return self.m, self.G
         host = (ctypes.c_int64 * 8)()
         L.call("tq_read_words", 1, (ctypes.c_void_p * 1)(self.words), (ctypes.c_size_t * 1)(40),
-               host, L.stream_handle())
+               host, L.stream_handle(), kernel={"k_collect_words": 80})
         G, m, nsel, nset, err = (int(host[i]) for i in range(5))
         if err or nsel != nset or m <= 0:
             raise kernels.DeferredJoinFailed()
         self.m, self.G = m, G
         for kern, nbytes in self.bytes_fn(m, G).items():
             L.trace_set_bytes(self.trace_at, kern, nbytes)
+        self.sorted_bytes(G)
         return m, G
 
 
@@ -399,7 +400,7 @@ def _key_bounds(bk):
     out = torch.empty(3, dtype=torch.int64, device=bk.device)
     ws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", n), bk.device, "gb_bounds")
     L.call("tq_groupby_bounds", ctypes.byref(d), None, n, L.ptr(out), L.ptr(ws),
-           L.stream_handle())
+           L.stream_handle(), kernel={"k_gb_bounds": n * bk.element_size()})
     lo, hi, _ = out.cpu().tolist()
     kernels.note_bounds(bk, lo, hi)
     return lo, hi

@@ -47,7 +47,7 @@ def filter_mask(tree, tensors, subquery_result=None, n=None):
     # read each loaded column once, write one mask byte per row
     alg = rows * (1 + b.load_bytes())
     L.call("tq_filter_mask", E.program_ptr(b), E.colset_ptr(b), rows, L.ptr(mask), L.ptr(counts),
-           L.stream_handle(), alg_bytes=alg, kernel="k_filter")
+           L.stream_handle(), kernel={"k_filter_conj": alg, "k_filter": alg})
     return mask, counts
 
 
@@ -107,9 +107,12 @@ def gather(idx: torch.Tensor, columns, null_fill=False):
     for i in range(0, len(columns), L.TQ_MAX_COLS):
         src = [c.contiguous() for c in columns[i:i + L.TQ_MAX_COLS]]
         fk = [8 if c.dtype == torch.float64 else 4 if c.dtype == torch.float32 else 0 for c in src]
+        rb = sum(L.row_bytes(c) for c in src)
+        # the row ids, each gathered row read once and written once
         L.call("tq_gather", L.ptr(idx), m, len(src), L.ptr_array(src),
                L.int_array([L.row_bytes(c) for c in src]), L.int_array(fk),
-               L.ptr_array(outs[i:i + L.TQ_MAX_COLS]), 1 if null_fill else 0, L.stream_handle())
+               L.ptr_array(outs[i:i + L.TQ_MAX_COLS]), 1 if null_fill else 0, L.stream_handle(),
+               kernel={"k_gather": m * (8 + 2 * rb)})
     return outs
 
 
@@ -161,17 +164,22 @@ def sort_perm(keys, n: int, limit=None, device=None):
         arr[i].descending = 1 if desc else 0
     ws_bytes = L.size_call("tq_sort_ws_bytes", n)
     ws = L.workspace(ws_bytes, dev, "sort")
+    kb = sum(L.row_bytes(t) for t in hold)
+    # top-k rounds read every key once (later rounds read the few survivors); a packing pass
+    # reads the keys and writes one 12-byte (key, row) pair per row
+    sort_bytes = {"k_topk_block": n * kb, "k_topk_select": n * 12, "k_pack_keys": n * (kb + 12)}
     if pending:
         applied = ctypes.c_int32()
+        at = L.trace_len()
         L.call("tq_sort_perm_deferred", arr, len(keys), n, ctypes.c_void_p(df.group["ngroups"]),
                int(limit), L.ptr(perm), L.ptr(ws), ws_bytes, ctypes.byref(applied),
-               L.stream_handle())
+               L.stream_handle(), kernel=sort_bytes)
         if applied.value:
-            df.sorted = {"cap": m, "limit": int(limit)}
+            df.sorted = {"cap": m, "limit": int(limit), "trace": at, "kb": kb}
             return perm
         return sort_perm(keys, df.resolve()[1], limit, device)
     L.call("tq_sort_perm", arr, len(keys), n, -1 if limit is None else int(limit), L.ptr(perm),
-           L.ptr(ws), ws_bytes, L.stream_handle())
+           L.ptr(ws), ws_bytes, L.stream_handle(), kernel=sort_bytes)
     return perm
 
 
@@ -224,14 +232,25 @@ class JoinIndex:
         self.deferred = False   # pairs written by a deferred probe, not yet settled
         bm = L.ptr(bmask) if bmask is not None else None
         kb = known_bounds(self.bkey)
+        at = L.trace_len()
+        names = {"k_join_stats": 0, "k_join_clear": 0, "k_join_claim": 0, "k_join_bits": 0}
         if kb is not None:
             # bounds of the whole key column pick the layout: no statistics pass, no wait
             L.call("tq_join_build_ranged", L.ptr(self.bkey), L.dtype_code(self.bkey.dtype), bm, nb,
                    kb[0], kb[1], L.ptr(self.ws), ws_bytes, ctypes.byref(self.state),
-                   L.stream_handle())
+                   L.stream_handle(), kernel=names)
         else:
             L.call("tq_join_build", L.ptr(self.bkey), L.dtype_code(self.bkey.dtype), bm, nb,
-                   L.ptr(self.ws), ws_bytes, ctypes.byref(self.state), L.stream_handle())
+                   L.ptr(self.ws), ws_bytes, ctypes.byref(self.state), L.stream_handle(),
+                   kernel=names)
+        if at is not None:
+            # keys and mask read by the statistics and claim passes; the directory (4 bytes an
+            # entry, 16 in a hash table) cleared, then read to build its presence bitmap
+            rb = nb * (self.bkey.element_size() + (1 if bmask is not None else 0))
+            ent = int(self.state.range) * (16 if self.state.layout == L.TQ_JOIN_HASH else 4)
+            for k, v in {"k_join_stats": rb, "k_join_claim": rb, "k_join_clear": ent,
+                         "k_join_bits": ent + int(self.state.range) // 8}.items():
+                L.trace_set_bytes(at, k, v)
 
     @property
     def nsel(self):
@@ -351,7 +370,8 @@ class Deferred:
             sizes += [8, 8]
         host = (ctypes.c_int64 * 64)()
         L.call("tq_read_words", len(ptrs), (ctypes.c_void_p * len(ptrs))(*ptrs),
-               (ctypes.c_size_t * len(ptrs))(*sizes), host, L.stream_handle())
+               (ctypes.c_size_t * len(ptrs))(*sizes), host, L.stream_handle(),
+               kernel={"k_collect_words": 2 * sum(sizes)})
         m = int(host[0])
         ok = ctypes.c_int32(1)
         if j["check"]:
@@ -386,8 +406,16 @@ class Deferred:
         if err:
             raise DeferredJoinFailed()   # the fused reduce gave up: run the plain path
         self.G = g
-        L.trace_set_bytes(self.group["trace"], "k_gb_accum", self.m * self.group["row_bytes"])
+        for kern, nbytes in self.group["bytes"](self.m, g).items():
+            L.trace_set_bytes(self.group["trace"], kern, nbytes)
+        self.sorted_bytes(g)
         return self.m, g
+
+    def sorted_bytes(self, g):
+        """A deferred top-k was traced with the capacity as its row count: state the bytes of
+        the g rows it read."""
+        if self.sorted is not None and self.sorted.get("trace") is not None:
+            L.trace_set_bytes(self.sorted["trace"], "k_topk_block", g * self.sorted["kb"])
 
 
 _TLS = threading.local()
@@ -542,7 +570,9 @@ class GroupBy:
         else:
             bounds = torch.empty(2 * len(self.keys) + 1, dtype=torch.int64, device=dev)
             bws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", self.n), dev, "gb_bounds")
-            L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), L.ptr(bws), s)
+            L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), L.ptr(bws), s,
+                   kernel={"k_gb_bounds": self.n * (sum(k.element_size() for k in self.keys)
+                                                    + (1 if self.mask is not None else 0))})
             hb = bounds.cpu().tolist()
         nsel = hb[-1]
         self.nsel = nsel
@@ -577,6 +607,19 @@ class GroupBy:
             else:
                 odt = dt
             out_dtypes.append(odt)
+        kbytes = sum(k.element_size() for k in self.keys)
+        rec_bytes = 8 * (1 + len(self.ops))
+        out_bytes = kbytes + sum(dt.itemsize for dt in out_dtypes)
+
+        def gb_bytes(rows, groups):
+            # the accumulate pass reads every selected row's loaded columns; the partitioned
+            # mode splits that into a histogram of the keys, a scatter of one record per row and
+            # the partition reduce that reads the records and writes the groups
+            return {"k_gb_accum": rows * b.load_bytes() + (self.n if self.mask is not None else 0),
+                    "k_gbp_hist": rows * kbytes + (self.n if self.mask is not None else 0),
+                    "k_gbp_scatter": rows * (b.load_bytes() + rec_bytes),
+                    "k_gbp_reduce": rows * rec_bytes + groups * out_bytes,
+                    "k_gb_write_dense": groups * out_bytes}
         # outputs sized for the most groups there can be; the groups written are a prefix
         cap = max(1, min(nsel, self.slots))
         key_full = [torch.empty(cap, dtype=k.dtype, device=dev) for k in self.keys]
@@ -587,15 +630,18 @@ class GroupBy:
             L.call("tq_groupby_run_deferred", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b),
                    None, self.n, ctypes.c_void_p(df.join["total"]), mode, slots, L.ptr(ws),
                    wsb.value, L.ptr_array(key_full), L.ptr_array(agg_full), cap,
-                   ctypes.byref(ngd), ctypes.byref(errd), s, alg_bytes=0, kernel="k_gb_accum")
+                   ctypes.byref(ngd), ctypes.byref(errd), s, kernel=gb_bytes(0, 0))
             df.group = {"ngroups": ngd.value, "err": errd.value, "outs": key_full + agg_full,
-                        "cap": cap, "trace": at, "row_bytes": b.load_bytes(), "ws": ws}
+                        "cap": cap, "trace": at, "bytes": gb_bytes, "ws": ws}
             return key_full, agg_full
         ng = ctypes.c_int64()
+        at = L.trace_len()
         L.call("tq_groupby_run", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), mptr, self.n,
                mode, slots, L.ptr(ws), wsb.value, L.ptr_array(key_full), L.ptr_array(agg_full), cap,
-               ctypes.byref(ng), s, alg_bytes=alg, kernel="k_gb_accum")
+               ctypes.byref(ng), s, kernel=gb_bytes(nsel, 0))
         G = int(ng.value)
+        for kern, nbytes in gb_bytes(nsel, G).items():
+            L.trace_set_bytes(at, kern, nbytes)
         key_out = [o[:G] for o in key_full]
         agg_out = [o[:G] for o in agg_full]
         return key_out, agg_out
