@@ -80,7 +80,10 @@ enum tq_opcode {
   /* dst <- reg[c] ? reg[a] : reg[b] (raw bits) */
   TQ_OP_WHERE,
   /* dst <- all(string column a [row, 0:len] == pool[off:off+len]) (imm = off | len << 32);
-     EQ also requires the column's remaining bytes to be 0 (torch pads literals with 0) */
+     EQ also requires the column's remaining bytes to be 0 (torch pads literals with 0).
+     For every string opcode, c != 0 narrows the row to its first c bytes (a substring column:
+     the column set entry points at the substring's first byte and keeps the full row width as
+     its row stride) */
   TQ_OP_STR_EQ, TQ_OP_STR_PREFIX,
   /* dst <- calendar year of int64 day count since 1990-01-01 (bucketize on year edges) */
   TQ_OP_YEAR,
@@ -89,6 +92,10 @@ enum tq_opcode {
      reference's like_mask scans (operators/like.py:15-25, filter.py:121-127);
      imm = off1 | len1 << 16 | off2 << 32 | len2 << 48 in the string pool */
   TQ_OP_STR_LIKE2,
+  /* dst <- string column a holds pool[off:off+len] at some position q with q + len <= width,
+     padding bytes included, as like_contains scans (operators/like.py:3-11; filter.py:56-62,
+     which also takes EndsWith this way); imm = off | len << 32 */
+  TQ_OP_STR_CONTAINS,
   TQ_OP_COUNT_
 };
 
@@ -224,6 +231,20 @@ int tq_sort_perm(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n, 
 int tq_sort_perm_deferred(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n,
                           const int64_t* n_dev, int64_t limit, int64_t* perm_out, void* ws,
                           size_t ws_bytes, int32_t* applied /* host out */, void* stream);
+
+/* Dense ranks of rows in a sort order: perm (a permutation of 0..n-1, e.g. tq_sort_perm's by
+ * the same keys) lists the rows sorted; rank_out[row] receives the number of distinct keys
+ * before the row's key in that order (equal keys share a rank: floats as the sort compares
+ * them, -0.0 equal to +0.0 and NaN to NaN; 2-D int8 keys by their row bytes), rep_out[r] the
+ * first row of rank r in the order, and *nranks (device) the number of distinct keys.
+ * rep_out holds n entries.  ws: tq_rank_ws_bytes(n).  A group-by over keys that do not pack
+ * into integer ranges (string rows, floats) groups by these ranks and maps them back through
+ * rep_out.  Replaces the key stacking and torch.unique(hash) of operators/aggregate.py:302-327
+ * for such keys. */
+size_t tq_rank_ws_bytes(int64_t n);
+int tq_rank_sorted(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n,
+                   const int64_t* perm, int32_t* rank_out, int64_t* rep_out, int64_t* nranks,
+                   void* ws, size_t ws_bytes, void* stream);
 
 /* Copy n small device words (bytes[i] each, 512 bytes in all at most) to host_out, packed at
  * 8-byte-aligned offsets, with one synchronisation of the stream.  Replaces the `.item()` /

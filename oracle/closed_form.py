@@ -58,18 +58,31 @@ def join_pairs_brute(bkeys, bmask, pkeys, pmask):
     return pairs
 
 
+def _key_part(col, r):
+    """A hashable, ordered stand-in for row r of a key column: a 2-D (string) row is its tuple
+    of bytes; a float is its value with -0.0 as +0.0 and every NaN one value above +inf."""
+    if col.ndim == 2:
+        return tuple(int(b) for b in col[r])
+    v = col[r]
+    if col.dtype.kind == 'f':
+        v = float(v)
+        return (1, 0.0) if math.isnan(v) else (0, 0.0 if v == 0 else v)
+    return int(v)
+
+
 def group_reduce(keys, mask, values):
     """Exact grouping of selected rows by the key tuple.
 
-    keys: list of integer arrays; values: {name: (op, array)} with op in sum/avg/count/min/max.
-    Returns (sorted unique key tuples as a list of arrays, {name: result}); float sums are
+    keys: list of arrays (integer, float, or 2-D int8 strings); values: {name: (op, array)}
+    with op in sum/avg/count/min/max.  Returns (one array per key holding each group's key
+    value from its first row, groups in ascending key order, {name: result}); float sums are
     accumulated sequentially in row order, integer sums exactly."""
     n = len(keys[0]) if keys else len(next(iter(values.values()))[1])
     sel = np.arange(n) if mask is None else np.flatnonzero(mask)
     groups = {}
     order = []
     for r in sel:
-        k = tuple(int(kk[r]) for kk in keys)
+        k = tuple(_key_part(kk, r) for kk in keys)
         if k not in groups:
             groups[k] = []
             order.append(k)
@@ -99,8 +112,14 @@ def group_reduce(keys, mask, values):
             elif op == 'max':
                 res.append(max([0] + vals.tolist()))
         out[name] = res
-    key_cols = [np.array([k[i] for k in ks]) for i in range(len(keys))]
+    key_cols = [np.array([kk[groups[k][0]] for k in ks]) for kk in keys]
     return key_cols, out
+
+
+def year_of_day(d: int) -> int:
+    """Calendar year of day d counted from 1990-01-01 (the date epoch of conversion.py:9)."""
+    import datetime
+    return (datetime.date(1990, 1, 1) + datetime.timedelta(days=int(d))).year
 
 
 def _ranks(x: np.ndarray):

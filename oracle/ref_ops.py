@@ -113,6 +113,16 @@ def filter_evaluate(tensors, tree, rows, subquery_result, leaf_function=None):
             t = tensors[get_id(tree[1][0])].tensor
             name = tree[1][1]
             return torch.eq(t[:, :len(name)], str_codes(name, len(name))).all(dim=1)
+        if head in ('EndsWith', 'Contains'):
+            # filter.py:56-62: both through like_contains
+            t = tensors[get_id(tree[1][0])].tensor
+            return like_contains(t, tree[1][1])
+        if head == 'substring':
+            # filter.py:71-76
+            assert len(tree[1]) == 3
+            sub, l, w = tree[1]
+            l, w = int(l) - 1, int(w)
+            return filter_evaluate(tensors, sub, rows, subquery_result, leaf_function)[:, l:l + w]
         if head == 'NOT':
             return ~filter_evaluate(tensors, tree[1], rows, subquery_result, leaf_function)
         if head == 'isnotnull':
@@ -159,6 +169,18 @@ def filter_evaluate(tensors, tree, rows, subquery_result, leaf_function=None):
         other = torch.full((lhs.shape[0],), rhs, dtype=lhs.dtype) if isinstance(rhs, float) else rhs
         return torch.isclose(lhs, other.to(lhs.dtype) if torch.is_tensor(other) else other, atol=EPS)
     return ops[op](lhs, rhs)
+
+
+def like_contains(a, word):
+    """operators/like.py:3-11: rows of the [n, m] string tensor `a` that hold `word` at some
+    position pos in [0, m - len(word)], padding bytes included."""
+    p1 = str_codes(word, len(word))
+    n1 = p1.shape[0]
+    n, m = a.shape
+    appear = torch.zeros(n, dtype=torch.bool)
+    for pos in range(m - n1 + 1):
+        appear.logical_or_((a[:, pos:pos + n1] == p1).all(dim=-1))
+    return appear
 
 
 def like_mask(a, w1, w2):
@@ -345,11 +367,20 @@ def agg_evaluate(tensors, tree, cnt_rows, cnt_groups, inverted_index):
     if len(tree) == 2:
         head = tree[0]
         if head == 'year':
+            # aggregate.py:97-101 with conversion.py:10-12: bucketize on the cumulative year
+            # lengths 1990..1999 indexes the 10-entry year_range, so a day count of 3652
+            # (2000-01-01) or more indexes past its end and fails
             r = agg_evaluate(tensors, tree[1], cnt_rows, cnt_groups, inverted_index)
             edges = torch.tensor([365, 730, 1096, 1461, 1826, 2191, 2557, 2922, 3287, 3652])
-            return torch.tensor(list(range(1990, 2001)))[torch.bucketize(r, edges, right=True)]
+            return torch.tensor(list(range(1990, 2000)))[torch.bucketize(r, edges, right=True)]
         if head == 'NOT':
             return ~agg_evaluate(tensors, tree[1], cnt_rows, cnt_groups, inverted_index)
+        if head == 'substring':
+            # aggregate.py:115-120
+            assert len(tree[1]) == 3
+            sub, l, w = tree[1]
+            l, w = int(l) - 1, int(w)
+            return agg_evaluate(tensors, sub, cnt_rows, cnt_groups, inverted_index)[:, l:l + w]
         if head == 'cast':
             literal, _, cast_type = tree[1][0].split(' ')
             if cast_type == 'double':
@@ -414,6 +445,20 @@ def agg_evaluate(tensors, tree, cnt_rows, cnt_groups, inverted_index):
     return ops[op](a, b)
 
 
+def _key_ordinals(t):
+    """int64 columns whose lexicographic order and equality are those of the key values:
+    integers as they are, 2-D string rows byte by byte, floats by an order-preserving map of
+    their bits with -0.0 taken as +0.0 and every NaN alike, above +inf (torch.sort's order)."""
+    t = t if t.dim() == 2 else t.unsqueeze(1)
+    if not t.is_floating_point():
+        return t.to(torch.int64)
+    x = t.to(torch.float64)
+    x = torch.where(x == 0, torch.zeros_like(x), x)
+    x = torch.where(torch.isnan(x), torch.full_like(x, float('nan')), x)
+    b = x.view(torch.int64)
+    return torch.where(b < 0, b ^ 0x7FFFFFFFFFFFFFFF, b)
+
+
 def _unwrap(tree):
     while isinstance(tree, list) and len(tree) == 1:
         tree = tree[0]
@@ -454,20 +499,22 @@ def tqp_hash_aggregate(tensor_group, args):
             kids, ksz = [], []
         else:
             kids = [get_id(k) for k in args['keys']]
-            parts = [cols[k].tensor if cols[k].tensor.dim() == 2 else cols[k].tensor.unsqueeze(1)
-                     for k in kids]
-            ksz = np.cumsum([p.shape[1] for p in parts]).tolist()
-            stacked = torch.cat([p.to(torch.int64) for p in parts], dim=1)
+            # groups: distinct key tuples (aggregate.py:302-327 hashes the stacked key columns;
+            # the exact tuples are the groups a collision-free hash gives), numbered in
+            # ascending lexicographic order
+            stacked = torch.cat([_key_ordinals(cols[k].tensor) for k in kids], dim=1)
             uniq, inv = torch.unique(stacked, dim=0, return_inverse=True)
+            # each group's key values, read from its first row (exact for every dtype)
+            first = torch.full((uniq.shape[0],), n, dtype=torch.int64).scatter_reduce(
+                0, inv, torch.arange(n), reduce='amin')
         groups = uniq.shape[0]
         for tree in args['results']:
             tree = _unwrap(tree)
             oid = get_id(right_most(tree))
             if _is_key(tree):
                 kid = get_id(tree if isinstance(tree, str) else tree[0])
-                p = kids.index(kid)
-                lo = 0 if p == 0 else ksz[p - 1]
-                out[oid] = uniq[:, lo:ksz[p]].to(cols[kid].tensor.dtype)
+                kt = cols[kid].tensor
+                out[oid] = (kt if kt.dim() == 2 else kt.unsqueeze(1))[first]
             else:
                 out[oid] = agg_evaluate(cols, tree[0], n, groups, inv)
         res = [out[i].contiguous() for i in key_new + agg_ids]

@@ -549,3 +549,108 @@ def test_config2_full_size_properties(device):
     fk = c_attr[tables[333][1]] * 1000 + b_mod[tables[332][1]]
     u, n = np.unique(fk, return_counts=True)
     assert np.array_equal(u, key) and np.array_equal(n, cnt)
+
+
+def test_aggregate_year_case_division_not_equal(device):
+    """year, CASE WHEN, '/' (true division in float32 for integer operands) and '<>' inside
+    aggregates, computed by the expression kernel, against the oracle (whose own results are
+    pinned to the calendar and numpy in tests/test_oracle.py).  Day counts stay below 3652: the
+    reference's year table ends with 1999 and fails beyond it."""
+    g = np.random.default_rng(77)
+    n = 300_000
+    k = {10: ('string', g.integers(0, 3, n, dtype=np.int8)),
+         11: ('int', g.integers(0, 40, n, dtype=np.int32))}
+    vals = {20: ('date', g.integers(-5, 3652, n, dtype=np.int32)),
+            21: ('float', g.integers(1, 51, n).astype(np.float64)),
+            22: ('int', g.integers(-10, 10, n, dtype=np.int64))}
+    results = ['k#10', 'k#11',
+               [[['sum', [['year', ['d#20']]]], '#', '1'], 'AS', 'y#50'],
+               [[['sum', [['CASE', 'WHEN', ['x#22', '>', '0'], 'THEN', 'q#21', 'ELSE', '0.0',
+                           'END']]], '#', '2'], 'AS', 'cw#51'],
+               [[['sum', [['x#22', '/', '4']]], '#', '3'], 'AS', 'dv#52'],
+               [[['sum', [['x#22', '<>', '0']]], '#', '4'], 'AS', 'ne#53'],
+               [[['max', [['year', ['d#20']]]], '#', '5'], 'AS', 'my#54']]
+    agg_ids = [50, 51, 52, 53, 54]
+    gpu, cpu, errs = _agg_case(device, k, vals, g.random(n) < 0.85, results, [10, 11], agg_ids)
+    assert errs == [None, None], errs
+    for i in (10, 11):
+        _assert_col(gpu[i].tensor, cpu[i].tensor, f"key {i}")
+    for a in (50, 53, 54):
+        _assert_col(gpu[a].tensor, cpu[a].tensor, f"agg {a}")
+    _assert_col(gpu[51].tensor, cpu[51].tensor, "case sum", fp=True)
+    # float32 arguments: the oracle sums them with bincount in float64 in row order, the
+    # kernels in varying order (the relative error of a float32-valued f64 sum stays ~1e-13)
+    g51, c51 = gpu[52].tensor.cpu(), cpu[52].tensor
+    assert g51.dtype == c51.dtype and g51.shape == c51.shape
+    np.testing.assert_allclose(g51.numpy(), c51.numpy(), rtol=1e-12, atol=0)
+
+
+def test_filter_contains_endswith_substring_bit_exact(device):
+    """Contains / EndsWith (like_contains over the whole row) and substring IN / '=' over a 2-D
+    string column, alone, negated and in conjunctions, against the oracle's masks (pinned to
+    bytes.find and byte slices in tests/test_oracle.py)."""
+    from tqp_amd import kernels
+    g = np.random.default_rng(29)
+    n, w = 150_001, 25
+    s = np.array([ord(c) for c in "abcd "], dtype=np.int8)[g.integers(0, 5, (n, w))]
+    s[g.random((n, w)) < 0.08] = 0
+    rows = np.flatnonzero(g.random(n) < 0.2)
+    for r, p in zip(rows, g.integers(0, w - 4, rows.shape[0])):
+        s[r, p:p + 5] = [ord(c) for c in "green"]
+    tables = {4: ('string', s), 5: ('int', g.integers(0, 100, n, dtype=np.int32))}
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    for tree in (['Contains', ['c#4', 'green']],
+                 ['EndsWith', ['c#4', 'dab']],
+                 ['NOT', ['Contains', ['c#4', 'a b']]],
+                 [['Contains', ['c#4', 'cc']], 'AND', ['k#5', '<', '30']],
+                 [['substring', ['c#4', '3', '2']], 'IN', ['ab', 'gr', 'ee']],
+                 [['substring', ['c#4', '1', '3']], '=', 'abc'],
+                 [['substring', ['c#4', '20', '10']], 'IN', ['green', 'ab']],
+                 ['Contains', ['c#4', 'x' * 30]]):
+        print("string function:", tree, flush=True)
+        mask, counts = kernels.filter_mask(tree, gpu, None)
+        want = R.filter_evaluate(cpu, tree, None, None)
+        _assert_col(mask, want, f"mask {tree}")
+        assert int(counts.sum().item()) == int(want.sum().item())
+
+
+@pytest.mark.parametrize("case", ["string", "float", "string_and_int", "float32_masked"])
+def test_aggregate_string_and_float_keys(device, case):
+    """Group keys that do not pack into an integer range (2-D string rows of width 12, float64
+    and float32 values with -0.0 and +0.0 in one group) are grouped by their dense rank in sort
+    order; groups, key values (2-D [G, 12] for strings), counts and sums equal the oracle's
+    (aggregate.py:302-346, pinned to exact grouping in tests/test_oracle.py)."""
+    g = np.random.default_rng(sum(map(ord, case)))
+    n = 250_000
+    words = [b"ALGERIA", b"ARGENTINA", b"BRAZIL", b"CANADA", b"EGYPT", b"ETHIOPIA", b"FRANCE",
+             b"GERMANY", b"INDIA", b"INDONESIA", b"IRAN", b"IRAQ", b"JAPAN", b"JORDAN"]
+    names = np.zeros((len(words), 12), dtype=np.int8)
+    for i, wd in enumerate(words):
+        names[i, :len(wd)] = np.frombuffer(wd, dtype=np.int8)
+    fvals = np.array([-2.5, -0.0, 0.0, 0.125, 1.0, 3.75, 1e12, -1e-12], dtype=np.float64)
+    keys = {}
+    if case in ("string", "string_and_int"):
+        keys[10] = ('string', names[g.integers(0, len(words), n)])
+    if case == "string_and_int":
+        keys[11] = ('int', g.integers(0, 7, n, dtype=np.int32))
+    if case == "float":
+        keys[10] = ('float', fvals[g.integers(0, len(fvals), n)])
+    if case == "float32_masked":
+        keys[10] = ('float', fvals[g.integers(0, len(fvals), n)].astype(np.float32))
+    vals = {20: ('float', g.uniform(900, 105000, n)), 22: ('int', g.integers(-1000, 1000, n))}
+    mask = g.random(n) < 0.8 if case == "float32_masked" else None
+    key_ids = sorted(keys)
+    results = [f"k#{i}" for i in key_ids] + [
+        [[['sum', ['v#20']], '#', '1'], 'AS', 's#50'],
+        [[['count', ['1']], '#', '2'], 'AS', 'c#51'],
+        [[['max', ['i#22']], '#', '3'], 'AS', 'mx#52']]
+    gpu, cpu, errs = _agg_case(device, keys, vals, mask, results, key_ids, [50, 51, 52])
+    assert errs == [None, None], errs
+    for i in key_ids:
+        _assert_col(gpu[i].tensor, cpu[i].tensor, f"{case} key {i}")
+    _assert_col(gpu[51].tensor, cpu[51].tensor, f"{case} count")
+    _assert_col(gpu[52].tensor, cpu[52].tensor, f"{case} max")
+    _assert_col(gpu[50].tensor, cpu[50].tensor, f"{case} sum", fp=True)
+    if case.startswith("string"):
+        assert gpu[10].tensor.shape[1] == 12

@@ -340,3 +340,98 @@ def test_planted_mistakes_are_caught(monkeypatch):
     v = g.uniform(900, 105000, 10000)
     seq = CF.seq_sum(v)
     assert abs(float(np.sum(v.astype(np.float32), dtype=np.float32)) - seq) > 1e-12 * seq
+
+
+def test_aggregate_float_and_string_keys_group_by_exact_value():
+    """Float keys group by value (-0.0 with +0.0), 2-D string keys by their bytes; the key
+    outputs are the groups' values in their own dtype and shape (aggregate.py:302-346)."""
+    g = np.random.default_rng(41)
+    n = 4000
+    names = np.array([list(b"alpha\0"), list(b"beta\0\0"), list(b"alp\0\0\0"), list(b"zeta\0\0")],
+                     dtype=np.int8)
+    s = names[g.integers(0, 4, n)]
+    f = g.choice(np.array([-1.5, -0.0, 0.0, 0.25, 0.3, 1e300, -7.0]), n)
+    v = g.uniform(0, 10, n)
+    m = g.random(n) < 0.9
+    tg = {1: R.Col(torch.from_numpy(s), 'string'), 2: R.Col(torch.from_numpy(f), 'float'),
+          3: R.Col(torch.from_numpy(v), 'float'), 900: R.Col(torch.from_numpy(m), 'bool')}
+    res = ['a#1', 'b#2', [[['sum', ['v#3']], '#', '1'], 'AS', 's#20'],
+           [[['count', ['1']], '#', '2'], 'AS', 'c#21']]
+    args = _agg_args(['a#1', 'b#2'], res, [20, 21], [1, 2, 3], {1: 900, 2: 900, 3: 900})
+    R.tqp_hash_aggregate(tg, args)
+    keys, vals = CF.group_reduce([s, f], m, {'s': ('sum', v), 'c': ('count', np.ones(n))})
+    assert tg[1].tensor.dtype == torch.int8 and tg[1].tensor.shape[1] == 6
+    assert tg[1].tensor.numpy().tolist() == keys[0].tolist()
+    assert tg[2].tensor.dtype == torch.float64 and tg[2].tensor.shape[1] == 1
+    got_f = tg[2].tensor.reshape(-1).numpy()
+    assert np.array_equal(np.where(got_f == 0, 0.0, got_f), np.where(keys[1] == 0, 0.0, keys[1]))
+    assert len({(a, b) for a, b in zip(map(bytes, keys[0].astype(np.uint8)), keys[1])}) == len(got_f)
+    assert tg[21].tensor.tolist() == vals['c']
+    np.testing.assert_allclose(tg[20].tensor.numpy(), vals['s'], rtol=1e-12)
+
+
+def test_year_case_division_and_not_equal_match_closed_forms():
+    """year (aggregate.py:97-101) against the calendar, CASE WHEN (aggregate.py:74-85) against
+    numpy.where, '/' as true division and '<>' as numpy's !=, each inside a sum."""
+    g = np.random.default_rng(43)
+    n = 2500
+    day = g.integers(-5, 3652, n, dtype=np.int32)
+    q = g.integers(1, 51, n).astype(np.float64)
+    x = g.integers(-10, 10, n, dtype=np.int64)
+    k = g.integers(0, 3, n, dtype=np.int8)
+    tg = {1: R.Col(torch.from_numpy(k), 'string'), 2: R.Col(torch.from_numpy(day), 'date'),
+          3: R.Col(torch.from_numpy(q), 'float'), 4: R.Col(torch.from_numpy(x), 'int')}
+    res = ['k#1',
+           [[['sum', [['year', ['d#2']]]], '#', '1'], 'AS', 'y#20'],
+           [[['sum', [['CASE', 'WHEN', ['x#4', '>', '0'], 'THEN', 'q#3', 'ELSE', '0.0', 'END']]],
+             '#', '2'], 'AS', 'cw#21'],
+           [[['sum', [['x#4', '/', '4']]], '#', '3'], 'AS', 'dv#22'],
+           [[['sum', [['x#4', '<>', '0']]], '#', '4'], 'AS', 'ne#23']]
+    args = _agg_args(['k#1'], res, [20, 21, 22, 23], [1, 2, 3, 4])
+    R.tqp_hash_aggregate(tg, args)
+    years = np.array([CF.year_of_day(d) if d >= 0 else 1990 for d in day], dtype=np.int64)
+    _, vals = CF.group_reduce([k], None, {
+        'y': ('sum', years), 'cw': ('sum', np.where(x > 0, q, 0.0)),
+        'dv': ('sum', (x / 4).astype(np.float32)), 'ne': ('sum', (x != 0).astype(np.int64))})
+    assert tg[20].tensor.tolist() == vals['y'] and tg[20].tensor.dtype == torch.int64
+    np.testing.assert_allclose(tg[21].tensor.numpy(), vals['cw'], rtol=1e-12)
+    np.testing.assert_allclose(tg[22].tensor.numpy(), vals['dv'], rtol=1e-6)
+    assert tg[23].tensor.tolist() == vals['ne']
+
+
+def test_year_fails_past_1999_as_the_reference_does():
+    """A day count of 3652 (2000-01-01) indexes past the 10-entry year table."""
+    tg = {1: R.Col(torch.tensor([0, 0], dtype=torch.int8), 'string'),
+          2: R.Col(torch.tensor([100, 3652], dtype=torch.int32), 'date')}
+    args = _agg_args(['k#1'], ['k#1', [[['sum', [['year', ['d#2']]]], '#', '1'], 'AS', 'y#20']],
+                     [20], [1, 2])
+    with pytest.raises(IndexError):
+        R.tqp_hash_aggregate(tg, args)
+    assert CF.year_of_day(3651) == 1999 and CF.year_of_day(3652) == 2000
+
+
+def test_contains_endswith_substring_match_closed_forms():
+    """Contains and EndsWith (filter.py:56-62 via like.py:3-11: the word at any position of the
+    row, padding included) against bytes.find over each row; substring (filter.py:71-76) IN /
+    '=' against the row's bytes [l - 1, l - 1 + w) compared with the zero-padded literals."""
+    g = np.random.default_rng(47)
+    n, w = 3000, 15
+    s = np.array([ord(c) for c in "abcd "], dtype=np.int8)[g.integers(0, 5, (n, w))]
+    s[g.random((n, w)) < 0.1] = 0
+    rows = np.flatnonzero(g.random(n) < 0.2)
+    for r, p in zip(rows, g.integers(0, w - 4, rows.shape[0])):
+        s[r, p:p + 5] = [ord(c) for c in "green"]
+    tg = {1: R.Col(torch.from_numpy(s), 'string')}
+    raw = [bytes(row.astype(np.uint8)) for row in s]
+    for head in ('Contains', 'EndsWith'):
+        for word in ("green", "ab", "d a", "x"):
+            got = R.filter_evaluate(tg, [head, ['c#1', word]], None, None).numpy()
+            want = np.array([r.find(word.encode()) >= 0 for r in raw])
+            assert np.array_equal(got, want), (head, word)
+    got = R.filter_evaluate(tg, [['substring', ['c#1', '2', '3']], 'IN', ['abc', 'gre', 'ree']],
+                            None, None).numpy()
+    want = np.array([r[1:4] in (b"abc", b"gre", b"ree") for r in raw])
+    assert np.array_equal(got, want)
+    got = R.filter_evaluate(tg, [['substring', ['c#1', '1', '2']], '=', 'ab'], None, None).numpy()
+    want = np.array([r[0:2] == b"ab" for r in raw])
+    assert np.array_equal(got, want) and want.any()

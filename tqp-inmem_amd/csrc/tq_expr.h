@@ -109,6 +109,18 @@ __device__ __forceinline__ bool str_like2(const int8_t* sp, int w, const uint8_t
   return false;
 }
 
+// pool[off:off+len] at some position of one row's string bytes sp[0, w) (operators/like.py:3-11).
+__device__ __forceinline__ bool str_contains(const int8_t* sp, int w, const uint8_t* pool,
+                                             int64_t imm) {
+  const int off = (int)(imm & 0xffffffff), len = (int)(imm >> 32);
+  for (int pos = 0; pos + len <= w; ++pos) {
+    bool m = true;
+    for (int c = 0; c < len && m; ++c) m = sp[pos + c] == (int8_t)pool[off + c];
+    if (m) return true;
+  }
+  return false;
+}
+
 // One row's eight registers, kept as named scalars and selected by switch: an indexable array
 // would be placed in private scratch memory by the compiler.
 struct Regs {
@@ -153,19 +165,25 @@ __device__ __forceinline__ void run_program(const tq_program_t& P, const tq_cols
       case TQ_OP_LOAD: o = stage[(I.a * kSub + j) * kBlock + t]; break;
       case TQ_OP_CONST: o = (uint64_t)I.imm; break;
       case TQ_OP_STR_LIKE2:
-        o = valid && str_like2((const int8_t*)col_ptr(C, I.a) + row * (int64_t)col_width(C, I.a),
-                               col_width(C, I.a), P.strpool, I.imm);
+      case TQ_OP_STR_CONTAINS: {
+        const int stride = col_width(C, I.a);
+        const int8_t* sp = (const int8_t*)col_ptr(C, I.a) + row * (int64_t)stride;
+        const int w = I.c ? (int)I.c : stride;
+        o = valid && (I.op == TQ_OP_STR_LIKE2 ? str_like2(sp, w, P.strpool, I.imm)
+                                              : str_contains(sp, w, P.strpool, I.imm));
         break;
+      }
       case TQ_OP_STR_EQ:
       case TQ_OP_STR_PREFIX: {
         const int8_t* p = (const int8_t*)col_ptr(C, I.a);
-        const int w = col_width(C, I.a);
+        const int stride = col_width(C, I.a);
+        const int w = I.c ? (int)I.c : stride;
         const int off = (int)(I.imm & 0xffffffff);
         const int len = (int)(I.imm >> 32);
         const int upto = (I.op == TQ_OP_STR_EQ) ? w : len;
         bool eq = valid;
         if (eq) {
-          const int8_t* s = p + row * (int64_t)w;
+          const int8_t* s = p + row * (int64_t)stride;
           for (int c = 0; c < upto; ++c) {
             const int8_t want = c < len ? (int8_t)P.strpool[off + c] : (int8_t)0;
             eq = eq && (s[c] == want);
@@ -302,18 +320,25 @@ __device__ __forceinline__ void run_rows(const tq_program_t& P, const tq_colset_
     switch (I.op) {
       case TQ_OP_LOAD: TQ_EACH(stage[(I.a * kSub + j) * kBlock + t]) break;
       case TQ_OP_CONST: TQ_EACH(I.imm) break;
-      case TQ_OP_STR_LIKE2: {
+      case TQ_OP_STR_LIKE2:
+      case TQ_OP_STR_CONTAINS: {
         const int8_t* p = (const int8_t*)col_ptr(C, I.a);
-        const int w = col_width(C, I.a);
+        const int stride = col_width(C, I.a);
+        const int w = I.c ? (int)I.c : stride;
+        const bool like = I.op == TQ_OP_STR_LIKE2;
 #pragma unroll
-        for (int j = 0; j < kSub; ++j)
-          o[j] = valid[j] && str_like2(p + row[j] * (int64_t)w, w, P.strpool, I.imm);
+        for (int j = 0; j < kSub; ++j) {
+          const int8_t* sp = p + row[j] * (int64_t)stride;
+          o[j] = valid[j] && (like ? str_like2(sp, w, P.strpool, I.imm)
+                                   : str_contains(sp, w, P.strpool, I.imm));
+        }
         break;
       }
       case TQ_OP_STR_EQ:
       case TQ_OP_STR_PREFIX: {
         const int8_t* p = (const int8_t*)col_ptr(C, I.a);
-        const int w = col_width(C, I.a);
+        const int stride = col_width(C, I.a);
+        const int w = I.c ? (int)I.c : stride;
         const int off = (int)(I.imm & 0xffffffff);
         const int len = (int)(I.imm >> 32);
         const int upto = (I.op == TQ_OP_STR_EQ) ? w : len;
@@ -321,7 +346,7 @@ __device__ __forceinline__ void run_rows(const tq_program_t& P, const tq_colset_
         for (int j = 0; j < kSub; ++j) {
           bool eq = valid[j];
           if (eq) {
-            const int8_t* sp = p + row[j] * (int64_t)w;
+            const int8_t* sp = p + row[j] * (int64_t)stride;
             for (int c = 0; c < upto; ++c) {
               const int8_t want = c < len ? (int8_t)P.strpool[off + c] : (int8_t)0;
               eq = eq && (sp[c] == want);

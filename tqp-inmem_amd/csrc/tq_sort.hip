@@ -998,3 +998,134 @@ extern "C" int tq_sort_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, 
   const int64_t m = (limit >= 0 && limit < n) ? limit : n;
   return launch("k_perm_out", k_perm_out, stream_grid(m), kBlock, 0, s, perm, m, perm_out);
 }
+
+// ------------------------------------------------------------------ dense ranks of sorted rows
+// Rows in a stable sort order (perm) get the rank of their key among the distinct keys: the
+// number of key changes up to their position, so equal keys share a rank and ranks ascend
+// with the order.  Floats compare as the sort orders them (-0.0 equals +0.0, every NaN equals
+// every other NaN); a 2-D key compares its row bytes.  A group-by over keys that do not pack
+// into integer ranges (strings, floats) groups by these ranks instead.
+namespace tq {
+
+struct RankKeys {
+  const void* ptr[TQ_GB_MAX_KEYS];
+  int32_t dtype[TQ_GB_MAX_KEYS];
+  int32_t width[TQ_GB_MAX_KEYS];
+  int32_t n;
+};
+
+__device__ __forceinline__ bool rank_rows_equal(const RankKeys& K, int64_t a, int64_t b) {
+  for (int k = 0; k < K.n; ++k) {
+    const int dt = K.dtype[k];
+    const int w = K.width[k];
+    if (w > 1) {
+      const int8_t* p = (const int8_t*)K.ptr[k];
+      for (int c = 0; c < w; ++c)
+        if (p[a * w + c] != p[b * w + c]) return false;
+    } else if (dt == TQ_F64) {
+      const double x = ((const double*)K.ptr[k])[a], y = ((const double*)K.ptr[k])[b];
+      if (!(x == y || (x != x && y != y))) return false;
+    } else if (dt == TQ_F32) {
+      const float x = ((const float*)K.ptr[k])[a], y = ((const float*)K.ptr[k])[b];
+      if (!(x == y || (x != x && y != y))) return false;
+    } else if (load_int(K.ptr[k], dt, a) != load_int(K.ptr[k], dt, b)) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// Sorted positions tile0 + 16 * t + j (j < 16) of a 4096-position tile: bit j set when the
+// key differs from the previous position's (position 0 always starts a rank).
+__device__ __forceinline__ uint32_t rank_starts(const RankKeys& K, const int64_t* perm, int64_t n,
+                                                int64_t tile0) {
+  const int64_t first = tile0 + 16 * (int64_t)threadIdx.x;
+  uint32_t bits = 0;
+  if (first >= n) return 0;
+  int64_t prev = first > 0 ? perm[first - 1] : -1;
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = first + j;
+    if (i >= n) break;
+    const int64_t row = perm[i];
+    if (prev < 0 || !rank_rows_equal(K, prev, row)) bits |= 1u << j;
+    prev = row;
+  }
+  return bits;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rank_count(RankKeys K, const int64_t* __restrict__ perm,
+                                                       int64_t n, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t wc[kWaves];
+  const uint32_t c = wave_sum((uint32_t)__popc(rank_starts(K, perm, n, (int64_t)blockIdx.x * kTileRows)));
+  if (lane_id() == 0) wc[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) b += wc[i];
+    counts[blockIdx.x] = b;
+  }
+}
+
+// rank_out[perm[i]] = rank of position i; rep_out[r] = the first row (in sort order) of rank r.
+__global__ __launch_bounds__(kBlock) void k_rank_write(RankKeys K, const int64_t* __restrict__ perm,
+                                                       int64_t n, const int64_t* __restrict__ tile_off,
+                                                       int32_t* __restrict__ rank_out,
+                                                       int64_t* __restrict__ rep_out) {
+  __shared__ uint32_t sh[kWaves];
+  const int64_t tile0 = (int64_t)blockIdx.x * kTileRows;
+  const uint32_t bits = rank_starts(K, perm, n, tile0);
+  uint32_t tot;
+  const uint32_t before = block_exclusive_sum((uint32_t)__popc(bits), sh, &tot);
+  int64_t r = tile_off[blockIdx.x] + before - 1;  // rank of the position before this thread's
+  const int64_t first = tile0 + 16 * (int64_t)threadIdx.x;
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = first + j;
+    if (i >= n) break;
+    const int64_t row = perm[i];
+    if ((bits >> j) & 1u) {
+      ++r;
+      rep_out[r] = row;
+    }
+    rank_out[row] = (int32_t)r;
+  }
+}
+
+}  // namespace tq
+
+extern "C" size_t tq_rank_ws_bytes(int64_t n) {
+  const int64_t nt = ceil_div(n > 0 ? n : 1, kTileRows);
+  return align_up((size_t)nt * 4, 256) + align_up((size_t)nt * 8, 256) + tq_scan_ws_bytes(nt);
+}
+
+extern "C" int tq_rank_sorted(const tq_sortkey_t* keys, int32_t nkeys, int64_t n,
+                              const int64_t* perm, int32_t* rank_out, int64_t* rep_out,
+                              int64_t* nranks, void* ws, size_t ws_bytes, void* stream) {
+  if (!keys || nkeys < 1 || nkeys > TQ_GB_MAX_KEYS || !nranks)
+    return set_error(TQ_EINVAL, "rank: bad keys");
+  if (n < 0 || n >= ((int64_t)1 << 31)) return set_error(TQ_EUNSUPPORTED, "rank: %lld rows", (long long)n);
+  if (n > 0 && (!perm || !rank_out || !rep_out || !ws))
+    return set_error(TQ_EINVAL, "rank: null argument");
+  if (ws_bytes < tq_rank_ws_bytes(n)) return set_error(TQ_ECAPACITY, "rank: workspace too small");
+  RankKeys K{};
+  K.n = nkeys;
+  for (int k = 0; k < nkeys; ++k) {
+    if (!keys[k].ptr && n > 0) return set_error(TQ_EINVAL, "rank: null key column");
+    K.ptr[k] = keys[k].ptr;
+    K.dtype[k] = keys[k].dtype;
+    K.width[k] = keys[k].width > 0 ? keys[k].width : 1;
+    if (K.width[k] > 1 && K.dtype[k] != TQ_I8)
+      return set_error(TQ_EUNSUPPORTED, "rank: 2-D keys must be int8 strings");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nt = ceil_div(n > 0 ? n : 1, kTileRows);
+  char* w = (char*)ws;
+  uint32_t* counts = (uint32_t*)w;
+  int64_t* off = (int64_t*)(w + align_up((size_t)nt * 4, 256));
+  void* sws = w + align_up((size_t)nt * 4, 256) + align_up((size_t)nt * 8, 256);
+  if (n == 0) return fill_u64(nranks, 0, 1, s);
+  TQ_TRY(launch("k_rank_count", k_rank_count, (unsigned)nt, kBlock, 0, s, K, perm, n, counts));
+  TQ_TRY(tq_scan_tiles(counts, nt, off, nranks, sws, s));
+  return launch("k_rank_write", k_rank_write, (unsigned)nt, kBlock, 0, s, K, perm, n,
+                (const int64_t*)off, rank_out, rep_out);
+}

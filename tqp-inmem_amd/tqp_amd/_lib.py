@@ -40,6 +40,7 @@ OPCODES = [
     "STR_EQ", "STR_PREFIX",
     "YEAR",
     "STR_LIKE2",
+    "STR_CONTAINS",
 ]
 OP = {name: i for i, name in enumerate(OPCODES)}
 
@@ -171,6 +172,8 @@ _SIGNATURES = {
                                I32),
     "tq_join_settle": ([P, P, P], I32),
     "tq_read_words": ([I32, P, P, P, P], I32),
+    "tq_rank_ws_bytes": ([I64], SZ),
+    "tq_rank_sorted": ([P, I32, I64, P, P, P, P, P, SZ, P], I32),
     "tq_groupjoin_pres_bytes": ([I64], SZ),
     "tq_groupjoin_ws_bytes": ([I64, I64, I64, I32, I32], SZ),
     "tq_groupjoin_run": ([P, P, P, P, SZ, P, SZ, P, P], I32),

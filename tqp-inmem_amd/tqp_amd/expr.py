@@ -28,13 +28,16 @@ L = _lib
 
 class Value:
     """A compiled sub-expression: a register of a dtype, a Python literal, a string literal,
-    or a 2-D string column (which only string predicates can consume)."""
+    or a 2-D string column (which only string predicates can consume).  A string column made by
+    `substring` is narrow: its column-set entry starts at the substring's first byte, keeps the
+    full row width as its stride, and `width` is the substring's length."""
 
-    __slots__ = ("kind", "reg", "dtype", "value", "col", "width")
+    __slots__ = ("kind", "reg", "dtype", "value", "col", "width", "narrow")
 
-    def __init__(self, kind, reg=None, dtype=None, value=None, col=None, width=None):
+    def __init__(self, kind, reg=None, dtype=None, value=None, col=None, width=None,
+                 narrow=False):
         self.kind, self.reg, self.dtype, self.value = kind, reg, dtype, value
-        self.col, self.width = col, width
+        self.col, self.width, self.narrow = col, width, narrow
 
     @property
     def is_lit(self):
@@ -77,7 +80,8 @@ class ProgramBuilder:
         self.prog = L.Program()
         self.cols = L.ColSet()
         self.col_tensors = []       # keep tensors alive while the program is in use
-        self.col_ids = {}           # (address, dtype, shape) -> colset index
+        self.col_offsets = []       # byte offset of each colset entry into its tensor
+        self.col_ids = {}           # (address, dtype, shape, offset) -> colset index
         self.load_of = {}           # colset index -> load slot
         self.nload = 0
         self.ninstr = 0
@@ -88,8 +92,8 @@ class ProgramBuilder:
         self.n_col = None
 
     # ---------------------------------------------------------------- columns and registers
-    def column(self, t: torch.Tensor) -> int:
-        key = (t.data_ptr(), t.dtype, tuple(t.shape))
+    def column(self, t: torch.Tensor, offset: int = 0) -> int:
+        key = (t.data_ptr(), t.dtype, tuple(t.shape), offset)
         if key in self.col_ids:
             return self.col_ids[key]
         idx = len(self.col_tensors)
@@ -99,8 +103,9 @@ class ProgramBuilder:
         if not t.is_contiguous():
             raise ValueError("expression columns must be contiguous")
         self.col_tensors.append(t)
+        self.col_offsets.append(offset)
         self.col_ids[key] = idx
-        self.cols.ptr[idx] = t.data_ptr()
+        self.cols.ptr[idx] = t.data_ptr() + offset
         self.cols.width[idx] = t.shape[1] if t.dim() == 2 else 1
         return idx
 
@@ -264,13 +269,44 @@ class ProgramBuilder:
             raise NotImplementedError("string operand in a boolean expression")
         return v
 
+    def substring(self, v: Value, start: int, length: int) -> Value:
+        """Columns [start - 1, start - 1 + length) of a 2-D string column, as the reference's
+        `[:, l:l + w]` slice (filter.py:71-76, aggregate.py:115-120), clamped to the row."""
+        if v.kind != "col2d":
+            raise NotImplementedError("substring of a non-string value")
+        lo = start - 1
+        if lo < 0 or length < 0:
+            raise NotImplementedError("substring with a start before the first character")
+        t = self.col_tensors[v.col]
+        full = int(self.cols.width[v.col])
+        base = self.col_offsets[v.col]
+        lo = min(base + lo, full)
+        hi = min(lo + length, base + v.width, full)
+        if hi - lo > 255:
+            raise NotImplementedError("substring longer than 255 bytes")
+        return Value("col2d", col=self.column(t, lo), width=hi - lo, dtype=v.dtype, narrow=True)
+
+    @staticmethod
+    def _narrow(col: Value) -> int:
+        """The string instructions' c field: the compared width of a narrow column, else 0."""
+        return col.width if col.narrow else 0
+
     def str_eq(self, col: Value, s: str, prefix=False) -> Value:
         if col.kind != "col2d":
             raise NotImplementedError("string comparison needs a 2-D string column")
         if not prefix and len(s) > col.width:
             raise ValueError(f"string literal '{s}' longer than column width {col.width}")
         r = self.alloc()
-        self.emit("STR_PREFIX" if prefix else "STR_EQ", dst=r, a=col.col, imm=self.pool_string(s))
+        self.emit("STR_PREFIX" if prefix else "STR_EQ", dst=r, a=col.col, c=self._narrow(col),
+                  imm=self.pool_string(s))
+        return Value("reg", reg=r, dtype=torch.bool)
+
+    def str_contains(self, col: Value, s: str) -> Value:
+        """Rows of a 2-D string column holding s at some position (like.py:3-11)."""
+        if col.kind != "col2d":
+            raise NotImplementedError("Contains needs a 2-D string column")
+        r = self.alloc()
+        self.emit("STR_CONTAINS", dst=r, a=col.col, c=self._narrow(col), imm=self.pool_string(s))
         return Value("reg", reg=r, dtype=torch.bool)
 
     def str_like2(self, col: Value, w1: str, w2: str) -> Value:
@@ -280,7 +316,7 @@ class ProgramBuilder:
         p1, p2 = self.pool_string(w1), self.pool_string(w2)
         imm = ((p1 & 0xFFFF) | ((p1 >> 32) << 16) | ((p2 & 0xFFFF) << 32) | ((p2 >> 32) << 48))
         r = self.alloc()
-        self.emit("STR_LIKE2", dst=r, a=col.col, imm=imm)
+        self.emit("STR_LIKE2", dst=r, a=col.col, c=self._narrow(col), imm=imm)
         return Value("reg", reg=r, dtype=torch.bool)
 
     def load_bytes(self) -> int:
@@ -325,6 +361,14 @@ def compile_predicate(b: ProgramBuilder, tree, tensors, subquery_result=None, le
             if head == 'StartsWith':
                 col = b.load_column(tensors[get_id(tree[1][0])].tensor)
                 return b.str_eq(col, tree[1][1], prefix=True)
+            if head in ('Contains', 'EndsWith'):
+                # filter.py:56-62: both run like_contains over the whole row
+                col = b.load_column(tensors[get_id(tree[1][0])].tensor)
+                return b.str_contains(col, tree[1][1])
+            if head == 'substring':
+                assert len(tree[1]) == 3
+                sub, l, w = tree[1]
+                return b.substring(ev(sub), int(l), int(w))
             if head == 'NOT':
                 v = b.truth(ev(tree[1]))
                 b.emit("NOT", dst=v.reg, a=v.reg)
@@ -436,6 +480,10 @@ def compile_row_expr(b: ProgramBuilder, tree, tensors, agg_ref=None) -> Value:
             if head == 'StartsWith':
                 col = b.load_column(tensors[get_id(tree[1][0])].tensor)
                 return b.str_eq(col, tree[1][1], prefix=True)
+            if head == 'substring':
+                assert len(tree[1]) == 3
+                sub, l, w = tree[1]
+                return b.substring(ev(sub), int(l), int(w))
             if head == 'year':
                 v = b.to_dtype(ev(tree[1]), torch.int64)
                 b.emit("YEAR", dst=v.reg, a=v.reg)
@@ -525,12 +573,12 @@ def cached_program(key, tensors, compile_fn):
         prog, refs, n_col, lb, extra = ent
         cols = L.ColSet()
         col_tensors = []
-        for i, (cid, dt, tail, width) in enumerate(refs):
+        for i, (cid, dt, tail, width, off) in enumerate(refs):
             v = tensors.get(cid)
             t = v.tensor if v is not None else None
             if t is None or t.dtype != dt or tuple(t.shape[1:]) != tail or not t.is_contiguous():
                 break
-            cols.ptr[i] = t.data_ptr()
+            cols.ptr[i] = t.data_ptr() + off
             cols.width[i] = width
             col_tensors.append(t)
         else:
@@ -546,7 +594,7 @@ def cached_program(key, tensors, compile_fn):
         cid = ids.get(id(t))
         if cid is None:
             return b
-        refs.append((cid, t.dtype, tuple(t.shape[1:]), int(b.cols.width[i])))
+        refs.append((cid, t.dtype, tuple(t.shape[1:]), int(b.cols.width[i]), b.col_offsets[i]))
     if len(_PROGRAMS) >= _PROGRAMS_MAX:
         _PROGRAMS.clear()
     _PROGRAMS[key] = (b.prog, refs, b.n_col, b.load_bytes(), extra)

@@ -442,14 +442,21 @@ class GroupBy:
 
     def __init__(self, keys, aggs, tensors, mask=None, n=None):
         self.keys = []
+        # keys that do not pack into an integer range (2-D string rows, floats) are grouped by
+        # their dense rank in sort order: position -> (the key column, first row of each rank)
+        self.ranked = {}
         for k in keys:
             L.require_gpu(k, "group key")
-            if k.dim() == 2:
-                if k.shape[1] != 1:
-                    raise NotImplementedError("multi-column (string) group keys")
+            if k.dim() == 2 and k.shape[1] == 1:
                 k = k.reshape(-1)
-            if k.dtype.is_floating_point:
-                raise NotImplementedError("floating-point group keys")
+            if k.dim() == 2 or k.dtype.is_floating_point:
+                df = _deferred()
+                if df is not None and df.join is not None and df.m is None:
+                    n = df.resolve_join()   # ranks are taken over the real rows
+                rows = k.shape[0] if n is None else n
+                rank, first = dense_ranks(k, rows)
+                self.ranked[len(self.keys)] = (k, first)
+                k = rank
             self.keys.append(k.contiguous())
         if len(self.keys) > L.TQ_GB_MAX_KEYS:
             raise NotImplementedError("more than 8 group keys")
@@ -642,9 +649,41 @@ class GroupBy:
         G = int(ng.value)
         for kern, nbytes in gb_bytes(nsel, G).items():
             L.trace_set_bytes(at, kern, nbytes)
-        key_out = [o[:G] for o in key_full]
+        key_out = self._key_values([o[:G] for o in key_full])
         agg_out = [o[:G] for o in agg_full]
         return key_out, agg_out
+
+    def _key_values(self, key_out):
+        """Ranked keys' groups back to their values: each group's rank picks its first row."""
+        for pos, (col, first) in self.ranked.items():
+            rows = gather(key_out[pos], [first])[0]
+            key_out[pos] = gather(rows, [col])[0]
+        return key_out
+
+
+def dense_ranks(t: torch.Tensor, n: int):
+    """Dense ranks of the first n rows of a key column in ascending sort order (string rows
+    byte-wise, floats as torch.sort orders them): (int32 rank per row, int64 first row of each
+    rank).  Reads the number of ranks back (one synchronisation) and records it as the rank
+    column's bounds."""
+    t = t[:n].contiguous()
+    dev = t.device
+    perm = sort_perm([(t, False)], n, device=dev)
+    rank = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    first = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    nr = torch.empty(1, dtype=torch.int64, device=dev)
+    key = (L.SortKey * 1)()
+    key[0].ptr = t.data_ptr()
+    key[0].dtype = L.dtype_code(t.dtype)
+    key[0].width = t.shape[1] if t.dim() == 2 else 1
+    wsb = L.size_call("tq_rank_ws_bytes", n)
+    ws = L.workspace(wsb, dev, "rank")
+    L.call("tq_rank_sorted", key, 1, n, L.ptr(perm), L.ptr(rank), L.ptr(first), L.ptr(nr),
+           L.ptr(ws), wsb, L.stream_handle())
+    d = int(nr.item())
+    rank = rank[:n]
+    note_bounds(rank, 0, max(d - 1, 0))
+    return rank, first
 
 
 def mark_rows(idx: torch.Tensor, n: int, device):

@@ -157,7 +157,9 @@ def agg_group(columns, key_ids, results, mask=None):
     for out_id, kind, what in plans:
         if kind == 'key':
             pos = key_ids.index(what)
-            out[out_id] = key_out[pos].reshape(-1, 1).to(columns[what].tensor.dtype)
+            kv = key_out[pos]
+            # keys come out 2-D, [G, 1] or [G, width] for string rows (aggregate.py:343-346)
+            out[out_id] = (kv if kv.dim() == 2 else kv.reshape(-1, 1)).to(columns[what].tensor.dtype)
             continue
         body = _strip_alias(what)
         if id(body) in by_call:
