@@ -537,6 +537,20 @@ int tq_groupjoin_run(const tq_groupjoin_desc_t* desc /* host */, const tq_progra
                      const tq_colset_t* cols /* host */, void* pres, size_t pres_bytes, void* ws,
                      size_t ws_bytes, int64_t** words /* host out */, void* stream);
 
+/* tq_groupjoin_run without any allocation, for callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The workspace and the presence bytes live in a scratch array
+ * the library holds on every GPU it runs on (64 MiB, until the process ends); desc->epoch is
+ * ignored (the library keeps the epochs of its scratch).  The caller passes the inputs and
+ * outputs as for tq_groupjoin_run, plus words_out (device, 8 int64) for the result words.
+ * Fails with TQ_ECAPACITY, launching nothing, when the presence bytes and the workspace
+ * (tq_groupjoin_pres_bytes + tq_groupjoin_ws_bytes) exceed the scratch.  Calls that overlap on
+ * one GPU share its scratch and race. */
+int tq_groupjoin_run_noalloc(const tq_groupjoin_desc_t* desc /* host */,
+                             const tq_program_t* prog /* host */,
+                             const tq_colset_t* cols /* host */, int64_t* words_out,
+                             void* stream);
+
 /* ---------------------------------------------------------------- batched pipelines */
 /* Many filter -> join -> group-by pipelines in one persistent launch.  A join pipeline (mode 0)
  * joins its selected build rows (unique build keys) with its selected probe rows on key equality

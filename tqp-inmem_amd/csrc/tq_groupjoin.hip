@@ -714,9 +714,13 @@ extern "C" size_t tq_groupjoin_pres_bytes(int64_t key_range) {
   return align_up((size_t)(key_range > 0 ? key_range : 1), 256);
 }
 
-extern "C" int tq_groupjoin_run(const tq_groupjoin_desc_t* d, const tq_program_t* prog,
-                                const tq_colset_t* cols, void* pres, size_t pres_bytes, void* ws,
-                                size_t ws_bytes, int64_t** words, void* stream) {
+namespace tq {
+
+// The launches of one call.  The result words go to words_ext when given, else into ws (their
+// address is returned through *words).
+static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program_t* prog,
+                  const tq_colset_t* cols, void* pres, size_t pres_bytes, void* ws,
+                  size_t ws_bytes, int64_t* words_ext, int64_t** words, void* stream) {
   if (!d || !prog || !cols || !words) return set_error(TQ_EINVAL, "groupjoin: null argument");
   if (d->nb < 0 || d->np < 0) return set_error(TQ_EINVAL, "groupjoin: negative row count");
   if ((d->nb > 0 && !d->bkey) || (d->np > 0 && !d->pkey))
@@ -732,7 +736,7 @@ extern "C" int tq_groupjoin_run(const tq_groupjoin_desc_t* d, const tq_program_t
   if (prog->nseg != d->naggs)
     return set_error(TQ_EINVAL, "groupjoin: %d program segments for %d aggregates", prog->nseg,
                      d->naggs);
-  if (d->epoch < 1 || d->epoch > 255) return set_error(TQ_EINVAL, "groupjoin: epoch %u", d->epoch);
+  if (epoch < 1 || epoch > 255) return set_error(TQ_EINVAL, "groupjoin: epoch %u", epoch);
   GjPlan p;
   TQ_TRY(gj_plan(d->nb, d->np, range, d->naggs, prog->nload, &p));
   if (!ws || ws_bytes < p.bytes)
@@ -756,7 +760,7 @@ extern "C" int tq_groupjoin_run(const tq_groupjoin_desc_t* d, const tq_program_t
   O.cap = d->out_cap;
   hipStream_t s = (hipStream_t)stream;
   char* w = (char*)ws;
-  int64_t* wd = (int64_t*)(w + p.off_words);
+  int64_t* wd = words_ext ? words_ext : (int64_t*)(w + p.off_words);
   uint64_t* flags = (uint64_t*)(w + p.off_flags);
   uint32_t* btile = (uint32_t*)(w + p.off_btile);
   uint32_t* loff = (uint32_t*)(w + p.off_loff);
@@ -764,20 +768,20 @@ extern "C" int tq_groupjoin_run(const tq_groupjoin_desc_t* d, const tq_program_t
   *words = wd;
   const unsigned bgrid = (unsigned)(d->nb > 0 ? p.nbt : 1);
   TQ_TRY(launch("k_gj_build", k_gj_build, bgrid, kBlock, 0, s, d->bkey, (int)d->bkey_dtype,
-                d->bmask, d->nb, d->key_lo, (uint64_t)range, (uint8_t*)pres, (uint32_t)d->epoch,
+                d->bmask, d->nb, d->key_lo, (uint64_t)range, (uint8_t*)pres, epoch,
                 btile, flags, p.S, wd));
   if (d->np > 0) {
     const size_t lds = (size_t)p.stage_words * 8 + (size_t)((p.S + 1) & ~1) * 4 +
                        (size_t)kGjChunk * 4;
     TQ_TRY(gj_launch_lds("k_gj_probe", k_gj_probe, (unsigned)p.W, lds, s, G, *prog, *cols,
                          d->pkey, (int)d->pkey_dtype, d->pmask, d->np, d->key_lo,
-                         (uint64_t)range, (const uint8_t*)pres, (uint32_t)d->epoch, p.sb, p.S,
+                         (uint64_t)range, (const uint8_t*)pres, epoch, p.sb, p.S,
                          p.W, loff, rec, p.stage_words));
   } else {
     // no probe rows: every chunk's slice offsets are zero
     TQ_TRY(fill_u32(loff, 0u, (int64_t)(p.S + 1) * p.W, s));
   }
-  GjCheck K{(const uint8_t*)pres, (uint32_t)d->epoch, (uint64_t)range, btile,
+  GjCheck K{(const uint8_t*)pres, epoch, (uint64_t)range, btile,
             d->nb > 0 ? p.nbt : 0, wd, gj_spin_ticks()};
   const size_t rlds = ((size_t)8 * d->naggs + 4) << p.sb;
   // consecutive slices on one XCD when the remapped grid is resident all at once
@@ -788,4 +792,80 @@ extern "C" int tq_groupjoin_run(const tq_groupjoin_desc_t* d, const tq_program_t
   return gj_launch_lds("k_gj_reduce", k_gj_reduce, rgrid, rlds, s, G, p.sb, p.S, p.W,
                        (const uint32_t*)loff, (const uint64_t*)rec, d->key_lo, O, flags, K,
                        remap ? xper : 0);
+}
+
+}  // namespace tq
+
+extern "C" int tq_groupjoin_run(const tq_groupjoin_desc_t* d, const tq_program_t* prog,
+                                const tq_colset_t* cols, void* pres, size_t pres_bytes, void* ws,
+                                size_t ws_bytes, int64_t** words, void* stream) {
+  return gj_run(d, d ? d->epoch : 0u, prog, cols, pres, pres_bytes, ws, ws_bytes, nullptr, words,
+                stream);
+}
+
+// ------------------------------------------------------------------ without allocation
+// Callers that cannot allocate (graph capture, serving loops, harnesses that forbid it) run the
+// fused call on this library's own scratch: a file-scope device array, one copy per GPU, held
+// until the process ends.  Its front holds the presence bytes (their size only grows, zeroed
+// when it grows and when the epochs are used up), the rest the workspace.  Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU (different streams, or host threads)
+// race and give wrong results.
+namespace tq {
+
+constexpr size_t kGjScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[kGjScratchBytes];
+
+struct GjScratch {
+  uint8_t* ptr = nullptr;  // device address of this GPU's copy of `scratch`
+  size_t pres = 0;         // bytes at the front kept for presence bytes
+  uint32_t epoch = 255;    // last epoch used (255: zero before the next call)
+};
+static std::mutex g_scratch_mu;
+static std::map<int, GjScratch> g_scratch;
+
+}  // namespace tq
+
+extern "C" int tq_groupjoin_run_noalloc(const tq_groupjoin_desc_t* d, const tq_program_t* prog,
+                                        const tq_colset_t* cols, int64_t* words_out,
+                                        void* stream) {
+  if (!d || !prog || !cols || !words_out)
+    return set_error(TQ_EINVAL, "groupjoin (no allocation): null argument");
+  if (d->key_hi < d->key_lo)
+    return set_error(TQ_EINVAL, "groupjoin (no allocation): key bounds hi < lo");
+  const uint64_t urange = (uint64_t)d->key_hi - (uint64_t)d->key_lo + 1;
+  if (urange == 0 || urange > ((uint64_t)1 << 40))
+    return set_error(TQ_EUNSUPPORTED, "groupjoin (no allocation): key range too wide");
+  GjPlan p;
+  TQ_TRY(gj_plan(d->nb, d->np, (int64_t)urange, d->naggs, prog->nload, &p));
+  const size_t pres_b = tq_groupjoin_pres_bytes((int64_t)urange);
+  int dev = 0;
+  TQ_TRY_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  GjScratch& sc = g_scratch[dev];
+  const size_t front = pres_b > sc.pres ? pres_b : sc.pres;
+  if (front + p.bytes > kGjScratchBytes)
+    return set_error(TQ_ECAPACITY,
+                     "groupjoin (no allocation): %zu bytes of scratch needed (presence %zu, "
+                     "workspace %zu), the library holds %zu",
+                     front + p.bytes, front, p.bytes, kGjScratchBytes);
+  if (!sc.ptr) {
+    // this GPU's copy of the array: its device address, taken once with this device current
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(TQ_EHIP, "groupjoin (no allocation): scratch address: %s",
+                       hipGetErrorString(e));
+    sc.ptr = (uint8_t*)ptr;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (front > sc.pres || sc.epoch >= 255) {
+    // presence bytes grown over workspace bytes, or every epoch used: start again from zero
+    TQ_TRY(fill_u32(sc.ptr, 0u, (int64_t)(front / 4), s));
+    sc.pres = front;
+    sc.epoch = 0;
+  }
+  ++sc.epoch;
+  int64_t* unused = nullptr;
+  return gj_run(d, sc.epoch, prog, cols, sc.ptr, sc.pres, sc.ptr + sc.pres,
+                kGjScratchBytes - sc.pres, words_out, &unused, stream);
 }
