@@ -654,3 +654,54 @@ def test_aggregate_string_and_float_keys(device, case):
     _assert_col(gpu[50].tensor, cpu[50].tensor, f"{case} sum", fp=True)
     if case.startswith("string"):
         assert gpu[10].tensor.shape[1] == 12
+
+
+def test_aggregate_evaluate_over_a_group_index(device):
+    """aggregate.evaluate (aggregate.py:58-233) over rows numbered into groups by the caller:
+    an aggregate call, an expression over aggregates, and a row expression, against the
+    oracle's agg_evaluate on the same index."""
+    from tqp_amd.operators import aggregate
+    g = np.random.default_rng(91)
+    n, G = 200_000, 37
+    inv = g.integers(0, G, n)
+    inv[:G] = np.arange(G)                   # every group holds rows
+    v = g.uniform(900, 105000, n)
+    d = g.integers(0, 11, n) / 100.0
+    q = g.integers(1, 51, n).astype(np.float64)
+    tables = {20: ('float', v), 21: ('float', d), 22: ('float', q)}
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    ginv = torch.from_numpy(inv).to(device)
+    cinv = torch.from_numpy(inv)
+    for tree in ([['sum', [['v#20', '*', ['1.0', '-', 'd#21']]]], '#', '1'],
+                 [['0.2', '*', ['avg', ['q#22']]], '#', '2'],
+                 [['sum', ['v#20']], '/', ['count', ['1']]],
+                 [['max', ['q#22']]],
+                 ['v#20', '*', ['1.0', '-', 'd#21']]):
+        got = aggregate.evaluate('cuda', gpu, tree, n, G, ginv)
+        want = R.agg_evaluate(cpu, tree, n, G, cinv)
+        print("evaluate:", tree, flush=True)
+        _assert_col(got, want, f"evaluate {tree}", fp=True)
+
+
+def test_filter_evaluate_with_leaf_function(device):
+    """filter.evaluate with leaf_function (filter.py:27-35, the join condition path of
+    hashjoin.py:15-22): leaves come from the function, here columns gathered by a row index."""
+    from tqp_amd import kernels
+    from tqp_amd.operators import filter as F
+    g = np.random.default_rng(93)
+    n, m = 100_000, 60_000
+    a = g.integers(0, 100, n, dtype=np.int32)
+    b = g.uniform(0, 10, n)
+    idx = g.integers(0, n, m)
+    ga = torch.from_numpy(a).to(device)
+    gbv = torch.from_numpy(b).to(device)
+    gi = torch.from_numpy(idx).to(device)
+    gcols = {1: ga, 2: gbv}
+    ccols = {1: torch.from_numpy(a), 2: torch.from_numpy(b)}
+    tree = [['#1', '<', '40'], 'AND', ['#2', '>=', '2.5']]
+    got = F.evaluate('cuda', {}, tree, m, None,
+                     leaf_function=lambda cid: kernels.gather(gi, [gcols[cid]])[0])
+    want = R.filter_evaluate({}, tree, m, None,
+                             leaf_function=lambda cid: ccols[cid][torch.from_numpy(idx)])
+    _assert_col(got, want, "leaf mask")

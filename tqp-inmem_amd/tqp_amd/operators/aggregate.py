@@ -177,9 +177,49 @@ def agg_group(columns, key_ids, results, mask=None):
 
 
 def evaluate(device_name, tensors, tree, cnt_rows, cnt_groups, inverted_index):
-    """The reference's per-aggregate evaluation over a precomputed group index has no
-    counterpart here: all aggregates of an operator are reduced together by agg_group."""
-    raise NotImplementedError("use agg_group: aggregates are reduced in one fused launch")
+    """One result tree over rows already numbered into groups (aggregate.py:58-233): a row
+    expression gives one value per row; a tree with aggregate calls gives one value per group,
+    group g being the rows with inverted_index == g (0 <= g < cnt_groups, each present).
+
+    The aggregate calls of the tree reduce together in one group-by launch keyed by the group
+    index (a dense key range, so no hashing), and arithmetic over their results runs as one
+    expression launch over the groups.  agg_group is the operator's own path (it numbers the
+    groups itself, exactly)."""
+    if not gpu_index_ok(inverted_index, cnt_rows):
+        raise ValueError("inverted_index must hold one group id per row")
+    calls = E.find_aggregates(tree, [])
+    body = _strip_alias(_unwrap(tree))
+    if not calls:
+        b = E.ProgramBuilder()
+        v = E.compile_row_expr(b, body, tensors)
+        if v.kind == "lit":
+            return v.value
+        if v.kind != "reg":
+            raise NotImplementedError("string-valued row expression")
+        b.end_segment(v)
+        b.finish()
+        return kernels.eval_program(b, cnt_rows, v.dtype, inverted_index.device)
+    idx = inverted_index if inverted_index.dtype == torch.int64 else inverted_index.to(torch.int64)
+    kernels.note_bounds(idx, 0, max(int(cnt_groups) - 1, 0))
+    gb = kernels.GroupBy([idx], [(c[0], c[1]) for c in calls], tensors, n=cnt_rows)
+    key_out, agg_out = gb.run()
+    if key_out[0].shape[0] != cnt_groups:
+        raise ValueError(f"{key_out[0].shape[0]} of {cnt_groups} groups hold rows")
+    by_call = {id(c): t for c, t in zip(calls, agg_out)}
+    if id(body) in by_call:
+        return by_call[id(body)]
+    b = E.ProgramBuilder()
+    v = E.compile_row_expr(b, body, tensors, agg_ref=by_call)
+    if v.kind == "lit":
+        v = b.const(v.value, E.literal_dtype(v.value))
+    b.end_segment(v)
+    b.finish()
+    return kernels.eval_program(b, cnt_groups, v.dtype, agg_out[0].device)
+
+
+def gpu_index_ok(idx, n):
+    return torch.is_tensor(idx) and idx.is_cuda and idx.dim() == 1 and idx.shape[0] == n \
+        and not idx.dtype.is_floating_point
 
 
 _STATIC = {}

@@ -9,6 +9,7 @@ ride along on the mask Variable for the consumer's compaction.
 """
 import torch
 
+from .. import expr as E
 from .. import kernels
 from ..IO.vortex_pipeline import InMemoryPipeline
 from ..utility.logger import message_logger, perf_logger
@@ -16,11 +17,21 @@ from ..variable import Variable, VariableState
 
 
 def evaluate(device_name, tensors, tree, rows, subquery_result, leaf_function=None):
-    """Bool mask of `tree` over `tensors` (operators/filter.py:14-162), computed on the GPU."""
-    if leaf_function is not None:
-        raise NotImplementedError("leaf_function evaluation runs through the join's pair path")
-    mask, _ = kernels.filter_mask(tree, tensors, subquery_result, n=rows)
-    return mask
+    """Bool mask of `tree` over `tensors` (operators/filter.py:14-162), computed on the GPU.
+
+    With `leaf_function`, column leaves are `leaf_function(id)` instead of `tensors[id]` (the
+    join's condition evaluation over gathered pair columns, hashjoin.py:15-22); the tree then
+    runs as one expression-kernel launch over those columns."""
+    if leaf_function is None:
+        mask, _ = kernels.filter_mask(tree, tensors, subquery_result, n=rows)
+        return mask
+    b = E.ProgramBuilder()
+    E.compile_predicate(b, tree, tensors, subquery_result, leaf_function=leaf_function)
+    n = b.n if b.n is not None else rows
+    if n is None:
+        n = next(iter(tensors.values())).tensor.shape[0]
+    dev = b.col_tensors[0].device if b.col_tensors else next(iter(tensors.values())).tensor.device
+    return kernels.eval_program(b, n, torch.bool, dev)
 
 
 def tqp_filter(gpu_enable, tensor_group, args, subquery_result, cpu_mem_pool, gpu_mem_pool, name):
