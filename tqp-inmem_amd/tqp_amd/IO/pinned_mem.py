@@ -1,16 +1,24 @@
-"""Memory-pool objects the executor hands to every operator.
+"""Memory-pool objects the executor hands to every operator (IO/pinned_mem.py:10-136).
 
-The reference (IO/pinned_mem.py:10-136) carves operator outputs out of a 40 GB device slab with a
-Python first-fit over 40 MB blocks and copies every output into it.  Here operator outputs are
-allocated by torch's caching allocator (the HIP kernels write them in place, so the extra copy is
-gone) and carry `backing_mem_ptr=None`.  The pools keep the reference's accounting surface
-(`malloc_like`, `free`, `free_all`, utilisation) so executor code that passes them around and
-frees them per query runs unchanged; `MemoryError` on exhaustion is kept.
+`GPUMemory` is a real device slab: a `memory.DeviceArena` of `capacity_gb` with a first-fit
+sub-allocator (256-byte grain instead of the reference's 40 MB blocks).  `executor.execute`
+makes it the active arena for the query, so the kernels write operator outputs and workspaces
+into the slab directly; the reference's per-output copy into the pool
+(IO/vortex_pipeline.py:44-49) is gone.  `free_all` after each query returns everything, as the
+reference's main loop does (main.py:293-294); exhaustion raises MemoryError.
+
+`PinnedMemory` keeps the reference's accounting surface only: host tables are not staged
+through it on this path (the tables are resident in HBM before the query runs).
 """
 import torch
 
+from .. import memory
+
 
 class MemoryPool:
+    """Block accounting with the reference's surface (`malloc_like`, `malloc`, `free`,
+    `free_all`, `get_utilization`)."""
+
     def __init__(self, capacity_gb: float, block_size_mb: float, device='cpu'):
         self.pool_size_bytes = int(capacity_gb * 1000 ** 3)
         self.block_size_bytes = int(block_size_mb * 1000 ** 2)
@@ -59,8 +67,34 @@ class PinnedMemory(MemoryPool):
         super().__init__(capacity_gb, block_size_mb, device='cpu')
 
 
-class GPUMemory(MemoryPool):
+class GPUMemory:
+    """A device slab of `capacity_gb` (decimal GB) sub-allocated first-fit."""
     DEVICE = 'cuda:0'
 
-    def __init__(self, capacity_gb: float, block_size_mb: float):
-        super().__init__(capacity_gb, block_size_mb, device=self.DEVICE)
+    def __init__(self, capacity_gb: float, block_size_mb: float, device=None):
+        nbytes = int(capacity_gb * 1000 ** 3)
+        if nbytes < int(block_size_mb * 1000 ** 2):
+            raise ValueError("Pool size must be larger than the block size.")
+        self.arena = memory.DeviceArena(nbytes, device or self.DEVICE)
+        self.pool_size_bytes = self.arena.nbytes
+
+    def malloc_like(self, subject: torch.Tensor):
+        """(handle, tensor shaped like `subject`) in the slab."""
+        nbytes = subject.numel() * subject.element_size()
+        ptr = self.arena.alloc(nbytes)
+        t = self.arena.view(ptr, nbytes).view(subject.dtype).view(subject.shape)
+        return ptr, t
+
+    def malloc(self, num_elements, dtype):
+        nbytes = int(num_elements) * torch.empty((), dtype=dtype).element_size()
+        ptr = self.arena.alloc(nbytes)
+        return ptr, self.arena.view(ptr, nbytes).view(dtype)
+
+    def free(self, ptr: int):
+        self.arena.free(ptr)
+
+    def free_all(self):
+        self.arena.free_all()
+
+    def get_utilization(self):
+        return self.arena.used_bytes() / self.arena.nbytes

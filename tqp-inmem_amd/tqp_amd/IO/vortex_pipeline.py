@@ -9,8 +9,9 @@ output into the GPU memory pool.  Here:
 * with `fuse_masks=True` nothing is compacted: each masked column reaches the operator as its
   full tensor with `Variable.mask` set, and the operator's kernels skip unselected rows (the
   join and group-by kernels do this);
-* outputs stay in the tensors the kernels wrote (torch's caching allocator), so the pool copy is
-  gone and `memory_ptr` is None.
+* outputs stay in the tensors the kernels wrote (the query's device arena, a `GPUMemory` slab,
+  when there is one, else torch's caching allocator), so the pool copy is gone and
+  `memory_ptr` is None: the arena is freed as a whole after the query.
 
 Everything else follows the reference: mask columns are dropped unless `pass_mask`, all-empty
 inputs short-circuit to empty float32 tensors, a 0-d result is wrapped in a list.

@@ -10,6 +10,8 @@ import os
 
 import torch
 
+from . import memory
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtqhip.so")
 
@@ -334,8 +336,12 @@ def workspace(nbytes: int, device, role=None) -> torch.Tensor:
     """Raw device workspace.  With a role, the buffer is kept per (role, device, stream) and
     handed out again to the next call of that role on the same stream, which runs after the
     previous one's kernels in stream order; a holder must not keep using its buffer once
-    another call of its role has been issued."""
+    another call of its role has been issued.  While a device arena is active on the thread,
+    the workspace comes from it."""
     nbytes = max(int(nbytes), 256)
+    arena = memory.active()
+    if arena is not None:
+        return arena.workspace(nbytes, role)
     if role is None:
         return torch.empty(nbytes, dtype=torch.uint8, device=device)
     key = (role, str(device), torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))

@@ -24,6 +24,7 @@ import torch
 
 from . import _lib as L
 from . import expr as E
+from . import memory
 from .conversion import get_id, right_most
 from .operators.aggregate import _is_key, _strip_alias, _unwrap
 
@@ -182,8 +183,8 @@ def _key_bounds(t: torch.Tensor):
     d.nkeys = 1
     d.key_dtype[0] = L.dtype_code(t.dtype)
     d.key_ptr[0] = t.data_ptr()
-    out = torch.empty(3, dtype=torch.int64, device=t.device)
-    ws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", t.shape[0]), t.device)
+    out = memory.empty(3, dtype=torch.int64, device=t.device)
+    ws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", t.shape[0]), t.device, "gb_bounds")
     L.call("tq_groupby_bounds", ctypes.byref(d), None, t.shape[0], L.ptr(out), L.ptr(ws),
            L.stream_handle())
     lo, hi, _ = out.cpu().tolist()
@@ -296,19 +297,16 @@ class BatchedEngine:
             layout.append(lay)
         self.stage_loads = stage_loads
         self.lds_table_bytes = lds_table
-        offs, arena_bytes = [], 0
-        for lay in layout:
-            offs.append(arena_bytes)
-            arena_bytes += _table_bytes(lay["rng"], lay["ab"].nseg, T)[-1]
-        self.arena = torch.empty(max(arena_bytes, 256), dtype=torch.uint8, device=dev)
-        self.ngroups = torch.empty(max(len(layout), 1), dtype=torch.int64, device=dev)
-        self.flags = torch.zeros(max(len(layout), 1), dtype=torch.int32, device=dev)
-        base = self.arena.data_ptr()
+        # one table per pipeline (cleared by the launch's first phase)
+        self.tables = [memory.empty(max(_table_bytes(lay["rng"], lay["ab"].nseg, T)[-1], 256),
+                                    dtype=torch.uint8, device=dev) for lay in layout]
+        self.ngroups = memory.empty(max(len(layout), 1), dtype=torch.int64, device=dev)
+        self.flags = memory.zeros(max(len(layout), 1), dtype=torch.int32, device=dev)
         items = [[] for _ in range(PHASES)]
         arr = (BatchPipe * max(len(layout), 1))()
         ops = {'sum': L.TQ_AGG_SUM, 'count': L.TQ_AGG_COUNT, 'avg': L.TQ_AGG_AVG,
                'min': L.TQ_AGG_MIN, 'max': L.TQ_AGG_MAX}
-        for k, (lay, off) in enumerate(zip(layout, offs)):
+        for k, (lay, table) in enumerate(zip(layout, self.tables)):
             sp = self.specs[lay["i"]]
             d = arr[k]
             d.bprog, d.bcols = lay["bb"].prog, lay["bb"].cols
@@ -320,19 +318,19 @@ class BatchedEngine:
             d.lds_table = lay["lds"]
             d.key_dtype = L.dtype_code(lay["key_dtype"])
             cnt_off, acc_off, tc_off, _ = _table_bytes(lay["rng"], d.naggs, T)
-            p = base + off
+            p = table.data_ptr()
             d.dir, d.cnt, d.acc, d.tile_counts = p, p + cnt_off, p + acc_off, p + tc_off
             cap = lay["cap"]
             keys_out = []
             if sp.mode == 0:
-                key_out = torch.empty(cap, dtype=lay["key_dtype"], device=dev)
+                key_out = memory.empty(cap, dtype=lay["key_dtype"], device=dev)
                 d.key_out = key_out.data_ptr()
                 keys_out.append(key_out)
             else:
                 cols, mins, ranges, strides = lay["gkeys"]
                 d.nkeys = len(cols)
                 for j, c in enumerate(cols):
-                    t = torch.empty(cap, dtype=c.dtype, device=dev)
+                    t = memory.empty(cap, dtype=c.dtype, device=dev)
                     keys_out.append(t)
                     d.gkey_out[j] = t.data_ptr()
                     d.gkey_min[j], d.gkey_range[j], d.gkey_stride[j] = mins[j], ranges[j], strides[j]
@@ -349,7 +347,7 @@ class BatchedEngine:
                     odt = dt if dt.is_floating_point else torch.int64
                 else:
                     odt = dt
-                t = torch.empty(cap, dtype=odt, device=dev)
+                t = memory.empty(cap, dtype=odt, device=dev)
                 agg_outs.append(t)
                 d.agg_out[a] = t.data_ptr()
             d.ngroups = self.ngroups.data_ptr() + 8 * k
@@ -371,11 +369,11 @@ class BatchedEngine:
         self.phase_start = (ctypes.c_int64 * (PHASES + 1))(*starts)
         self.nitems = len(flat)
         it = torch.tensor(flat if flat else [(0, 0)], dtype=torch.int32).reshape(-1)
-        self.items = it.to(dev)
-        self.desc = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+        self.items = memory.to_device(it, dev)
+        self.desc = memory.to_device(torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8), dev)
         self._desc_host = arr
-        self.ws = torch.empty(max(L.size_call("tq_batch_ws_bytes", len(layout)), 256),
-                              dtype=torch.uint8, device=dev)
+        self.ws = memory.empty(max(L.size_call("tq_batch_ws_bytes", len(layout)), 256),
+                               dtype=torch.uint8, device=dev)
         self.npipes = len(layout)
 
     # ---------------------------------------------------------------- execution

@@ -12,7 +12,7 @@ import time
 
 import torch
 
-from . import groupjoin, kernels
+from . import groupjoin, kernels, memory
 from .operators.aggregate import result_ids, results_are_direct, tqp_hash_aggregate
 from .operators.filter import tqp_filter
 from .operators.hashjoin import join_vortex
@@ -250,7 +250,17 @@ def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem
     Aggregate runs together with it (one synchronisation instead of two); when the aggregate
     groups by the join key alone and `fuse` is set, the two run as one fused groupjoin whose
     pairs are never written (the join's output columns are computed if something reads them).
-    The results are the same."""
+    The results are the same.  With a `GPUMemory` as `gpu_mem_pool`, operator outputs and
+    workspaces are placed in its slab; otherwise in the thread's active device arena, if any,
+    else in torch's allocator."""
+    arena = getattr(gpu_mem_pool, 'arena', None)
+    with memory.use(arena if arena is not None else memory.active()):
+        return _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool,
+                        gpu_enable, op_times, sync, fuse)
+
+
+def _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool, gpu_enable,
+             op_times, sync, fuse):
     num_rows = None
     i = 0
     while i < len(plan):

@@ -22,7 +22,7 @@ import torch
 
 from . import _lib as L
 from . import expr as E
-from . import kernels
+from . import kernels, memory
 from .conversion import get_id
 from .operators.aggregate import _is_key, _parse_results, _strip_alias, _unwrap
 from .variable import Variable, VariableState, torch_to_type
@@ -136,7 +136,8 @@ _PRES = {}   # (device, stream) -> [byte tensor, last epoch]
 
 def _presence(nbytes, device):
     """The byte array of this stream (zeroed when allocated or grown, and when its 255 epochs
-    are used up) and the epoch for the next call."""
+    are used up) and the epoch for the next call.  It outlives queries and requests (its epochs
+    are what spare the clearing), so it comes from torch, never from a device arena."""
     key = (str(device), torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
     ent = _PRES.get(key)
     if ent is None or ent[0].numel() < nbytes:
@@ -192,6 +193,7 @@ class _JoinReplay:
         self.saved_group = saved_group
         self.saved_args = saved_args
         self.cols = None
+        self.arena = memory.active()      # the replay allocates where the query did
 
     def column(self, cid):
         if self.cols is None:
@@ -199,7 +201,8 @@ class _JoinReplay:
             active = kernels._deferred()
             kernels.set_deferred(None)     # the replay is an ordinary, settled join
             try:
-                self.run_join(tg, copy.deepcopy(self.saved_args))
+                with memory.use(self.arena):
+                    self.run_join(tg, copy.deepcopy(self.saved_args))
             finally:
                 kernels.set_deferred(active)
             self.cols = {k: tg[k].tensor for k in self.saved_args['materialized in']}
@@ -305,8 +308,8 @@ def run(tensor_group, ja, aa, p, run_join):
     out_dtypes = [_out_dtype(op, dt) for op, dt in zip(ops, arg_dtypes)]
     key_dtype = tensor_group[p.gkey].tensor.dtype
     cap = max(1, min(rng, np_))
-    key_full = torch.empty(cap, dtype=key_dtype, device=dev)
-    agg_full = [torch.empty(cap, dtype=dt, device=dev) for dt in out_dtypes]
+    key_full = memory.empty(cap, dtype=key_dtype, device=dev)
+    agg_full = [memory.empty(cap, dtype=dt, device=dev) for dt in out_dtypes]
     pres, epoch = _presence(pres_bytes, dev)
     ws = L.workspace(ws_bytes, dev, "groupjoin")
 
@@ -397,7 +400,7 @@ def _key_bounds(bk):
     d.nkeys = 1
     d.key_dtype[0] = L.dtype_code(bk.dtype)
     d.key_ptr[0] = bk.data_ptr()
-    out = torch.empty(3, dtype=torch.int64, device=bk.device)
+    out = memory.empty(3, dtype=torch.int64, device=bk.device)
     ws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", n), bk.device, "gb_bounds")
     L.call("tq_groupby_bounds", ctypes.byref(d), None, n, L.ptr(out), L.ptr(ws),
            L.stream_handle(), kernel={"k_gb_bounds": n * bk.element_size()})

@@ -13,6 +13,7 @@ import torch
 
 from . import _lib as L
 from . import expr as E
+from . import memory
 
 
 def _dev(t):
@@ -37,8 +38,8 @@ def filter_mask(tree, tensors, subquery_result=None, n=None):
     if rows is None:
         rows = next(iter(tensors.values())).tensor.shape[0]
     dev = b.col_tensors[0].device if b.col_tensors else next(iter(tensors.values())).tensor.device
-    mask = torch.empty(rows, dtype=torch.bool, device=dev)
-    counts = torch.empty(max(ntiles(rows), 1), dtype=torch.int32, device=dev)
+    mask = memory.empty(rows, dtype=torch.bool, device=dev)
+    counts = memory.empty(max(ntiles(rows), 1), dtype=torch.int32, device=dev)
     if not b.col_tensors:
         # no column referenced (e.g. only isnotnull): the kernel's column set is never read,
         # point it at the mask
@@ -53,7 +54,7 @@ def filter_mask(tree, tensors, subquery_result=None, n=None):
 
 def mask_tile_counts(mask: torch.Tensor):
     n = mask.shape[0]
-    counts = torch.empty(max(ntiles(n), 1), dtype=torch.int32, device=mask.device)
+    counts = memory.empty(max(ntiles(n), 1), dtype=torch.int32, device=mask.device)
     L.call("tq_mask_tile_counts", L.ptr(mask), n, L.ptr(counts), L.stream_handle())
     return counts
 
@@ -61,8 +62,8 @@ def mask_tile_counts(mask: torch.Tensor):
 def scan_tiles(counts: torch.Tensor, nt: int):
     """Exclusive scan of tile counts -> (offsets int64[nt], total int64[1]) on the device."""
     dev = counts.device
-    offs = torch.empty(max(nt, 1), dtype=torch.int64, device=dev)
-    total = torch.empty(1, dtype=torch.int64, device=dev)
+    offs = memory.empty(max(nt, 1), dtype=torch.int64, device=dev)
+    total = memory.empty(1, dtype=torch.int64, device=dev)
     ws = L.workspace(L.size_call("tq_scan_ws_bytes", nt), dev, "scan")
     L.call("tq_scan_tiles", L.ptr(counts), nt, L.ptr(offs), L.ptr(total), L.ptr(ws),
            L.stream_handle())
@@ -84,7 +85,7 @@ def compact(mask: torch.Tensor, columns, tile_counts=None):
         tile_counts = mask_tile_counts(mask)
     offs, total = scan_tiles(tile_counts, nt)
     m = int(total.item()) if n > 0 else 0
-    outs = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=c.device) for c in columns]
+    outs = [memory.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=c.device) for c in columns]
     for i in range(0, len(columns), L.TQ_MAX_COLS):
         src = [c.contiguous() for c in columns[i:i + L.TQ_MAX_COLS]]
         dst = outs[i:i + L.TQ_MAX_COLS]
@@ -101,7 +102,7 @@ def gather(idx: torch.Tensor, columns, null_fill=False):
     """`[c[idx] for c in columns]`; with null_fill, idx == -64 yields -64 (index_with_null)."""
     m = idx.shape[0]
     idx = idx.to(torch.int64).contiguous()
-    outs = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=c.device) for c in columns]
+    outs = [memory.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=c.device) for c in columns]
     if m == 0:
         return outs
     for i in range(0, len(columns), L.TQ_MAX_COLS):
@@ -117,7 +118,7 @@ def gather(idx: torch.Tensor, columns, null_fill=False):
 
 
 def eval_program(b: E.ProgramBuilder, n: int, out_dtype, device):
-    out = torch.empty(n, dtype=out_dtype, device=device)
+    out = memory.empty(n, dtype=out_dtype, device=device)
     if n > 0:
         if not b.col_tensors:
             b.column(out)
@@ -147,7 +148,7 @@ def sort_perm(keys, n: int, limit=None, device=None):
             pending = False
     m = n if limit is None or limit < 0 or limit > n else int(limit)
     dev = device if device is not None else keys[0][0].device
-    perm = torch.empty(m, dtype=torch.int64, device=dev)
+    perm = memory.empty(m, dtype=torch.int64, device=dev)
     if n == 0 or m == 0:
         return perm
     arr = (L.SortKey * max(len(keys), 1))()
@@ -302,9 +303,9 @@ class JoinIndex:
         if d is not None and d.join is None and not want_index and np_ > 0:
             # outputs for every probe row (a unique build gives a probe row one pair at most);
             # the pair count stays on the device until the deferral resolves
-            lout = [torch.empty((np_,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev)
+            lout = [memory.empty((np_,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev)
                     for c in lsrc]
-            rout = [torch.empty((np_,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev)
+            rout = [memory.empty((np_,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev)
                     for c in rsrc]
             tot, chk, applied = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int32()
             at = L.trace_len()
@@ -323,10 +324,10 @@ class JoinIndex:
                np_, L.ptr(pws), ctypes.byref(total), s, alg_bytes=count_bytes,
                kernel="k_join_count")
         m = int(total.value)
-        lout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in lsrc]
-        rout = [torch.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in rsrc]
-        bidx = torch.empty(m, dtype=torch.int64, device=dev) if want_index else None
-        pidx = torch.empty(m, dtype=torch.int64, device=dev) if want_index else None
+        lout = [memory.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in lsrc]
+        rout = [memory.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=dev) for c in rsrc]
+        bidx = memory.empty(m, dtype=torch.int64, device=dev) if want_index else None
+        pidx = memory.empty(m, dtype=torch.int64, device=dev) if want_index else None
         if m > 0:
             L.call("tq_join_write", ctypes.byref(self.state), L.ptr(pkey), L.dtype_code(pkey.dtype),
                    pm, np_, L.ptr(pws), len(lsrc), lsp, lba, L.ptr_array(lout), len(rsrc), rsp, rba,
@@ -488,8 +489,8 @@ class GroupBy:
         self.n = rows
         if not b.col_tensors:
             # no column read (e.g. count(*) without keys): the column set is never read
-            self._dummy = torch.zeros(max(rows, 1), dtype=torch.int8,
-                                      device=mask.device if mask is not None else 'cuda')
+            self._dummy = memory.zeros(max(rows, 1), dtype=torch.int8,
+                                       device=mask.device if mask is not None else 'cuda')
             b.col_tensors = [self._dummy]
             b.cols.ptr[0] = self._dummy.data_ptr()
             b.cols.width[0] = 1
@@ -575,7 +576,7 @@ class GroupBy:
             # every row is selected and the key ranges are already known: no bounds pass
             hb = [v for b in known for v in b] + [self.n]
         else:
-            bounds = torch.empty(2 * len(self.keys) + 1, dtype=torch.int64, device=dev)
+            bounds = memory.empty(2 * len(self.keys) + 1, dtype=torch.int64, device=dev)
             bws = L.workspace(L.size_call("tq_groupby_bounds_ws_bytes", self.n), dev, "gb_bounds")
             L.call("tq_groupby_bounds", ctypes.byref(d), mptr, self.n, L.ptr(bounds), L.ptr(bws), s,
                    kernel={"k_gb_bounds": self.n * (sum(k.element_size() for k in self.keys)
@@ -629,8 +630,8 @@ class GroupBy:
                     "k_gb_write_dense": groups * out_bytes}
         # outputs sized for the most groups there can be; the groups written are a prefix
         cap = max(1, min(nsel, self.slots))
-        key_full = [torch.empty(cap, dtype=k.dtype, device=dev) for k in self.keys]
-        agg_full = [torch.empty(cap, dtype=odt, device=dev) for odt in out_dtypes]
+        key_full = [memory.empty(cap, dtype=k.dtype, device=dev) for k in self.keys]
+        agg_full = [memory.empty(cap, dtype=odt, device=dev) for odt in out_dtypes]
         if pend:
             ngd, errd = ctypes.c_void_p(), ctypes.c_void_p()
             at = L.trace_len()
@@ -669,9 +670,9 @@ def dense_ranks(t: torch.Tensor, n: int):
     t = t[:n].contiguous()
     dev = t.device
     perm = sort_perm([(t, False)], n, device=dev)
-    rank = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
-    first = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
-    nr = torch.empty(1, dtype=torch.int64, device=dev)
+    rank = memory.empty(max(n, 1), dtype=torch.int32, device=dev)
+    first = memory.empty(max(n, 1), dtype=torch.int64, device=dev)
+    nr = memory.empty(1, dtype=torch.int64, device=dev)
     key = (L.SortKey * 1)()
     key[0].ptr = t.data_ptr()
     key[0].dtype = L.dtype_code(t.dtype)
@@ -688,7 +689,7 @@ def dense_ranks(t: torch.Tensor, n: int):
 
 def mark_rows(idx: torch.Tensor, n: int, device):
     """bool[n] with True at every row listed in idx."""
-    flags = torch.empty(n, dtype=torch.bool, device=device)
+    flags = memory.empty(n, dtype=torch.bool, device=device)
     if n > 0:
         L.call("tq_mark_rows", L.ptr(idx) if idx.numel() else None, idx.numel(), L.ptr(flags), n,
                L.stream_handle())
@@ -724,7 +725,7 @@ def compact_rows(mask: torch.Tensor):
     n = mask.shape[0]
     offs, total = scan_tiles(mask_tile_counts(mask), ntiles(n))
     m = int(total.item()) if n > 0 else 0
-    out = torch.empty(m, dtype=torch.int64, device=mask.device)
+    out = memory.empty(m, dtype=torch.int64, device=mask.device)
     if m > 0:
         src = (ctypes.c_void_p * 1)()
         src[0] = None

@@ -19,7 +19,7 @@ expression kernel.
 """
 import torch
 
-from .. import constants, kernels
+from .. import constants, kernels, memory
 from .. import expr as E
 from ..conversion import get_id, right_most
 from ..IO.vortex_pipeline import InMemoryPipeline
@@ -87,12 +87,12 @@ def _zero_row_aggregates(keys, calls, gb):
     out = []
     for c, dt in zip(calls, gb.arg_dtypes):
         if c[0] in ('sum', 'count'):
-            out.append(torch.zeros(1, dtype=torch.int64, device=dev))
+            out.append(memory.zeros(1, dtype=torch.int64, device=dev))
         elif c[0] == 'min':
             mx = torch.finfo(dt).max if dt.is_floating_point else torch.iinfo(dt).max
-            out.append(torch.full((1,), mx, dtype=dt, device=dev))
+            out.append(memory.full((1,), mx, dtype=dt, device=dev))
         else:
-            out.append(torch.zeros(1, dtype=dt, device=dev))
+            out.append(memory.zeros(1, dtype=dt, device=dev))
     return out
 
 

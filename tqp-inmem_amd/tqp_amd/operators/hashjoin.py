@@ -20,7 +20,7 @@ from typing import Literal
 
 import torch
 
-from .. import constants, kernels
+from .. import constants, kernels, memory
 from .. import expr as E
 from ..IO.vortex_pipeline import InMemoryPipeline
 from ..utility.logger import message_logger, perf_logger
@@ -142,22 +142,22 @@ def join_kernel_new(gpu_enable, tensors_left, tensors_right, left_key, right_key
         if type == 'right-semi':
             sel = matched if pmask is None else kernels.mask_and(pmask, matched)
             right_out_idx = kernels.compact_rows(sel)
-            left_out_idx = torch.empty(0, dtype=torch.int64, device=dev)
+            left_out_idx = memory.empty(0, dtype=torch.int64, device=dev)
         else:
             unmatched = kernels.mask_and_not(pmask, matched, nR, dev)
             rest = kernels.compact_rows(unmatched)
             if type == 'right-anti':
                 right_out_idx = rest
-                left_out_idx = torch.empty(0, dtype=torch.int64, device=dev)
+                left_out_idx = memory.empty(0, dtype=torch.int64, device=dev)
             else:  # right-outer: matched pairs, then unmatched probe rows with a NULL left side
-                right_out_idx = torch.cat([pidx, rest])
+                right_out_idx = memory.cat([pidx, rest], dev)
                 left_out_idx = bidx
                 nulls = rest.shape[0]
 
     null_left = type == 'right-outer' and left_out_idx.numel() > 0
     if null_left:
-        left_out_idx = torch.cat(
-            (left_out_idx, torch.full((nulls,), constants.null, dtype=torch.int64, device=dev)))
+        left_out_idx = memory.cat(
+            [left_out_idx, memory.full((nulls,), constants.null, dtype=torch.int64, device=dev)], dev)
 
     result = []
     lne = left_nonempty()

@@ -7,10 +7,14 @@ Requests are parameterised pipelines of three shapes:
 
 The base tables stay resident; a request only carries its constants.  The loop:
 
-* **Admission.** Admits queued requests in arrival order while the estimated intermediate memory
-  of everything in flight stays within the HBM budget and an execution slot is free.  The estimate
-  is `footprint_bytes`, an upper bound on masks, join outputs and group tables.  A request larger
-  than the whole budget is rejected with MemoryError.
+* **Admission.** Admits queued requests in arrival order while an execution slot is free and
+  the backend can reserve the request's memory.  On the GPU the reservation is a range of one
+  pre-sized device slab (`memory.DeviceArena`) of the request's `footprint`, an upper bound on
+  its masks, join outputs, group tables and workspaces; the request's operators allocate inside
+  it (MemoryError, reported for that request, if they would need more), and it is returned when
+  the request's results have been copied out.  Admission therefore counts the slab's real free
+  bytes: a request that does not fit waits for earlier ones to finish.  A request larger than
+  the whole slab is rejected with MemoryError.
 * **Continuous batching.** Every pass it moves all admitted shape-0 and shape-1 requests into one
   persistent batched launch (`batched.BatchedEngine`: group-by pipelines and join pipelines).  Requests that arrive while that launch runs are
   admitted as soon as slots and memory free up, into the next launch on another stream.  Batches
@@ -34,13 +38,14 @@ import numpy as np
 
 class Request:
     __slots__ = ("rid", "shape", "params", "arrival", "admitted", "done", "result", "error",
-                 "footprint")
+                 "footprint", "refused")
 
     def __init__(self, rid, shape, params, arrival):
         self.rid, self.shape, self.params, self.arrival = rid, shape, params, arrival
         self.admitted = self.done = None
         self.result = self.error = None
         self.footprint = 0
+        self.refused = 0          # admission passes that found no room for it
 
 
 def poisson_arrivals(rate, n, seed):
@@ -77,20 +82,19 @@ def validate(req):
 
 class Server:
     """The admission / continuous-batching loop.  `backend` provides:
-    footprint(req) -> bytes, batchable(req) -> bool, submit_batch(reqs) -> handle,
+    footprint(req) -> bytes, capacity() -> bytes, reserve(req) -> bool (False: no room now),
+    release(reqs), batchable(req) -> bool, submit_batch(reqs) -> handle,
     submit_single(req) -> handle, poll(handle) -> bool, collect(handle) -> {rid: result}."""
 
-    def __init__(self, backend, budget_bytes=200 * 10**9, max_inflight=4, max_batch=64):
+    def __init__(self, backend, max_inflight=4, max_batch=64):
         self.backend = backend
-        self.budget = budget_bytes
         self.max_inflight = max_inflight
         self.max_batch = max_batch
 
     def run(self, requests, clock=time.perf_counter):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue = collections.deque()
-        inflight = []                  # (handle, [requests], footprint)
-        used = 0
+        inflight = []                  # (handle, [requests])
         t0 = clock()
         finished = []
         while pending or queue or inflight:
@@ -100,8 +104,10 @@ class Server:
                 try:
                     validate(r)
                     r.footprint = self.backend.footprint(r)
-                    if r.footprint > self.budget:
-                        raise MemoryError(f"request {r.rid} needs {r.footprint} bytes, budget {self.budget}")
+                    cap = self.backend.capacity()
+                    if r.footprint > cap:
+                        raise MemoryError(f"request {r.rid} needs {r.footprint} bytes, "
+                                          f"device memory {cap}")
                     queue.append(r)
                 except (ValueError, MemoryError) as e:
                     r.error = e
@@ -109,9 +115,10 @@ class Server:
                     finished.append(r)
             # retire finished work
             still = []
-            for h, reqs, fp in inflight:
+            for h, reqs in inflight:
                 if self.backend.poll(h):
                     res = self.backend.collect(h)
+                    self.backend.release(reqs)
                     tdone = clock() - t0
                     for r in reqs:
                         r.done = tdone
@@ -120,27 +127,26 @@ class Server:
                         else:
                             r.result = res.get(r.rid)
                         finished.append(r)
-                    used -= fp
                 else:
-                    still.append((h, reqs, fp))
+                    still.append((h, reqs))
             inflight = still
             # admit: FIFO while memory and slots allow
             batch, singles = [], []
             while queue and len(inflight) + (1 if batch else 0) + len(singles) < self.max_inflight:
                 r = queue[0]
-                if used + r.footprint > self.budget:
+                if not self.backend.reserve(r):
+                    r.refused += 1
                     break
                 queue.popleft()
-                used += r.footprint
                 r.admitted = clock() - t0
                 if self.backend.batchable(r) and len(batch) < self.max_batch:
                     batch.append(r)
                 else:
                     singles.append(r)
             if batch:
-                inflight.append((self.backend.submit_batch(batch), batch, sum(r.footprint for r in batch)))
+                inflight.append((self.backend.submit_batch(batch), batch))
             for r in singles:
-                inflight.append((self.backend.submit_single(r), [r], r.footprint))
+                inflight.append((self.backend.submit_single(r), [r]))
             if not batch and not singles:
                 if pending and not queue and not inflight:
                     wait = pending[0].arrival - (clock() - t0)
@@ -160,12 +166,13 @@ def latency_stats(finished):
 
 # ==================================================================== GPU backend
 class GpuBackend:
-    """Executes requests on the device over resident tables (configs 0, 1 and 2)."""
+    """Executes requests on the device over resident tables (configs 0, 1 and 2), each in the
+    range of the device slab reserved for it at admission."""
 
-    def __init__(self, device, c2_scale=1.0):
+    def __init__(self, device, c2_scale=1.0, arena_bytes=32 * 2**30):
         import torch
 
-        from . import workloads
+        from . import memory, workloads
         self.torch = torch
         self.device = device
         t0 = workloads.config0_tables()
@@ -188,6 +195,10 @@ class GpuBackend:
         self._batch_streams = [torch.cuda.Stream(), torch.cuda.Stream()]
         self._next = 0
         self._bounds = {}
+        self.memory = memory
+        self.arena = memory.DeviceArena(arena_bytes, device)
+        self._held = {}            # rid -> the request's child arena
+        self.peak = {}             # shape -> most arena bytes a request of it used
 
     def plan(self, req):
         w = self.workloads
@@ -198,15 +209,38 @@ class GpuBackend:
         return w.config2_plan()
 
     def footprint(self, req):
-        """Upper bound on the request's intermediate device memory (bytes)."""
+        """Upper bound on the device memory one request allocates (bytes), on whichever path it
+        takes: every output and workspace of its operators, none freed before it ends.  The
+        per-operator path needs the most; measured peaks (arena high-water marks) were
+        1.30 MB for shape 0, 238 MB for shape 1 (42 MB in a batch), and 321 MB / 5.51 GB for
+        shape 2 over 3 M / 60 M fact rows."""
         if req.shape == 0:
-            n = 1_000_000
-            return n + 6 * 64 * 8 + 4 * n // 1024            # mask + group table + tile counts
+            n = self.base[0][1].tensor.shape[0]
+            return 2 * n + (1 << 16)                          # 2.07 MB at 1 M rows
         if req.shape == 1:
-            nb, np_ = 1_500_000, 6_000_000
-            return nb + np_ + np_ * 24 + nb * 16 + nb * 16   # masks, join output, index, groups
+            nb = self.base[1][101].tensor.shape[0]
+            np_ = self.base[1][201].tensor.shape[0]
+            return 40 * np_ + 24 * nb                         # 276 MB at 1.5 M x 6 M rows
         n = self.base[2][331].tensor.shape[0]
-        return 3 * n * 32 + 16 * 100_000                     # three join outputs + group table
+        na = self.base[2][301].tensor.shape[0]
+        nbb = self.base[2][311].tensor.shape[0]
+        return 96 * n + 64 * (na + nbb)                       # 6.85 GB at 60 M rows
+
+    def capacity(self):
+        return self.arena.nbytes
+
+    def reserve(self, req):
+        try:
+            self._held[req.rid] = self.arena.reserve(req.footprint)
+        except MemoryError:
+            return False
+        return True
+
+    def release(self, reqs):
+        for r in reqs:
+            a = self._held.pop(r.rid, None)
+            if a is not None:
+                a.release()
 
     def batchable(self, req):
         return req.shape in (0, 1)
@@ -222,17 +256,20 @@ class GpuBackend:
                 specs.append(batched.match_plan(self.plan(r), self.base[1]))
             else:
                 specs.append(batched.match_groupby_plan(self.plan(r), self.base[0]))
-        with torch.cuda.stream(stream):
+        held = self.memory.ArenaSet(self._held[r.rid] for r in reqs)
+        with torch.cuda.stream(stream), self.memory.use(held):
             eng = batched.BatchedEngine(specs, key_bounds=self._bounds)
             eng.run()
             ev = torch.cuda.Event()
             ev.record(stream)
-        return ("batch", eng, ev, reqs)
+        return ("batch", eng, ev, reqs, stream, held)
 
     def submit_single(self, req):
         torch = self.torch
+        memory = self.memory
         result = {}
         ev_box = {}
+        arena = self._held[req.rid]
 
         def work():
             from . import executor
@@ -240,10 +277,14 @@ class GpuBackend:
                 with torch.cuda.stream(self._single_stream):
                     tg = dict(self.base[req.shape])
                     try:
-                        executor.execute(copy.deepcopy(self.plan(req)), tg)
-                        result[req.rid] = _result_columns(req.shape, tg)
+                        with memory.use(arena):
+                            executor.execute(copy.deepcopy(self.plan(req)), tg)
+                            cols = _result_columns(req.shape, tg)
+                        # copied out of the reservation, which is returned once the copies ran
+                        result[req.rid] = {k: v.clone() for k, v in cols.items()}
                     except Exception as e:     # reported per request
                         result[req.rid] = e
+                    self.peak[req.shape] = max(self.peak.get(req.shape, 0), arena.peak)
                     ev = torch.cuda.Event()
                     ev.record(self._single_stream)
                     ev_box["ev"] = ev
@@ -259,16 +300,24 @@ class GpuBackend:
 
     def collect(self, h):
         if h[0] == "batch":
-            eng, reqs = h[1], h[3]
+            torch = self.torch
+            eng, reqs, stream, held = h[1], h[3], h[4], h[5]
             out, rerun = eng.results(sync=False)
             res = {}
-            for k, r in enumerate(reqs):
-                if k not in out:
-                    res[r.rid] = RuntimeError(f"request {r.rid} needs the per-operator path")
-                elif r.shape == 1:
-                    res[r.rid] = {301: out[k][301], 101: out[k][101]}
-                else:
-                    res[r.rid] = {c: out[k][c] for c in (6, 7, 22, 23, 24, 25, 26, 27, 28)}
+            # results are copied out of the reservations on the batch's stream, and the copies
+            # finish before the reservations are returned
+            with torch.cuda.stream(stream):
+                for k, r in enumerate(reqs):
+                    if k not in out:
+                        res[r.rid] = RuntimeError(f"request {r.rid} needs the per-operator path")
+                    elif r.shape == 1:
+                        res[r.rid] = {301: out[k][301].clone(), 101: out[k][101].clone()}
+                    else:
+                        res[r.rid] = {c: out[k][c].clone() for c in (6, 7, 22, 23, 24, 25, 26, 27, 28)}
+            stream.synchronize()
+            per = sum(a.peak for a in held.arenas) // max(len(held.arenas), 1)
+            for r in reqs:
+                self.peak[r.shape] = max(self.peak.get(r.shape, 0), per)
             return res
         return h[3]
 

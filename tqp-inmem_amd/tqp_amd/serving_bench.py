@@ -1,7 +1,7 @@
 """Open-loop sweep of the served form (config 4): Poisson arrivals at several rates over the
-resident config-0/1/2 tables, mix 40/40/20, 200 GB admission budget.  Reports sustained
-requests/s and input rows/s with p50/p99 latency per rate, and checks every completed result
-against the same request run alone through the per-operator path."""
+resident config-0/1/2 tables, mix 40/40/20, a 200 GB device arena for the requests' memory.
+Reports sustained requests/s and input rows/s with p50/p99 latency per rate, and checks every
+completed result against the same request run alone through the per-operator path."""
 import copy
 import time
 
@@ -34,14 +34,14 @@ def _same(a, b):
 
 
 def sweep(dev, rates=(50, 200, 500, 1000, 2000), n_requests=200, budget=200 * 10**9):
-    be = serving.GpuBackend(dev)
+    be = serving.GpuBackend(dev, arena_bytes=budget)
     out = {"resident_bytes": be.resident_bytes, "budget_bytes": budget, "mix": [0.4, 0.4, 0.2],
            "rates": []}
     # warm the kernels and the allocator once
-    serving.Server(be, budget_bytes=budget).run(serving.make_requests(20, 1000, seed=7))
+    serving.Server(be).run(serving.make_requests(20, 1000, seed=7))
     for rate in rates:
         reqs = serving.make_requests(n_requests, rate, seed=int(rate))
-        srv = serving.Server(be, budget_bytes=budget)
+        srv = serving.Server(be)
         t0 = time.perf_counter()
         done = srv.run(reqs)
         wall = time.perf_counter() - t0
