@@ -40,6 +40,9 @@ def parse():
                     help="synchronise after every operator, as the reference executor does")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once by a plain copy kernel (PMC traffic calibration)")
+    ap.add_argument("--only", type=int, choices=(0, 2, 3), default=None,
+                    help="run only the config-0, config-2 or config-3 (batched) section and print "
+                         "its JSON (for per-workload profiles)")
     return ap.parse_args()
 
 
@@ -158,13 +161,13 @@ CONFIG_WORKLOADS = {
 }
 
 
-def configs_section(dev, steps=20, warmup=5):
+def configs_section(dev, steps=20, warmup=5, cfgs=(0, 2)):
     """Configs 0 and 2 in the plain per-operator form: input rows/s, ms per pipeline and the
     dominant kernel's achieved HBM bandwidth (dispatch-stamped events, as for config 1)."""
     import torch
     from tqp_amd import _lib, executor, workloads
     out = {}
-    for cfg in (0, 2):
+    for cfg in cfgs:
         tables = getattr(workloads, f"config{cfg}_tables")()
         plan = getattr(workloads, f"config{cfg}_plan")()
         base = workloads.to_tensor_group(tables, dev)
@@ -261,6 +264,18 @@ def main():
     from tqp_amd import _lib, executor, workloads
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        x = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device=dev)
+        y = x.clone()
+        torch.cuda.synchronize()
+        del x, y
+    if args.only is not None:
+        if args.only == 3:
+            sec = batched_section(args.batch_pipes, dev)
+        else:
+            sec = configs_section(dev, steps=args.steps, warmup=args.warmup, cfgs=(args.only,))
+        print(json.dumps(sec if args.only != 3 else {"config3": sec}), flush=True)
+        return
     tables = workloads.config1_tables()
     plan = workloads.config1_plan()
     base = workloads.to_tensor_group(tables, dev)
@@ -278,11 +293,6 @@ def main():
         executor.execute(copies.pop() if copies else copy.deepcopy(plan), tg, sync=op_sync)
         return tg
 
-    if args.calibrate_copy:
-        x = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device=dev)
-        y = x.clone()
-        torch.cuda.synchronize()
-        del x, y
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

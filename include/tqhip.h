@@ -161,6 +161,13 @@ int tq_timing_read(tq_ktime_t* out, int64_t cap, int64_t* n, int64_t* dropped);
 int tq_filter_mask(const tq_program_t* prog /* host */, const tq_colset_t* cols /* host */,
                    int64_t n, uint8_t* mask_out, uint32_t* tile_counts, void* stream);
 
+/* 1 when segment 0 of a predicate program is a conjunction of column-versus-constant
+ * comparisons (at most 8, over integer, f32 or f64 columns, isnotnull counting as true), the
+ * form tq_filter_mask runs without its interpreter and tq_groupjoin_run can fuse; else 0.
+ * Host-side only (launches nothing).  A query of this library, no reference counterpart. */
+int tq_filter_is_conjunction(const tq_program_t* prog /* host */,
+                             const tq_colset_t* cols /* host */);
+
 /* Evaluate segment 0 of a program per row into a column of out_bytes-wide values: the low
  * bytes of the result register (bool/int8: 1, int16: 2, int32/f32: 4, int64/f64: 8).
  * Replaces the elementwise ATen chains of operators/aggregate.py:197-233 when they combine
@@ -527,6 +534,16 @@ typedef struct {
   int32_t out_key_dtype;     /* TQ_BOOL..TQ_I64                                              */
   void* out_agg[TQ_GB_MAX_AGGS];
   int64_t out_cap;           /* rows of the outputs: min(hi - lo + 1, np) always suffices    */
+  /* Filters fused into the call (NULL: none): a build row is selected when bmask selects it
+   * and bpred holds for it, a probe row likewise with pmask and ppred.  Each predicate is a
+   * program of one segment over columns of nb (build) or np (probe) rows that
+   * tq_filter_is_conjunction accepts; the call fails with TQ_EUNSUPPORTED otherwise.  They
+   * replace the tqp_filter calls (operators/filter.py:164-205) whose masks the join would
+   * read, so the masks are never written. */
+  const tq_program_t* bpred;  /* host */
+  const tq_colset_t* bpred_cols;
+  const tq_program_t* ppred;  /* host */
+  const tq_colset_t* ppred_cols;
 } tq_groupjoin_desc_t;
 
 size_t tq_groupjoin_pres_bytes(int64_t key_range);

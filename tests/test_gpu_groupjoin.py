@@ -335,3 +335,43 @@ def test_epochs_wrap_around(device):
         if i % 27 == 0 or i >= 250:
             _assert_col(tg[101].tensor, wants[seg][0], f"call {i} keys")
             _assert_col(tg[301].tensor, wants[seg][1], f"call {i} revenue", fp=True)
+
+
+def test_filters_fused_into_the_groupjoin(device, monkeypatch):
+    """Both filters are conjunctions of comparisons: the groupjoin evaluates them in its own
+    kernels and their masks are never computed; read afterwards, a mask is the oracle's."""
+    from tqp_amd import workloads
+    from tqp_amd.operators.filter import LazyMask
+    tables = workloads.config1_tables(n_build=200_000, n_probe=800_000)
+    plan = workloads.config1_plan()
+    seen = _Seen(monkeypatch)
+    gpu, n = _run(tables, plan, device)
+    assert seen.seen == ["fused"], seen.seen
+    for mid in (10001, 10002):
+        assert isinstance(gpu[mid], LazyMask) and not gpu[mid].materialized, mid
+    cpu, _ = _oracle(tables, plan)
+    _assert_col(gpu[101].tensor, cpu[101].tensor, "top keys (fused filters)")
+    _assert_col(gpu[301].tensor, cpu[301].tensor, "top revenue (fused filters)", fp=True)
+    masks = R.to_cols(tables)
+    R.execute(copy.deepcopy(plan[:2]), masks)
+    for mid in (10001, 10002):
+        _assert_col(gpu[mid].tensor, masks[mid].tensor, f"mask {mid} read later")
+        assert gpu[mid].tile_counts is not None
+
+
+def test_filter_that_is_not_a_conjunction_is_computed(device, monkeypatch):
+    """An OR on the probe side cannot be fused: its mask is computed and the groupjoin reads it;
+    the build side's conjunction is still fused."""
+    from tqp_amd import workloads
+    from tqp_amd.operators.filter import LazyMask
+    tables = workloads.config1_tables(n_build=200_000, n_probe=800_000)
+    plan = workloads.config1_plan()
+    plan[1][1]['filter tree'] = [['ship_day#204', '>', '1900'], 'OR', ['ship_day#204', '<', '300']]
+    seen = _Seen(monkeypatch)
+    gpu, n = _run(tables, plan, device)
+    assert seen.seen == ["fused"], seen.seen
+    assert isinstance(gpu[10001], LazyMask) and not gpu[10001].materialized
+    assert gpu[10002].materialized if isinstance(gpu[10002], LazyMask) else True
+    cpu, _ = _oracle(tables, plan)
+    _assert_col(gpu[101].tensor, cpu[101].tensor, "top keys (OR filter)")
+    _assert_col(gpu[301].tensor, cpu[301].tensor, "top revenue (OR filter)", fp=True)

@@ -6,6 +6,7 @@
 // the tile with wave ballots + mbcnt and places the tile at its exclusive-scan offset, so the
 // output keeps input order exactly as `tensor[mask]` does.
 #include "tq_common.h"
+#include "tq_conj.h"
 #include "tq_expr.h"
 
 namespace tq {
@@ -73,42 +74,9 @@ __global__ __launch_bounds__(kBlock) void k_filter(tq_program_t P, tq_colset_t C
 }
 
 // ------------------------------------------------------------------ comparison conjunctions
-// Most predicates are conjunctions of column-versus-constant comparisons (isnotnull compiles to
-// a constant true).  tq_filter_mask recognises such a program by following its registers and
-// runs k_filter_conj instead of the interpreter: per load slot, a thread reads its 16 rows of a
-// 4096-row tile (four runs of four consecutive rows, all loads in flight together) and tests them against
-// every term on that slot; the mask bytes of a run go out as one 32-bit store.
-constexpr int kMaxTerms = 8;
-
-struct Conj {
-  int32_t nterm;
-  int32_t nslot;
-  int32_t slot_dtype[TQ_MAX_LOADS];
-  const void* slot_col[TQ_MAX_LOADS];
-  int32_t term_slot[kMaxTerms];
-  int32_t term_op[kMaxTerms];
-  int64_t term_imm[kMaxTerms];
-};
-
-template <typename T>
-__device__ __forceinline__ uint32_t cmp_bits(int op, const T (&x)[kItems], T c) {
-  uint32_t b = 0;
-  switch (op) {
-#define TQ_CMP(CASE, EXPR)                                                  \
-  case CASE:                                                                \
-    _Pragma("unroll") for (int j = 0; j < kItems; ++j) b |= (uint32_t)(EXPR) << j; \
-    break;
-    TQ_CMP(0, x[j] < c)
-    TQ_CMP(1, x[j] <= c)
-    TQ_CMP(2, x[j] > c)
-    TQ_CMP(3, x[j] >= c)
-    TQ_CMP(4, x[j] == c)
-    default: TQ_CMP(5, x[j] != c)
-#undef TQ_CMP
-  }
-  return b;
-}
-
+// Most predicates are conjunctions of column-versus-constant comparisons (tq_conj.h):
+// tq_filter_mask runs them through k_filter_conj instead of the interpreter, and writes
+// a run's four mask bytes as one 32-bit store.
 __global__ __launch_bounds__(kBlock) void k_filter_conj(Conj C, int64_t n,
                                                         uint8_t* __restrict__ mask,
                                                         uint32_t* __restrict__ tile_counts) {
@@ -116,31 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_conj(Conj C, int64_t n,
   const int64_t tile = blockIdx.x;
   const int t = threadIdx.x;
   uint32_t acc = load_sel16(nullptr, tile * kTileRows, n);
-  for (int s = 0; s < C.nslot; ++s) {
-    uint64_t v[kItems];
-    load_raw16(C.slot_col[s], C.slot_dtype[s], tile * kTileRows, n, v);
-    for (int k = 0; k < C.nterm; ++k) {
-      if (C.term_slot[k] != s) continue;
-      const int op = C.term_op[k];
-      const int64_t imm = C.term_imm[k];
-      if (op < 6) {
-        int64_t x[kItems];
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) x[j] = (int64_t)v[j];
-        acc &= cmp_bits<int64_t>(op, x, imm);
-      } else if (op < 12) {
-        double x[kItems];
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) x[j] = __longlong_as_double((long long)v[j]);
-        acc &= cmp_bits<double>(op - 6, x, __longlong_as_double(imm));
-      } else {
-        float x[kItems];
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) x[j] = __uint_as_float((uint32_t)v[j]);
-        acc &= cmp_bits<float>(op - 12, x, __uint_as_float((uint32_t)imm));
-      }
-    }
-  }
+  acc = conj_sel16(C, tile * kTileRows, n, acc);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int64_t r = tile * kTileRows + (int64_t)q * (4 * kBlock) + 4 * t;
@@ -165,93 +109,6 @@ __global__ __launch_bounds__(kBlock) void k_filter_conj(Conj C, int64_t n,
     }
   }
 }
-
-// Recognise a conjunction of column-versus-constant comparisons in segment 0 of a program.
-// Compare opcodes map to 0..5 (lt, le, gt, ge, eq, ne) plus 6 for f64 and 12 for f32.
-static int cmp_code(int op) {
-  if (op >= TQ_OP_LT_I64 && op <= TQ_OP_NE_I64) return op - TQ_OP_LT_I64;
-  if (op >= TQ_OP_LT_F64 && op <= TQ_OP_NE_F64) return 6 + op - TQ_OP_LT_F64;
-  if (op >= TQ_OP_LT_F32 && op <= TQ_OP_NE_F32) return 12 + op - TQ_OP_LT_F32;
-  return -1;
-}
-
-static bool match_conj(const tq_program_t& P, const tq_colset_t& cols, Conj* C) {
-  enum { UNK, CONSTV, LOADV, TERMS };
-  struct RS { int kind; int64_t imm; int slot; uint32_t terms; };
-  RS R[TQ_NREG];
-  for (int i = 0; i < TQ_NREG; ++i) R[i] = RS{UNK, 0, 0, 0};
-  Conj all{};
-  for (int pc = 0; pc < P.seg_end[0]; ++pc) {
-    const tq_instr_t& I = P.code[pc];
-    const int d = I.dst & (TQ_NREG - 1);
-    const RS a = R[I.a & (TQ_NREG - 1)], b = R[I.b & (TQ_NREG - 1)];
-    const int cc = cmp_code(I.op);
-    if (I.op == TQ_OP_NOP) continue;
-    if (I.op == TQ_OP_CONST) {
-      R[d] = RS{CONSTV, I.imm, 0, 0};
-    } else if (I.op == TQ_OP_LOAD) {
-      if (I.a >= P.nload) return false;
-      R[d] = RS{LOADV, 0, I.a, 0};
-    } else if (cc >= 0) {
-      int code = cc, slot;
-      int64_t imm;
-      if (a.kind == LOADV && b.kind == CONSTV) {
-        slot = a.slot;
-        imm = b.imm;
-      } else if (a.kind == CONSTV && b.kind == LOADV) {
-        static const int flip[6] = {2, 3, 0, 1, 4, 5};  // c < x  <=>  x > c, ...
-        code = (cc / 6) * 6 + flip[cc % 6];
-        slot = b.slot;
-        imm = a.imm;
-      } else {
-        return false;
-      }
-      const int dt = P.load_dtype[slot];
-      const bool ok = cc < 6 ? (dt <= TQ_I64) : (cc < 12 ? dt == TQ_F64 : dt == TQ_F32);
-      if (!ok || all.nterm == kMaxTerms || cols.width[P.load_col[slot]] != 1) return false;
-      all.term_slot[all.nterm] = slot;
-      all.term_op[all.nterm] = code;
-      all.term_imm[all.nterm] = imm;
-      R[d] = RS{TERMS, 0, 0, 1u << all.nterm};
-      ++all.nterm;
-    } else if (I.op == TQ_OP_AND) {
-      // a constant operand must be true (nonzero); the result holds both operands' terms
-      if ((a.kind != TERMS && !(a.kind == CONSTV && a.imm != 0)) ||
-          (b.kind != TERMS && !(b.kind == CONSTV && b.imm != 0)))
-        return false;
-      const uint32_t tm = (a.kind == TERMS ? a.terms : 0u) | (b.kind == TERMS ? b.terms : 0u);
-      R[d] = tm ? RS{TERMS, 0, 0, tm} : RS{CONSTV, 1, 0, 0};
-    } else {
-      return false;
-    }
-  }
-  const RS res = R[P.seg_result[0] & (TQ_NREG - 1)];
-  if (!(res.kind == TERMS || (res.kind == CONSTV && res.imm != 0))) return false;
-  // keep the terms the result is the conjunction of
-  *C = Conj{};
-  for (int k = 0; k < all.nterm; ++k) {
-    if (res.kind != TERMS || !((res.terms >> k) & 1u)) continue;
-    C->term_slot[C->nterm] = all.term_slot[k];
-    C->term_op[C->nterm] = all.term_op[k];
-    C->term_imm[C->nterm] = all.term_imm[k];
-    ++C->nterm;
-  }
-  // load only the slots some kept term reads, renumbered densely
-  int remap[TQ_MAX_LOADS];
-  for (int sl = 0; sl < TQ_MAX_LOADS; ++sl) remap[sl] = -1;
-  for (int k = 0; k < C->nterm; ++k) {
-    const int sl = C->term_slot[k];
-    if (remap[sl] < 0) {
-      remap[sl] = C->nslot;
-      C->slot_dtype[C->nslot] = P.load_dtype[sl];
-      C->slot_col[C->nslot] = cols.ptr[P.load_col[sl]];
-      ++C->nslot;
-    }
-    C->term_slot[k] = remap[sl];
-  }
-  return true;
-}
-
 
 __global__ __launch_bounds__(kBlock) void k_eval(tq_program_t P, tq_colset_t C, int64_t n,
                                                  void* __restrict__ out, int out_bytes) {
@@ -546,6 +403,14 @@ extern "C" int tq_filter_mask(const tq_program_t* prog, const tq_colset_t* cols,
                   (hipStream_t)stream, C, n, mask_out, tile_counts);
   return launch("k_filter", k_filter, (unsigned)tiles, kBlock, stage_bytes(*prog),
                 (hipStream_t)stream, *prog, *cols, n, mask_out, tile_counts);
+}
+
+extern "C" int tq_filter_is_conjunction(const tq_program_t* prog, const tq_colset_t* cols) {
+  Conj C;
+  if (!prog || !cols || prog->nseg < 1 || prog->n_instr > TQ_MAX_INSTR ||
+      prog->seg_end[0] > prog->n_instr || prog->nload > TQ_MAX_LOADS)
+    return 0;
+  return match_conj(*prog, *cols, &C) ? 1 : 0;
 }
 
 extern "C" int tq_eval_column(const tq_program_t* prog, const tq_colset_t* cols, int64_t n,

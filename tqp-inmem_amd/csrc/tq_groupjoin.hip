@@ -31,6 +31,7 @@
 
 #include "tq_agg.h"
 #include "tq_common.h"
+#include "tq_conj.h"
 #include "tq_expr.h"
 
 namespace tq {
@@ -73,9 +74,10 @@ struct GjOut {
 
 // ------------------------------------------------------------------ 1. build
 // Workgroup 0 also zeroes the reduce's look-back words (flags[0..nflags)) and the words the
-// host reads back (the later kernels of the call only add to or OR into them).
+// host reads back (the later kernels of the call only add to or OR into them).  A fused filter
+// (BC.nslot > 0) selects rows in the tile's own loads, and its mask is never written.
 __global__ __launch_bounds__(kBlock) void k_gj_build(const void* __restrict__ bkeys, int dt,
-                                                     const uint8_t* __restrict__ bmask,
+                                                     const uint8_t* __restrict__ bmask, Conj BC,
                                                      int64_t nb, int64_t lo, uint64_t range,
                                                      uint8_t* __restrict__ pres, uint32_t epoch,
                                                      uint32_t* __restrict__ tile_counts,
@@ -91,7 +93,8 @@ __global__ __launch_bounds__(kBlock) void k_gj_build(const void* __restrict__ bk
   if (tile0 >= nb) return;  // nb == 0: the launch only clears the words
   uint64_t raw[kItems];
   load_raw16(bkeys, dt, tile0, nb, raw);
-  const uint32_t sel = load_sel16(bmask, tile0, nb);
+  uint32_t sel = load_sel16(bmask, tile0, nb);
+  if (BC.nslot > 0) sel = conj_sel16(BC, tile0, nb, sel);
   bool oob = false;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
@@ -158,7 +161,7 @@ __device__ __forceinline__ void gather_loads(const tq_program_t& P, const tq_col
 // values}; a chunk's records are written in slice order by consecutive threads.
 __global__ __launch_bounds__(kBlock) void k_gj_probe(
     GjSpec G, tq_program_t P, tq_colset_t C, const void* __restrict__ pkeys, int pdt,
-    const uint8_t* __restrict__ pmask, int64_t np, int64_t lo, uint64_t range,
+    const uint8_t* __restrict__ pmask, Conj PC, int64_t np, int64_t lo, uint64_t range,
     const uint8_t* __restrict__ pres, uint32_t epoch, int sb, int S, int64_t W,
     uint32_t* __restrict__ loff, uint64_t* __restrict__ rec, int32_t stage_words) {
   extern __shared__ uint64_t smem[];
@@ -179,6 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_probe(
     const int64_t tile0 = base + (int64_t)q * kTileRows;
     load_raw16(pkeys, pdt, tile0, np, raw[q]);  // rows past np read row 0 (unselected)
     sel[q] = load_sel16(pmask, tile0, np);
+    if (PC.nslot > 0) sel[q] = conj_sel16(PC, tile0, np, sel[q]);  // a fused filter
   }
   uint32_t hit[kGjTiles];
 #pragma unroll
@@ -683,6 +687,17 @@ static int gj_launch_lds(const char* name, void (*kernel)(P...), unsigned grid, 
   return launch(name, kernel, grid, kBlock, lds, s, std::forward<A>(a)...);
 }
 
+// A fused filter's conjunction (none: nslot = 0).
+static int gj_pred(const tq_program_t* P, const tq_colset_t* C, const char* side, Conj* out) {
+  *out = Conj{};
+  if (!P) return TQ_OK;
+  if (!C || P->nseg < 1 || P->n_instr > TQ_MAX_INSTR || P->seg_end[0] > P->n_instr ||
+      P->nload > TQ_MAX_LOADS || !match_conj(*P, *C, out))
+    return set_error(TQ_EUNSUPPORTED,
+                     "groupjoin: the %s predicate is not a conjunction of comparisons", side);
+  return TQ_OK;
+}
+
 static int64_t gj_spin_ticks() {
   // a look-back wait gives up after 100 ms of wall-clock time
   static int64_t ticks = 0;
@@ -728,6 +743,9 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
   TQ_TRY(gj_check_dtype(d->bkey_dtype, "the build key"));
   TQ_TRY(gj_check_dtype(d->pkey_dtype, "the probe key"));
   TQ_TRY(gj_check_dtype(d->out_key_dtype, "the group key output"));
+  Conj BC{}, PC{};
+  TQ_TRY(gj_pred(d->bpred, d->bpred_cols, "build", &BC));
+  TQ_TRY(gj_pred(d->ppred, d->ppred_cols, "probe", &PC));
   if (d->key_hi < d->key_lo) return set_error(TQ_EINVAL, "groupjoin: key bounds hi < lo");
   const uint64_t urange = (uint64_t)d->key_hi - (uint64_t)d->key_lo + 1;
   if (urange == 0 || urange > ((uint64_t)1 << 40))
@@ -768,13 +786,13 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
   *words = wd;
   const unsigned bgrid = (unsigned)(d->nb > 0 ? p.nbt : 1);
   TQ_TRY(launch("k_gj_build", k_gj_build, bgrid, kBlock, 0, s, d->bkey, (int)d->bkey_dtype,
-                d->bmask, d->nb, d->key_lo, (uint64_t)range, (uint8_t*)pres, epoch,
+                d->bmask, BC, d->nb, d->key_lo, (uint64_t)range, (uint8_t*)pres, epoch,
                 btile, flags, p.S, wd));
   if (d->np > 0) {
     const size_t lds = (size_t)p.stage_words * 8 + (size_t)((p.S + 1) & ~1) * 4 +
                        (size_t)kGjChunk * 4;
     TQ_TRY(gj_launch_lds("k_gj_probe", k_gj_probe, (unsigned)p.W, lds, s, G, *prog, *cols,
-                         d->pkey, (int)d->pkey_dtype, d->pmask, d->np, d->key_lo,
+                         d->pkey, (int)d->pkey_dtype, d->pmask, PC, d->np, d->key_lo,
                          (uint64_t)range, (const uint8_t*)pres, epoch, p.sb, p.S,
                          p.W, loff, rec, p.stage_words));
   } else {

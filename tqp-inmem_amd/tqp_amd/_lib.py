@@ -126,7 +126,9 @@ class GroupJoinDesc(ctypes.Structure):
                 ("agg_op", ctypes.c_int32 * TQ_MAX_SEGS),
                 ("agg_dtype", ctypes.c_int32 * TQ_MAX_SEGS),
                 ("out_key", ctypes.c_void_p), ("out_key_dtype", ctypes.c_int32),
-                ("out_agg", ctypes.c_void_p * TQ_MAX_SEGS), ("out_cap", ctypes.c_int64)]
+                ("out_agg", ctypes.c_void_p * TQ_MAX_SEGS), ("out_cap", ctypes.c_int64),
+                ("bpred", ctypes.c_void_p), ("bpred_cols", ctypes.c_void_p),
+                ("ppred", ctypes.c_void_p), ("ppred_cols", ctypes.c_void_p)]
 
 
 GJ_GROUPS, GJ_PAIRS, GJ_NSEL, GJ_NSET, GJ_ERR = range(5)
@@ -144,6 +146,7 @@ _SIGNATURES = {
     "tq_timing_count": ([], I64),
     "tq_timing_read": ([P, I64, P, P], I32),
     "tq_filter_mask": ([P, P, I64, P, P, P], I32),
+    "tq_filter_is_conjunction": ([P, P], I32),
     "tq_mask_tile_counts": ([P, I64, P, P], I32),
     "tq_scan_ws_bytes": ([I64], SZ),
     "tq_scan_tiles": ([P, I64, P, P, P, P], I32),

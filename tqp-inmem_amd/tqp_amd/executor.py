@@ -278,9 +278,11 @@ def _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool, gp
         i += 1
         t0 = time.perf_counter()
         if name == 'Filter':
+            # unsynchronised: the mask is computed when first read, or never when a fused
+            # consumer evaluates the predicate in its own kernels
             num_rows = tqp_filter(gpu_enable, tensor_group, args, subquery_result,
                                   cpu_mem_pool=cpu_mem_pool, gpu_mem_pool=gpu_mem_pool,
-                                  name=f"filter {u}")
+                                  name=f"filter {u}", lazy=not sync)
         elif name == 'Aggregate':
             num_rows = tqp_hash_aggregate(gpu_enable, tensor_group, args,
                                           cpu_mem_pool=cpu_mem_pool, gpu_mem_pool=gpu_mem_pool,

@@ -15,7 +15,6 @@ repeated selected build key, a build key outside the column's recorded bounds, a
 pairs or a look-back that gave up voids the fused result: the executor then runs the two
 operators as usual from the state saved before the join.
 """
-import copy
 import ctypes
 
 import torch
@@ -25,6 +24,7 @@ from . import expr as E
 from . import kernels, memory
 from .conversion import get_id
 from .operators.aggregate import _is_key, _parse_results, _strip_alias, _unwrap
+from .operators.filter import LazyMask
 from .variable import Variable, VariableState, torch_to_type
 
 OPS = kernels.GroupBy.OPS
@@ -102,16 +102,18 @@ def plan(tensor_group, ja, aa):
         return None
     if pmask_id is not None and pmask_id not in tensor_group:
         return None
-    for mid in (bmask_id, pmask_id):
-        if mid is not None:
-            m = tensor_group[mid].tensor
-            if m.dtype != torch.bool or m.dim() != 1:
+    for mid, n in ((bmask_id, bk.shape[0]), (pmask_id, pk.shape[0])):
+        if mid is None:
+            continue
+        mv = tensor_group[mid]
+        if isinstance(mv, LazyMask) and not mv.materialized:
+            if mv.rows != n:          # (a mask not yet computed is a 1-D bool of `rows` rows)
                 return None
-    if bmask_id is not None and tensor_group[bmask_id].tensor.shape[0] != bk.reshape(-1).shape[0]:
-        return None
-    if pmask_id is not None and tensor_group[pmask_id].tensor.shape[0] != pk.reshape(-1).shape[0]:
-        return None
-    np_ = pk.reshape(-1).shape[0]
+            continue
+        m = mv.tensor
+        if m.dtype != torch.bool or m.dim() != 1 or m.shape[0] != n:
+            return None
+    np_ = pk.shape[0]
     for i in arg_ids:
         t = tensor_group[i].tensor
         if t.shape[0] != np_:
@@ -183,6 +185,31 @@ class GroupJoinDeferred(kernels.Deferred):
         return m, G
 
 
+def _flat(t):
+    """A key column as a contiguous 1-D tensor (a view when it already is one)."""
+    if t.dim() != 1:
+        t = t.reshape(-1)
+    return t if t.is_contiguous() else t.contiguous()
+
+
+def _clone(v):
+    """A shallow copy of a Variable (of any subclass): a new object with the same fields."""
+    c = object.__new__(type(v))
+    c.__dict__.update(v.__dict__)
+    return c
+
+
+def _copy_args(x):
+    """A copy of plan arguments (nested dicts, lists and tuples of immutable leaves)."""
+    if isinstance(x, dict):
+        return {k: _copy_args(v) for k, v in x.items()}
+    if isinstance(x, list):
+        return [_copy_args(v) for v in x]
+    if isinstance(x, tuple):
+        return tuple(_copy_args(v) for v in x)
+    return x
+
+
 # ---------------------------------------------------------------- lazily materialised join outputs
 class _JoinReplay:
     """Runs the join itself (operators/hashjoin.py:179-248) on the state saved before the fused
@@ -197,12 +224,12 @@ class _JoinReplay:
 
     def column(self, cid):
         if self.cols is None:
-            tg = {k: copy.copy(v) for k, v in self.saved_group.items()}
+            tg = {k: _clone(v) for k, v in self.saved_group.items()}
             active = kernels._deferred()
             kernels.set_deferred(None)     # the replay is an ordinary, settled join
             try:
                 with memory.use(self.arena):
-                    self.run_join(tg, copy.deepcopy(self.saved_args))
+                    self.run_join(tg, _copy_args(self.saved_args))
             finally:
                 kernels.set_deferred(active)
             self.cols = {k: tg[k].tensor for k in self.saved_args['materialized in']}
@@ -259,8 +286,8 @@ def run(tensor_group, ja, aa, p, run_join):
     Raises kernels.DeferredJoinFailed before changing anything when the launch cannot be made
     (no key bounds, a key range the kernels do not cover)."""
     lk, rk = p.lk, p.rk
-    bk = tensor_group[lk].tensor.reshape(-1).contiguous()
-    pk = tensor_group[rk].tensor.reshape(-1).contiguous()
+    bk = _flat(tensor_group[lk].tensor)
+    pk = _flat(tensor_group[rk].tensor)
     dev = pk.device
     kb = kernels.known_bounds(tensor_group[lk].tensor)
     if kb is None:
@@ -271,9 +298,9 @@ def run(tensor_group, ja, aa, p, run_join):
             raise kernels.DeferredJoinFailed()
     lo, hi = kb
     rng = hi - lo + 1
-    bmask = tensor_group[p.bmask_id].tensor if p.bmask_id is not None else None
-    pmask = tensor_group[p.pmask_id].tensor if p.pmask_id is not None else None
     nb, np_ = bk.shape[0], pk.shape[0]
+    bmask, bpred = _mask_or_predicate(tensor_group, p.bmask_id)
+    pmask, ppred = _mask_or_predicate(tensor_group, p.pmask_id)
 
     probe_cols = {i: tensor_group[i] for i in p.arg_ids}
 
@@ -324,12 +351,13 @@ def run(tensor_group, ja, aa, p, run_join):
         d.agg_dtype[j] = L.dtype_code(dt)
         d.out_agg[j] = agg_full[j].data_ptr()
     d.out_key, d.out_key_dtype, d.out_cap = key_full.data_ptr(), L.dtype_code(key_dtype), cap
+    if bpred is not None:
+        d.bpred = ctypes.addressof(bpred.prog)
+        d.bpred_cols = ctypes.addressof(bpred.cols)
+    if ppred is not None:
+        d.ppred = ctypes.addressof(ppred.prog)
+        d.ppred_cols = ctypes.addressof(ppred.cols)
 
-    # the state before the join, for the replay of the join's outputs and for the rerun when
-    # the fused result is void
-    saved_join_args = copy.deepcopy(ja)
-    replay = _JoinReplay(run_join, {k: copy.copy(v) for k, v in tensor_group.items()},
-                         saved_join_args)
     key_type = tensor_group[p.gkey].tensor_type
 
     words = ctypes.c_void_p()
@@ -340,7 +368,10 @@ def run(tensor_group, ja, aa, p, run_join):
     rec_bytes = 8 * (1 + naggs)
     out_bytes = key_dtype.itemsize + sum(dt.itemsize for dt in out_dtypes)
     names = ("k_gj_build", "k_gj_probe", "k_gj_reduce")
-    first = (nb * (kbytes + (1 if bmask is not None else 0)), 0, 0)
+    # a fused filter reads its columns in place of the mask byte
+    bsel = (1 if bmask is not None else 0) + (bpred.load_bytes() if bpred is not None else 0)
+    psel = (1 if pmask is not None else 0) + (ppred.load_bytes() if ppred is not None else 0)
+    first = (nb * (kbytes + bsel), 0, 0)
     L.call("tq_groupjoin_run", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), L.ptr(pres),
            pres.numel(), L.ptr(ws), ws_bytes, ctypes.byref(words), L.stream_handle(),
            alg_bytes=first, kernel=names)
@@ -349,13 +380,19 @@ def run(tensor_group, ja, aa, p, run_join):
         # probe: key and mask of every probe row, every gathered column read once over its
         # table (the matches touch nearly every line), one record per pair; reduce: the
         # records and the presence bytes, then the groups' key and aggregate columns
-        return {"k_gj_probe": np_ * (pkbytes + (1 if pmask is not None else 0) + load_bytes)
-                + m * rec_bytes,
+        return {"k_gj_probe": np_ * (pkbytes + psel + load_bytes) + m * rec_bytes,
                 "k_gj_reduce": m * rec_bytes + rng + G * out_bytes}
 
     df = GroupJoinDeferred(words.value, {"ngroups": words.value, "outs": [key_full] + agg_full,
                                          "cap": cap, "ws": ws, "pres": pres},
                            at, bytes_fn)
+
+    # the join's inputs and arguments as they were before the join (nothing has changed them
+    # yet), for the replay of its outputs; Variables are copied, as a later operator may
+    # reassign their fields
+    join_ids = set(ja['left group']) | set(ja['right group']) | set(ja.get('mask map', {}).values())
+    replay = _JoinReplay(run_join, {k: _clone(v) for k, v in tensor_group.items() if k in join_ids},
+                         _copy_args(ja))
 
     # ---- side effects of join_vortex (hashjoin.py:198-238): mask map pops, outputs replaced
     lgroup, rgroup = ja['left group'], ja['right group']
@@ -389,6 +426,19 @@ def run(tensor_group, ja, aa, p, run_join):
     df.written = {**{agg_ids[i]: "agg" for i in range(len(vals))},
                   **{kid: "key" for kid in key_ids_new}}
     return df
+
+
+def _mask_or_predicate(tensor_group, mid):
+    """(mask tensor, None), or (None, bound conjunction) for a filter mask not yet computed
+    whose predicate the kernels evaluate themselves, or (None, None) without a mask."""
+    if mid is None:
+        return None, None
+    mv = tensor_group[mid]
+    if isinstance(mv, LazyMask) and not mv.materialized:
+        b = mv.conjunction()
+        if b is not None:
+            return None, b
+    return mv.tensor, None
 
 
 def _key_bounds(bk):

@@ -30,10 +30,15 @@ def ntiles(n: int) -> int:
     return (n + tr - 1) // tr
 
 
+def filter_program(tree, tensors, subquery_result=None):
+    """The predicate tree compiled and bound to its columns (cached per tree)."""
+    return E.cached_program(("filter", repr(tree), repr(subquery_result)), tensors,
+                            lambda pb: E.compile_predicate(pb, tree, tensors, subquery_result))
+
+
 def filter_mask(tree, tensors, subquery_result=None, n=None):
     """Bool mask of a predicate tree (operators/filter.py:14-162) plus per-tile counts."""
-    b = E.cached_program(("filter", repr(tree), repr(subquery_result)), tensors,
-                         lambda pb: E.compile_predicate(pb, tree, tensors, subquery_result))
+    b = filter_program(tree, tensors, subquery_result)
     rows = b.n if b.n is not None else n
     if rows is None:
         rows = next(iter(tensors.values())).tensor.shape[0]

@@ -6,9 +6,15 @@
 compact by the mask later (late materialisation).  The whole tree runs as one HIP kernel (the
 reference launches one ATen kernel per tree node) that also writes per-tile match counts, which
 ride along on the mask Variable for the consumer's compaction.
+
+With `lazy=True` (the executor's unsynchronised mode) the mask is a `LazyMask`: its kernel runs
+when a consumer first reads `.tensor`, and a consumer that can evaluate the predicate inside its
+own kernels (the fused groupjoin, for a conjunction of comparisons) never reads it, so the mask
+is never written.  The mask's value is the same either way.
 """
 import torch
 
+from .. import _lib as L
 from .. import expr as E
 from .. import kernels
 from ..IO.vortex_pipeline import InMemoryPipeline
@@ -34,7 +40,70 @@ def evaluate(device_name, tensors, tree, rows, subquery_result, leaf_function=No
     return kernels.eval_program(b, n, torch.bool, dev)
 
 
-def tqp_filter(gpu_enable, tensor_group, args, subquery_result, cpu_mem_pool, gpu_mem_pool, name):
+class LazyMask(Variable):
+    """A filter's bool mask whose kernel runs when `.tensor` or `.tile_counts` is first read."""
+
+    def __init__(self, tree, columns, subquery_result):
+        self._tree = tree
+        self._cols = columns          # {id: Variable} snapshot of the filter's inputs
+        self._sub = subquery_result
+        self._t = None
+        self._tc = None
+        self._conj = False            # not yet asked
+        self.rows = next(iter(columns.values())).tensor.shape[0]
+        super().__init__(None, 'bool', None, VariableState.GPU)
+
+    def _run(self):
+        self._t, self._tc = kernels.filter_mask(self._tree, self._cols, self._sub)
+        self._cols = None
+
+    @property
+    def tensor(self):
+        if self._t is None:
+            self._run()
+        return self._t
+
+    @tensor.setter
+    def tensor(self, t):
+        if t is not None:
+            self._t = t
+
+    @property
+    def tile_counts(self):
+        if self._t is None:
+            self._run()
+        return self._tc
+
+    @tile_counts.setter
+    def tile_counts(self, v):
+        if v is not None:
+            self._tc = v
+
+    @property
+    def materialized(self):
+        return self._t is not None
+
+    def conjunction(self):
+        """The predicate bound to its columns when it is a conjunction of column-versus-constant
+        comparisons over columns of `rows` rows (what a consumer's kernel can evaluate itself),
+        else None.  Only for a mask not yet computed."""
+        if self._conj is False:
+            self._conj = None
+            if self._t is None:
+                b = kernels.filter_program(self._tree, self._cols, self._sub)
+                if b.n == self.rows and b.col_tensors and \
+                        L.load().tq_filter_is_conjunction(E.program_ptr(b), E.colset_ptr(b)):
+                    self._conj = b
+        return self._conj
+
+    def __repr__(self):
+        if self._t is None:
+            return "Variable (filter mask, not yet computed, tensor type = 'bool')"
+        return super().__repr__()
+
+
+def tqp_filter(gpu_enable, tensor_group, args, subquery_result, cpu_mem_pool, gpu_mem_pool, name,
+               lazy=False):
     perf_logger().start(name)
     assert 'mask map' not in args, "Filter not accepting masks"
     if not gpu_enable:
@@ -44,6 +113,14 @@ def tqp_filter(gpu_enable, tensor_group, args, subquery_result, cpu_mem_pool, gp
 
     if len(input_columns) == 0:
         message_logger().debug("Filter not dispatched")
+        perf_logger().stop(name)
+        return -1
+
+    if lazy and all(type(v) is Variable and v.tensor.is_cuda and v.tensor.numel() > 0
+                    and v.tensor.is_contiguous() for v in input_columns.values()):
+        cols = {k: Variable(v.tensor, v.tensor_type, None, VariableState.GPU)
+                for k, v in input_columns.items()}
+        tensor_group[args['out mask id']] = LazyMask(tree, cols, subquery_result)
         perf_logger().stop(name)
         return -1
 
