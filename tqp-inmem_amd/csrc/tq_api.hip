@@ -110,20 +110,20 @@ extern "C" int tq_timing_read(tq_ktime_t* out, int64_t cap, int64_t* n, int64_t*
 
 namespace tq {
 
-int64_t resident_blocks(const void* kernel, size_t lds) {
+int64_t resident_blocks(const void* kernel, size_t lds, int block) {
   static std::mutex mu;
-  static std::map<std::tuple<int, const void*, size_t>, int64_t> cache;
+  static std::map<std::tuple<int, const void*, size_t, int>, int64_t> cache;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
     (void)hipGetLastError();
     return 1024;
   }
   std::lock_guard<std::mutex> g(mu);
-  const auto key = std::make_tuple(dev, kernel, lds);
+  const auto key = std::make_tuple(dev, kernel, lds, block);
   const auto it = cache.find(key);
   if (it != cache.end()) return it->second;
   int per = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, lds) != hipSuccess ||
       per < 1) {
     (void)hipGetLastError();
     per = 1;  // e.g. more LDS than a workgroup gets before it is allowed: one per CU

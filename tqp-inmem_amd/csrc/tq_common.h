@@ -178,9 +178,9 @@ __device__ __forceinline__ int64_t run_row(int64_t tile0, int j) {
   return tile0 + (int64_t)(j >> 2) * (4 * kBlock) + 4 * threadIdx.x + (j & 3);
 }
 
-template <typename T, typename Conv>
+template <typename T, typename O, typename Conv>
 __device__ __forceinline__ void load_rows16(const T* p, int64_t tile0, int64_t n,
-                                            uint64_t (&v)[kItems], Conv conv) {
+                                            O (&v)[kItems], Conv conv) {
   T x[kItems];
   const bool full = tile0 + kTileRows <= n;
   const bool vec = (sizeof(T) <= 8) && ((uintptr_t)p % (sizeof(T) * 4 < 16 ? sizeof(T) * 4 : 16)) == 0;
@@ -218,7 +218,7 @@ __device__ __forceinline__ void load_rows16(const T* p, int64_t tile0, int64_t n
     }
   }
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) v[j] = run_row(tile0, j) < n ? conv(x[j]) : 0ull;
+  for (int j = 0; j < kItems; ++j) v[j] = run_row(tile0, j) < n ? (O)conv(x[j]) : (O)0;
 }
 
 // Column values of the thread's 16 tile rows as the expression programs hold them: integers
@@ -244,6 +244,26 @@ __device__ __forceinline__ void load_raw16(const void* p, int dt, int64_t tile0,
       break;
     default:
       load_rows16((const uint64_t*)p, tile0, n, v, [](uint64_t x) { return x; });
+      break;
+  }
+}
+
+// The same in 32 bits, for columns of at most 4 bytes (TQ_BOOL..TQ_I32, TQ_F32): integers
+// sign-extended to 32 bits, bool as 0/1, f32 as its bits.
+__device__ __forceinline__ void load_raw16_32(const void* p, int dt, int64_t tile0, int64_t n,
+                                              uint32_t (&v)[kItems]) {
+  switch (dt) {
+    case TQ_BOOL:
+      load_rows16((const uint8_t*)p, tile0, n, v, [](uint8_t x) { return (uint32_t)(x != 0); });
+      break;
+    case TQ_I8:
+      load_rows16((const int8_t*)p, tile0, n, v, [](int8_t x) { return (uint32_t)(int32_t)x; });
+      break;
+    case TQ_I16:
+      load_rows16((const int16_t*)p, tile0, n, v, [](int16_t x) { return (uint32_t)(int32_t)x; });
+      break;
+    default:  // TQ_I32, TQ_F32
+      load_rows16((const uint32_t*)p, tile0, n, v, [](uint32_t x) { return x; });
       break;
   }
 }
@@ -324,7 +344,7 @@ unsigned stream_grid(int64_t m);
 
 // Workgroups of `kernel` (kBlock threads, `lds` bytes of dynamic LDS) resident on the current
 // device at once: occupancy per CU times the CUs (cached per device, kernel and LDS size).
-int64_t resident_blocks(const void* kernel, size_t lds);
+int64_t resident_blocks(const void* kernel, size_t lds, int block = kBlock);
 
 // Grid of a kernel whose workgroups stride over `items` work items: one resident round, so no
 // workgroup starts only after others have finished all of theirs.

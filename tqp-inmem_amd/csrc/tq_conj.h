@@ -41,6 +41,56 @@ __device__ __forceinline__ uint32_t cmp_bits(int op, const T (&x)[kItems], T c) 
   return b;
 }
 
+// The terms on load slot s applied to that slot's 16 values of a tile, ANDed into acc.
+__device__ __forceinline__ uint32_t conj_apply(const Conj& C, int s, const uint64_t (&v)[kItems],
+                                               uint32_t acc) {
+  for (int k = 0; k < C.nterm; ++k) {
+    if (C.term_slot[k] != s) continue;
+    const int op = C.term_op[k];
+    const int64_t imm = C.term_imm[k];
+    if (op < 6) {
+      int64_t x[kItems];
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) x[j] = (int64_t)v[j];
+      acc &= cmp_bits<int64_t>(op, x, imm);
+    } else if (op < 12) {
+      double x[kItems];
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) x[j] = __longlong_as_double((long long)v[j]);
+      acc &= cmp_bits<double>(op - 6, x, __longlong_as_double(imm));
+    } else {
+      float x[kItems];
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) x[j] = __uint_as_float((uint32_t)v[j]);
+      acc &= cmp_bits<float>(op - 12, x, __uint_as_float((uint32_t)imm));
+    }
+  }
+  return acc;
+}
+
+// conj_apply over values loaded in 32 bits (load_raw16_32: slot dtypes of at most 4 bytes, so
+// no f64 term reads them).
+__device__ __forceinline__ uint32_t conj_apply32(const Conj& C, int s, const uint32_t (&v)[kItems],
+                                                 uint32_t acc) {
+  for (int k = 0; k < C.nterm; ++k) {
+    if (C.term_slot[k] != s) continue;
+    const int op = C.term_op[k];
+    const int64_t imm = C.term_imm[k];
+    if (op < 6) {
+      int64_t x[kItems];
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) x[j] = (int64_t)(int32_t)v[j];
+      acc &= cmp_bits<int64_t>(op, x, imm);
+    } else {
+      float x[kItems];
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) x[j] = __uint_as_float(v[j]);
+      acc &= cmp_bits<float>(op - 12, x, __uint_as_float((uint32_t)imm));
+    }
+  }
+  return acc;
+}
+
 // The rows of a 4096-row tile (thread t: the 16 rows run_row(tile0, j)) that pass every term,
 // as bits of `acc` (rows past n must already be clear in it).
 __device__ __forceinline__ uint32_t conj_sel16(const Conj& C, int64_t tile0, int64_t n,
@@ -48,27 +98,7 @@ __device__ __forceinline__ uint32_t conj_sel16(const Conj& C, int64_t tile0, int
   for (int s = 0; s < C.nslot; ++s) {
     uint64_t v[kItems];
     load_raw16(C.slot_col[s], C.slot_dtype[s], tile0, n, v);
-    for (int k = 0; k < C.nterm; ++k) {
-      if (C.term_slot[k] != s) continue;
-      const int op = C.term_op[k];
-      const int64_t imm = C.term_imm[k];
-      if (op < 6) {
-        int64_t x[kItems];
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) x[j] = (int64_t)v[j];
-        acc &= cmp_bits<int64_t>(op, x, imm);
-      } else if (op < 12) {
-        double x[kItems];
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) x[j] = __longlong_as_double((long long)v[j]);
-        acc &= cmp_bits<double>(op - 6, x, __longlong_as_double(imm));
-      } else {
-        float x[kItems];
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) x[j] = __uint_as_float((uint32_t)v[j]);
-        acc &= cmp_bits<float>(op - 12, x, __uint_as_float((uint32_t)imm));
-      }
-    }
+    acc = conj_apply(C, s, v, acc);
   }
   return acc;
 }

@@ -47,6 +47,7 @@ constexpr size_t kGjTableLds = 48 * 1024;       // reduce table: count + aggrega
 enum { GJ_GROUPS = 0, GJ_PAIRS, GJ_NSEL, GJ_NSET, GJ_ERR, GJ_NWORDS = 8 };
 enum { GJ_ERR_BOUNDS = 1, GJ_ERR_WAIT = 2, GJ_ERR_CAP = 4 };
 constexpr uint32_t kOobBit = 0x80000000u;       // in a build tile count: a key outside [lo, hi]
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;        // a probe row with no key offset to look up
 
 // A slice's look-back word: bit 0 published, then its groups, its stamped keys (each at most
 // 2^sb <= 4096 < 2^14) and its records (the pairs whose key lies in the slice).
@@ -159,7 +160,7 @@ __device__ __forceinline__ void gather_loads(const tq_program_t& P, const tq_col
 // loff[s * W + c] = first record of slice s in chunk c's region (s < S), loff[S * W + c] =
 // the chunk's matches.  A record is 1 + naggs words: {slot within the slice, accumulator-domain
 // values}; a chunk's records are written in slice order by consecutive threads.
-__global__ __launch_bounds__(kBlock) void k_gj_probe(
+__global__ __launch_bounds__(kBlock, 3) void k_gj_probe(
     GjSpec G, tq_program_t P, tq_colset_t C, const void* __restrict__ pkeys, int pdt,
     const uint8_t* __restrict__ pmask, Conj PC, int64_t np, int64_t lo, uint64_t range,
     const uint8_t* __restrict__ pres, uint32_t epoch, int sb, int S, int64_t W,
@@ -174,37 +175,71 @@ __global__ __launch_bounds__(kBlock) void k_gj_probe(
   const int64_t base = c * kGjChunk;
   const uint32_t smask = (1u << sb) - 1u;
   for (int i = t; i < S; i += kBlock) hist[i] = 0;
-  // every key and mask byte of the chunk, then every presence byte, before any is used
-  uint64_t raw[kGjTiles][kItems];
+  // every key, mask byte and filter value of the chunk, then every presence byte, before any
+  // is used; a row's key offset is kNoKey when the row is unselected or its key is outside
+  // [lo, hi] (offsets fit 32 bits: the range is at most kGjMaxSlices << 12 keys)
+  uint32_t off[kGjTiles][kItems];
   uint32_t sel[kGjTiles];
+  uint8_t b[kGjTiles][kItems];  // presence bytes
+  const bool p32 = PC.nslot == 0 ||
+                   (PC.nslot == 1 && (PC.slot_dtype[0] <= TQ_I32 || PC.slot_dtype[0] == TQ_F32));
+  if (pdt <= TQ_I32 && p32) {
+    // keys and the filter's column in 32 bits (fewer registers, all four loads in flight)
+    uint32_t kv[kGjTiles][kItems];
 #pragma unroll
-  for (int q = 0; q < kGjTiles; ++q) {
-    const int64_t tile0 = base + (int64_t)q * kTileRows;
-    load_raw16(pkeys, pdt, tile0, np, raw[q]);  // rows past np read row 0 (unselected)
-    sel[q] = load_sel16(pmask, tile0, np);
-    if (PC.nslot > 0) sel[q] = conj_sel16(PC, tile0, np, sel[q]);  // a fused filter
+    for (int q = 0; q < kGjTiles; ++q) {
+      const int64_t tile0 = base + (int64_t)q * kTileRows;
+      load_raw16_32(pkeys, pdt, tile0, np, kv[q]);  // rows past np read row 0 (unselected)
+      sel[q] = load_sel16(pmask, tile0, np);
+    }
+    uint32_t pv[kGjTiles][kItems];
+    if (PC.nslot == 1) {
+#pragma unroll
+      for (int q = 0; q < kGjTiles; ++q)
+        load_raw16_32(PC.slot_col[0], PC.slot_dtype[0], base + (int64_t)q * kTileRows, np, pv[q]);
+    }
+    // a tile's presence lookups go out as soon as its own loads are in, the next tile's loads
+    // still in flight
+#pragma unroll
+    for (int q = 0; q < kGjTiles; ++q) {
+      if (PC.nslot == 1) sel[q] = conj_apply32(PC, 0, pv[q], sel[q]);
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) {
+        const uint64_t o = (uint64_t)(int64_t)(int32_t)kv[q][j] - (uint64_t)lo;
+        off[q][j] = ((sel[q] >> j) & 1u) && o < range ? (uint32_t)o : kNoKey;
+        b[q][j] = off[q][j] != kNoKey ? pres[off[q][j]] : (uint8_t)0;
+      }
+    }
+  } else {
+    // 8-byte keys or filter columns, or filters over several columns: a tile at a time
+#pragma unroll
+    for (int q = 0; q < kGjTiles; ++q) {
+      const int64_t tile0 = base + (int64_t)q * kTileRows;
+      uint64_t raw[kItems];
+      load_raw16(pkeys, pdt, tile0, np, raw);
+      sel[q] = load_sel16(pmask, tile0, np);
+      if (PC.nslot > 0) sel[q] = conj_sel16(PC, tile0, np, sel[q]);
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) {
+        const uint64_t o = raw[j] - (uint64_t)lo;
+        off[q][j] = ((sel[q] >> j) & 1u) && o < range ? (uint32_t)o : kNoKey;
+        b[q][j] = off[q][j] != kNoKey ? pres[off[q][j]] : (uint8_t)0;
+      }
+    }
   }
   uint32_t hit[kGjTiles];
 #pragma unroll
   for (int q = 0; q < kGjTiles; ++q) {
-    uint8_t b[kItems];
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const uint64_t off = raw[q][j] - (uint64_t)lo;
-      const bool ok = ((sel[q] >> j) & 1u) && off < range;
-      b[j] = ok ? pres[off] : (uint8_t)0;
-    }
     hit[q] = 0;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) hit[q] |= (b[j] == (uint8_t)epoch ? 1u : 0u) << j;
-    hit[q] &= sel[q];
+    for (int j = 0; j < kItems; ++j) hit[q] |= (b[q][j] == (uint8_t)epoch ? 1u : 0u) << j;
   }
   __syncthreads();  // hist cleared
 #pragma unroll
   for (int q = 0; q < kGjTiles; ++q)
 #pragma unroll
     for (int j = 0; j < kItems; ++j)
-      if ((hit[q] >> j) & 1u) atomicAdd(&hist[(raw[q][j] - (uint64_t)lo) >> sb], 1u);
+      if ((hit[q] >> j) & 1u) atomicAdd(&hist[off[q][j] >> sb], 1u);
   __syncthreads();
   // exclusive scan of the slice counts (in place: hist becomes the slices' write cursors)
   uint32_t M;
@@ -236,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_probe(
 #pragma unroll
     for (int j = 0; j < kItems; ++j)
       if ((hit[q] >> j) & 1u) {
-        const uint32_t slot = (uint32_t)(raw[q][j] - (uint64_t)lo);
+        const uint32_t slot = off[q][j];
         const uint32_t lrow = (uint32_t)(q * kTileRows + (j >> 2) * (4 * kBlock) + 4 * t + (j & 3));
         srt[atomicAdd(&hist[slot >> sb], 1u)] = (lrow << sb) | (slot & smask);
       }
@@ -324,25 +359,28 @@ __device__ __forceinline__ uint32_t slice_keys(const GjCheck& K, int s, int sb, 
   return c;
 }
 
-// Dynamic LDS: acc[naggs][ps] u64, cnt[ps] u32.  Static: the current batch of chunks.
+// A reduce workgroup has kRB threads (8 waves: a slice's records, slots and writes spread over
+// twice the waves of a 256-thread group, and two such groups fit a CU's LDS).
+constexpr int kRB = 512;
+constexpr int kRWaves = kRB / 64;
+constexpr int kRItems = 4096 / kRB;  // slots per thread (a slice has at most 4096)
+
+// Dynamic LDS: acc[naggs][ps] u64, cnt[ps] u32.
 // With xper > 0 the grid is 8 * xper workgroups and workgroup b (which the dispatcher places on
 // XCD b % 8) reduces slice (b % 8) * xper + b / 8: consecutive slices run on one XCD, so the
 // record lines one slice fetches beyond its own run (the neighbouring slices' records of the same
 // chunk) are found in that XCD's L2 by the next slice.  The host takes this mapping only when
 // every workgroup of the grid is resident at once, which the look-back (a slice waits for every
 // slice below it, whatever its workgroup index) then needs.
-__global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, int64_t W,
+__global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int64_t W,
                                                       const uint32_t* __restrict__ loff,
                                                       const uint64_t* __restrict__ rec,
                                                       int64_t lo, GjOut O,
                                                       uint64_t* __restrict__ flags, GjCheck K,
                                                       int xper) {
   extern __shared__ uint64_t smem[];
-  __shared__ uint32_t pre[kGjBatch + 1];
-  __shared__ uint32_t st[kGjBatch];
-  __shared__ uint32_t sh[kWaves];
-  __shared__ uint32_t wocc[kItems][kWaves];
-  __shared__ uint32_t s_keys;
+  __shared__ uint32_t wocc[kRItems][kRWaves];
+  __shared__ uint32_t s_keys, s_nrec;
   __shared__ int64_t s_off;
   const int ps = 1 << sb;
   uint64_t* lacc = smem;
@@ -353,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   const int rw = 1 + G.naggs;
   // the presence bytes and the first batch's chunk offsets are in flight while the table clears
   const uint4 kq = slice_bytes(K, s, sb);
-  constexpr int kPer = kGjBatch / kBlock;
+  constexpr int kPer = kGjBatch / kRB;
   uint32_t la[kPer], lb[kPer];
   auto load_batch = [&](int64_t wb) {
     const int nbw = (int)(W - wb < kGjBatch ? W - wb : kGjBatch);
@@ -365,58 +403,54 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
     }
   };
   load_batch(0);
-  for (int i = t; i < ps; i += kBlock) {
+  for (int i = t; i < ps; i += kRB) {
     lcnt[i] = 0;
     for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
   }
-  if (t == 0) s_keys = 0;
+  if (t == 0) {
+    s_keys = 0;
+    s_nrec = 0;
+  }
   __syncthreads();
   {
     const uint32_t kc = wave_sum(slice_keys(K, s, sb, kq));
     if (lane_id() == 0 && kc) atomicAdd(&s_keys, kc);
   }
   const uint64_t smask = (uint64_t)ps - 1;
-  uint32_t nrec = 0;  // this slice's records (uniform)
   for (int64_t wb = 0; wb < W; wb += kGjBatch) {
     const int nbw = (int)(W - wb < kGjBatch ? W - wb : kGjBatch);
-    // this batch's chunks: first record of slice s and how many
-    uint32_t cnt[kPer], sum = 0;
+    // thread t expands the runs of slice s in its own chunks t * kPer + k of the batch
+    uint32_t cnt[kPer], mine = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int i = t * kPer + k;
-      cnt[k] = 0;
-      if (i < nbw) {
-        st[i] = la[k];
-        cnt[k] = lb[k] - la[k];
-      }
-      sum += cnt[k];
+      cnt[k] = t * kPer + k < nbw ? lb[k] - la[k] : 0u;
+      mine += cnt[k];
     }
-    uint32_t tot;
-    uint32_t run = block_exclusive_sum(sum, sh, &tot);
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int i = t * kPer + k;
-      if (i < nbw) pre[i] = run;
-      run += cnt[k];
+    {
+      const uint32_t wn = wave_sum(mine);
+      if (lane_id() == 0 && wn) atomicAdd(&s_nrec, wn);
     }
-    if (t == 0) pre[nbw] = tot;
-    nrec += tot;
-    __syncthreads();
     constexpr int U = 8;  // records per thread whose loads are issued together
-    for (uint32_t i0 = t; i0 < tot; i0 += U * kBlock) {
+    for (uint32_t p0 = 0; p0 < mine; p0 += U) {
       int64_t at[U];
       bool ok[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const uint32_t idx = i0 + u * kBlock;
-        ok[u] = idx < tot;
-        // the chunk holding record idx: the last i with pre[i] <= idx
-        int lo_i = 0, hi_i = nbw;  // pre[lo_i] <= idx < pre[hi_i]
-        while (hi_i - lo_i > 1) {
-          const int mid = (lo_i + hi_i) >> 1;
-          if (pre[mid] <= idx) lo_i = mid; else hi_i = mid;
-        }
-        at[u] = ok[u] ? ((wb + lo_i) * kGjChunk + st[lo_i] + (idx - pre[lo_i])) : 0;
+        // record p0 + u of the thread's runs: run k, position q in it
+        uint32_t q = p0 + u;
+        ok[u] = q < mine;
+        int k = 0;
+#pragma unroll
+        for (int kk = 0; kk + 1 < kPer; ++kk)
+          if (k == kk && q >= cnt[kk]) {
+            q -= cnt[kk];
+            k = kk + 1;
+          }
+        uint32_t first = la[0];
+#pragma unroll
+        for (int kk = 1; kk < kPer; ++kk)
+          if (k == kk) first = la[kk];
+        at[u] = ok[u] ? ((wb + t * kPer + k) * kGjChunk + first + q) : 0;
       }
       if (G.naggs == 1) {
         ulonglong2 x[U];
@@ -452,12 +486,13 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
     __syncthreads();
     if (wb + kGjBatch < W) load_batch(wb + kGjBatch);
   }
-  // occupied slots in slot order: slot i = j*256 + t, ranked by wave ballots
+  const uint32_t nrec = s_nrec;  // this slice's records
+  // occupied slots in slot order: slot i = j * kRB + t, ranked by wave ballots
   const int w = t >> 6;
-  uint64_t bal[kItems];
+  uint64_t bal[kRItems];
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const int i = j * kBlock + t;
+  for (int j = 0; j < kRItems; ++j) {
+    const int i = j * kRB + t;
     bal[j] = ballot(i < ps && lcnt[i] > 0);
     if (lane_id() == 0) wocc[j][w] = (uint32_t)__popcll(bal[j]);
   }
@@ -466,15 +501,15 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   // slice waited on belongs to a workgroup that is resident or done)
   uint64_t own = 0;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j)
+  for (int j = 0; j < kRItems; ++j)
 #pragma unroll
-    for (int v = 0; v < kWaves; ++v) own += wocc[j][v];
+    for (int v = 0; v < kRWaves; ++v) own += wocc[j][v];
   if (t == 0) gj_st(flags + s, lb_word(own, s_keys, nrec));
   {
     uint64_t before = 0, keys = 0, recs = 0;
     bool okw = true;
     const int64_t t0 = wall_clock64();
-    for (int q0 = 0; q0 < s && okw; q0 += kBlock) {
+    for (int q0 = 0; q0 < s && okw; q0 += kRB) {
       const int q = q0 + t;
       uint64_t f = 1ull;  // threads past s - 1 add nothing
       while (true) {
@@ -492,8 +527,8 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
         recs += lb_recs(f);
       }
     }
-    __shared__ uint64_t lbs[3][kWaves];
-    __shared__ uint32_t lbok[kWaves];
+    __shared__ uint64_t lbs[3][kRWaves];
+    __shared__ uint32_t lbok[kRWaves];
     before = wave_sum(before);
     keys = wave_sum(keys);
     recs = wave_sum(recs);
@@ -508,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
       uint64_t b = 0, k = 0, r = 0;
       bool ok = true;
 #pragma unroll
-      for (int v = 0; v < kWaves; ++v) {
+      for (int v = 0; v < kRWaves; ++v) {
         b += lbs[0][v];
         k += lbs[1][v];
         r += lbs[2][v];
@@ -526,16 +561,16 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   __syncthreads();
   const int64_t obase = s_off;
   const int64_t kbase = lo + ((int64_t)s << sb);
-  // output row of slot j * 256 + t (-1: unoccupied, or no valid offset)
-  int64_t opos[kItems];
+  // output row of slot j * kRB + t (-1: unoccupied, or no valid offset)
+  int64_t opos[kRItems];
   {
     uint32_t run = 0;
     bool over = false;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    for (int j = 0; j < kRItems; ++j) {
       uint32_t before = 0, tot = 0;
 #pragma unroll
-      for (int v = 0; v < kWaves; ++v) {
+      for (int v = 0; v < kRWaves; ++v) {
         const uint32_t cc = wocc[j][v];
         tot += cc;
         before += (v < w) ? cc : 0u;
@@ -550,9 +585,9 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
   }
   // one tight loop per output column, the dtype dispatched once per column
 #define TQ_PUT(T, VAL)                                     \
-  _Pragma("unroll") for (int j = 0; j < kItems; ++j) {     \
+  _Pragma("unroll") for (int j = 0; j < kRItems; ++j) {    \
     if (opos[j] < 0) continue;                             \
-    const int i = j * kBlock + t;                          \
+    const int i = j * kRB + t;                             \
     ((T*)dst)[opos[j]] = (VAL);                            \
   }
   {
@@ -576,9 +611,9 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
     } else {
       const int op = G.op[a], kind = G.kind[a], dt = G.arg_dtype[a];
 #pragma unroll
-      for (int j = 0; j < kItems; ++j) {
+      for (int j = 0; j < kRItems; ++j) {
         if (opos[j] < 0) continue;
-        const int i = j * kBlock + t;
+        const int i = j * kRB + t;
         write_agg(dst, opos[j], op, kind, dt, acc[i], lcnt[i]);
       }
     }
@@ -589,15 +624,15 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
     // selected build rows, and whether a key fell outside the bounds
     int64_t nsel = 0;
     uint32_t oob = 0;
-    for (int64_t i = t; i < K.nbt; i += kBlock) {
+    for (int64_t i = t; i < K.nbt; i += kRB) {
       const uint32_t v = K.btile[i];
       nsel += v & ~kOobBit;
       oob |= v & kOobBit;
     }
     nsel = wave_sum(nsel);
     const bool any_oob = ballot(oob != 0) != 0;
-    __shared__ int64_t red[kWaves];
-    __shared__ uint32_t roob[kWaves];
+    __shared__ int64_t red[kRWaves];
+    __shared__ uint32_t roob[kRWaves];
     if (lane_id() == 0) {
       red[w] = nsel;
       roob[w] = any_oob ? 1u : 0u;
@@ -607,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_gj_reduce(GjSpec G, int sb, int S, i
       int64_t a = 0;
       uint32_t e = 0;
 #pragma unroll
-      for (int i = 0; i < kWaves; ++i) {
+      for (int i = 0; i < kRWaves; ++i) {
         a += red[i];
         e |= roob[i];
       }
@@ -666,8 +701,8 @@ static int gj_check_dtype(int dt, const char* what) {
 }
 
 template <typename... P, typename... A>
-static int gj_launch_lds(const char* name, void (*kernel)(P...), unsigned grid, size_t lds,
-                         hipStream_t s, A&&... a) {
+static int gj_launch_lds(const char* name, void (*kernel)(P...), unsigned grid, int block,
+                         size_t lds, hipStream_t s, A&&... a) {
   if (lds > (size_t)kLdsBig) return set_error(TQ_ECAPACITY, "%s: %zu bytes of LDS", name, lds);
   if (lds > 64 * 1024) {
     static std::mutex mu;
@@ -684,7 +719,7 @@ static int gj_launch_lds(const char* name, void (*kernel)(P...), unsigned grid, 
       have = true;
     }
   }
-  return launch(name, kernel, grid, kBlock, lds, s, std::forward<A>(a)...);
+  return launch(name, kernel, grid, block, lds, s, std::forward<A>(a)...);
 }
 
 // A fused filter's conjunction (none: nslot = 0).
@@ -791,7 +826,7 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
   if (d->np > 0) {
     const size_t lds = (size_t)p.stage_words * 8 + (size_t)((p.S + 1) & ~1) * 4 +
                        (size_t)kGjChunk * 4;
-    TQ_TRY(gj_launch_lds("k_gj_probe", k_gj_probe, (unsigned)p.W, lds, s, G, *prog, *cols,
+    TQ_TRY(gj_launch_lds("k_gj_probe", k_gj_probe, (unsigned)p.W, kBlock, lds, s, G, *prog, *cols,
                          d->pkey, (int)d->pkey_dtype, d->pmask, PC, d->np, d->key_lo,
                          (uint64_t)range, (const uint8_t*)pres, epoch, p.sb, p.S,
                          p.W, loff, rec, p.stage_words));
@@ -805,9 +840,9 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
   // consecutive slices on one XCD when the remapped grid is resident all at once
   const int xper = (int)ceil_div(p.S, 8);
   const bool remap = p.S > 8 && rlds <= 64 * 1024 &&
-                     (int64_t)8 * xper <= resident_blocks((const void*)k_gj_reduce, rlds);
+                     (int64_t)8 * xper <= resident_blocks((const void*)k_gj_reduce, rlds, kRB);
   const unsigned rgrid = remap ? (unsigned)(8 * xper) : (unsigned)p.S;
-  return gj_launch_lds("k_gj_reduce", k_gj_reduce, rgrid, rlds, s, G, p.sb, p.S, p.W,
+  return gj_launch_lds("k_gj_reduce", k_gj_reduce, rgrid, kRB, rlds, s, G, p.sb, p.S, p.W,
                        (const uint32_t*)loff, (const uint64_t*)rec, d->key_lo, O, flags, K,
                        remap ? xper : 0);
 }
