@@ -544,6 +544,10 @@ typedef struct {
   const tq_colset_t* bpred_cols;
   const tq_program_t* ppred;  /* host */
   const tq_colset_t* ppred_cols;
+  /* 0 (or 3): the whole call.  1: only the build's launch (prog and cols may be NULL; the
+   * aggregate and output fields are not read), so it runs while the caller prepares the rest;
+   * 2: the rest, after a phase-1 call on the same stream with the same desc, pres and ws. */
+  int32_t phases;
 } tq_groupjoin_desc_t;
 
 size_t tq_groupjoin_pres_bytes(int64_t key_range);

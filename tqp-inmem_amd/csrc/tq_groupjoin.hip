@@ -771,7 +771,10 @@ namespace tq {
 static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program_t* prog,
                   const tq_colset_t* cols, void* pres, size_t pres_bytes, void* ws,
                   size_t ws_bytes, int64_t* words_ext, int64_t** words, void* stream) {
-  if (!d || !prog || !cols || !words) return set_error(TQ_EINVAL, "groupjoin: null argument");
+  if (!d || !words) return set_error(TQ_EINVAL, "groupjoin: null argument");
+  const int ph = d->phases == 0 ? 3 : d->phases;  // bit 0: the build, bit 1: probe and reduce
+  if (ph < 1 || ph > 3) return set_error(TQ_EINVAL, "groupjoin: phases %d", d->phases);
+  if ((ph & 2) && (!prog || !cols)) return set_error(TQ_EINVAL, "groupjoin: null program");
   if (d->nb < 0 || d->np < 0) return set_error(TQ_EINVAL, "groupjoin: negative row count");
   if ((d->nb > 0 && !d->bkey) || (d->np > 0 && !d->pkey))
     return set_error(TQ_EINVAL, "groupjoin: null key column");
@@ -779,38 +782,41 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
   TQ_TRY(gj_check_dtype(d->pkey_dtype, "the probe key"));
   TQ_TRY(gj_check_dtype(d->out_key_dtype, "the group key output"));
   Conj BC{}, PC{};
-  TQ_TRY(gj_pred(d->bpred, d->bpred_cols, "build", &BC));
-  TQ_TRY(gj_pred(d->ppred, d->ppred_cols, "probe", &PC));
+  if (ph & 1) TQ_TRY(gj_pred(d->bpred, d->bpred_cols, "build", &BC));
+  if (ph & 2) TQ_TRY(gj_pred(d->ppred, d->ppred_cols, "probe", &PC));
   if (d->key_hi < d->key_lo) return set_error(TQ_EINVAL, "groupjoin: key bounds hi < lo");
   const uint64_t urange = (uint64_t)d->key_hi - (uint64_t)d->key_lo + 1;
   if (urange == 0 || urange > ((uint64_t)1 << 40))
     return set_error(TQ_EUNSUPPORTED, "groupjoin: key range too wide");
   const int64_t range = (int64_t)urange;
-  if (prog->nseg != d->naggs)
+  if ((ph & 2) && prog->nseg != d->naggs)
     return set_error(TQ_EINVAL, "groupjoin: %d program segments for %d aggregates", prog->nseg,
                      d->naggs);
   if (epoch < 1 || epoch > 255) return set_error(TQ_EINVAL, "groupjoin: epoch %u", epoch);
   GjPlan p;
-  TQ_TRY(gj_plan(d->nb, d->np, range, d->naggs, prog->nload, &p));
+  TQ_TRY(gj_plan(d->nb, d->np, range, d->naggs, (ph & 2) ? prog->nload : 1, &p));
   if (!ws || ws_bytes < p.bytes)
     return set_error(TQ_ECAPACITY, "groupjoin: workspace of %zu bytes, %zu needed", ws_bytes, p.bytes);
   if (!pres || pres_bytes < tq_groupjoin_pres_bytes(range))
     return set_error(TQ_ECAPACITY, "groupjoin: presence array of %zu bytes, %zu needed",
                      pres_bytes, tq_groupjoin_pres_bytes(range));
-  if (d->out_cap > 0 && !d->out_key) return set_error(TQ_EINVAL, "groupjoin: null key output");
   GjSpec G{};
   GjOut O{};
-  G.naggs = d->naggs;
-  for (int a = 0; a < d->naggs; ++a) {
-    TQ_TRY(agg_kind_init(d->agg_op[a], d->agg_dtype[a], &G.kind[a], &G.init[a]));
-    G.arg_dtype[a] = d->agg_dtype[a];
-    G.op[a] = d->agg_op[a];
-    O.agg[a] = d->out_agg[a];
-    if (d->out_cap > 0 && !O.agg[a]) return set_error(TQ_EINVAL, "groupjoin: null aggregate output");
+  if (ph & 2) {
+    if (d->out_cap > 0 && !d->out_key) return set_error(TQ_EINVAL, "groupjoin: null key output");
+    G.naggs = d->naggs;
+    for (int a = 0; a < d->naggs; ++a) {
+      TQ_TRY(agg_kind_init(d->agg_op[a], d->agg_dtype[a], &G.kind[a], &G.init[a]));
+      G.arg_dtype[a] = d->agg_dtype[a];
+      G.op[a] = d->agg_op[a];
+      O.agg[a] = d->out_agg[a];
+      if (d->out_cap > 0 && !O.agg[a])
+        return set_error(TQ_EINVAL, "groupjoin: null aggregate output");
+    }
+    O.key = d->out_key;
+    O.key_dtype = d->out_key_dtype;
+    O.cap = d->out_cap;
   }
-  O.key = d->out_key;
-  O.key_dtype = d->out_key_dtype;
-  O.cap = d->out_cap;
   hipStream_t s = (hipStream_t)stream;
   char* w = (char*)ws;
   int64_t* wd = words_ext ? words_ext : (int64_t*)(w + p.off_words);
@@ -820,9 +826,11 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
   uint64_t* rec = (uint64_t*)(w + p.off_rec);
   *words = wd;
   const unsigned bgrid = (unsigned)(d->nb > 0 ? p.nbt : 1);
-  TQ_TRY(launch("k_gj_build", k_gj_build, bgrid, kBlock, 0, s, d->bkey, (int)d->bkey_dtype,
-                d->bmask, BC, d->nb, d->key_lo, (uint64_t)range, (uint8_t*)pres, epoch,
-                btile, flags, p.S, wd));
+  if (ph & 1)
+    TQ_TRY(launch("k_gj_build", k_gj_build, bgrid, kBlock, 0, s, d->bkey, (int)d->bkey_dtype,
+                  d->bmask, BC, d->nb, d->key_lo, (uint64_t)range, (uint8_t*)pres, epoch,
+                  btile, flags, p.S, wd));
+  if (!(ph & 2)) return TQ_OK;
   if (d->np > 0) {
     const size_t lds = (size_t)p.stage_words * 8 + (size_t)((p.S + 1) & ~1) * 4 +
                        (size_t)kGjChunk * 4;
@@ -883,6 +891,8 @@ extern "C" int tq_groupjoin_run_noalloc(const tq_groupjoin_desc_t* d, const tq_p
                                         void* stream) {
   if (!d || !prog || !cols || !words_out)
     return set_error(TQ_EINVAL, "groupjoin (no allocation): null argument");
+  if (d->phases != 0 && d->phases != 3)
+    return set_error(TQ_EINVAL, "groupjoin (no allocation): runs whole calls (phases 0)");
   if (d->key_hi < d->key_lo)
     return set_error(TQ_EINVAL, "groupjoin (no allocation): key bounds hi < lo");
   const uint64_t urange = (uint64_t)d->key_hi - (uint64_t)d->key_lo + 1;

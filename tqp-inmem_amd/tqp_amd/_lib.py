@@ -128,7 +128,8 @@ class GroupJoinDesc(ctypes.Structure):
                 ("out_key", ctypes.c_void_p), ("out_key_dtype", ctypes.c_int32),
                 ("out_agg", ctypes.c_void_p * TQ_MAX_SEGS), ("out_cap", ctypes.c_int64),
                 ("bpred", ctypes.c_void_p), ("bpred_cols", ctypes.c_void_p),
-                ("ppred", ctypes.c_void_p), ("ppred_cols", ctypes.c_void_p)]
+                ("ppred", ctypes.c_void_p), ("ppred_cols", ctypes.c_void_p),
+                ("phases", ctypes.c_int32)]
 
 
 GJ_GROUPS, GJ_PAIRS, GJ_NSEL, GJ_NSET, GJ_ERR = range(5)

@@ -16,6 +16,7 @@ pairs or a look-back that gave up voids the fused result: the executor then runs
 operators as usual from the state saved before the join.
 """
 import ctypes
+import functools
 
 import torch
 
@@ -44,8 +45,23 @@ def _one_mask(ids, mask_map):
     return masks.pop() if len(masks) == 1 else False
 
 
-def plan(tensor_group, ja, aa):
-    """A _Plan when (Join ja, Aggregate aa) can run fused, else None.  Nothing is modified."""
+_STATIC = {}           # argument signature -> _Plan (or None: never fusable)
+_STATIC_MAX = 256
+
+
+def _signature(ja, aa):
+    """Everything in the two nodes' arguments that decides fusability and the fused program."""
+    return (ja.get('name'), ja.get('join type'), 'join condition' in ja,
+            tuple(ja.get('left keys', ())), tuple(ja.get('right keys', ())),
+            tuple(ja.get('left group', ())), tuple(ja.get('right group', ())),
+            tuple(ja.get('materialized in', ())), tuple(sorted(ja.get('mask map', {}).items())),
+            aa.get('name'), bool(aa.get('is partial')), tuple(aa.get('input ids', ())),
+            repr(aa.get('keys')), repr(aa.get('results')),
+            tuple(sorted(aa.get('mask map', {}).items())), tuple(aa.get('out agg ids', ())))
+
+
+def _static_plan(ja, aa):
+    """The checks and parsing that depend on the arguments alone (cached by their signature)."""
     if ja.get('name') != 'Join' or aa.get('name') != 'Aggregate' or aa.get('is partial'):
         return None
     if ja.get('join type') != 'Inner' or 'join condition' in ja:
@@ -56,16 +72,7 @@ def plan(tensor_group, ja, aa):
     left, right = set(ja['left group']), set(ja['right group'])
     mat = list(ja['materialized in'])
     mask_map = ja.get('mask map', {})
-    if lk not in left or rk not in right or lk not in tensor_group or rk not in tensor_group:
-        return None
-    bk, pk = tensor_group[lk].tensor, tensor_group[rk].tensor
-    for t in (bk, pk):
-        if t.dim() == 2 and t.shape[1] == 1:
-            continue
-        if t.dim() != 1:
-            return None
-    if bk.dtype.is_floating_point or pk.dtype.is_floating_point or bk.dtype == torch.bool \
-            or pk.dtype == torch.bool or not bk.is_cuda or not pk.is_cuda:
+    if lk not in left or rk not in right:
         return None
     # the aggregate reads only the join's outputs, unmasked, and groups by the join key alone
     ids = list(aa['input ids'])
@@ -98,13 +105,40 @@ def plan(tensor_group, ja, aa):
     pmask_id = _one_mask([rk] + sorted(arg_ids), mask_map)
     if pmask_id is False:
         return None
-    if bmask_id is not None and bmask_id not in tensor_group:
+    return _Plan(lk=lk, rk=rk, gkey=gkey, calls=calls, plans=plans, arg_ids=sorted(arg_ids),
+                 bmask_id=bmask_id, pmask_id=pmask_id,
+                 prog_key=("groupjoin", repr([c for c in calls])))
+
+
+def plan(tensor_group, ja, aa):
+    """A _Plan when (Join ja, Aggregate aa) can run fused, else None.  Nothing is modified."""
+    sig = _signature(ja, aa)
+    if sig in _STATIC:
+        p = _STATIC[sig]
+    else:
+        p = _static_plan(ja, aa)
+        if len(_STATIC) >= _STATIC_MAX:
+            _STATIC.clear()
+        _STATIC[sig] = p
+    if p is None:
         return None
-    if pmask_id is not None and pmask_id not in tensor_group:
+    # the columns themselves
+    if p.lk not in tensor_group or p.rk not in tensor_group:
         return None
-    for mid, n in ((bmask_id, bk.shape[0]), (pmask_id, pk.shape[0])):
+    bk, pk = tensor_group[p.lk].tensor, tensor_group[p.rk].tensor
+    for t in (bk, pk):
+        if t.dim() == 2 and t.shape[1] == 1:
+            continue
+        if t.dim() != 1:
+            return None
+    if bk.dtype.is_floating_point or pk.dtype.is_floating_point or bk.dtype == torch.bool \
+            or pk.dtype == torch.bool or not bk.is_cuda or not pk.is_cuda:
+        return None
+    for mid, n in ((p.bmask_id, bk.shape[0]), (p.pmask_id, pk.shape[0])):
         if mid is None:
             continue
+        if mid not in tensor_group:
+            return None
         mv = tensor_group[mid]
         if isinstance(mv, LazyMask) and not mv.materialized:
             if mv.rows != n:          # (a mask not yet computed is a 1-D bool of `rows` rows)
@@ -114,12 +148,10 @@ def plan(tensor_group, ja, aa):
         if m.dtype != torch.bool or m.dim() != 1 or m.shape[0] != n:
             return None
     np_ = pk.shape[0]
-    for i in arg_ids:
-        t = tensor_group[i].tensor
-        if t.shape[0] != np_:
+    for i in p.arg_ids:
+        if tensor_group[i].tensor.shape[0] != np_:
             return None
-    return _Plan(lk=lk, rk=rk, gkey=gkey, calls=calls, plans=plans, arg_ids=sorted(arg_ids),
-                 bmask_id=bmask_id, pmask_id=pmask_id)
+    return p
 
 
 def _collect_ids(tree, out):
@@ -299,9 +331,35 @@ def run(tensor_group, ja, aa, p, run_join):
     lo, hi = kb
     rng = hi - lo + 1
     nb, np_ = bk.shape[0], pk.shape[0]
+    naggs = len(p.calls)
+    ws_bytes, pres_bytes = _sizes(nb, np_, rng, naggs, 1)
+    if ws_bytes == 0:
+        raise kernels.DeferredJoinFailed()
     bmask, bpred = _mask_or_predicate(tensor_group, p.bmask_id)
-    pmask, ppred = _mask_or_predicate(tensor_group, p.pmask_id)
+    pres, epoch = _presence(pres_bytes, dev)
+    ws = L.workspace(ws_bytes, dev, "groupjoin")
+    key_dtype = tensor_group[p.gkey].tensor.dtype
 
+    # the build's launch goes out first and runs while the rest of the call is prepared
+    d = L.GroupJoinDesc()
+    d.bkey, d.bkey_dtype = bk.data_ptr(), L.dtype_code(bk.dtype)
+    d.bmask, d.nb = (bmask.data_ptr() if bmask is not None else None), nb
+    d.pkey, d.pkey_dtype = pk.data_ptr(), L.dtype_code(pk.dtype)
+    d.np = np_
+    d.key_lo, d.key_hi, d.epoch, d.naggs = lo, hi, epoch, naggs
+    d.out_key_dtype = L.dtype_code(key_dtype)
+    if bpred is not None:
+        d.bpred = ctypes.addressof(bpred.prog)
+        d.bpred_cols = ctypes.addressof(bpred.cols)
+    # a fused filter reads its columns in place of the mask byte
+    bsel = (1 if bmask is not None else 0) + (bpred.load_bytes() if bpred is not None else 0)
+    words = ctypes.c_void_p()
+    d.phases = 1
+    L.call("tq_groupjoin_run", ctypes.byref(d), None, None, L.ptr(pres), pres.numel(), L.ptr(ws),
+           ws_bytes, ctypes.byref(words), L.stream_handle(),
+           kernel={"k_gj_build": nb * (bk.element_size() + bsel)})
+
+    pmask, ppred = _mask_or_predicate(tensor_group, p.pmask_id)
     probe_cols = {i: tensor_group[i] for i in p.arg_ids}
 
     def compile_fn(b):
@@ -317,64 +375,41 @@ def run(tensor_group, ja, aa, p, run_join):
         return dtypes
 
     try:
-        b = E.cached_program(("groupjoin", repr([c for c in p.calls])), probe_cols, compile_fn)
+        b = E.cached_program(p.prog_key, probe_cols, compile_fn)
     except NotImplementedError:
-        raise kernels.DeferredJoinFailed()
+        raise kernels.DeferredJoinFailed()   # the build launch left nothing another call reads
     arg_dtypes = list(b.extra)
     if not b.col_tensors:
         # no column read (count(*) only): point the column set at the probe key
         b.col_tensors = [pk]
         b.cols.ptr[0] = pk.data_ptr()
         b.cols.width[0] = 1
-    naggs = len(p.calls)
-    ws_bytes = L.size_call("tq_groupjoin_ws_bytes", nb, np_, rng, naggs, b.nload)
-    if ws_bytes == 0:
-        raise kernels.DeferredJoinFailed()
-    pres_bytes = L.size_call("tq_groupjoin_pres_bytes", rng)
     ops = [OPS[c[0]] for c in p.calls]
     out_dtypes = [_out_dtype(op, dt) for op, dt in zip(ops, arg_dtypes)]
-    key_dtype = tensor_group[p.gkey].tensor.dtype
     cap = max(1, min(rng, np_))
     key_full = memory.empty(cap, dtype=key_dtype, device=dev)
     agg_full = [memory.empty(cap, dtype=dt, device=dev) for dt in out_dtypes]
-    pres, epoch = _presence(pres_bytes, dev)
-    ws = L.workspace(ws_bytes, dev, "groupjoin")
-
-    d = L.GroupJoinDesc()
-    d.bkey, d.bkey_dtype = bk.data_ptr(), L.dtype_code(bk.dtype)
-    d.bmask, d.nb = (bmask.data_ptr() if bmask is not None else None), nb
-    d.pkey, d.pkey_dtype = pk.data_ptr(), L.dtype_code(pk.dtype)
-    d.pmask, d.np = (pmask.data_ptr() if pmask is not None else None), np_
-    d.key_lo, d.key_hi, d.epoch, d.naggs = lo, hi, epoch, naggs
+    d.pmask = pmask.data_ptr() if pmask is not None else None
     for j, (op, dt) in enumerate(zip(ops, arg_dtypes)):
         d.agg_op[j] = op
         d.agg_dtype[j] = L.dtype_code(dt)
         d.out_agg[j] = agg_full[j].data_ptr()
-    d.out_key, d.out_key_dtype, d.out_cap = key_full.data_ptr(), L.dtype_code(key_dtype), cap
-    if bpred is not None:
-        d.bpred = ctypes.addressof(bpred.prog)
-        d.bpred_cols = ctypes.addressof(bpred.cols)
+    d.out_key, d.out_cap = key_full.data_ptr(), cap
     if ppred is not None:
         d.ppred = ctypes.addressof(ppred.prog)
         d.ppred_cols = ctypes.addressof(ppred.cols)
 
-    key_type = tensor_group[p.gkey].tensor_type
-
-    words = ctypes.c_void_p()
     at = L.trace_len()
-    kbytes = bk.element_size()
     pkbytes = pk.element_size()
     load_bytes = b.load_bytes()
     rec_bytes = 8 * (1 + naggs)
     out_bytes = key_dtype.itemsize + sum(dt.itemsize for dt in out_dtypes)
-    names = ("k_gj_build", "k_gj_probe", "k_gj_reduce")
-    # a fused filter reads its columns in place of the mask byte
-    bsel = (1 if bmask is not None else 0) + (bpred.load_bytes() if bpred is not None else 0)
     psel = (1 if pmask is not None else 0) + (ppred.load_bytes() if ppred is not None else 0)
-    first = (nb * (kbytes + bsel), 0, 0)
+    d.phases = 2
     L.call("tq_groupjoin_run", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), L.ptr(pres),
            pres.numel(), L.ptr(ws), ws_bytes, ctypes.byref(words), L.stream_handle(),
-           alg_bytes=first, kernel=names)
+           alg_bytes=(0, 0), kernel=("k_gj_probe", "k_gj_reduce"))
+    key_type = tensor_group[p.gkey].tensor_type
 
     def bytes_fn(m, G):
         # probe: key and mask of every probe row, every gathered column read once over its
@@ -426,6 +461,13 @@ def run(tensor_group, ja, aa, p, run_join):
     df.written = {**{agg_ids[i]: "agg" for i in range(len(vals))},
                   **{kid: "key" for kid in key_ids_new}}
     return df
+
+
+@functools.lru_cache(maxsize=256)
+def _sizes(nb, np_, rng, naggs, nload):
+    """(workspace bytes, presence bytes) of a call of this shape."""
+    return (L.size_call("tq_groupjoin_ws_bytes", nb, np_, rng, naggs, nload),
+            L.size_call("tq_groupjoin_pres_bytes", rng))
 
 
 def _mask_or_predicate(tensor_group, mid):

@@ -40,6 +40,22 @@ def evaluate(device_name, tensors, tree, rows, subquery_result, leaf_function=No
     return kernels.eval_program(b, n, torch.bool, dev)
 
 
+_CONJ = {}   # id(program) -> (program, is a conjunction): the answer for a cached program
+
+
+def _is_conjunction(b):
+    """Whether the kernels' conjunction form covers the bound predicate program b (the answer
+    depends on the program and its columns' widths, fixed for a cached program)."""
+    ent = _CONJ.get(id(b.prog))
+    if ent is not None and ent[0] is b.prog:
+        return ent[1]
+    yes = bool(L.load().tq_filter_is_conjunction(E.program_ptr(b), E.colset_ptr(b)))
+    if len(_CONJ) > 512:
+        _CONJ.clear()
+    _CONJ[id(b.prog)] = (b.prog, yes)
+    return yes
+
+
 class LazyMask(Variable):
     """A filter's bool mask whose kernel runs when `.tensor` or `.tile_counts` is first read."""
 
@@ -91,8 +107,7 @@ class LazyMask(Variable):
             self._conj = None
             if self._t is None:
                 b = kernels.filter_program(self._tree, self._cols, self._sub)
-                if b.n == self.rows and b.col_tensors and \
-                        L.load().tq_filter_is_conjunction(E.program_ptr(b), E.colset_ptr(b)):
+                if b.n == self.rows and b.col_tensors and _is_conjunction(b):
                     self._conj = b
         return self._conj
 
