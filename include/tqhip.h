@@ -175,6 +175,14 @@ int tq_filter_is_conjunction(const tq_program_t* prog /* host */,
 int tq_eval_column(const tq_program_t* prog /* host */, const tq_colset_t* cols /* host */,
                    int64_t n, void* out, int32_t out_bytes, void* stream);
 
+/* Every segment of a program per row, segment a into outs[a] (device) as out_bytes[a]-wide
+ * values, the loads staged once for all segments.  Replaces the per-expression `evaluate`
+ * calls of operators/project.py:26-40 (each an elementwise ATen chain, then a cast through
+ * float64). */
+int tq_eval_columns(const tq_program_t* prog /* host */, const tq_colset_t* cols /* host */,
+                    int64_t n, void* const* outs /* host array */,
+                    const int32_t* out_bytes /* host array */, void* stream);
+
 /* Per-tile popcount of an existing bool mask (for masks not made by tq_filter_mask). */
 int tq_mask_tile_counts(const uint8_t* mask, int64_t n, uint32_t* tile_counts, void* stream);
 

@@ -549,6 +549,46 @@ def tqp_sort(tensor_group, args, with_limit):
     return rows
 
 
+# ====================================================================== project
+def tqp_project(tensor_group, args):
+    """operators/project.py:10-69: renames, then every used expression that is not a bare or
+    renamed column is evaluated per row (aggregate.py's evaluate), cast through float64 and back
+    to its own dtype, and stored under `materialized out` in order."""
+    for orig, new in args['rename map'].items():
+        tensor_group[new] = tensor_group[orig]
+    if len(args['used expr']) == 0:
+        return -1
+    cols = {k: tensor_group[k] for k in args['materialized in']}
+    rows = next(iter(cols.values())).tensor.shape[0]
+    results, types = [], []
+    for tree in args['used expr']:
+        while isinstance(tree, list) and len(tree) == 1:
+            tree = tree[0]
+        if isinstance(tree, str):
+            continue
+        if tree[1] == 'AS' and isinstance(tree[0], str):
+            continue
+        r = agg_evaluate(cols, tree[0] if len(tree) > 1 and tree[1] == 'AS' else tree, rows,
+                         None, None)
+        types.append(r.dtype)
+        results.append(r.to(torch.float64))
+    for i, oid in enumerate(args['materialized out']):
+        t = results[i].to(types[i])
+        tensor_group[oid] = Col(t, _type_name(types[i]))
+    return -1
+
+
+def _type_name(dt):
+    """variable.py's torch_to_type."""
+    if dt == torch.float64:
+        return 'float'
+    if dt == torch.int32:
+        return 'date'
+    if dt == torch.int64:
+        return 'int'
+    return 'string'
+
+
 # ====================================================================== executor
 JOIN_TYPE = {'Inner': 'inner', 'LeftSemi': 'right-semi', 'LeftOuter': 'right-outer',
              'LeftAnti': 'right-anti'}
@@ -563,6 +603,8 @@ def execute(plan, tensor_group, subquery_result=None, op_times=None):
             tqp_filter(tensor_group, args, subquery_result)
         elif name == 'Aggregate':
             tqp_hash_aggregate(tensor_group, args)
+        elif name == 'Project':
+            tqp_project(tensor_group, args)
         elif name == 'Sort':
             tqp_sort(tensor_group, args, False)
         elif name == 'take ordered':

@@ -705,3 +705,72 @@ def test_filter_evaluate_with_leaf_function(device):
     want = R.filter_evaluate({}, tree, m, None,
                              leaf_function=lambda cid: ccols[cid][torch.from_numpy(idx)])
     _assert_col(got, want, "leaf mask")
+
+
+# ------------------------------------------------------------------ project
+def _project_case(n, seed=82):
+    g = np.random.default_rng(seed)
+    tables = {202: ('float', g.uniform(900, 100000, n)),
+              203: ('float', np.round(g.integers(0, 11, n) / 100.0, 2)),
+              204: ('date', g.integers(0, 3652, n, dtype=np.int32)),
+              205: ('float', np.round(g.integers(0, 9, n) / 100.0, 2)),
+              206: ('int', g.integers(1, 51, n).astype(np.int64))}
+    args = {'name': 'Project', 'rename map': {202: 602},
+            'used expr': [
+                'price#202',
+                [['discount#203', 'AS', 'disc#603']],
+                [[['price#202', '*', ['1.0', '-', 'discount#203']], 'AS', 'rev#604']],
+                [[[['price#202', '*', ['1.0', '-', 'discount#203']], '*', ['1.0', '+', 'tax#205']],
+                  'AS', 'charge#605']],
+                [[['year', 'ship_day#204'], 'AS', 'y#606']],
+                [[['CASE', 'WHEN', ['q#206', '>', '25'], 'THEN', 'q#206', 'ELSE', '0', 'END'],
+                  'AS', 'bigq#607']],
+                [[['ship_day#204', '<', '1200'], 'AS', 'early#608']]],
+            'materialized in': [202, 203, 204, 205, 206],
+            'materialized out': [604, 605, 606, 607, 608]}
+    return tables, args
+
+
+def test_project_bit_exact(device):
+    """project.py:10-69 in one expression launch: products, year, CASE, a comparison; renames."""
+    from tqp_amd import executor
+    tables, args = _project_case(2_000_000)
+    gpu = _gpu_group(tables, device)
+    cpu = R.to_cols(tables)
+    print("project: 5 expressions over 2M rows", flush=True)
+    executor.execute([(1, copy.deepcopy(args))], gpu)
+    R.execute([(1, copy.deepcopy(args))], cpu)
+    assert gpu[602] is gpu[202] and 603 not in gpu
+    for k in (604, 605):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"projected {k}", fp=True)
+    for k in (606, 607, 608):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"projected {k}")
+        assert gpu[k].tensor_type == cpu[k].tensor_type
+
+
+def test_project_many_expressions_then_aggregate(device):
+    """Fourteen expressions (more than one program's outputs), then a group-by over projected
+    columns, with and without per-operator synchronisation."""
+    from tqp_amd import executor
+    tables, args = _project_case(300_000, seed=83)
+    extra = [[[['q#206', '*', str(i)], 'AS', f'm{i}#{620 + i}']] for i in range(9)]
+    args['used expr'] = args['used expr'] + extra
+    args['materialized out'] = args['materialized out'] + [620 + i for i in range(9)]
+    agg = {'name': 'Aggregate', 'input ids': [607, 604, 628], 'keys': ['bigq#607'],
+           'results': ['bigq#607',
+                       [[[['sum', ['rev#604']], '#', '50'], 'AS', 'srev#701']],
+                       [[[['max', ['m8#628']], '#', '51'], 'AS', 'mx#702']]],
+           'out agg ids': [701, 702], 'is partial': False}
+    plan = [(1, args), (2, agg)]
+    cpu = R.to_cols(tables)
+    R.execute(copy.deepcopy(plan), cpu)
+    for sync in (True, False):
+        gpu = _gpu_group(tables, device)
+        print(f"project + aggregate, sync={sync}", flush=True)
+        executor.execute(copy.deepcopy(plan), gpu, sync=sync)
+        torch.cuda.synchronize()
+        _assert_col(gpu[607].tensor, cpu[607].tensor, "group keys")
+        _assert_col(gpu[701].tensor, cpu[701].tensor, "sum of a projected column", fp=True)
+        _assert_col(gpu[702].tensor, cpu[702].tensor, "max of a projected column")
+        for i in range(9):
+            _assert_col(gpu[620 + i].tensor.cpu(), cpu[620 + i].tensor, f"m{i}")

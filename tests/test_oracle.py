@@ -435,3 +435,52 @@ def test_contains_endswith_substring_match_closed_forms():
     got = R.filter_evaluate(tg, [['substring', ['c#1', '1', '2']], '=', 'ab'], None, None).numpy()
     want = np.array([r[0:2] == b"ab" for r in raw])
     assert np.array_equal(got, want) and want.any()
+
+
+# ------------------------------------------------------------------ project
+PROJECT_ARGS = {
+    'name': 'Project',
+    'rename map': {202: 602},
+    'used expr': [
+        'price#202',                                                   # a bare column: skipped
+        [['discount#203', 'AS', 'disc#603']],                          # a renamed column: skipped
+        [[['price#202', '*', ['1.0', '-', 'discount#203']], 'AS', 'rev#604']],
+        [[[['price#202', '*', ['1.0', '-', 'discount#203']], '*', ['1.0', '+', 'tax#205']],
+          'AS', 'charge#605']],
+        [[['year', 'ship_day#204'], 'AS', 'y#606']],
+        [[['CASE', 'WHEN', ['q#206', '>', '25'], 'THEN', 'q#206', 'ELSE', '0', 'END'],
+          'AS', 'bigq#607']],
+        [[['ship_day#204', '<', '1200'], 'AS', 'early#608']],
+    ],
+    'materialized in': [202, 203, 204, 205, 206],
+    'materialized out': [604, 605, 606, 607, 608],
+}
+
+
+def project_tables(g, n):
+    return {202: ('float', g.uniform(900, 100000, n)),
+            203: ('float', np.round(g.integers(0, 11, n) / 100.0, 2)),
+            204: ('date', g.integers(0, 3652, n, dtype=np.int32)),
+            205: ('float', np.round(g.integers(0, 9, n) / 100.0, 2)),
+            206: ('int', g.integers(1, 51, n).astype(np.int64))}
+
+
+def test_project_matches_closed_forms():
+    """project.py:10-69: bare and renamed columns give no output, the rename map aliases, and
+    each evaluated expression is the per-row value (through float64 and back to its dtype)."""
+    import copy
+    g = np.random.default_rng(81)
+    t = project_tables(g, 4000)
+    tg = R.to_cols(t)
+    assert R.tqp_project(tg, copy.deepcopy(PROJECT_ARGS)) == -1
+    p, d, day, tax, q = (t[k][1] for k in (202, 203, 204, 205, 206))
+    assert tg[602] is tg[202]
+    np.testing.assert_array_equal(tg[604].tensor.numpy(), p * (1.0 - d))
+    np.testing.assert_array_equal(tg[605].tensor.numpy(), p * (1.0 - d) * (1.0 + tax))
+    assert tg[606].tensor.dtype == torch.int64
+    np.testing.assert_array_equal(tg[606].tensor.numpy(), [CF.year_of_day(x) for x in day])
+    np.testing.assert_array_equal(tg[607].tensor.numpy(), np.where(q > 25, q, 0))
+    assert tg[608].tensor.dtype == torch.bool
+    np.testing.assert_array_equal(tg[608].tensor.numpy(), day < 1200)
+    assert [tg[k].tensor_type for k in (604, 606, 608)] == ['float', 'int', 'string']
+    assert 603 not in tg

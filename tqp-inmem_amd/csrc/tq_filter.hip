@@ -145,6 +145,49 @@ __global__ __launch_bounds__(kBlock) void k_eval(tq_program_t P, tq_colset_t C, 
   }
 }
 
+// Every segment of a program per row, segment a into out[a] (one staging of the loads for all;
+// a thread's stage entries are its own, as in k_eval).
+struct OutSet {
+  void* out[TQ_MAX_SEGS];
+  int32_t bytes[TQ_MAX_SEGS];
+};
+
+__device__ __forceinline__ void store_low(void* out, int bytes, int64_t row, uint64_t v) {
+  switch (bytes) {
+    case 1: ((uint8_t*)out)[row] = (uint8_t)v; break;
+    case 2: ((uint16_t*)out)[row] = (uint16_t)v; break;
+    case 4: ((uint32_t*)out)[row] = (uint32_t)v; break;
+    default: ((uint64_t*)out)[row] = v; break;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_eval_multi(tq_program_t P, tq_colset_t C, int64_t n,
+                                                       OutSet O) {
+  extern __shared__ uint64_t stage[];
+  const int t = threadIdx.x;
+  for (int64_t sub = blockIdx.x; sub * kSubRows < n; sub += gridDim.x) {
+    const int64_t sub0 = sub * kSubRows;
+    bool live[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) live[j] = sub0 + (int64_t)j * kBlock + t < n;
+    stage_loads(P, C, sub0, live, stage);
+    int64_t rows[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) rows[j] = sub0 + (int64_t)j * kBlock + t;
+    RegsN R{};
+    int begin = 0;
+    for (int a = 0; a < P.nseg; ++a) {
+      const int end = P.seg_end[a];
+      run_rows(P, C, begin, end, rows, live, stage, R);
+      begin = end;
+      const V4 V = R.get(P.seg_result[a]);
+#pragma unroll
+      for (int j = 0; j < kSub; ++j)
+        if (live[j]) store_low(O.out[a], O.bytes[a], rows[j], V.v[j]);
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_mask_counts(const uint8_t* __restrict__ mask,
                                                         int64_t n,
                                                         uint32_t* __restrict__ tile_counts) {
@@ -424,6 +467,28 @@ extern "C" int tq_eval_column(const tq_program_t* prog, const tq_colset_t* cols,
   int64_t g = ceil_div(n, kSubRows);
   if (g > 4096) g = 4096;
   return launch("k_eval", k_eval, (unsigned)g, kBlock, stage_bytes(*prog), (hipStream_t)stream, *prog, *cols, n, out, out_bytes);
+}
+
+extern "C" int tq_eval_columns(const tq_program_t* prog, const tq_colset_t* cols, int64_t n,
+                               void* const* outs, const int32_t* out_bytes, void* stream) {
+  if (!prog || !cols || !outs || !out_bytes) return set_error(TQ_EINVAL, "eval: null argument");
+  if (prog->nseg < 1 || prog->nseg > TQ_MAX_SEGS || prog->n_instr > TQ_MAX_INSTR ||
+      prog->nload > TQ_MAX_LOADS)
+    return set_error(TQ_EINVAL, "eval: malformed program");
+  OutSet O{};
+  for (int a = 0; a < prog->nseg; ++a) {
+    const int b = out_bytes[a];
+    if (b != 1 && b != 2 && b != 4 && b != 8)
+      return set_error(TQ_EINVAL, "eval: output %d width %d", a, b);
+    if (n > 0 && !outs[a]) return set_error(TQ_EINVAL, "eval: null output %d", a);
+    O.out[a] = outs[a];
+    O.bytes[a] = b;
+  }
+  if (n <= 0) return TQ_OK;
+  int64_t g = ceil_div(n, kSubRows);
+  if (g > 4096) g = 4096;
+  return launch("k_eval_multi", k_eval_multi, (unsigned)g, kBlock, stage_bytes(*prog),
+                (hipStream_t)stream, *prog, *cols, n, O);
 }
 
 extern "C" int tq_mask_tile_counts(const uint8_t* mask, int64_t n, uint32_t* tile_counts,

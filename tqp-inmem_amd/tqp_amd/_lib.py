@@ -148,6 +148,7 @@ _SIGNATURES = {
     "tq_timing_read": ([P, I64, P, P], I32),
     "tq_filter_mask": ([P, P, I64, P, P, P], I32),
     "tq_filter_is_conjunction": ([P, P], I32),
+    "tq_eval_columns": ([P, P, I64, P, P, P], I32),
     "tq_mask_tile_counts": ([P, I64, P, P], I32),
     "tq_scan_ws_bytes": ([I64], SZ),
     "tq_scan_tiles": ([P, I64, P, P, P, P], I32),

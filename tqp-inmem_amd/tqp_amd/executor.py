@@ -16,6 +16,7 @@ from . import groupjoin, kernels, memory
 from .operators.aggregate import result_ids, results_are_direct, tqp_hash_aggregate
 from .operators.filter import tqp_filter
 from .operators.hashjoin import join_vortex
+from .operators.project import tqp_project
 from .operators.sort import tqp_sort
 
 JOIN_TYPE = {'Inner': 'inner', 'LeftSemi': 'right-semi', 'LeftOuter': 'right-outer',
@@ -287,6 +288,9 @@ def _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool, gp
             num_rows = tqp_hash_aggregate(gpu_enable, tensor_group, args,
                                           cpu_mem_pool=cpu_mem_pool, gpu_mem_pool=gpu_mem_pool,
                                           name=f"aggregation {u}")
+        elif name == 'Project':
+            num_rows = tqp_project(gpu_enable, tensor_group, args, cpu_mem_pool=cpu_mem_pool,
+                                   gpu_mem_pool=gpu_mem_pool, name=f"project {u}")
         elif name == 'Sort':
             num_rows = tqp_sort(gpu_enable, tensor_group, args, with_limit=False, name=f"sort {u}")
         elif name == 'take ordered':

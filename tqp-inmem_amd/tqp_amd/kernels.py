@@ -122,6 +122,20 @@ def gather(idx: torch.Tensor, columns, null_fill=False):
     return outs
 
 
+def eval_programs(b: E.ProgramBuilder, n: int, out_dtypes, device):
+    """Every segment of b per row into its own column (dtype out_dtypes[a]), one launch."""
+    outs = [memory.empty(n, dtype=dt, device=device) for dt in out_dtypes]
+    if n > 0:
+        if not b.col_tensors:
+            b.column(outs[0])
+        ptrs = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        widths = L.int_array([o.element_size() for o in outs])
+        rb = sum(o.element_size() for o in outs)
+        L.call("tq_eval_columns", E.program_ptr(b), E.colset_ptr(b), n, ptrs, widths,
+               L.stream_handle(), kernel={"k_eval_multi": n * (b.load_bytes() + rb)})
+    return outs
+
+
 def eval_program(b: E.ProgramBuilder, n: int, out_dtype, device):
     out = memory.empty(n, dtype=out_dtype, device=device)
     if n > 0:
