@@ -589,6 +589,38 @@ def _type_name(dt):
     return 'string'
 
 
+# ====================================================================== scan
+SCAN_TYPES = {"identifier": "int", "integer": "int", "decimal": "float", "date": "date",
+              "variable text": "string", "fixed text": "string"}
+
+
+def tqp_scan(tensor_group, args, tensors_path, schema_table, SF):
+    """operators/scan.py:8-54 on the host: the first parameter of each column's TorchScript
+    archive, typed by the schema, cast to the engine's dtypes (dates int32, floating float64,
+    text int8, else int64); the primary key marked sorted.  Returns the row count."""
+    import os
+    rows = 0
+    for name, cid in zip(args['output names'], args['output ids']):
+        name = name.upper()
+        path = os.path.join(tensors_path, f'SF{SF}-tensor-{name}.pth')
+        t = list(torch.jit.load(path, map_location='cpu').parameters())[0].detach()
+        ty = SCAN_TYPES[schema_table[name].split(',')[0]]
+        rows = t.shape[0]
+        if ty == 'date':
+            t = t.to(torch.int32)
+        elif torch.is_floating_point(t):
+            t = t.to(torch.float64)
+        elif ty == 'string':
+            t = t.to(torch.int8)
+        else:
+            assert t.dim() == 1
+            t = t.to(torch.int64)
+        col = Col(t, ty)
+        col.is_sorted = name == schema_table['PRIMARY_KEY']
+        tensor_group[cid] = col
+    return rows
+
+
 # ====================================================================== executor
 JOIN_TYPE = {'Inner': 'inner', 'LeftSemi': 'right-semi', 'LeftOuter': 'right-outer',
              'LeftAnti': 'right-anti'}

@@ -93,6 +93,25 @@ def test_tensors_are_views_of_the_slab():
     assert torch.all(a.slab[:240].view(torch.float64) == 1.5)
 
 
+def test_noted_bounds_survive_writes_elsewhere_in_the_slab():
+    """Each arena tensor has its own version counter: bounds noted on it stay known while other
+    tensors of the slab are written, and are dropped when it is written itself."""
+    from tqp_amd import kernels
+    a = memory.DeviceArena(1 << 16, device='cpu')
+    c = memory.DeviceArena(1 << 14, device='cpu', slab=a.slab[4096:4096 + (1 << 14)])
+    for arena in (a, c):
+        t = arena.empty(100, torch.int64)
+        t.copy_(torch.arange(100))
+        kernels.note_bounds(t, 0, 99)
+        u = arena.empty(50, torch.int64)
+        u.fill_(7)
+        arena.slab[-8:].zero_()
+        assert kernels.known_bounds(t) == (0, 99)
+        assert t.data_ptr() == arena.slab.data_ptr() and torch.equal(t, torch.arange(100))
+        t[0] = 5
+        assert kernels.known_bounds(t) is None
+
+
 def test_child_reservations_and_release():
     root = memory.DeviceArena(10_000, device='cpu')
     c1 = root.reserve(3000)

@@ -17,6 +17,7 @@ from .operators.aggregate import result_ids, results_are_direct, tqp_hash_aggreg
 from .operators.filter import tqp_filter
 from .operators.hashjoin import join_vortex
 from .operators.project import tqp_project
+from .operators.scan import tqp_scan
 from .operators.sort import tqp_sort
 
 JOIN_TYPE = {'Inner': 'inner', 'LeftSemi': 'right-semi', 'LeftOuter': 'right-outer',
@@ -244,7 +245,8 @@ def _cut_to_counts(tensor_group, before, stages, df, m, g):
 
 
 def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem_pool=None,
-            gpu_enable=True, op_times=None, sync=True, fuse=True):
+            gpu_enable=True, op_times=None, sync=True, fuse=True, config_json=None, schema=None,
+            scale_factor=None):
     """Run `plan` (list of (u, args)) over `tensor_group` in place; returns the last row count.
 
     Without per-operator synchronisation (sync=False), a Join whose pairs only feed the next
@@ -253,15 +255,16 @@ def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem
     pairs are never written (the join's output columns are computed if something reads them).
     The results are the same.  With a `GPUMemory` as `gpu_mem_pool`, operator outputs and
     workspaces are placed in its slab; otherwise in the thread's active device arena, if any,
-    else in torch's allocator."""
+    else in torch's allocator.  Scan nodes read their tables' column files as main.py:138-149
+    does, with config_json, schema (the tables' schemas by name) and scale_factor."""
     arena = getattr(gpu_mem_pool, 'arena', None)
     with memory.use(arena if arena is not None else memory.active()):
         return _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool,
-                        gpu_enable, op_times, sync, fuse)
+                        gpu_enable, op_times, sync, fuse, (config_json, schema, scale_factor))
 
 
 def _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool, gpu_enable,
-             op_times, sync, fuse):
+             op_times, sync, fuse, scan_ctx):
     num_rows = None
     i = 0
     while i < len(plan):
@@ -278,7 +281,12 @@ def _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool, gp
             continue
         i += 1
         t0 = time.perf_counter()
-        if name == 'Filter':
+        if name == 'Scan':
+            config_json, schema, sf = scan_ctx
+            num_rows = tqp_scan(gpu_enable, tensor_group, args, config_json,
+                                schema[args['table name']], SF=sf, mem_pool=gpu_mem_pool,
+                                name=f"scan {u}")
+        elif name == 'Filter':
             # unsynchronised: the mask is computed when first read, or never when a fused
             # consumer evaluates the predicate in its own kernels
             num_rows = tqp_filter(gpu_enable, tensor_group, args, subquery_result,

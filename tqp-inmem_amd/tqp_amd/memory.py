@@ -127,7 +127,13 @@ class DeviceArena:
             return list(zip(self._starts, self._lens))
 
     def view(self, start: int, nbytes: int) -> torch.Tensor:
-        return self.slab[start:start + int(nbytes)]
+        """Bytes [start, start + nbytes) of the slab as a tensor of its own: set_ on the slab's
+        storage rather than a slice, so the tensor has its own version counter and a write
+        elsewhere in the slab does not look like a write to it (column statistics noted on a
+        tensor stay valid until the tensor itself is written)."""
+        t = torch.empty(0, dtype=torch.uint8, device=self.slab.device)
+        return t.set_(self.slab.untyped_storage(), self.slab.storage_offset() + int(start),
+                      (int(nbytes),))
 
     # ---------------------------------------------------------------- tensors
     def empty(self, shape, dtype) -> torch.Tensor:
