@@ -9,6 +9,7 @@ achieved HBM bandwidth from HIP events over the timed region (the dominant kerne
 host's cores, on a bounded sample of the same workload).
 """
 import argparse
+import collections
 import copy
 import json
 import os
@@ -38,6 +39,9 @@ def parse():
                     help="skip the config-0 and config-2 plain-form lines")
     ap.add_argument("--op-sync", action="store_true",
                     help="synchronise after every operator, as the reference executor does")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="queries queued on the stream at once (1: each completes before the "
+                         "next is prepared)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once by a plain copy kernel (PMC traffic calibration)")
     ap.add_argument("--only", type=int, choices=(0, 2, 3), default=None,
@@ -288,30 +292,46 @@ def main():
     def fresh_plans(k):
         copies[:] = [copy.deepcopy(plan) for _ in range(k)]
 
-    def step(op_sync=args.op_sync):
-        tg = dict(base)
-        executor.execute(copies.pop() if copies else copy.deepcopy(plan), tg, sync=op_sync)
-        return tg
+    def run(k, op_sync=args.op_sync, inflight=args.inflight):
+        """k queries; with inflight > 1 the next query's kernels are queued before the previous
+        one's counts are collected (the host prepares query i+1 while the device runs query i).
+        Every query runs whole: its results are settled when the loop ends."""
+        pend = collections.deque()
+        for _ in range(k):
+            tg = dict(base)
+            q = executor.execute(copies.pop() if copies else copy.deepcopy(plan), tg,
+                                 sync=op_sync, wait=op_sync or inflight <= 1)
+            if isinstance(q, executor.PendingQuery):
+                pend.append(q)
+                while len(pend) >= inflight:
+                    pend.popleft().result()
+        while pend:
+            pend.popleft().result()
 
-    for _ in range(args.warmup):
-        step()
+    run(args.warmup)
+    run(args.warmup, inflight=1)
     torch.cuda.synchronize()
     # timed region: the whole pipeline, kernels untimed
     fresh_plans(args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    run(args.steps)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # the same queries one at a time: each completes before the next is prepared
+    fresh_plans(args.steps)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    run(args.steps, inflight=1)
+    torch.cuda.synchronize()
+    dt_seq = time.perf_counter() - t3
     # the same steps again with every kernel launch carrying device-stamped start/stop events
     # (the kernel durations and achieved bandwidths), then with the reference executor's
     # per-operator synchronisation (main.py:273-274), for comparison
     fresh_plans(args.steps)
     _lib.start_trace()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    run(args.steps)
     torch.cuda.synchronize()
     dt_traced = time.perf_counter() - t1
     calls, launches = _lib.stop_trace()
@@ -329,8 +349,7 @@ def main():
     fresh_plans(args.steps)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    for _ in range(args.steps):
-        step(True)
+    run(args.steps, op_sync=True)
     torch.cuda.synchronize()
     side = {"kernel_timed": round(dt_traced / args.steps * 1e3, 4),
             "op_sync": round((time.perf_counter() - t2) / args.steps * 1e3, 4)}
@@ -359,7 +378,10 @@ def main():
         "data": "synthetic (seeded PCG64 tables of the config-1 shape)",
         "config": {"workload": "filter -> sort-based equi-join -> group-by sum -> top-10; build "
                                "1,500,000 rows, probe 6,000,000 rows, ~264k groups",
-                   "input_rows": rows, "form": "plain per-operator launches",
+                   "input_rows": rows,
+                   "form": "plain form (one query after another, no batching); the two filters "
+                           "evaluated inside the fused join + group-by (groupjoin) kernels",
+                   "inflight": 1 if args.op_sync else args.inflight,
                    "op_sync": bool(args.op_sync)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d.get("GBps"),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d.get("frac"),
@@ -367,6 +389,7 @@ def main():
                      "alg_bytes_per_launch": d.get("alg_bytes_per_launch"),
                      "avg_us": d["avg_us"]},
         "join_aggregate": ja,
+        "ms_per_step_sequential": round(dt_seq / args.steps * 1e3, 4),
         "ms_per_step_kernel_timed": side["kernel_timed"],
         "ms_per_step_op_sync": side["op_sync"],
         "kernels": kernels,

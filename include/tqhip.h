@@ -267,6 +267,18 @@ int tq_rank_sorted(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n
 int tq_read_words(int32_t n, const void* const* dev /* host */, const size_t* bytes /* host */,
                   int64_t* host_out /* host out */, void* stream);
 
+/* The same read in two halves, so that the host can queue the next query's work before it
+ * collects this one's counts.  Replaces the same reads (e.g. operators/hashjoin.py:76), which the
+ * reference makes synchronously.  Each thread has, per device, tq_word_slots() pinned host
+ * slots of 64 words.  tq_words_post gathers the n runs (whole aligned 8-byte words, 64 in all
+ * at most) into `slot` with one launch on `stream` and records the slot's event after it;
+ * tq_words_wait blocks on that event and copies the slot's words to host_out.  A slot is posted
+ * again only after its wait (TQ_EINVAL otherwise). */
+int32_t tq_word_slots(void);
+int tq_words_post(int32_t n, const void* const* dev /* host */, const size_t* bytes /* host */,
+                  int32_t slot, void* stream);
+int tq_words_wait(int32_t slot, int64_t* host_out /* host out, 64 words */);
+
 /* ---------------------------------------------------------------- equi-join */
 /* Host-side handle of a built join index (filled by tq_join_build, read by the probe passes).
  * Keys are order-transformed (sign bit flipped; 32-bit for build keys of up to 32 bits, else

@@ -898,6 +898,86 @@ extern "C" int tq_read_words(int32_t n, const void* const* dev, const size_t* by
   return read_back_n(n, dev, bytes, host_out, (hipStream_t)stream);
 }
 
+// Reads the host collects later.  Per thread and device, a ring of TQ_WORD_SLOTS pinned,
+// coherent host slots of 64 words: a post gathers the words into a slot with one
+// k_collect_words launch and records the slot's event after it on the stream; a wait blocks on
+// that event and copies the slot out.  The host queues further work on the stream in between.
+constexpr int kWordSlots = 16;
+constexpr int kRingDevices = 16;
+struct WordRing {
+  int64_t* host = nullptr;
+  uint64_t* dev = nullptr;
+  hipEvent_t ev[kWordSlots] = {};
+  size_t bytes[kWordSlots] = {};
+  bool pending[kWordSlots] = {};
+};
+static thread_local WordRing rings[kRingDevices];
+
+static int word_ring(WordRing** out) {
+  int d = 0;
+  TQ_TRY_HIP(hipGetDevice(&d));
+  if (d < 0 || d >= kRingDevices) return set_error(TQ_EUNSUPPORTED, "device %d has no word ring", d);
+  WordRing& r = rings[d];
+  if (!r.host) {
+    int64_t* h = nullptr;
+    TQ_TRY_HIP(hipHostMalloc((void**)&h, kWordSlots * 64 * sizeof(int64_t), hipHostMallocCoherent));
+    uint64_t* dp = nullptr;
+    if (hipHostGetDevicePointer((void**)&dp, h, 0) != hipSuccess || !dp) {
+      (void)hipHostFree(h);
+      return set_error(TQ_EHIP, "word ring: no device view of the host slots");
+    }
+    for (int i = 0; i < kWordSlots; ++i) {
+      if (!r.ev[i]) TQ_TRY_HIP(hipEventCreateWithFlags(&r.ev[i], hipEventDisableTiming));
+    }
+    r.dev = dp;
+    r.host = h;
+  }
+  *out = &r;
+  return TQ_OK;
+}
+
+extern "C" int32_t tq_word_slots(void) { return kWordSlots; }
+
+extern "C" int tq_words_post(int32_t n, const void* const* dev, const size_t* bytes, int32_t slot,
+                             void* stream) {
+  if (n < 1 || n > kMaxWordRuns || !dev || !bytes)
+    return set_error(TQ_EINVAL, "words post: %d runs (1..%d)", n, kMaxWordRuns);
+  if (slot < 0 || slot >= kWordSlots) return set_error(TQ_EINVAL, "words post: slot %d", slot);
+  WordRing* r = nullptr;
+  TQ_TRY(word_ring(&r));
+  if (r->pending[slot]) return set_error(TQ_EINVAL, "words post: slot %d is still pending", slot);
+  WordRuns W{};
+  W.n = n;
+  int32_t o = 0;
+  for (int i = 0; i < n; ++i) {
+    if (bytes[i] % 8 != 0 || ((uintptr_t)dev[i] & 7) != 0)
+      return set_error(TQ_EINVAL, "words post: run %d is not whole aligned words", i);
+    W.src[i] = (const uint64_t*)dev[i];
+    W.nw[i] = (int32_t)(bytes[i] / 8);
+    W.off[i] = o;
+    o += W.nw[i];
+    if (o > 64) return set_error(TQ_EINVAL, "words post: more than 64 words");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  TQ_TRY(launch("k_collect_words", k_collect_words, 1, 64, 0, s, W, r->dev + (size_t)slot * 64));
+  TQ_TRY_HIP(hipEventRecord(r->ev[slot], s));
+  r->bytes[slot] = (size_t)o * 8;
+  r->pending[slot] = true;
+  return TQ_OK;
+}
+
+extern "C" int tq_words_wait(int32_t slot, int64_t* host_out) {
+  if (slot < 0 || slot >= kWordSlots || !host_out)
+    return set_error(TQ_EINVAL, "words wait: slot %d", slot);
+  WordRing* r = nullptr;
+  TQ_TRY(word_ring(&r));
+  if (!r->pending[slot]) return set_error(TQ_EINVAL, "words wait: slot %d was not posted", slot);
+  r->pending[slot] = false;
+  TQ_TRY_HIP(hipEventSynchronize(r->ev[slot]));
+  memcpy(host_out, r->host + (size_t)slot * 64, r->bytes[slot]);
+  return TQ_OK;
+}
+
 // Top-k for a limit of at most 256 over keys that pack into one 64-bit sort key, with no host
 // synchronisation: rounds of per-workgroup threshold selection for a limit of at most 64, else
 // packing and rounds of per-tile radix selection.

@@ -179,6 +179,9 @@ _SIGNATURES = {
                                I32),
     "tq_join_settle": ([P, P, P], I32),
     "tq_read_words": ([I32, P, P, P, P], I32),
+    "tq_word_slots": ([], I32),
+    "tq_words_post": ([I32, P, P, I32, P], I32),
+    "tq_words_wait": ([I32, P], I32),
     "tq_rank_ws_bytes": ([I64], SZ),
     "tq_rank_sorted": ([P, I32, I64, P, P, P, P, P, SZ, P], I32),
     "tq_groupjoin_pres_bytes": ([I64], SZ),

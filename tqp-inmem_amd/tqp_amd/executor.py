@@ -13,7 +13,8 @@ import time
 import torch
 
 from . import groupjoin, kernels, memory
-from .operators.aggregate import result_ids, results_are_direct, tqp_hash_aggregate
+from .operators.aggregate import (forget_parses, result_ids, results_are_direct,
+                                  tqp_hash_aggregate)
 from .operators.filter import tqp_filter
 from .operators.hashjoin import join_vortex
 from .operators.project import tqp_project
@@ -139,12 +140,14 @@ def _run_deferred(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu_
 
 
 def _run_groupjoin(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu_enable,
-                   op_times):
+                   op_times, defer=False):
     """plan[i:i + count] (Join, Aggregate[, take ordered]) with the join and the aggregate fused
     (groupjoin.py) when the aggregate groups by the join key alone; returns None, with nothing
     changed, when they cannot be fused.  The counts are read back once at the end; a void
     fused result (repeated or out-of-bounds build keys, no pairs, a look-back that gave up)
-    reruns the nodes per operator from the state before the join."""
+    reruns the nodes per operator from the state before the join.  With `defer`, the kernels
+    and the read of the counts are queued and a `_Finish` is returned, whose call collects the
+    counts, settles the outputs and returns the row count."""
     (uj, ja), (ua, aa) = plan[i], plan[i + 1]
     p = groupjoin.plan(tensor_group, ja, aa)
     if p is None:
@@ -163,57 +166,98 @@ def _run_groupjoin(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool, gpu
         return None
     times = [time.perf_counter() - t0]
     after_agg = dict(tensor_group)
-    ok = True
-    rows_sorted = None
+    state = {"ok": True, "rows_sorted": None}
     try:
         if count == 3:
             us, sa = plan[i + 2]
             t1 = time.perf_counter()
             kernels.set_deferred(df)
             try:
-                rows_sorted = tqp_sort(gpu_enable, tensor_group, sa, with_limit=True,
-                                       name=f"sort lim {us}")
+                state["rows_sorted"] = tqp_sort(gpu_enable, tensor_group, sa, with_limit=True,
+                                                name=f"sort lim {us}")
             finally:
                 kernels.set_deferred(None)
             times.append(time.perf_counter() - t1)
-        t2 = time.perf_counter()
-        m, g = df.resolve()
-        times[-1] += time.perf_counter() - t2
+        df.post()
     except kernels.DeferredJoinFailed:
-        ok = False
+        state["ok"] = False
     nodes = plan[i:i + count]
-    if ok:
-        for v in df.lazy:
-            v.rows = m
-        for k, v in tensor_group.items():
-            if after_agg.get(k) is v and k in df.written:
-                v.tensor = v.tensor[:g]                       # written by the aggregate
-            elif count == 3 and saved_group.get(k) is not v and after_agg.get(k) is not v \
-                    and df.sorted is not None and v.tensor.shape[0] == df.sorted["cap"]:
-                v.tensor = v.tensor[:min(g, df.sorted["limit"])]   # written by the top-k
-        if count == 3:
-            num_rows = g if rows_sorted == df.group["cap"] else rows_sorted
-        else:
-            first = next(iter(tensor_group.values()))
-            num_rows = first.rows if isinstance(first, groupjoin.LazyJoinColumn) \
-                and not first.materialized else first.tensor.size(0)
-        if op_times is not None:
-            op_times.setdefault(f"Join+Aggregate {uj}+{ua}", []).append(times[0])
+
+    def finish():
+        ok = state["ok"]
+        if ok:
+            t2 = time.perf_counter()
+            try:
+                m, g = df.resolve()
+            except kernels.DeferredJoinFailed:
+                ok = False
+            times[-1] += time.perf_counter() - t2
+        if ok:
+            for v in df.lazy:
+                v.rows = m
+            for k, v in tensor_group.items():
+                if after_agg.get(k) is v and k in df.written:
+                    v.tensor = v.tensor[:g]                       # written by the aggregate
+                elif count == 3 and saved_group.get(k) is not v and after_agg.get(k) is not v \
+                        and df.sorted is not None and v.tensor.shape[0] == df.sorted["cap"]:
+                    v.tensor = v.tensor[:min(g, df.sorted["limit"])]   # written by the top-k
             if count == 3:
-                op_times.setdefault(f"take ordered {plan[i + 2][0]}", []).append(times[1])
+                rows_sorted = state["rows_sorted"]
+                num_rows = g if rows_sorted == df.group["cap"] else rows_sorted
+            else:
+                first = next(iter(tensor_group.values()))
+                num_rows = first.rows if isinstance(first, groupjoin.LazyJoinColumn) \
+                    and not first.materialized else first.tensor.size(0)
+            if op_times is not None:
+                op_times.setdefault(f"Join+Aggregate {uj}+{ua}", []).append(times[0])
+                if count == 3:
+                    op_times.setdefault(f"take ordered {plan[i + 2][0]}", []).append(times[1])
+            return num_rows
+        tensor_group.clear()
+        tensor_group.update(saved_group)
+        if saved_mask_map is not None:
+            ja['mask map'] = saved_mask_map
+        aa['input ids'][:] = saved_ids
+        num_rows = None
+        for u, args in nodes:
+            t0 = time.perf_counter()
+            num_rows = _run_node(tensor_group, u, args, cpu_mem_pool, gpu_mem_pool, gpu_enable)
+            if op_times is not None:
+                op_times.setdefault(f"{args['name']} {u}", []).append(time.perf_counter() - t0)
         return num_rows
-    tensor_group.clear()
-    tensor_group.update(saved_group)
-    if saved_mask_map is not None:
-        ja['mask map'] = saved_mask_map
-    aa['input ids'][:] = saved_ids
-    num_rows = None
-    for u, args in nodes:
-        t0 = time.perf_counter()
-        num_rows = _run_node(tensor_group, u, args, cpu_mem_pool, gpu_mem_pool, gpu_enable)
-        if op_times is not None:
-            op_times.setdefault(f"{args['name']} {u}", []).append(time.perf_counter() - t0)
-    return num_rows
+
+    return _Finish(finish) if defer else finish()
+
+
+class _Finish:
+    """The rest of a fused group whose kernels are queued: called once, it returns the row
+    count."""
+
+    def __init__(self, fn):
+        self.fn = fn
+
+
+class PendingQuery:
+    """A plan whose last kernels are queued on the stream (`execute(..., wait=False)`).
+    `result()` collects their counts, settles `tensor_group` and returns the row count that
+    `execute` returns; the tensor group is not to be read before it.  Work queued on the same
+    stream afterwards (the next query's) runs after this query's kernels."""
+
+    def __init__(self, finish, arena):
+        self._finish = finish
+        self._arena = arena
+        self._rows = None
+        self._done = False
+
+    def done(self):
+        return self._done
+
+    def result(self):
+        if not self._done:
+            with memory.use(self._arena):
+                self._rows = self._finish.fn()
+            self._done = True
+        return self._rows
 
 
 def _cut_to_counts(tensor_group, before, stages, df, m, g):
@@ -246,7 +290,7 @@ def _cut_to_counts(tensor_group, before, stages, df, m, g):
 
 def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem_pool=None,
             gpu_enable=True, op_times=None, sync=True, fuse=True, config_json=None, schema=None,
-            scale_factor=None):
+            scale_factor=None, wait=True):
     """Run `plan` (list of (u, args)) over `tensor_group` in place; returns the last row count.
 
     Without per-operator synchronisation (sync=False), a Join whose pairs only feed the next
@@ -256,15 +300,30 @@ def execute(plan, tensor_group, subquery_result=None, cpu_mem_pool=None, gpu_mem
     The results are the same.  With a `GPUMemory` as `gpu_mem_pool`, operator outputs and
     workspaces are placed in its slab; otherwise in the thread's active device arena, if any,
     else in torch's allocator.  Scan nodes read their tables' column files as main.py:138-149
-    does, with config_json, schema (the tables' schemas by name) and scale_factor."""
+    does, with config_json, schema (the tables' schemas by name) and scale_factor.
+
+    With wait=False (and sync=False) the call returns a `PendingQuery` as soon as the plan's
+    kernels are queued, when the plan ends with a fused join and aggregate (and top-k): the
+    host prepares the next query while the device runs this one, and `result()` returns the
+    row count.  Other plans complete inside the call, and their PendingQuery is already done."""
+    forget_parses()
     arena = getattr(gpu_mem_pool, 'arena', None)
-    with memory.use(arena if arena is not None else memory.active()):
-        return _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool,
-                        gpu_enable, op_times, sync, fuse, (config_json, schema, scale_factor))
+    arena = arena if arena is not None else memory.active()
+    with memory.use(arena):
+        out = _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool,
+                       gpu_enable, op_times, sync, fuse, (config_json, schema, scale_factor),
+                       defer=not wait and not sync)
+    if isinstance(out, _Finish):
+        return PendingQuery(out, arena)
+    if wait:
+        return out
+    q = PendingQuery(None, arena)
+    q._rows, q._done = out, True
+    return q
 
 
 def _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool, gpu_enable,
-             op_times, sync, fuse, scan_ctx):
+             op_times, sync, fuse, scan_ctx, defer=False):
     num_rows = None
     i = 0
     while i < len(plan):
@@ -272,8 +331,11 @@ def _execute(plan, tensor_group, subquery_result, cpu_mem_pool, gpu_mem_pool, gp
         name = args['name']
         count = _deferrable(plan, i) if not sync and gpu_enable else 0
         if count:
+            last = defer and i + count == len(plan)
             num_rows = _run_groupjoin(plan, i, count, tensor_group, cpu_mem_pool, gpu_mem_pool,
-                                      gpu_enable, op_times) if fuse else None
+                                      gpu_enable, op_times, defer=last) if fuse else None
+            if isinstance(num_rows, _Finish):
+                return num_rows
             if num_rows is None:
                 num_rows = _run_deferred(plan, i, count, tensor_group, cpu_mem_pool,
                                          gpu_mem_pool, gpu_enable, op_times)

@@ -197,16 +197,23 @@ class GroupJoinDeferred(kernels.Deferred):
         self.group = group
         self.trace_at = trace_at
         self.bytes_fn = bytes_fn
+        self.ticket = None
 
     def resolve_join(self):
         return self.resolve()[0]
 
+    def post(self):
+        """Queue the read of the counts behind the work on the stream (the top-k's included);
+        `resolve` collects them."""
+        if self.ticket is None and self.G is None:
+            self.ticket = kernels.post_words([self.words], [40], 80)
+
     def resolve(self):
         if self.G is not None:
             return self.m, self.G
-        host = (ctypes.c_int64 * 8)()
-        L.call("tq_read_words", 1, (ctypes.c_void_p * 1)(self.words), (ctypes.c_size_t * 1)(40),
-               host, L.stream_handle(), kernel={"k_collect_words": 80})
+        self.post()
+        ticket, self.ticket = self.ticket, None
+        host = kernels.wait_words(ticket)
         G, m, nsel, nset, err = (int(host[i]) for i in range(5))
         if err or nsel != nset or m <= 0:
             raise kernels.DeferredJoinFailed()

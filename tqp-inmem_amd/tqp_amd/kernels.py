@@ -356,6 +356,54 @@ class JoinIndex:
 
 
 # ---------------------------------------------------------------- deferred row counts
+class _WordSlots(threading.local):
+    """The free word slots (tq_words_post) of this thread, per device."""
+
+    def __init__(self):
+        self.free = {}
+
+
+_SLOTS = _WordSlots()
+
+
+def post_words(ptrs, sizes, kernel_bytes=None):
+    """Queue a read of small device words behind the work already on the current stream
+    (tq_words_post); returns the ticket that `wait_words` collects."""
+    dev = torch.cuda.current_device()
+    free = _SLOTS.free.get(dev)
+    if free is None:
+        free = _SLOTS.free[dev] = list(range(L.load().tq_word_slots() - 1, -1, -1))
+    if not free:
+        raise RuntimeError("more word reads in flight on this thread than it has slots "
+                           f"({L.load().tq_word_slots()})")
+    slot = free.pop()
+    n = len(ptrs)
+    try:
+        L.call("tq_words_post", n, (ctypes.c_void_p * n)(*ptrs), (ctypes.c_size_t * n)(*sizes),
+               slot, L.stream_handle(),
+               kernel={"k_collect_words": kernel_bytes if kernel_bytes is not None
+                       else 2 * sum(sizes)})
+    except BaseException:
+        free.append(slot)
+        raise
+    return dev, slot
+
+
+def wait_words(ticket):
+    """The words of a `post_words` ticket (a ctypes array of 64 int64), once they are in."""
+    dev, slot = ticket
+    host = (ctypes.c_int64 * 64)()
+    try:
+        if dev == torch.cuda.current_device():
+            L.call("tq_words_wait", slot, host)
+        else:
+            with torch.cuda.device(dev):
+                L.call("tq_words_wait", slot, host)
+    finally:
+        _SLOTS.free[dev].append(slot)
+    return host
+
+
 class DeferredJoinFailed(Exception):
     """A deferred join's build keys were not unique (or left the bounds its build was given):
     its pairs are void and the operators run again without deferral."""

@@ -101,13 +101,26 @@ def _zero_row_aggregates(keys, calls, gb):
 _PARSED = {}
 
 
+_PARSED_BY_ID = {}   # id(result list) -> (the list, its parse): one query's repeated lookups
+
+
+def forget_parses():
+    """Drop the by-identity memo (the executor calls this as a query starts, so a result list
+    edited in place between queries is parsed again)."""
+    _PARSED_BY_ID.clear()
+
+
 def _parse_results(results):
     """(calls, plans) of a result list: the aggregate call nodes in evaluation order, and per
     result ('key', key id) or ('agg', tree).  The returned trees are a cached parse, shared by
     every call with equal text."""
+    ent = _PARSED_BY_ID.get(id(results))
+    if ent is not None and ent[0] is results and ent[2] == len(results):
+        return ent[1]
     key = repr(results)
     hit = _PARSED.get(key)
     if hit is not None:
+        _remember(results, hit)
         return hit
     calls = []
     plans = []
@@ -126,7 +139,16 @@ def _parse_results(results):
     if len(_PARSED) > 256:
         _PARSED.clear()
     _PARSED[key] = (calls, plans)
+    _remember(results, (calls, plans))
     return calls, plans
+
+
+def _remember(results, parse):
+    # the list is held with its parse (so its id is not reused) and its length checked on a
+    # hit; the executor reads a result list without editing it
+    if len(_PARSED_BY_ID) >= 64:
+        _PARSED_BY_ID.clear()
+    _PARSED_BY_ID[id(results)] = (results, parse, len(results))
 
 
 def results_are_direct(results):
