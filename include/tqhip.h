@@ -247,6 +247,18 @@ int tq_sort_perm_deferred(const tq_sortkey_t* keys /* host */, int32_t nkeys, in
                           const int64_t* n_dev, int64_t limit, int64_t* perm_out, void* ws,
                           size_t ws_bytes, int32_t* applied /* host out */, void* stream);
 
+/* A take-ordered in one call: the top-k of tq_sort_perm_deferred (n_dev may be null: all n
+ * rows) and, in its last round's kernel, the permuted rows of ncols columns (row_bytes[c] bytes
+ * per row) into dst[c][0 .. min(limit, n)), entries past the candidates taking row 0.  Applies
+ * (*applied = 1) for a limit of 1..64 over keys that pack into 64 bits; otherwise nothing is
+ * launched.  Replaces operators/sort.py:20-54 (the limit's perm[:limit] and the per-column
+ * gathers) for the small limits of a take-ordered. */
+int tq_topk_gather(const tq_sortkey_t* keys /* host */, int32_t nkeys, int64_t n,
+                   const int64_t* n_dev, int64_t limit, int32_t ncols,
+                   const void* const* src /* host */, const int32_t* row_bytes /* host */,
+                   void* const* dst /* host */, int64_t* perm_out, void* ws, size_t ws_bytes,
+                   int32_t* applied /* host out */, void* stream);
+
 /* Dense ranks of rows in a sort order: perm (a permutation of 0..n-1, e.g. tq_sort_perm's by
  * the same keys) lists the rows sorted; rank_out[row] receives the number of distinct keys
  * before the row's key in that order (equal keys share a rank: floats as the sort compares

@@ -600,18 +600,25 @@ __device__ __forceinline__ void field_keys(const FieldPack& fp, int64_t base, in
   }
 }
 
+// Row r of every column of gp into row i of its output (the last round's gather).
+__device__ __forceinline__ void gather_row(const ColPack& gp, int64_t i, int64_t r) {
+  for (int c = 0; c < gp.n; ++c) copy_row(gp.dst[c], i, gp.src[c], r, gp.bytes[c]);
+}
+
 // One round.  Source: the key fields of rows 0..m-1 (kin == null) or m (key, row) pairs; with
 // n_dev (key fields only) the rows are the first min(m, *n_dev), a count a producer earlier on
 // the stream left in device memory.  Writes the workgroup's k smallest in order to
 // kout/vout[blockIdx.x*k ..] (sentinels when it has fewer), or, in the last round (perm_out !=
-// null, one workgroup), the rows of the first nout as the permutation; permutation entries past
-// the candidates (only when n_dev cut the rows below nout) get row 0.
+// null, one workgroup), the rows of the first nout as the permutation, and those rows of the
+// gp columns into row 0..nout-1 of their outputs; permutation entries past the candidates
+// (only when n_dev cut the rows below nout) get row 0.
 __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint64_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin, int64_t m,
                                                        const int64_t* __restrict__ n_dev,
                                                        int k, uint64_t* __restrict__ kout,
                                                        uint32_t* __restrict__ vout,
-                                                       int64_t* __restrict__ perm_out, int64_t nout) {
+                                                       int64_t* __restrict__ perm_out, int64_t nout,
+                                                       ColPack gp) {
   // (key, row) pairs as 16-byte LDS entries: one broadcast read per pair in the counting loops
   __shared__ ulonglong2 mins[kBlock];
   __shared__ ulonglong2 surv[kSurvMax];
@@ -627,7 +634,10 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
     // no candidate in this workgroup (a grid sized for a capacity): only the sentinels, or
     // the zero permutation, that the full pass would write with no survivors
     if (perm_out) {
-      for (int64_t i = t; i < nout; i += kBlock) perm_out[i] = 0;
+      for (int64_t i = t; i < nout; i += kBlock) {
+        perm_out[i] = 0;
+        gather_row(gp, i, 0);
+      }
     } else {
       for (int i = t; i < k; i += kBlock) {
         kout[(int64_t)blockIdx.x * k + i] = ~0ull;
@@ -747,7 +757,10 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
       rank += pair_less(e.x, (uint32_t)e.y, xk, xr) ? 1 : 0;
     }
     if (perm_out) {
-      if (rank < nout) perm_out[rank] = xr;
+      if (rank < nout) {
+        perm_out[rank] = xr;
+        gather_row(gp, rank, xr);
+      }
     } else if (rank < k) {
       kout[(int64_t)blockIdx.x * k + rank] = xk;
       vout[(int64_t)blockIdx.x * k + rank] = xr;
@@ -759,7 +772,10 @@ __global__ __launch_bounds__(kBlock) void k_topk_block(FieldPack fp, const uint6
       vout[(int64_t)blockIdx.x * k + i] = 0xFFFFFFFFu;
     }
   } else {
-    for (int64_t i = s + t; i < nout; i += kBlock) perm_out[i] = 0;
+    for (int64_t i = s + t; i < nout; i += kBlock) {
+      perm_out[i] = 0;
+      gather_row(gp, i, 0);
+    }
   }
 }
 
@@ -983,8 +999,10 @@ extern "C" int tq_words_wait(int32_t slot, int64_t* host_out) {
 // packing and rounds of per-tile radix selection.
 static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t limit,
                      int64_t* perm_out, void* ws, size_t ws_bytes, hipStream_t s, bool* done,
-                     const int64_t* n_dev = nullptr) {
+                     const int64_t* n_dev = nullptr, const ColPack* gather = nullptr) {
   *done = false;
+  ColPack gp{};
+  if (gather) gp = *gather;
   FieldPack fp{};
   int bits = 0;
   for (int k = 0; k < nkeys; ++k) {
@@ -1016,7 +1034,7 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
       const bool last = blocks == 1;
       TQ_TRY(launch("k_topk_block", k_topk_block, (unsigned)blocks, kBlock, 0, s, fp, ki, vi, m,
                     ki ? (const int64_t*)nullptr : n_dev, k, ko, vo,
-                    last ? perm_out : (int64_t*)nullptr, out));
+                    last ? perm_out : (int64_t*)nullptr, out, gp));
       if (last) break;
       m = blocks * k;
       ki = ko;
@@ -1027,7 +1045,8 @@ static int topk_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t
     *done = true;
     return TQ_OK;
   }
-  if (n_dev) return TQ_OK;  // a device row count is read by the small-limit rounds only
+  // a device row count is read, and columns gathered, by the small-limit rounds only
+  if (n_dev || gp.n) return TQ_OK;
   TQ_TRY(launch("k_pack_keys", k_pack_keys<uint64_t>, stream_grid(n), kBlock, 0, s, fp, nullptr, n, w.k0, w.v0));
   uint64_t *ki = w.k0, *ko = w.k1;
   uint32_t *vi = nullptr, *vo = w.v1;
@@ -1059,6 +1078,33 @@ extern "C" int tq_sort_perm_deferred(const tq_sortkey_t* keys, int32_t nkeys, in
   bool done = false;
   TQ_TRY(topk_perm(keys, nkeys, n, limit, perm_out, ws, ws_bytes, (hipStream_t)stream, &done,
                    n_dev));
+  *applied = done ? 1 : 0;
+  return TQ_OK;
+}
+
+extern "C" int tq_topk_gather(const tq_sortkey_t* keys, int32_t nkeys, int64_t n,
+                              const int64_t* n_dev, int64_t limit, int32_t ncols,
+                              const void* const* src, const int32_t* row_bytes,
+                              void* const* dst, int64_t* perm_out, void* ws, size_t ws_bytes,
+                              int32_t* applied, void* stream) {
+  if (!applied || !perm_out) return set_error(TQ_EINVAL, "top-k gather: null argument");
+  *applied = 0;
+  if (n <= 0 || limit < 1 || limit > kSmallK) return TQ_OK;
+  if (nkeys < 1 || !keys) return set_error(TQ_EINVAL, "top-k gather: bad key list");
+  if (ncols < 0 || ncols > TQ_MAX_COLS || (ncols > 0 && (!src || !row_bytes || !dst)))
+    return set_error(TQ_EINVAL, "top-k gather: %d columns (0..%d)", ncols, TQ_MAX_COLS);
+  ColPack gp{};
+  gp.n = ncols;
+  for (int c = 0; c < ncols; ++c) {
+    if (!src[c] || !dst[c] || row_bytes[c] < 1)
+      return set_error(TQ_EINVAL, "top-k gather: column %d", c);
+    gp.src[c] = src[c];
+    gp.dst[c] = dst[c];
+    gp.bytes[c] = row_bytes[c];
+  }
+  bool done = false;
+  TQ_TRY(topk_perm(keys, nkeys, n, limit, perm_out, ws, ws_bytes, (hipStream_t)stream, &done,
+                   n_dev, &gp));
   *applied = done ? 1 : 0;
   return TQ_OK;
 }

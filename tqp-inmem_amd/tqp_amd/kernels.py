@@ -203,6 +203,56 @@ def sort_perm(keys, n: int, limit=None, device=None):
     return perm
 
 
+def take_ordered_deferred(keys, cols, n: int, limit: int, device):
+    """A take-ordered of at most 64 rows over the outputs of the active deferral's group-by
+    (group count still on the device): the sort and the gathers of `cols` in one call
+    (tq_topk_gather), the gathers inside the top-k's last kernel.  Returns the gathered
+    columns (min(n, limit) rows, the first min(groups, limit) valid), or None when this form
+    does not apply (then nothing was launched)."""
+    df = _deferred()
+    if df is None or df.group is None or df.G is not None or n != df.group["cap"] \
+            or not 1 <= limit <= 64 or len(cols) > L.TQ_MAX_COLS:
+        return None
+    outs = {t.data_ptr() for t in df.group["outs"]}
+    arr = (L.SortKey * len(keys))()
+    kb = 0
+    for i, (t, desc) in enumerate(keys):
+        if t.dim() == 2 and t.shape[1] == 1:
+            t = t.reshape(-1)
+        if t.dim() != 1 or not t.is_contiguous() or t.data_ptr() not in outs:
+            return None
+        arr[i].ptr = t.data_ptr()
+        arr[i].dtype = L.dtype_code(t.dtype)
+        arr[i].width = 1
+        arr[i].descending = 1 if desc else 0
+        kb += t.element_size()
+    m = min(n, limit)
+    nc = len(cols)
+    src = (ctypes.c_void_p * max(nc, 1))()
+    dst = (ctypes.c_void_p * max(nc, 1))()
+    rb = (ctypes.c_int32 * max(nc, 1))()
+    gathered = []
+    for i, c in enumerate(cols):
+        L.require_gpu(c, "sort input")
+        if not c.is_contiguous():
+            return None
+        o = memory.empty((m,) + tuple(c.shape[1:]), dtype=c.dtype, device=device)
+        gathered.append(o)
+        src[i], dst[i], rb[i] = c.data_ptr(), o.data_ptr(), L.row_bytes(c)
+    perm = L.workspace(8 * m, device, "topk_perm")
+    ws_bytes = L.size_call("tq_sort_ws_bytes", n)
+    ws = L.workspace(ws_bytes, device, "sort")
+    applied = ctypes.c_int32()
+    at = L.trace_len()
+    L.call("tq_topk_gather", arr, len(keys), n, ctypes.c_void_p(df.group["ngroups"]), limit, nc,
+           src, rb, dst, L.ptr(perm), L.ptr(ws), ws_bytes, ctypes.byref(applied),
+           L.stream_handle(), kernel={"k_topk_block": n * kb})
+    if not applied.value:
+        return None
+    df.sorted = {"cap": m, "limit": int(limit), "trace": at, "kb": kb}
+    return gathered
+
+
 # ---------------------------------------------------------------- value bounds
 # Host-known value bounds of integer columns: a join writes its key outputs from keys whose
 # range its build already measured, so a group-by over those columns can plan without its own

@@ -27,10 +27,14 @@ def tqp_sort(gpu_enable, tensor_group, args, with_limit, name="sort"):
     if k0.numel() * k0.element_size() >= 8_000_000_000:
         raise RuntimeError("sort input exceeds in-memory limit")
     keys = [(tensor_group[k].tensor, bool(d)) for k, d in zip(key_ids, key_descs)]
-    perm = kernels.sort_perm(keys, rows, limit, device=k0.device)
-
     cols = [tensor_group[k].tensor for k in input_ids]
-    outs = kernels.gather(perm, cols)
+    outs = None
+    if limit is not None and 1 <= limit <= 64:
+        # over a deferred group-by's outputs: top-k and gathers in one launch sequence
+        outs = kernels.take_ordered_deferred(keys, cols, rows, int(limit), k0.device)
+    if outs is None:
+        perm = kernels.sort_perm(keys, rows, limit, device=k0.device)
+        outs = kernels.gather(perm, cols)
     for key_id, t in zip(input_ids, outs):
         tensor_group[key_id] = Variable(t, tensor_group[key_id].tensor_type)
     perf_logger().stop(name)
