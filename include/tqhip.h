@@ -612,15 +612,23 @@ int tq_groupjoin_run_noalloc(const tq_groupjoin_desc_t* desc /* host */,
  * 8 integer key columns packed in mixed radix (as tq_groupby_accumulate does).
  * The group table is addressed by build key - kmin over the build key's [min, max] range, so it
  * is the join directory and the aggregation table at once, and groups come out in key order.
- * Work is a device queue of (pipeline, tile) items in five phases -- clear the table, insert
- * build rows, probe + accumulate, count groups per tile, write groups -- where an item waits
- * (bounded wait) for its pipeline's previous phase to finish.  No host synchronisation occurs;
- * the launch sequence can be captured into a graph.  A duplicate selected build key sets
- * flags[0] bit 0 (the caller reruns that pipeline through the per-operator path).
+ * Work is a device queue of (pipeline, tile) items in six phases -- clear the table, insert
+ * build rows, probe, reduce records, count groups per tile, write groups -- where an item waits
+ * (bounded wait) for its pipeline's last earlier phase that has items.  A pipeline accumulates
+ * in one of three ways: into an LDS copy of a small table (lds_table), with global atomics, or
+ * (records) without atomics as the fused groupjoin does: each TQ_BATCH_CHUNK-row probe chunk writes one
+ * record {slot within its key slice, accumulator values} per pair into the chunk's own region,
+ * slice by slice (slice offsets in loff), and a reduce item per key slice folds the slice's
+ * records from every chunk in LDS and writes the slice's cnt/acc slots.  No host
+ * synchronisation occurs; the launch sequence can be captured into a graph.  A duplicate
+ * selected build key sets flags[0] bit 0 (the caller reruns that pipeline through the
+ * per-operator path).
  * Replaces, per pipeline, the whole chain tqp_filter (operators/filter.py:164-205) x2 ->
  * join_vortex (operators/hashjoin.py:179-248) -> tqp_hash_aggregate (operators/aggregate.py:
  * 236-393) grouped by the join key. */
-#define TQ_BATCH_PHASES 5
+#define TQ_BATCH_PHASES 6
+#define TQ_BATCH_CHUNK 32768   /* probe rows per item of a records pipeline          */
+#define TQ_BATCH_MAX_SLICES 2048
 
 typedef struct {
   tq_program_t bprog;  /* build predicate; load slot 0 must be the build key column  */
@@ -638,7 +646,7 @@ typedef struct {
   uint32_t* dir;       /* device [range]: build row + 1, 0 = empty                     */
   uint32_t* cnt;       /* device [range]: rows per group                               */
   uint64_t* acc;       /* device [naggs * range]: accumulators                         */
-  uint32_t* tile_counts; /* device [ntiles[3]]                                         */
+  uint32_t* tile_counts; /* device [ntiles[4]]                                         */
   void* key_out;       /* device [min(nb, range)] group keys (join pipelines)          */
   void* agg_out[TQ_MAX_SEGS];
   int64_t* ngroups;    /* device [1]                                                   */
@@ -652,22 +660,33 @@ typedef struct {
   int64_t gkey_range[TQ_GB_MAX_KEYS];
   int32_t gkey_dtype[TQ_GB_MAX_KEYS];
   void* gkey_out[TQ_GB_MAX_KEYS];
+  int32_t records;     /* 1: the probe writes records and phase 3 reduces them            */
+  int32_t sb;          /* records: a key slice is 2^sb table slots                        */
+  int32_t nslices;     /* records: ceil(range / 2^sb), at most TQ_BATCH_MAX_SLICES          */
+  int32_t pad_;
+  uint64_t* rec;       /* records: device [np * (1 + naggs)] words                         */
+  uint32_t* loff;      /* records: device [(nslices + 1) * ntiles[2]]: slice s of chunk c
+                        * starts at record loff[s * ntiles[2] + c] of the chunk's region;
+                        * row nslices holds the chunk's records                           */
 } tq_batch_pipe_t;
 
 /* Counters the launch needs (queue head, per-pipeline phase completion, error word). */
 size_t tq_batch_ws_bytes(int32_t npipes);
 
 /* Run the batch.  pipes: device array of npipes descriptors; items: device int32 pairs
- * (pipeline, tile), phase-major, with phase_start[0..5] (host) the first item of each phase;
- * tile_rows: rows (or table slots) per item, a multiple of 1024; grid: persistent workgroups;
- * stage_loads: the largest nload of any program; lds_table_bytes: the largest LDS table
- * (range * (4 + 8 * naggs)) of the pipelines with lds_table set.  ws is cleared by the call
+ * (pipeline, tile), phase-major, with phase_start[0..6] (host) the first item of each phase;
+ * tile_rows: rows (or table slots) per item, a multiple of 1024 (a records pipeline's probe
+ * items are TQ_BATCH_CHUNK rows); grid: persistent workgroups; stage_loads: the largest nload of
+ * any program; lds_table_bytes: LDS bytes beside the stage, at least the largest LDS table
+ * (range * (4 + 8 * naggs)) of the lds_table pipelines and, with a records pipeline, the
+ * records scratch (TQ_BATCH_CHUNK / 32 + TQ_BATCH_MAX_SLICES words) and its slice tables
+ * (2^sb * (4 + 8 * naggs)); at most 48 KB.  ws is cleared by the call
  * (kernels only).  An item's wait for its pipeline's previous phase polls with relaxed loads
  * and gives up after wait_us microseconds of wall-clock time (the device's constant-rate
  * clock): ws error word 1 then means a wait timed out (the launch drains without finishing and
  * the caller reruns the pipelines per operator). */
 int tq_batch_run(const tq_batch_pipe_t* pipes, int32_t npipes, const int32_t* items,
-                 const int64_t* phase_start /* host, 6 */, int32_t tile_rows, int32_t grid,
+                 const int64_t* phase_start /* host, 7 */, int32_t tile_rows, int32_t grid,
                  int32_t stage_loads, int32_t lds_table_bytes, void* ws, uint32_t wait_us,
                  void* stream);
 

@@ -1,12 +1,19 @@
 // Batched filter -> join -> group-by pipelines in one persistent launch.
 //
 // Workgroups pop (pipeline, tile) items from a device queue (one atomic per item) in phase-major
-// order: clear the pipeline's group table, insert selected build rows, probe selected rows and
-// accumulate, count groups per table tile, write groups.  An item of phase k waits until every
-// tile of phase k-1 of its own pipeline has retired (per-pipeline completion counters with
-// agent-scope release/acquire; the wait is bounded and reports an error instead of hanging).
-// Items are popped in order by running workgroups, so the items waited on are already held by
-// resident workgroups and the queue always drains.
+// order: clear the pipeline's group table, insert selected build rows, probe selected rows, reduce
+// the probe's records (records pipelines), count groups per table tile, write groups.  An item of
+// phase k waits until every item of its own pipeline's last earlier phase that has items has
+// retired (per-pipeline completion counters with agent-scope release/acquire; the wait is bounded
+// and reports an error instead of hanging).  Items are popped in order by running workgroups, so
+// the items waited on are already held by resident workgroups and the queue always drains.
+//
+// The probe accumulates in one of three ways.  A table that fits LDS is accumulated there per item
+// and merged once.  A records pipeline (a join pipeline whose table does not fit) writes, per
+// probe chunk, one record {slot within its key slice, accumulator values} per pair, grouped by
+// key slice in the chunk's own region, and a reduce item per slice folds that slice's records
+// from every chunk in LDS and writes the slice's cnt/acc slots: no global atomics and no table
+// clearing beyond the directory.  The remaining pipelines accumulate with global atomics.
 //
 // The group table of a pipeline is addressed by build key - kmin: it is the join directory
 // (build row per key) and the aggregation table (pair count and accumulators per key), so the
@@ -66,8 +73,13 @@ __device__ __forceinline__ uint64_t acc_init(int op, int dt) {
                     : dt == TQ_BOOL ? 1 : INT64_MAX);
 }
 
-// Phase 0: clear table slots [s0, s1).
+// Phase 0: clear table slots [s0, s1) (a records pipeline's reduce writes every cnt/acc slot
+// itself, so only its directory is cleared).
 __device__ void phase_clear(const tq_batch_pipe_t& B, int64_t s0, int64_t s1) {
+  if (B.records) {
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) B.dir[s] = 0;
+    return;
+  }
   for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
     B.dir[s] = 0;
     B.cnt[s] = 0;
@@ -197,6 +209,151 @@ __device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
   }
 }
 
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+static_assert(TQ_BATCH_MAX_SLICES % kBlock == 0, "slice counts scan in whole rows per thread");
+
+// Phase 2 of a records pipeline: probe chunk c (TQ_BATCH_CHUNK rows).  Pass 1 evaluates the
+// predicate, looks each selected row's key up in the directory, keeps one hit bit per row (a
+// wave ballot per 64 rows) and counts the pairs per key slice; the counts' exclusive scan is
+// the chunk's row of loff and the slices' write cursors.  Pass 2 reloads the key of each paired
+// row, evaluates its aggregate arguments (in row order) and writes its record at its slice's
+// cursor.  lbuf: the hit bits (u64 per 64 rows), then the slice counts.
+__device__ void phase_probe_rec(const tq_batch_pipe_t& B, int64_t c, uint64_t* stage,
+                                uint32_t* lbuf, uint32_t* sh) {
+  const int t = threadIdx.x, l = lane_id(), w = t >> 6;
+  const tq_program_t& P = B.pprog;
+  const tq_program_t& A = B.aprog;
+  const int64_t r0 = c * TQ_BATCH_CHUNK;
+  const int64_t r1 = min(r0 + (int64_t)TQ_BATCH_CHUNK, B.np);
+  const int S = B.nslices, sb = B.sb;
+  const int64_t W = B.ntiles[2];
+  uint64_t* hm = (uint64_t*)lbuf;
+  uint32_t* hist = lbuf + TQ_BATCH_CHUNK / 32;
+  for (int i = t; i < S; i += kBlock) hist[i] = 0;
+  __syncthreads();
+  for (int64_t sub0 = r0; sub0 < r1; sub0 += kSubRows) {
+    bool live[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) live[j] = sub0 + (int64_t)j * kBlock + t < r1;
+    stage_loads(P, B.pcols, sub0, live, stage);
+#pragma unroll 1
+    for (int j = 0; j < kSub; ++j) {
+      bool hit = false;
+      if (live[j]) {
+        const int64_t row = sub0 + (int64_t)j * kBlock + t;
+        Regs R{};
+        run_program(P, B.pcols, 0, P.seg_end[0], j, row, true, stage, R);
+        if (R.get(P.seg_result[0])) {
+          const int64_t s = slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]);
+          if (s >= 0 && B.dir[s] != 0u) {
+            hit = true;
+            atomicAdd(&hist[(uint32_t)s >> sb], 1u);
+          }
+        }
+      }
+      // rows sub0 - r0 + j * kBlock + 64 w + [0, 64): one word of hit bits per wave
+      const uint64_t bits = ballot(hit);
+      if (l == 0) hm[(sub0 - r0 + (int64_t)j * kBlock) / 64 + w] = bits;
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the slice counts (hist becomes the slices' write cursors)
+  constexpr int kPer = TQ_BATCH_MAX_SLICES / kBlock;
+  uint32_t v[kPer], sum = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = t * kPer + k;
+    v[k] = i < S ? hist[i] : 0u;
+    sum += v[k];
+  }
+  uint32_t M;
+  uint32_t run = block_exclusive_sum(sum, sh, &M);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = t * kPer + k;
+    if (i < S) {
+      hist[i] = run;
+      B.loff[(int64_t)i * W + c] = run;
+    }
+    run += v[k];
+  }
+  if (t == 0) B.loff[(int64_t)S * W + c] = M;
+  __syncthreads();
+  const int rw = 1 + B.naggs;
+  const uint32_t smask = (1u << sb) - 1u;
+  uint64_t* crec = B.rec + r0 * rw;
+  for (int64_t sub0 = r0; sub0 < r1; sub0 += kSubRows) {
+    bool hit[kSub];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      hit[j] = (hm[(sub0 - r0 + (int64_t)j * kBlock) / 64 + w] >> l) & 1ull;
+      any |= hit[j];
+    }
+    if (!__any(any)) continue;  // no pair in the wave: its loads are skipped (no LDS barrier here)
+    // the key (load slot 0 of the predicate) of each paired row, then the aggregate arguments
+    stage_loads(P, B.pcols, sub0, hit, stage);
+    uint32_t sv[kSub];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j)
+      sv[j] = hit[j] ? (uint32_t)slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]) : 0u;
+    stage_loads(A, B.acols, sub0, hit, stage);
+#pragma unroll 1
+    for (int j = 0; j < kSub; ++j) {
+      if (!hit[j]) continue;
+      const int64_t row = sub0 + (int64_t)j * kBlock + t;
+      const uint32_t pos = atomicAdd(&hist[sv[j] >> sb], 1u);
+      uint64_t* r = crec + (int64_t)pos * rw;
+      r[0] = sv[j] & smask;
+      Regs R{};
+      int begin = 0;
+      for (int a = 0; a < B.naggs; ++a) {
+        const int end = A.seg_end[a];
+        run_program(A, B.acols, begin, end, j, row, true, stage, R);
+        begin = end;
+        r[1 + a] = to_acc(acc_kind(B.agg_op[a], B.agg_dtype[a]), B.agg_dtype[a],
+                          R.get(A.seg_result[a]));
+      }
+    }
+  }
+}
+
+// Phase 3 of a records pipeline: key slice s.  Its records in every chunk (loff column s) are
+// folded into an LDS table of the slice's 2^sb slots, which then overwrites the slice's cnt/acc
+// slots (coalesced, no atomics).  lbuf: cnt[2^sb], then acc[naggs][2^sb].
+__device__ void phase_reduce(const tq_batch_pipe_t& B, int s, uint32_t* lbuf) {
+  const int t = threadIdx.x;
+  const int64_t ns = (int64_t)1 << B.sb;
+  const int64_t g0 = (int64_t)s * ns;
+  const int64_t n = min(ns, B.range - g0);
+  uint32_t* lcnt = lbuf;
+  uint64_t* lacc = (uint64_t*)(lbuf + ns);
+  for (int64_t i = t; i < n; i += kBlock) {
+    lcnt[i] = 0;
+    for (int a = 0; a < B.naggs; ++a) lacc[a * ns + i] = acc_init(B.agg_op[a], B.agg_dtype[a]);
+  }
+  __syncthreads();
+  const int64_t W = B.ntiles[2];
+  const int rw = 1 + B.naggs;
+  for (int64_t c = t; c < W; c += kBlock) {
+    const uint32_t e0 = B.loff[(int64_t)s * W + c];
+    const uint32_t e1 = B.loff[(int64_t)(s + 1) * W + c];
+    const uint64_t* cr = B.rec + c * TQ_BATCH_CHUNK * rw;
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint64_t* r = cr + (int64_t)e * rw;
+      const uint32_t slot = (uint32_t)r[0];
+      atomicAdd(&lcnt[slot], 1u);
+      for (int a = 0; a < B.naggs; ++a)
+        acc_apply(&lacc[a * ns + slot], acc_kind(B.agg_op[a], B.agg_dtype[a]), r[1 + a]);
+    }
+  }
+  __syncthreads();
+  for (int64_t i = t; i < n; i += kBlock) {
+    B.cnt[g0 + i] = lcnt[i];
+    for (int a = 0; a < B.naggs; ++a) B.acc[(int64_t)a * B.range + g0 + i] = lacc[a * ns + i];
+  }
+}
+
 __device__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
   v = wave_sum(v);
   if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
@@ -254,7 +411,34 @@ __device__ void phase_write(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, in
     run += tot;
     __syncthreads();
   }
-  if (tile == B.ntiles[4] - 1 && threadIdx.x == 0) *B.ngroups = run;
+  if (tile == B.ntiles[5] - 1 && threadIdx.x == 0) *B.ngroups = run;
+}
+
+// Completions a workgroup has not yet published: the phase they belong to, and (pipeline,
+// items) runs.  They are published -- one release fence (the write-back of the XCD's L2) for all
+// of them, then the counters -- when the workgroup takes an item of another phase, when the list
+// is full, and when it leaves the queue.  A workgroup's unpublished completions are therefore
+// always of the phase of the item it is working on, and no workgroup waits on a phase that is not
+// earlier than its own item's, so every awaited completion is published or held by a workgroup
+// that is working, not waiting.
+constexpr int kPubMax = 32;
+struct PubList {
+  int n;
+  int phase;
+  int32_t pipe[kPubMax];
+  uint32_t count[kPubMax];
+};
+
+__device__ void publish(PubList& L, uint32_t* done) {
+  __syncthreads();  // every wave's stores have reached the L2
+  if (threadIdx.x == 0 && L.n > 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (int i = 0; i < L.n; ++i)
+      __hip_atomic_fetch_add(done + (int64_t)L.pipe[i] * TQ_BATCH_PHASES + L.phase, L.count[i],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    L.n = 0;
+  }
+  __syncthreads();
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restrict__ pipes,
@@ -267,11 +451,19 @@ __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restr
   __shared__ int s_abort;
   __shared__ uint32_t sh[kWaves];
   __shared__ uint32_t wsum[kWaves];
+  __shared__ PubList pub;
+  __shared__ int s_acq_pipe, s_acq_phase;  // the dependency this workgroup last acquired
   const int64_t nitems = ps.s[TQ_BATCH_PHASES];
+  if (threadIdx.x == 0) {
+    pub.n = 0;
+    pub.phase = -1;
+    s_acq_pipe = -1;
+    s_acq_phase = -1;
+  }
   while (true) {
     if (threadIdx.x == 0) {
       s_item = (int64_t)atomicAdd(ws.head, 1ull);
-      s_abort = load_acquire(ws.err) != 0;
+      s_abort = __hip_atomic_load(ws.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
     }
     __syncthreads();
     const int64_t it = s_item;
@@ -283,11 +475,15 @@ __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restr
     int phase = 0;
     while (phase < TQ_BATCH_PHASES - 1 && it >= ps.s[phase + 1]) ++phase;
     const tq_batch_pipe_t& B = pipes[p];
-    if (phase > 0) {
+    // completions of an earlier phase go out before this item waits on anything
+    if (pub.n > 0 && pub.phase != phase) publish(pub, ws.done);
+    int prev = phase - 1;  // the pipeline's last earlier phase that has items
+    while (prev >= 0 && B.ntiles[prev] == 0) --prev;
+    if (prev >= 0 && !(s_acq_pipe == p && s_acq_phase == prev)) {
       if (threadIdx.x == 0) {
-        // wait for the pipeline's previous phase, at most wait_ticks of wall-clock time
-        const uint32_t need = (uint32_t)B.ntiles[phase - 1];
-        const uint32_t* d = ws.done + (int64_t)p * TQ_BATCH_PHASES + (phase - 1);
+        // wait for that phase, at most wait_ticks of wall-clock time
+        const uint32_t need = (uint32_t)B.ntiles[prev];
+        const uint32_t* d = ws.done + (int64_t)p * TQ_BATCH_PHASES + prev;
         const int64_t t0 = wall_clock64();
         bool late = false;
         while (load_relaxed(d) < need) {
@@ -303,21 +499,38 @@ __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restr
       __syncthreads();
       if (s_abort) break;
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __syncthreads();  // every thread has read s_acq_* above
+      if (threadIdx.x == 0) {
+        s_acq_pipe = p;
+        s_acq_phase = prev;
+      }
     }
     const int64_t lo = (int64_t)tile * tile_rows;
     switch (phase) {
       case 0: phase_clear(B, lo, min(lo + tile_rows, B.range)); break;
       case 1: phase_build(B, lo, min(lo + tile_rows, B.nb), stage); break;
-      case 2: phase_probe(B, lo, min(lo + tile_rows, B.np), stage, ltab); break;
-      case 3: phase_count(B, lo, min(lo + tile_rows, B.range), tile, sh); break;
+      case 2:
+        if (B.records) phase_probe_rec(B, tile, stage, (uint32_t*)ltab, sh);
+        else phase_probe(B, lo, min(lo + tile_rows, B.np), stage, ltab);
+        break;
+      case 3: phase_reduce(B, tile, (uint32_t*)ltab); break;
+      case 4: phase_count(B, lo, min(lo + tile_rows, B.range), tile, sh); break;
       default: phase_write(B, lo, min(lo + tile_rows, B.range), tile, sh, wsum); break;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (threadIdx.x == 0)
-      __hip_atomic_fetch_add(ws.done + (int64_t)p * TQ_BATCH_PHASES + phase, 1u, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    if (pub.n == kPubMax) publish(pub, ws.done);
+    if (threadIdx.x == 0) {
+      if (pub.n > 0 && pub.pipe[pub.n - 1] == p) {
+        ++pub.count[pub.n - 1];
+      } else {
+        pub.pipe[pub.n] = p;
+        pub.count[pub.n] = 1;
+        ++pub.n;
+      }
+      pub.phase = phase;
+    }
   }
+  publish(pub, ws.done);
 }
 
 static BatchWs carve_batch(void* base, int32_t npipes) {

@@ -6,7 +6,9 @@ join key with aggregates over probe columns -- the shape of TPC-H Q3's core and 
 workload), compiles their predicates and aggregate arguments into register programs, lays out
 one group table per pipeline (addressed by build key - min key; it is the join directory and the
 aggregation table at once), and runs all of them with `tq_batch_run`: a persistent kernel that
-drains a device queue of (pipeline, tile) items.  `run()` issues only kernel launches, so it can
+drains a device queue of (pipeline, tile) items.  A join pipeline whose table does not fit LDS
+aggregates through records (one per pair, grouped by key slice per probe chunk) and a reduce
+per key slice, as the fused groupjoin does, rather than with global atomics.  `run()` issues only kernel launches, so it can
 be captured into a graph and replayed; `results()` synchronises once and slices the outputs.
 
 Outputs per pipeline are what `tqp_hash_aggregate` leaves in the tensor group: the group key
@@ -28,7 +30,11 @@ from . import memory
 from .conversion import get_id, right_most
 from .operators.aggregate import _is_key, _strip_alias, _unwrap
 
-PHASES = 5
+PHASES = 6
+CHUNK = 32768             # probe rows per item of a records pipeline
+MAX_SLICES = 2048
+REDUCE_LDS = 32 * 1024    # a records pipeline's slice table: 2^sb * (4 + 8 * naggs) bytes
+LDS_EXTRA_MAX = 48 * 1024
 
 
 class BatchPipe(ctypes.Structure):
@@ -50,7 +56,10 @@ class BatchPipe(ctypes.Structure):
                 ("gkey_stride", ctypes.c_int64 * L.TQ_GB_MAX_KEYS),
                 ("gkey_range", ctypes.c_int64 * L.TQ_GB_MAX_KEYS),
                 ("gkey_dtype", ctypes.c_int32 * L.TQ_GB_MAX_KEYS),
-                ("gkey_out", ctypes.c_void_p * L.TQ_GB_MAX_KEYS)]
+                ("gkey_out", ctypes.c_void_p * L.TQ_GB_MAX_KEYS),
+                ("records", ctypes.c_int32), ("sb", ctypes.c_int32),
+                ("nslices", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("rec", ctypes.c_void_p), ("loff", ctypes.c_void_p)]
 
 LDS_TABLE_MAX = 32 * 1024
 
@@ -194,8 +203,10 @@ def _key_bounds(t: torch.Tensor):
 class BatchedEngine:
     """Prepare once, then run() (graph-capturable) and results()."""
 
-    def __init__(self, specs, tile_rows=4096, grid=None, wait_us=100_000, key_bounds=None):
+    def __init__(self, specs, tile_rows=4096, grid=None, wait_us=100_000, key_bounds=None,
+                 records=True):
         self.specs = list(specs)
+        self.records = records    # False: every large table accumulates with global atomics
         self.tile_rows = int(tile_rows)
         self.device = torch.device("cuda:0")
         props = torch.cuda.get_device_properties(self.device)
@@ -289,24 +300,45 @@ class BatchedEngine:
             if not ab.col_tensors:
                 ab.column(next(iter(sp.probe_cols.values())))
             lay.update(ab=ab, arg_dtypes=arg_dtypes)
-            tbytes = lay["rng"] * (4 + 8 * len(sp.aggs))
+            naggs = len(sp.aggs)
+            tbytes = lay["rng"] * (4 + 8 * naggs)
             lay["lds"] = 1 if tbytes <= LDS_TABLE_MAX else 0
+            lay["records"] = 0
             if lay["lds"]:
                 lds_table = max(lds_table, tbytes)
+            elif sp.mode == 0 and self.records:
+                # records: the largest slice whose LDS table fits, if the slices are few enough
+                sb = 12
+                while sb > 6 and (1 << sb) * (4 + 8 * naggs) > REDUCE_LDS:
+                    sb -= 1
+                nsl = _ceil(lay["rng"], 1 << sb)
+                if (1 << sb) * (4 + 8 * naggs) <= REDUCE_LDS and nsl <= MAX_SLICES:
+                    lay.update(records=1, sb=sb, nslices=nsl)
+                    lds_table = max(lds_table, (CHUNK // 32 + MAX_SLICES) * 4,
+                                    (1 << sb) * (4 + 8 * naggs))
             stage_loads = max(stage_loads, lay["bb"].nload, lay["pb"].nload, ab.nload)
             layout.append(lay)
+        if lds_table > LDS_EXTRA_MAX or stage_loads * 8 * 1024 + lds_table > 64 * 1024:
+            # the records scratch does not fit beside the stage: atomics for those pipelines
+            for lay in layout:
+                lay["records"] = 0
+            lds_table = max([lay["rng"] * (4 + 8 * len(self.specs[lay["i"]].aggs))
+                             for lay in layout if lay["lds"]] or [0])
         self.stage_loads = stage_loads
         self.lds_table_bytes = lds_table
-        # one table per pipeline (cleared by the launch's first phase)
+        # one table per pipeline (cleared by the launch's first phase), and a records
+        # pipeline's records and slice offsets
         self.tables = [memory.empty(max(_table_bytes(lay["rng"], lay["ab"].nseg, T)[-1], 256),
                                     dtype=torch.uint8, device=dev) for lay in layout]
+        self.records = [memory.empty(max(_records_bytes(lay)[-1], 256), dtype=torch.uint8,
+                                     device=dev) if lay["records"] else None for lay in layout]
         self.ngroups = memory.empty(max(len(layout), 1), dtype=torch.int64, device=dev)
         self.flags = memory.zeros(max(len(layout), 1), dtype=torch.int32, device=dev)
         items = [[] for _ in range(PHASES)]
         arr = (BatchPipe * max(len(layout), 1))()
         ops = {'sum': L.TQ_AGG_SUM, 'count': L.TQ_AGG_COUNT, 'avg': L.TQ_AGG_AVG,
                'min': L.TQ_AGG_MIN, 'max': L.TQ_AGG_MAX}
-        for k, (lay, table) in enumerate(zip(layout, self.tables)):
+        for k, (lay, table, recbuf) in enumerate(zip(layout, self.tables, self.records)):
             sp = self.specs[lay["i"]]
             d = arr[k]
             d.bprog, d.bcols = lay["bb"].prog, lay["bb"].cols
@@ -353,7 +385,15 @@ class BatchedEngine:
             d.ngroups = self.ngroups.data_ptr() + 8 * k
             d.flags = self.flags.data_ptr() + 4 * k
             rng, nb, np_ = lay["rng"], lay["nb"], lay["np"]
-            nt = [_ceil(rng, T), _ceil(nb, T), _ceil(np_, T), _ceil(rng, T), _ceil(rng, T)]
+            if lay["records"]:
+                d.records, d.sb, d.nslices = 1, lay["sb"], lay["nslices"]
+                loff_off, _ = _records_bytes(lay)
+                d.rec, d.loff = recbuf.data_ptr(), recbuf.data_ptr() + loff_off
+                nt = [_ceil(rng, T), _ceil(nb, T), _ceil(np_, CHUNK), lay["nslices"],
+                      _ceil(rng, T), _ceil(rng, T)]
+            else:
+                nt = [_ceil(rng, T), _ceil(nb, T), _ceil(np_, T), 0, _ceil(rng, T),
+                      _ceil(rng, T)]
             for ph in range(PHASES):
                 d.ntiles[ph] = nt[ph]
                 items[ph].append((nt[ph], k))
@@ -419,6 +459,14 @@ def _table_bytes(rng, naggs, tile):
     acc_off = cnt_off + _align(rng * 4)
     tc_off = acc_off + _align(rng * 8 * max(naggs, 1))
     return cnt_off, acc_off, tc_off, tc_off + _align(_ceil(rng, tile) * 4)
+
+
+def _records_bytes(lay):
+    """Offset of the slice offsets in a records pipeline's buffer, and its total size: records
+    of up to one per probe row (1 + naggs words each), then (nslices + 1) x chunks u32."""
+    rw = 1 + lay["ab"].nseg
+    loff_off = _align(max(lay["np"], 1) * rw * 8)
+    return loff_off, loff_off + _align((lay["nslices"] + 1) * max(_ceil(lay["np"], CHUNK), 1) * 4)
 
 
 def _ceil(a, b):
