@@ -217,7 +217,8 @@ def batched_section(pipes_n, dev):
     rows = int(probe.sum() + build.sum())
     specs = [batched.match_plan(p, g) for g, p in pipes]
     tune = batched.autotune(specs, key=f"{torch.cuda.get_device_name(0)}|config3|{pipes_n}")
-    eng = batched.BatchedEngine(specs, tile_rows=tune["tile_rows"], grid=tune["grid"])
+    eng = batched.BatchedEngine(specs, tile_rows=tune["tile_rows"], grid=tune["grid"],
+                                records=tune.get("records", False))
     eng.run()
     out, rerun = eng.results()
     reps = 5

@@ -620,9 +620,9 @@ int tq_groupjoin_run_noalloc(const tq_groupjoin_desc_t* desc /* host */,
  * record {slot within its key slice, accumulator values} per pair into the chunk's own region,
  * slice by slice (slice offsets in loff), and a reduce item per key slice folds the slice's
  * records from every chunk in LDS and writes the slice's cnt/acc slots.  No host
- * synchronisation occurs; the launch sequence can be captured into a graph.  A duplicate
- * selected build key sets flags[0] bit 0 (the caller reruns that pipeline through the
- * per-operator path).
+ * synchronisation occurs; the launch sequence can be captured into a graph.  A selected build
+ * key outside [kmin, kmin + range) sets flags[0] bit 1, and a repeated one makes the two
+ * keycount words differ (the caller reruns such a pipeline through the per-operator path).
  * Replaces, per pipeline, the whole chain tqp_filter (operators/filter.py:164-205) x2 ->
  * join_vortex (operators/hashjoin.py:179-248) -> tqp_hash_aggregate (operators/aggregate.py:
  * 236-393) grouped by the join key. */
@@ -643,7 +643,7 @@ typedef struct {
   int32_t key_dtype;   /* dtype of the build key (and of the group key output)        */
   int32_t agg_op[TQ_MAX_SEGS];
   int32_t agg_dtype[TQ_MAX_SEGS];
-  uint32_t* dir;       /* device [range]: build row + 1, 0 = empty                     */
+  uint8_t* pres;       /* join pipelines: device [range]: 1 = a selected build key      */
   uint32_t* cnt;       /* device [range]: rows per group                               */
   uint64_t* acc;       /* device [naggs * range]: accumulators                         */
   uint32_t* tile_counts; /* device [ntiles[4]]                                         */
@@ -668,7 +668,19 @@ typedef struct {
   uint32_t* loff;      /* records: device [(nslices + 1) * ntiles[2]]: slice s of chunk c
                         * starts at record loff[s * ntiles[2] + c] of the chunk's region;
                         * row nslices holds the chunk's records                           */
+  int64_t* keycount;   /* join pipelines: device [2]: selected build rows, and distinct
+                        * selected build keys (they differ when a build key repeats)       */
+  int32_t fast[2];     /* set by tq_batch_prepare: the build / probe predicate runs as a
+                        * conjunction of column-versus-constant terms over whole tiles     */
+  uint64_t prep[2][32]; /* set by tq_batch_prepare (opaque)                                */
 } tq_batch_pipe_t;
+
+/* Fill the fast[] and prep[] fields of npipes host descriptors (before they are copied to the
+ * device): a join pipeline's build or probe predicate that is a conjunction of column-versus-
+ * constant comparisons is then evaluated over whole 4096-row tiles with all loads in flight, its
+ * key looked up in the presence bits, and its aggregate arguments evaluated for the joined rows
+ * only.  Returns the number of predicates prepared (>= 0) or a TQ_E* code. */
+int tq_batch_prepare(tq_batch_pipe_t* pipes /* host */, int32_t npipes);
 
 /* Counters the launch needs (queue head, per-pipeline phase completion, error word). */
 size_t tq_batch_ws_bytes(int32_t npipes);

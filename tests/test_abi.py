@@ -74,9 +74,9 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         "tq_join_state_t": (_lib.JoinState, ["layout", "base", "range", "bits", "rows", "pending", "nb", "key_and"]),
         "tq_ktime_t": (_lib.KTime, ["ms"]),
         "tq_groupby_desc_t": (_lib.GroupByDesc, ["key_ptr", "key_stride", "agg_dtype", "nsel", "nrows"]),
-        "tq_batch_pipe_t": (batched.BatchPipe, ["pprog", "acols", "dir", "agg_out", "ntiles",
+        "tq_batch_pipe_t": (batched.BatchPipe, ["pprog", "acols", "pres", "agg_out", "ntiles",
                                                 "gkey_min", "gkey_out", "records", "nslices",
-                                                "rec", "loff"]),
+                                                "rec", "loff", "keycount", "fast", "prep"]),
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void){']
     for cname, (_, fields) in checks.items():

@@ -16,20 +16,24 @@ def _plain(group, plan):
     return tg
 
 
-@pytest.mark.parametrize("records", [True, False])
-def test_batched_matches_plain(device, records):
+@pytest.mark.parametrize("records,fast", [(True, True), (False, True), (True, False),
+                                          (False, False)])
+def test_batched_matches_plain(device, records, fast):
     """records=True: tables beyond LDS aggregate through records and slice reduces; False: with
-    global atomics."""
+    global atomics.  fast=True: conjunctive predicates over whole tiles; False: the per-row
+    interpreter."""
     from tqp_amd import batched, workloads
     probe, build, consts = workloads.config3_sizes(n_pipes=24, lo=1000, hi=300_000)
     pipes = [workloads.config3_pipeline(i, probe[i], build[i], consts[i][0], consts[i][1], device)
              for i in range(len(probe))]
     specs = [batched.match_plan(plan, group) for group, plan in pipes]
     assert all(s is not None for s in specs)
-    eng = batched.BatchedEngine(specs, tile_rows=2048, records=records)
+    eng = batched.BatchedEngine(specs, tile_rows=2048, records=records, fast=fast)
     nrec = sum(int(d.records) for d in eng._desc_host[:eng.npipes])
-    print("records pipelines:", nrec, "of", eng.npipes, flush=True)
+    print("records pipelines:", nrec, "of", eng.npipes, "fast predicates:",
+          eng.fast_predicates, flush=True)
     assert (nrec > 0) == records
+    assert (eng.fast_predicates == 2 * eng.npipes) == fast
     for rep in range(2):                     # a replay must give the same results
         eng.run()
         out, rerun = eng.results()

@@ -21,13 +21,43 @@
 // without sorting.  Predicates and aggregate arguments run on the expression interpreter with the
 // same LDS staging as the per-operator kernels.
 #include "tq_agg.h"
+#include <cstring>
 #include <map>
 #include <mutex>
 
 #include "tq_common.h"
+#include "tq_conj.h"
 #include "tq_expr.h"
 
 namespace tq {
+
+static_assert(sizeof(Conj) <= sizeof(((tq_batch_pipe_t*)nullptr)->prep[0]),
+              "a prepared predicate fits its descriptor field");
+
+__device__ __forceinline__ const Conj& prepared(const tq_batch_pipe_t& B, int side) {
+  return *(const Conj*)B.prep[side];
+}
+
+
+// The prepared predicate (terms over the program's load slots) for row j of the staged sub-tile.
+__device__ __forceinline__ bool stage_conj(const Conj& C, const uint64_t* stage, int j) {
+  bool ok = true;
+  for (int k = 0; k < C.nterm; ++k)
+    ok = ok && term_holds(C.term_op[k], C.term_imm[k],
+                          stage[(C.term_slot[k] * kSub + j) * kBlock + threadIdx.x]);
+  return ok;
+}
+
+// Row j of the sub-tile passes the build or probe predicate: the prepared terms, else the
+// interpreter.
+__device__ __forceinline__ bool passes(const tq_batch_pipe_t& B, int side,
+                                       const tq_program_t& P, const tq_colset_t& cols, int j,
+                                       int64_t row, const uint64_t* stage) {
+  if (B.fast[side]) return stage_conj(prepared(B, side), stage, j);
+  Regs R{};
+  run_program(P, cols, 0, P.seg_end[0], j, row, true, stage, R);
+  return R.get(P.seg_result[0]) != 0;
+}
 
 struct PhaseStarts {
   int64_t s[TQ_BATCH_PHASES + 1];
@@ -73,17 +103,26 @@ __device__ __forceinline__ uint64_t acc_init(int op, int dt) {
                     : dt == TQ_BOOL ? 1 : INT64_MAX);
 }
 
+__device__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
+  v = wave_sum(v);
+  if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) t += sh[w];
+  __syncthreads();
+  return t;
+}
+
 // Phase 0: clear table slots [s0, s1) (a records pipeline's reduce writes every cnt/acc slot
-// itself, so only its directory is cleared).
+// itself, so only its presence bytes are cleared); the first tile also zeroes the key counts.
 __device__ void phase_clear(const tq_batch_pipe_t& B, int64_t s0, int64_t s1) {
-  if (B.records) {
-    for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) B.dir[s] = 0;
-    return;
+  if (s0 == 0 && B.keycount && threadIdx.x < 2) B.keycount[threadIdx.x] = 0;
+  if (B.pres) {
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) B.pres[s] = 0;
   }
-  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
-    B.dir[s] = 0;
-    B.cnt[s] = 0;
-  }
+  if (B.records) return;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) B.cnt[s] = 0;
   for (int a = 0; a < B.naggs; ++a) {
     const uint64_t init = acc_init(B.agg_op[a], B.agg_dtype[a]);
     uint64_t* acc = B.acc + (int64_t)a * B.range;
@@ -91,10 +130,14 @@ __device__ void phase_clear(const tq_batch_pipe_t& B, int64_t s0, int64_t s1) {
   }
 }
 
-// Phase 1: insert selected build rows [r0, r1) into the directory.
-__device__ void phase_build(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, uint64_t* stage) {
+// Phase 1: mark the keys of the selected build rows [r0, r1) present (plain byte stores) and
+// add the item's selected rows to keycount[0]; phase 4 adds the distinct present keys to
+// keycount[1], so a repeated key makes them differ.
+__device__ void phase_build(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, uint64_t* stage,
+                            uint32_t* sh) {
   const int t = threadIdx.x;
   const tq_program_t& P = B.bprog;
+  uint32_t nsel = 0;
   for (int64_t sub0 = r0; sub0 < r1; sub0 += kSubRows) {
     bool live[kSub];
 #pragma unroll
@@ -104,18 +147,19 @@ __device__ void phase_build(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
     for (int j = 0; j < kSub; ++j) {
       if (!live[j]) continue;
       const int64_t row = sub0 + (int64_t)j * kBlock + t;
-      Regs R{};
-      run_program(P, B.bcols, 0, P.seg_end[0], j, row, true, stage, R);
-      if (!R.get(P.seg_result[0])) continue;
+      if (!passes(B, 0, P, B.bcols, j, row, stage)) continue;
       const int64_t key = (int64_t)stage[(0 * kSub + j) * kBlock + t];
       const int64_t s = slot_of(B, key);
       if (s < 0) {
         atomicOr((unsigned int*)B.flags, 2u);  // key outside the declared range
         continue;
       }
-      if (atomicCAS(&B.dir[s], 0u, (uint32_t)(row + 1)) != 0u) atomicOr((unsigned int*)B.flags, 1u);
+      B.pres[s] = 1;
+      ++nsel;
     }
   }
+  nsel = block_sum_u32(nsel, sh);
+  if (t == 0 && nsel) atomicAdd((unsigned long long*)&B.keycount[0], (unsigned long long)nsel);
 }
 
 // Packed group key of a group-by pipeline from its staged key slots.
@@ -161,19 +205,21 @@ __device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
       slot[j] = -1;
       if (!live[j]) continue;
       const int64_t row = sub0 + (int64_t)j * kBlock + t;
-      Regs R{};
-      run_program(P, B.pcols, 0, P.seg_end[0], j, row, true, stage, R);
-      if (!R.get(P.seg_result[0])) continue;
+      if (!passes(B, 1, P, B.pcols, j, row, stage)) continue;
       if (B.mode == 1) {
         hit[j] = true;
         slot[j] = pack_gkey(B, stage, j);
         continue;
       }
-      const int64_t s = slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]);
-      if (s >= 0 && B.dir[s] != 0u) {
-        hit[j] = true;
-        slot[j] = s;
-      }
+      slot[j] = slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]);
+    }
+    if (B.mode == 0) {
+      // the presence bytes of the four rows, all loads in flight together
+      uint8_t b[kSub];
+#pragma unroll
+      for (int j = 0; j < kSub; ++j) b[j] = slot[j] >= 0 ? B.pres[slot[j]] : (uint8_t)0;
+#pragma unroll
+      for (int j = 0; j < kSub; ++j) hit[j] = b[j] != 0;
     }
     // every thread has read its own staged words; the aggregate stage reuses the buffer
     stage_loads(A, B.acols, sub0, hit, stage);
@@ -241,11 +287,9 @@ __device__ void phase_probe_rec(const tq_batch_pipe_t& B, int64_t c, uint64_t* s
       bool hit = false;
       if (live[j]) {
         const int64_t row = sub0 + (int64_t)j * kBlock + t;
-        Regs R{};
-        run_program(P, B.pcols, 0, P.seg_end[0], j, row, true, stage, R);
-        if (R.get(P.seg_result[0])) {
+        if (passes(B, 1, P, B.pcols, j, row, stage)) {
           const int64_t s = slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]);
-          if (s >= 0 && B.dir[s] != 0u) {
+          if (s >= 0 && B.pres[s] != 0) {
             hit = true;
             atomicAdd(&hist[(uint32_t)s >> sb], 1u);
           }
@@ -354,24 +398,21 @@ __device__ void phase_reduce(const tq_batch_pipe_t& B, int s, uint32_t* lbuf) {
   }
 }
 
-__device__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
-  v = wave_sum(v);
-  if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  uint32_t t = 0;
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) t += sh[w];
-  __syncthreads();
-  return t;
-}
 
 // Phase 3: occupied slots of [s0, s1) -> tile_counts[tile].
 __device__ void phase_count(const tq_batch_pipe_t& B, int64_t s0, int64_t s1, int tile, uint32_t* sh) {
   const bool force0 = B.mode == 1 && B.nkeys == 0;
-  uint32_t c = 0;
-  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) c += (B.cnt[s] > 0) || (force0 && s == 0);
+  uint32_t c = 0, k = 0;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
+    c += (B.cnt[s] > 0) || (force0 && s == 0);
+    if (B.pres) k += B.pres[s] != 0;
+  }
   c = block_sum_u32(c, sh);
   if (threadIdx.x == 0) B.tile_counts[tile] = c;
+  if (B.pres) {
+    k = block_sum_u32(k, sh);
+    if (threadIdx.x == 0 && k) atomicAdd((unsigned long long*)&B.keycount[1], (unsigned long long)k);
+  }
 }
 
 // Phase 4: write the groups of [s0, s1) after the groups of earlier tiles.
@@ -508,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restr
     const int64_t lo = (int64_t)tile * tile_rows;
     switch (phase) {
       case 0: phase_clear(B, lo, min(lo + tile_rows, B.range)); break;
-      case 1: phase_build(B, lo, min(lo + tile_rows, B.nb), stage); break;
+      case 1: phase_build(B, lo, min(lo + tile_rows, B.nb), stage, sh); break;
       case 2:
         if (B.records) phase_probe_rec(B, tile, stage, (uint32_t*)ltab, sh);
         else phase_probe(B, lo, min(lo + tile_rows, B.np), stage, ltab);
@@ -545,6 +586,31 @@ static BatchWs carve_batch(void* base, int32_t npipes) {
 }  // namespace tq
 
 using namespace tq;
+
+extern "C" int tq_batch_prepare(tq_batch_pipe_t* pipes, int32_t npipes) {
+  if (npipes < 0 || (npipes > 0 && !pipes)) return set_error(TQ_EINVAL, "batch prepare: bad arguments");
+  int n = 0;
+  for (int i = 0; i < npipes; ++i) {
+    tq_batch_pipe_t& B = pipes[i];
+    for (int side = 0; side < 2; ++side) {
+      B.fast[side] = 0;
+      memset(B.prep[side], 0, sizeof(B.prep[side]));
+      if (B.mode != 0 || !B.pres) continue;
+      const tq_program_t& P = side ? B.pprog : B.bprog;
+      const tq_colset_t& cols = side ? B.pcols : B.bcols;
+      // load slot 0 is the key, read as an integer whatever the predicate reads
+      if (P.nload < 1 || P.load_dtype[0] < TQ_I8 || P.load_dtype[0] > TQ_I64 ||
+          cols.width[P.load_col[0]] != 1)
+        continue;
+      Conj C{};
+      if (!match_conj(P, cols, &C, false)) continue;
+      memcpy(B.prep[side], &C, sizeof(C));
+      B.fast[side] = 1;
+      ++n;
+    }
+  }
+  return n;
+}
 
 extern "C" size_t tq_batch_ws_bytes(int32_t npipes) {
   return 512 + align_up((size_t)(npipes > 0 ? npipes : 1) * TQ_BATCH_PHASES * 4, 256);

@@ -91,6 +91,42 @@ __device__ __forceinline__ uint32_t conj_apply32(const Conj& C, int s, const uin
   return acc;
 }
 
+// One term against one staged value (the stage's conventions: integers sign-extended to 64 bits,
+// f64 as its bits, f32 as its bits in the low half).
+__device__ __forceinline__ bool term_holds(int op, int64_t imm, uint64_t v) {
+  if (op < 6) {
+    const int64_t x = (int64_t)v;
+    switch (op) {
+      case 0: return x < imm;
+      case 1: return x <= imm;
+      case 2: return x > imm;
+      case 3: return x >= imm;
+      case 4: return x == imm;
+      default: return x != imm;
+    }
+  }
+  if (op < 12) {
+    const double x = __longlong_as_double((long long)v), c = __longlong_as_double(imm);
+    switch (op - 6) {
+      case 0: return x < c;
+      case 1: return x <= c;
+      case 2: return x > c;
+      case 3: return x >= c;
+      case 4: return x == c;
+      default: return x != c;
+    }
+  }
+  const float x = __uint_as_float((uint32_t)v), c = __uint_as_float((uint32_t)imm);
+  switch (op - 12) {
+    case 0: return x < c;
+    case 1: return x <= c;
+    case 2: return x > c;
+    case 3: return x >= c;
+    case 4: return x == c;
+    default: return x != c;
+  }
+}
+
 // The rows of a 4096-row tile (thread t: the 16 rows run_row(tile0, j)) that pass every term,
 // as bits of `acc` (rows past n must already be clear in it).
 __device__ __forceinline__ uint32_t conj_sel16(const Conj& C, int64_t tile0, int64_t n,
@@ -112,7 +148,11 @@ inline int cmp_code(int op) {
   return -1;
 }
 
-inline bool match_conj(const tq_program_t& P, const tq_colset_t& cols, Conj* C) {
+// With remap == false the terms keep the program's own load slots (term_slot indexes P's load
+// table) and nslot / slot_* are left empty: for kernels that evaluate the terms over the
+// program's staged loads.
+inline bool match_conj(const tq_program_t& P, const tq_colset_t& cols, Conj* C,
+                       bool remap = true) {
   enum { UNK, CONSTV, LOADV, TERMS };
   struct RS { int kind; int64_t imm; int slot; uint32_t terms; };
   RS R[TQ_NREG];
@@ -173,18 +213,19 @@ inline bool match_conj(const tq_program_t& P, const tq_colset_t& cols, Conj* C) 
     C->term_imm[C->nterm] = all.term_imm[k];
     ++C->nterm;
   }
+  if (!remap) return true;
   // load only the slots some kept term reads, renumbered densely
-  int remap[TQ_MAX_LOADS];
-  for (int sl = 0; sl < TQ_MAX_LOADS; ++sl) remap[sl] = -1;
+  int dense[TQ_MAX_LOADS];
+  for (int sl = 0; sl < TQ_MAX_LOADS; ++sl) dense[sl] = -1;
   for (int k = 0; k < C->nterm; ++k) {
     const int sl = C->term_slot[k];
-    if (remap[sl] < 0) {
-      remap[sl] = C->nslot;
+    if (dense[sl] < 0) {
+      dense[sl] = C->nslot;
       C->slot_dtype[C->nslot] = P.load_dtype[sl];
       C->slot_col[C->nslot] = cols.ptr[P.load_col[sl]];
       ++C->nslot;
     }
-    C->term_slot[k] = remap[sl];
+    C->term_slot[k] = dense[sl];
   }
   return true;
 }

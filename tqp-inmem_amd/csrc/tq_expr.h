@@ -247,6 +247,39 @@ __device__ __forceinline__ void run_program(const tq_program_t& P, const tq_cols
   }
 }
 
+// Gather the program's load slots at rows[j] (live rows only) into the stage, in the layout
+// stage_loads uses, so run_rows can evaluate them.
+__device__ __forceinline__ void gather_loads(const tq_program_t& P, const tq_colset_t& C,
+                                             const int64_t (&rows)[kSub],
+                                             const bool (&live)[kSub],
+                                             uint64_t* __restrict__ stage) {
+  const int t = threadIdx.x;
+  for (int s = 0; s < P.nload; ++s) {
+    const void* col = col_ptr(C, P.load_col[s]);
+    const int dt = P.load_dtype[s];
+    uint64_t v[kSub];
+#define TQ_GATHER(T, EXPR)                                 \
+  {                                                        \
+    const T* p = (const T*)col;                            \
+    _Pragma("unroll") for (int j = 0; j < kSub; ++j) {     \
+      const int64_t r = rows[j];                           \
+      v[j] = live[j] ? (EXPR) : 0;                         \
+    }                                                      \
+  }
+    switch (dt) {
+      case TQ_BOOL: TQ_GATHER(uint8_t, (uint64_t)(p[r] != 0)) break;
+      case TQ_I8: TQ_GATHER(int8_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_I16: TQ_GATHER(int16_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_I32: TQ_GATHER(int32_t, (uint64_t)(int64_t)p[r]) break;
+      case TQ_F32: TQ_GATHER(uint32_t, (uint64_t)p[r]) break;
+      default: TQ_GATHER(uint64_t, p[r]) break;
+    }
+#undef TQ_GATHER
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) stage[(s * kSub + j) * kBlock + t] = v[j];
+  }
+}
+
 // The kSub rows of a thread's sub-tile together: one Regs per row (each a set of named scalars
 // selected by switch), reached only with compile-time row indices, so nothing goes to scratch.
 struct V4 {

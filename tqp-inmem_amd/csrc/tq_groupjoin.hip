@@ -122,39 +122,6 @@ __global__ __launch_bounds__(kBlock) void k_gj_build(const void* __restrict__ bk
 }
 
 // ------------------------------------------------------------------ 2. probe
-// Gather the program's load slots at rows[j] (live rows only) into the stage, in the layout
-// stage_loads uses, so run_rows can evaluate them.
-__device__ __forceinline__ void gather_loads(const tq_program_t& P, const tq_colset_t& C,
-                                             const int64_t (&rows)[kSub],
-                                             const bool (&live)[kSub],
-                                             uint64_t* __restrict__ stage) {
-  const int t = threadIdx.x;
-  for (int s = 0; s < P.nload; ++s) {
-    const void* col = col_ptr(C, P.load_col[s]);
-    const int dt = P.load_dtype[s];
-    uint64_t v[kSub];
-#define TQ_GATHER(T, EXPR)                                 \
-  {                                                        \
-    const T* p = (const T*)col;                            \
-    _Pragma("unroll") for (int j = 0; j < kSub; ++j) {     \
-      const int64_t r = rows[j];                           \
-      v[j] = live[j] ? (EXPR) : 0;                         \
-    }                                                      \
-  }
-    switch (dt) {
-      case TQ_BOOL: TQ_GATHER(uint8_t, (uint64_t)(p[r] != 0)) break;
-      case TQ_I8: TQ_GATHER(int8_t, (uint64_t)(int64_t)p[r]) break;
-      case TQ_I16: TQ_GATHER(int16_t, (uint64_t)(int64_t)p[r]) break;
-      case TQ_I32: TQ_GATHER(int32_t, (uint64_t)(int64_t)p[r]) break;
-      case TQ_F32: TQ_GATHER(uint32_t, (uint64_t)p[r]) break;
-      default: TQ_GATHER(uint64_t, p[r]) break;
-    }
-#undef TQ_GATHER
-#pragma unroll
-    for (int j = 0; j < kSub; ++j) stage[(s * kSub + j) * kBlock + t] = v[j];
-  }
-}
-
 // Dynamic LDS: stage (nload * kSubRows u64), hist[S] u32, the chunk's matches in slice order
 // u32[kGjChunk] as (local row << sb) | slot within the slice.
 // loff[s * W + c] = first record of slice s in chunk c's region (s < S), loff[S * W + c] =

@@ -159,6 +159,7 @@ _SIGNATURES = {
     "tq_sort_ws_bytes": ([I64], SZ),
     "tq_sort_perm": ([P, I32, I64, I64, P, P, SZ, P], I32),
     "tq_sort_perm_deferred": ([P, I32, I64, P, I64, P, P, SZ, P, P], I32),
+    "tq_batch_prepare": ([P, I32], I32),
     "tq_topk_gather": ([P, I32, I64, P, I64, I32, P, P, P, P, P, SZ, P, P], I32),
     "tq_join_build_ws_bytes": ([I64], SZ),
     "tq_join_build": ([P, I32, P, I64, P, SZ, P, P], I32),

@@ -46,7 +46,7 @@ class BatchPipe(ctypes.Structure):
                 ("naggs", ctypes.c_int32), ("key_dtype", ctypes.c_int32),
                 ("agg_op", ctypes.c_int32 * L.TQ_MAX_SEGS),
                 ("agg_dtype", ctypes.c_int32 * L.TQ_MAX_SEGS),
-                ("dir", ctypes.c_void_p), ("cnt", ctypes.c_void_p), ("acc", ctypes.c_void_p),
+                ("pres", ctypes.c_void_p), ("cnt", ctypes.c_void_p), ("acc", ctypes.c_void_p),
                 ("tile_counts", ctypes.c_void_p), ("key_out", ctypes.c_void_p),
                 ("agg_out", ctypes.c_void_p * L.TQ_MAX_SEGS),
                 ("ngroups", ctypes.c_void_p), ("flags", ctypes.c_void_p),
@@ -59,7 +59,9 @@ class BatchPipe(ctypes.Structure):
                 ("gkey_out", ctypes.c_void_p * L.TQ_GB_MAX_KEYS),
                 ("records", ctypes.c_int32), ("sb", ctypes.c_int32),
                 ("nslices", ctypes.c_int32), ("pad_", ctypes.c_int32),
-                ("rec", ctypes.c_void_p), ("loff", ctypes.c_void_p)]
+                ("rec", ctypes.c_void_p), ("loff", ctypes.c_void_p),
+                ("keycount", ctypes.c_void_p), ("fast", ctypes.c_int32 * 2),
+                ("prep", (ctypes.c_uint64 * 32) * 2)]
 
 LDS_TABLE_MAX = 32 * 1024
 
@@ -204,9 +206,10 @@ class BatchedEngine:
     """Prepare once, then run() (graph-capturable) and results()."""
 
     def __init__(self, specs, tile_rows=4096, grid=None, wait_us=100_000, key_bounds=None,
-                 records=True):
+                 records=False, fast=True):
         self.specs = list(specs)
-        self.records = records    # False: every large table accumulates with global atomics
+        self.records = records    # True: large join tables accumulate through records
+        self.fast = fast          # False: every predicate runs on the per-row interpreter
         self.tile_rows = int(tile_rows)
         self.device = torch.device("cuda:0")
         props = torch.cuda.get_device_properties(self.device)
@@ -334,6 +337,8 @@ class BatchedEngine:
                                      device=dev) if lay["records"] else None for lay in layout]
         self.ngroups = memory.empty(max(len(layout), 1), dtype=torch.int64, device=dev)
         self.flags = memory.zeros(max(len(layout), 1), dtype=torch.int32, device=dev)
+        # per join pipeline: selected build rows, distinct selected build keys
+        self.keycount = memory.zeros(2 * max(len(layout), 1), dtype=torch.int64, device=dev)
         items = [[] for _ in range(PHASES)]
         arr = (BatchPipe * max(len(layout), 1))()
         ops = {'sum': L.TQ_AGG_SUM, 'count': L.TQ_AGG_COUNT, 'avg': L.TQ_AGG_AVG,
@@ -351,7 +356,10 @@ class BatchedEngine:
             d.key_dtype = L.dtype_code(lay["key_dtype"])
             cnt_off, acc_off, tc_off, _ = _table_bytes(lay["rng"], d.naggs, T)
             p = table.data_ptr()
-            d.dir, d.cnt, d.acc, d.tile_counts = p, p + cnt_off, p + acc_off, p + tc_off
+            d.cnt, d.acc, d.tile_counts = p + cnt_off, p + acc_off, p + tc_off
+            if sp.mode == 0:
+                d.pres = p
+                d.keycount = self.keycount.data_ptr() + 16 * k
             cap = lay["cap"]
             keys_out = []
             if sp.mode == 0:
@@ -406,6 +414,13 @@ class BatchedEngine:
             for ntile, k in sorted(items[ph], key=lambda x: -x[0]):
                 flat.extend((k, t) for t in range(ntile))
             starts.append(len(flat))
+        # conjunctive build / probe predicates run over whole tiles (fills fast[] and prep[])
+        self.fast_predicates = 0
+        if self.fast:
+            n = L.load().tq_batch_prepare(arr, len(layout))
+            if n < 0:
+                raise RuntimeError("tq_batch_prepare: " + L.load().tq_last_error().decode())
+            self.fast_predicates = n
         self.phase_start = (ctypes.c_int64 * (PHASES + 1))(*starts)
         self.nitems = len(flat)
         it = torch.tensor(flat if flat else [(0, 0)], dtype=torch.int32).reshape(-1)
@@ -435,13 +450,15 @@ class BatchedEngine:
             raise RuntimeError("batched launch aborted: a phase wait exceeded its time limit")
         ng = self.ngroups.cpu().tolist()
         fl = self.flags.cpu().tolist()
+        kc = self.keycount.cpu().tolist()
         out, rerun = {}, list(self.rejected)
         for k, (i, keys_out, agg_outs) in enumerate(self.outputs):
-            if fl[k]:
+            sp = self.specs[i]
+            if fl[k] or (sp.mode == 0 and kc[2 * k] != kc[2 * k + 1]):
+                # a build key outside the bounds, or a repeated one
                 rerun.append(i)
                 continue
             G = ng[k]
-            sp = self.specs[i]
             ids = [sp.key_out_id] if sp.mode == 0 else sp.key_out_ids
             res = {oid: t[:G].reshape(-1, 1) for oid, t in zip(ids, keys_out)}
             for (oid, _, _), t in zip(sp.aggs, agg_outs):
@@ -454,8 +471,9 @@ class BatchedEngine:
 
 
 def _table_bytes(rng, naggs, tile):
-    """Offsets of cnt, acc and tile_counts in one pipeline's table, and its total size."""
-    cnt_off = _align(rng * 4)
+    """Offsets of cnt, acc and tile_counts in one pipeline's table (the presence bytes first),
+    and its total size."""
+    cnt_off = _align(rng)
     acc_off = cnt_off + _align(rng * 4)
     tc_off = acc_off + _align(rng * 8 * max(naggs, 1))
     return cnt_off, acc_off, tc_off, tc_off + _align(_ceil(rng, tile) * 4)
@@ -482,8 +500,9 @@ TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.js
 
 
 def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None):
-    """Pick tile rows and persistent grid size for a batch by timing candidates on it.
-    Results are cached in tuning.json under `key` (device name + workload signature)."""
+    """Pick tile rows, persistent grid size and the accumulation of large tables (global atomics
+    or records with slice reduces) for a batch by timing candidates on it.  Results are cached
+    in tuning.json under `key` (device name + workload signature)."""
     dev = torch.device("cuda:0")
     props = torch.cuda.get_device_properties(dev)
     key = key or f"{props.name}|{len(specs)}"
@@ -494,11 +513,14 @@ def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None):
     if key in table:
         return table[key]
     cu = props.multi_processor_count
-    candidates = candidates or [(t, cu * g) for t in (2048, 4096, 8192) for g in (2, 4, 8)]
+    candidates = candidates or [(t, cu * g, r) for t in (4096, 8192) for g in (2, 4, 8)
+                                for r in (False, True)]
     best, timings = None, {}
     bounds = {}
-    for tile, grid in candidates:
-        eng = BatchedEngine(specs, tile_rows=tile, grid=grid, key_bounds=bounds)
+    for cand in candidates:
+        tile, grid = cand[0], cand[1]
+        records = cand[2] if len(cand) > 2 else True
+        eng = BatchedEngine(specs, tile_rows=tile, grid=grid, key_bounds=bounds, records=records)
         eng.run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -506,11 +528,12 @@ def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None):
             eng.run()
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / repeats
-        timings[f"{tile}x{grid}"] = dt
+        timings[f"{tile}x{grid}{'r' if records else 'a'}"] = dt
         if best is None or dt < best[0]:
-            best = (dt, tile, grid)
+            best = (dt, tile, grid, records)
         del eng
-    choice = {"tile_rows": best[1], "grid": best[2], "seconds": best[0], "timings": timings}
+    choice = {"tile_rows": best[1], "grid": best[2], "records": best[3], "seconds": best[0],
+              "timings": timings}
     table[key] = choice
     if cache:
         try:
