@@ -283,6 +283,57 @@ __device__ __forceinline__ uint32_t load_sel16(const uint8_t* mask, int64_t tile
   return s & exist;
 }
 
+// ------------------------------------------------------------------ issue-only tile loads
+// The loaders above convert each value right after loading it, inside a switch on the dtype, so
+// a kernel that calls them for several columns waits for one column's loads before issuing the
+// next column's.  For a full tile of a column aligned to its run (four values), issue_runs only
+// issues the four run loads of the thread (4 bytes per run for 1-byte values, 8 for 2-byte, 16
+// for 4-byte) and touches nothing it loaded; run_value32 converts afterwards.  A kernel issues
+// every column's runs, then converts, so all its loads are in flight together.
+template <int W> struct RunRaw;
+template <> struct RunRaw<1> { uint32_t r[4]; };
+template <> struct RunRaw<2> { uint2 r[4]; };
+template <> struct RunRaw<4> { uint4 r[4]; };
+
+template <int W>
+__device__ __forceinline__ void issue_runs(const void* p, int64_t tile0, RunRaw<W>& out) {
+  const uint8_t* b = (const uint8_t*)p;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t row = run_row(tile0, 4 * q);
+    if constexpr (W == 1) out.r[q] = *(const uint32_t*)(b + row);
+    else if constexpr (W == 2) out.r[q] = *(const uint2*)(b + 2 * row);
+    else out.r[q] = *(const uint4*)(b + 4 * row);
+  }
+}
+
+// Value j (of the thread's 16 tile rows) in load_raw16_32's convention: bool as 0/1, 1- and
+// 2-byte integers sign-extended, 4-byte values as their bits.  j is a compile-time index after
+// unrolling, so the run and lane selects fold away.
+template <int W>
+__device__ __forceinline__ uint32_t run_value32(const RunRaw<W>& x, int j, int dt) {
+  const int q = j >> 2, i = j & 3;
+  if constexpr (W == 1) {
+    const uint32_t byte = (x.r[q] >> (8 * i)) & 0xFFu;
+    return dt == TQ_BOOL ? (uint32_t)(byte != 0) : (uint32_t)(int32_t)(int8_t)byte;
+  } else if constexpr (W == 2) {
+    const uint32_t w = (i < 2 ? x.r[q].x : x.r[q].y) >> (16 * (i & 1));
+    return (uint32_t)(int32_t)(int16_t)(w & 0xFFFFu);
+  } else {
+    return i == 0 ? x.r[q].x : i == 1 ? x.r[q].y : i == 2 ? x.r[q].z : x.r[q].w;
+  }
+}
+
+// Bytes per value of a dtype code, on either side.
+__host__ __device__ __forceinline__ int value_bytes(int dtype) {
+  return dtype <= TQ_I8 ? 1 : dtype == TQ_I16 ? 2 : (dtype == TQ_I32 || dtype == TQ_F32) ? 4 : 8;
+}
+
+// A column whose full tiles issue_runs can read: W-byte values with the run (4 * W bytes) aligned.
+__device__ __forceinline__ bool runs_ok(const void* p, int w) {
+  return w <= 4 && ((uintptr_t)p % (uintptr_t)(4 * w)) == 0;
+}
+
 inline int dtype_bytes(int dtype) {
   switch (dtype) {
     case TQ_BOOL: case TQ_I8: return 1;
