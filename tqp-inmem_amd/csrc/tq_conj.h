@@ -139,6 +139,100 @@ __device__ __forceinline__ uint32_t conj_sel16(const Conj& C, int64_t tile0, int
   return acc;
 }
 
+// ------------------------------------------------------------------ full-tile heads
+// A full tile with 4-byte keys, a conjunction whose terms read at most one column of at most 4
+// bytes, and aligned columns loads through issue_runs: the keys, the filter values and the mask
+// bytes of the tile (of every tile a kernel takes at once) are all issued before the first is
+// used.  FW is the filter column's width (0: no filter column), MASK whether a mask column is
+// read; the terms applied are C's terms on load slot cs, to the values of column fcol (dtype
+// fdt).  C stays where the caller keeps it (kernel arguments, a descriptor in global memory), so
+// its terms are read as uniform scalars.
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;  // a row with no key offset to look up
+
+template <int FW, bool MASK>
+__device__ __forceinline__ uint32_t conj_sel_runs(const RunRaw<(FW > 0 ? FW : 1)>& fr,
+                                                  const RunRaw<1>& mr, const Conj& C, int cs,
+                                                  int fdt) {
+  uint32_t s = 0xFFFFu;
+  if constexpr (MASK) {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) s &= ~((run_value32<1>(mr, j, TQ_BOOL) == 0u ? 1u : 0u) << j);
+  }
+  if constexpr (FW > 0) {
+    uint32_t pv[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) pv[j] = run_value32<FW>(fr, j, fdt);
+    s = conj_apply32(C, cs, pv, s);
+  }
+  return s;
+}
+
+// One full tile: the selection bits, and the keys sign-extended to 64 bits in raw.
+template <int FW, bool MASK>
+__device__ __forceinline__ uint32_t tile_head(const void* keys, const uint8_t* mask,
+                                              const void* fcol, const Conj& C, int cs, int fdt,
+                                              int64_t tile0, uint64_t (&raw)[kItems]) {
+  RunRaw<4> kr;
+  RunRaw<(FW > 0 ? FW : 1)> fr;
+  RunRaw<1> mr;
+  issue_runs<4>(keys, tile0, kr);
+  if constexpr (FW > 0) issue_runs<FW>(fcol, tile0, fr);
+  if constexpr (MASK) issue_runs<1>(mask, tile0, mr);
+  const uint32_t s = conj_sel_runs<FW, MASK>(fr, mr, C, cs, fdt);
+#pragma unroll
+  for (int j = 0; j < kItems; ++j)
+    raw[j] = (uint64_t)(int64_t)(int32_t)run_value32<4>(kr, j, TQ_I32);
+  return s;
+}
+
+// T consecutive full tiles from row base: every load issued tile by tile, then per tile the
+// selection, the key offsets (key - lo, kNoKey when unselected or outside [lo, lo + range);
+// range < 2^32 - 1) and the presence-byte lookups (a tile's lookups go out while the next
+// tile's loads are still in flight).
+template <int T, int FW, bool MASK>
+__device__ __forceinline__ void probe_head(const void* keys, const uint8_t* mask,
+                                           const void* fcol, const Conj& C, int cs, int fdt,
+                                           int64_t base, int64_t lo, uint64_t range,
+                                           const uint8_t* __restrict__ pres,
+                                           uint32_t (&off)[T][kItems], uint8_t (&b)[T][kItems],
+                                           uint32_t (&sel)[T]) {
+  RunRaw<4> kr[T];
+  RunRaw<(FW > 0 ? FW : 1)> fr[T];
+  RunRaw<1> mr[T];
+#pragma unroll
+  for (int q = 0; q < T; ++q) {
+    const int64_t tile0 = base + (int64_t)q * kTileRows;
+    issue_runs<4>(keys, tile0, kr[q]);
+    if constexpr (FW > 0) issue_runs<FW>(fcol, tile0, fr[q]);
+    if constexpr (MASK) issue_runs<1>(mask, tile0, mr[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < T; ++q) {
+    sel[q] = conj_sel_runs<FW, MASK>(fr[q], mr[q], C, cs, fdt);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const uint64_t o =
+          (uint64_t)(int64_t)(int32_t)run_value32<4>(kr[q], j, TQ_I32) - (uint64_t)lo;
+      off[q][j] = ((sel[q] >> j) & 1u) && o < range ? (uint32_t)o : kNoKey;
+      b[q][j] = off[q][j] != kNoKey ? pres[off[q][j]] : (uint8_t)0;
+    }
+  }
+}
+
+// Which head a full tile takes: -1 for the general loaders, else FW * 2 + MASK.  nfilt: 0 for
+// no filter column, 1 for the column fcol of dtype fdt.
+__device__ __forceinline__ int head_case(int kdt, const void* keys, const uint8_t* mask,
+                                         int nfilt, const void* fcol, int fdt) {
+  if (kdt != TQ_I32 || !runs_ok(keys, 4) || nfilt > 1) return -1;
+  if (mask && !runs_ok(mask, 1)) return -1;
+  int fw = 0;
+  if (nfilt == 1) {
+    fw = value_bytes(fdt);
+    if (fw > 4 || !runs_ok(fcol, fw)) return -1;
+  }
+  return fw * 2 + (mask ? 1 : 0);
+}
+
 // Recognise a conjunction of column-versus-constant comparisons in segment 0 of a program.
 // Compare opcodes map to 0..5 (lt, le, gt, ge, eq, ne) plus 6 for f64 and 12 for f32.
 inline int cmp_code(int op) {

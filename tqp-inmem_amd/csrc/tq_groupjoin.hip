@@ -47,7 +47,6 @@ constexpr size_t kGjTableLds = 48 * 1024;       // reduce table: count + aggrega
 enum { GJ_GROUPS = 0, GJ_PAIRS, GJ_NSEL, GJ_NSET, GJ_ERR, GJ_NWORDS = 8 };
 enum { GJ_ERR_BOUNDS = 1, GJ_ERR_WAIT = 2, GJ_ERR_CAP = 4 };
 constexpr uint32_t kOobBit = 0x80000000u;       // in a build tile count: a key outside [lo, hi]
-constexpr uint32_t kNoKey = 0xFFFFFFFFu;        // a probe row with no key offset to look up
 
 // A slice's look-back word: bit 0 published, then its groups, its stamped keys (each at most
 // 2^sb <= 4096 < 2^14) and its records (the pairs whose key lies in the slice).
@@ -73,94 +72,6 @@ struct GjOut {
   int64_t cap;
 };
 
-// ------------------------------------------------------------------ full-tile heads
-// A full tile with 4-byte keys, at most one filter column of at most 4 bytes and aligned
-// columns loads through issue_runs: the keys, the filter values and the mask bytes of the tile
-// (of both probe tiles) are all issued before the first is used.  FW is the filter column's
-// width (0: no filter column), MASK whether a mask column is read.  Returns the selection bits;
-// the keys come back sign-extended to 64 bits.
-template <int FW, bool MASK>
-__device__ __forceinline__ uint32_t gj_sel_head(const RunRaw<4>& kr,
-                                                const RunRaw<(FW > 0 ? FW : 1)>& fr,
-                                                const RunRaw<1>& mr, const Conj& C, int fdt) {
-  uint32_t s = 0xFFFFu;
-  if constexpr (MASK) {
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) s &= ~((run_value32<1>(mr, j, TQ_BOOL) == 0u ? 1u : 0u) << j);
-  }
-  if constexpr (FW > 0) {
-    uint32_t pv[kItems];
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) pv[j] = run_value32<FW>(fr, j, fdt);
-    s = conj_apply32(C, 0, pv, s);
-  }
-  (void)kr;
-  return s;
-}
-
-template <int FW, bool MASK>
-__device__ __forceinline__ uint32_t gj_build_head(const void* keys, const uint8_t* mask,
-                                                  const Conj& C, int64_t tile0,
-                                                  uint64_t (&raw)[kItems]) {
-  RunRaw<4> kr;
-  RunRaw<(FW > 0 ? FW : 1)> fr;
-  RunRaw<1> mr;
-  issue_runs<4>(keys, tile0, kr);
-  if constexpr (FW > 0) issue_runs<FW>(C.slot_col[0], tile0, fr);
-  if constexpr (MASK) issue_runs<1>(mask, tile0, mr);
-  const uint32_t s = gj_sel_head<FW, MASK>(kr, fr, mr, C, FW > 0 ? C.slot_dtype[0] : 0);
-#pragma unroll
-  for (int j = 0; j < kItems; ++j) raw[j] = (uint64_t)(int64_t)(int32_t)run_value32<4>(kr, j, TQ_I32);
-  return s;
-}
-
-// Both tiles of a full probe chunk: every load issued tile by tile, then per tile the selection,
-// the key offsets and the presence lookups (tile 0's lookups go out while tile 1's loads are
-// still in flight).
-template <int FW, bool MASK>
-__device__ __forceinline__ void gj_probe_head(const void* keys, const uint8_t* mask, const Conj& C,
-                                              int64_t base, int64_t lo, uint64_t range,
-                                              const uint8_t* __restrict__ pres,
-                                              uint32_t (&off)[kGjTiles][kItems],
-                                              uint8_t (&b)[kGjTiles][kItems],
-                                              uint32_t (&sel)[kGjTiles]) {
-  RunRaw<4> kr[kGjTiles];
-  RunRaw<(FW > 0 ? FW : 1)> fr[kGjTiles];
-  RunRaw<1> mr[kGjTiles];
-#pragma unroll
-  for (int q = 0; q < kGjTiles; ++q) {
-    const int64_t tile0 = base + (int64_t)q * kTileRows;
-    issue_runs<4>(keys, tile0, kr[q]);
-    if constexpr (FW > 0) issue_runs<FW>(C.slot_col[0], tile0, fr[q]);
-    if constexpr (MASK) issue_runs<1>(mask, tile0, mr[q]);
-  }
-  const int fdt = FW > 0 ? C.slot_dtype[0] : 0;
-#pragma unroll
-  for (int q = 0; q < kGjTiles; ++q) {
-    sel[q] = gj_sel_head<FW, MASK>(kr[q], fr[q], mr[q], C, fdt);
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-      const uint64_t o =
-          (uint64_t)(int64_t)(int32_t)run_value32<4>(kr[q], j, TQ_I32) - (uint64_t)lo;
-      off[q][j] = ((sel[q] >> j) & 1u) && o < range ? (uint32_t)o : kNoKey;
-      b[q][j] = off[q][j] != kNoKey ? pres[off[q][j]] : (uint8_t)0;
-    }
-  }
-}
-
-// Which head a full tile takes: -1 for the general loaders, else FW * 2 + MASK.
-__device__ __forceinline__ int gj_head_case(int kdt, const void* keys, const uint8_t* mask,
-                                            const Conj& C) {
-  if (kdt != TQ_I32 || !runs_ok(keys, 4) || C.nslot > 1) return -1;
-  if (mask && !runs_ok(mask, 1)) return -1;
-  int fw = 0;
-  if (C.nslot == 1) {
-    fw = value_bytes(C.slot_dtype[0]);
-    if (fw > 4 || !runs_ok(C.slot_col[0], fw)) return -1;
-  }
-  return fw * 2 + (mask ? 1 : 0);
-}
-
 // ------------------------------------------------------------------ 1. build
 // Workgroup 0 also zeroes the reduce's look-back words (flags[0..nflags)) and the words the
 // host reads back (the later kernels of the call only add to or OR into them).  A fused filter
@@ -182,15 +93,19 @@ __global__ __launch_bounds__(kBlock) void k_gj_build(const void* __restrict__ bk
   if (tile0 >= nb) return;  // nb == 0: the launch only clears the words
   uint64_t raw[kItems];
   uint32_t sel;
-  switch (tile0 + kTileRows <= nb ? gj_head_case(dt, bkeys, bmask, BC) : -1) {
-    case 0: sel = gj_build_head<0, false>(bkeys, bmask, BC, tile0, raw); break;
-    case 1: sel = gj_build_head<0, true>(bkeys, bmask, BC, tile0, raw); break;
-    case 2: sel = gj_build_head<1, false>(bkeys, bmask, BC, tile0, raw); break;
-    case 3: sel = gj_build_head<1, true>(bkeys, bmask, BC, tile0, raw); break;
-    case 4: sel = gj_build_head<2, false>(bkeys, bmask, BC, tile0, raw); break;
-    case 5: sel = gj_build_head<2, true>(bkeys, bmask, BC, tile0, raw); break;
-    case 8: sel = gj_build_head<4, false>(bkeys, bmask, BC, tile0, raw); break;
-    case 9: sel = gj_build_head<4, true>(bkeys, bmask, BC, tile0, raw); break;
+  switch (tile0 + kTileRows <= nb
+              ? head_case(dt, bkeys, bmask, BC.nslot, BC.slot_col[0], BC.slot_dtype[0]) : -1) {
+#define TQ_HEAD(FW, M) \
+  sel = tile_head<FW, M>(bkeys, bmask, BC.slot_col[0], BC, 0, BC.slot_dtype[0], tile0, raw)
+    case 0: TQ_HEAD(0, false); break;
+    case 1: TQ_HEAD(0, true); break;
+    case 2: TQ_HEAD(1, false); break;
+    case 3: TQ_HEAD(1, true); break;
+    case 4: TQ_HEAD(2, false); break;
+    case 5: TQ_HEAD(2, true); break;
+    case 8: TQ_HEAD(4, false); break;
+    case 9: TQ_HEAD(4, true); break;
+#undef TQ_HEAD
     default:
       load_raw16(bkeys, dt, tile0, nb, raw);
       sel = load_sel16(bmask, tile0, nb);
@@ -253,9 +168,13 @@ __global__ __launch_bounds__(kBlock, 3) void k_gj_probe(
   uint8_t b[kGjTiles][kItems];  // presence bytes
   const bool p32 = PC.nslot == 0 ||
                    (PC.nslot == 1 && (PC.slot_dtype[0] <= TQ_I32 || PC.slot_dtype[0] == TQ_F32));
-  const int head = base + kGjChunk <= np ? gj_head_case(pdt, pkeys, pmask, PC) : -1;
+  const int head = base + kGjChunk <= np
+                       ? head_case(pdt, pkeys, pmask, PC.nslot, PC.slot_col[0], PC.slot_dtype[0])
+                       : -1;
   if (head >= 0) {
-#define TQ_HEAD(FW, M) gj_probe_head<FW, M>(pkeys, pmask, PC, base, lo, range, pres, off, b, sel)
+#define TQ_HEAD(FW, M)                                                                  \
+  probe_head<kGjTiles, FW, M>(pkeys, pmask, PC.slot_col[0], PC, 0, PC.slot_dtype[0], base, lo, \
+                              range, pres, off, b, sel)
     switch (head) {
       case 0: TQ_HEAD(0, false); break;
       case 1: TQ_HEAD(0, true); break;
