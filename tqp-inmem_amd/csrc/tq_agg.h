@@ -3,6 +3,8 @@
 // combined with device-scope atomics, which also merge LDS partials into global tables.
 #pragma once
 
+#include <cstring>
+
 #include "tq_common.h"
 #include "tq_expr.h"
 
@@ -13,6 +15,9 @@ enum AccKind { ACC_FSUM = 0, ACC_ISUM, ACC_COUNT, ACC_FMIN, ACC_FMAX, ACC_IMIN, 
 constexpr unsigned long long kEmpty = ~0ull;
 constexpr int kLdsBudget = 64 * 1024;
 constexpr int kLdsBig = 152 * 1024;  // gfx950 gives a workgroup up to 160 KB of LDS on request
+
+enum DirectForm { DF_X = 0, DF_ONE_MINUS, DF_ONE_PLUS };
+constexpr int kDirectFactors = 4;
 
 struct GbSpec {
   int32_t nkeys, naggs;
@@ -29,6 +34,16 @@ struct GbSpec {
   const uint32_t* dict_bits[TQ_GB_MAX_KEYS];
   const uint32_t* dict_rank[TQ_GB_MAX_KEYS];
   const int64_t* dict_val[TQ_GB_MAX_KEYS];
+  // Direct arguments (direct != 0, set by match_direct): every aggregate's argument is either a
+  // constant, whose accumulator bits are dconst[a] (nf[a] == 0), or the left-to-right product of
+  // nf[a] factors over f64 load slots (factor i reads slot fslot[a][i] as x and is x, 1 - x or
+  // 1 + x by fform[a][i]), which the kernels evaluate from the staged loads without the
+  // instruction interpreter.
+  int32_t direct;
+  int8_t nf[TQ_GB_MAX_AGGS];
+  int8_t fslot[TQ_GB_MAX_AGGS][kDirectFactors];
+  int8_t fform[TQ_GB_MAX_AGGS][kDirectFactors];
+  uint64_t dconst[TQ_GB_MAX_AGGS];
 };
 
 // Packed-key digit of key k's value v: its offset in the key's range, or its rank among the
@@ -59,6 +74,139 @@ __device__ __forceinline__ uint64_t to_acc(int kind, int dt, uint64_t v) {
     case ACC_COUNT: return isf ? (uint64_t)(d != -64.0) : (uint64_t)((int64_t)v != -64);
     default: return v;  // integer domain
   }
+}
+
+// The accumulator-domain values of aggregate a for the thread's kSub staged rows, evaluated
+// directly (G.direct): the same operations in the same order as the argument's program, so the
+// values are bit-identical to the interpreter's.
+__device__ __forceinline__ void direct_args(const GbSpec& G, int a, const uint64_t* stage,
+                                            uint64_t (&v)[kSub]) {
+  const int nf = G.nf[a];
+  if (nf == 0) {
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) v[j] = G.dconst[a];
+    return;
+  }
+  double p[kSub];
+  for (int i = 0; i < nf; ++i) {
+    const int s = G.fslot[a][i], form = G.fform[a][i];
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) {
+      const double x = as_f64(stage[(s * kSub + j) * kBlock + threadIdx.x]);
+      const double f = form == DF_X ? x : form == DF_ONE_MINUS ? 1.0 - x : 1.0 + x;
+      p[j] = i == 0 ? f : p[j] * f;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) v[j] = to_acc(G.kind[a], TQ_F64, f64_bits(p[j]));
+}
+
+// Host twin of to_acc (for a constant argument's accumulator bits).
+inline uint64_t to_acc_host(int kind, int dt, uint64_t v) {
+  const bool isf = dt == TQ_F64 || dt == TQ_F32;
+  double d;
+  if (dt == TQ_F64) {
+    memcpy(&d, &v, 8);
+  } else if (dt == TQ_F32) {
+    float f;
+    const uint32_t lo = (uint32_t)v;
+    memcpy(&f, &lo, 4);
+    d = (double)f;
+  } else {
+    d = 0.0;
+  }
+  uint64_t out;
+  switch (kind) {
+    case ACC_FSUM: case ACC_FMIN: case ACC_FMAX: {
+      const double x = isf ? d : (double)(int64_t)v;
+      memcpy(&out, &x, 8);
+      return out;
+    }
+    case ACC_COUNT: return isf ? (uint64_t)(d != -64.0) : (uint64_t)((int64_t)v != -64);
+    default: return v;
+  }
+}
+
+// Recognise aggregate arguments that direct_args can evaluate: per segment of the program, a
+// constant, an f64 column, 1 - column, 1 + column, or a left-deep product of up to
+// kDirectFactors such factors (what TPC-H's sums over price, discount and tax compile to).
+// Follows the registers the way match_conj does; anything else leaves G->direct at 0.
+inline void match_direct(const tq_program_t& P, GbSpec* G) {
+  enum { UNK, COL, CST, PROD };
+  struct RV {
+    int kind;
+    int slot;
+    uint64_t bits;
+    int nf;
+    int8_t fs[kDirectFactors], ff[kDirectFactors];
+  };
+  uint64_t one;
+  {
+    const double d = 1.0;
+    memcpy(&one, &d, 8);
+  }
+  G->direct = 0;
+  if (P.nseg < G->naggs) return;
+  for (int a = 0; a < G->naggs; ++a) {
+    RV R[TQ_NREG];
+    for (int i = 0; i < TQ_NREG; ++i) R[i] = RV{UNK, 0, 0, 0, {0}, {0}};
+    const int begin = a == 0 ? 0 : P.seg_end[a - 1], end = P.seg_end[a];
+    if (begin < 0 || end > P.n_instr || end > TQ_MAX_INSTR || begin > end) return;
+    auto factor = [&](const RV& x) {  // a column as a one-factor product
+      if (x.kind != COL) return x;
+      RV f{PROD, 0, 0, 1, {(int8_t)x.slot}, {(int8_t)DF_X}};
+      return f;
+    };
+    for (int pc = begin; pc < end; ++pc) {
+      const tq_instr_t& I = P.code[pc];
+      const int d = I.dst & (TQ_NREG - 1);
+      const RV A = R[I.a & (TQ_NREG - 1)], B = R[I.b & (TQ_NREG - 1)];
+      RV out{UNK, 0, 0, 0, {0}, {0}};
+      switch (I.op) {
+        case TQ_OP_NOP: continue;
+        case TQ_OP_LOAD:
+          if (I.a < P.nload && P.load_dtype[I.a] == TQ_F64) out = RV{COL, I.a, 0, 0, {0}, {0}};
+          break;
+        case TQ_OP_CONST: out = RV{CST, 0, (uint64_t)I.imm, 0, {0}, {0}}; break;
+        case TQ_OP_SUB_F64:
+          if (A.kind == CST && A.bits == one && B.kind == COL)
+            out = RV{PROD, 0, 0, 1, {(int8_t)B.slot}, {(int8_t)DF_ONE_MINUS}};
+          break;
+        case TQ_OP_ADD_F64:
+          if (A.kind == CST && A.bits == one && B.kind == COL)
+            out = RV{PROD, 0, 0, 1, {(int8_t)B.slot}, {(int8_t)DF_ONE_PLUS}};
+          else if (A.kind == COL && B.kind == CST && B.bits == one)
+            out = RV{PROD, 0, 0, 1, {(int8_t)A.slot}, {(int8_t)DF_ONE_PLUS}};
+          break;
+        case TQ_OP_MUL_F64: {
+          const RV L = factor(A), Rt = factor(B);
+          if (L.kind == PROD && Rt.kind == PROD && Rt.nf == 1 && L.nf < kDirectFactors) {
+            out = L;
+            out.fs[out.nf] = Rt.fs[0];
+            out.ff[out.nf] = Rt.ff[0];
+            ++out.nf;
+          }
+          break;
+        }
+        default: break;
+      }
+      R[d] = out;
+    }
+    const RV res = factor(R[P.seg_result[a] & (TQ_NREG - 1)]);
+    if (res.kind == CST) {
+      G->nf[a] = 0;
+      G->dconst[a] = to_acc_host(G->kind[a], G->arg_dtype[a], res.bits);
+    } else if (res.kind == PROD && G->arg_dtype[a] == TQ_F64) {
+      G->nf[a] = (int8_t)res.nf;
+      for (int i = 0; i < res.nf; ++i) {
+        G->fslot[a][i] = res.fs[i];
+        G->fform[a][i] = res.ff[i];
+      }
+    } else {
+      return;
+    }
+  }
+  G->direct = 1;
 }
 
 // Two accumulators of one kind combined (no atomics).

@@ -142,7 +142,7 @@ __device__ void phase_build(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
     bool live[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) live[j] = sub0 + (int64_t)j * kBlock + t < r1;
-    stage_loads(P, B.bcols, sub0, live, stage);
+    stage_loads_by_slot(P, B.bcols, sub0, live, stage);
 #pragma unroll 1
     for (int j = 0; j < kSub; ++j) {
       if (!live[j]) continue;
@@ -196,7 +196,7 @@ __device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
     bool live[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) live[j] = sub0 + (int64_t)j * kBlock + t < r1;
-    stage_loads(P, B.pcols, sub0, live, stage);
+    stage_loads_by_slot(P, B.pcols, sub0, live, stage);
     bool hit[kSub];
     int64_t slot[kSub];
 #pragma unroll 1
@@ -222,7 +222,7 @@ __device__ void phase_probe(const tq_batch_pipe_t& B, int64_t r0, int64_t r1, ui
       for (int j = 0; j < kSub; ++j) hit[j] = b[j] != 0;
     }
     // every thread has read its own staged words; the aggregate stage reuses the buffer
-    stage_loads(A, B.acols, sub0, hit, stage);
+    stage_loads_by_slot(A, B.acols, sub0, hit, stage);
 #pragma unroll 1
     for (int j = 0; j < kSub; ++j) {
       if (!hit[j]) continue;
@@ -281,7 +281,7 @@ __device__ void phase_probe_rec(const tq_batch_pipe_t& B, int64_t c, uint64_t* s
     bool live[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) live[j] = sub0 + (int64_t)j * kBlock + t < r1;
-    stage_loads(P, B.pcols, sub0, live, stage);
+    stage_loads_by_slot(P, B.pcols, sub0, live, stage);
 #pragma unroll 1
     for (int j = 0; j < kSub; ++j) {
       bool hit = false;
@@ -336,12 +336,12 @@ __device__ void phase_probe_rec(const tq_batch_pipe_t& B, int64_t c, uint64_t* s
     }
     if (!__any(any)) continue;  // no pair in the wave: its loads are skipped (no LDS barrier here)
     // the key (load slot 0 of the predicate) of each paired row, then the aggregate arguments
-    stage_loads(P, B.pcols, sub0, hit, stage);
+    stage_loads_by_slot(P, B.pcols, sub0, hit, stage);
     uint32_t sv[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j)
       sv[j] = hit[j] ? (uint32_t)slot_of(B, (int64_t)stage[(0 * kSub + j) * kBlock + t]) : 0u;
-    stage_loads(A, B.acols, sub0, hit, stage);
+    stage_loads_by_slot(A, B.acols, sub0, hit, stage);
 #pragma unroll 1
     for (int j = 0; j < kSub; ++j) {
       if (!hit[j]) continue;
@@ -559,7 +559,12 @@ __global__ __launch_bounds__(kBlock) void k_batch(const tq_batch_pipe_t* __restr
       default: phase_write(B, lo, min(lo + tile_rows, B.range), tile, sh, wsum); break;
     }
     __syncthreads();
-    if (pub.n == kPubMax) publish(pub, ws.done);
+    // every wave reads the list length before thread 0 appends to it: a wave that read it after
+    // the append would call publish (and its barriers) alone and fall out of step with the
+    // workgroup's other waves
+    const bool full = pub.n == kPubMax;
+    __syncthreads();
+    if (full) publish(pub, ws.done);
     if (threadIdx.x == 0) {
       if (pub.n > 0 && pub.pipe[pub.n - 1] == p) {
         ++pub.count[pub.n - 1];

@@ -51,18 +51,98 @@ inline size_t stage_bytes(const tq_program_t& P) {
   return (size_t)(P.nload > 0 ? P.nload : 1) * kSubRows * sizeof(uint64_t);
 }
 
+// One load form for every column width: the aligned 8-byte word that holds value r of a w-byte
+// column (w = 1, 2, 4 or 8).  A width switch around loads of different sizes makes the compiler
+// merge their results into one register, which waits for each load before the next is issued;
+// one load form has no merge, so every slot's loads are in flight together.  An aligned 8-byte
+// word lies inside the 4 KB page of the value it holds, so it reads no page the value's own load
+// would not.  word_bits extracts the value afterwards.
+__device__ __forceinline__ uint64_t load_word(const void* col, int w, int64_t r) {
+  // pointer arithmetic from col (not an integer round trip) keeps the global address space, so
+  // the load is a global load, counted only by vmcnt
+  const uint8_t* p = (const uint8_t*)col + r * (int64_t)w;
+  return *(const uint64_t*)(p - ((uintptr_t)p & 7));
+}
+__device__ __forceinline__ uint64_t word_bits(uint64_t word, const void* col, int w, int64_t r) {
+  if (w == 8) return word;
+  const uintptr_t a = (uintptr_t)col + (uintptr_t)(r * (int64_t)w);
+  return (word >> (8 * (a & 7))) & ((1ull << (8 * w)) - 1);
+}
+
+// Raw bits of a dt-typed value -> the program's convention: integers sign-extended to 64 bits,
+// bool as 0/1, f64 as its bits, f32 as its bits in the low half.
+__device__ __forceinline__ uint64_t bits_to_value(int dt, uint64_t b) {
+  switch (dt) {
+    case TQ_BOOL: return (uint64_t)((b & 0xFFu) != 0);
+    case TQ_I8: return (uint64_t)(int64_t)(int8_t)(uint8_t)b;
+    case TQ_I16: return (uint64_t)(int64_t)(int16_t)(uint16_t)b;
+    case TQ_I32: return (uint64_t)(int64_t)(int32_t)(uint32_t)b;
+    case TQ_F32: return (uint64_t)(uint32_t)b;
+    default: return b;
+  }
+}
+
+// Every load slot's values of rows[j] (j < kSub) into the stage: all slots' loads are issued
+// before any is converted (a per-slot load-then-store would wait for each slot in turn), then
+// each value is converted and parked.  Rows with live[j] == false are zeroed (the caller points
+// them at a row that exists).
+__device__ __forceinline__ void stage_rows(const tq_program_t& P, const tq_colset_t& C,
+                                           const int64_t (&rows)[kSub], const bool (&live)[kSub],
+                                           uint64_t* __restrict__ stage) {
+  const int t = threadIdx.x;
+  static_assert(TQ_MAX_LOADS == 8, "one line per load slot below");
+  const int n = P.nload;
+  // straight-line per slot, so every index into raw is a constant whatever the inliner does
+  uint64_t raw[TQ_MAX_LOADS][kSub];
+#define TQ_ISSUE(S)                                                                 \
+  if (S < n) {                                                                      \
+    const void* col = col_ptr(C, P.load_col[S]);                                    \
+    const int w = value_bytes(P.load_dtype[S]);                                     \
+    raw[S][0] = load_word(col, w, rows[0]);                                         \
+    raw[S][1] = load_word(col, w, rows[1]);                                         \
+    raw[S][2] = load_word(col, w, rows[2]);                                         \
+    raw[S][3] = load_word(col, w, rows[3]);                                         \
+  }
+  TQ_ISSUE(0) TQ_ISSUE(1) TQ_ISSUE(2) TQ_ISSUE(3) TQ_ISSUE(4) TQ_ISSUE(5) TQ_ISSUE(6) TQ_ISSUE(7)
+#undef TQ_ISSUE
+#define TQ_PARK1(S, J) \
+  stage[(S * kSub + J) * kBlock + t] =                                                     \
+      live[J] ? bits_to_value(dt, word_bits(raw[S][J], col, w, rows[J])) : 0;
+#define TQ_PARK(S)                                                                         \
+  if (S < n) {                                                                             \
+    const void* col = col_ptr(C, P.load_col[S]);                                           \
+    const int dt = P.load_dtype[S];                                                        \
+    const int w = value_bytes(dt);                                                         \
+    TQ_PARK1(S, 0) TQ_PARK1(S, 1) TQ_PARK1(S, 2) TQ_PARK1(S, 3)                            \
+  }
+  TQ_PARK(0) TQ_PARK(1) TQ_PARK(2) TQ_PARK(3) TQ_PARK(4) TQ_PARK(5) TQ_PARK(6) TQ_PARK(7)
+#undef TQ_PARK
+#undef TQ_PARK1
+}
+
 // Phase 1: stage every load slot for the sub-tile starting at row sub0.  Rows with
-// live[j] == false (past the end, or unselected by a mask) are not read.
+// live[j] == false (past the end, or unselected by a mask) read row 0 of the column (which
+// exists whenever a sub-tile does) and are zeroed, so the loads are unconditional.
 __device__ __forceinline__ void stage_loads(const tq_program_t& P, const tq_colset_t& C,
                                             int64_t sub0, const bool (&live)[kSub],
                                             uint64_t* __restrict__ stage) {
+  int64_t rows[kSub];
+#pragma unroll
+  for (int j = 0; j < kSub; ++j) rows[j] = live[j] ? sub0 + (int64_t)j * kBlock + threadIdx.x : 0;
+  stage_rows(P, C, rows, live, stage);
+}
+
+// The same staging one load slot at a time (each slot's loads, then its values parked, then the
+// next slot's loads).  The batched kernel stages this way: its phase functions were compiled
+// and validated with it, and with stage_rows they compile as out-of-line calls.
+__device__ __forceinline__ void stage_loads_by_slot(const tq_program_t& P, const tq_colset_t& C,
+                                                    int64_t sub0, const bool (&live)[kSub],
+                                                    uint64_t* __restrict__ stage) {
   const int t = threadIdx.x;
   for (int s = 0; s < P.nload; ++s) {
     const void* col = col_ptr(C, P.load_col[s]);
     const int dt = P.load_dtype[s];
     uint64_t v[kSub];
-// Rows that are not live read row 0 of the column (which exists whenever a sub-tile does) and
-// are zeroed afterwards: unconditional loads are all in flight together.
 #define TQ_STAGE(T, EXPR)                                          \
   {                                                                \
     const T* p = (const T*)col;                                    \
@@ -248,36 +328,15 @@ __device__ __forceinline__ void run_program(const tq_program_t& P, const tq_cols
 }
 
 // Gather the program's load slots at rows[j] (live rows only) into the stage, in the layout
-// stage_loads uses, so run_rows can evaluate them.
+// stage_loads uses, so run_rows can evaluate them.  Rows that are not live read row 0.
 __device__ __forceinline__ void gather_loads(const tq_program_t& P, const tq_colset_t& C,
                                              const int64_t (&rows)[kSub],
                                              const bool (&live)[kSub],
                                              uint64_t* __restrict__ stage) {
-  const int t = threadIdx.x;
-  for (int s = 0; s < P.nload; ++s) {
-    const void* col = col_ptr(C, P.load_col[s]);
-    const int dt = P.load_dtype[s];
-    uint64_t v[kSub];
-#define TQ_GATHER(T, EXPR)                                 \
-  {                                                        \
-    const T* p = (const T*)col;                            \
-    _Pragma("unroll") for (int j = 0; j < kSub; ++j) {     \
-      const int64_t r = rows[j];                           \
-      v[j] = live[j] ? (EXPR) : 0;                         \
-    }                                                      \
-  }
-    switch (dt) {
-      case TQ_BOOL: TQ_GATHER(uint8_t, (uint64_t)(p[r] != 0)) break;
-      case TQ_I8: TQ_GATHER(int8_t, (uint64_t)(int64_t)p[r]) break;
-      case TQ_I16: TQ_GATHER(int16_t, (uint64_t)(int64_t)p[r]) break;
-      case TQ_I32: TQ_GATHER(int32_t, (uint64_t)(int64_t)p[r]) break;
-      case TQ_F32: TQ_GATHER(uint32_t, (uint64_t)p[r]) break;
-      default: TQ_GATHER(uint64_t, p[r]) break;
-    }
-#undef TQ_GATHER
+  int64_t r[kSub];
 #pragma unroll
-    for (int j = 0; j < kSub; ++j) stage[(s * kSub + j) * kBlock + t] = v[j];
-  }
+  for (int j = 0; j < kSub; ++j) r[j] = live[j] ? rows[j] : 0;
+  stage_rows(P, C, r, live, stage);
 }
 
 // The kSub rows of a thread's sub-tile together: one Regs per row (each a set of named scalars

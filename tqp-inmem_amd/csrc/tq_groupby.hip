@@ -256,19 +256,30 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
     int64_t rows[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) rows[j] = sub0 + (int64_t)j * kBlock + t;
-    RegsN R{};
-    int begin = 0;
-    for (int a = 0; a < G.naggs; ++a) {
-      const int end = P.seg_end[a];
-      run_rows(P, C, begin, end, rows, live, stage, R);
-      begin = end;
-      const V4 V = R.get(P.seg_result[a]);
-      const uint64_t* v = V.v;
+    if (G.direct) {
+      // the arguments straight from the staged loads (no interpreter)
+      for (int a = 0; a < G.naggs; ++a) {
+        uint64_t v[kSub];
+        direct_args(G, a, stage, v);
 #pragma unroll
-      for (int j = 0; j < kSub; ++j)
-        if (live[j])
-          acc_apply(&abase[j][(int64_t)a * astride[j] + aslot[j]], G.kind[a],
-                    to_acc(G.kind[a], G.arg_dtype[a], v[j]));
+        for (int j = 0; j < kSub; ++j)
+          if (live[j]) acc_apply(&abase[j][(int64_t)a * astride[j] + aslot[j]], G.kind[a], v[j]);
+      }
+    } else {
+      RegsN R{};
+      int begin = 0;
+      for (int a = 0; a < G.naggs; ++a) {
+        const int end = P.seg_end[a];
+        run_rows(P, C, begin, end, rows, live, stage, R);
+        begin = end;
+        const V4 V = R.get(P.seg_result[a]);
+        const uint64_t* v = V.v;
+#pragma unroll
+        for (int j = 0; j < kSub; ++j)
+          if (live[j])
+            acc_apply(&abase[j][(int64_t)a * astride[j] + aslot[j]], G.kind[a],
+                      to_acc(G.kind[a], G.arg_dtype[a], v[j]));
+      }
     }
   }
   if (BASE == TQ_GB_DENSE_LDS) {
@@ -573,18 +584,24 @@ __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
     RegsN R{};
     int begin = 0;
     for (int a = 0; a < G.naggs; ++a) {
-      const int end = P.seg_end[a];
-      run_rows(P, C, begin, end, rows, live, stage, R);
-      begin = end;
-      const V4 V = R.get(P.seg_result[a]);
+      uint64_t v[kSub];
+      if (G.direct) {
+        direct_args(G, a, stage, v);  // the arguments straight from the staged loads
+      } else {
+        const int end = P.seg_end[a];
+        run_rows(P, C, begin, end, rows, live, stage, R);
+        begin = end;
+        const V4 V = R.get(P.seg_result[a]);
+#pragma unroll
+        for (int j = 0; j < kSub; ++j) v[j] = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+      }
 #pragma unroll
       for (int j = 0; j < kSub; ++j) {
         if (!live[j]) continue;
-        const uint64_t v = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
         if (G.naggs == 1)
-          *(ulonglong2*)(rec + (int64_t)pos[j] * 2) = make_ulonglong2(slot[j], v);
+          *(ulonglong2*)(rec + (int64_t)pos[j] * 2) = make_ulonglong2(slot[j], v[j]);
         else
-          rec[(int64_t)pos[j] * rw + 1 + a] = v;
+          rec[(int64_t)pos[j] * rw + 1 + a] = v[j];
       }
     }
     if (G.naggs != 1) {
@@ -1067,6 +1084,7 @@ static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpe
     TQ_TRY(agg_kind_init(d->agg_op[a], d->agg_dtype[a], &G->kind[a], &G->init[a]));
   }
   if (prog && prog->nseg < d->naggs) return set_error(TQ_EINVAL, "group-by: program has %d segments for %d aggregates", prog->nseg, d->naggs);
+  if (prog) match_direct(*prog, G);
   return TQ_OK;
 }
 
