@@ -1109,6 +1109,78 @@ extern "C" int tq_topk_gather(const tq_sortkey_t* keys, int32_t nkeys, int64_t n
   return TQ_OK;
 }
 
+namespace tq {
+
+// ------------------------------------------------------------------ small sorts
+// Up to kSmallSort rows whose key fields pack into one 64-bit key sort in one workgroup by
+// rank: the packed keys go to LDS, and row i's position is the number of rows whose key is
+// smaller, plus the earlier rows with an equal key (so equal keys keep their input order: the
+// stable sort the radix passes give).  Every lane reads the same key at each step (an LDS
+// broadcast) and the count needs no barrier.  One launch, where the radix path takes its
+// passes, scans and a read-back of the key bit range.
+constexpr int kSmallSort = 2048;
+constexpr int kSmallBlock = 1024;
+
+__global__ __launch_bounds__(kSmallBlock) void k_sort_small(FieldPack fp, int64_t n, int64_t m,
+                                                            int64_t* __restrict__ perm_out) {
+  __shared__ uint64_t key[kSmallSort];
+  const int t = threadIdx.x;
+  const int nn = (int)n;
+  for (int i = t; i < nn; i += kSmallBlock) {
+    uint64_t k = 0;
+    for (int f = 0; f < fp.n; ++f) k |= field_bits(fp.f[f], i);
+    key[i] = k;
+  }
+  __syncthreads();
+  for (int i = t; i < nn; i += kSmallBlock) {
+    const uint64_t ki = key[i];
+    uint32_t r = 0;
+    int j = 0;
+    for (; j + 4 <= nn; j += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t kj = key[j + u];
+        r += (kj < ki || (kj == ki && j + u < i)) ? 1u : 0u;
+      }
+    }
+    for (; j < nn; ++j) {
+      const uint64_t kj = key[j];
+      r += (kj < ki || (kj == ki && j < i)) ? 1u : 0u;
+    }
+    if (r < m) perm_out[r] = i;
+  }
+}
+
+// The small path, when it applies (one packed key of at most 64 bits, n <= kSmallSort): true
+// in *done after launching it.
+static int sort_small(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t m,
+                      int64_t* perm_out, hipStream_t s, bool* done) {
+  *done = false;
+  if (n > kSmallSort || nkeys < 1) return TQ_OK;
+  FieldPack fp{};
+  int bits = 0;
+  for (int k = 0; k < nkeys; ++k) {
+    const int w = keys[k].width > 0 ? keys[k].width : 1;
+    if (w > 1 && keys[k].dtype != TQ_I8 && keys[k].dtype != TQ_BOOL) return TQ_OK;
+    bits += w * bits_of(keys[k].dtype);
+    if (bits > 64 || fp.n + w > 16) return TQ_OK;
+    for (int c = 0; c < w; ++c)
+      fp.f[fp.n++] = Field{keys[k].ptr, keys[k].dtype, w, c, keys[k].descending ? 1 : 0,
+                           bits_of(keys[k].dtype), 0};
+  }
+  // most significant field first: shifts from the least significant end
+  int shift = 0;
+  for (int f = fp.n - 1; f >= 0; --f) {
+    fp.f[f].shift = shift;
+    shift += fp.f[f].bits;
+  }
+  TQ_TRY(launch("k_sort_small", k_sort_small, 1, kSmallBlock, 0, s, fp, n, m, perm_out));
+  *done = true;
+  return TQ_OK;
+}
+
+}  // namespace tq
+
 extern "C" int tq_sort_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, int64_t limit,
                             int64_t* perm_out, void* ws, size_t ws_bytes, void* stream) {
   if (n <= 0) return TQ_OK;
@@ -1119,9 +1191,14 @@ extern "C" int tq_sort_perm(const tq_sortkey_t* keys, int32_t nkeys, int64_t n, 
     TQ_TRY(topk_perm(keys, nkeys, n, limit, perm_out, ws, ws_bytes, s, &done));
     if (done) return TQ_OK;
   }
+  const int64_t m = (limit >= 0 && limit < n) ? limit : n;
+  {
+    bool done;
+    TQ_TRY(sort_small(keys, nkeys, n, m, perm_out, s, &done));
+    if (done) return TQ_OK;
+  }
   uint32_t* perm;
   TQ_TRY(sort_perm_u32(keys, nkeys, n, ws, ws_bytes, s, &perm));
-  const int64_t m = (limit >= 0 && limit < n) ? limit : n;
   return launch("k_perm_out", k_perm_out, stream_grid(m), kBlock, 0, s, perm, m, perm_out);
 }
 
