@@ -97,8 +97,21 @@ __device__ __forceinline__ void direct_args(const GbSpec& G, int a, const uint64
       p[j] = i == 0 ? f : p[j] * f;
     }
   }
+  // the accumulator domain chosen once for the rows
+  switch (G.kind[a]) {
+    case ACC_COUNT:
 #pragma unroll
-  for (int j = 0; j < kSub; ++j) v[j] = to_acc(G.kind[a], TQ_F64, f64_bits(p[j]));
+      for (int j = 0; j < kSub; ++j) v[j] = (uint64_t)(p[j] != -64.0);
+      break;
+    case ACC_ISUM: case ACC_IMIN: case ACC_IMAX:
+#pragma unroll
+      for (int j = 0; j < kSub; ++j) v[j] = to_acc(G.kind[a], TQ_F64, f64_bits(p[j]));
+      break;
+    default:
+#pragma unroll
+      for (int j = 0; j < kSub; ++j) v[j] = f64_bits(p[j]);
+      break;
+  }
 }
 
 // Host twin of to_acc (for a constant argument's accumulator bits).

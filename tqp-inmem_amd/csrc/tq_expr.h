@@ -105,18 +105,28 @@ __device__ __forceinline__ void stage_rows(const tq_program_t& P, const tq_colse
   }
   TQ_ISSUE(0) TQ_ISSUE(1) TQ_ISSUE(2) TQ_ISSUE(3) TQ_ISSUE(4) TQ_ISSUE(5) TQ_ISSUE(6) TQ_ISSUE(7)
 #undef TQ_ISSUE
-#define TQ_PARK1(S, J) \
+  // the dtype is switched on once per slot, outside the rows
+#define TQ_PARK1(S, J, DT, W)                                                              \
   stage[(S * kSub + J) * kBlock + t] =                                                     \
-      live[J] ? bits_to_value(dt, word_bits(raw[S][J], col, w, rows[J])) : 0;
+      live[J] ? bits_to_value(DT, word_bits(raw[S][J], col, W, rows[J])) : 0;
+#define TQ_PARKDT(S, DT, W) \
+  TQ_PARK1(S, 0, DT, W) TQ_PARK1(S, 1, DT, W) TQ_PARK1(S, 2, DT, W) TQ_PARK1(S, 3, DT, W)
 #define TQ_PARK(S)                                                                         \
   if (S < n) {                                                                             \
     const void* col = col_ptr(C, P.load_col[S]);                                           \
-    const int dt = P.load_dtype[S];                                                        \
-    const int w = value_bytes(dt);                                                         \
-    TQ_PARK1(S, 0) TQ_PARK1(S, 1) TQ_PARK1(S, 2) TQ_PARK1(S, 3)                            \
+    switch (P.load_dtype[S]) {                                                             \
+      case TQ_BOOL: TQ_PARKDT(S, TQ_BOOL, 1) break;                                        \
+      case TQ_I8: TQ_PARKDT(S, TQ_I8, 1) break;                                            \
+      case TQ_I16: TQ_PARKDT(S, TQ_I16, 2) break;                                          \
+      case TQ_I32: TQ_PARKDT(S, TQ_I32, 4) break;                                          \
+      case TQ_F32: TQ_PARKDT(S, TQ_F32, 4) break;                                          \
+      case TQ_I64: TQ_PARKDT(S, TQ_I64, 8) break;                                          \
+      default: TQ_PARKDT(S, TQ_F64, 8) break;                                              \
+    }                                                                                      \
   }
   TQ_PARK(0) TQ_PARK(1) TQ_PARK(2) TQ_PARK(3) TQ_PARK(4) TQ_PARK(5) TQ_PARK(6) TQ_PARK(7)
 #undef TQ_PARK
+#undef TQ_PARKDT
 #undef TQ_PARK1
 }
 

@@ -209,19 +209,19 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
     bool live[kSub];
     select_rows(mask, sub0, n, live);
     stage_loads(P, C, sub0, live, stage);
-    // the table slot of every live row: LDS (dense-LDS tables and the hash front) or global
-    uint64_t* abase[kSub];
-    int64_t astride[kSub], aslot[kSub];
+    // the table slot of every live row: LDS (dense-LDS tables and the hash front; inl[j]) or
+    // global.  The accumulators are then addressed through lacc or T.acc themselves, never a
+    // pointer that may be either: an LDS-or-global pointer compiles to flat atomics, which are
+    // far slower on LDS than its own atomics.
+    bool inl[kSub];
+    int64_t aslot[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) {
-      abase[j] = T.acc;
-      astride[j] = T.slots;
+      inl[j] = BASE == TQ_GB_DENSE_LDS;
       aslot[j] = 0;
       if (!live[j]) continue;
       const uint64_t p = pack_key(G, stage, j);
       if (BASE == TQ_GB_DENSE_LDS) {
-        abase[j] = lacc;
-        astride[j] = cap;
         aslot[j] = LANES ? (int64_t)p * 64 + lane : (int64_t)p;
         atomicAdd(&lcnt[aslot[j]], 1u);
         continue;
@@ -242,8 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
           h = (h + 1) & m;
         }
         if (ls >= 0) {
-          abase[j] = lacc;
-          astride[j] = cap;
+          inl[j] = true;
           aslot[j] = ls;
           atomicAdd(&lcnt[ls], 1u);
           continue;
@@ -256,14 +255,33 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
     int64_t rows[kSub];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) rows[j] = sub0 + (int64_t)j * kBlock + t;
+    // one switch on the accumulator kind per aggregate, its four rows inside (a switch per row
+    // costs the scalar unit, which every wave of the CU shares, more than the rows' memory)
+#define TQ_ADD(J, A, V, K)                                                              \
+  if (live[J]) {                                                                        \
+    if (BASE == TQ_GB_DENSE_LDS || (FRONT && inl[J]))                                   \
+      acc_apply(lacc + (int64_t)(A) * cap + aslot[J], K, (V));                          \
+    else                                                                                \
+      acc_apply(T.acc + (int64_t)(A) * T.slots + aslot[J], K, (V));                     \
+  }
+#define TQ_ADD4(A, V, K) TQ_ADD(0, A, V[0], K) TQ_ADD(1, A, V[1], K) TQ_ADD(2, A, V[2], K) \
+  TQ_ADD(3, A, V[3], K)
+#define TQ_APPLY(A, V)                                           \
+  switch (G.kind[A]) {                                           \
+    case ACC_FSUM: TQ_ADD4(A, V, ACC_FSUM) break;                \
+    case ACC_ISUM: TQ_ADD4(A, V, ACC_ISUM) break;                \
+    case ACC_COUNT: TQ_ADD4(A, V, ACC_COUNT) break;              \
+    case ACC_FMIN: TQ_ADD4(A, V, ACC_FMIN) break;                \
+    case ACC_FMAX: TQ_ADD4(A, V, ACC_FMAX) break;                \
+    case ACC_IMIN: TQ_ADD4(A, V, ACC_IMIN) break;                \
+    default: TQ_ADD4(A, V, ACC_IMAX) break;                      \
+  }
     if (G.direct) {
       // the arguments straight from the staged loads (no interpreter)
       for (int a = 0; a < G.naggs; ++a) {
         uint64_t v[kSub];
         direct_args(G, a, stage, v);
-#pragma unroll
-        for (int j = 0; j < kSub; ++j)
-          if (live[j]) acc_apply(&abase[j][(int64_t)a * astride[j] + aslot[j]], G.kind[a], v[j]);
+        TQ_APPLY(a, v)
       }
     } else {
       RegsN R{};
@@ -273,14 +291,15 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
         run_rows(P, C, begin, end, rows, live, stage, R);
         begin = end;
         const V4 V = R.get(P.seg_result[a]);
-        const uint64_t* v = V.v;
+        uint64_t v[kSub];
 #pragma unroll
-        for (int j = 0; j < kSub; ++j)
-          if (live[j])
-            acc_apply(&abase[j][(int64_t)a * astride[j] + aslot[j]], G.kind[a],
-                      to_acc(G.kind[a], G.arg_dtype[a], v[j]));
+        for (int j = 0; j < kSub; ++j) v[j] = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+        TQ_APPLY(a, v)
       }
     }
+#undef TQ_APPLY
+#undef TQ_ADD4
+#undef TQ_ADD
   }
   if (BASE == TQ_GB_DENSE_LDS) {
     // this workgroup's table -> part[blockIdx.x][slot][count, aggregates...]
@@ -446,6 +465,21 @@ __global__ __launch_bounds__(kBlock) void k_gb_write_dense(GbSpec G, OutCols O, 
       if ((occ >> j) & 1u)
         store_int(O.key[k], G.key_dtype[k], base + rel[j],
                   key_of(G, k, (uint64_t)(tile0 + (int64_t)j * kBlock + t)));
+  }
+  if (T.slots <= kBlock) {
+    // a table of at most one slot per thread: every aggregate's accumulator is loaded before
+    // the first is written (a loop of load-then-write per aggregate waits once per aggregate)
+    uint64_t va[TQ_GB_MAX_AGGS];
+#pragma unroll
+    for (int a = 0; a < TQ_GB_MAX_AGGS; ++a)
+      va[a] = (a < G.naggs && (occ & 1u)) ? T.acc[(int64_t)a * T.slots + t] : 0ull;
+    if (occ & 1u) {
+#pragma unroll
+      for (int a = 0; a < TQ_GB_MAX_AGGS; ++a)
+        if (a < G.naggs)
+          write_agg(O.agg[a], base + rel[0], O.op[a], G.kind[a], G.arg_dtype[a], va[a], cnt[0]);
+    }
+    return;
   }
   for (int a = 0; a < G.naggs; ++a) {
     const uint64_t* acc = T.acc + (int64_t)a * T.slots;
