@@ -19,6 +19,19 @@ constexpr int kLdsBig = 152 * 1024;  // gfx950 gives a workgroup up to 160 KB of
 enum DirectForm { DF_X = 0, DF_ONE_MINUS, DF_ONE_PLUS };
 constexpr int kDirectFactors = 4;
 
+// Direct arguments (direct != 0, set by match_direct): every aggregate's argument is either a
+// constant, whose accumulator bits are dconst[a] (nf[a] == 0), or the left-to-right product of
+// nf[a] factors over f64 load slots (factor i reads slot fslot[a][i] as x and is x, 1 - x or
+// 1 + x by fform[a][i]), which the kernels evaluate from the staged loads without the
+// instruction interpreter.
+struct DirectArgs {
+  int32_t direct;
+  int8_t nf[TQ_GB_MAX_AGGS];
+  int8_t fslot[TQ_GB_MAX_AGGS][kDirectFactors];
+  int8_t fform[TQ_GB_MAX_AGGS][kDirectFactors];
+  uint64_t dconst[TQ_GB_MAX_AGGS];
+};
+
 struct GbSpec {
   int32_t nkeys, naggs;
   int32_t key_slot[TQ_GB_MAX_KEYS];  // load slot holding key i
@@ -34,16 +47,7 @@ struct GbSpec {
   const uint32_t* dict_bits[TQ_GB_MAX_KEYS];
   const uint32_t* dict_rank[TQ_GB_MAX_KEYS];
   const int64_t* dict_val[TQ_GB_MAX_KEYS];
-  // Direct arguments (direct != 0, set by match_direct): every aggregate's argument is either a
-  // constant, whose accumulator bits are dconst[a] (nf[a] == 0), or the left-to-right product of
-  // nf[a] factors over f64 load slots (factor i reads slot fslot[a][i] as x and is x, 1 - x or
-  // 1 + x by fform[a][i]), which the kernels evaluate from the staged loads without the
-  // instruction interpreter.
-  int32_t direct;
-  int8_t nf[TQ_GB_MAX_AGGS];
-  int8_t fslot[TQ_GB_MAX_AGGS][kDirectFactors];
-  int8_t fform[TQ_GB_MAX_AGGS][kDirectFactors];
-  uint64_t dconst[TQ_GB_MAX_AGGS];
+  DirectArgs D;
 };
 
 // Packed-key digit of key k's value v: its offset in the key's range, or its rank among the
@@ -79,17 +83,17 @@ __device__ __forceinline__ uint64_t to_acc(int kind, int dt, uint64_t v) {
 // The accumulator-domain values of aggregate a for the thread's kSub staged rows, evaluated
 // directly (G.direct): the same operations in the same order as the argument's program, so the
 // values are bit-identical to the interpreter's.
-__device__ __forceinline__ void direct_args(const GbSpec& G, int a, const uint64_t* stage,
-                                            uint64_t (&v)[kSub]) {
-  const int nf = G.nf[a];
+__device__ __forceinline__ void direct_args(const DirectArgs& D, int kind, int a,
+                                            const uint64_t* stage, uint64_t (&v)[kSub]) {
+  const int nf = D.nf[a];
   if (nf == 0) {
 #pragma unroll
-    for (int j = 0; j < kSub; ++j) v[j] = G.dconst[a];
+    for (int j = 0; j < kSub; ++j) v[j] = D.dconst[a];
     return;
   }
   double p[kSub];
   for (int i = 0; i < nf; ++i) {
-    const int s = G.fslot[a][i], form = G.fform[a][i];
+    const int s = D.fslot[a][i], form = D.fform[a][i];
 #pragma unroll
     for (int j = 0; j < kSub; ++j) {
       const double x = as_f64(stage[(s * kSub + j) * kBlock + threadIdx.x]);
@@ -98,14 +102,14 @@ __device__ __forceinline__ void direct_args(const GbSpec& G, int a, const uint64
     }
   }
   // the accumulator domain chosen once for the rows
-  switch (G.kind[a]) {
+  switch (kind) {
     case ACC_COUNT:
 #pragma unroll
       for (int j = 0; j < kSub; ++j) v[j] = (uint64_t)(p[j] != -64.0);
       break;
     case ACC_ISUM: case ACC_IMIN: case ACC_IMAX:
 #pragma unroll
-      for (int j = 0; j < kSub; ++j) v[j] = to_acc(G.kind[a], TQ_F64, f64_bits(p[j]));
+      for (int j = 0; j < kSub; ++j) v[j] = to_acc(kind, TQ_F64, f64_bits(p[j]));
       break;
     default:
 #pragma unroll
@@ -144,7 +148,8 @@ inline uint64_t to_acc_host(int kind, int dt, uint64_t v) {
 // constant, an f64 column, 1 - column, 1 + column, or a left-deep product of up to
 // kDirectFactors such factors (what TPC-H's sums over price, discount and tax compile to).
 // Follows the registers the way match_conj does; anything else leaves G->direct at 0.
-inline void match_direct(const tq_program_t& P, GbSpec* G) {
+inline void match_direct(const tq_program_t& P, int naggs, const int32_t* kind,
+                         const int32_t* arg_dtype, DirectArgs* D) {
   enum { UNK, COL, CST, PROD };
   struct RV {
     int kind;
@@ -158,9 +163,9 @@ inline void match_direct(const tq_program_t& P, GbSpec* G) {
     const double d = 1.0;
     memcpy(&one, &d, 8);
   }
-  G->direct = 0;
-  if (P.nseg < G->naggs) return;
-  for (int a = 0; a < G->naggs; ++a) {
+  D->direct = 0;
+  if (P.nseg < naggs) return;
+  for (int a = 0; a < naggs; ++a) {
     RV R[TQ_NREG];
     for (int i = 0; i < TQ_NREG; ++i) R[i] = RV{UNK, 0, 0, 0, {0}, {0}};
     const int begin = a == 0 ? 0 : P.seg_end[a - 1], end = P.seg_end[a];
@@ -207,19 +212,19 @@ inline void match_direct(const tq_program_t& P, GbSpec* G) {
     }
     const RV res = factor(R[P.seg_result[a] & (TQ_NREG - 1)]);
     if (res.kind == CST) {
-      G->nf[a] = 0;
-      G->dconst[a] = to_acc_host(G->kind[a], G->arg_dtype[a], res.bits);
-    } else if (res.kind == PROD && G->arg_dtype[a] == TQ_F64) {
-      G->nf[a] = (int8_t)res.nf;
+      D->nf[a] = 0;
+      D->dconst[a] = to_acc_host(kind[a], arg_dtype[a], res.bits);
+    } else if (res.kind == PROD && arg_dtype[a] == TQ_F64) {
+      D->nf[a] = (int8_t)res.nf;
       for (int i = 0; i < res.nf; ++i) {
-        G->fslot[a][i] = res.fs[i];
-        G->fform[a][i] = res.ff[i];
+        D->fslot[a][i] = res.fs[i];
+        D->fform[a][i] = res.ff[i];
       }
     } else {
       return;
     }
   }
-  G->direct = 1;
+  D->direct = 1;
 }
 
 // Two accumulators of one kind combined (no atomics).

@@ -276,11 +276,11 @@ __global__ __launch_bounds__(kBlock) void k_gb_accum(GbSpec G, tq_program_t P, t
     case ACC_IMIN: TQ_ADD4(A, V, ACC_IMIN) break;                \
     default: TQ_ADD4(A, V, ACC_IMAX) break;                      \
   }
-    if (G.direct) {
+    if (G.D.direct) {
       // the arguments straight from the staged loads (no interpreter)
       for (int a = 0; a < G.naggs; ++a) {
         uint64_t v[kSub];
-        direct_args(G, a, stage, v);
+        direct_args(G.D, G.kind[a], a, stage, v);
         TQ_APPLY(a, v)
       }
     } else {
@@ -619,8 +619,8 @@ __global__ __launch_bounds__(kBlock) void k_gbp_scatter(
     int begin = 0;
     for (int a = 0; a < G.naggs; ++a) {
       uint64_t v[kSub];
-      if (G.direct) {
-        direct_args(G, a, stage, v);  // the arguments straight from the staged loads
+      if (G.D.direct) {
+        direct_args(G.D, G.kind[a], a, stage, v);  // the arguments straight from the staged loads
       } else {
         const int end = P.seg_end[a];
         run_rows(P, C, begin, end, rows, live, stage, R);
@@ -1118,7 +1118,7 @@ static int make_spec(const tq_groupby_desc_t* d, const tq_program_t* prog, GbSpe
     TQ_TRY(agg_kind_init(d->agg_op[a], d->agg_dtype[a], &G->kind[a], &G->init[a]));
   }
   if (prog && prog->nseg < d->naggs) return set_error(TQ_EINVAL, "group-by: program has %d segments for %d aggregates", prog->nseg, d->naggs);
-  if (prog) match_direct(*prog, G);
+  if (prog) match_direct(*prog, G->naggs, G->kind, G->arg_dtype, &G->D);
   return TQ_OK;
 }
 

@@ -63,6 +63,7 @@ struct GjSpec {
   int32_t arg_dtype[TQ_GB_MAX_AGGS];
   int32_t op[TQ_GB_MAX_AGGS];
   uint64_t init[TQ_GB_MAX_AGGS];
+  DirectArgs D;  // the arguments evaluated straight from the gathered loads, when they allow it
 };
 
 struct GjOut {
@@ -297,14 +298,21 @@ __global__ __launch_bounds__(kBlock, 3) void k_gj_probe(
     RegsN R{};
     int begin = 0;
     for (int a = 0; a < G.naggs; ++a) {
-      const int end = P.seg_end[a];
-      run_rows(P, C, begin, end, rows, live, stage, R);
-      begin = end;
-      const V4 V = R.get(P.seg_result[a]);
+      uint64_t va[kSub];
+      if (G.D.direct) {
+        direct_args(G.D, G.kind[a], a, stage, va);
+      } else {
+        const int end = P.seg_end[a];
+        run_rows(P, C, begin, end, rows, live, stage, R);
+        begin = end;
+        const V4 V = R.get(P.seg_result[a]);
+#pragma unroll
+        for (int j = 0; j < kSub; ++j) va[j] = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+      }
 #pragma unroll
       for (int j = 0; j < kSub; ++j) {
         if (!live[j]) continue;
-        const uint64_t v = to_acc(G.kind[a], G.arg_dtype[a], V.v[j]);
+        const uint64_t v = va[j];
         const int64_t e = e0 + j * kBlock + t;
         if (G.naggs == 1)
           *(ulonglong2*)(crec + e * 2) = make_ulonglong2(slot[j], v);
@@ -816,6 +824,7 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
       if (d->out_cap > 0 && !O.agg[a])
         return set_error(TQ_EINVAL, "groupjoin: null aggregate output");
     }
+    match_direct(*prog, G.naggs, G.kind, G.arg_dtype, &G.D);
     O.key = d->out_key;
     O.key_dtype = d->out_key_dtype;
     O.cap = d->out_cap;
