@@ -255,7 +255,9 @@ def batched_section(pipes_n, dev):
             "batched_rows_per_s": rows / t_batch, "plain_rows_per_s": rows / t_plain,
             "speedup": t_plain / t_batch, "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
             "identical_results": bool(same), "rerun_plain": rerun,
-            "tile_rows": tune["tile_rows"], "grid": tune["grid"]}
+            "tile_rows": tune["tile_rows"], "grid": tune["grid"],
+            "large_tables": "records" if tune.get("records") else "atomics",
+            "tuning_ms": {k: round(v * 1e3, 3) for k, v in tune.get("timings", {}).items()}}
 
 
 def main():
