@@ -623,37 +623,67 @@ __global__ __launch_bounds__(kBlock) void k_join_write_unique(
   }
 }
 
-// Write pass for repeated build keys: repeats the lookups and expands each probe row's bucket.
+// Write pass for repeated build keys.  The lookups are repeated, then the tile's pairs are
+// spread over the whole workgroup in output order: each probe row's first output (within the
+// tile) and bucket start go to LDS, and thread t writes the tile's outputs t, t + 256, ...,
+// finding the probe row of each by a search over the first outputs (forward from the row of
+// its previous output).  A probe row whose key repeats many times in the build is expanded by
+// all 256 threads with consecutive stores, not by one lane.  Output order: probe rows
+// ascending, each row's build rows in bucket order.
 template <typename K, bool HASH, bool UNIQUE, bool NEED_ROW>
 __global__ __launch_bounds__(kBlock) void k_join_write(
     JoinView J, const void* __restrict__ pkeys, int pdt, const uint8_t* __restrict__ pmask,
     int64_t np, int64_t ntiles, const int64_t* __restrict__ tile_off, ColPack L, ColPack Rc,
     int64_t* __restrict__ build_idx, int64_t* __restrict__ probe_idx) {
   __shared__ uint32_t sh[kWaves];
+  __shared__ uint32_t first[kTileRows + 1];  // row's first output in the tile; [4096] = total
+  __shared__ uint32_t bucket[kTileRows];     // row's first position in J.rows
+  const int t = threadIdx.x;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t c[kItems], a[kItems];
     probe_rows<K, HASH, UNIQUE, true>(J, pkeys, pdt, pmask, np, tile, c, a);
-    int64_t base = tile_off[tile];
+    uint32_t base = 0;
 #pragma unroll
     for (int q = 0; q < kRuns; ++q) {
       const uint32_t tc = c[4 * q] + c[4 * q + 1] + c[4 * q + 2] + c[4 * q + 3];
       uint32_t tot;
-      const uint32_t ex = block_exclusive_sum(tc, sh, &tot);
-      int64_t o = base + ex;
+      uint32_t o = base + block_exclusive_sum(tc, sh, &tot);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int j = 4 * q + i;
-        const int64_t r = tile_row(tile, j);
-        for (uint32_t k = 0; k < c[j]; ++k, ++o) {
-          const uint32_t b = J.rows[a[j] + k];
-          for (int x = 0; x < L.n; ++x) copy_row(L.dst[x], o, L.src[x], b, L.bytes[x]);
-          for (int x = 0; x < Rc.n; ++x) copy_row(Rc.dst[x], o, Rc.src[x], r, Rc.bytes[x]);
-          if (build_idx) build_idx[o] = b;
-          if (probe_idx) probe_idx[o] = r;
-        }
+        const int lr = q * (4 * kBlock) + 4 * t + i;  // tile_row(tile, 4q + i) - tile * 4096
+        first[lr] = o;
+        bucket[lr] = a[4 * q + i];
+        o += c[4 * q + i];
       }
       base += tot;
     }
+    if (t == 0) first[kTileRows] = base;
+    __syncthreads();
+    const int64_t obase = tile_off[tile];
+    const int64_t row0 = tile * kTileRows;
+    int r = 0;  // first[r] <= o for every output o this thread still writes
+    for (uint32_t o = t; o < base; o += kBlock) {
+      // the row whose outputs [first[r], first[r + 1]) hold o: gallop from r, then bisect
+      int lo = r, step = 1;
+      while (lo + step < kTileRows && first[lo + step] <= o) {
+        lo += step;
+        step <<= 1;
+      }
+      int hi = lo + step < kTileRows ? lo + step : kTileRows;  // first[hi] > o
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= o) lo = mid; else hi = mid;
+      }
+      r = lo;
+      const uint32_t b = J.rows[bucket[r] + (o - first[r])];
+      const int64_t pr = row0 + r;
+      const int64_t out = obase + o;
+      for (int x = 0; x < L.n; ++x) copy_row(L.dst[x], out, L.src[x], b, L.bytes[x]);
+      for (int x = 0; x < Rc.n; ++x) copy_row(Rc.dst[x], out, Rc.src[x], pr, Rc.bytes[x]);
+      if (build_idx) build_idx[out] = b;
+      if (probe_idx) probe_idx[out] = pr;
+    }
+    __syncthreads();  // the next tile rewrites first[] and bucket[]
   }
 }
 
