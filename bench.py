@@ -44,9 +44,10 @@ def parse():
                          "next is prepared)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once by a plain copy kernel (PMC traffic calibration)")
-    ap.add_argument("--only", type=int, choices=(0, 2, 3), default=None,
-                    help="run only the config-0, config-2 or config-3 (batched) section and print "
-                         "its JSON (for per-workload profiles)")
+    ap.add_argument("--only", type=int, choices=(0, 2, 3, 5), default=None,
+                    help="run only the config-0, config-2, config-3 (batched) or the extra "
+                         "(5: repeated-key join, large sorts) section and print its JSON (for "
+                         "per-workload profiles)")
     return ap.parse_args()
 
 
@@ -205,6 +206,94 @@ def configs_section(dev, steps=20, warmup=5, cfgs=(0, 2)):
     return out
 
 
+EXTRA_WORKLOADS = {
+    "join_repeated_keys": "inner join, build 2,000,000 i32 keys (~4 rows per key, one hot key on "
+                          "4,000 rows), probe 20,000,000 i32 keys (1,000 on the hot key); "
+                          "outputs build key + i64 payload, probe key + f64 payload",
+    "sort_16M_f64_i32": "full stable sort of 16,000,000 rows by (f64 descending, i32 ascending)",
+    "sort_64M_i64": "full stable sort of 64,000,000 i64 keys",
+}
+
+
+def extra_section(dev, reps=5):
+    """Workloads no config line covers: an inner join whose build keys repeat (with a hot key)
+    and full LSD radix sorts above a million rows.  Per workload: wall time of one call
+    (synchronised), device time of its kernels (dispatch-stamped) and the achieved bandwidth of
+    the kernels with stated algorithmic bytes.  A radix scatter pass reads and writes a (key,
+    row) pair per row: n * 2 * (key bytes + 4)."""
+    import numpy as np
+    import torch
+    from tqp_amd import _lib, kernels
+    g = np.random.default_rng(11)
+    out = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        _lib.start_trace()
+        for _ in range(reps):
+            fn()
+        calls, launches = _lib.stop_trace()
+        return wall, kernel_table(calls, launches), launches
+
+    nb, np_ = 2_000_000, 20_000_000
+    bk = g.integers(1, 500_000, nb, dtype=np.int32)
+    bk[g.choice(nb, 4000, replace=False)] = 0
+    pk = g.integers(1, 600_000, np_, dtype=np.int32)
+    pk[g.choice(np_, 1000, replace=False)] = 0
+    bk_t, pk_t = torch.from_numpy(bk).to(dev), torch.from_numpy(pk).to(dev)
+    bv = torch.from_numpy(g.integers(0, 1 << 40, nb, dtype=np.int64)).to(dev)
+    pv = torch.from_numpy(g.random(np_)).to(dev)
+    pairs = [0]
+
+    def join():
+        idx = kernels.JoinIndex(bk_t)
+        lo, ro, _, _ = idx.probe(pk_t, None, [bk_t, bv], [pk_t, pv])
+        pairs[0] = int(lo[0].shape[0])
+
+    wall, kt, _ = timed(join)
+    out["join_repeated_keys"] = {"workload": EXTRA_WORKLOADS["join_repeated_keys"],
+                                 "pairs": pairs[0], "ms_per_join": wall * 1e3,
+                                 "kernel_ms_per_join": round(sum(v["total_ms"] for v in
+                                                                 kt.values()) / reps, 4),
+                                 "kernels": dict(list(kt.items())[:6])}
+    del bk_t, pk_t, bv, pv
+    for name, n, make in (
+            ("sort_16M_f64_i32", 16_000_000,
+             lambda n: [(torch.from_numpy(g.random(n)).to(dev), True),
+                        (torch.from_numpy(g.integers(-1000, 1000, n, dtype=np.int32)).to(dev),
+                         False)]),
+            ("sort_64M_i64", 64_000_000,
+             lambda n: [(torch.from_numpy(g.integers(-(1 << 62), 1 << 62, n,
+                                                     dtype=np.int64)).to(dev), False)])):
+        keys = make(n)
+        wall, kt, launches = timed(lambda: kernels.sort_perm(keys, n))
+        # a scatter pass of 64-bit keys moves 8 + 4 bytes a row in and out (the f64 + i32 sort
+        # mixes 64- and 32-bit passes, so only its rows/s is stated)
+        kb = 8
+        sc = [ms for k, ms in launches if k.startswith("k_radix_scatter")] \
+            if name == "sort_64M_i64" else []
+        row = {"workload": EXTRA_WORKLOADS[name], "rows": n, "ms_per_sort": wall * 1e3,
+               "rows_per_s": n / wall,
+               "kernel_ms_per_sort": round(sum(v["total_ms"] for v in kt.values()) / reps, 4),
+               "kernels": dict(list(kt.items())[:6])}
+        if sc:
+            avg = sum(sc) / len(sc)
+            gbs = n * 2 * (kb + 4) / (avg * 1e-3) / 1e9
+            row["scatter_pass"] = {"passes_per_sort": len(sc) / reps, "avg_us": round(avg * 1e3, 2),
+                                   "alg_bytes": n * 2 * (kb + 4), "GBps": round(gbs, 1),
+                                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        out[name] = row
+        del keys
+        torch.cuda.empty_cache()
+    return out
+
+
 def batched_section(pipes_n, dev):
     """Config 3: the same pipelines as one persistent batched launch and as plain per-operator
     launches; rows/s of both and whether every result matched."""
@@ -218,7 +307,9 @@ def batched_section(pipes_n, dev):
     specs = [batched.match_plan(p, g) for g, p in pipes]
     tune = batched.autotune(specs, key=f"{torch.cuda.get_device_name(0)}|config3|{pipes_n}")
     eng = batched.BatchedEngine(specs, tile_rows=tune["tile_rows"], grid=tune["grid"],
-                                records=tune.get("records", False))
+                                records=tune.get("records", False),
+                                lds_table_max=tune.get("lds_table_max"),
+                                reduce_lds=tune.get("reduce_lds"))
     eng.run()
     out, rerun = eng.results()
     reps = 5
@@ -257,6 +348,7 @@ def batched_section(pipes_n, dev):
             "identical_results": bool(same), "rerun_plain": rerun,
             "tile_rows": tune["tile_rows"], "grid": tune["grid"],
             "large_tables": "records" if tune.get("records") else "atomics",
+            "lds_table_max": tune.get("lds_table_max"), "reduce_lds": tune.get("reduce_lds"),
             "tuning_ms": {k: round(v * 1e3, 3) for k, v in tune.get("timings", {}).items()}}
 
 
@@ -279,9 +371,11 @@ def main():
     if args.only is not None:
         if args.only == 3:
             sec = batched_section(args.batch_pipes, dev)
+        elif args.only == 5:
+            sec = extra_section(dev)
         else:
             sec = configs_section(dev, steps=args.steps, warmup=args.warmup, cfgs=(args.only,))
-        print(json.dumps(sec if args.only != 3 else {"config3": sec}), flush=True)
+        print(json.dumps({"config3": sec} if args.only == 3 else sec), flush=True)
         return
     tables = workloads.config1_tables()
     plan = workloads.config1_plan()
@@ -399,6 +493,7 @@ def main():
     }
     if not args.no_configs:
         line["configs"] = configs_section(dev)
+        line["extra"] = extra_section(dev)
     if not args.no_batched:
         line["batched"] = batched_section(args.batch_pipes, dev)
     if args.serve:

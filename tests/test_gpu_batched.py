@@ -48,6 +48,31 @@ def test_batched_matches_plain(device, records, fast):
                                        rtol=1e-12, atol=0, err_msg=f"sums of pipeline {i}")
 
 
+@pytest.mark.parametrize("tmax,rlds", [(16 * 1024, 16 * 1024), (48 * 1024, 32 * 1024)])
+def test_batched_lds_capacities(device, tmax, rlds):
+    """The LDS capacities the autotuner chooses among (the whole-table limit, the records slice
+    table) change where pipelines accumulate, not their results."""
+    from tqp_amd import batched, workloads
+    probe, build, consts = workloads.config3_sizes(n_pipes=12, lo=1000, hi=200_000)
+    pipes = [workloads.config3_pipeline(i, probe[i], build[i], consts[i][0], consts[i][1], device)
+             for i in range(len(probe))]
+    specs = [batched.match_plan(plan, group) for group, plan in pipes]
+    eng = batched.BatchedEngine(specs, tile_rows=2048, records=True, lds_table_max=tmax,
+                                reduce_lds=rlds)
+    print("LDS table bytes", eng.lds_table_bytes, "records pipelines",
+          sum(int(d.records) for d in eng._desc_host[:eng.npipes]), flush=True)
+    eng.run()
+    out, rerun = eng.results()
+    assert rerun == []
+    for i, (group, plan) in enumerate(pipes):
+        ref = _plain(group, plan)
+        got = out[i]
+        assert torch.equal(got[101].cpu(), ref[101].tensor.cpu()), f"keys of pipeline {i}"
+        assert torch.equal(got[302].cpu(), ref[302].tensor.cpu()), f"counts of pipeline {i}"
+        np.testing.assert_allclose(got[301].cpu().numpy(), ref[301].tensor.cpu().numpy(),
+                                   rtol=1e-12, atol=0, err_msg=f"sums of pipeline {i}")
+
+
 def test_batched_graph_replay(device):
     """The launch sequence captured in a graph replays to the eager result."""
     from tqp_amd import batched, workloads

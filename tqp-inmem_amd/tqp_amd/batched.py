@@ -206,9 +206,13 @@ class BatchedEngine:
     """Prepare once, then run() (graph-capturable) and results()."""
 
     def __init__(self, specs, tile_rows=4096, grid=None, wait_us=100_000, key_bounds=None,
-                 records=False, fast=True):
+                 records=False, fast=True, lds_table_max=None, reduce_lds=None):
         self.specs = list(specs)
         self.records = records    # True: large join tables accumulate through records
+        # LDS capacities: a pipeline whose whole table (4 + 8 * aggregates bytes a key) fits
+        # lds_table_max is accumulated in LDS; a records pipeline's slice table fits reduce_lds
+        self.lds_table_max = int(lds_table_max or LDS_TABLE_MAX)
+        self.reduce_lds = int(reduce_lds or REDUCE_LDS)
         self.fast = fast          # False: every predicate runs on the per-row interpreter
         self.tile_rows = int(tile_rows)
         self.device = torch.device("cuda:0")
@@ -305,26 +309,30 @@ class BatchedEngine:
             lay.update(ab=ab, arg_dtypes=arg_dtypes)
             naggs = len(sp.aggs)
             tbytes = lay["rng"] * (4 + 8 * naggs)
-            lay["lds"] = 1 if tbytes <= LDS_TABLE_MAX else 0
+            lay["lds"] = 1 if tbytes <= self.lds_table_max else 0
             lay["records"] = 0
             if lay["lds"]:
                 lds_table = max(lds_table, tbytes)
             elif sp.mode == 0 and self.records:
                 # records: the largest slice whose LDS table fits, if the slices are few enough
                 sb = 12
-                while sb > 6 and (1 << sb) * (4 + 8 * naggs) > REDUCE_LDS:
+                while sb > 6 and (1 << sb) * (4 + 8 * naggs) > self.reduce_lds:
                     sb -= 1
                 nsl = _ceil(lay["rng"], 1 << sb)
-                if (1 << sb) * (4 + 8 * naggs) <= REDUCE_LDS and nsl <= MAX_SLICES:
+                if (1 << sb) * (4 + 8 * naggs) <= self.reduce_lds and nsl <= MAX_SLICES:
                     lay.update(records=1, sb=sb, nslices=nsl)
                     lds_table = max(lds_table, (CHUNK // 32 + MAX_SLICES) * 4,
                                     (1 << sb) * (4 + 8 * naggs))
             stage_loads = max(stage_loads, lay["bb"].nload, lay["pb"].nload, ab.nload)
             layout.append(lay)
-        if lds_table > LDS_EXTRA_MAX or stage_loads * 8 * 1024 + lds_table > 64 * 1024:
-            # the records scratch does not fit beside the stage: atomics for those pipelines
+        budget = min(LDS_EXTRA_MAX, 64 * 1024 - stage_loads * 8 * 1024)
+        if lds_table > budget:
+            # the records scratch does not fit beside the stage: atomics for those pipelines,
+            # and whole tables that do not fit beside it accumulate in global memory
             for lay in layout:
                 lay["records"] = 0
+                if lay["lds"] and lay["rng"] * (4 + 8 * len(self.specs[lay["i"]].aggs)) > budget:
+                    lay["lds"] = 0
             lds_table = max([lay["rng"] * (4 + 8 * len(self.specs[lay["i"]].aggs))
                              for lay in layout if lay["lds"]] or [0])
         self.stage_loads = stage_loads
@@ -499,10 +507,14 @@ def _align(n, a=256):
 TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
-def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None):
+def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None,
+             lds_candidates=None):
     """Pick tile rows, persistent grid size and the accumulation of large tables (global atomics
-    or records with slice reduces) for a batch by timing candidates on it.  Results are cached
-    in tuning.json under `key` (device name + workload signature)."""
+    or records with slice reduces) for a batch by timing candidates on it, then, for that
+    choice, the LDS capacities (whole-table limit, records slice table) from `lds_candidates`.
+    Results are cached in tuning.json under `key` (device name + workload signature).  The
+    batched path has no radix pass (tables are addressed by key - min), so there are no radix
+    bits to choose."""
     dev = torch.device("cuda:0")
     props = torch.cuda.get_device_properties(dev)
     key = key or f"{props.name}|{len(specs)}"
@@ -532,7 +544,28 @@ def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None):
         if best is None or dt < best[0]:
             best = (dt, tile, grid, records)
         del eng
-    choice = {"tile_rows": best[1], "grid": best[2], "records": best[3], "seconds": best[0],
+    lds_candidates = lds_candidates or [(t, r) for t in (16 * 1024, 32 * 1024, 48 * 1024)
+                                        for r in ((16 * 1024, 32 * 1024) if best[3] else
+                                                  (REDUCE_LDS,))]
+    best_lds = (best[0], LDS_TABLE_MAX, REDUCE_LDS)
+    for tmax, rlds in lds_candidates:
+        if (tmax, rlds) == (LDS_TABLE_MAX, REDUCE_LDS):
+            continue          # timed above
+        eng = BatchedEngine(specs, tile_rows=best[1], grid=best[2], key_bounds=bounds,
+                            records=best[3], lds_table_max=tmax, reduce_lds=rlds)
+        eng.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            eng.run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / repeats
+        timings[f"{best[1]}x{best[2]}{'r' if best[3] else 'a'} lds{tmax >> 10}k/{rlds >> 10}k"] = dt
+        if dt < best_lds[0]:
+            best_lds = (dt, tmax, rlds)
+        del eng
+    choice = {"tile_rows": best[1], "grid": best[2], "records": best[3],
+              "lds_table_max": best_lds[1], "reduce_lds": best_lds[2], "seconds": best_lds[0],
               "timings": timings}
     table[key] = choice
     if cache:

@@ -249,6 +249,16 @@ def _copy_args(x):
     return x
 
 
+def _snapshot_args(ja):
+    """The join's arguments as they are now, safe from the in-place edits a join makes to them
+    (join_vortex pops 'mask map' entries, hashjoin.py:196-197): a new dict whose dict and list
+    values are copied one level deep.  The fused path only takes joins without conditions, whose
+    other values are flat lists of ids and names; the replay deep-copies the snapshot before
+    running the join."""
+    return {k: dict(v) if type(v) is dict else list(v) if type(v) is list else v
+            for k, v in ja.items()}
+
+
 # ---------------------------------------------------------------- lazily materialised join outputs
 class _JoinReplay:
     """Runs the join itself (operators/hashjoin.py:179-248) on the state saved before the fused
@@ -434,7 +444,7 @@ def run(tensor_group, ja, aa, p, run_join):
     # reassign their fields
     join_ids = set(ja['left group']) | set(ja['right group']) | set(ja.get('mask map', {}).values())
     replay = _JoinReplay(run_join, {k: _clone(v) for k, v in tensor_group.items() if k in join_ids},
-                         _copy_args(ja))
+                         _snapshot_args(ja))
 
     # ---- side effects of join_vortex (hashjoin.py:198-238): mask map pops, outputs replaced
     lgroup, rgroup = ja['left group'], ja['right group']
