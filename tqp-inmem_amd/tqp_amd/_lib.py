@@ -354,9 +354,18 @@ def workspace(nbytes: int, device, role=None) -> torch.Tensor:
         return arena.workspace(nbytes, role)
     if role is None:
         return torch.empty(nbytes, dtype=torch.uint8, device=device)
-    key = (role, str(device), torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
-    buf = _WORKSPACES.get(key)
-    if buf is None or buf.numel() < nbytes:
+    # the buffer is kept per device and per the current device's current stream
+    cur = torch._C._cuda_getDevice()
+    key = (role, device, cur, torch._C._cuda_getCurrentRawStream(cur))
+    ent = _WORKSPACES.get(key)
+    if ent is None or ent[1] < nbytes:
         buf = torch.empty(nbytes + nbytes // 4, dtype=torch.uint8, device=device)
-        _WORKSPACES[key] = buf
-    return buf[:nbytes]
+        ent = (buf, buf.numel(), {})
+        _WORKSPACES[key] = ent
+    views = ent[2]             # views of the buffer by size, made once
+    v = views.get(nbytes)
+    if v is None:
+        if len(views) > 64:
+            views.clear()
+        v = views[nbytes] = ent[0][:nbytes]
+    return v

@@ -6,6 +6,7 @@ Outputs are allocated here with torch's caching allocator, workspaces likewise; 
 launches kernels on the current stream.
 """
 import ctypes
+import functools
 import threading
 import weakref
 
@@ -147,6 +148,11 @@ def eval_program(b: E.ProgramBuilder, n: int, out_dtype, device):
 
 
 # ---------------------------------------------------------------- sort
+@functools.lru_cache(maxsize=256)
+def _sort_ws_bytes(n: int) -> int:
+    return L.size_call("tq_sort_ws_bytes", n)
+
+
 def sort_perm(keys, n: int, limit=None, device=None):
     """Stable multi-key sort permutation (int64[min(n, limit)]).  keys: list of
     (tensor, descending) from most to least significant; 2-D int8 keys compare row-wise.
@@ -182,7 +188,7 @@ def sort_perm(keys, n: int, limit=None, device=None):
         arr[i].dtype = L.dtype_code(t.dtype)
         arr[i].width = t.shape[1] if t.dim() == 2 else 1
         arr[i].descending = 1 if desc else 0
-    ws_bytes = L.size_call("tq_sort_ws_bytes", n)
+    ws_bytes = _sort_ws_bytes(n)
     ws = L.workspace(ws_bytes, dev, "sort")
     kb = sum(L.row_bytes(t) for t in hold)
     # top-k rounds read every key once (later rounds read the few survivors); a packing pass
@@ -240,7 +246,7 @@ def take_ordered_deferred(keys, cols, n: int, limit: int, device):
         gathered.append(o)
         src[i], dst[i], rb[i] = c.data_ptr(), o.data_ptr(), L.row_bytes(c)
     perm = L.workspace(8 * m, device, "topk_perm")
-    ws_bytes = L.size_call("tq_sort_ws_bytes", n)
+    ws_bytes = _sort_ws_bytes(n)
     ws = L.workspace(ws_bytes, device, "sort")
     applied = ctypes.c_int32()
     at = L.trace_len()
