@@ -630,11 +630,13 @@ __global__ __launch_bounds__(kBlock) void k_join_write_unique(
 // its previous output).  A probe row whose key repeats many times in the build is expanded by
 // all 256 threads with consecutive stores, not by one lane.  Output order: probe rows
 // ascending, each row's build rows in bucket order.
+// With bypos, L's sources are the build columns already in bucket order (k_join_bucket_cols),
+// indexed by sorted position: a probe row's pairs read one short run of each.
 template <typename K, bool HASH, bool UNIQUE, bool NEED_ROW>
 __global__ __launch_bounds__(kBlock) void k_join_write(
     JoinView J, const void* __restrict__ pkeys, int pdt, const uint8_t* __restrict__ pmask,
     int64_t np, int64_t ntiles, const int64_t* __restrict__ tile_off, ColPack L, ColPack Rc,
-    int64_t* __restrict__ build_idx, int64_t* __restrict__ probe_idx) {
+    int64_t* __restrict__ build_idx, int64_t* __restrict__ probe_idx, int bypos) {
   __shared__ uint32_t sh[kWaves];
   __shared__ uint32_t first[kTileRows + 1];  // row's first output in the tile; [4096] = total
   __shared__ uint32_t bucket[kTileRows];     // row's first position in J.rows
@@ -675,15 +677,30 @@ __global__ __launch_bounds__(kBlock) void k_join_write(
         if (first[mid] <= o) lo = mid; else hi = mid;
       }
       r = lo;
-      const uint32_t b = J.rows[bucket[r] + (o - first[r])];
+      const uint32_t pos = bucket[r] + (o - first[r]);
+      const uint32_t b = (!bypos || build_idx) ? J.rows[pos] : 0u;
+      const int64_t li = bypos ? (int64_t)pos : (int64_t)b;
       const int64_t pr = row0 + r;
       const int64_t out = obase + o;
-      for (int x = 0; x < L.n; ++x) copy_row(L.dst[x], out, L.src[x], b, L.bytes[x]);
+      for (int x = 0; x < L.n; ++x) copy_row(L.dst[x], out, L.src[x], li, L.bytes[x]);
       for (int x = 0; x < Rc.n; ++x) copy_row(Rc.dst[x], out, Rc.src[x], pr, Rc.bytes[x]);
       if (build_idx) build_idx[out] = b;
       if (probe_idx) probe_idx[out] = pr;
     }
     __syncthreads();  // the next tile rewrites first[] and bucket[]
+  }
+}
+
+// Build-side columns of a repeated-key join in bucket order: B.dst[x][p] = L.src[x][rows[p]]
+// for the nsel sorted positions.  One pass over the selected build rows per probe, after which
+// the write pass reads a probe row's build values as one short run per column instead of one
+// random line per pair.
+__global__ __launch_bounds__(kBlock) void k_join_bucket_cols(const uint32_t* __restrict__ rows,
+                                                             int64_t nsel, ColPack L, ColPack B) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nsel; p += stride) {
+    const uint32_t r = rows[p];
+    for (int x = 0; x < L.n; ++x) copy_row(B.dst[x], p, L.src[x], r, L.bytes[x]);
   }
 }
 
@@ -982,9 +999,32 @@ static int write_impl(const tq_join_state_t* st, const void* pkeys, int32_t pdt,
     return launch("k_join_write", (k_join_write_unique<K, HASH, false>),
                   resident_grid(k_join_write_unique<K, HASH, false>, 0, nt), kBlock, 0, s, J,
                   pkeys, pdt, nt, (const uint16_t*)w.hits, off, L, R, bidx, pidx);
+  // repeated keys: the build columns go to bucket order first when they fit the build
+  // workspace's sort-key buffers (free once the index is built: only the sorted rows and the
+  // directory are read after it)
+  ColPack Lw = L;
+  int bypos = 0;
+  if (L.n > 0 && st->nsel > 0) {
+    const BuildWs bw = carve_build(st->ws, st->nb);
+    const size_t avail = (size_t)((char*)bw.v0 - (char*)bw.k0);
+    size_t need = 0;
+    for (int x = 0; x < L.n; ++x) need += align_up((size_t)st->nsel * L.bytes[x], 256);
+    if (need <= avail) {
+      ColPack B = L;
+      char* p = (char*)bw.k0;
+      for (int x = 0; x < L.n; ++x) {
+        B.dst[x] = p;
+        Lw.src[x] = p;
+        p += align_up((size_t)st->nsel * L.bytes[x], 256);
+      }
+      TQ_TRY(launch("k_join_bucket_cols", k_join_bucket_cols, stream_grid(st->nsel), kBlock, 0,
+                    s, (const uint32_t*)st->rows, st->nsel, L, B));
+      bypos = 1;
+    }
+  }
   return launch("k_join_write", (k_join_write<K, HASH, false, true>),
                 resident_grid(k_join_write<K, HASH, false, true>, 0, nt), kBlock, 0, s, J, pkeys,
-                pdt, pmask, np, nt, off, L, R, bidx, pidx);
+                pdt, pmask, np, nt, off, Lw, R, bidx, pidx, bypos);
 }
 
 // Calls F<K, HASH, UNIQUE>::run(args...) for the state's key width and layout.
