@@ -205,6 +205,35 @@ def test_several_aggregates_either_join_key(device, monkeypatch, group_key, key_
     assert n == ncpu
 
 
+def test_counts_of_constants_come_from_pair_counts(device, monkeypatch):
+    """count(1) (twice) next to a sum of an expression and a count of a column: the counts of a
+    constant have no record word (they are the slots' pair counts), the others keep theirs, and
+    every column equals the oracle's."""
+    from tqp_amd import groupjoin
+    g = np.random.default_rng(77)
+    tables = _multi_agg_tables(g, 120_000, 500_000)
+    results = ['key#101',
+               [[[['sum', [['price#202', '*', ['1.0', '-', 'discount#203']]]], '#', '40'], 'AS', 'rev#301']],
+               [[[['count', ['1']], '#', '41'], 'AS', 'cnt#302']],
+               [[[['count', ['discount#203']], '#', '42'], 'AS', 'cntd#303']],
+               [[[['count', ['1']], '#', '43'], 'AS', 'cnt2#304']]]
+    plan = _multi_agg_plan()
+    plan[3][1]['input ids'] = [101, 202, 203]
+    plan[3][1]['results'] = results
+    plan[3][1]['out agg ids'] = [301, 302, 303, 304]
+    groupjoin._REC_WORDS.clear()
+    seen = _Seen(monkeypatch)
+    gpu, n = _run(tables, plan, device)
+    assert seen.seen == ["fused"], seen.seen
+    assert list(groupjoin._REC_WORDS.values()) == [3]   # slot, revenue, count(discount)
+    cpu, ncpu = _oracle(tables, plan)
+    _assert_col(gpu[101].tensor, cpu[101].tensor, "group keys")
+    for k in (302, 303, 304):
+        _assert_col(gpu[k].tensor, cpu[k].tensor, f"agg {k}")
+    _assert_col(gpu[301].tensor, cpu[301].tensor, "agg 301", fp=True)
+    assert n == ncpu
+
+
 def test_case_when_argument(device, monkeypatch):
     """A CASE WHEN argument (aggregate.py:74-85) over probe columns, fused."""
     g = np.random.default_rng(72)

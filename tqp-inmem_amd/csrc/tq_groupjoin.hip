@@ -64,6 +64,13 @@ struct GjSpec {
   int32_t op[TQ_GB_MAX_AGGS];
   uint64_t init[TQ_GB_MAX_AGGS];
   DirectArgs D;  // the arguments evaluated straight from the gathered loads, when they allow it
+  // Accumulated aggregates: a count of a constant that is not the null sentinel equals the
+  // slot's pair count, which the reduce keeps anyway, so it has no record word and no table
+  // column (word[a] = -1); the others are record word / table column word[a] < nacc, and
+  // acc_agg[w] is the aggregate of word w.
+  int32_t nacc;
+  int32_t word[TQ_GB_MAX_AGGS];
+  int32_t acc_agg[TQ_GB_MAX_AGGS];
 };
 
 struct GjOut {
@@ -280,7 +287,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_gj_probe(
         srt[atomicAdd(&hist[slot >> sb], 1u)] = (lrow << sb) | (slot & smask);
       }
   __syncthreads();
-  const int rw = 1 + G.naggs;
+  const int rw = 1 + G.nacc;
   uint64_t* crec = rec + base * rw;
   for (uint32_t e0 = 0; e0 < M; e0 += kSubRows) {
     bool live[kSub];
@@ -298,6 +305,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_gj_probe(
     RegsN R{};
     int begin = 0;
     for (int a = 0; a < G.naggs; ++a) {
+      const int wd = G.word[a];
+      if (wd < 0) continue;  // a count the reduce takes from the pair counts (direct form only)
       uint64_t va[kSub];
       if (G.D.direct) {
         direct_args(G.D, G.kind[a], a, stage, va);
@@ -314,13 +323,13 @@ __global__ __launch_bounds__(kBlock, 3) void k_gj_probe(
         if (!live[j]) continue;
         const uint64_t v = va[j];
         const int64_t e = e0 + j * kBlock + t;
-        if (G.naggs == 1)
+        if (G.nacc == 1)
           *(ulonglong2*)(crec + e * 2) = make_ulonglong2(slot[j], v);
         else
-          crec[e * rw + 1 + a] = v;
+          crec[e * rw + 1 + wd] = v;
       }
     }
-    if (G.naggs != 1) {
+    if (G.nacc != 1) {
 #pragma unroll
       for (int j = 0; j < kSub; ++j)
         if (live[j]) crec[(int64_t)(e0 + j * kBlock + t) * rw] = slot[j];
@@ -376,7 +385,7 @@ constexpr int kRB = 512;
 constexpr int kRWaves = kRB / 64;
 constexpr int kRItems = 4096 / kRB;  // slots per thread (a slice has at most 4096)
 
-// Dynamic LDS: acc[naggs][ps] u64, cnt[ps] u32.
+// Dynamic LDS: acc[nacc][ps] u64, cnt[ps] u32.
 // With xper > 0 the grid is 8 * xper workgroups and workgroup b (which the dispatcher places on
 // XCD b % 8) reduces slice (b % 8) * xper + b / 8: consecutive slices run on one XCD, so the
 // record lines one slice fetches beyond its own run (the neighbouring slices' records of the same
@@ -395,11 +404,11 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
   __shared__ int64_t s_off;
   const int ps = 1 << sb;
   uint64_t* lacc = smem;
-  uint32_t* lcnt = (uint32_t*)(smem + (int64_t)G.naggs * ps);
+  uint32_t* lcnt = (uint32_t*)(smem + (int64_t)G.nacc * ps);
   const int t = threadIdx.x;
   const int s = xper > 0 ? (int)(blockIdx.x % 8) * xper + (int)(blockIdx.x / 8) : (int)blockIdx.x;
   if (s >= S) return;  // the remapped grid's unused places
-  const int rw = 1 + G.naggs;
+  const int rw = 1 + G.nacc;
   // the presence bytes and the first batch's chunk offsets are in flight while the table clears
   const uint4 kq = slice_bytes(K, s, sb);
   constexpr int kPer = kGjBatch / kRB;
@@ -416,7 +425,7 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
   load_batch(0);
   for (int i = t; i < ps; i += kRB) {
     lcnt[i] = 0;
-    for (int a = 0; a < G.naggs; ++a) lacc[a * ps + i] = G.init[a];
+    for (int w = 0; w < G.nacc; ++w) lacc[w * ps + i] = G.init[G.acc_agg[w]];
   }
   if (t == 0) {
     s_keys = 0;
@@ -463,12 +472,12 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
           if (k == kk) first = la[kk];
         at[u] = ok[u] ? ((wb + t * kPer + k) * kGjChunk + first + q) : 0;
       }
-      if (G.naggs == 1) {
+      if (G.nacc == 1) {
         ulonglong2 x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
           x[u] = ok[u] ? *(const ulonglong2*)(rec + at[u] * 2) : make_ulonglong2(0, 0);
-        const int kind = G.kind[0];
+        const int kind = G.kind[G.acc_agg[0]];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           if (!ok[u]) continue;
@@ -483,14 +492,14 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
 #pragma unroll
         for (int u = 0; u < U; ++u)
           if (ok[u]) atomicAdd(&lcnt[sl[u]], 1u);
-        for (int a = 0; a < G.naggs; ++a) {
+        for (int w = 0; w < G.nacc; ++w) {
           uint64_t v[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) v[u] = ok[u] ? rec[at[u] * rw + 1 + a] : 0;
-          const int kind = G.kind[a];
+          for (int u = 0; u < U; ++u) v[u] = ok[u] ? rec[at[u] * rw + 1 + w] : 0;
+          const int kind = G.kind[G.acc_agg[w]];
 #pragma unroll
           for (int u = 0; u < U; ++u)
-            if (ok[u]) acc_apply(&lacc[a * ps + sl[u]], kind, v[u]);
+            if (ok[u]) acc_apply(&lacc[w * ps + sl[u]], kind, v[u]);
         }
       }
     }
@@ -612,7 +621,11 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
   }
   for (int a = 0; a < G.naggs; ++a) {
     void* dst = O.agg[a];
-    const uint64_t* acc = lacc + (int64_t)a * ps;
+    if (G.word[a] < 0) {
+      TQ_PUT(int64_t, (int64_t)lcnt[i])       // a count of a constant: the slot's pairs
+      continue;
+    }
+    const uint64_t* acc = lacc + (int64_t)G.word[a] * ps;
     if ((G.op[a] == TQ_AGG_SUM || G.op[a] == TQ_AGG_MIN || G.op[a] == TQ_AGG_MAX) &&
         G.arg_dtype[a] == TQ_F64) {
       TQ_PUT(double, as_f64(acc[i]))          // f64 accumulators are the f64 results
@@ -744,6 +757,20 @@ static int gj_pred(const tq_program_t* P, const tq_colset_t* C, const char* side
   return TQ_OK;
 }
 
+// The direct argument forms and the accumulated aggregates of G (kind, op, arg_dtype, naggs
+// set): a count(c), c a constant other than the null sentinel (accumulator value 1 per pair),
+// needs no record word and no table column.
+static void gj_acc_words(const tq_program_t& prog, GjSpec* G) {
+  match_direct(prog, G->naggs, G->kind, G->arg_dtype, &G->D);
+  G->nacc = 0;
+  for (int a = 0; a < G->naggs; ++a) {
+    const bool pairs = G->D.direct && G->op[a] == TQ_AGG_COUNT && G->D.nf[a] == 0 &&
+                       G->D.dconst[a] == 1ull;
+    G->word[a] = pairs ? -1 : G->nacc;
+    if (!pairs) G->acc_agg[G->nacc++] = a;
+  }
+}
+
 static int64_t gj_spin_ticks() {
   // a look-back wait gives up after 100 ms of wall-clock time
   static int64_t ticks = 0;
@@ -769,6 +796,22 @@ extern "C" size_t tq_groupjoin_ws_bytes(int64_t nb, int64_t np, int64_t key_rang
   GjPlan p;
   if (gj_plan(nb, np, key_range, naggs, nload, &p) != TQ_OK) return 0;
   return p.bytes;
+}
+
+extern "C" int32_t tq_groupjoin_record_words(const tq_program_t* prog, int32_t naggs,
+                                             const int32_t* agg_op, const int32_t* agg_dtype) {
+  if (!prog || !agg_op || !agg_dtype || naggs < 1 || naggs > TQ_GB_MAX_AGGS ||
+      prog->nseg != naggs)
+    return -1;
+  GjSpec G{};
+  G.naggs = naggs;
+  for (int a = 0; a < naggs; ++a) {
+    if (agg_kind_init(agg_op[a], agg_dtype[a], &G.kind[a], &G.init[a]) != TQ_OK) return -1;
+    G.arg_dtype[a] = agg_dtype[a];
+    G.op[a] = agg_op[a];
+  }
+  gj_acc_words(*prog, &G);
+  return 1 + G.nacc;
 }
 
 extern "C" size_t tq_groupjoin_pres_bytes(int64_t key_range) {
@@ -824,7 +867,7 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
       if (d->out_cap > 0 && !O.agg[a])
         return set_error(TQ_EINVAL, "groupjoin: null aggregate output");
     }
-    match_direct(*prog, G.naggs, G.kind, G.arg_dtype, &G.D);
+    gj_acc_words(*prog, &G);
     O.key = d->out_key;
     O.key_dtype = d->out_key_dtype;
     O.cap = d->out_cap;
@@ -856,7 +899,7 @@ static int gj_run(const tq_groupjoin_desc_t* d, uint32_t epoch, const tq_program
   }
   GjCheck K{(const uint8_t*)pres, epoch, (uint64_t)range, btile,
             d->nb > 0 ? p.nbt : 0, wd, gj_spin_ticks()};
-  const size_t rlds = ((size_t)8 * d->naggs + 4) << p.sb;
+  const size_t rlds = ((size_t)8 * G.nacc + 4) << p.sb;
   // consecutive slices on one XCD when the remapped grid is resident all at once
   const int xper = (int)ceil_div(p.S, 8);
   const bool remap = p.S > 8 && rlds <= 64 * 1024 &&

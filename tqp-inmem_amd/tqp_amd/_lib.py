@@ -188,6 +188,7 @@ _SIGNATURES = {
     "tq_rank_sorted": ([P, I32, I64, P, P, P, P, P, SZ, P], I32),
     "tq_groupjoin_pres_bytes": ([I64], SZ),
     "tq_groupjoin_ws_bytes": ([I64, I64, I64, I32, I32], SZ),
+    "tq_groupjoin_record_words": ([P, I32, P, P], I32),
     "tq_groupjoin_run": ([P, P, P, P, SZ, P, SZ, P, P], I32),
     "tq_groupjoin_run_noalloc": ([P, P, P, P, P], I32),
     "tq_batch_ws_bytes": ([I32], SZ),

@@ -419,7 +419,7 @@ def run(tensor_group, ja, aa, p, run_join):
     at = L.trace_len()
     pkbytes = pk.element_size()
     load_bytes = b.load_bytes()
-    rec_bytes = 8 * (1 + naggs)
+    rec_bytes = 8 * _record_words(p.prog_key, b, d, naggs)
     out_bytes = key_dtype.itemsize + sum(dt.itemsize for dt in out_dtypes)
     psel = (1 if pmask is not None else 0) + (ppred.load_bytes() if ppred is not None else 0)
     d.phases = 2
@@ -478,6 +478,25 @@ def run(tensor_group, ja, aa, p, run_join):
     df.written = {**{agg_ids[i]: "agg" for i in range(len(vals))},
                   **{kid: "key" for kid in key_ids_new}}
     return df
+
+
+_REC_WORDS = {}
+
+
+def _record_words(prog_key, b, d, naggs):
+    """Words per probe record (tq_groupjoin_record_words), per program and aggregate dtypes."""
+    key = (prog_key, tuple(d.agg_op[:naggs]), tuple(d.agg_dtype[:naggs]))
+    w = _REC_WORDS.get(key)
+    if w is None:
+        w = int(L.load().tq_groupjoin_record_words(E.program_ptr(b), naggs,
+                                                   ctypes.addressof(d.agg_op),
+                                                   ctypes.addressof(d.agg_dtype)))
+        if w < 1:
+            w = 1 + naggs
+        if len(_REC_WORDS) > 256:
+            _REC_WORDS.clear()
+        _REC_WORDS[key] = w
+    return w
 
 
 @functools.lru_cache(maxsize=256)
