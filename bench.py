@@ -493,9 +493,12 @@ def main():
     }
     if not args.no_configs:
         line["configs"] = configs_section(dev)
-        line["extra"] = extra_section(dev)
     if not args.no_batched:
         line["batched"] = batched_section(args.batch_pipes, dev)
+    if not args.no_configs:
+        # after the batched comparison: its multi-GB inputs and workspaces stay cached by the
+        # allocator and the role workspaces, which the plain path's timing must not inherit
+        line["extra"] = extra_section(dev)
     if args.serve:
         from tqp_amd import serving_bench
         line["served"] = serving_bench.sweep(dev)
