@@ -517,8 +517,8 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
     if (lane_id() == 0) wocc[j][w] = (uint32_t)__popcll(bal[j]);
   }
   __syncthreads();
-  // publish this slice's counts, then add up those of the slices below it, 256 per round (every
-  // slice waited on belongs to a workgroup that is resident or done)
+  // publish this slice's counts, then add up those of the slices below it, two per thread and
+  // round (every slice waited on belongs to a workgroup that is resident or done)
   uint64_t own = 0;
 #pragma unroll
   for (int j = 0; j < kRItems; ++j)
@@ -529,22 +529,28 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
     uint64_t before = 0, keys = 0, recs = 0;
     bool okw = true;
     const int64_t t0 = wall_clock64();
-    for (int q0 = 0; q0 < s && okw; q0 += kRB) {
-      const int q = q0 + t;
-      uint64_t f = 1ull;  // threads past s - 1 add nothing
+    for (int q0 = 0; q0 < s && okw; q0 += 2 * kRB) {
+      const int qa = q0 + t, qb = q0 + kRB + t;
+      uint64_t fa = 1ull, fb = 1ull;  // threads past s - 1 add nothing
       while (true) {
-        if (q < s) f = gj_ld(flags + q);
-        if (ballot(f == 0) == 0) break;
+        if (qa < s) fa = gj_ld(flags + qa);
+        if (qb < s) fb = gj_ld(flags + qb);
+        if (ballot(fa == 0 || fb == 0) == 0) break;
         if (wall_clock64() - t0 > K.spin_ticks) {
           okw = false;
           break;
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      if (q < s) {
-        before += lb_groups(f);
-        keys += lb_keys(f);
-        recs += lb_recs(f);
+      if (qa < s) {
+        before += lb_groups(fa);
+        keys += lb_keys(fa);
+        recs += lb_recs(fa);
+      }
+      if (qb < s) {
+        before += lb_groups(fb);
+        keys += lb_keys(fb);
+        recs += lb_recs(fb);
       }
     }
     __shared__ uint64_t lbs[3][kRWaves];
@@ -643,9 +649,9 @@ __global__ __launch_bounds__(kRB) void k_gj_reduce(GjSpec G, int sb, int S, int6
     }
   }
 #undef TQ_PUT
-  if (s == S - 1) {
-    // the last slice (which no workgroup waits on) also sums the build's tile counts: the
-    // selected build rows, and whether a key fell outside the bounds
+  if (s == 0) {
+    // slice 0 (which waits on no other) also sums the build's tile counts: the selected build
+    // rows, and whether a key fell outside the bounds
     int64_t nsel = 0;
     uint32_t oob = 0;
     for (int64_t i = t; i < K.nbt; i += kRB) {
