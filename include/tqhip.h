@@ -589,7 +589,7 @@ size_t tq_groupjoin_ws_bytes(int64_t nb, int64_t np, int64_t key_range, int32_t 
 /* Words per probe record of a call with this program and these aggregates (host only, for byte
  * accounting): 1 (the slot) plus one per accumulated aggregate; a count of a constant other than
  * the null sentinel is taken from the slots' pair counts and has no word.  -1 for bad input.
- * No counterpart in the reference: it describes this build's intermediate records. */
+ * No counterpart in the reference: it describes the fused kernels' intermediate records. */
 int32_t tq_groupjoin_record_words(const tq_program_t* prog /* host */, int32_t naggs,
                                   const int32_t* agg_op /* host */,
                                   const int32_t* agg_dtype /* host */);
