@@ -309,7 +309,8 @@ def batched_section(pipes_n, dev):
     eng = batched.BatchedEngine(specs, tile_rows=tune["tile_rows"], grid=tune["grid"],
                                 records=tune.get("records", False),
                                 lds_table_max=tune.get("lds_table_max"),
-                                reduce_lds=tune.get("reduce_lds"))
+                                reduce_lds=tune.get("reduce_lds"),
+                                fused_min_rows=tune.get("fused_min_rows"))
     eng.run()
     out, rerun = eng.results()
     reps = 5
@@ -349,6 +350,7 @@ def batched_section(pipes_n, dev):
             "tile_rows": tune["tile_rows"], "grid": tune["grid"],
             "large_tables": "records" if tune.get("records") else "atomics",
             "lds_table_max": tune.get("lds_table_max"), "reduce_lds": tune.get("reduce_lds"),
+            "fused_min_rows": tune.get("fused_min_rows"), "fused_pipelines": len(eng.fused),
             "tuning_ms": {k: round(v * 1e3, 3) for k, v in tune.get("timings", {}).items()}}
 
 

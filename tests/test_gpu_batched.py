@@ -73,13 +73,47 @@ def test_batched_lds_capacities(device, tmax, rlds):
                                    rtol=1e-12, atol=0, err_msg=f"sums of pipeline {i}")
 
 
-def test_batched_graph_replay(device):
-    """The launch sequence captured in a graph replays to the eager result."""
+@pytest.mark.parametrize("fused_min_rows", [1, 50_000])
+def test_batched_fused_pipelines_match_plain(device, fused_min_rows):
+    """Join pipelines from fused_min_rows probe rows up run as fused groupjoin calls after the
+    persistent kernel (1: every pipeline, and no persistent launch); two runs in a row give the
+    plain path's results."""
+    from tqp_amd import batched, workloads
+    probe, build, consts = workloads.config3_sizes(n_pipes=24, lo=1000, hi=300_000)
+    pipes = [workloads.config3_pipeline(i, probe[i], build[i], consts[i][0], consts[i][1], device)
+             for i in range(len(probe))]
+    specs = [batched.match_plan(plan, group) for group, plan in pipes]
+    eng = batched.BatchedEngine(specs, tile_rows=2048, fused_min_rows=fused_min_rows)
+    print("fused pipelines:", len(eng.fused), "persistent:", eng.npipes, flush=True)
+    assert len(eng.fused) == sum(1 for n in probe if n >= fused_min_rows)
+    assert eng.npipes == len(probe) - len(eng.fused)
+    for rep in range(2):
+        eng.run()
+        out, rerun = eng.results()
+        for i, (group, plan) in enumerate(pipes):
+            ref = _plain(group, plan)
+            if i in rerun:
+                # only a fused call without pairs goes back to the per-operator path
+                assert eng.fused and ref[101].tensor.shape[0] == 0, f"pipeline {i} rerun"
+                continue
+            got = out[i]
+            assert torch.equal(got[101].cpu(), ref[101].tensor.cpu()), f"keys of pipeline {i}"
+            assert torch.equal(got[302].cpu(), ref[302].tensor.cpu()), f"counts of pipeline {i}"
+            np.testing.assert_allclose(got[301].cpu().numpy(), ref[301].tensor.cpu().numpy(),
+                                       rtol=1e-12, atol=0, err_msg=f"sums of pipeline {i}")
+
+
+@pytest.mark.parametrize("fused_min_rows", [None, 10_000])
+def test_batched_graph_replay(device, fused_min_rows):
+    """The launch sequence captured in a graph replays to the eager result (with fused
+    groupjoin calls: their shared presence bytes are cleared inside the graph)."""
     from tqp_amd import batched, workloads
     probe, build, consts = workloads.config3_sizes(n_pipes=8, lo=1000, hi=50_000)
     pipes = [workloads.config3_pipeline(i, probe[i], build[i], consts[i][0], consts[i][1], device)
              for i in range(len(probe))]
-    eng = batched.BatchedEngine([batched.match_plan(p, g) for g, p in pipes], tile_rows=2048)
+    eng = batched.BatchedEngine([batched.match_plan(p, g) for g, p in pipes], tile_rows=2048,
+                                fused_min_rows=fused_min_rows)
+    print("fused pipelines:", len(eng.fused), flush=True)
     eng.run()
     eager, _ = eng.results()
     eager = {i: {k: v.clone() for k, v in r.items()} for i, r in eager.items()}

@@ -206,8 +206,12 @@ class BatchedEngine:
     """Prepare once, then run() (graph-capturable) and results()."""
 
     def __init__(self, specs, tile_rows=4096, grid=None, wait_us=100_000, key_bounds=None,
-                 records=False, fast=True, lds_table_max=None, reduce_lds=None):
+                 records=False, fast=True, lds_table_max=None, reduce_lds=None,
+                 fused_min_rows=None):
         self.specs = list(specs)
+        # join pipelines with at least this many probe rows (and conjunctive filters) run as
+        # fused groupjoin launches recorded by run() instead of items of the persistent kernel
+        self.fused_min_rows = fused_min_rows
         self.records = records    # True: large join tables accumulate through records
         # LDS capacities: a pipeline whose whole table (4 + 8 * aggregates bytes a key) fits
         # lds_table_max is accumulated in LDS; a records pipeline's slice table fits reduce_lds
@@ -251,6 +255,92 @@ class BatchedEngine:
         return dict(i=i, bb=bb, pb=pb, lo=lo, rng=rng, cap=min(nb, rng), nb=nb, np=np_,
                     key_dtype=bk.dtype, gkeys=None)
 
+    def _conjunction(self, tree, cols):
+        """A predicate program over `cols` that the groupjoin kernels evaluate in their own
+        loads (a conjunction of column-versus-constant comparisons), else None."""
+        b = E.ProgramBuilder()
+        E.compile_predicate(b, tree, _Cols(cols))
+        b.finish()
+        if not b.col_tensors or not L.load().tq_filter_is_conjunction(E.program_ptr(b),
+                                                                      E.colset_ptr(b)):
+            return None
+        return b
+
+    def _plan_fused(self, i, sp):
+        """Set up join pipeline i as one fused groupjoin call (tq_groupjoin_run: build, probe
+        and reduce launches) when it is large enough and its parts fit the fused kernels; False
+        leaves it to the persistent kernel."""
+        bk = sp.build_cols[sp.build_key].reshape(-1)
+        pk = sp.probe_cols[sp.probe_key].reshape(-1)
+        nb, np_ = bk.shape[0], pk.shape[0]
+        if np_ < self.fused_min_rows or nb == 0 or not bk.is_contiguous() or \
+                not pk.is_contiguous():
+            return False
+        if bk.dtype.is_floating_point or pk.dtype.is_floating_point or \
+                torch.bool in (bk.dtype, pk.dtype):
+            return False
+        lo, hi = self._bounds(bk)
+        rng = hi - lo + 1
+        if rng < 1 or rng > 4 * nb + 65536:
+            return False
+        bpred = self._conjunction(sp.build_pred, sp.build_cols)
+        ppred = self._conjunction(sp.probe_pred, sp.probe_cols)
+        if bpred is None or ppred is None or bpred.n != nb or ppred.n != np_:
+            return False
+        ab = E.ProgramBuilder()
+        arg_dtypes = []
+        for _, op, tree in sp.aggs:
+            v = E.compile_row_expr(ab, tree, _Cols(sp.probe_cols))
+            if v.kind == "lit":
+                v = ab.const(v.value, E.literal_dtype(v.value))
+            if v.kind != "reg":
+                return False
+            arg_dtypes.append(v.dtype)
+            ab.end_segment(v)
+        ab.finish()
+        if not ab.col_tensors:
+            ab.column(pk)
+        naggs = len(sp.aggs)
+        wsb = L.size_call("tq_groupjoin_ws_bytes", nb, np_, rng, naggs, ab.nload)
+        if wsb == 0:
+            return False
+        ops = {'sum': L.TQ_AGG_SUM, 'count': L.TQ_AGG_COUNT, 'avg': L.TQ_AGG_AVG,
+               'min': L.TQ_AGG_MIN, 'max': L.TQ_AGG_MAX}
+        dev = self.device
+        cap = max(1, min(rng, np_))
+        key_out = memory.empty(cap, dtype=bk.dtype, device=dev)
+        agg_outs = []
+        d = L.GroupJoinDesc()
+        d.bkey, d.bkey_dtype, d.nb = bk.data_ptr(), L.dtype_code(bk.dtype), nb
+        d.pkey, d.pkey_dtype, d.np = pk.data_ptr(), L.dtype_code(pk.dtype), np_
+        d.key_lo, d.key_hi, d.naggs = lo, hi, naggs
+        d.bpred, d.bpred_cols = ctypes.addressof(bpred.prog), ctypes.addressof(bpred.cols)
+        d.ppred, d.ppred_cols = ctypes.addressof(ppred.prog), ctypes.addressof(ppred.cols)
+        for a, ((_, op, _), dt) in enumerate(zip(sp.aggs, arg_dtypes)):
+            d.agg_op[a] = ops[op]
+            d.agg_dtype[a] = L.dtype_code(dt)
+            if op == 'count':
+                odt = torch.int64
+            elif op == 'avg':
+                odt = torch.float32 if dt == torch.float32 else torch.float64
+            elif op == 'sum':
+                odt = dt if dt.is_floating_point else torch.int64
+            else:
+                odt = dt
+            t = memory.empty(cap, dtype=odt, device=dev)
+            agg_outs.append(t)
+            d.out_agg[a] = t.data_ptr()
+        d.out_key, d.out_key_dtype, d.out_cap = key_out.data_ptr(), L.dtype_code(bk.dtype), cap
+        d.phases = 0
+        # every call's build stamps its keys with its own epoch in the shared presence bytes
+        d.epoch = len(self.fused) % 255 + 1
+        ws = memory.empty(wsb, dtype=torch.uint8, device=dev)
+        self.fused.append(dict(i=i, d=d, keep=(bk, pk, bpred, ppred, ab), ws=ws,
+                               keys_out=[key_out], agg_outs=agg_outs,
+                               pres_bytes=L.size_call("tq_groupjoin_pres_bytes", rng),
+                               words=None))
+        return True
+
     def _plan_group(self, i, sp):
         cols = [sp.probe_cols[k].reshape(-1) for k in sp.keys]
         n = next(iter(sp.probe_cols.values())).shape[0]
@@ -290,7 +380,10 @@ class BatchedEngine:
         stage_loads = 1
         lds_table = 0
         layout = []
+        self.fused = []
         for i, sp in enumerate(self.specs):
+            if self.fused_min_rows is not None and sp.mode == 0 and self._plan_fused(i, sp):
+                continue
             lay = self._plan_join(i, sp) if sp.mode == 0 else self._plan_group(i, sp)
             if lay is None:
                 self.rejected.append(i)
@@ -438,15 +531,29 @@ class BatchedEngine:
         self.ws = memory.empty(max(L.size_call("tq_batch_ws_bytes", len(layout)), 256),
                                dtype=torch.uint8, device=dev)
         self.npipes = len(layout)
+        # presence bytes shared by the fused calls (cleared by run() before the first and every
+        # 255th call, so no call finds another's stamps under its own epoch)
+        self.fused_pres = memory.empty(max([f["pres_bytes"] for f in self.fused] or [256]),
+                                       dtype=torch.uint8, device=dev) if self.fused else None
 
     # ---------------------------------------------------------------- execution
     def run(self):
-        """Launch the batch on the current stream (kernel launches only)."""
-        if self.npipes == 0:
-            return
-        L.call("tq_batch_run", L.ptr(self.desc), self.npipes, L.ptr(self.items), self.phase_start,
-               self.tile_rows, self.grid, self.stage_loads, self.lds_table_bytes, L.ptr(self.ws),
-               ctypes.c_uint32(self.wait_us), L.stream_handle())
+        """Launch the batch on the current stream (kernel launches only): the persistent
+        kernel's pipelines, then each fused pipeline's groupjoin call."""
+        s = L.stream_handle()
+        if self.npipes:
+            L.call("tq_batch_run", L.ptr(self.desc), self.npipes, L.ptr(self.items),
+                   self.phase_start, self.tile_rows, self.grid, self.stage_loads,
+                   self.lds_table_bytes, L.ptr(self.ws), ctypes.c_uint32(self.wait_us), s)
+        for j, f in enumerate(self.fused):
+            if j % 255 == 0:
+                self.fused_pres.zero_()
+            ab = f["keep"][4]
+            words = ctypes.c_void_p()
+            L.call("tq_groupjoin_run", ctypes.byref(f["d"]), E.program_ptr(ab), E.colset_ptr(ab),
+                   L.ptr(self.fused_pres), self.fused_pres.numel(), L.ptr(f["ws"]),
+                   f["ws"].numel(), ctypes.byref(words), s)
+            f["words"] = words.value
 
     def results(self, sync=True):
         """Returns ({spec index: {out id: tensor}}, [spec indices to rerun]).  With sync=False
@@ -472,6 +579,29 @@ class BatchedEngine:
             for (oid, _, _), t in zip(sp.aggs, agg_outs):
                 res[oid] = t[:G]
             out[i] = res
+        if self.fused:
+            # each fused call's words (groups, pairs, selected build rows, distinct selected
+            # build keys, error bits) sit in its own workspace: one gather and one copy
+            rows = []
+            for f in self.fused:
+                if f["words"] is None:
+                    raise RuntimeError("batched results() before run()")
+                off = f["words"] - f["ws"].data_ptr()
+                if off < 0 or off % 8 or off + 40 > f["ws"].numel():
+                    raise RuntimeError("groupjoin words outside the call's workspace")
+                rows.append(f["ws"][off:off + 40].view(torch.int64))
+            for f, (G, m, nsel, nset, err) in zip(self.fused, torch.stack(rows).cpu().tolist()):
+                i = f["i"]
+                if err or nsel != nset or m <= 0:
+                    # a repeated or out-of-bounds build key, a look-back that gave up, or no
+                    # pairs: the per-operator path gives the reference's result
+                    rerun.append(i)
+                    continue
+                sp = self.specs[i]
+                res = {sp.key_out_id: f["keys_out"][0][:G].reshape(-1, 1)}
+                for (oid, _, _), t in zip(sp.aggs, f["agg_outs"]):
+                    res[oid] = t[:G]
+                out[i] = res
         return out, sorted(rerun)
 
     def reset_flags(self):
@@ -508,10 +638,12 @@ TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.js
 
 
 def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None,
-             lds_candidates=None):
+             lds_candidates=None, fused_candidates=None):
     """Pick tile rows, persistent grid size and the accumulation of large tables (global atomics
     or records with slice reduces) for a batch by timing candidates on it, then, for that
-    choice, the LDS capacities (whole-table limit, records slice table) from `lds_candidates`.
+    choice, the LDS capacities (whole-table limit, records slice table) from `lds_candidates`,
+    then the probe-row threshold from which join pipelines run as fused groupjoin calls
+    (`fused_candidates`; None, no fused calls, is always a candidate).
     Results are cached in tuning.json under `key` (device name + workload signature).  The
     batched path has no radix pass (tables are addressed by key - min), so there are no radix
     bits to choose."""
@@ -564,9 +696,27 @@ def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None,
         if dt < best_lds[0]:
             best_lds = (dt, tmax, rlds)
         del eng
+    # then which join pipelines run as fused groupjoin calls: none, or those with at least
+    # this many probe rows
+    best_fused = (best_lds[0], None)
+    for fm in fused_candidates if fused_candidates is not None else (1_000_000, 250_000):
+        eng = BatchedEngine(specs, tile_rows=best[1], grid=best[2], key_bounds=bounds,
+                            records=best[3], lds_table_max=best_lds[1], reduce_lds=best_lds[2],
+                            fused_min_rows=fm)
+        eng.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            eng.run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / repeats
+        timings[f"fused >= {fm} probe rows ({len(eng.fused)} pipelines)"] = dt
+        if dt < best_fused[0]:
+            best_fused = (dt, fm)
+        del eng
     choice = {"tile_rows": best[1], "grid": best[2], "records": best[3],
-              "lds_table_max": best_lds[1], "reduce_lds": best_lds[2], "seconds": best_lds[0],
-              "timings": timings}
+              "lds_table_max": best_lds[1], "reduce_lds": best_lds[2],
+              "fused_min_rows": best_fused[1], "seconds": best_fused[0], "timings": timings}
     table[key] = choice
     if cache:
         try:
