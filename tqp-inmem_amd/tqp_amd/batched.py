@@ -207,8 +207,12 @@ class BatchedEngine:
 
     def __init__(self, specs, tile_rows=4096, grid=None, wait_us=100_000, key_bounds=None,
                  records=False, fast=True, lds_table_max=None, reduce_lds=None,
-                 fused_min_rows=None):
+                 fused_min_rows=None, fused_streams=3):
         self.specs = list(specs)
+        # the fused calls are spread round-robin over this many side streams (default
+        # priority), forked from and joined back into the launching stream
+        self.fused_streams = max(int(fused_streams), 1)
+        self._sides = None
         # join pipelines with at least this many probe rows (and conjunctive filters) run as
         # fused groupjoin launches recorded by run() instead of items of the persistent kernel
         self.fused_min_rows = fused_min_rows
@@ -531,29 +535,62 @@ class BatchedEngine:
         self.ws = memory.empty(max(L.size_call("tq_batch_ws_bytes", len(layout)), 256),
                                dtype=torch.uint8, device=dev)
         self.npipes = len(layout)
-        # presence bytes shared by the fused calls (cleared by run() before the first and every
-        # 255th call, so no call finds another's stamps under its own epoch)
-        self.fused_pres = memory.empty(max([f["pres_bytes"] for f in self.fused] or [256]),
-                                       dtype=torch.uint8, device=dev) if self.fused else None
+        # presence bytes of the fused calls, one array per side stream: the calls of one stream
+        # run one after another and stamp their keys with their own epochs (the arrays are
+        # cleared by run() before the first and every 255th call, so no call finds another's
+        # stamps under its own epoch); calls on different streams overlap in time, and keys of
+        # one would overwrite the other's stamps in a shared array
+        nb_pres = max([f["pres_bytes"] for f in self.fused] or [256])
+        self.fused_pres = [memory.empty(nb_pres, dtype=torch.uint8, device=dev)
+                           for _ in range(self.fused_streams)] if self.fused else []
 
     # ---------------------------------------------------------------- execution
     def run(self):
         """Launch the batch on the current stream (kernel launches only): the persistent
         kernel's pipelines, then each fused pipeline's groupjoin call."""
         s = L.stream_handle()
+        main = torch.cuda.current_stream()
+        if self.fused:
+            if self._sides is None:
+                self._sides = _side_streams(self.device, self.fused_streams)
+            self._fork(main)
         if self.npipes:
             L.call("tq_batch_run", L.ptr(self.desc), self.npipes, L.ptr(self.items),
                    self.phase_start, self.tile_rows, self.grid, self.stage_loads,
                    self.lds_table_bytes, L.ptr(self.ws), ctypes.c_uint32(self.wait_us), s)
         for j, f in enumerate(self.fused):
-            if j % 255 == 0:
-                self.fused_pres.zero_()
+            if j and j % 255 == 0:
+                # epochs start over: every call so far is done before the bytes are cleared
+                self._join(main)
+                self._fork(main)
             ab = f["keep"][4]
             words = ctypes.c_void_p()
-            L.call("tq_groupjoin_run", ctypes.byref(f["d"]), E.program_ptr(ab), E.colset_ptr(ab),
-                   L.ptr(self.fused_pres), self.fused_pres.numel(), L.ptr(f["ws"]),
-                   f["ws"].numel(), ctypes.byref(words), s)
+            side = j % len(self._sides)
+            pres = self.fused_pres[side]
+            with torch.cuda.stream(self._sides[side]):
+                L.call("tq_groupjoin_run", ctypes.byref(f["d"]), E.program_ptr(ab),
+                       E.colset_ptr(ab), L.ptr(pres), pres.numel(), L.ptr(f["ws"]),
+                       f["ws"].numel(), ctypes.byref(words), L.stream_handle())
             f["words"] = words.value
+        if self.fused:
+            self._join(main)
+
+    def _fork(self, main):
+        """Clear the fused calls' presence bytes on `main`, then let the side streams start
+        after it."""
+        for pres in self.fused_pres:
+            pres.zero_()
+        ev = torch.cuda.Event()
+        ev.record(main)
+        for st in self._sides:
+            st.wait_event(ev)
+
+    def _join(self, main):
+        """`main` continues after everything queued on the side streams."""
+        for st in self._sides:
+            ev = torch.cuda.Event()
+            ev.record(st)
+            main.wait_event(ev)
 
     def results(self, sync=True):
         """Returns ({spec index: {out id: tensor}}, [spec indices to rerun]).  With sync=False
@@ -606,6 +643,19 @@ class BatchedEngine:
 
     def reset_flags(self):
         self.flags.zero_()
+
+
+_SIDES = {}
+
+
+def _side_streams(device, n):
+    """n side streams of `device` (default priority), made once and shared by every engine: a
+    process holds few streams, as the device has few hardware queues to map them on."""
+    key = (torch.device(device).index or 0, n)
+    st = _SIDES.get(key)
+    if st is None:
+        st = _SIDES[key] = [torch.cuda.Stream(device=device) for _ in range(n)]
+    return st
 
 
 def _table_bytes(rng, naggs, tile):
@@ -696,25 +746,29 @@ def autotune(specs, candidates=None, repeats=3, cache=TUNE_CACHE, key=None,
         if dt < best_lds[0]:
             best_lds = (dt, tmax, rlds)
         del eng
-    # then which join pipelines run as fused groupjoin calls: none, or those with at least
-    # this many probe rows
-    best_fused = (best_lds[0], None)
-    for fm in fused_candidates if fused_candidates is not None else (1_000_000, 250_000):
-        eng = BatchedEngine(specs, tile_rows=best[1], grid=best[2], key_bounds=bounds,
-                            records=best[3], lds_table_max=best_lds[1], reduce_lds=best_lds[2],
-                            fused_min_rows=fm)
-        eng.run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(repeats):
+    # then which join pipelines run as fused groupjoin calls (none, or those with at least
+    # this many probe rows), with the persistent grid as tuned or halved (its workgroups hold
+    # the CUs until its queue drains, so a smaller grid leaves room for the fused calls)
+    best_fused = (best_lds[0], None, best[2])
+    fms = fused_candidates if fused_candidates is not None else \
+        (2_000_000, 1_000_000, 500_000, 250_000)
+    for fm in fms:
+        for grid in (best[2], max(best[2] // 2, 1)):
+            eng = BatchedEngine(specs, tile_rows=best[1], grid=grid, key_bounds=bounds,
+                                records=best[3], lds_table_max=best_lds[1],
+                                reduce_lds=best_lds[2], fused_min_rows=fm)
             eng.run()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / repeats
-        timings[f"fused >= {fm} probe rows ({len(eng.fused)} pipelines)"] = dt
-        if dt < best_fused[0]:
-            best_fused = (dt, fm)
-        del eng
-    choice = {"tile_rows": best[1], "grid": best[2], "records": best[3],
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(repeats):
+                eng.run()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / repeats
+            timings[f"fused >= {fm} probe rows ({len(eng.fused)} pipelines), grid {grid}"] = dt
+            if dt < best_fused[0]:
+                best_fused = (dt, fm, grid)
+            del eng
+    choice = {"tile_rows": best[1], "grid": best_fused[2], "records": best[3],
               "lds_table_max": best_lds[1], "reduce_lds": best_lds[2],
               "fused_min_rows": best_fused[1], "seconds": best_fused[0], "timings": timings}
     table[key] = choice
