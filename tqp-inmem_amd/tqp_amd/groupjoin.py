@@ -357,7 +357,7 @@ def run(tensor_group, ja, aa, p, run_join):
     ws = L.workspace(ws_bytes, dev, "groupjoin")
     key_dtype = tensor_group[p.gkey].tensor.dtype
 
-    # the build's launch goes out first and runs while the rest of the call is prepared
+    # the descriptor: the build side first, then the probe side and the outputs
     d = L.GroupJoinDesc()
     d.bkey, d.bkey_dtype = bk.data_ptr(), L.dtype_code(bk.dtype)
     d.bmask, d.nb = (bmask.data_ptr() if bmask is not None else None), nb
@@ -371,10 +371,7 @@ def run(tensor_group, ja, aa, p, run_join):
     # a fused filter reads its columns in place of the mask byte
     bsel = (1 if bmask is not None else 0) + (bpred.load_bytes() if bpred is not None else 0)
     words = ctypes.c_void_p()
-    d.phases = 1
-    L.call("tq_groupjoin_run", ctypes.byref(d), None, None, L.ptr(pres), pres.numel(), L.ptr(ws),
-           ws_bytes, ctypes.byref(words), L.stream_handle(),
-           kernel={"k_gj_build": nb * (bk.element_size() + bsel)})
+    build_bytes = nb * (bk.element_size() + bsel)
 
     pmask, ppred = _mask_or_predicate(tensor_group, p.pmask_id)
     probe_cols = {i: tensor_group[i] for i in p.arg_ids}
@@ -394,7 +391,7 @@ def run(tensor_group, ja, aa, p, run_join):
     try:
         b = E.cached_program(p.prog_key, probe_cols, compile_fn)
     except NotImplementedError:
-        raise kernels.DeferredJoinFailed()   # the build launch left nothing another call reads
+        raise kernels.DeferredJoinFailed()   # nothing has been launched
     arg_dtypes = list(b.extra)
     if not b.col_tensors:
         # no column read (count(*) only): point the column set at the probe key
@@ -422,10 +419,12 @@ def run(tensor_group, ja, aa, p, run_join):
     rec_bytes = 8 * _record_words(p.prog_key, b, d, naggs)
     out_bytes = key_dtype.itemsize + sum(dt.itemsize for dt in out_dtypes)
     psel = (1 if pmask is not None else 0) + (ppred.load_bytes() if ppred is not None else 0)
-    d.phases = 2
+    # one call launches the build, the probe and the reduce (the step is host-bound: a
+    # separate earlier build launch would only add a call)
+    d.phases = 0
     L.call("tq_groupjoin_run", ctypes.byref(d), E.program_ptr(b), E.colset_ptr(b), L.ptr(pres),
            pres.numel(), L.ptr(ws), ws_bytes, ctypes.byref(words), L.stream_handle(),
-           alg_bytes=(0, 0), kernel=("k_gj_probe", "k_gj_reduce"))
+           kernel={"k_gj_build": build_bytes, "k_gj_probe": 0, "k_gj_reduce": 0})
     key_type = tensor_group[p.gkey].tensor_type
 
     def bytes_fn(m, G):
